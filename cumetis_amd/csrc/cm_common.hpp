@@ -1,0 +1,133 @@
+// cm_common.hpp -- device-side building blocks shared by the coarsening
+// kernels: wave64 scans/reductions and the decoupled look-back tile chain.
+//
+// Target: gfx950 (CDNA4), 64-lane wavefronts.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define CM_WAVE 64
+
+namespace cm {
+
+__device__ __forceinline__ int lane() { return (int)__lane_id(); }
+
+// ---------------------------------------------------------------- wave ops
+template <typename T>
+__device__ __forceinline__ T wave_incl_scan(T v)
+{
+  const int l = lane();
+#pragma unroll
+  for (int d = 1; d < CM_WAVE; d <<= 1) {
+    T o = __shfl_up(v, d, CM_WAVE);
+    if (l >= d) v += o;
+  }
+  return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v)
+{
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, CM_WAVE);
+  return v;
+}
+
+// Number of set bits of `mask` below this lane.
+__device__ __forceinline__ int mask_rank(uint64_t mask)
+{
+  return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                        __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+// ------------------------------------------------------ decoupled look-back
+// One 64-bit status word per tile: bits 62..63 state (0 = not ready,
+// 1 = tile aggregate, 2 = inclusive prefix), bits 0..61 the value.  The value
+// travels in the same atomic word as its state, so relaxed agent-scope atomics
+// suffice and no cache maintenance is needed across XCDs.
+constexpr uint64_t ST_AGG = 1ull << 62;
+constexpr uint64_t ST_INC = 2ull << 62;
+constexpr uint64_t ST_VAL = (1ull << 62) - 1;
+
+__device__ __forceinline__ void st_store(uint64_t *p, uint64_t v)
+{
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t st_load(const uint64_t *p)
+{
+  return __hip_atomic_load(const_cast<uint64_t *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Called by all 64 lanes of ONE wave of the tile.  Publishes the tile's
+// aggregate, walks back over predecessors 64 at a time, publishes the
+// inclusive prefix and returns the exclusive prefix (same value in all lanes).
+// Tiles must be numbered in dispatch order (see tile_ticket) so that every
+// predecessor has started and the wait terminates.
+__device__ __forceinline__ uint64_t lookback(uint64_t *status, int tile, uint64_t agg)
+{
+  const int l = lane();
+  if (l == 0) st_store(&status[tile], (tile == 0 ? ST_INC : ST_AGG) | agg);
+  if (tile == 0) return 0;
+  uint64_t excl = 0;
+  int base = tile - 1;
+  for (;;) {
+    const int t = base - l;
+    const uint64_t s = t >= 0 ? st_load(&status[t]) : ST_INC;
+    const uint32_t state = (uint32_t)(s >> 62);
+    const uint64_t incm = __ballot(state == 2);
+    const uint64_t zerom = __ballot(state == 0);
+    const int first = incm ? __ffsll((unsigned long long)incm) - 1 : 64;
+    const uint64_t consider = first >= 63 ? ~0ull : ((2ull << first) - 1);
+    if (zerom & consider) continue;  // a predecessor has not published yet
+    excl += wave_sum<uint64_t>(l <= first ? (s & ST_VAL) : 0);
+    if (first < 64) break;
+    base -= CM_WAVE;
+  }
+  if (l == 0) st_store(&status[tile], ST_INC | (excl + agg));
+  return excl;
+}
+
+// Dynamic tile number in dispatch order (one atomic per workgroup).
+__device__ __forceinline__ int tile_ticket(unsigned *ctr)
+{
+  __shared__ int s_tile;
+  if (threadIdx.x == 0) s_tile = (int)atomicAdd(ctr, 1u);
+  __syncthreads();
+  return s_tile;
+}
+
+// ------------------------------------------------------ block scan (256 thr)
+// Exclusive block-wide scan for a 256-thread block; returns the exclusive
+// prefix of `v`; `total` receives the block sum.  Uses `sh` (4 entries).
+template <typename T>
+__device__ __forceinline__ T block256_excl_scan(T v, T &total, T *sh)
+{
+  const int w = threadIdx.x >> 6, l = lane();
+  T inc = wave_incl_scan<T>(v);
+  if (l == CM_WAVE - 1) sh[w] = inc;
+  __syncthreads();
+  T before = 0;
+  total = 0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    T x = sh[i];
+    if (i < w) before += x;
+    total += x;
+  }
+  __syncthreads();
+  return before + inc - v;
+}
+
+// ------------------------------------------------------------- edge order
+// splitmix64 finaliser (Steele, Lea, Flood 2014), top 32 bits.
+__device__ __forceinline__ uint32_t edge_hash(uint64_t seed, uint32_t a, uint32_t b)
+{
+  uint64_t z = seed ^ (((uint64_t)a << 32) | b);
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  return (uint32_t)(z >> 32);
+}
+
+}  // namespace cm

@@ -1,0 +1,561 @@
+// cm_contract.hip -- (b) coarse numbering and (c) contraction.
+//
+// (b) replaces findc1..findc4 + thrust scan + host scan (cu:1621-1694,
+//     cu:1750-1789, cu:1304): one single-pass decoupled look-back scan over
+//     the fine vertices computes, for every representative i (i <= match[i]),
+//     its coarse id, the coarse vertex weight and the offset of its gather
+//     segment (deg(i) + deg(match[i])); a second pass copies cmap to partners.
+// (c) replaces cuMetis_gpu_create_cgraph (cu:1291-1406): gather the two fine
+//     rows of each coarse vertex as keys (coarse neighbour, position) ->
+//     binned segmented sort -> per-segment duplicate merge -> look-back scan
+//     of coarse degrees (cxadj) -> emit rows in ascending or first-seen order.
+//     First-seen order reproduces the reference CPU contraction (cu:737-825)
+//     byte for byte: distinct neighbours ranked by first position, self entry
+//     replaced by the last entry.
+#include "cm_common.hpp"
+#include "cm_internal.hpp"
+#include "cm_segsort.hpp"
+
+namespace cm {
+
+constexpr int GATHER_LONG = 64;   // fine rows longer than this gather by wave
+constexpr int SMALL_SEG = 64;     // coarse segments up to this merge per thread
+
+// ------------------------------------------------------------- numbering
+template <typename OffT>
+__global__ __launch_bounds__(1024) void k_number(int64_t n, const OffT *__restrict__ xadj,
+                                                 const int32_t *__restrict__ match,
+                                                 const int32_t *__restrict__ vwgt,
+                                                 int32_t *__restrict__ cmap, int32_t *__restrict__ rep,
+                                                 OffT *__restrict__ segoff, int32_t *__restrict__ cvwgt,
+                                                 uint64_t *st_c, uint64_t *st_l, unsigned *ctr,
+                                                 int64_t *totals)
+{
+  __shared__ uint64_t sh_c[16], sh_l[16];
+  const int tile = tile_ticket(ctr);
+  const int64_t i = (int64_t)tile * 1024 + threadIdx.x;
+  uint64_t flag = 0, len = 0;
+  int32_t u = 0;
+  if (i < n) {
+    u = match[i];
+    flag = (i <= u);
+    if (flag) {
+      len = (uint64_t)(xadj[i + 1] - xadj[i]);
+      if (u != i) len += (uint64_t)(xadj[u + 1] - xadj[u]);
+    }
+  }
+  const int w = threadIdx.x >> 6, l = lane();
+  const uint64_t ic = wave_incl_scan<uint64_t>(flag), il = wave_incl_scan<uint64_t>(len);
+  if (l == CM_WAVE - 1) {
+    sh_c[w] = ic;
+    sh_l[w] = il;
+  }
+  __syncthreads();
+  if (w == 0) {
+    const uint64_t a = l < 16 ? sh_c[l] : 0, b = l < 16 ? sh_l[l] : 0;
+    const uint64_t ia = wave_incl_scan<uint64_t>(a), ib = wave_incl_scan<uint64_t>(b);
+    const uint64_t ta = __shfl(ia, 15, CM_WAVE), tb = __shfl(ib, 15, CM_WAVE);
+    const uint64_t ea = lookback(st_c, tile, ta);
+    const uint64_t eb = lookback(st_l, tile, tb);
+    if (l < 16) {
+      sh_c[l] = ea + ia - a;
+      sh_l[l] = eb + ib - b;
+    }
+  }
+  __syncthreads();
+  if (i < n) {
+    const uint64_t c = sh_c[w] + ic - flag, off = sh_l[w] + il - len;
+    if (flag) {
+      cmap[i] = (int32_t)c;
+      rep[c] = (int32_t)i;
+      segoff[c] = (OffT)off;
+      cvwgt[c] = vwgt[i] + (u != i ? vwgt[u] : 0);
+    }
+    if (i == n - 1) {
+      totals[0] = (int64_t)(c + flag);
+      segoff[c + flag] = (OffT)(off + len);
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_number_fill(int64_t n, const int32_t *__restrict__ match,
+                                                     int32_t *__restrict__ cmap)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const int32_t u = match[i];
+    if (u < i) cmap[i] = cmap[u];
+  }
+}
+
+template <typename OffT>
+static int64_t number_impl(cm_ctx *ctx, const Level &L, const int32_t *match, int32_t *cmap,
+                           int32_t *rep, OffT *segoff, int32_t *cvwgt)
+{
+  const int64_t n = L.n;
+  if (n == 0) {
+    fill_u32(ctx, segoff, 0u, sizeof(OffT) / 4);
+    return 0;
+  }
+  const int64_t tiles = (n + 1023) / 1024;
+  ensure_status(ctx, 2 * tiles);
+  fill_u64(ctx, ctx->d_status, 0ull, 2 * tiles);
+  fill_u32(ctx, ctx->d_ctr, 0u, 1);
+  launch(ctx, "number_scan", (double)sizeof(OffT) * (n + 1) * 2 + 16.0 * n, k_number<OffT>,
+         dim3((unsigned)tiles), dim3(1024), 0, n, (const OffT *)L.xadj, match, (const int32_t *)L.vwgt,
+         cmap, rep, segoff, cvwgt, ctx->d_status, ctx->d_status + tiles, ctx->d_ctr, ctx->d_small);
+  launch(ctx, "number_fill", 12.0 * n, k_number_fill, dim3(nblocks(n, 256)), dim3(256), 0, n, match,
+         cmap);
+  CM_HIP(hipMemcpyAsync(ctx->h_small, ctx->d_small, sizeof(int64_t), hipMemcpyDeviceToHost,
+                        ctx->stream));
+  sync(ctx);
+  return ctx->h_small[0];
+}
+
+int64_t number(cm_ctx *ctx, const Level &L, const int32_t *d_match, int32_t *d_cmap,
+               int32_t *d_rep, void *d_segoff, int32_t *d_cvwgt)
+{
+  return L.off64 ? number_impl(ctx, L, d_match, d_cmap, d_rep, (int64_t *)d_segoff, d_cvwgt)
+                 : number_impl(ctx, L, d_match, d_cmap, d_rep, (int32_t *)d_segoff, d_cvwgt);
+}
+
+// ------------------------------------------------- generic look-back scan
+// out[i] = sum of in[0..i), out[n] = total; tiles of 1024.
+template <typename OffT>
+__global__ __launch_bounds__(1024) void k_scan_excl(int64_t n, const int32_t *__restrict__ in,
+                                                    OffT *__restrict__ out, uint64_t *st,
+                                                    unsigned *ctr, int64_t *total)
+{
+  __shared__ uint64_t sh[16];
+  const int tile = tile_ticket(ctr);
+  const int64_t i = (int64_t)tile * 1024 + threadIdx.x;
+  const uint64_t v = i < n ? (uint64_t)(uint32_t)in[i] : 0;
+  const int w = threadIdx.x >> 6, l = lane();
+  const uint64_t iv = wave_incl_scan<uint64_t>(v);
+  if (l == CM_WAVE - 1) sh[w] = iv;
+  __syncthreads();
+  if (w == 0) {
+    const uint64_t a = l < 16 ? sh[l] : 0;
+    const uint64_t ia = wave_incl_scan<uint64_t>(a);
+    const uint64_t ta = __shfl(ia, 15, CM_WAVE);
+    const uint64_t ea = lookback(st, tile, ta);
+    if (l < 16) sh[l] = ea + ia - a;
+  }
+  __syncthreads();
+  if (i < n) {
+    const uint64_t ex = sh[w] + iv - v;
+    out[i] = (OffT)ex;
+    if (i == n - 1) {
+      out[n] = (OffT)(ex + v);
+      total[0] = (int64_t)(ex + v);
+    }
+  }
+}
+
+template <typename OffT>
+static int64_t scan_excl(cm_ctx *ctx, int64_t n, const int32_t *in, OffT *out)
+{
+  if (n == 0) {
+    fill_u32(ctx, out, 0u, sizeof(OffT) / 4);
+    return 0;
+  }
+  const int64_t tiles = (n + 1023) / 1024;
+  ensure_status(ctx, tiles);
+  fill_u64(ctx, ctx->d_status, 0ull, tiles);
+  fill_u32(ctx, ctx->d_ctr, 0u, 1);
+  launch(ctx, "scan_cxadj", 4.0 * n + sizeof(OffT) * (n + 1), k_scan_excl<OffT>, dim3((unsigned)tiles),
+         dim3(1024), 0, n, in, out, ctx->d_status, ctx->d_ctr, ctx->d_small + 1);
+  CM_HIP(hipMemcpyAsync(ctx->h_small + 1, ctx->d_small + 1, sizeof(int64_t), hipMemcpyDeviceToHost,
+                        ctx->stream));
+  sync(ctx);
+  return ctx->h_small[1];
+}
+
+// ------------------------------------------------------------------ gather
+// Fine vertex i writes its row into the segment of its coarse vertex: the
+// representative's row first, then the partner's.  Key = (coarse neighbour,
+// position in segment), value = edge weight.
+template <typename OffT>
+__global__ __launch_bounds__(256) void k_gather(int64_t n, const OffT *__restrict__ xadj,
+                                                const int32_t *__restrict__ adjncy,
+                                                const int32_t *__restrict__ adjwgt,
+                                                const int32_t *__restrict__ match,
+                                                const int32_t *__restrict__ cmap,
+                                                const OffT *__restrict__ segoff,
+                                                uint64_t *__restrict__ keys, uint32_t *__restrict__ vals)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const OffT b = xadj[i], e = xadj[i + 1];
+  if ((int64_t)(e - b) > GATHER_LONG) return;
+  const int32_t u = match[i];
+  const int64_t s0 = (int64_t)segoff[cmap[i]];
+  const uint32_t local = u < i ? (uint32_t)(xadj[u + 1] - xadj[u]) : 0u;
+  for (OffT j = b; j < e; ++j) {
+    const uint32_t pos = local + (uint32_t)(j - b);
+    keys[s0 + pos] = ((uint64_t)(uint32_t)cmap[adjncy[j]] << 32) | pos;
+    vals[s0 + pos] = (uint32_t)adjwgt[j];
+  }
+}
+
+template <typename OffT>
+__global__ __launch_bounds__(256) void k_gather_long(const int32_t *list, const unsigned *list_n,
+                                                     const OffT *__restrict__ xadj,
+                                                     const int32_t *__restrict__ adjncy,
+                                                     const int32_t *__restrict__ adjwgt,
+                                                     const int32_t *__restrict__ match,
+                                                     const int32_t *__restrict__ cmap,
+                                                     const OffT *__restrict__ segoff,
+                                                     uint64_t *__restrict__ keys,
+                                                     uint32_t *__restrict__ vals)
+{
+  const unsigned nl = *list_n;
+  const int l = lane();
+  const int64_t nw = (int64_t)gridDim.x * (blockDim.x / CM_WAVE);
+  for (int64_t wi = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / CM_WAVE; wi < nl; wi += nw) {
+    const int32_t i = list[wi];
+    const OffT b = xadj[i], e = xadj[i + 1];
+    const int32_t u = match[i];
+    const int64_t s0 = (int64_t)segoff[cmap[i]];
+    const uint32_t local = u < i ? (uint32_t)(xadj[u + 1] - xadj[u]) : 0u;
+    for (OffT j = b + l; j < e; j += CM_WAVE) {
+      const uint32_t pos = local + (uint32_t)(j - b);
+      keys[s0 + pos] = ((uint64_t)(uint32_t)cmap[adjncy[j]] << 32) | pos;
+      vals[s0 + pos] = (uint32_t)adjwgt[j];
+    }
+  }
+}
+
+template <typename OffT>
+__global__ __launch_bounds__(256) void k_select_long_rows(int64_t n, const OffT *xadj, int32_t *list,
+                                                          unsigned *count)
+{
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool is_long = v < n && (int64_t)(xadj[v + 1] - xadj[v]) > GATHER_LONG;
+  const uint64_t mk = __ballot(is_long);
+  if (!mk) return;
+  const int leader = __ffsll((unsigned long long)mk) - 1;
+  unsigned base = 0;
+  if (lane() == leader) base = atomicAdd(count, (unsigned)__popcll(mk));
+  base = __shfl(base, leader, CM_WAVE);
+  if (is_long) list[base + mask_rank(mk)] = (int32_t)v;
+}
+
+// ---------------------------------------------------------- merge: count
+// Sorted segment of coarse vertex c: a head is the first key of a run of
+// equal coarse neighbours; its low word is the neighbour's first position.
+// cdeg[c] = heads other than c.  First-seen mode also ranks the non-self
+// heads by first position (written to rank[s0+pos]) and records the self
+// head's position and rank.
+__device__ __forceinline__ uint32_t khi(uint64_t k) { return (uint32_t)(k >> 32); }
+__device__ __forceinline__ uint32_t klo(uint64_t k) { return (uint32_t)k; }
+
+template <bool FS, typename OffT>
+__global__ __launch_bounds__(256) void k_count_small(int64_t nc, const OffT *__restrict__ segoff,
+                                                     const uint64_t *__restrict__ keys,
+                                                     int32_t *__restrict__ rank,
+                                                     int32_t *__restrict__ cdeg,
+                                                     int32_t *__restrict__ selfpos,
+                                                     int32_t *__restrict__ selfrank)
+{
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  const int64_t s0 = (int64_t)segoff[c], s1 = (int64_t)segoff[c + 1];
+  const int len = (int)(s1 - s0);
+  if (len > SMALL_SEG) return;
+  if (FS)
+    for (int p = 0; p < len; ++p) rank[s0 + p] = -1;
+  int cnt = 0, ps = -1;
+  uint32_t prev = 0xFFFFFFFFu;
+  for (int64_t e = s0; e < s1; ++e) {
+    const uint64_t k = keys[e];
+    if (khi(k) != prev) {
+      prev = khi(k);
+      if (prev == (uint32_t)c) ps = (int)klo(k);
+      else {
+        ++cnt;
+        if (FS) rank[s0 + klo(k)] = 1;
+      }
+    }
+  }
+  if (FS) {
+    int r = 0, sr = -1;
+    for (int p = 0; p < len; ++p) {
+      if (p == ps) sr = r;
+      if (rank[s0 + p] >= 0) rank[s0 + p] = r++;
+    }
+    selfpos[c] = ps;
+    selfrank[c] = sr;
+  }
+  cdeg[c] = cnt;
+}
+
+template <bool FS, typename OffT>
+__global__ __launch_bounds__(256) void k_count_large(const int32_t *__restrict__ list, int64_t nlist,
+                                                     const OffT *__restrict__ segoff,
+                                                     const uint64_t *__restrict__ keys,
+                                                     int32_t *__restrict__ rank,
+                                                     int32_t *__restrict__ cdeg,
+                                                     int32_t *__restrict__ selfpos,
+                                                     int32_t *__restrict__ selfrank)
+{
+  __shared__ int64_t sh[4];
+  __shared__ int s_ps, s_sr;
+  const int t = threadIdx.x;
+  for (int64_t g = blockIdx.x; g < nlist; g += gridDim.x) {
+    const int32_t c = list[g];
+    const int64_t s0 = (int64_t)segoff[c], s1 = (int64_t)segoff[c + 1];
+    const int len = (int)(s1 - s0);
+    if (FS)
+      for (int p = t; p < len; p += 256) rank[s0 + p] = -1;
+    if (t == 0) {
+      s_ps = -1;
+      s_sr = -1;
+    }
+    __syncthreads();
+    int64_t cnt = 0;
+    for (int64_t e = s0 + t; e < s1; e += 256) {
+      const uint64_t k = keys[e];
+      const bool head = e == s0 || khi(keys[e - 1]) != khi(k);
+      if (head) {
+        if (khi(k) == (uint32_t)c) s_ps = (int)klo(k);
+        else {
+          ++cnt;
+          if (FS) rank[s0 + klo(k)] = 1;
+        }
+      }
+    }
+    int64_t total;
+    block256_excl_scan<int64_t>(cnt, total, sh);
+    if (FS) {
+      const int ps = s_ps;
+      int64_t carry = 0;
+      for (int base = 0; base < len; base += 256) {
+        const int p = base + t;
+        const int64_t f = (p < len && rank[s0 + p] >= 0) ? 1 : 0;
+        int64_t tot;
+        const int64_t ex = block256_excl_scan<int64_t>(f, tot, sh);
+        if (f) rank[s0 + p] = (int32_t)(carry + ex);
+        if (p == ps) s_sr = (int)(carry + ex);
+        carry += tot;
+      }
+    }
+    __syncthreads();
+    if (t == 0) {
+      cdeg[c] = (int32_t)total;
+      if (FS) {
+        selfpos[c] = s_ps;
+        selfrank[c] = s_sr;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ----------------------------------------------------------- merge: emit
+template <bool FS>
+__device__ __forceinline__ int64_t final_slot(int r, uint32_t p, int ps, int sr, int64_t cdeg)
+{
+  if (!FS) return r;
+  // position in the reference's list (self included), then swap-with-last
+  const int64_t r1 = r + ((ps >= 0 && (uint32_t)ps < p) ? 1 : 0);
+  return (ps >= 0 && r1 == cdeg) ? (int64_t)sr : r1;
+}
+
+template <bool FS, typename OffT>
+__global__ __launch_bounds__(256) void k_emit_small(int64_t nc, const OffT *__restrict__ segoff,
+                                                    const uint64_t *__restrict__ keys,
+                                                    const uint32_t *__restrict__ vals,
+                                                    const int32_t *__restrict__ rank,
+                                                    const int32_t *__restrict__ selfpos,
+                                                    const int32_t *__restrict__ selfrank,
+                                                    const OffT *__restrict__ cxadj,
+                                                    int32_t *__restrict__ cadjncy,
+                                                    int32_t *__restrict__ cadjwgt)
+{
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  const int64_t s0 = (int64_t)segoff[c], s1 = (int64_t)segoff[c + 1];
+  if (s1 - s0 > SMALL_SEG) return;
+  const int64_t out = (int64_t)cxadj[c], deg = (int64_t)cxadj[c + 1] - out;
+  const int ps = FS ? selfpos[c] : -1, sr = FS ? selfrank[c] : -1;
+  int r = 0;
+  for (int64_t e = s0; e < s1;) {
+    const uint64_t k = keys[e];
+    const uint32_t nb = khi(k);
+    uint32_t w = vals[e];
+    int64_t e2 = e + 1;
+    while (e2 < s1 && khi(keys[e2]) == nb) w += vals[e2++];
+    if (nb != (uint32_t)c) {
+      const int rr = FS ? rank[s0 + klo(k)] : r;
+      const int64_t slot = out + final_slot<FS>(rr, klo(k), ps, sr, deg);
+      cadjncy[slot] = (int32_t)nb;
+      cadjwgt[slot] = (int32_t)w;
+      ++r;
+    }
+    e = e2;
+  }
+}
+
+template <bool FS, typename OffT>
+__global__ __launch_bounds__(256) void k_emit_large(const int32_t *__restrict__ list, int64_t nlist,
+                                                    const OffT *__restrict__ segoff,
+                                                    const uint64_t *__restrict__ keys,
+                                                    const uint32_t *__restrict__ vals,
+                                                    const int32_t *__restrict__ rank,
+                                                    const int32_t *__restrict__ selfpos,
+                                                    const int32_t *__restrict__ selfrank,
+                                                    const OffT *__restrict__ cxadj,
+                                                    int32_t *__restrict__ cadjncy,
+                                                    int32_t *__restrict__ cadjwgt)
+{
+  __shared__ int64_t sh[4];
+  const int t = threadIdx.x;
+  for (int64_t g = blockIdx.x; g < nlist; g += gridDim.x) {
+    const int32_t c = list[g];
+    const int64_t s0 = (int64_t)segoff[c], s1 = (int64_t)segoff[c + 1];
+    const int64_t out = (int64_t)cxadj[c], deg = (int64_t)cxadj[c + 1] - out;
+    const int ps = FS ? selfpos[c] : -1, sr = FS ? selfrank[c] : -1;
+    int64_t carry = 0;
+    for (int64_t base = s0; base < s1; base += 256) {
+      const int64_t e = base + t;
+      uint64_t k = 0;
+      bool head = false;
+      if (e < s1) {
+        k = keys[e];
+        head = (e == s0 || khi(keys[e - 1]) != khi(k)) && khi(k) != (uint32_t)c;
+      }
+      int64_t ex = 0;
+      if (!FS) {
+        int64_t tot;
+        ex = block256_excl_scan<int64_t>(head ? 1 : 0, tot, sh);
+        ex += carry;
+        carry += tot;
+      }
+      if (head) {
+        const uint32_t nb = khi(k);
+        uint32_t w = vals[e];
+        for (int64_t e2 = e + 1; e2 < s1 && khi(keys[e2]) == nb; ++e2) w += vals[e2];
+        const int rr = FS ? rank[s0 + klo(k)] : (int)ex;
+        const int64_t slot = out + final_slot<FS>(rr, klo(k), ps, sr, deg);
+        cadjncy[slot] = (int32_t)nb;
+        cadjwgt[slot] = (int32_t)w;
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------- host side
+template <typename OffT>
+static int64_t contract_impl(cm_ctx *ctx, const Level &L, const int32_t *match, const int32_t *cmap,
+                             int64_t nc, const OffT *segoff, int row_order, OffT *cxadj,
+                             int32_t *cadjncy, int32_t *cadjwgt)
+{
+  const int64_t n = L.n, m = L.m;
+  const OffT *xadj = (const OffT *)L.xadj;
+  const bool fs = row_order == CM_ROW_FIRST_SEEN;
+  if (nc == 0) {
+    fill_u32(ctx, cxadj, 0u, sizeof(OffT) / 4);
+    return 0;
+  }
+  uint64_t *keys = (uint64_t *)ctx->pool.alloc(sizeof(uint64_t) * (m > 0 ? m : 1));
+  uint32_t *vals = (uint32_t *)ctx->pool.alloc(sizeof(uint32_t) * (m > 0 ? m : 1));
+  int32_t *list = (int32_t *)ctx->pool.alloc(sizeof(int32_t) * n);
+  int32_t *cdeg = (int32_t *)ctx->pool.alloc(sizeof(int32_t) * nc);
+  int32_t *rank = fs ? (int32_t *)ctx->pool.alloc(sizeof(int32_t) * (m > 0 ? m : 1)) : nullptr;
+  int32_t *selfpos = fs ? (int32_t *)ctx->pool.alloc(sizeof(int32_t) * nc) : nullptr;
+  int32_t *selfrank = fs ? (int32_t *)ctx->pool.alloc(sizeof(int32_t) * nc) : nullptr;
+  unsigned *list_n = ctx->d_ctr + 2;
+  fill_u32(ctx, list_n, 0u, 1);
+  const unsigned gn = nblocks(n, 256), gc = nblocks(nc, 256);
+  const double gather_bytes = (double)sizeof(OffT) * (n + 1) + 8.0 * m + 12.0 * n + 4.0 * m + 12.0 * m;
+  launch(ctx, "contract_select_long", 0.0, k_select_long_rows<OffT>, dim3(gn), dim3(256), 0, n, xadj,
+         list, list_n);
+  launch(ctx, "contract_gather", gather_bytes, k_gather<OffT>, dim3(gn), dim3(256), 0, n, xadj,
+         L.adjncy, L.adjwgt, match, cmap, segoff, keys, vals);
+  launch(ctx, "contract_gather_long", 0.0, k_gather_long<OffT>, dim3(1024), dim3(256), 0,
+         (const int32_t *)list, (const unsigned *)list_n, xadj, L.adjncy, L.adjwgt, match, cmap,
+         segoff, keys, vals);
+  SegBins bins;
+  bin_segments(ctx, nc, segoff, sizeof(OffT) == 8, bins);
+  sort_binned(ctx, bins, segoff, sizeof(OffT) == 8, m, keys, vals);
+  const int32_t *large = bins.list + bins.off[SEG_LARGE_BIN];
+  const int64_t nlarge = bins.off[SEG_NBINS] - bins.off[SEG_LARGE_BIN];
+  const unsigned glarge = (unsigned)std::min<int64_t>(nlarge > 0 ? nlarge : 1, 8192);
+  if (fs) {
+    launch(ctx, "contract_count", 0.0, k_count_small<true, OffT>, dim3(gc), dim3(256), 0, nc, segoff,
+           (const uint64_t *)keys, rank, cdeg, selfpos, selfrank);
+    if (nlarge)
+      launch(ctx, "contract_count_large", 0.0, k_count_large<true, OffT>, dim3(glarge), dim3(256), 0,
+             large, nlarge, segoff, (const uint64_t *)keys, rank, cdeg, selfpos, selfrank);
+  } else {
+    launch(ctx, "contract_count", 0.0, k_count_small<false, OffT>, dim3(gc), dim3(256), 0, nc, segoff,
+           (const uint64_t *)keys, rank, cdeg, selfpos, selfrank);
+    if (nlarge)
+      launch(ctx, "contract_count_large", 0.0, k_count_large<false, OffT>, dim3(glarge), dim3(256), 0,
+             large, nlarge, segoff, (const uint64_t *)keys, rank, cdeg, selfpos, selfrank);
+  }
+  const int64_t mc = scan_excl<OffT>(ctx, nc, cdeg, cxadj);
+  const double emit_bytes = 12.0 * m + (double)sizeof(OffT) * (nc + 1) + 8.0 * mc;
+  if (fs) {
+    launch(ctx, "contract_emit", emit_bytes, k_emit_small<true, OffT>, dim3(gc), dim3(256), 0, nc,
+           segoff, (const uint64_t *)keys, (const uint32_t *)vals, (const int32_t *)rank,
+           (const int32_t *)selfpos, (const int32_t *)selfrank, (const OffT *)cxadj, cadjncy, cadjwgt);
+    if (nlarge)
+      launch(ctx, "contract_emit_large", 0.0, k_emit_large<true, OffT>, dim3(glarge), dim3(256), 0,
+             large, nlarge, segoff, (const uint64_t *)keys, (const uint32_t *)vals,
+             (const int32_t *)rank, (const int32_t *)selfpos, (const int32_t *)selfrank,
+             (const OffT *)cxadj, cadjncy, cadjwgt);
+  } else {
+    launch(ctx, "contract_emit", emit_bytes, k_emit_small<false, OffT>, dim3(gc), dim3(256), 0, nc,
+           segoff, (const uint64_t *)keys, (const uint32_t *)vals, (const int32_t *)rank,
+           (const int32_t *)selfpos, (const int32_t *)selfrank, (const OffT *)cxadj, cadjncy, cadjwgt);
+    if (nlarge)
+      launch(ctx, "contract_emit_large", 0.0, k_emit_large<false, OffT>, dim3(glarge), dim3(256), 0,
+             large, nlarge, segoff, (const uint64_t *)keys, (const uint32_t *)vals,
+             (const int32_t *)rank, (const int32_t *)selfpos, (const int32_t *)selfrank,
+             (const OffT *)cxadj, cadjncy, cadjwgt);
+  }
+  sync(ctx);
+  release_bins(ctx, bins);
+  ctx->pool.release(keys);
+  ctx->pool.release(vals);
+  ctx->pool.release(list);
+  ctx->pool.release(cdeg);
+  if (fs) {
+    ctx->pool.release(rank);
+    ctx->pool.release(selfpos);
+    ctx->pool.release(selfrank);
+  }
+  return mc;
+}
+
+int64_t contract(cm_ctx *ctx, const Level &L, const int32_t *d_match, const int32_t *d_cmap,
+                 int64_t nc, const int32_t *d_rep, const void *d_segoff, int row_order,
+                 void *d_cxadj, int32_t *d_cadjncy, int32_t *d_cadjwgt)
+{
+  (void)d_rep;
+  return L.off64 ? contract_impl(ctx, L, d_match, d_cmap, nc, (const int64_t *)d_segoff, row_order,
+                                 (int64_t *)d_cxadj, d_cadjncy, d_cadjwgt)
+                 : contract_impl(ctx, L, d_match, d_cmap, nc, (const int32_t *)d_segoff, row_order,
+                                 (int32_t *)d_cxadj, d_cadjncy, d_cadjwgt);
+}
+
+// --------------------------------------------------------- (d) projection
+// where_f[i] = where_c[cmap[i]] (cu:3067-3075).
+__global__ __launch_bounds__(256) void k_project(int64_t n, const int32_t *__restrict__ cmap,
+                                                 const int32_t *__restrict__ where_c,
+                                                 int32_t *__restrict__ where_f)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) where_f[i] = where_c[cmap[i]];
+}
+
+void project(cm_ctx *ctx, int64_t n, const int32_t *d_cmap, const int32_t *d_where_c,
+             int32_t *d_where_f)
+{
+  launch(ctx, "project", 12.0 * n, k_project, dim3(nblocks(n, 256)), dim3(256), 0, n, d_cmap,
+         d_where_c, d_where_f);
+}
+
+}  // namespace cm

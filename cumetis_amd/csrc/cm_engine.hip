@@ -1,0 +1,655 @@
+// cm_engine.hip -- contexts, device memory pool, the multilevel coarsening
+// driver (replaces cuMetis_coarsen, cu:1824-1850) and the C ABI.
+#include <string.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "cm_common.hpp"
+#include "cm_internal.hpp"
+#include "cm_segsort.hpp"
+
+struct cm_hier {
+  std::vector<cm::Level> levels;
+  std::vector<void *> owned;  // pool blocks released with the hierarchy
+};
+
+namespace cm {
+
+// ------------------------------------------------------------------- pool
+static size_t size_class(size_t bytes)
+{
+  size_t c = 256;
+  while (c < bytes) c <<= 1;
+  return c;
+}
+
+Pool::~Pool()
+{
+  for (auto &kv : free_)
+    for (void *p : kv.second) (void)hipFree(p);
+  for (auto &kv : live_) (void)hipFree(kv.first);
+}
+
+void *Pool::alloc(size_t bytes)
+{
+  const size_t c = size_class(bytes ? bytes : 1);
+  auto it = free_.find(c);
+  void *p = nullptr;
+  if (it != free_.end() && !it->second.empty()) {
+    p = it->second.back();
+    it->second.pop_back();
+  } else {
+    hipError_t e = hipMalloc(&p, c);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      trim();
+      e = hipMalloc(&p, c);
+      if (e != hipSuccess) {
+        (void)hipGetLastError();
+        throw Error(CM_ERR_NOMEM, "device allocation of " + std::to_string(c) + " bytes failed");
+      }
+    }
+    reserved_ += c;
+  }
+  live_[p] = c;
+  in_use_ += c;
+  return p;
+}
+
+void Pool::release(void *p)
+{
+  if (!p) return;
+  auto it = live_.find(p);
+  if (it == live_.end()) return;
+  in_use_ -= it->second;
+  free_[it->second].push_back(p);
+  live_.erase(it);
+}
+
+void Pool::trim()
+{
+  for (auto &kv : free_) {
+    for (void *p : kv.second) {
+      (void)hipFree(p);
+      reserved_ -= kv.first;
+    }
+    kv.second.clear();
+  }
+}
+
+// ----------------------------------------------------------- small kernels
+__global__ void k_fill_u32(uint32_t *p, uint32_t v, int64_t count)
+{
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+__global__ void k_fill_u64(uint64_t *p, uint64_t v, int64_t count)
+{
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+__global__ __launch_bounds__(256) void k_sum_i32(int64_t n, const int32_t *a, unsigned long long *out)
+{
+  int64_t s = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    s += a[i];
+  s = wave_sum<int64_t>(s);
+  if (lane() == 0) atomicAdd(out, (unsigned long long)s);
+}
+
+void fill_u32(cm_ctx *ctx, void *p, uint32_t v, int64_t count)
+{
+  if (count <= 0) return;
+  launch(ctx, "fill", 4.0 * count, k_fill_u32, dim3((unsigned)std::min<int64_t>(nblocks(count, 256), 4096)),
+         dim3(256), 0, (uint32_t *)p, v, count);
+}
+
+void fill_u64(cm_ctx *ctx, void *p, uint64_t v, int64_t count)
+{
+  if (count <= 0) return;
+  launch(ctx, "fill", 8.0 * count, k_fill_u64, dim3((unsigned)std::min<int64_t>(nblocks(count, 256), 4096)),
+         dim3(256), 0, (uint64_t *)p, v, count);
+}
+
+void ensure_status(cm_ctx *ctx, size_t words)
+{
+  if (words <= ctx->status_cap) return;
+  if (ctx->d_status) ctx->pool.release(ctx->d_status);
+  size_t cap = 1024;
+  while (cap < words) cap <<= 1;
+  ctx->d_status = (uint64_t *)ctx->pool.alloc(cap * sizeof(uint64_t));
+  ctx->status_cap = cap;
+}
+
+void sync(cm_ctx *ctx)
+{
+  CM_HIP(hipStreamSynchronize(ctx->stream));
+  if (ctx->timing) ctx->collect_timing();
+}
+
+static int64_t device_sum(cm_ctx *ctx, int64_t n, const int32_t *a)
+{
+  unsigned long long *d = (unsigned long long *)(ctx->d_small + 8);
+  fill_u64(ctx, d, 0ull, 1);
+  launch(ctx, "sum", 4.0 * n, k_sum_i32, dim3((unsigned)std::min<int64_t>(nblocks(n, 256), 2048)),
+         dim3(256), 0, n, a, d);
+  CM_HIP(hipMemcpyAsync(ctx->h_small + 8, d, 8, hipMemcpyDeviceToHost, ctx->stream));
+  sync(ctx);
+  return ctx->h_small[8];
+}
+
+// ----------------------------------------------------------- graph inputs
+static int offset_bytes(int t) { return t == CM_OFF64 ? 8 : 4; }
+
+static void validate_view(const cm_graph_view *g)
+{
+  CM_REQUIRE(g != nullptr, CM_ERR_INVALID, "graph view is NULL");
+  CM_REQUIRE(g->n >= 0 && g->m >= 0, CM_ERR_INVALID, "negative graph size");
+  CM_REQUIRE(g->offset_type == CM_OFF32 || g->offset_type == CM_OFF64, CM_ERR_INVALID,
+             "offset_type must be CM_OFF32 or CM_OFF64");
+  CM_REQUIRE(g->n < (int64_t)INT32_MAX, CM_ERR_UNSUPPORTED, "vertex count must fit int32");
+  CM_REQUIRE(g->offset_type == CM_OFF64 || g->m <= (int64_t)INT32_MAX, CM_ERR_UNSUPPORTED,
+             "more than 2^31-1 entries needs CM_OFF64");
+  CM_REQUIRE(g->m <= (int64_t)UINT32_MAX, CM_ERR_UNSUPPORTED, "more than 2^32-1 entries");
+  CM_REQUIRE(g->xadj != nullptr, CM_ERR_INVALID, "xadj is NULL");
+  CM_REQUIRE(g->m == 0 || g->adjncy != nullptr, CM_ERR_INVALID, "adjncy is NULL");
+}
+
+// Level 0 of a hierarchy from a view: device views are referenced (unit
+// weights are materialised), host views are copied into pool memory.
+static Level level0(cm_ctx *ctx, const cm_graph_view *g, cm_hier *h)
+{
+  Level L;
+  L.n = g->n;
+  L.m = g->m;
+  L.off64 = g->offset_type == CM_OFF64;
+  const size_t ob = offset_bytes(g->offset_type);
+  auto own = [&](size_t bytes) {
+    void *p = ctx->pool.alloc(bytes);
+    h->owned.push_back(p);
+    return p;
+  };
+  if (g->on_device) {
+    L.xadj = const_cast<void *>(g->xadj);
+    L.adjncy = const_cast<int32_t *>(g->adjncy);
+    L.adjwgt = const_cast<int32_t *>(g->adjwgt);
+    L.vwgt = const_cast<int32_t *>(g->vwgt);
+  } else {
+    L.xadj = own(ob * (L.n + 1));
+    CM_HIP(hipMemcpyAsync(L.xadj, g->xadj, ob * (L.n + 1), hipMemcpyHostToDevice, ctx->stream));
+    L.adjncy = (int32_t *)own(4 * (L.m > 0 ? L.m : 1));
+    if (L.m)
+      CM_HIP(hipMemcpyAsync(L.adjncy, g->adjncy, 4 * L.m, hipMemcpyHostToDevice, ctx->stream));
+    if (g->adjwgt) {
+      L.adjwgt = (int32_t *)own(4 * (L.m > 0 ? L.m : 1));
+      if (L.m)
+        CM_HIP(hipMemcpyAsync(L.adjwgt, g->adjwgt, 4 * L.m, hipMemcpyHostToDevice, ctx->stream));
+    }
+    if (g->vwgt) {
+      L.vwgt = (int32_t *)own(4 * (L.n > 0 ? L.n : 1));
+      if (L.n) CM_HIP(hipMemcpyAsync(L.vwgt, g->vwgt, 4 * L.n, hipMemcpyHostToDevice, ctx->stream));
+    }
+  }
+  if (!L.adjwgt) {
+    L.adjwgt = (int32_t *)own(4 * (L.m > 0 ? L.m : 1));
+    fill_u32(ctx, L.adjwgt, 1u, L.m);
+  }
+  if (!L.vwgt) {
+    L.vwgt = (int32_t *)own(4 * (L.n > 0 ? L.n : 1));
+    fill_u32(ctx, L.vwgt, 1u, L.n);
+  }
+  if (!g->on_device && g->vwgt) {
+    int64_t s = 0;
+    for (int64_t i = 0; i < L.n; ++i) s += g->vwgt[i];
+    L.tvwgt = s;
+  } else if (!g->vwgt) {
+    L.tvwgt = L.n;
+  } else {
+    L.tvwgt = device_sum(ctx, L.n, L.vwgt);
+  }
+  return L;
+}
+
+int log2_floor(int a)
+{
+  int i;
+  for (i = 1; a > 1; i++, a = a >> 1) {
+  }
+  return i - 1;
+}
+
+// cu:3634
+int ref_threshold(int64_t n, int nparts)
+{
+  const int l = log2_floor(nparts);
+  const int64_t a = l > 0 ? n / (20 * l) : n;
+  const int64_t b = 30 * (int64_t)nparts;
+  return (int)std::max(a, b);
+}
+
+// One coarsening level: match -> number -> contract (cu:1699-1806, 1291-1406).
+static Level coarsen_level(cm_ctx *ctx, cm_hier *h, Level &F, int32_t maxvwgt, uint64_t seed,
+                           int max_rounds, int row_order)
+{
+  const size_t ob = F.off64 ? 8 : 4;
+  auto own = [&](size_t bytes) {
+    void *p = ctx->pool.alloc(bytes);
+    h->owned.push_back(p);
+    return p;
+  };
+  const int64_t n = F.n, m = F.m;
+  F.match = (int32_t *)own(4 * (n > 0 ? n : 1));
+  F.cmap = (int32_t *)own(4 * (n > 0 ? n : 1));
+  F.rounds = match_handshake(ctx, F, maxvwgt, seed, max_rounds, F.match);
+  int32_t *rep = (int32_t *)ctx->pool.alloc(4 * (n > 0 ? n : 1));
+  void *segoff = ctx->pool.alloc(ob * (n + 1));
+  Level C;
+  C.off64 = F.off64;
+  C.vwgt = (int32_t *)own(4 * (n > 0 ? n : 1));
+  const int64_t nc = number(ctx, F, F.match, F.cmap, rep, segoff, C.vwgt);
+  C.n = nc;
+  C.xadj = own(ob * (nc + 1));
+  C.adjncy = (int32_t *)own(4 * (m > 0 ? m : 1));
+  C.adjwgt = (int32_t *)own(4 * (m > 0 ? m : 1));
+  C.m = contract(ctx, F, F.match, F.cmap, nc, rep, segoff, row_order, C.xadj, C.adjncy, C.adjwgt);
+  C.tvwgt = F.tvwgt;  // cu:1400
+  C.owns_graph = true;
+  ctx->pool.release(rep);
+  ctx->pool.release(segoff);
+  return C;
+}
+
+}  // namespace cm
+
+// ------------------------------------------------------------- cm_ctx
+hipEvent_t cm_ctx::ev_get()
+{
+  if (!ev_free.empty()) {
+    hipEvent_t e = ev_free.back();
+    ev_free.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  CM_HIP(hipEventCreate(&e));
+  return e;
+}
+
+void cm_ctx::collect_timing()
+{
+  for (auto &t : pending) {
+    float ms = 0.f;
+    CM_HIP(hipEventElapsedTime(&ms, t.a, t.b));
+    auto &s = stats[t.name];
+    s.count += 1;
+    s.total_ms += ms;
+    s.algo_bytes += t.bytes;
+    ev_free.push_back(t.a);
+    ev_free.push_back(t.b);
+  }
+  pending.clear();
+}
+
+// ----------------------------------------------------------------- C ABI
+#define CM_API_BEGIN try {
+#define CM_API_END(ctx)                                     \
+  }                                                         \
+  catch (const cm::Error &e)                                \
+  {                                                         \
+    if (ctx) (ctx)->err = e.what();                         \
+    return e.code;                                          \
+  }                                                         \
+  catch (const std::exception &e)                           \
+  {                                                         \
+    if (ctx) (ctx)->err = e.what();                         \
+    return CM_ERR_INVALID;                                  \
+  }                                                         \
+  return CM_OK;
+
+extern "C" {
+
+int cm_ctx_create(int device, void *stream, cm_ctx **out)
+{
+  if (!out) return CM_ERR_INVALID;
+  *out = nullptr;
+  cm_ctx *ctx = new cm_ctx();
+  try {
+    int ndev = 0;
+    CM_HIP(hipGetDeviceCount(&ndev));
+    CM_REQUIRE(device >= 0 && device < ndev, CM_ERR_INVALID, "no such device");
+    ctx->device = device;
+    CM_HIP(hipSetDevice(device));
+    if (stream) {
+      ctx->stream = (hipStream_t)stream;
+    } else {
+      CM_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+      ctx->own_stream = true;
+    }
+    ctx->d_ctr = (unsigned *)ctx->pool.alloc(sizeof(unsigned) * cm::CTR_SLOTS);
+    ctx->d_small = (int64_t *)ctx->pool.alloc(sizeof(int64_t) * cm::SMALL_SLOTS);
+    CM_HIP(hipHostMalloc((void **)&ctx->h_small, sizeof(int64_t) * cm::SMALL_SLOTS, hipHostMallocDefault));
+    cm::fill_u32(ctx, ctx->d_ctr, 0u, cm::CTR_SLOTS);
+    cm::sync(ctx);
+  } catch (const cm::Error &e) {
+    int code = e.code;
+    delete ctx;
+    return code;
+  }
+  *out = ctx;
+  return CM_OK;
+}
+
+int cm_ctx_destroy(cm_ctx *ctx)
+{
+  if (!ctx) return CM_ERR_INVALID;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  for (auto &t : ctx->pending) {
+    (void)hipEventDestroy(t.a);
+    (void)hipEventDestroy(t.b);
+  }
+  for (auto e : ctx->ev_free) (void)hipEventDestroy(e);
+  if (ctx->h_small) (void)hipHostFree(ctx->h_small);
+  if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return CM_OK;
+}
+
+const char *cm_ctx_last_error(const cm_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int cm_ctx_sync(cm_ctx *ctx)
+{
+  if (!ctx) return CM_ERR_INVALID;
+  CM_API_BEGIN
+  cm::sync(ctx);
+  CM_API_END(ctx)
+}
+
+int cm_ctx_set_timing(cm_ctx *ctx, int on)
+{
+  if (!ctx) return CM_ERR_INVALID;
+  CM_API_BEGIN
+  cm::sync(ctx);
+  ctx->timing = on != 0;
+  CM_API_END(ctx)
+}
+
+int cm_ctx_reset_stats(cm_ctx *ctx)
+{
+  if (!ctx) return CM_ERR_INVALID;
+  CM_API_BEGIN
+  cm::sync(ctx);
+  ctx->stats.clear();
+  CM_API_END(ctx)
+}
+
+int cm_ctx_kernel_stat(cm_ctx *ctx, int i, char *name, int name_cap, int64_t *count, double *total_ms,
+                       double *algo_bytes)
+{
+  if (!ctx) return CM_ERR_INVALID;
+  CM_API_BEGIN
+  cm::sync(ctx);
+  CM_REQUIRE(i >= 0 && i < (int)ctx->stats.size(), CM_ERR_INVALID, "stat index out of range");
+  auto it = ctx->stats.begin();
+  std::advance(it, i);
+  if (name && name_cap > 0) {
+    strncpy(name, it->first.c_str(), name_cap - 1);
+    name[name_cap - 1] = 0;
+  }
+  if (count) *count = it->second.count;
+  if (total_ms) *total_ms = it->second.total_ms;
+  if (algo_bytes) *algo_bytes = it->second.algo_bytes;
+  CM_API_END(ctx)
+}
+
+int cm_ctx_pool_bytes(cm_ctx *ctx, int64_t *reserved, int64_t *in_use)
+{
+  if (!ctx) return CM_ERR_INVALID;
+  if (reserved) *reserved = (int64_t)ctx->pool.bytes_reserved();
+  if (in_use) *in_use = (int64_t)ctx->pool.bytes_in_use();
+  return CM_OK;
+}
+
+int cm_dmalloc(cm_ctx *ctx, size_t bytes, void **out)
+{
+  if (!ctx || !out) return CM_ERR_INVALID;
+  CM_API_BEGIN
+  CM_HIP(hipSetDevice(ctx->device));
+  *out = ctx->pool.alloc(bytes);
+  CM_API_END(ctx)
+}
+
+int cm_dfree(cm_ctx *ctx, void *p)
+{
+  if (!ctx) return CM_ERR_INVALID;
+  CM_API_BEGIN
+  cm::sync(ctx);
+  ctx->pool.release(p);
+  CM_API_END(ctx)
+}
+
+int cm_memcpy_h2d(cm_ctx *ctx, void *dst, const void *src, size_t bytes)
+{
+  if (!ctx) return CM_ERR_INVALID;
+  CM_API_BEGIN
+  if (bytes) CM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  cm::sync(ctx);
+  CM_API_END(ctx)
+}
+
+int cm_memcpy_d2h(cm_ctx *ctx, void *dst, const void *src, size_t bytes)
+{
+  if (!ctx) return CM_ERR_INVALID;
+  CM_API_BEGIN
+  if (bytes) CM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  cm::sync(ctx);
+  CM_API_END(ctx)
+}
+
+int cm_coarsen(cm_ctx *ctx, const cm_graph_view *g, const cm_coarsen_params *p, cm_hier **out)
+{
+  if (!ctx || !out || !p) return CM_ERR_INVALID;
+  *out = nullptr;
+  cm_hier *h = new cm_hier();
+  try {
+    CM_HIP(hipSetDevice(ctx->device));
+    cm::validate_view(g);
+    CM_REQUIRE(p->nparts >= 1, CM_ERR_INVALID, "nparts must be >= 1");
+    CM_REQUIRE(p->row_order == CM_ROW_FIRST_SEEN || p->row_order == CM_ROW_ASCENDING, CM_ERR_INVALID,
+               "row_order must be CM_ROW_FIRST_SEEN or CM_ROW_ASCENDING");
+    h->levels.push_back(cm::level0(ctx, g, h));
+    const int T = p->threshold > 0 ? p->threshold : cm::ref_threshold(g->n, p->nparts);
+    const int32_t maxvwgt =
+        p->maxvwgt > 0 ? p->maxvwgt : (int32_t)(1.5 * (double)h->levels[0].tvwgt / T);
+    for (;;) {
+      cm::Level C = cm::coarsen_level(ctx, h, h->levels.back(), maxvwgt, p->seed, p->max_rounds,
+                                      p->row_order);
+      const int64_t nf = h->levels.back().n;
+      h->levels.push_back(C);
+      const bool more = C.n > T && C.n < 0.75 * nf && C.m > C.n / 2;  // cu:1843-1845
+      if (!more) break;
+      if (p->max_levels > 0 && (int)h->levels.size() > p->max_levels) break;
+    }
+    cm::sync(ctx);
+  } catch (const cm::Error &e) {
+    ctx->err = e.what();
+    for (void *q : h->owned) ctx->pool.release(q);
+    delete h;
+    return e.code;
+  }
+  *out = h;
+  return CM_OK;
+}
+
+int cm_hier_num_levels(const cm_hier *h) { return h ? (int)h->levels.size() : CM_ERR_INVALID; }
+
+int cm_hier_level(const cm_hier *h, int level, cm_level_info *out)
+{
+  if (!h || !out || level < 0 || level >= (int)h->levels.size()) return CM_ERR_INVALID;
+  const cm::Level &L = h->levels[level];
+  out->n = L.n;
+  out->m = L.m;
+  out->offset_type = L.off64 ? CM_OFF64 : CM_OFF32;
+  out->rounds = L.rounds;
+  out->xadj = L.xadj;
+  out->adjncy = L.adjncy;
+  out->adjwgt = L.adjwgt;
+  out->vwgt = L.vwgt;
+  out->match = L.match;
+  out->cmap = L.cmap;
+  out->tvwgt = L.tvwgt;
+  return CM_OK;
+}
+
+int cm_hier_copy_level(cm_ctx *ctx, const cm_hier *h, int level, void *xadj, int32_t *adjncy,
+                       int32_t *adjwgt, int32_t *vwgt, int32_t *match, int32_t *cmap)
+{
+  if (!ctx || !h || level < 0 || level >= (int)h->levels.size()) return CM_ERR_INVALID;
+  CM_API_BEGIN
+  const cm::Level &L = h->levels[level];
+  const size_t ob = L.off64 ? 8 : 4;
+  auto cp = [&](void *dst, const void *src, size_t bytes) {
+    if (dst && src && bytes)
+      CM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  };
+  cp(xadj, L.xadj, ob * (L.n + 1));
+  cp(adjncy, L.adjncy, 4 * L.m);
+  cp(adjwgt, L.adjwgt, 4 * L.m);
+  cp(vwgt, L.vwgt, 4 * L.n);
+  cp(match, L.match, 4 * L.n);
+  cp(cmap, L.cmap, 4 * L.n);
+  cm::sync(ctx);
+  CM_API_END(ctx)
+}
+
+int cm_project(cm_ctx *ctx, const cm_hier *h, int level, const int32_t *d_where_c, int32_t *d_where_f)
+{
+  if (!ctx || !h || level < 0 || level + 1 >= (int)h->levels.size()) return CM_ERR_INVALID;
+  CM_API_BEGIN
+  const cm::Level &L = h->levels[level];
+  cm::project(ctx, L.n, L.cmap, d_where_c, d_where_f);
+  CM_API_END(ctx)
+}
+
+int cm_hier_release(cm_ctx *ctx, cm_hier *h)
+{
+  if (!ctx || !h) return CM_ERR_INVALID;
+  CM_API_BEGIN
+  cm::sync(ctx);
+  for (void *q : h->owned) ctx->pool.release(q);
+  delete h;
+  CM_API_END(ctx)
+}
+
+int cm_match(cm_ctx *ctx, const cm_graph_view *g, int32_t maxvwgt, uint64_t seed, int32_t max_rounds,
+             int32_t *d_match, int32_t *rounds)
+{
+  if (!ctx) return CM_ERR_INVALID;
+  CM_API_BEGIN
+  cm::validate_view(g);
+  CM_REQUIRE(g->on_device, CM_ERR_INVALID, "cm_match needs a device view");
+  cm_hier tmp;
+  cm::Level L = cm::level0(ctx, g, &tmp);
+  int r = 0;
+  try {
+    r = cm::match_handshake(ctx, L, maxvwgt, seed, max_rounds, d_match);
+  } catch (...) {
+    for (void *q : tmp.owned) ctx->pool.release(q);
+    throw;
+  }
+  cm::sync(ctx);
+  for (void *q : tmp.owned) ctx->pool.release(q);
+  if (rounds) *rounds = r;
+  CM_API_END(ctx)
+}
+
+int cm_number(cm_ctx *ctx, int64_t n, const void *d_xadj, int32_t offset_type, const int32_t *d_match,
+              int32_t *d_cmap, int64_t *nc)
+{
+  if (!ctx || n < 0) return CM_ERR_INVALID;
+  CM_API_BEGIN
+  cm::Level L;
+  L.n = n;
+  L.off64 = offset_type == CM_OFF64;
+  L.xadj = const_cast<void *>(d_xadj);
+  const size_t ob = L.off64 ? 8 : 4;
+  int32_t *vw = (int32_t *)ctx->pool.alloc(4 * (n > 0 ? n : 1));
+  cm::fill_u32(ctx, vw, 1u, n);
+  L.vwgt = vw;
+  int32_t *rep = (int32_t *)ctx->pool.alloc(4 * (n > 0 ? n : 1));
+  int32_t *cv = (int32_t *)ctx->pool.alloc(4 * (n > 0 ? n : 1));
+  void *seg = ctx->pool.alloc(ob * (n + 1));
+  int64_t r = cm::number(ctx, L, d_match, d_cmap, rep, seg, cv);
+  cm::sync(ctx);
+  ctx->pool.release(vw);
+  ctx->pool.release(rep);
+  ctx->pool.release(cv);
+  ctx->pool.release(seg);
+  if (nc) *nc = r;
+  CM_API_END(ctx)
+}
+
+int cm_contract(cm_ctx *ctx, const cm_graph_view *g, const int32_t *d_match, const int32_t *d_cmap,
+                int64_t nc, int32_t row_order, void *d_cxadj, int32_t *d_cadjncy, int32_t *d_cadjwgt,
+                int32_t *d_cvwgt, int64_t *mc)
+{
+  if (!ctx) return CM_ERR_INVALID;
+  CM_API_BEGIN
+  cm::validate_view(g);
+  CM_REQUIRE(g->on_device, CM_ERR_INVALID, "cm_contract needs a device view");
+  cm_hier tmp;
+  cm::Level L = cm::level0(ctx, g, &tmp);
+  const size_t ob = L.off64 ? 8 : 4;
+  const int64_t n = L.n;
+  int32_t *cm2 = (int32_t *)ctx->pool.alloc(4 * (n > 0 ? n : 1));
+  int32_t *rep = (int32_t *)ctx->pool.alloc(4 * (n > 0 ? n : 1));
+  void *seg = ctx->pool.alloc(ob * (n + 1));
+  int64_t out = 0;
+  try {
+    // numbering recomputed to obtain the segment offsets; it must agree
+    // with the caller's cmap, which is what the gather uses
+    const int64_t nc2 = cm::number(ctx, L, d_match, cm2, rep, seg, d_cvwgt);
+    CM_REQUIRE(nc2 == nc, CM_ERR_INVALID, "nc does not match the matching");
+    out = cm::contract(ctx, L, d_match, d_cmap, nc, rep, seg, row_order, d_cxadj, d_cadjncy, d_cadjwgt);
+  } catch (...) {
+    ctx->pool.release(cm2);
+    ctx->pool.release(rep);
+    ctx->pool.release(seg);
+    for (void *q : tmp.owned) ctx->pool.release(q);
+    throw;
+  }
+  cm::sync(ctx);
+  ctx->pool.release(cm2);
+  ctx->pool.release(rep);
+  ctx->pool.release(seg);
+  for (void *q : tmp.owned) ctx->pool.release(q);
+  if (mc) *mc = out;
+  CM_API_END(ctx)
+}
+
+int cm_segsort_i32(cm_ctx *ctx, int32_t *d_keys, int32_t *d_vals, int32_t n, const int32_t *d_segs,
+                   int32_t length)
+{
+  if (!ctx || n < 0 || length < 0) return CM_ERR_INVALID;
+  CM_API_BEGIN
+  cm::segsort_i32(ctx, d_keys, d_vals, n, d_segs, length);
+  CM_API_END(ctx)
+}
+
+int cm_project_raw(cm_ctx *ctx, int64_t n, const int32_t *d_cmap, const int32_t *d_where_c,
+                   int32_t *d_where_f)
+{
+  if (!ctx || n < 0) return CM_ERR_INVALID;
+  CM_API_BEGIN
+  cm::project(ctx, n, d_cmap, d_where_c, d_where_f);
+  cm::sync(ctx);
+  CM_API_END(ctx)
+}
+
+}  // extern "C"

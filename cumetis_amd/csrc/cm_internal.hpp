@@ -1,0 +1,155 @@
+// cm_internal.hpp -- host-side internals of the coarsening engine: device
+// memory pool, per-context state, kernel launch with optional hipEvent timing,
+// and the level record of the hierarchy.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/cumetis_amd.h"
+
+namespace cm {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define CM_HIP(expr)                                                                    \
+  do {                                                                                  \
+    hipError_t _e = (expr);                                                             \
+    if (_e != hipSuccess)                                                               \
+      throw ::cm::Error(CM_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+  } while (0)
+
+#define CM_REQUIRE(cond, code, msg)                      \
+  do {                                                   \
+    if (!(cond)) throw ::cm::Error((code), (msg));       \
+  } while (0)
+
+// Caching device allocator: power-of-two size classes, never returns memory
+// to the runtime before the context dies, so steady-state coarsening calls
+// allocate nothing.
+class Pool {
+ public:
+  ~Pool();
+  void *alloc(size_t bytes);
+  void release(void *p);
+  size_t bytes_reserved() const { return reserved_; }
+  size_t bytes_in_use() const { return in_use_; }
+  void trim();
+
+ private:
+  std::map<size_t, std::vector<void *>> free_;
+  std::map<void *, size_t> live_;
+  size_t reserved_ = 0, in_use_ = 0;
+};
+
+struct KStat {
+  int64_t count = 0;
+  double total_ms = 0.0;
+  double algo_bytes = 0.0;
+};
+
+struct Timed {
+  const char *name;
+  hipEvent_t a, b;
+  double bytes;
+};
+
+struct Level {
+  int64_t n = 0, m = 0;
+  bool off64 = false;
+  void *xadj = nullptr;
+  int32_t *adjncy = nullptr, *adjwgt = nullptr, *vwgt = nullptr;
+  int32_t *cmap = nullptr, *match = nullptr;
+  int64_t tvwgt = 0;
+  bool owns_graph = false;  // xadj/adjncy/adjwgt/vwgt come from the pool
+  int rounds = 0;           // handshake rounds used at this level
+};
+
+}  // namespace cm
+
+struct cm_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  cm::Pool pool;
+  std::string err;
+  bool timing = false;
+  std::vector<cm::Timed> pending;
+  std::vector<hipEvent_t> ev_free;
+  std::map<std::string, cm::KStat> stats;
+  // small persistent device buffers
+  unsigned *d_ctr = nullptr;      // tile tickets / counters (CTR_SLOTS)
+  int64_t *d_small = nullptr;     // totals written by kernels (SMALL_SLOTS)
+  int64_t *h_small = nullptr;     // pinned mirror of d_small
+  uint64_t *d_status = nullptr;   // look-back status words
+  size_t status_cap = 0;          // in words
+  int64_t last_rounds = 0;
+
+  hipEvent_t ev_get();
+  void collect_timing();
+};
+
+namespace cm {
+
+constexpr int CTR_SLOTS = 64;
+constexpr int SMALL_SLOTS = 4096;
+
+void ensure_status(cm_ctx *ctx, size_t words);
+
+template <typename K, typename... Args>
+inline void launch(cm_ctx *ctx, const char *name, double bytes, K kernel, dim3 grid, dim3 block,
+                   size_t shmem, Args... args)
+{
+  if (grid.x == 0 || grid.y == 0 || grid.z == 0) return;
+  hipEvent_t a = nullptr, b = nullptr;
+  if (ctx->timing) {
+    a = ctx->ev_get();
+    CM_HIP(hipEventRecord(a, ctx->stream));
+  }
+  hipLaunchKernelGGL(kernel, grid, block, shmem, ctx->stream, args...);
+  CM_HIP(hipGetLastError());
+  if (ctx->timing) {
+    b = ctx->ev_get();
+    CM_HIP(hipEventRecord(b, ctx->stream));
+    ctx->pending.push_back(Timed{name, a, b, bytes});
+  }
+}
+
+inline unsigned nblocks(int64_t work, int per_block)
+{
+  int64_t b = (work + per_block - 1) / per_block;
+  return (unsigned)(b < 1 ? 1 : b);
+}
+
+void sync(cm_ctx *ctx);
+void fill_u32(cm_ctx *ctx, void *p, uint32_t v, int64_t count);
+void fill_u64(cm_ctx *ctx, void *p, uint64_t v, int64_t count);
+
+// ---- subsystem entry points (device pointers, ctx stream) ----
+// (a) matching; returns rounds.  match[] ends complete (partner or self).
+int match_handshake(cm_ctx *ctx, const Level &L, int32_t maxvwgt, uint64_t seed, int max_rounds,
+                    int32_t *d_match);
+// (b) numbering: cmap for every fine vertex; rep/segoff/cvwgt per coarse vertex.
+//     Returns nc; segoff[nc] = m.
+int64_t number(cm_ctx *ctx, const Level &L, const int32_t *d_match, int32_t *d_cmap,
+               int32_t *d_rep, void *d_segoff, int32_t *d_cvwgt);
+// (c) contraction of L into coarse CSR; returns mc.  cxadj/cadjncy/cadjwgt
+//     must hold nc+1 / L.m / L.m entries.
+int64_t contract(cm_ctx *ctx, const Level &L, const int32_t *d_match, const int32_t *d_cmap,
+                 int64_t nc, const int32_t *d_rep, const void *d_segoff, int row_order,
+                 void *d_cxadj, int32_t *d_cadjncy, int32_t *d_cadjwgt);
+// segmented sort of 64-bit keys with 32-bit values, segments segoff[0..nseg]
+void segsort_u64(cm_ctx *ctx, int64_t nseg, const void *d_segoff, bool off64, int64_t total,
+                 uint64_t *d_keys, uint32_t *d_vals);
+// (d) projection
+void project(cm_ctx *ctx, int64_t n, const int32_t *d_cmap, const int32_t *d_where_c,
+             int32_t *d_where_f);
+
+}  // namespace cm

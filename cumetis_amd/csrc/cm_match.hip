@@ -1,0 +1,244 @@
+// cm_match.hip -- (a) deterministic seeded handshake heavy-edge matching.
+//
+// Replaces the reference's matching steps (cu:1699-1747: init cu:1411-1419,
+// per-row weight sort cu:1723, racy cuda_shem2 cu:1563-1595, conflict cleanup
+// cu:1600-1630).  Synchronous rounds over the round-start state:
+//   propose: an unmatched v keeps last round's proposal if that vertex is
+//            still unmatched (it is then still v's best), otherwise scans its
+//            row for the maximum eligible edge -- unmatched neighbour u with
+//            vwgt[v]+vwgt[u] <= maxvwgt -- under the strict total order
+//            (adjwgt, hash(seed, min, max), min, max); no candidate => dead.
+//   accept:  dead => self;  mutual proposals => matched.
+// The heaviest remaining eligible edge is always mutual, so every round makes
+// progress and the loop ends with a maximal matching.  Ordering only by
+// vertex ids and the seed makes the result independent of launch shape and
+// of a graph's position inside a batch.  The fine CSR is never reordered
+// (the reference sorts every row in place, cu:1723).
+#include "cm_common.hpp"
+#include "cm_internal.hpp"
+
+namespace cm {
+
+constexpr int LONG_DEG = 64;  // rows longer than this are scanned by a whole wave
+constexpr int ROUND_BATCH = 8;  // rounds queued between host checks
+
+// cnt[2r] = vertices that matched or died in round r, cnt[2r+1] = still active.
+__device__ __forceinline__ bool round_live(const unsigned *cnt, int r)
+{
+  if (r == 0) return true;
+  return cnt[2 * (r - 1)] != 0 && cnt[2 * (r - 1) + 1] != 0;
+}
+
+__device__ __forceinline__ bool key_better(uint64_t a1, uint64_t a2, uint64_t b1, uint64_t b2)
+{
+  return a1 > b1 || (a1 == b1 && a2 > b2);
+}
+
+__global__ __launch_bounds__(256) void k_match_init(int64_t n, int32_t *match, int32_t *prop)
+{
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v < n) {
+    match[v] = -1;
+    prop[v] = -1;
+  }
+}
+
+template <typename OffT>
+__global__ __launch_bounds__(256) void k_select_long(int64_t n, const OffT *xadj, int32_t *list,
+                                                     unsigned *count)
+{
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool is_long = v < n && (int64_t)(xadj[v + 1] - xadj[v]) > LONG_DEG;
+  const uint64_t mk = __ballot(is_long);
+  if (!mk) return;
+  const int leader = __ffsll((unsigned long long)mk) - 1;
+  unsigned base = 0;
+  if (lane() == leader) base = atomicAdd(count, (unsigned)__popcll(mk));
+  base = __shfl(base, leader, CM_WAVE);
+  if (is_long) list[base + mask_rank(mk)] = (int32_t)v;
+}
+
+template <typename OffT>
+__global__ __launch_bounds__(256) void k_propose(int64_t n, const OffT *__restrict__ xadj,
+                                                 const int32_t *__restrict__ adjncy,
+                                                 const int32_t *__restrict__ adjwgt,
+                                                 const int32_t *__restrict__ vwgt, int32_t maxvwgt,
+                                                 uint64_t seed, const int32_t *__restrict__ match,
+                                                 int32_t *__restrict__ prop, const unsigned *cnt, int r)
+{
+  if (!round_live(cnt, r)) return;
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n || match[v] != -1) return;
+  const OffT b = xadj[v], e = xadj[v + 1];
+  if ((int64_t)(e - b) > LONG_DEG) return;
+  const int32_t p = prop[v];
+  if (p >= 0 && match[p] == -1) return;
+  const int32_t vw = vwgt[v];
+  int32_t best = -2;
+  uint64_t bk1 = 0, bk2 = 0;
+  for (OffT j = b; j < e; ++j) {
+    const int32_t u = adjncy[j];
+    const uint32_t w = (uint32_t)adjwgt[j];
+    if (best != -2 && w < (uint32_t)(bk1 >> 32)) continue;
+    if (u == (int32_t)v || match[u] != -1 || vw + vwgt[u] > maxvwgt) continue;
+    const uint32_t a = (uint32_t)min((int64_t)u, v), bb = (uint32_t)max((int64_t)u, v);
+    const uint64_t k1 = ((uint64_t)w << 32) | edge_hash(seed, a, bb);
+    const uint64_t k2 = ((uint64_t)a << 32) | bb;
+    if (best == -2 || key_better(k1, k2, bk1, bk2)) {
+      best = u;
+      bk1 = k1;
+      bk2 = k2;
+    }
+  }
+  prop[v] = best;
+}
+
+// One wave per long row; lanes stride the row, then a wave arg-max.
+template <typename OffT>
+__global__ __launch_bounds__(256) void k_propose_long(const int32_t *list, const unsigned *list_n,
+                                                      const OffT *__restrict__ xadj,
+                                                      const int32_t *__restrict__ adjncy,
+                                                      const int32_t *__restrict__ adjwgt,
+                                                      const int32_t *__restrict__ vwgt,
+                                                      int32_t maxvwgt, uint64_t seed,
+                                                      const int32_t *__restrict__ match,
+                                                      int32_t *__restrict__ prop, const unsigned *cnt,
+                                                      int r)
+{
+  if (!round_live(cnt, r)) return;
+  const unsigned nl = *list_n;
+  const int l = lane();
+  const int64_t nw = (int64_t)gridDim.x * (blockDim.x / CM_WAVE);
+  for (int64_t wi = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / CM_WAVE; wi < nl; wi += nw) {
+    const int32_t v = list[wi];
+    if (match[v] != -1) continue;
+    const int32_t p = prop[v];
+    if (p >= 0 && match[p] == -1) continue;
+    const OffT b = xadj[v], e = xadj[v + 1];
+    const int32_t vw = vwgt[v];
+    int32_t best = -2;
+    uint64_t bk1 = 0, bk2 = 0;
+    for (OffT j = b + l; j < e; j += CM_WAVE) {
+      const int32_t u = adjncy[j];
+      const uint32_t w = (uint32_t)adjwgt[j];
+      if (best != -2 && w < (uint32_t)(bk1 >> 32)) continue;
+      if (u == v || match[u] != -1 || vw + vwgt[u] > maxvwgt) continue;
+      const uint32_t a = (uint32_t)min(u, v), bb = (uint32_t)max(u, v);
+      const uint64_t k1 = ((uint64_t)w << 32) | edge_hash(seed, a, bb);
+      const uint64_t k2 = ((uint64_t)a << 32) | bb;
+      if (best == -2 || key_better(k1, k2, bk1, bk2)) {
+        best = u;
+        bk1 = k1;
+        bk2 = k2;
+      }
+    }
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) {
+      const int32_t ob = __shfl_xor(best, d, CM_WAVE);
+      const uint64_t o1 = __shfl_xor(bk1, d, CM_WAVE);
+      const uint64_t o2 = __shfl_xor(bk2, d, CM_WAVE);
+      if (ob != -2 && (best == -2 || key_better(o1, o2, bk1, bk2))) {
+        best = ob;
+        bk1 = o1;
+        bk2 = o2;
+      }
+    }
+    if (l == 0) prop[v] = best;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_accept(int64_t n, int32_t *match, const int32_t *prop,
+                                                unsigned *cnt, int r)
+{
+  if (!round_live(cnt, r)) return;
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  bool changed = false, active = false;
+  if (v < n && match[v] == -1) {
+    const int32_t p = prop[v];
+    if (p == -2) {
+      match[v] = (int32_t)v;
+      changed = true;
+    } else if (p >= 0 && prop[p] == (int32_t)v) {
+      match[v] = p;
+      changed = true;
+    } else {
+      active = true;
+    }
+  }
+  const uint64_t cm_ = __ballot(changed), am = __ballot(active);
+  if (lane() == 0) {
+    if (cm_) atomicAdd(&cnt[2 * r], (unsigned)__popcll(cm_));
+    if (am) atomicAdd(&cnt[2 * r + 1], (unsigned)__popcll(am));
+  }
+}
+
+__global__ __launch_bounds__(256) void k_match_finish(int64_t n, int32_t *match)
+{
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v < n && match[v] == -1) match[v] = (int32_t)v;
+}
+
+template <typename OffT>
+static int match_impl(cm_ctx *ctx, const Level &L, int32_t maxvwgt, uint64_t seed, int max_rounds,
+                      int32_t *match)
+{
+  const int64_t n = L.n;
+  if (n == 0) return 0;
+  const OffT *xadj = (const OffT *)L.xadj;
+  int32_t *prop = (int32_t *)ctx->pool.alloc(sizeof(int32_t) * n);
+  int32_t *list = (int32_t *)ctx->pool.alloc(sizeof(int32_t) * n);
+  unsigned *cnt = (unsigned *)ctx->pool.alloc(sizeof(unsigned) * (2 * (size_t)max_rounds + 2));
+  unsigned *list_n = cnt + 2 * (size_t)max_rounds;
+  fill_u32(ctx, cnt, 0u, 2 * (int64_t)max_rounds + 2);
+  const unsigned g = nblocks(n, 256);
+  // per-round bytes: rows of active vertices; only the first round is priced
+  const double row_bytes = (double)sizeof(OffT) * (n + 1) + 8.0 * L.m + 8.0 * n;
+  launch(ctx, "match_init", 8.0 * n, k_match_init, dim3(g), dim3(256), 0, n, match, prop);
+  launch(ctx, "match_select_long", (double)sizeof(OffT) * (n + 1), k_select_long<OffT>, dim3(g),
+         dim3(256), 0, n, xadj, list, list_n);
+  const unsigned glong = 1024;  // waves stride over the long list
+  int r = 0;
+  std::vector<unsigned> h(2 * ROUND_BATCH);
+  for (;;) {
+    const int r0 = r;
+    for (int i = 0; i < ROUND_BATCH && r < max_rounds; ++i, ++r) {
+      launch(ctx, "match_propose", r == 0 ? row_bytes : 0.0, k_propose<OffT>, dim3(g), dim3(256),
+             0, n, xadj, L.adjncy, L.adjwgt, L.vwgt, maxvwgt, seed, (const int32_t *)match, prop,
+             (const unsigned *)cnt, r);
+      launch(ctx, "match_propose_long", 0.0, k_propose_long<OffT>, dim3(glong), dim3(256), 0,
+             (const int32_t *)list, (const unsigned *)list_n, xadj, L.adjncy, L.adjwgt, L.vwgt,
+             maxvwgt, seed, (const int32_t *)match, prop, (const unsigned *)cnt, r);
+      launch(ctx, "match_accept", 12.0 * n, k_accept, dim3(g), dim3(256), 0, n, match,
+             (const int32_t *)prop, cnt, r);
+    }
+    const int nr = r - r0;
+    CM_HIP(hipMemcpyAsync(h.data(), cnt + 2 * r0, sizeof(unsigned) * 2 * nr, hipMemcpyDeviceToHost,
+                          ctx->stream));
+    sync(ctx);
+    const unsigned ch = h[2 * (nr - 1)], act = h[2 * (nr - 1) + 1];
+    if (ch == 0 || act == 0 || r >= max_rounds) break;
+  }
+  // rounds that did work: round 0, then every round whose predecessor changed
+  // something and left active vertices
+  std::vector<unsigned> all(2 * (size_t)r);
+  CM_HIP(hipMemcpyAsync(all.data(), cnt, sizeof(unsigned) * 2 * r, hipMemcpyDeviceToHost,
+                        ctx->stream));
+  launch(ctx, "match_finish", 4.0 * n, k_match_finish, dim3(g), dim3(256), 0, n, match);
+  sync(ctx);
+  int rounds = 1;
+  while (rounds < r && all[2 * (rounds - 1)] != 0 && all[2 * (rounds - 1) + 1] != 0) ++rounds;
+  ctx->pool.release(prop);
+  ctx->pool.release(list);
+  ctx->pool.release(cnt);
+  return rounds;
+}
+
+int match_handshake(cm_ctx *ctx, const Level &L, int32_t maxvwgt, uint64_t seed, int max_rounds,
+                    int32_t *d_match)
+{
+  if (max_rounds <= 0) max_rounds = 4096;
+  return L.off64 ? match_impl<int64_t>(ctx, L, maxvwgt, seed, max_rounds, d_match)
+                 : match_impl<int32_t>(ctx, L, maxvwgt, seed, max_rounds, d_match);
+}
+
+}  // namespace cm

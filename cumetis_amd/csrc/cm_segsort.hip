@@ -1,0 +1,432 @@
+// cm_segsort.hip -- binned segmented sort of 64-bit keys carrying 32-bit
+// values.  Plays the role of the reference's bb_segsort
+// (include/bb_segsort.h:35-192, bb_bin.h, bb_comput_s.h, bb_comput_l.h),
+// redesigned for wave64:
+//   bin 0..5   (2..64 keys)   one sub-wave group of W = 2..64 lanes per
+//                             segment, one key per lane, bitonic network over
+//                             cross-lane shuffles;
+//   bin 6..11  (65..4096)     one 256-thread workgroup per segment, bitonic
+//                             network in LDS (padded to 128..4096);
+//   bin 12     (> 4096)       hub rows: 4096-key LDS tile sorts, then
+//                             merge-path passes that double the run width.
+// Keys are unique within a segment wherever this engine uses the sort (they
+// carry the original position in their low bits), so the result is fully
+// determined.
+#include "cm_common.hpp"
+#include "cm_internal.hpp"
+#include "cm_segsort.hpp"
+
+#include <algorithm>
+
+namespace cm {
+
+__device__ __forceinline__ int seg_class(int64_t len)
+{
+  if (len <= 1) return -1;
+  if (len > 4096) return 12;
+  // ceil(log2(len)) - 1 for len in [2, 4096]
+  const int lg = 64 - __clzll((unsigned long long)(len - 1));
+  return lg - 1;
+}
+
+template <typename OffT>
+__global__ __launch_bounds__(256) void k_bin_count(int64_t nseg, const OffT *segoff, unsigned *cnt)
+{
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int cls = s < nseg ? seg_class((int64_t)(segoff[s + 1] - segoff[s])) : -1;
+  for (int c = 0; c < SEG_NBINS; ++c) {
+    const uint64_t mk = __ballot(cls == c);
+    if (mk && lane() == __ffsll((unsigned long long)mk) - 1) atomicAdd(&cnt[c], (unsigned)__popcll(mk));
+  }
+}
+
+template <typename OffT>
+__global__ __launch_bounds__(256) void k_bin_scatter(int64_t nseg, const OffT *segoff,
+                                                     const unsigned *cnt, unsigned *cursor,
+                                                     int32_t *list)
+{
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int cls = s < nseg ? seg_class((int64_t)(segoff[s + 1] - segoff[s])) : -1;
+  unsigned base = 0;
+  for (int c = 0; c < SEG_NBINS; ++c) {
+    const uint64_t mk = __ballot(cls == c);
+    if (mk) {
+      const int leader = __ffsll((unsigned long long)mk) - 1;
+      unsigned pos = 0;
+      if (lane() == leader) pos = atomicAdd(&cursor[c], (unsigned)__popcll(mk));
+      pos = __shfl(pos, leader, CM_WAVE);
+      if (cls == c) list[base + pos + mask_rank(mk)] = (int32_t)s;
+    }
+    base += cnt[c];
+  }
+}
+
+// ------------------------------------------------ bins 0..5: in registers
+template <int W, typename OffT>
+__global__ __launch_bounds__(256) void k_sort_wave(const int32_t *list, int64_t count,
+                                                   const OffT *segoff, uint64_t *keys,
+                                                   uint32_t *vals)
+{
+  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t g = gid / W;
+  const int t = (int)(gid % W);
+  uint64_t key = ~0ull;
+  uint32_t val = 0;
+  int64_t s0 = 0;
+  int len = 0;
+  if (g < count) {
+    const int32_t s = list[g];
+    s0 = (int64_t)segoff[s];
+    len = (int)((int64_t)segoff[s + 1] - s0);
+    if (t < len) {
+      key = keys[s0 + t];
+      val = vals[s0 + t];
+    }
+  }
+#pragma unroll
+  for (int k = 2; k <= W; k <<= 1) {
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      const uint64_t ok = __shfl_xor(key, j, CM_WAVE);
+      const uint32_t ov = __shfl_xor(val, j, CM_WAVE);
+      const bool up = (t & k) == 0;
+      const bool lower = (t & j) == 0;
+      const bool take = (lower == up) ? (ok < key) : (ok > key);
+      if (take) {
+        key = ok;
+        val = ov;
+      }
+    }
+  }
+  if (g < count && t < len) {
+    keys[s0 + t] = key;
+    vals[s0 + t] = val;
+  }
+}
+
+// ------------------------------------------------ bins 6..11: LDS bitonic
+template <int S>
+__device__ __forceinline__ void lds_sort(uint64_t *keys, uint32_t *vals, int64_t s0, int len,
+                                         uint64_t *sk, uint32_t *sv)
+{
+  for (int i = threadIdx.x; i < S; i += blockDim.x) {
+    if (i < len) {
+      sk[i] = keys[s0 + i];
+      sv[i] = vals[s0 + i];
+    } else {
+      sk[i] = ~0ull;
+      sv[i] = 0;
+    }
+  }
+  __syncthreads();
+  for (int k = 2; k <= S; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int t = threadIdx.x; t < S / 2; t += blockDim.x) {
+        const int i = 2 * j * (t / j) + (t % j);
+        const int l = i + j;
+        const bool up = (i & k) == 0;
+        const uint64_t a = sk[i], b = sk[l];
+        if (up ? (a > b) : (a < b)) {
+          sk[i] = b;
+          sk[l] = a;
+          const uint32_t x = sv[i];
+          sv[i] = sv[l];
+          sv[l] = x;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int i = threadIdx.x; i < len; i += blockDim.x) {
+    keys[s0 + i] = sk[i];
+    vals[s0 + i] = sv[i];
+  }
+}
+
+template <int S, typename OffT>
+__global__ __launch_bounds__(256) void k_sort_lds(const int32_t *list, int64_t count,
+                                                  const OffT *segoff, uint64_t *keys,
+                                                  uint32_t *vals)
+{
+  __shared__ uint64_t sk[S];
+  __shared__ uint32_t sv[S];
+  for (int64_t g = blockIdx.x; g < count; g += gridDim.x) {
+    const int32_t s = list[g];
+    const int64_t s0 = (int64_t)segoff[s];
+    const int len = (int)((int64_t)segoff[s + 1] - s0);
+    lds_sort<S>(keys, vals, s0, len, sk, sv);
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void k_sort_tiles(const MergeDesc *tiles, int64_t count,
+                                                    uint64_t *keys, uint32_t *vals)
+{
+  __shared__ uint64_t sk[HUB_TILE];
+  __shared__ uint32_t sv[HUB_TILE];
+  for (int64_t g = blockIdx.x; g < count; g += gridDim.x) {
+    const MergeDesc d = tiles[g];
+    lds_sort<HUB_TILE>(keys, vals, d.a + d.d0, d.alen, sk, sv);
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------ bin 12: merge-path passes
+// One workgroup writes HUB_TILE outputs of the merge of runs A=[a, a+alen)
+// and B=[a+alen, a+alen+blen); each thread locates its first output with a
+// co-rank binary search and merges MERGE_ITEMS outputs sequentially.
+constexpr int MERGE_ITEMS = HUB_TILE / 256;
+
+__global__ __launch_bounds__(256) void k_merge(const MergeDesc *descs, int64_t count,
+                                               const uint64_t *sk, const uint32_t *sv,
+                                               uint64_t *dk, uint32_t *dv)
+{
+  for (int64_t g = blockIdx.x; g < count; g += gridDim.x) {
+    const MergeDesc q = descs[g];
+    const int alen = q.alen, blen = q.blen, total = alen + blen;
+    const uint64_t *A = sk + q.a;
+    const uint64_t *B = A + alen;
+    const int d = q.d0 + threadIdx.x * MERGE_ITEMS;
+    if (d >= total) continue;
+    const int dend = min(d + MERGE_ITEMS, total);
+    int lo = max(0, d - blen), hi = min(d, alen);
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (A[mid] <= B[d - mid - 1]) lo = mid + 1;
+      else hi = mid;
+    }
+    int i = lo, j = d - lo;
+    for (int o = d; o < dend; ++o) {
+      const bool ta = j >= blen || (i < alen && A[i] <= B[j]);
+      if (ta) {
+        dk[q.a + o] = A[i];
+        dv[q.a + o] = sv[q.a + i];
+        ++i;
+      } else {
+        dk[q.a + o] = B[j];
+        dv[q.a + o] = sv[q.a + alen + j];
+        ++j;
+      }
+    }
+  }
+}
+
+template <typename OffT>
+__global__ __launch_bounds__(256) void k_hub_info(const int32_t *list, int64_t count,
+                                                  const OffT *segoff, int64_t *info)
+{
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= count) return;
+  const int32_t s = list[g];
+  info[2 * g] = (int64_t)segoff[s];
+  info[2 * g + 1] = (int64_t)segoff[s + 1] - (int64_t)segoff[s];
+}
+
+// ---------------------------------------------------------------- host side
+template <typename OffT>
+static void bin_impl(cm_ctx *ctx, int64_t nseg, const OffT *segoff, SegBins &b)
+{
+  b.list = (int32_t *)ctx->pool.alloc(sizeof(int32_t) * (nseg > 0 ? nseg : 1));
+  unsigned *cnt = ctx->d_ctr + 32;     // 13 counters
+  unsigned *cursor = ctx->d_ctr + 48;  // 13 cursors
+  fill_u32(ctx, ctx->d_ctr + 32, 0u, 32);
+  const unsigned grid = nblocks(nseg, 256);
+  launch(ctx, "seg_bin_count", 0.0, k_bin_count<OffT>, dim3(grid), dim3(256), 0, nseg, segoff, cnt);
+  launch(ctx, "seg_bin_scatter", 0.0, k_bin_scatter<OffT>, dim3(grid), dim3(256), 0, nseg, segoff,
+         (const unsigned *)cnt, cursor, b.list);
+  unsigned h[SEG_NBINS];
+  CM_HIP(hipMemcpyAsync(h, cnt, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+  sync(ctx);
+  b.off[0] = 0;
+  for (int c = 0; c < SEG_NBINS; ++c) b.off[c + 1] = b.off[c] + h[c];
+}
+
+void bin_segments(cm_ctx *ctx, int64_t nseg, const void *segoff, bool off64, SegBins &b)
+{
+  if (off64) bin_impl(ctx, nseg, (const int64_t *)segoff, b);
+  else bin_impl(ctx, nseg, (const int32_t *)segoff, b);
+}
+
+void release_bins(cm_ctx *ctx, SegBins &b)
+{
+  if (b.list) ctx->pool.release(b.list);
+  b.list = nullptr;
+}
+
+static unsigned cap_grid(int64_t count)
+{
+  return (unsigned)std::min<int64_t>(count, 65535 * 8);
+}
+
+template <typename OffT>
+static void sort_impl(cm_ctx *ctx, const SegBins &b, const OffT *segoff, int64_t total,
+                      uint64_t *keys, uint32_t *vals)
+{
+  const int32_t *L = b.list;
+#define CM_WAVE_BIN(CLS, W)                                                                       \
+  {                                                                                               \
+    const int64_t c = b.off[CLS + 1] - b.off[CLS];                                                \
+    if (c) launch(ctx, "segsort_wave", 16.0 * c * W, k_sort_wave<W, OffT>,                        \
+                  dim3(nblocks(c * W, 256)), dim3(256), 0, L + b.off[CLS], c, segoff, keys, vals); \
+  }
+  CM_WAVE_BIN(0, 2) CM_WAVE_BIN(1, 4) CM_WAVE_BIN(2, 8) CM_WAVE_BIN(3, 16) CM_WAVE_BIN(4, 32)
+  CM_WAVE_BIN(5, 64)
+#undef CM_WAVE_BIN
+#define CM_LDS_BIN(CLS, S)                                                                        \
+  {                                                                                               \
+    const int64_t c = b.off[CLS + 1] - b.off[CLS];                                                \
+    if (c) launch(ctx, "segsort_lds", 0.0, k_sort_lds<S, OffT>, dim3(cap_grid(c)), dim3(256), 0,  \
+                  L + b.off[CLS], c, segoff, keys, vals);                                         \
+  }
+  CM_LDS_BIN(6, 128) CM_LDS_BIN(7, 256) CM_LDS_BIN(8, 512) CM_LDS_BIN(9, 1024)
+  CM_LDS_BIN(10, 2048) CM_LDS_BIN(11, 4096)
+#undef CM_LDS_BIN
+
+  const int64_t nhub = b.off[13] - b.off[12];
+  if (!nhub) return;
+  // hub rows: fetch (start, len), build tile and merge descriptors on the host
+  int64_t *d_info = (int64_t *)ctx->pool.alloc(sizeof(int64_t) * 2 * nhub);
+  launch(ctx, "hub_info", 0.0, k_hub_info<OffT>, dim3(nblocks(nhub, 256)), dim3(256), 0,
+         L + b.off[12], nhub, segoff, d_info);
+  std::vector<int64_t> info(2 * nhub);
+  CM_HIP(hipMemcpyAsync(info.data(), d_info, sizeof(int64_t) * 2 * nhub, hipMemcpyDeviceToHost,
+                        ctx->stream));
+  sync(ctx);
+  ctx->pool.release(d_info);
+  int64_t maxlen = 0, hubtotal = 0;
+  std::vector<MergeDesc> descs;
+  for (int64_t h = 0; h < nhub; ++h) {
+    const int64_t st = info[2 * h], len = info[2 * h + 1];
+    maxlen = std::max(maxlen, len);
+    hubtotal += len;
+    for (int64_t t = 0; t < len; t += HUB_TILE)
+      descs.push_back(MergeDesc{st, (int32_t)std::min<int64_t>(HUB_TILE, len - t), 0, (int32_t)t, 0});
+  }
+  MergeDesc *d_desc = nullptr;
+  size_t desc_cap = 0;
+  auto upload = [&](const std::vector<MergeDesc> &v) {
+    if (v.size() > desc_cap) {
+      if (d_desc) ctx->pool.release(d_desc);
+      desc_cap = v.size();
+      d_desc = (MergeDesc *)ctx->pool.alloc(sizeof(MergeDesc) * desc_cap);
+    }
+    CM_HIP(hipMemcpyAsync(d_desc, v.data(), sizeof(MergeDesc) * v.size(), hipMemcpyHostToDevice,
+                          ctx->stream));
+    sync(ctx);  // the host vector is reused
+  };
+  upload(descs);
+  launch(ctx, "segsort_hub_tile", 0.0, k_sort_tiles, dim3(cap_grid((int64_t)descs.size())),
+         dim3(256), 0, (const MergeDesc *)d_desc, (int64_t)descs.size(), keys, vals);
+  uint64_t *k2 = (uint64_t *)ctx->pool.alloc(sizeof(uint64_t) * (size_t)total);
+  uint32_t *v2 = (uint32_t *)ctx->pool.alloc(sizeof(uint32_t) * (size_t)total);
+  uint64_t *sk = keys, *dk = k2;
+  uint32_t *sv = vals, *dv = v2;
+  for (int64_t w = HUB_TILE; w < maxlen; w *= 2) {
+    descs.clear();
+    for (int64_t h = 0; h < nhub; ++h) {
+      const int64_t st = info[2 * h], len = info[2 * h + 1];
+      for (int64_t p = 0; p < len; p += 2 * w) {
+        const int64_t alen = std::min<int64_t>(w, len - p);
+        const int64_t blen = std::max<int64_t>(0, std::min<int64_t>(w, len - p - alen));
+        for (int64_t d0 = 0; d0 < alen + blen; d0 += HUB_TILE)
+          descs.push_back(MergeDesc{st + p, (int32_t)alen, (int32_t)blen, (int32_t)d0, 0});
+      }
+    }
+    upload(descs);
+    launch(ctx, "segsort_hub_merge", 24.0 * hubtotal, k_merge, dim3(cap_grid((int64_t)descs.size())),
+           dim3(256), 0, (const MergeDesc *)d_desc, (int64_t)descs.size(), (const uint64_t *)sk,
+           (const uint32_t *)sv, dk, dv);
+    std::swap(sk, dk);
+    std::swap(sv, dv);
+  }
+  if (sk != keys) {  // odd number of passes: copy the hub rows back
+    descs.clear();
+    for (int64_t h = 0; h < nhub; ++h) {
+      const int64_t st = info[2 * h], len = info[2 * h + 1];
+      for (int64_t d0 = 0; d0 < len; d0 += HUB_TILE)
+        descs.push_back(MergeDesc{st, (int32_t)len, 0, (int32_t)d0, 0});
+    }
+    upload(descs);
+    launch(ctx, "segsort_hub_copy", 24.0 * hubtotal, k_merge, dim3(cap_grid((int64_t)descs.size())),
+           dim3(256), 0, (const MergeDesc *)d_desc, (int64_t)descs.size(), (const uint64_t *)sk,
+           (const uint32_t *)sv, keys, vals);
+  }
+  sync(ctx);
+  ctx->pool.release(k2);
+  ctx->pool.release(v2);
+  if (d_desc) ctx->pool.release(d_desc);
+}
+
+void sort_binned(cm_ctx *ctx, const SegBins &b, const void *segoff, bool off64, int64_t total,
+                 uint64_t *keys, uint32_t *vals)
+{
+  if (off64) sort_impl(ctx, b, (const int64_t *)segoff, total, keys, vals);
+  else sort_impl(ctx, b, (const int32_t *)segoff, total, keys, vals);
+}
+
+void segsort_u64(cm_ctx *ctx, int64_t nseg, const void *d_segoff, bool off64, int64_t total,
+                 uint64_t *d_keys, uint32_t *d_vals)
+{
+  SegBins b;
+  bin_segments(ctx, nseg, d_segoff, off64, b);
+  sort_binned(ctx, b, d_segoff, off64, total, d_keys, d_vals);
+  release_bins(ctx, b);
+}
+
+// ------------------------------------------- bb_segsort<int,int> interface
+// Composite key: order-preserving map of the signed key in the high word,
+// position within the segment in the low word.
+__global__ __launch_bounds__(256) void k_pack_i32(int32_t n, const int32_t *segoff, int32_t nseg,
+                                                  const int32_t *k, const int32_t *v,
+                                                  uint64_t *ck, uint32_t *cv)
+{
+  const int32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  int lo = 0, hi = nseg;  // last segment start <= e
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (segoff[mid] <= e) lo = mid;
+    else hi = mid;
+  }
+  const uint32_t pos = (uint32_t)(e - segoff[lo]);
+  ck[e] = ((uint64_t)((uint32_t)k[e] ^ 0x80000000u) << 32) | pos;
+  cv[e] = (uint32_t)v[e];
+}
+
+__global__ __launch_bounds__(256) void k_unpack_i32(int32_t n, const uint64_t *ck,
+                                                    const uint32_t *cv, int32_t *k, int32_t *v)
+{
+  const int32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  k[e] = (int32_t)((uint32_t)(ck[e] >> 32) ^ 0x80000000u);
+  v[e] = (int32_t)cv[e];
+}
+
+__global__ void k_make_segoff(int32_t n, const int32_t *segs, int32_t len, int32_t *segoff)
+{
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < len) segoff[i] = segs[i];
+  if (i == len) segoff[i] = n;
+}
+
+void segsort_i32(cm_ctx *ctx, int32_t *keys, int32_t *vals, int32_t n, const int32_t *segs,
+                 int32_t length)
+{
+  if (n <= 0 || length <= 0) return;
+  int32_t *segoff = (int32_t *)ctx->pool.alloc(sizeof(int32_t) * (length + 1));
+  uint64_t *ck = (uint64_t *)ctx->pool.alloc(sizeof(uint64_t) * n);
+  uint32_t *cv = (uint32_t *)ctx->pool.alloc(sizeof(uint32_t) * n);
+  launch(ctx, "segsort_i32_prep", 0.0, k_make_segoff, dim3(nblocks(length + 1, 256)), dim3(256), 0,
+         n, segs, length, segoff);
+  launch(ctx, "segsort_i32_pack", 0.0, k_pack_i32, dim3(nblocks(n, 256)), dim3(256), 0, n,
+         (const int32_t *)segoff, length, (const int32_t *)keys, (const int32_t *)vals, ck, cv);
+  segsort_u64(ctx, length, segoff, false, n, ck, cv);
+  launch(ctx, "segsort_i32_unpack", 0.0, k_unpack_i32, dim3(nblocks(n, 256)), dim3(256), 0, n,
+         (const uint64_t *)ck, (const uint32_t *)cv, keys, vals);
+  sync(ctx);
+  ctx->pool.release(segoff);
+  ctx->pool.release(ck);
+  ctx->pool.release(cv);
+}
+
+}  // namespace cm

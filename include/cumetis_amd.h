@@ -1,0 +1,190 @@
+/*
+ * cumetis_amd.h -- C ABI of the MI355X (gfx950) multilevel coarsening engine.
+ *
+ * Plain pointers and sizes only.  Citations ("cu:") are to cuMetis.cu of the
+ * reference, which exposes one public function (include/cuMetis.h:18-19) and
+ * otherwise an internal operator boundary around its GPU coarsening.
+ *
+ * Conventions
+ *   - every function returns CM_OK (0) or a negative cm_status; none exits;
+ *   - cm_ctx_last_error() gives the message of the last failure on a context;
+ *   - device pointers are stream-ordered on the context's stream;
+ *   - the caller owns its inputs; the library owns a hierarchy until
+ *     cm_hier_release();
+ *   - contexts are independent; one context must not be used by two host
+ *     threads at once.
+ */
+#ifndef CUMETIS_AMD_H
+#define CUMETIS_AMD_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum cm_status {
+  CM_OK = 0,
+  CM_ERR_INVALID = -1,     /* bad argument */
+  CM_ERR_HIP = -2,         /* HIP runtime error */
+  CM_ERR_NOMEM = -3,       /* device allocation failed */
+  CM_ERR_UNSUPPORTED = -4, /* size or combination not supported */
+  CM_ERR_INPUT = -5        /* malformed graph input */
+};
+
+enum cm_offset_type { CM_OFF32 = 0, CM_OFF64 = 1 };
+
+/* Row order of coarse adjacency lists.
+ *   CM_ROW_FIRST_SEEN: byte-identical to the reference CPU contraction
+ *     (cu:737-825): distinct coarse neighbours in first-seen order over
+ *     row(v) then row(u), v = smaller endpoint, self entry replaced by the
+ *     last one.
+ *   CM_ROW_ASCENDING: ascending coarse neighbour ids, the order the
+ *     reference GPU contraction produces (cu:1332 then cu:1196-1271). */
+enum cm_row_order { CM_ROW_FIRST_SEEN = 0, CM_ROW_ASCENDING = 1 };
+
+typedef struct cm_ctx cm_ctx;
+typedef struct cm_hier cm_hier;
+
+/* A CSR graph.  xadj has n+1 entries of the offset type; adjwgt/vwgt may be
+ * NULL for unit weights (cu:479-503).  on_device selects device or host
+ * pointers.  The graph must be symmetric, without self loops, with positive
+ * edge weights. */
+typedef struct {
+  int64_t n;
+  int64_t m;
+  int32_t offset_type;
+  int32_t on_device;
+  const void *xadj;
+  const int32_t *adjncy;
+  const int32_t *adjwgt;
+  const int32_t *vwgt;
+} cm_graph_view;
+
+/* Coarsening parameters.  threshold 0 = max(n/(20*floor(log2 k)), 30k)
+ * (cu:3634); maxvwgt 0 = (int)(1.5*tvwgt/threshold) (cu:1828). */
+typedef struct {
+  int32_t nparts;
+  int32_t threshold;
+  int32_t maxvwgt;
+  int32_t row_order;  /* cm_row_order */
+  int32_t max_levels; /* 0 = no limit */
+  int32_t max_rounds; /* handshake round cap, 0 = default (4096) */
+  uint64_t seed;
+  int32_t flags;      /* CM_FLAG_* */
+  int32_t reserved;
+} cm_coarsen_params;
+
+/* cm_coarsen_params.flags */
+#define CM_FLAG_TRUSTED_INPUT 1 /* skip the device-side validation of the input graph */
+
+/* One level of a hierarchy (device pointers owned by the hierarchy). */
+typedef struct {
+  int64_t n;
+  int64_t m;
+  int32_t offset_type;
+  int32_t rounds; /* matching rounds spent on this level (0 on the coarsest) */
+  void *xadj;
+  int32_t *adjncy;
+  int32_t *adjwgt;
+  int32_t *vwgt;
+  int32_t *match; /* NULL on the coarsest level */
+  int32_t *cmap;  /* NULL on the coarsest level */
+  int64_t tvwgt;
+} cm_level_info;
+
+/* ---------------------------------------------------------------- context */
+/* stream: a hipStream_t, or NULL to create a private non-blocking stream. */
+int cm_ctx_create(int device, void *stream, cm_ctx **out);
+int cm_ctx_destroy(cm_ctx *ctx);
+const char *cm_ctx_last_error(const cm_ctx *ctx);
+int cm_ctx_sync(cm_ctx *ctx);
+/* Per-kernel hipEvent timing on the context stream (off by default). */
+int cm_ctx_set_timing(cm_ctx *ctx, int on);
+int cm_ctx_reset_stats(cm_ctx *ctx);
+/* Stats of kernel i (0-based) in name order: returns CM_ERR_INVALID past the
+ * end.  name receives up to name_cap bytes. */
+int cm_ctx_kernel_stat(cm_ctx *ctx, int i, char *name, int name_cap, int64_t *count,
+                       double *total_ms, double *algo_bytes);
+int cm_ctx_pool_bytes(cm_ctx *ctx, int64_t *reserved, int64_t *in_use);
+
+/* Device memory helpers for callers without an allocator of their own. */
+int cm_dmalloc(cm_ctx *ctx, size_t bytes, void **out);
+int cm_dfree(cm_ctx *ctx, void *p);
+int cm_memcpy_h2d(cm_ctx *ctx, void *dst, const void *src, size_t bytes);
+int cm_memcpy_d2h(cm_ctx *ctx, void *dst, const void *src, size_t bytes);
+
+/* ------------------------------------------------------------- hierarchy */
+/* Multilevel coarsening: replaces cuMetis_coarsen (cu:1824-1850).  Levels are
+ * built until the reference stop rule fails (cu:1843-1845): continue while
+ * nc > threshold && nc < 0.75 n && mc > nc/2. */
+int cm_coarsen(cm_ctx *ctx, const cm_graph_view *g, const cm_coarsen_params *p, cm_hier **out);
+
+/* Batched coarsening: nb graphs concatenated (vertex ids local to each
+ * graph); graph b owns vertices [voff[b], voff[b+1]) and entries
+ * [eoff[b], eoff[b+1]) of the concatenated arrays.  Each graph gets its own
+ * hierarchy in out[b], bit-identical to coarsening it alone. */
+int cm_coarsen_batch(cm_ctx *ctx, int32_t nb, const cm_graph_view *concat, const int64_t *voff,
+                     const int64_t *eoff, const cm_coarsen_params *params, cm_hier **out);
+
+int cm_hier_num_levels(const cm_hier *h);
+int cm_hier_level(const cm_hier *h, int level, cm_level_info *out);
+/* Copy one level to host arrays (any of them may be NULL to skip).  xadj
+ * uses the hierarchy's offset type. */
+int cm_hier_copy_level(cm_ctx *ctx, const cm_hier *h, int level, void *xadj, int32_t *adjncy,
+                       int32_t *adjwgt, int32_t *vwgt, int32_t *match, int32_t *cmap);
+/* Projection where_f[i] = where_c[cmap[i]] from level+1 to level:
+ * replaces cuMetis_kway_project / projectback (cu:3067-3087). */
+int cm_project(cm_ctx *ctx, const cm_hier *h, int level, const int32_t *d_where_c,
+               int32_t *d_where_f);
+int cm_hier_release(cm_ctx *ctx, cm_hier *h);
+
+/* -------------------------------------------------------- unit operators */
+/* Seeded handshake heavy-edge matching of one graph (device view):
+ * replaces cuMetis_gpu_match's matching steps (cu:1699-1747; kernels
+ * cu:1411-1419, 1563-1630).  d_match[i] = partner or i.  rounds may be NULL. */
+int cm_match(cm_ctx *ctx, const cm_graph_view *g, int32_t maxvwgt, uint64_t seed,
+             int32_t max_rounds, int32_t *d_match, int32_t *rounds);
+/* Coarse numbering: replaces findc2..findc4 + scan (cu:1634-1694,
+ * cu:1750-1789).  d_match must be complete (partner or self). */
+int cm_number(cm_ctx *ctx, int64_t n, const void *d_xadj, int32_t offset_type,
+              const int32_t *d_match, int32_t *d_cmap, int64_t *nc);
+/* Contraction given match and cmap: replaces cuMetis_gpu_create_cgraph
+ * (cu:1291-1406).  d_cxadj: nc+1 offsets; d_cadjncy/d_cadjwgt: room for m
+ * entries; d_cvwgt: nc.  *mc receives the coarse entry count. */
+int cm_contract(cm_ctx *ctx, const cm_graph_view *g, const int32_t *d_match,
+                const int32_t *d_cmap, int64_t nc, int32_t row_order, void *d_cxadj,
+                int32_t *d_cadjncy, int32_t *d_cadjwgt, int32_t *d_cvwgt, int64_t *mc);
+/* Segmented sort, the bb_segsort<int,int> role (include/bb_segsort.h:35-36):
+ * segments start at d_segs[0..length), the last ends at n; keys ascending,
+ * values follow their keys.  Unlike bb_segsort the order of equal keys is
+ * fixed (by original position). */
+int cm_segsort_i32(cm_ctx *ctx, int32_t *d_keys, int32_t *d_vals, int32_t n,
+                   const int32_t *d_segs, int32_t length);
+/* Projection on raw arrays (cu:3067-3075). */
+int cm_project_raw(cm_ctx *ctx, int64_t n, const int32_t *d_cmap, const int32_t *d_where_c,
+                   int32_t *d_where_f);
+
+/* ------------------------------------------------------ METIS-style API */
+/* Drop-in for cuMetis_PartGraph (include/cuMetis.h:18-19, cu:3622-3670):
+ * host pointers, writes part[n]; vwgt/adjwgt may be NULL; tpwgts/ubvec are
+ * ignored as in the reference (uniform targets, ub 1.03).  Unlike the
+ * reference, *objval receives the edge cut, and errors return a negative
+ * cm_status instead of exiting.  Returns 1 on success like the reference. */
+int cuMetis_PartGraph(int *nvtxs, int *xadj, int *adjncy, int *vwgt, int *adjwgt, int *nparts,
+                      float *tpwgts, float *ubvec, int *objval, int *part);
+
+/* Same pipeline with explicit options and per-phase timings (ms). */
+typedef struct {
+  double total_ms, coarsen_ms, init_ms, uncoarsen_ms;
+  int32_t levels;
+  int32_t coarsest_n;
+  int64_t edgecut;
+} cm_part_report;
+int cm_partition(cm_ctx *ctx, const cm_graph_view *g, const cm_coarsen_params *p, int32_t *part,
+                 cm_part_report *report);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

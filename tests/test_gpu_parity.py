@@ -1,0 +1,179 @@
+"""GPU parity: every operator of the HIP engine against the CPU oracle, bit
+for bit, through the C ABI."""
+import numpy as np
+import pytest
+
+from cases import named_cases, with_weights
+from conftest import say
+from cumetis_amd import graphs
+from cumetis_amd._lib import CM_ROW_ASCENDING, CM_ROW_FIRST_SEEN
+from cumetis_amd.engine import Graph
+
+pytestmark = pytest.mark.gpu
+
+CASES = named_cases()
+
+
+def sort_rows(xadj, adjncy, adjwgt):
+    a = adjncy.copy()
+    w = adjwgt.copy()
+    for i in range(len(xadj) - 1):
+        s, e = int(xadj[i]), int(xadj[i + 1])
+        o = np.argsort(a[s:e], kind="stable")
+        a[s:e] = a[s:e][o]
+        w[s:e] = w[s:e][o]
+    return a, w
+
+
+def maxv_for(g, k=8, orc=None):
+    T = orc.threshold(g.n, k)
+    return orc.maxvwgt(int(g.n if g.vwgt is None else g.vwgt.sum()), T)
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_number(engine, orc, name):
+    g = with_weights(CASES[name])
+    say(f"[number] case={name} n={g.n} m={g.m}")
+    orc.srand(1)
+    match, _, _, _ = orc.cpu_match(g.xadj, g.adjncy, g.adjwgt, g.vwgt, maxv_for(g, orc=orc))
+    cmap, nc = engine.number(g.xadj, match)
+    ecmap, enc = orc.number(match)
+    assert nc == enc
+    np.testing.assert_array_equal(cmap, ecmap)
+
+
+@pytest.mark.parametrize("order", [CM_ROW_FIRST_SEEN, CM_ROW_ASCENDING])
+@pytest.mark.parametrize("matcher", ["cpu", "handshake"])
+@pytest.mark.parametrize("name", list(CASES))
+def test_contract(engine, orc, name, matcher, order):
+    g = with_weights(CASES[name])
+    say(f"[contract] case={name} matcher={matcher} order={order} n={g.n} m={g.m}")
+    mv = maxv_for(g, orc=orc)
+    if matcher == "cpu":
+        orc.srand(1)
+        match, cmap, nc, _ = orc.cpu_match(g.xadj, g.adjncy, g.adjwgt, g.vwgt, mv)
+    else:
+        match, _ = orc.handshake_match(g.xadj, g.adjncy, g.adjwgt, g.vwgt, mv, seed=11)
+        cmap, nc = orc.number(match)
+    cx, ca, cw, cv = engine.contract(g, match, cmap, nc, order)
+    ex, ea, ew, ev = orc.cpu_contract(g.xadj, g.adjncy, g.adjwgt, g.vwgt, match, cmap, nc)
+    np.testing.assert_array_equal(cx.astype(np.int64), ex)
+    np.testing.assert_array_equal(cv, ev)
+    if order == CM_ROW_ASCENDING:
+        ea, ew = sort_rows(ex, ea, ew)
+    np.testing.assert_array_equal(ca, ea)
+    np.testing.assert_array_equal(cw, ew)
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_match_bit_exact(engine, orc, name):
+    g = with_weights(CASES[name])
+    say(f"[match] case={name} n={g.n} m={g.m}")
+    mv = maxv_for(g, orc=orc)
+    for seed in (1, 12345):
+        m_gpu, r_gpu = engine.match(g, mv, seed=seed)
+        m_orc, r_orc = orc.handshake_match(g.xadj, g.adjncy, g.adjwgt, g.vwgt, mv, seed=seed)
+        np.testing.assert_array_equal(m_gpu, m_orc)
+        assert r_gpu == r_orc
+
+
+def test_match_repeatable(engine, orc):
+    g = with_weights(CASES["rgg3000w16"])
+    say("[match repeat] rgg3000w16")
+    mv = maxv_for(g, orc=orc)
+    a, _ = engine.match(g, mv, seed=99)
+    b, _ = engine.match(g, mv, seed=99)
+    c, _ = engine.match(g, mv, seed=100)
+    np.testing.assert_array_equal(a, b)
+    assert not np.array_equal(a, c)
+
+
+def _segsort_case(rng, sizes):
+    segs = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int32)
+    n = int(np.sum(sizes))
+    keys = rng.integers(-1000, 1000, size=n).astype(np.int32)
+    vals = np.arange(n, dtype=np.int32)
+    return keys, vals, segs
+
+
+@pytest.mark.parametrize("sizes", [
+    [0, 1, 2, 3, 4, 5, 7, 8, 9, 16, 17, 31, 32, 33, 63, 64],
+    [65, 100, 128, 129, 255, 256, 257, 511, 512, 513, 1000, 1024, 1025, 2047, 2048, 2049, 4095, 4096],
+    [4097, 5000, 8192, 8193, 20000, 70001],
+    list(np.random.default_rng(3).integers(0, 300, size=2000)),
+])
+def test_segsort_i32(engine, sizes):
+    rng = np.random.default_rng(len(sizes))
+    keys, vals, segs = _segsort_case(rng, sizes)
+    say(f"[segsort] nseg={len(segs)} n={len(keys)} max={max(sizes)}")
+    k, v = engine.segsort(keys, vals, segs)
+    bounds = list(segs) + [len(keys)]
+    for s in range(len(segs)):
+        a, b = bounds[s], bounds[s + 1]
+        o = np.argsort(keys[a:b], kind="stable")
+        np.testing.assert_array_equal(k[a:b], keys[a:b][o])
+        np.testing.assert_array_equal(v[a:b], vals[a:b][o])
+
+
+def test_project(engine, orc):
+    rng = np.random.default_rng(7)
+    nc, n = 5000, 12000
+    cmap = rng.integers(0, nc, size=n).astype(np.int32)
+    wc = rng.integers(0, 64, size=nc).astype(np.int32)
+    say("[project] n=12000")
+    np.testing.assert_array_equal(engine.project(cmap, wc), orc.project(cmap, wc))
+
+
+@pytest.mark.parametrize("order", [CM_ROW_FIRST_SEEN, CM_ROW_ASCENDING])
+@pytest.mark.parametrize("name,k", [("grid100w100", 8), ("delaunay4720", 8), ("rmat12", 16), ("hub", 4)])
+def test_hierarchy(engine, orc, name, k, order):
+    if name == "grid100w100":
+        g = graphs.grid2d(100, 100, wmax=100, seed=9)
+    elif name == "rmat12":
+        g = graphs.rmat(12, 16, seed=4, offsets=np.int32)
+    else:
+        g = CASES[name]
+    g = with_weights(g)
+    say(f"[hierarchy] case={name} k={k} order={order}")
+    h = engine.coarsen(g, k, seed=5, row_order=order)
+    ref = orc.coarsen(g.xadj, g.adjncy, g.adjwgt, g.vwgt, k, matcher="handshake", seed=5)
+    assert h.num_levels == len(ref)
+    for i, R in enumerate(ref):
+        L = h.level(i)
+        assert L.n == R["n"]
+        np.testing.assert_array_equal(L.xadj.astype(np.int64), R["xadj"])
+        np.testing.assert_array_equal(L.vwgt, R["vwgt"])
+        ea, ew = R["adjncy"], R["adjwgt"]
+        if order == CM_ROW_ASCENDING and i > 0:
+            ea, ew = sort_rows(R["xadj"], ea, ew)
+        np.testing.assert_array_equal(L.adjncy, ea)
+        np.testing.assert_array_equal(L.adjwgt, ew)
+        if i < len(ref) - 1:
+            np.testing.assert_array_equal(L.match, R["match"])
+            np.testing.assert_array_equal(L.cmap, R["cmap"])
+    h.release()
+
+
+def test_hierarchy_int64_offsets(engine, orc):
+    g = with_weights(graphs.grid2d(60, 50, wmax=20, seed=1))
+    g64 = Graph(g.xadj.astype(np.int64), g.adjncy, g.adjwgt, g.vwgt)
+    say("[hierarchy int64] grid60x50")
+    a = engine.coarsen(g, 8, seed=3)
+    b = engine.coarsen(g64, 8, seed=3)
+    assert a.num_levels == b.num_levels
+    for i in range(a.num_levels):
+        la, lb = a.level(i), b.level(i)
+        assert lb.xadj.dtype == np.int64
+        np.testing.assert_array_equal(la.xadj.astype(np.int64), lb.xadj)
+        np.testing.assert_array_equal(la.adjncy, lb.adjncy)
+        np.testing.assert_array_equal(la.adjwgt, lb.adjwgt)
+
+
+def test_hierarchy_device_input(engine, orc):
+    g = with_weights(graphs.grid2d(40, 40, wmax=9, seed=2))
+    say("[hierarchy device view] grid40x40")
+    a = engine.coarsen(g, 8, seed=3)
+    b = engine.coarsen(g, 8, seed=3, on_device=True)
+    assert a.num_levels == b.num_levels
+    for i in range(a.num_levels):
+        np.testing.assert_array_equal(a.level(i).adjncy, b.level(i).adjncy)
