@@ -145,6 +145,93 @@ static int64_t device_sum(cm_ctx *ctx, int64_t n, const int32_t *a)
   return ctx->h_small[8];
 }
 
+// ------------------------------------------------------ input validation
+// bad[0] |= 1 offsets, 2 neighbour id out of range, 4 self loop, 8 weight <= 0
+template <typename OffT>
+__global__ __launch_bounds__(256) void k_check_graph(int64_t n, int64_t m, const OffT *xadj,
+                                                     const int32_t *adjncy, const int32_t *adjwgt,
+                                                     unsigned *bad)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > n) return;
+  unsigned f = 0;
+  if (i == 0 && xadj[0] != 0) f |= 1;
+  if (i == n) {
+    if ((int64_t)xadj[n] != m) f |= 1;
+  } else {
+    const int64_t b = (int64_t)xadj[i], e = (int64_t)xadj[i + 1];
+    if (b > e || b < 0 || e > m) {
+      f |= 1;
+    } else {
+      for (int64_t j = b; j < e; ++j) {
+        const int32_t u = adjncy[j];
+        if (u < 0 || u >= n) f |= 2;
+        else if (u == i) f |= 4;
+        if (adjwgt && adjwgt[j] <= 0) f |= 8;
+      }
+    }
+  }
+  if (f) atomicOr(bad, f);
+}
+
+// partner in range and match[match[i]] == i
+__global__ __launch_bounds__(256) void k_check_match(int64_t n, const int32_t *match, unsigned *bad)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int32_t u = match[i];
+  if (u < 0 || u >= n || match[u] != i) atomicOr(bad, 16u);
+}
+
+__global__ __launch_bounds__(256) void k_check_equal(int64_t n, const int32_t *a, const int32_t *b,
+                                                     unsigned *bad)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && a[i] != b[i]) atomicOr(bad, 32u);
+}
+
+static unsigned read_bad(cm_ctx *ctx)
+{
+  unsigned h = 0;
+  CM_HIP(hipMemcpyAsync(&h, ctx->d_ctr + 3, sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
+  sync(ctx);
+  return h;
+}
+
+static void check_graph(cm_ctx *ctx, int64_t n, int64_t m, bool off64, const void *xadj,
+                        const int32_t *adjncy, const int32_t *adjwgt)
+{
+  fill_u32(ctx, ctx->d_ctr + 3, 0u, 1);
+  if (off64)
+    launch(ctx, "validate_graph", 0.0, k_check_graph<int64_t>, dim3(nblocks(n + 1, 256)), dim3(256), 0,
+           n, m, (const int64_t *)xadj, adjncy, adjwgt, ctx->d_ctr + 3);
+  else
+    launch(ctx, "validate_graph", 0.0, k_check_graph<int32_t>, dim3(nblocks(n + 1, 256)), dim3(256), 0,
+           n, m, (const int32_t *)xadj, adjncy, adjwgt, ctx->d_ctr + 3);
+  const unsigned f = read_bad(ctx);
+  CM_REQUIRE(!(f & 1), CM_ERR_INPUT, "xadj is not a valid offset array for n, m");
+  CM_REQUIRE(!(f & 2), CM_ERR_INPUT, "adjncy holds a vertex id outside [0, n)");
+  CM_REQUIRE(!(f & 4), CM_ERR_INPUT, "graph has a self loop");
+  CM_REQUIRE(!(f & 8), CM_ERR_INPUT, "edge weights must be positive");
+}
+
+static void check_match(cm_ctx *ctx, int64_t n, const int32_t *match)
+{
+  fill_u32(ctx, ctx->d_ctr + 3, 0u, 1);
+  launch(ctx, "validate_match", 0.0, k_check_match, dim3(nblocks(n, 256)), dim3(256), 0, n, match,
+         ctx->d_ctr + 3);
+  CM_REQUIRE(read_bad(ctx) == 0, CM_ERR_INVALID,
+             "match must be a complete involution (partner or self for every vertex)");
+}
+
+static void check_equal(cm_ctx *ctx, int64_t n, const int32_t *a, const int32_t *b, const char *msg)
+{
+  fill_u32(ctx, ctx->d_ctr + 3, 0u, 1);
+  launch(ctx, "validate_equal", 0.0, k_check_equal, dim3(nblocks(n, 256)), dim3(256), 0, n, a, b,
+         ctx->d_ctr + 3);
+  CM_REQUIRE(read_bad(ctx) == 0, CM_ERR_INVALID, msg);
+}
+
 // ----------------------------------------------------------- graph inputs
 static int offset_bytes(int t) { return t == CM_OFF64 ? 8 : 4; }
 
@@ -164,7 +251,7 @@ static void validate_view(const cm_graph_view *g)
 
 // Level 0 of a hierarchy from a view: device views are referenced (unit
 // weights are materialised), host views are copied into pool memory.
-static Level level0(cm_ctx *ctx, const cm_graph_view *g, cm_hier *h)
+static Level level0(cm_ctx *ctx, const cm_graph_view *g, cm_hier *h, bool validate = true)
 {
   Level L;
   L.n = g->n;
@@ -197,6 +284,7 @@ static Level level0(cm_ctx *ctx, const cm_graph_view *g, cm_hier *h)
       if (L.n) CM_HIP(hipMemcpyAsync(L.vwgt, g->vwgt, 4 * L.n, hipMemcpyHostToDevice, ctx->stream));
     }
   }
+  if (validate) check_graph(ctx, L.n, L.m, L.off64, L.xadj, L.adjncy, L.adjwgt);
   if (!L.adjwgt) {
     L.adjwgt = (int32_t *)own(4 * (L.m > 0 ? L.m : 1));
     fill_u32(ctx, L.adjwgt, 1u, L.m);
@@ -463,7 +551,7 @@ int cm_coarsen(cm_ctx *ctx, const cm_graph_view *g, const cm_coarsen_params *p, 
     CM_REQUIRE(p->nparts >= 1, CM_ERR_INVALID, "nparts must be >= 1");
     CM_REQUIRE(p->row_order == CM_ROW_FIRST_SEEN || p->row_order == CM_ROW_ASCENDING, CM_ERR_INVALID,
                "row_order must be CM_ROW_FIRST_SEEN or CM_ROW_ASCENDING");
-    h->levels.push_back(cm::level0(ctx, g, h));
+    h->levels.push_back(cm::level0(ctx, g, h, !(p->flags & CM_FLAG_TRUSTED_INPUT)));
     const int T = p->threshold > 0 ? p->threshold : cm::ref_threshold(g->n, p->nparts);
     const int32_t maxvwgt =
         p->maxvwgt > 0 ? p->maxvwgt : (int32_t)(1.5 * (double)h->levels[0].tvwgt / T);
@@ -579,6 +667,7 @@ int cm_number(cm_ctx *ctx, int64_t n, const void *d_xadj, int32_t offset_type, c
   L.off64 = offset_type == CM_OFF64;
   L.xadj = const_cast<void *>(d_xadj);
   const size_t ob = L.off64 ? 8 : 4;
+  cm::check_match(ctx, n, d_match);
   int32_t *vw = (int32_t *)ctx->pool.alloc(4 * (n > 0 ? n : 1));
   cm::fill_u32(ctx, vw, 1u, n);
   L.vwgt = vw;
@@ -614,8 +703,10 @@ int cm_contract(cm_ctx *ctx, const cm_graph_view *g, const int32_t *d_match, con
   try {
     // numbering recomputed to obtain the segment offsets; it must agree
     // with the caller's cmap, which is what the gather uses
+    cm::check_match(ctx, n, d_match);
     const int64_t nc2 = cm::number(ctx, L, d_match, cm2, rep, seg, d_cvwgt);
     CM_REQUIRE(nc2 == nc, CM_ERR_INVALID, "nc does not match the matching");
+    cm::check_equal(ctx, n, cm2, d_cmap, "cmap must be the canonical numbering of match");
     out = cm::contract(ctx, L, d_match, d_cmap, nc, rep, seg, row_order, d_cxadj, d_cadjncy, d_cadjwgt);
   } catch (...) {
     ctx->pool.release(cm2);

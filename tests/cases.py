@@ -8,7 +8,7 @@ from cumetis_amd import graphs
 from cumetis_amd.engine import Graph
 
 
-def hub_graph(seed: int = 5, side: int = 64, hub_deg: int = 6000, mids=(90, 700, 3000)) -> Graph:
+def hub_graph(seed: int = 5, side: int = 90, hub_deg: int = 6000, mids=(90, 700, 3000)) -> Graph:
     """Grid plus one hub row above 4096 entries and medium rows of the given
     degrees, so that every sort bin of the contraction is exercised."""
     rng = np.random.default_rng(seed)

@@ -1,0 +1,333 @@
+"""Pins the CPU oracle (no GPU): glibc known answers, closed forms, brute
+force on tiny graphs, and an independent algorithm for the matching."""
+import itertools
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from cases import named_cases, with_weights
+from cumetis_amd import graphs
+
+CASES = named_cases()
+
+# glibc rand() with the default seed (1): first outputs of the published
+# TYPE_3 additive feedback generator.
+GLIBC_SEED1 = [1804289383, 846930886, 1681692777, 1714636915, 1957747793, 424238335,
+               719885386, 1649760492, 596516649, 1189641421]
+
+
+def test_rand_known_answers(orc):
+    orc.srand(1)
+    assert [orc.rand() for _ in range(10)] == GLIBC_SEED1
+
+
+def test_randarrayofp_small_known_answer(orc):
+    # n < 10: n swaps p[rand%n] <-> p[rand%n] (cu:649-656), worked by hand
+    # from GLIBC_SEED1: (3,1) (2,0) (3,0) (1,2) (4,1)
+    orc.srand(1)
+    assert list(orc.randarrayofp(5, 0)) == [1, 4, 3, 2, 0]
+
+
+def test_randarrayofp_is_permutation(orc):
+    orc.srand(1)
+    p = orc.randarrayofp(1000, 125)
+    assert sorted(p) == list(range(1000))
+    assert not np.array_equal(p, np.arange(1000))
+
+
+def test_thresholds_known_answers(orc):
+    # cu:3634 and cu:1828 for the benchmark configurations
+    assert orc.threshold(4720, 8) == 240 and orc.maxvwgt(4720, 240) == 29
+    assert orc.threshold(525625, 64) == 4380 and orc.maxvwgt(525625, 4380) == 180
+    assert orc.threshold(4194304, 256) == 26214 and orc.maxvwgt(4194304, 26214) == 240
+    assert orc.threshold(4096000, 128) == 29257 and orc.maxvwgt(4096000, 29257) == 210
+
+
+def test_stop_rule(orc):
+    # cu:1843-1845: continue while nc > T && nc < 0.75 nf && mc > nc/2
+    assert orc.coarsen_continue(1000, 2000, 3000, 240)
+    assert not orc.coarsen_continue(240, 2000, 3000, 240)
+    assert not orc.coarsen_continue(1500, 2000, 3000, 240)   # 1500 == 0.75*2000
+    assert not orc.coarsen_continue(1000, 2000, 500, 240)    # 500 == 1000/2
+    assert orc.coarsen_continue(1001, 2000, 501, 240)         # 501 > 1001/2 = 500
+
+
+def test_cpu_match_known_answer(orc):
+    # path 0-1-2-3, weights 5,1,7; tperm (n<10 branch) = [3,1,0,2], aved = 0
+    # so perm = tperm: 3 takes 2 (w 7), 1 takes 0 (w 5)
+    g = graphs.from_edges(4, np.array([0, 1, 2]), np.array([1, 2, 3]), np.array([5, 1, 7]))
+    orc.srand(1)
+    match, cmap, nc, perm = orc.cpu_match(g.xadj, g.adjncy, g.adjwgt, None, 100)
+    assert list(perm) == [3, 1, 0, 2]
+    assert list(match) == [1, 0, 3, 2]
+    assert list(cmap) == [0, 0, 1, 1] and nc == 2
+
+
+def _check_matching(g, match, maxv, vw):
+    n = g.n
+    assert np.array_equal(match[match], np.arange(n)), "not an involution"
+    for i in range(n):
+        u = match[i]
+        if u != i:
+            row = g.adjncy[g.xadj[i]:g.xadj[i + 1]]
+            assert u in row, "partner is not a neighbour"
+            assert vw[i] + vw[u] <= maxv
+    # maximal: no eligible edge joins two unmatched vertices
+    single = match == np.arange(n)
+    src = np.repeat(np.arange(n), np.diff(g.xadj))
+    bad = single[src] & single[g.adjncy] & (vw[src] + vw[g.adjncy] <= maxv)
+    assert not bad.any(), "matching is not maximal"
+
+
+@pytest.mark.parametrize("name", [k for k in CASES if k != "isolated10"])
+def test_cpu_match_invariants(orc, name):
+    g = with_weights(CASES[name])
+    T = orc.threshold(g.n, 8)
+    maxv = orc.maxvwgt(int(g.vwgt.sum()), T)
+    orc.srand(1)
+    match, cmap, nc, perm = orc.cpu_match(g.xadj, g.adjncy, g.adjwgt, g.vwgt, maxv)
+    _check_matching(g, match, maxv, g.vwgt)
+    assert sorted(perm) == list(range(g.n))
+    # visit order is bucketed by min(degree, aved) (cu:854-860)
+    deg = np.diff(g.xadj)
+    aved = int(0.7 * (int(g.xadj[-1]) // g.n))
+    assert np.all(np.diff(np.minimum(deg, aved)[perm]) >= 0)
+
+
+def _number_closed_form(match):
+    n = len(match)
+    rep = np.arange(n) <= match
+    rank = np.cumsum(rep) - 1
+    return rank[np.minimum(np.arange(n), match)], int(rep.sum())
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_number_closed_form(orc, seed):
+    g = with_weights(CASES["rgg3000w16"])
+    match, _ = orc.handshake_match(g.xadj, g.adjncy, g.adjwgt, g.vwgt, 10, seed=seed)
+    cmap, nc = orc.number(match)
+    ecmap, enc = _number_closed_form(match)
+    assert nc == enc and np.array_equal(cmap, ecmap)
+
+
+def _ptap(g, cmap, nc):
+    n = g.n
+    A = sp.csr_matrix((g.adjwgt.astype(np.int64), g.adjncy, g.xadj.astype(np.int64)), shape=(n, n))
+    P = sp.csr_matrix((np.ones(n, np.int64), (np.arange(n), cmap)), shape=(n, nc))
+    Cm = (P.T @ A @ P).tolil()
+    Cm.setdiag(0)
+    Cm = Cm.tocsr()
+    Cm.eliminate_zeros()
+    Cm.sort_indices()
+    return Cm
+
+
+def _first_seen_rows(g, match, cmap, nc):
+    """Independent statement of the row order of cu:762-819: distinct keys by
+    first index in row(v)+row(u); a matched pair's self key is replaced by
+    the last key."""
+    rows = []
+    for v in range(g.n):
+        u = match[v]
+        if u < v:
+            continue
+        keys = cmap[g.adjncy[g.xadj[v]:g.xadj[v + 1]]]
+        if u != v:
+            keys = np.concatenate([keys, cmap[g.adjncy[g.xadj[u]:g.xadj[u + 1]]]])
+        uk, first = np.unique(keys, return_index=True)
+        order = list(uk[np.argsort(first)])
+        c = cmap[v]
+        if u != v and c in order:
+            s = order.index(c)
+            last = order.pop()
+            if s < len(order):
+                order[s] = last
+        rows.append(order)
+    return rows
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_contract_closed_form(orc, name):
+    g = with_weights(CASES[name])
+    T = orc.threshold(g.n, 8)
+    maxv = orc.maxvwgt(int(g.vwgt.sum()), T)
+    orc.srand(1)
+    match, cmap, nc, _ = orc.cpu_match(g.xadj, g.adjncy, g.adjwgt, g.vwgt, maxv)
+    cx, ca, cw, cv = orc.cpu_contract(g.xadj, g.adjncy, g.adjwgt, g.vwgt, match, cmap, nc)
+    Cm = _ptap(g, cmap, nc)
+    B = sp.csr_matrix((cw.astype(np.int64), ca.copy(), cx.copy()), shape=(nc, nc))
+    B.sort_indices()
+    assert np.array_equal(B.indptr, Cm.indptr)
+    assert np.array_equal(B.indices, Cm.indices)
+    assert np.array_equal(B.data, Cm.data)
+    assert np.array_equal(cv, np.bincount(cmap, weights=g.vwgt, minlength=nc).astype(np.int32))
+    rows = _first_seen_rows(g, match, cmap, nc)
+    assert [list(ca[cx[c]:cx[c + 1]]) for c in range(nc)] == rows
+
+
+def _all_matchings(n, edges):
+    """Every matching (as a match array) of a tiny graph."""
+    out = []
+
+    def rec(i, match):
+        if i == n:
+            out.append(match.copy())
+            return
+        if match[i] != -1:
+            rec(i + 1, match)
+            return
+        match[i] = i
+        rec(i + 1, match)
+        match[i] = -1
+        for (a, b) in edges:
+            j = b if a == i else a if b == i else None
+            if j is not None and j > i and match[j] == -1:
+                match[i], match[j] = j, i
+                rec(i + 1, match)
+                match[i] = match[j] = -1
+
+    rec(0, np.full(n, -1, np.int32))
+    return out
+
+
+def test_contract_brute_force_tiny(orc):
+    rng = np.random.default_rng(0)
+    for trial in range(6):
+        n = 7
+        pairs = [(a, b) for a in range(n) for b in range(a + 1, n) if rng.random() < 0.45]
+        if not pairs:
+            continue
+        u, v = np.array(pairs).T
+        g = with_weights(graphs.from_edges(n, u, v, rng.integers(1, 9, size=len(pairs))))
+        g.vwgt[:] = rng.integers(1, 4, size=n)
+        for match in _all_matchings(n, pairs):
+            cmap, nc = orc.number(match)
+            cx, ca, cw, cv = orc.cpu_contract(g.xadj, g.adjncy, g.adjwgt, g.vwgt, match, cmap, nc)
+            Cm = _ptap(g, cmap, nc)
+            B = sp.csr_matrix((cw.astype(np.int64), ca.copy(), cx.copy()), shape=(nc, nc))
+            B.sort_indices()
+            assert np.array_equal(B.toarray(), Cm.toarray())
+            assert [list(ca[cx[c]:cx[c + 1]]) for c in range(nc)] == \
+                _first_seen_rows(g, match, cmap, nc)
+
+
+def _greedy_matching(orc, g, maxv, seed):
+    """Global greedy: edges by descending (w, hash, min, max), take an edge
+    when both ends are free.  With a strict total order this is the matching
+    that locally-dominant (handshake) matching converges to."""
+    n = g.n
+    src = np.repeat(np.arange(n), np.diff(g.xadj))
+    dst = g.adjncy
+    keep = (src < dst) & (g.vwgt[src] + g.vwgt[dst] <= maxv)
+    a, b, w = src[keep], dst[keep], g.adjwgt[keep]
+    h = np.array([orc.edge_hash(seed, int(x), int(y)) for x, y in zip(a, b)], np.uint64)
+    order = np.lexsort((b, a, h, w))[::-1]
+    match = np.arange(n)
+    free = np.ones(n, bool)
+    for e in order:
+        x, y = a[e], b[e]
+        if free[x] and free[y]:
+            match[x], match[y] = y, x
+            free[x] = free[y] = False
+    return match
+
+
+@pytest.mark.parametrize("name", ["grid32w100", "delaunay4720", "rgg3000w16", "rmat10", "hub", "path2"])
+@pytest.mark.parametrize("seed", [1, 77])
+def test_handshake_equals_greedy(orc, name, seed):
+    g = with_weights(CASES[name])
+    T = orc.threshold(g.n, 8)
+    maxv = orc.maxvwgt(int(g.vwgt.sum()), T)
+    match, rounds = orc.handshake_match(g.xadj, g.adjncy, g.adjwgt, g.vwgt, maxv, seed=seed)
+    _check_matching(g, match, maxv, g.vwgt)
+    assert np.array_equal(match, _greedy_matching(orc, g, maxv, seed))
+    assert rounds >= 1
+
+
+def test_handshake_brute_force_tiny(orc):
+    """On tiny graphs the handshake result is maximal and its weight is at
+    least half the maximum weight matching (1/2-approximation)."""
+    rng = np.random.default_rng(1)
+    for trial in range(8):
+        n = 7
+        pairs = [(a, b) for a in range(n) for b in range(a + 1, n) if rng.random() < 0.5]
+        if not pairs:
+            continue
+        u, v = np.array(pairs).T
+        w = rng.integers(1, 20, size=len(pairs))
+        g = with_weights(graphs.from_edges(n, u, v, w))
+        match, _ = orc.handshake_match(g.xadj, g.adjncy, g.adjwgt, g.vwgt, 100, seed=trial)
+        _check_matching(g, match, 100, g.vwgt)
+        wt = dict(((min(a, b), max(a, b)), int(x)) for a, b, x in zip(u, v, w))
+        weight = lambda m: sum(wt[(i, m[i])] for i in range(n) if m[i] > i)
+        best = max(weight(m if True else m) for m in
+                   [np.where(mm == -1, np.arange(n), mm) for mm in _all_matchings(n, pairs)])
+        assert 2 * weight(match) >= best
+
+
+def test_edge_hash_symmetric_inputs(orc):
+    assert orc.edge_hash(1, 3, 9) == orc.edge_hash(1, 3, 9)
+    assert orc.edge_hash(1, 3, 9) != orc.edge_hash(2, 3, 9)
+
+
+def test_project_and_cut_closed_forms(orc):
+    rng = np.random.default_rng(2)
+    g = with_weights(CASES["grid32w100"])
+    cmap = rng.integers(0, 100, size=g.n).astype(np.int32)
+    wc = rng.integers(0, 4, size=100).astype(np.int32)
+    wf = orc.project(cmap, wc)
+    assert np.array_equal(wf, wc[cmap])
+    src = np.repeat(np.arange(g.n), np.diff(g.xadj))
+    cut = int(g.adjwgt[wf[src] != wf[g.adjncy]].sum()) // 2
+    assert orc.edgecut(g.xadj, g.adjncy, g.adjwgt, wf) == cut
+    assert np.array_equal(orc.pwgts(g.vwgt, wf, 4), np.bincount(wf, weights=g.vwgt, minlength=4))
+
+
+def test_segsort_oracle(orc):
+    rng = np.random.default_rng(4)
+    sizes = rng.integers(0, 50, size=100)
+    so = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    k = rng.integers(0, 30, size=so[-1]).astype(np.uint64)
+    v = np.arange(so[-1], dtype=np.uint32)
+    sk, sv = orc.segsort_u64(so, k, v)
+    for s in range(100):
+        a, b = so[s], so[s + 1]
+        o = np.lexsort((v[a:b], k[a:b]))
+        assert np.array_equal(sk[a:b], k[a:b][o]) and np.array_equal(sv[a:b], v[a:b][o])
+
+
+@pytest.mark.parametrize("k", [2, 3, 8])
+def test_initpart_invariants(orc, k):
+    g = with_weights(graphs.grid2d(30, 20, wmax=10, seed=1))
+    orc.srand(1)
+    w1, obj1 = orc.initpart(g.xadj, g.adjncy, g.adjwgt, g.vwgt, k)
+    orc.srand(1)
+    w2, obj2 = orc.initpart(g.xadj, g.adjncy, g.adjwgt, g.vwgt, k)
+    assert np.array_equal(w1, w2) and obj1 == obj2
+    assert w1.min() == 0 and w1.max() == k - 1
+    pw = np.bincount(w1, minlength=k)
+    assert pw.min() > 0.7 * g.n / k and pw.max() < 1.3 * g.n / k
+    if k == 2:  # the bisection objective is the cut of the 2-way split
+        assert obj1 == orc.edgecut(g.xadj, g.adjncy, g.adjwgt, w1)
+
+
+def test_kway_refine_invariants(orc):
+    g = with_weights(graphs.grid2d(40, 40, wmax=10, seed=2))
+    k = 4
+    orc.srand(1)
+    where, _ = orc.initpart(g.xadj, g.adjncy, g.adjwgt, g.vwgt, k)
+    tv = int(g.vwgt.sum())
+    tp = np.float32(1.0 / k)
+    maxw = int(np.float32(np.float32(tp * np.float32(tv)) * np.float32(1.03)))
+    pw0 = np.bincount(where, weights=g.vwgt, minlength=k)
+    out = orc.kway_refine(g.xadj, g.adjncy, g.adjwgt, g.vwgt, k, where)
+    pw1 = np.bincount(out, weights=g.vwgt, minlength=k)
+    moved = out != where
+    # only candidates (external > internal weight) move
+    src = np.repeat(np.arange(g.n), np.diff(g.xadj))
+    ext = np.bincount(src, weights=g.adjwgt * (where[src] != where[g.adjncy]), minlength=g.n)
+    inn = np.bincount(src, weights=g.adjwgt * (where[src] == where[g.adjncy]), minlength=g.n)
+    assert np.all(ext[moved] > inn[moved])
+    # parts that started within the bound stay within it
+    assert np.all((pw1 <= maxw) | (pw0 > maxw))
