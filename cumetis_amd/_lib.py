@@ -83,6 +83,13 @@ _PROTOS = {
                                     C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "cm_partition": (C.c_int, [_P, C.POINTER(GraphView), C.POINTER(CoarsenParams), _P,
                                C.POINTER(PartReport)]),
+    "cm_read_metis": (C.c_int, [C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                C.POINTER(_P), C.POINTER(_P), C.POINTER(_P), C.POINTER(_P),
+                                C.c_char_p, C.c_int]),
+    "cm_free_host": (None, [_P]),
+    "cm_write_partition": (C.c_int, [C.c_char_p, _P, C.c_int64]),
+    "cm_initial_partition": (C.c_int, [C.c_int64, _P, _P, _P, _P, C.c_int32, _P,
+                                       C.POINTER(C.c_int64)]),
 }
 
 _lib = None

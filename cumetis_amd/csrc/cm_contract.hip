@@ -171,6 +171,22 @@ static int64_t scan_excl(cm_ctx *ctx, int64_t n, const int32_t *in, OffT *out)
   return ctx->h_small[1];
 }
 
+// exclusive scan of n int32 values into int32 offsets (out[n] = total);
+// used by the segment binning
+void scan_excl_i32(cm_ctx *ctx, int64_t n, const int32_t *in, int32_t *out)
+{
+  if (n == 0) {
+    fill_u32(ctx, out, 0u, 1);
+    return;
+  }
+  const int64_t tiles = (n + 1023) / 1024;
+  ensure_status(ctx, tiles);
+  fill_u64(ctx, ctx->d_status, 0ull, tiles);
+  fill_u32(ctx, ctx->d_ctr, 0u, 1);
+  launch(ctx, "scan_bins", 0.0, k_scan_excl<int32_t>, dim3((unsigned)tiles), dim3(1024), 0, n, in, out,
+         ctx->d_status, ctx->d_ctr, ctx->d_small + 2);
+}
+
 // ------------------------------------------------------------------ gather
 // Fine vertex i writes its row into the segment of its coarse vertex: the
 // representative's row first, then the partner's.  Key = (coarse neighbour,

@@ -10,11 +10,6 @@
 #include "cm_internal.hpp"
 #include "cm_segsort.hpp"
 
-struct cm_hier {
-  std::vector<cm::Level> levels;
-  std::vector<void *> owned;  // pool blocks released with the hierarchy
-};
-
 namespace cm {
 
 // ------------------------------------------------------------------- pool

@@ -74,6 +74,11 @@ struct Level {
 
 }  // namespace cm
 
+struct cm_hier {
+  std::vector<cm::Level> levels;
+  std::vector<void *> owned;  // pool blocks released with the hierarchy
+};
+
 struct cm_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -131,6 +136,7 @@ inline unsigned nblocks(int64_t work, int per_block)
 void sync(cm_ctx *ctx);
 void fill_u32(cm_ctx *ctx, void *p, uint32_t v, int64_t count);
 void fill_u64(cm_ctx *ctx, void *p, uint64_t v, int64_t count);
+void scan_excl_i32(cm_ctx *ctx, int64_t n, const int32_t *in, int32_t *out);
 
 // ---- subsystem entry points (device pointers, ctx stream) ----
 // (a) matching; returns rounds.  match[] ends complete (partner or self).

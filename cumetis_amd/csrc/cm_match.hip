@@ -22,7 +22,7 @@ namespace cm {
 constexpr int LONG_DEG = 64;  // rows longer than this are scanned by a whole wave
 constexpr int ROUND_BATCH = 8;  // rounds queued between host checks
 
-// cnt[2r] = vertices that matched or died in round r, cnt[2r+1] = still active.
+// cnt[2r] = 1 if a vertex matched or died in round r, cnt[2r+1] = 1 if any is still active.
 __device__ __forceinline__ bool round_live(const unsigned *cnt, int r)
 {
   if (r == 0) return true;
@@ -151,6 +151,12 @@ __global__ __launch_bounds__(256) void k_accept(int64_t n, int32_t *match, const
                                                 unsigned *cnt, int r)
 {
   if (!round_live(cnt, r)) return;
+  __shared__ int s_changed, s_active;
+  if (threadIdx.x == 0) {
+    s_changed = 0;
+    s_active = 0;
+  }
+  __syncthreads();
   const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   bool changed = false, active = false;
   if (v < n && match[v] == -1) {
@@ -165,10 +171,17 @@ __global__ __launch_bounds__(256) void k_accept(int64_t n, int32_t *match, const
       active = true;
     }
   }
-  const uint64_t cm_ = __ballot(changed), am = __ballot(active);
-  if (lane() == 0) {
-    if (cm_) atomicAdd(&cnt[2 * r], (unsigned)__popcll(cm_));
-    if (am) atomicAdd(&cnt[2 * r + 1], (unsigned)__popcll(am));
+  // round status words are flags (0 / 1): one plain store per block, and
+  // only while the flag is still clear, so thousands of blocks do not
+  // serialise on one address
+  if (__ballot(changed) && lane() == 0) s_changed = 1;
+  if (__ballot(active) && lane() == 0) s_active = 1;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (s_changed && __hip_atomic_load(&cnt[2 * r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
+      __hip_atomic_store(&cnt[2 * r], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (s_active && __hip_atomic_load(&cnt[2 * r + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
+      __hip_atomic_store(&cnt[2 * r + 1], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 

@@ -1,0 +1,159 @@
+// cm_partition.hip -- the multilevel k-way pipeline around the GPU coarsening
+// (cuMetis_kway_partition, cu:3544-3616, and cuMetis_PartGraph,
+// cu:3622-3670):
+//   GPU coarsening -> coarsest level to the host -> recursive-bisection
+//   initial partition (host) -> refinement of the coarsest level -> for each
+//   finer level: GPU projection, then one refinement pass.
+#include <stdlib.h>
+#include <string.h>
+
+#include <chrono>
+#include <vector>
+
+#include "cm_internal.hpp"
+#include "cm_refine.hpp"
+#include "host/cm_host.hpp"
+
+namespace {
+
+double now_ms()
+{
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
+cmh::HostGraph level_to_host(cm_ctx *ctx, const cm::Level &L)
+{
+  cmh::HostGraph g;
+  g.n = (int)L.n;
+  g.xadj.resize(L.n + 1);
+  g.adjncy.resize(L.m);
+  g.adjwgt.resize(L.m);
+  g.vwgt.resize(L.n);
+  if (L.off64) {
+    CM_HIP(hipMemcpyAsync(g.xadj.data(), L.xadj, 8 * (L.n + 1), hipMemcpyDeviceToHost, ctx->stream));
+  } else {
+    std::vector<int32_t> x(L.n + 1);
+    CM_HIP(hipMemcpyAsync(x.data(), L.xadj, 4 * (L.n + 1), hipMemcpyDeviceToHost, ctx->stream));
+    cm::sync(ctx);
+    for (int64_t i = 0; i <= L.n; ++i) g.xadj[i] = x[i];
+  }
+  if (L.m) {
+    CM_HIP(hipMemcpyAsync(g.adjncy.data(), L.adjncy, 4 * L.m, hipMemcpyDeviceToHost, ctx->stream));
+    CM_HIP(hipMemcpyAsync(g.adjwgt.data(), L.adjwgt, 4 * L.m, hipMemcpyDeviceToHost, ctx->stream));
+  }
+  if (L.n) CM_HIP(hipMemcpyAsync(g.vwgt.data(), L.vwgt, 4 * L.n, hipMemcpyDeviceToHost, ctx->stream));
+  cm::sync(ctx);
+  return g;
+}
+
+int64_t host_cut(const cm_graph_view *g, const int32_t *part)
+{
+  int64_t cut = 0;
+  for (int64_t i = 0; i < g->n; ++i) {
+    const int64_t b = g->offset_type == CM_OFF64 ? ((const int64_t *)g->xadj)[i] : ((const int32_t *)g->xadj)[i];
+    const int64_t e = g->offset_type == CM_OFF64 ? ((const int64_t *)g->xadj)[i + 1]
+                                                 : ((const int32_t *)g->xadj)[i + 1];
+    for (int64_t j = b; j < e; ++j)
+      if (part[i] != part[g->adjncy[j]]) cut += g->adjwgt ? g->adjwgt[j] : 1;
+  }
+  return cut / 2;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cm_partition(cm_ctx *ctx, const cm_graph_view *g, const cm_coarsen_params *p, int32_t *part,
+                 cm_part_report *rep)
+{
+  if (!ctx || !g || !p || !part) return CM_ERR_INVALID;
+  cm_hier *h = nullptr;
+  int32_t *w0 = nullptr, *w1 = nullptr;
+  try {
+    CM_REQUIRE(!g->on_device, CM_ERR_INVALID, "cm_partition takes a host graph");
+    CM_REQUIRE(p->nparts >= 1, CM_ERR_INVALID, "nparts must be >= 1");
+    const double t0 = now_ms();
+    if (p->nparts == 1 || g->n == 0) {
+      for (int64_t i = 0; i < g->n; ++i) part[i] = 0;
+      if (rep) *rep = cm_part_report{0, 0, 0, 0, 1, (int32_t)g->n, 0};
+      return CM_OK;
+    }
+    int rc = cm_coarsen(ctx, g, p, &h);
+    if (rc != CM_OK) return rc;
+    const double t1 = now_ms();
+    const int L = (int)h->levels.size();
+    const cm::Level &C = h->levels[L - 1];
+    cmh::HostGraph hc = level_to_host(ctx, C);
+    std::vector<int> where;
+    srand(1);  // the reference draws from an unseeded glibc stream (seed 1)
+    cmh::initial_partition(hc, p->nparts, where);
+    const double t2 = now_ms();
+    const int64_t tvwgt = h->levels[0].tvwgt;
+    std::vector<int64_t> pw(p->nparts, 0);
+    for (int i = 0; i < hc.n; ++i) pw[where[i]] += hc.vwgt[i];
+    w0 = (int32_t *)ctx->pool.alloc(4 * (C.n > 0 ? C.n : 1));
+    if (C.n) CM_HIP(hipMemcpyAsync(w0, where.data(), 4 * C.n, hipMemcpyHostToDevice, ctx->stream));
+    cm::refine_level(ctx, C, p->nparts, w0, pw, tvwgt);
+    for (int lev = L - 2; lev >= 0; --lev) {
+      const cm::Level &F = h->levels[lev];
+      w1 = (int32_t *)ctx->pool.alloc(4 * (F.n > 0 ? F.n : 1));
+      cm::project(ctx, F.n, F.cmap, w0, w1);
+      ctx->pool.release(w0);
+      w0 = w1;
+      w1 = nullptr;
+      cm::refine_level(ctx, F, p->nparts, w0, pw, tvwgt);
+    }
+    if (g->n) CM_HIP(hipMemcpyAsync(part, w0, 4 * g->n, hipMemcpyDeviceToHost, ctx->stream));
+    cm::sync(ctx);
+    const double t3 = now_ms();
+    if (rep) {
+      rep->total_ms = t3 - t0;
+      rep->coarsen_ms = t1 - t0;
+      rep->init_ms = t2 - t1;
+      rep->uncoarsen_ms = t3 - t2;
+      rep->levels = L;
+      rep->coarsest_n = (int32_t)C.n;
+      rep->edgecut = host_cut(g, part);
+    }
+    ctx->pool.release(w0);
+    cm_hier_release(ctx, h);
+  } catch (const cm::Error &e) {
+    ctx->err = e.what();
+    if (w0) ctx->pool.release(w0);
+    if (w1) ctx->pool.release(w1);
+    if (h) cm_hier_release(ctx, h);
+    return e.code;
+  } catch (const std::exception &e) {
+    ctx->err = e.what();
+    if (h) cm_hier_release(ctx, h);
+    return CM_ERR_INVALID;
+  }
+  return CM_OK;
+}
+
+// cu:3622-3670.  One process-wide context on device 0.
+int cuMetis_PartGraph(int *nvtxs, int *xadj, int *adjncy, int *vwgt, int *adjwgt, int *nparts,
+                      float *tpwgts, float *ubvec, int *objval, int *part)
+{
+  (void)tpwgts;
+  (void)ubvec;
+  static cm_ctx *ctx = nullptr;
+  if (!nvtxs || !xadj || !nparts || !part) return CM_ERR_INVALID;
+  if (!ctx) {
+    int rc = cm_ctx_create(0, nullptr, &ctx);
+    if (rc != CM_OK) return rc;
+  }
+  cm_graph_view v{*nvtxs, (int64_t)xadj[*nvtxs], CM_OFF32, 0, xadj, adjncy, adjwgt, vwgt};
+  cm_coarsen_params p{};
+  p.nparts = *nparts;
+  p.row_order = CM_ROW_FIRST_SEEN;
+  p.seed = 1;
+  cm_part_report rep{};
+  int rc = cm_partition(ctx, &v, &p, part, &rep);
+  if (rc != CM_OK) return rc;
+  if (objval) *objval = (int)rep.edgecut;
+  return 1;
+}
+
+}  // extern "C"

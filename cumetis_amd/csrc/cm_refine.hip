@@ -1,0 +1,196 @@
+// cm_refine.hip -- one k-way refinement pass on the GPU with a deterministic
+// move schedule (the reference's pass is cu:3244-3515 and races: its moves
+// check balance and then update part weights with separate atomics,
+// cu:3439-3443).
+//   1. candidates: vertices whose external edge weight exceeds the internal
+//      weight (cu:3244-3267);
+//   2. destination: per candidate, the connectivity to every part is summed
+//      in an LDS row owned by one wave; the destination is the first part
+//      with the strictly largest connectivity, or the own part when none
+//      beats it (cu:3272-3309);
+//   3. candidates are sorted by (destination, vertex) with the segmented
+//      sort (the reference sorts by destination, cu:3387);
+//   4. the host walks the sorted list once and accepts moves that keep both
+//      parts inside [int(w/1.03), int(w*1.03)] of their target
+//      (cu:3401-3488), destinations ascending; accepted moves are applied on
+//      the device.
+#include <vector>
+
+#include "cm_common.hpp"
+#include "cm_internal.hpp"
+#include "cm_refine.hpp"
+#include "host/cm_host.hpp"
+
+namespace cm {
+
+constexpr int REFINE_MAX_PARTS = 2048;
+
+template <typename OffT>
+__global__ __launch_bounds__(256) void k_refine_candidates(int64_t n, const OffT *__restrict__ xadj,
+                                                           const int32_t *__restrict__ adjncy,
+                                                           const int32_t *__restrict__ adjwgt,
+                                                           const int32_t *__restrict__ where,
+                                                           int32_t *list, unsigned *count)
+{
+  __shared__ unsigned s_cnt, s_base;
+  __shared__ unsigned s_w[4];
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  bool cand = false;
+  if (v < n) {
+    const int me = where[v];
+    int64_t in = 0, ex = 0;
+    for (OffT j = xadj[v]; j < xadj[v + 1]; ++j) {
+      if (where[adjncy[j]] == me) in += adjwgt[j];
+      else ex += adjwgt[j];
+    }
+    cand = ex > in;
+  }
+  const uint64_t mk = __ballot(cand);
+  const int w = threadIdx.x >> 6;
+  if (lane() == 0) s_w[w] = (unsigned)__popcll(mk);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    s_cnt = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+    s_base = s_cnt ? atomicAdd(count, s_cnt) : 0u;
+  }
+  __syncthreads();
+  if (cand) {
+    unsigned before = 0;
+    for (int i = 0; i < w; ++i) before += s_w[i];
+    list[s_base + before + mask_rank(mk)] = (int32_t)v;
+  }
+}
+
+// One wave per candidate: LDS connectivity row of nparts entries.
+template <typename OffT>
+__global__ __launch_bounds__(256) void k_refine_dest(const int32_t *list, const unsigned *count,
+                                                     const OffT *__restrict__ xadj,
+                                                     const int32_t *__restrict__ adjncy,
+                                                     const int32_t *__restrict__ adjwgt,
+                                                     const int32_t *__restrict__ where,
+                                                     const int32_t *__restrict__ vwgt, int nparts,
+                                                     uint64_t *key, uint32_t *idx, int32_t *from,
+                                                     int32_t *vw)
+{
+  extern __shared__ int32_t conn_all[];
+  const int w = threadIdx.x >> 6, l = lane();
+  int32_t *conn = conn_all + w * nparts;
+  const unsigned nc = *count;
+  const int64_t nw = (int64_t)gridDim.x * 4;
+  for (int p = l; p < nparts; p += CM_WAVE) conn[p] = 0;
+  for (int64_t q = (int64_t)blockIdx.x * 4 + w; q < nc; q += nw) {
+    const int32_t v = list[q];
+    const int me = where[v];
+    for (OffT j = xadj[v] + l; j < xadj[v + 1]; j += CM_WAVE) atomicAdd(&conn[where[adjncy[j]]], adjwgt[j]);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    int32_t best = -1, bp = nparts;
+    for (int p = l; p < nparts; p += CM_WAVE) {
+      const int32_t c = conn[p];
+      if (c > best) {
+        best = c;
+        bp = p;
+      }
+    }
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) {
+      const int32_t ob = __shfl_xor(best, d, CM_WAVE);
+      const int32_t op = __shfl_xor(bp, d, CM_WAVE);
+      if (ob > best || (ob == best && op < bp)) {
+        best = ob;
+        bp = op;
+      }
+    }
+    const int32_t own = conn[me];
+    const int to = best > own ? bp : me;
+    __builtin_amdgcn_wave_barrier();
+    for (OffT j = xadj[v] + l; j < xadj[v + 1]; j += CM_WAVE) conn[where[adjncy[j]]] = 0;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (l == 0) {
+      key[q] = ((uint64_t)(uint32_t)to << 32) | (uint32_t)v;
+      idx[q] = (uint32_t)q;
+      from[q] = me;
+      vw[q] = vwgt[v];
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_apply_moves(int64_t nm, const int32_t *moves, int32_t *where)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nm) where[moves[2 * i]] = moves[2 * i + 1];
+}
+
+template <typename OffT>
+static int refine_impl(cm_ctx *ctx, const Level &L, int nparts, int32_t *where, std::vector<int64_t> &pw,
+                       int64_t tvwgt)
+{
+  const int64_t n = L.n;
+  if (n == 0 || nparts <= 1) return 0;
+  CM_REQUIRE(nparts <= REFINE_MAX_PARTS, CM_ERR_UNSUPPORTED, "k-way refinement supports up to 2048 parts");
+  const OffT *xadj = (const OffT *)L.xadj;
+  int32_t *list = (int32_t *)ctx->pool.alloc(sizeof(int32_t) * n);
+  unsigned *count = ctx->d_ctr + 4;
+  fill_u32(ctx, count, 0u, 1);
+  launch(ctx, "refine_candidates", 0.0, k_refine_candidates<OffT>, dim3(nblocks(n, 256)), dim3(256), 0, n,
+         xadj, L.adjncy, L.adjwgt, (const int32_t *)where, list, count);
+  unsigned hc = 0;
+  CM_HIP(hipMemcpyAsync(&hc, count, sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
+  sync(ctx);
+  int accepted = 0;
+  if (hc > 0) {
+    uint64_t *key = (uint64_t *)ctx->pool.alloc(sizeof(uint64_t) * hc);
+    uint32_t *idx = (uint32_t *)ctx->pool.alloc(sizeof(uint32_t) * hc);
+    int32_t *from = (int32_t *)ctx->pool.alloc(sizeof(int32_t) * hc);
+    int32_t *vw = (int32_t *)ctx->pool.alloc(sizeof(int32_t) * hc);
+    const unsigned grid = (unsigned)std::min<int64_t>((hc + 3) / 4, 16384);
+    launch(ctx, "refine_dest", 0.0, k_refine_dest<OffT>, dim3(grid), dim3(256), sizeof(int32_t) * 4 * nparts,
+           (const int32_t *)list, (const unsigned *)count, xadj, L.adjncy, L.adjwgt, (const int32_t *)where,
+           (const int32_t *)L.vwgt, nparts, key, idx, from, vw);
+    int64_t *segoff = (int64_t *)ctx->pool.alloc(sizeof(int64_t) * 2);
+    int64_t hseg[2] = {0, (int64_t)hc};
+    CM_HIP(hipMemcpyAsync(segoff, hseg, sizeof(hseg), hipMemcpyHostToDevice, ctx->stream));
+    segsort_u64(ctx, 1, segoff, true, hc, key, idx);
+    std::vector<uint64_t> hkey(hc);
+    std::vector<uint32_t> hidx(hc);
+    std::vector<int> hfrom(hc), hvw(hc), sfrom(hc), svw(hc);
+    CM_HIP(hipMemcpyAsync(hkey.data(), key, 8 * hc, hipMemcpyDeviceToHost, ctx->stream));
+    CM_HIP(hipMemcpyAsync(hidx.data(), idx, 4 * hc, hipMemcpyDeviceToHost, ctx->stream));
+    CM_HIP(hipMemcpyAsync(hfrom.data(), from, 4 * hc, hipMemcpyDeviceToHost, ctx->stream));
+    CM_HIP(hipMemcpyAsync(hvw.data(), vw, 4 * hc, hipMemcpyDeviceToHost, ctx->stream));
+    sync(ctx);
+    for (unsigned q = 0; q < hc; ++q) {
+      sfrom[q] = hfrom[hidx[q]];
+      svw[q] = hvw[hidx[q]];
+    }
+    std::vector<int> moves;
+    accepted = cmh::schedule_moves(nparts, tvwgt, hkey, sfrom, svw, pw, moves);
+    if (accepted > 0) {
+      int32_t *dm = (int32_t *)ctx->pool.alloc(sizeof(int32_t) * moves.size());
+      CM_HIP(hipMemcpyAsync(dm, moves.data(), 4 * moves.size(), hipMemcpyHostToDevice, ctx->stream));
+      launch(ctx, "refine_apply", 0.0, k_apply_moves, dim3(nblocks(accepted, 256)), dim3(256), 0,
+             (int64_t)accepted, (const int32_t *)dm, where);
+      sync(ctx);
+      ctx->pool.release(dm);
+    }
+    ctx->pool.release(key);
+    ctx->pool.release(idx);
+    ctx->pool.release(from);
+    ctx->pool.release(vw);
+    ctx->pool.release(segoff);
+  }
+  ctx->pool.release(list);
+  return accepted;
+}
+
+int refine_level(cm_ctx *ctx, const Level &L, int nparts, int32_t *d_where, std::vector<int64_t> &pw,
+                 int64_t tvwgt)
+{
+  return L.off64 ? refine_impl<int64_t>(ctx, L, nparts, d_where, pw, tvwgt)
+                 : refine_impl<int32_t>(ctx, L, nparts, d_where, pw, tvwgt);
+}
+
+}  // namespace cm

@@ -29,36 +29,50 @@ __device__ __forceinline__ int seg_class(int64_t len)
   return lg - 1;
 }
 
+// Binning without contended atomics: every workgroup of 256 segments writes
+// its per-class counts to hist[c * nblocks + block]; one exclusive scan over
+// that class-major array gives each (class, block) its first slot, so the
+// lists are grouped by class with ascending segment ids inside a class.
 template <typename OffT>
-__global__ __launch_bounds__(256) void k_bin_count(int64_t nseg, const OffT *segoff, unsigned *cnt)
+__global__ __launch_bounds__(256) void k_bin_hist(int64_t nseg, const OffT *segoff, unsigned nb,
+                                                  int32_t *hist)
 {
+  __shared__ unsigned sh[SEG_NBINS];
+  if (threadIdx.x < SEG_NBINS) sh[threadIdx.x] = 0;
+  __syncthreads();
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int cls = s < nseg ? seg_class((int64_t)(segoff[s + 1] - segoff[s])) : -1;
   for (int c = 0; c < SEG_NBINS; ++c) {
     const uint64_t mk = __ballot(cls == c);
-    if (mk && lane() == __ffsll((unsigned long long)mk) - 1) atomicAdd(&cnt[c], (unsigned)__popcll(mk));
+    if (mk && lane() == 0) atomicAdd(&sh[c], (unsigned)__popcll(mk));
   }
+  __syncthreads();
+  if (threadIdx.x < SEG_NBINS) hist[(int64_t)threadIdx.x * nb + blockIdx.x] = (int32_t)sh[threadIdx.x];
 }
 
 template <typename OffT>
-__global__ __launch_bounds__(256) void k_bin_scatter(int64_t nseg, const OffT *segoff,
-                                                     const unsigned *cnt, unsigned *cursor,
-                                                     int32_t *list)
+__global__ __launch_bounds__(256) void k_bin_scatter(int64_t nseg, const OffT *segoff, unsigned nb,
+                                                     const int32_t *base, int32_t *list,
+                                                     int64_t *class_off)
 {
+  __shared__ unsigned wcnt[SEG_NBINS][4];
+  const int w = threadIdx.x >> 6;
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int cls = s < nseg ? seg_class((int64_t)(segoff[s + 1] - segoff[s])) : -1;
-  unsigned base = 0;
+  uint64_t mine = 0;
   for (int c = 0; c < SEG_NBINS; ++c) {
     const uint64_t mk = __ballot(cls == c);
-    if (mk) {
-      const int leader = __ffsll((unsigned long long)mk) - 1;
-      unsigned pos = 0;
-      if (lane() == leader) pos = atomicAdd(&cursor[c], (unsigned)__popcll(mk));
-      pos = __shfl(pos, leader, CM_WAVE);
-      if (cls == c) list[base + pos + mask_rank(mk)] = (int32_t)s;
-    }
-    base += cnt[c];
+    if (lane() == 0) wcnt[c][w] = (unsigned)__popcll(mk);
+    if (cls == c) mine = mk;
   }
+  __syncthreads();
+  if (cls >= 0) {
+    unsigned before = 0;
+    for (int i = 0; i < w; ++i) before += wcnt[cls][i];
+    list[base[(int64_t)cls * nb + blockIdx.x] + before + mask_rank(mine)] = (int32_t)s;
+  }
+  if (blockIdx.x == 0 && threadIdx.x <= SEG_NBINS)
+    class_off[threadIdx.x] = base[(int64_t)threadIdx.x * nb];
 }
 
 // ------------------------------------------------ bins 0..5: in registers
@@ -227,18 +241,21 @@ template <typename OffT>
 static void bin_impl(cm_ctx *ctx, int64_t nseg, const OffT *segoff, SegBins &b)
 {
   b.list = (int32_t *)ctx->pool.alloc(sizeof(int32_t) * (nseg > 0 ? nseg : 1));
-  unsigned *cnt = ctx->d_ctr + 32;     // 13 counters
-  unsigned *cursor = ctx->d_ctr + 48;  // 13 cursors
-  fill_u32(ctx, ctx->d_ctr + 32, 0u, 32);
-  const unsigned grid = nblocks(nseg, 256);
-  launch(ctx, "seg_bin_count", 0.0, k_bin_count<OffT>, dim3(grid), dim3(256), 0, nseg, segoff, cnt);
-  launch(ctx, "seg_bin_scatter", 0.0, k_bin_scatter<OffT>, dim3(grid), dim3(256), 0, nseg, segoff,
-         (const unsigned *)cnt, cursor, b.list);
-  unsigned h[SEG_NBINS];
-  CM_HIP(hipMemcpyAsync(h, cnt, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+  const unsigned nb = nblocks(nseg, 256);
+  const int64_t nh = (int64_t)SEG_NBINS * nb;
+  int32_t *hist = (int32_t *)ctx->pool.alloc(sizeof(int32_t) * (nh + 1));
+  int32_t *base = (int32_t *)ctx->pool.alloc(sizeof(int32_t) * (nh + 1));
+  launch(ctx, "seg_bin_hist", 0.0, k_bin_hist<OffT>, dim3(nb), dim3(256), 0, nseg, segoff, nb, hist);
+  scan_excl_i32(ctx, nh, hist, base);  // base[nh] = number of listed segments
+  int64_t *d_off = ctx->d_small + 16;
+  launch(ctx, "seg_bin_scatter", 0.0, k_bin_scatter<OffT>, dim3(nb), dim3(256), 0, nseg, segoff, nb,
+         (const int32_t *)base, b.list, d_off);
+  CM_HIP(hipMemcpyAsync(ctx->h_small + 16, d_off, sizeof(int64_t) * (SEG_NBINS + 1),
+                        hipMemcpyDeviceToHost, ctx->stream));
   sync(ctx);
-  b.off[0] = 0;
-  for (int c = 0; c < SEG_NBINS; ++c) b.off[c + 1] = b.off[c] + h[c];
+  for (int c = 0; c <= SEG_NBINS; ++c) b.off[c] = ctx->h_small[16 + c];
+  ctx->pool.release(hist);
+  ctx->pool.release(base);
 }
 
 void bin_segments(cm_ctx *ctx, int64_t nseg, const void *segoff, bool off64, SegBins &b)

@@ -1,0 +1,39 @@
+// cm_host.hpp -- host-side parts of the partitioning pipeline around the
+// GPU coarsening: METIS I/O, the recursive-bisection initial partition and
+// the k-way refinement move scheduler.
+#pragma once
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+namespace cmh {
+
+struct HostGraph {
+  int n = 0;
+  std::vector<int64_t> xadj;
+  std::vector<int> adjncy, adjwgt, vwgt;
+  int64_t m() const { return xadj.empty() ? 0 : xadj.back(); }
+};
+
+// METIS .graph reader with the checks of cu:3712-3851; throws
+// std::runtime_error with the reference's message on malformed input.
+HostGraph read_metis(const std::string &path);
+// "<graph>.part.<k>" writer, one part id per line (cu:3856-3870).
+void write_partition(const std::string &path, const int *part, int n);
+
+// Recursive multilevel bisection initial partition (cu:3091-3108 ->
+// cu:3046-3062 -> cu:2964-3025).  Uses glibc rand() in the reference's
+// order.  Returns the summed bisection cuts.
+int initial_partition(const HostGraph &g, int nparts, std::vector<int> &where);
+
+// Move scheduler of one k-way refinement pass (cu:3401-3488), serialised:
+// destinations ascending, candidates of one destination in ascending vertex
+// order, first moves with from <= to, then from > to.  cand is sorted by
+// (to, v); pw holds part weights and is updated; returns accepted count and
+// writes the accepted (v, to) pairs to moves.
+int schedule_moves(int nparts, int64_t tvwgt, const std::vector<uint64_t> &cand,
+                   const std::vector<int> &from, const std::vector<int> &vw,
+                   std::vector<int64_t> &pw, std::vector<int> &moves);
+
+}  // namespace cmh

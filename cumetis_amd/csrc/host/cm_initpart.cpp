@@ -1,0 +1,688 @@
+// cm_initpart.cpp -- the reference's host initial partition, restated in
+// C++: recursive multilevel bisection with its own CPU coarsening (HEM +
+// hash contraction, T = 200), BFS region growing, 2-way balancing and FM
+// refinement (cuMetis.cu cu:1879-3062, cu:3091-3108).  The random choices
+// draw glibc rand() in the reference's order (cu:287-300), so results follow
+// the reference stream exactly.
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <memory>
+#include <vector>
+
+#include "cm_host.hpp"
+
+namespace cmh {
+namespace {
+
+int rnd(int max) { return (int)((uint32_t)rand() % (uint32_t)max); }  // cu:287-300
+
+// cu:639-671: identity (flag) then m 4-element block swaps (n >= 10) or n
+// pair swaps (n < 10)
+void rand_perm(int n, int *p, int m, bool init)
+{
+  if (init)
+    for (int i = 0; i < n; ++i) p[i] = i;
+  if (n <= 0) return;
+  if (n < 10) {
+    for (int i = 0; i < n; ++i) {
+      const int v = rnd(n), u = rnd(n);
+      std::swap(p[v], p[u]);
+    }
+  } else {
+    for (int i = 0; i < m; ++i) {
+      const int v = rnd(n - 3), u = rnd(n - 3);
+      std::swap(p[v + 0], p[u + 2]);
+      std::swap(p[v + 1], p[u + 3]);
+      std::swap(p[v + 2], p[u + 0]);
+      std::swap(p[v + 3], p[u + 1]);
+    }
+  }
+}
+
+// Max-priority queue with float keys and a position index (cu:1976-2242).
+class MaxQueue {
+ public:
+  explicit MaxQueue(int cap) : heap_(cap > 0 ? cap : 1), pos_(cap > 0 ? cap : 1, -1) {}
+  void clear()
+  {
+    for (long i = size_ - 1; i >= 0; --i) pos_[heap_[i].val] = -1;
+    size_ = 0;
+  }
+  void push(int node, int key)
+  {
+    long i = size_++;
+    const float k = (float)key;
+    while (i > 0) {
+      const long j = (i - 1) >> 1;
+      if (!(k > heap_[j].key)) break;
+      move(i, j);
+      i = j;
+    }
+    put(i, k, node);
+  }
+  int pop()
+  {
+    if (size_ == 0) return -1;
+    --size_;
+    const int top = heap_[0].val;
+    pos_[top] = -1;
+    if (size_ > 0) sink(0, heap_[size_].key, heap_[size_].val);
+    return top;
+  }
+  void erase(int node)
+  {
+    const long i = pos_[node];
+    pos_[node] = -1;
+    if (--size_ > 0 && heap_[size_].val != node) {
+      const int last = heap_[size_].val;
+      const float nk = heap_[size_].key, ok = heap_[i].key;
+      if (nk > ok) swim(i, nk, last);
+      else sink(i, nk, last);
+    }
+  }
+  void update(int node, int key)
+  {
+    const long i = pos_[node];
+    const float nk = (float)key, ok = heap_[i].key;
+    if (nk > ok) swim(i, nk, node);
+    else sink(i, nk, node);
+  }
+
+ private:
+  struct Item {
+    float key;
+    int val;
+  };
+  std::vector<Item> heap_;
+  std::vector<long> pos_;
+  long size_ = 0;
+  void move(long to, long from)
+  {
+    heap_[to] = heap_[from];
+    pos_[heap_[to].val] = to;
+  }
+  void put(long i, float k, int node)
+  {
+    heap_[i].key = k;
+    heap_[i].val = node;
+    pos_[node] = i;
+  }
+  void swim(long i, float k, int node)
+  {
+    while (i > 0) {
+      const long j = (i - 1) >> 1;
+      if (!(k > heap_[j].key)) break;
+      move(i, j);
+      i = j;
+    }
+    put(i, k, node);
+  }
+  void sink(long i, float k, int node)
+  {
+    long j;
+    while ((j = 2 * i + 1) < size_) {
+      if (heap_[j].key > k) {
+        if (j + 1 < size_ && heap_[j + 1].key > heap_[j].key) ++j;
+      } else if (j + 1 < size_ && heap_[j + 1].key > k) {
+        ++j;
+      } else {
+        break;
+      }
+      move(i, j);
+      i = j;
+    }
+    put(i, k, node);
+  }
+};
+
+// Graph of one recursion step plus its 2-way state.
+struct Part2 {
+  int n = 0;
+  int64_t m = 0;
+  std::vector<int64_t> xadj;
+  std::vector<int> adjncy, adjwgt, vwgt, label, cmap;
+  std::vector<int> where, bndptr, bndlist, id, ed;
+  int pwgts[2] = {0, 0};
+  int tvwgt = 0, mincut = 0, nbnd = 0;
+  std::unique_ptr<Part2> coarser;
+  Part2 *finer = nullptr;
+
+  void set_tvwgt()
+  {
+    int s = 0;
+    for (int i = 0; i < n; ++i) s += vwgt[i];
+    tvwgt = s;
+  }
+  void alloc2way()
+  {
+    where.assign(n, 0);
+    bndptr.assign(n, -1);
+    bndlist.assign(n, 0);
+    id.assign(n, 0);
+    ed.assign(n, 0);
+  }
+  void bnd_insert(int i)
+  {
+    bndlist[nbnd] = i;
+    bndptr[i] = nbnd++;
+  }
+  void bnd_delete(int i)
+  {
+    bndlist[bndptr[i]] = bndlist[--nbnd];
+    bndptr[bndlist[nbnd]] = bndptr[i];
+    bndptr[i] = -1;
+  }
+};
+
+struct Settings {
+  float ub = 1.03f;      // cu:390
+  int niter = 10;        // cu:377
+  int threshold = 200;   // cu:378
+  float balance[2] = {0, 0};
+};
+
+// cu:830-903 (HEM with degree-bucketed random visit order) + cu:737-825
+int coarsen_once(Part2 &g, int maxvwgt, Part2 &c)
+{
+  const int n = g.n;
+  std::vector<int> match(n, -1), perm(n), tperm(n), d(n);
+  g.cmap.assign(n, 0);
+  rand_perm(n, tperm.data(), n / 8, true);
+  const int aved = (int)(0.7 * (double)(int)(g.xadj[n] / n));
+  for (int i = 0; i < n; ++i) d[i] = (int)std::min<int64_t>(g.xadj[i + 1] - g.xadj[i], aved);
+  {
+    std::vector<int> cnt(aved + 2, 0);
+    for (int i = 0; i < n; ++i) cnt[d[i]]++;
+    int run = 0;
+    for (int k = 0; k <= aved + 1; ++k) {
+      const int t = cnt[k];
+      cnt[k] = run;
+      run += t;
+    }
+    for (int k = 0; k < n; ++k) perm[cnt[d[tperm[k]]]++] = tperm[k];
+  }
+  for (int k = 0; k < n; ++k) {
+    const int i = perm[k];
+    if (match[i] != -1) continue;
+    int best = i, bw = -1;
+    for (int64_t j = g.xadj[i]; j < g.xadj[i + 1]; ++j) {
+      const int u = g.adjncy[j];
+      if (match[u] == -1 && bw < g.adjwgt[j] && g.vwgt[i] + g.vwgt[u] <= maxvwgt) {
+        best = u;
+        bw = g.adjwgt[j];
+      }
+      if (best == i && 3 * g.vwgt[i] < maxvwgt) best = -1;
+    }
+    if (best != -1) {
+      match[i] = best;
+      match[best] = i;
+    }
+  }
+  int nc = 0;
+  for (int i = 0; i < n; ++i) {
+    if (match[i] == -1) {
+      match[i] = i;
+      g.cmap[i] = nc++;
+    } else if (i <= match[i]) {
+      g.cmap[i] = g.cmap[match[i]] = nc++;
+    }
+  }
+  // first-seen hash contraction
+  c.n = nc;
+  c.xadj.assign(nc + 1, 0);
+  c.adjncy.resize(std::max<int64_t>(g.m, 1));
+  c.adjwgt.resize(std::max<int64_t>(g.m, 1));
+  c.vwgt.assign(nc, 0);
+  std::vector<int64_t> slot(nc, -1);
+  int64_t out = 0;
+  int cv = 0;
+  for (int v = 0; v < n; ++v) {
+    const int u = match[v];
+    if (u < v) continue;
+    const int64_t base = out;
+    auto absorb = [&](int x) {
+      for (int64_t j = g.xadj[x]; j < g.xadj[x + 1]; ++j) {
+        const int k = g.cmap[g.adjncy[j]];
+        if (slot[k] == -1) {
+          slot[k] = out;
+          c.adjncy[out] = k;
+          c.adjwgt[out++] = g.adjwgt[j];
+        } else {
+          c.adjwgt[slot[k]] += g.adjwgt[j];
+        }
+      }
+    };
+    c.vwgt[cv] = g.vwgt[v];
+    absorb(v);
+    if (u != v) {
+      c.vwgt[cv] += g.vwgt[u];
+      absorb(u);
+      const int64_t s = slot[cv];
+      if (s != -1) {  // self entry takes the last entry's place (cu:804-808)
+        --out;
+        c.adjncy[s] = c.adjncy[out];
+        c.adjwgt[s] = c.adjwgt[out];
+        slot[cv] = -1;
+      }
+    }
+    for (int64_t j = base; j < out; ++j) slot[c.adjncy[j]] = -1;
+    c.xadj[++cv] = out;
+  }
+  c.m = out;
+  c.set_tvwgt();
+  return nc;
+}
+
+// cu:1854-1875 with T = 200
+Part2 *cpu_coarsen(Part2 &g, Settings &s)
+{
+  const int maxvwgt = (int)(1.5 * g.tvwgt / s.threshold);
+  Part2 *cur = &g;
+  do {
+    cur->coarser.reset(new Part2());
+    Part2 *c = cur->coarser.get();
+    coarsen_once(*cur, maxvwgt, *c);
+    c->finer = cur;
+    cur = c;
+  } while (cur->n > s.threshold && cur->n < 0.75 * cur->finer->n && cur->m > cur->n / 2);
+  return cur;
+}
+
+void compute_2way(Part2 &g)  // cu:1895-1944
+{
+  g.pwgts[0] = g.pwgts[1] = 0;
+  std::fill(g.bndptr.begin(), g.bndptr.end(), -1);
+  g.nbnd = 0;
+  for (int i = 0; i < g.n; ++i) g.pwgts[g.where[i]] += g.vwgt[i];
+  int cut = 0;
+  for (int i = 0; i < g.n; ++i) {
+    int in = 0, ex = 0;
+    const int me = g.where[i];
+    for (int64_t j = g.xadj[i]; j < g.xadj[i + 1]; ++j)
+      (g.where[g.adjncy[j]] == me ? in : ex) += g.adjwgt[j];
+    g.id[i] = in;
+    g.ed[i] = ex;
+    if (ex > 0 || g.xadj[i] == g.xadj[i + 1]) {
+      g.bnd_insert(i);
+      cut += ex;
+    }
+  }
+  g.mincut = cut / 2;
+}
+
+// Move higain to `to` and update neighbour gains (shared by balance and FM).
+template <typename OnNeighbour>
+void flip(Part2 &g, int h, int to, OnNeighbour &&on_nb)
+{
+  g.where[h] = to;
+  std::swap(g.id[h], g.ed[h]);
+  if (g.ed[h] == 0 && g.xadj[h] < g.xadj[h + 1]) g.bnd_delete(h);
+  for (int64_t j = g.xadj[h]; j < g.xadj[h + 1]; ++j) {
+    const int k = g.adjncy[j];
+    const int kw = to == g.where[k] ? g.adjwgt[j] : -g.adjwgt[j];
+    g.id[k] += kw;
+    g.ed[k] -= kw;
+    on_nb(k);
+  }
+}
+
+void balance_moves(Part2 &g, const float *ntp)  // cu:2247-2362
+{
+  const int n = g.n;
+  const int t0 = (int)(g.tvwgt * ntp[0]);
+  const int tp[2] = {t0, g.tvwgt - t0};
+  const int mindiff = abs(tp[0] - g.pwgts[0]);
+  const int from = g.pwgts[0] < tp[0] ? 1 : 0, to = 1 - from;
+  std::vector<int> moved(n, -1), perm(std::max(n, 1));
+  MaxQueue q(n);
+  rand_perm(g.nbnd, perm.data(), g.nbnd / 5, true);
+  auto eligible = [&](int k) { return moved[k] == -1 && g.where[k] == from && g.vwgt[k] <= mindiff; };
+  for (int ii = 0; ii < g.nbnd; ++ii) {
+    const int v = g.bndlist[perm[ii]];
+    if (g.where[v] == from && g.vwgt[v] <= mindiff) q.push(v, g.ed[v] - g.id[v]);
+  }
+  int cut = g.mincut;
+  for (int s = 0; s < n; ++s) {
+    const int h = q.pop();
+    if (h == -1 || g.pwgts[to] + g.vwgt[h] > tp[to]) break;
+    cut -= g.ed[h] - g.id[h];
+    g.pwgts[to] += g.vwgt[h];
+    g.pwgts[from] -= g.vwgt[h];
+    moved[h] = s;
+    flip(g, h, to, [&](int k) {
+      if (g.bndptr[k] != -1) {
+        if (g.ed[k] == 0) {
+          g.bnd_delete(k);
+          if (eligible(k)) q.erase(k);
+        } else if (eligible(k)) {
+          q.update(k, g.ed[k] - g.id[k]);
+        }
+      } else if (g.ed[k] > 0) {
+        g.bnd_insert(k);
+        if (eligible(k)) q.push(k, g.ed[k] - g.id[k]);
+      }
+    });
+  }
+  g.mincut = cut;
+}
+
+void balance_2way(Part2 &g, Settings &s, const float *ntp)  // cu:2367-2378 (2way_bal)
+{
+  float worst = -1.0f;
+  for (int j = 0; j < 2; ++j) worst = std::max(worst, g.pwgts[j] * s.balance[j] - s.ub);
+  if (worst <= 0) return;
+  if (fabsf(ntp[0] * g.tvwgt - g.pwgts[0]) < 3 * g.tvwgt / g.n) return;
+  balance_moves(g, ntp);
+}
+
+void fm_refine(Part2 &g, const float *ntp, int niter)  // cu:2367-2531
+{
+  const int n = g.n;
+  const int t0 = (int)(g.tvwgt * ntp[0]);
+  const int tp[2] = {t0, g.tvwgt - t0};
+  const double lim = std::min(std::max(0.01 * n, 15.0), 100.0);
+  const int limit = (int)lim;
+  const int tot = g.pwgts[0] + g.pwgts[1];
+  const int avgvwgt = std::min(tot / 20, 2 * tot / n);
+  std::vector<int> moved(n, -1), swaps(std::max(n, 1)), perm(std::max(n, 1));
+  MaxQueue q0(n), q1(n);
+  MaxQueue *qs[2] = {&q0, &q1};
+  const int origdiff = abs(tp[0] - g.pwgts[0]);
+  for (int pass = 0; pass < niter; ++pass) {
+    q0.clear();
+    q1.clear();
+    int best_at = -1, newcut, mincut, initcut;
+    newcut = mincut = initcut = g.mincut;
+    int mindiff = abs(tp[0] - g.pwgts[0]);
+    rand_perm(g.nbnd, perm.data(), g.nbnd, true);
+    for (int ii = 0; ii < g.nbnd; ++ii) {
+      const int v = g.bndlist[perm[ii]];
+      qs[g.where[v]]->push(v, g.ed[v] - g.id[v]);
+    }
+    int ns = 0;
+    for (; ns < n; ++ns) {
+      const int from = (tp[0] - g.pwgts[0] < tp[1] - g.pwgts[1]) ? 0 : 1, to = 1 - from;
+      const int h = qs[from]->pop();
+      if (h == -1) break;
+      newcut -= g.ed[h] - g.id[h];
+      g.pwgts[to] += g.vwgt[h];
+      g.pwgts[from] -= g.vwgt[h];
+      const int diff = abs(tp[0] - g.pwgts[0]);
+      if ((newcut < mincut && diff <= origdiff + avgvwgt) || (newcut == mincut && diff < mindiff)) {
+        mincut = newcut;
+        mindiff = diff;
+        best_at = ns;
+      } else if (ns - best_at > limit) {
+        newcut += g.ed[h] - g.id[h];
+        g.pwgts[from] += g.vwgt[h];
+        g.pwgts[to] -= g.vwgt[h];
+        break;
+      }
+      moved[h] = ns;
+      swaps[ns] = h;
+      flip(g, h, to, [&](int k) {
+        if (g.bndptr[k] != -1) {
+          if (g.ed[k] == 0) {
+            g.bnd_delete(k);
+            if (moved[k] == -1) qs[g.where[k]]->erase(k);
+          } else if (moved[k] == -1) {
+            qs[g.where[k]]->update(k, g.ed[k] - g.id[k]);
+          }
+        } else if (g.ed[k] > 0) {
+          g.bnd_insert(k);
+          if (moved[k] == -1) qs[g.where[k]]->push(k, g.ed[k] - g.id[k]);
+        }
+      });
+    }
+    for (int i = 0; i < ns; ++i) moved[swaps[i]] = -1;
+    for (--ns; ns > best_at; --ns) {  // roll back past the best prefix
+      const int h = swaps[ns];
+      const int to = g.where[h] = (g.where[h] + 1) % 2;
+      std::swap(g.id[h], g.ed[h]);
+      if (g.ed[h] == 0 && g.bndptr[h] != -1 && g.xadj[h] < g.xadj[h + 1]) g.bnd_delete(h);
+      else if (g.ed[h] > 0 && g.bndptr[h] == -1) g.bnd_insert(h);
+      g.pwgts[to] += g.vwgt[h];
+      g.pwgts[(to + 1) % 2] -= g.vwgt[h];
+      for (int64_t j = g.xadj[h]; j < g.xadj[h + 1]; ++j) {
+        const int k = g.adjncy[j];
+        const int kw = to == g.where[k] ? g.adjwgt[j] : -g.adjwgt[j];
+        g.id[k] += kw;
+        g.ed[k] -= kw;
+        if (g.bndptr[k] != -1 && g.ed[k] == 0) g.bnd_delete(k);
+        if (g.bndptr[k] == -1 && g.ed[k] > 0) g.bnd_insert(k);
+      }
+    }
+    g.mincut = mincut;
+    if (best_at <= 0 || mincut == initcut) break;
+  }
+}
+
+void grow_bisection(Part2 &g, Settings &s, const float *ntp, int niparts)  // cu:2536-2661
+{
+  const int n = g.n;
+  g.alloc2way();
+  std::vector<int> best(n), queue(std::max(n, 1)), seen(n);
+  const int onemax = (int)(s.ub * g.tvwgt * ntp[1]);
+  const int onemin = (int)((1.0 / s.ub) * g.tvwgt * ntp[1]);
+  int bestcut = 0;
+  for (int it = 0; it < niparts; ++it) {
+    std::fill(g.where.begin(), g.where.end(), 1);
+    std::fill(seen.begin(), seen.end(), 0);
+    int pw0 = 0, pw1 = g.tvwgt;
+    queue[0] = rnd(n);
+    seen[queue[0]] = 1;
+    int first = 0, last = 1, nleft = n - 1;
+    bool stalled = false;
+    for (;;) {
+      if (first == last) {  // component exhausted: restart from a random unseen vertex
+        if (nleft == 0 || stalled) break;
+        int k = rnd(nleft), i = 0;
+        for (; i < n; ++i)
+          if (seen[i] == 0) {
+            if (k == 0) break;
+            --k;
+          }
+        queue[0] = i;
+        seen[i] = 1;
+        first = 0;
+        last = 1;
+        --nleft;
+      }
+      const int i = queue[first++];
+      if (pw0 > 0 && pw1 - g.vwgt[i] < onemin) {
+        stalled = true;
+        continue;
+      }
+      g.where[i] = 0;
+      pw0 += g.vwgt[i];
+      pw1 -= g.vwgt[i];
+      if (pw1 <= onemax) break;
+      stalled = false;
+      for (int64_t j = g.xadj[i]; j < g.xadj[i + 1]; ++j) {
+        const int k = g.adjncy[j];
+        if (!seen[k]) {
+          queue[last++] = k;
+          seen[k] = 1;
+          --nleft;
+        }
+      }
+    }
+    compute_2way(g);
+    balance_2way(g, s, ntp);
+    fm_refine(g, ntp, s.niter);
+    if (it == 0 || bestcut > g.mincut) {
+      bestcut = g.mincut;
+      best = g.where;
+      if (bestcut == 0) break;
+    }
+  }
+  g.mincut = bestcut;
+  g.where = best;
+}
+
+void project_2way(Part2 &g)  // cu:2693-2764
+{
+  Part2 &c = *g.coarser;
+  g.alloc2way();
+  for (int i = 0; i < g.n; ++i) {
+    const int k = g.cmap[i];
+    g.where[i] = c.where[k];
+    g.cmap[i] = c.bndptr[k];
+  }
+  g.nbnd = 0;
+  for (int i = 0; i < g.n; ++i) {
+    int in = 0, ex = 0;
+    if (g.cmap[i] == -1) {
+      for (int64_t j = g.xadj[i]; j < g.xadj[i + 1]; ++j) in += g.adjwgt[j];
+    } else {
+      const int me = g.where[i];
+      for (int64_t j = g.xadj[i]; j < g.xadj[i + 1]; ++j)
+        (g.where[g.adjncy[j]] == me ? in : ex) += g.adjwgt[j];
+    }
+    g.id[i] = in;
+    g.ed[i] = ex;
+    if (ex > 0 || g.xadj[i] == g.xadj[i + 1]) g.bnd_insert(i);
+  }
+  g.mincut = c.mincut;
+  g.pwgts[0] = c.pwgts[0];
+  g.pwgts[1] = c.pwgts[1];
+  g.coarser.reset();
+}
+
+int multilevel_bisect(Part2 &g, Settings &s, const float *tp)  // cu:2806-2829
+{
+  const float rev = (float)(1.0 / (g.tvwgt > 0 ? g.tvwgt : 1));
+  for (int i = 0; i < 2; ++i) s.balance[i] = rev / tp[i];
+  Part2 *c = cpu_coarsen(g, s);
+  grow_bisection(*c, s, tp, 5);
+  Part2 *x = c;  // cu:2769-2802
+  compute_2way(*x);
+  for (;;) {
+    balance_2way(*x, s, tp);
+    fm_refine(*x, tp, s.niter);
+    if (x == &g) break;
+    x = x->finer;
+    project_2way(*x);
+  }
+  return g.mincut;
+}
+
+void split(Part2 &g, Part2 &l, Part2 &r)  // cu:2834-2959
+{
+  Part2 *side[2] = {&l, &r};
+  std::vector<int> rename(g.n);
+  int cnt[2] = {0, 0};
+  for (int i = 0; i < g.n; ++i) rename[i] = cnt[g.where[i]]++;
+  for (int p = 0; p < 2; ++p) {
+    side[p]->n = cnt[p];
+    side[p]->xadj.assign(1, 0);
+    side[p]->xadj.reserve(cnt[p] + 1);
+    side[p]->vwgt.reserve(cnt[p]);
+    side[p]->label.reserve(cnt[p]);
+  }
+  for (int i = 0; i < g.n; ++i) {
+    const int p = g.where[i];
+    Part2 &s = *side[p];
+    const bool interior = g.bndptr[i] == -1;
+    for (int64_t j = g.xadj[i]; j < g.xadj[i + 1]; ++j) {
+      const int k = g.adjncy[j];
+      if (interior || g.where[k] == p) {
+        s.adjncy.push_back(rename[k]);
+        s.adjwgt.push_back(g.adjwgt[j]);
+      }
+    }
+    s.vwgt.push_back(g.vwgt[i]);
+    s.label.push_back(g.label[i]);
+    s.xadj.push_back((int64_t)s.adjncy.size());
+  }
+  for (int p = 0; p < 2; ++p) {
+    side[p]->m = side[p]->xadj.back();
+    side[p]->set_tvwgt();
+  }
+}
+
+int recursive_bisect(std::unique_ptr<Part2> g, Settings &s, int nparts, int *part, float *tp,
+                     int fpart)  // cu:2964-3025
+{
+  if (g->n == 0) return 0;
+  float sum = 0;
+  for (int i = 0; i < (nparts >> 1); ++i) sum += tp[i];
+  const float tp2[2] = {sum, 1.0f - sum};
+  int obj = multilevel_bisect(*g, s, tp2);
+  for (int i = 0; i < g->n; ++i) part[g->label[i]] = g->where[i] + fpart;
+  std::unique_ptr<Part2> l, r;
+  if (nparts > 2) {
+    l.reset(new Part2());
+    r.reset(new Part2());
+    split(*g, *l, *r);
+  }
+  g.reset();
+  const float f0 = (float)(1.0 / sum), f1 = (float)(1.0 / (1.0 - sum));
+  for (int i = 0; i < (nparts >> 1); ++i) tp[i] *= f0;
+  for (int i = nparts >> 1; i < nparts; ++i) tp[i] *= f1;
+  if (nparts > 3) {
+    obj += recursive_bisect(std::move(l), s, nparts >> 1, part, tp, fpart);
+    obj += recursive_bisect(std::move(r), s, nparts - (nparts >> 1), part, tp + (nparts >> 1),
+                            fpart + (nparts >> 1));
+  } else if (nparts == 3) {
+    obj += recursive_bisect(std::move(r), s, nparts - (nparts >> 1), part, tp + (nparts >> 1),
+                            fpart + (nparts >> 1));
+  }
+  return obj;
+}
+
+}  // namespace
+
+int initial_partition(const HostGraph &hg, int nparts, std::vector<int> &where)
+{
+  where.assign(hg.n, 0);
+  if (nparts <= 1 || hg.n == 0) return 0;
+  std::unique_ptr<Part2> g(new Part2());
+  g->n = hg.n;
+  g->m = hg.m();
+  g->xadj = hg.xadj;
+  g->adjncy = hg.adjncy;
+  g->adjwgt = hg.adjwgt;
+  g->vwgt = hg.vwgt;
+  g->label.resize(hg.n);
+  for (int i = 0; i < hg.n; ++i) g->label[i] = i;
+  g->set_tvwgt();
+  Settings s;
+  std::vector<float> tp(nparts, (float)(1.0 / nparts));
+  return recursive_bisect(std::move(g), s, nparts, where.data(), tp.data(), 0);
+}
+
+int schedule_moves(int nparts, int64_t tvwgt, const std::vector<uint64_t> &cand,
+                   const std::vector<int> &from, const std::vector<int> &vw,
+                   std::vector<int64_t> &pw, std::vector<int> &moves)
+{
+  // cu:3401-3412: limits from float targets 1/k and ub 1.03
+  const float tp = (float)(1.0 / nparts);
+  const int maxw = (int)(tp * (float)tvwgt * 1.03f);
+  const int minw = (int)(tp * (float)tvwgt / 1.03f);
+  moves.clear();
+  std::vector<char> done(cand.size(), 0);
+  for (int pass = 0; pass < 2; ++pass) {
+    for (size_t q = 0; q < cand.size(); ++q) {
+      const int to = (int)(cand[q] >> 32), v = (int)(uint32_t)cand[q];
+      const int me = from[q];
+      if (done[q] || (pass == 0 ? me > to : me <= to)) continue;
+      const int64_t a = pw[to] + vw[q], b = pw[me] - vw[q];
+      if (a >= minw && a <= maxw && b >= minw && b <= maxw) {
+        if (me != to) {  // a move within one part leaves the weights unchanged
+          pw[to] = a;
+          pw[me] = b;
+          moves.push_back(v);
+          moves.push_back(to);
+        }
+        done[q] = 1;
+      }
+    }
+  }
+  return (int)moves.size() / 2;
+}
+
+}  // namespace cmh

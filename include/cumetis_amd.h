@@ -184,6 +184,21 @@ typedef struct {
 int cm_partition(cm_ctx *ctx, const cm_graph_view *g, const cm_coarsen_params *p, int32_t *part,
                  cm_part_report *report);
 
+/* ------------------------------------------------------------ host parts */
+/* METIS .graph reader (cu:3712-3851): arrays are malloc'd, release each
+ * with cm_free_host.  Malformed input returns CM_ERR_INPUT with the message
+ * in err (the reference exits instead).  Self loops are rejected. */
+int cm_read_metis(const char *path, int64_t *n, int64_t *m, int64_t **xadj, int32_t **adjncy,
+                  int32_t **adjwgt, int32_t **vwgt, char *err, int errcap);
+void cm_free_host(void *p);
+/* "<graph>.part.<k>" writer: one part id per line (cu:3856-3870). */
+int cm_write_partition(const char *path, const int32_t *part, int64_t n);
+/* The host initial partition alone (cuMetis_initialpartition, cu:3091-3108;
+ * recursive bisection cu:2964-3025).  Draws glibc rand() from its current
+ * state; *objval receives the summed bisection cuts. */
+int cm_initial_partition(int64_t n, const int64_t *xadj, const int32_t *adjncy, const int32_t *adjwgt,
+                         const int32_t *vwgt, int32_t nparts, int32_t *where, int64_t *objval);
+
 #ifdef __cplusplus
 }
 #endif

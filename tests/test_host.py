@@ -1,0 +1,97 @@
+"""Host parts of the product (no GPU): the C++ initial partition against the
+oracle's restatement, and the METIS reader/writer of the C ABI."""
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from cases import named_cases, with_weights
+from cumetis_amd import _lib, graphs
+
+GOLD = Path(__file__).resolve().parent / "golden"
+
+
+def product_initpart(g, k, seed=1):
+    lib = _lib.load()
+    import oracle.oracle as O
+    O.srand(seed)  # both draw from the same glibc stream
+    x = np.ascontiguousarray(g.xadj, np.int64)
+    a = np.ascontiguousarray(g.adjncy, np.int32)
+    w = np.ascontiguousarray(g.adjwgt, np.int32)
+    v = np.ascontiguousarray(g.vwgt, np.int32)
+    where = np.zeros(max(g.n, 1), np.int32)
+    obj = C.c_int64()
+    rc = lib.cm_initial_partition(g.n, x.ctypes.data, a.ctypes.data, w.ctypes.data, v.ctypes.data,
+                                  k, where.ctypes.data, C.byref(obj))
+    assert rc == 0
+    return where[:g.n], obj.value
+
+
+@pytest.mark.parametrize("name,k", [("delaunay4720", 8), ("grid32w100", 5), ("rgg3000w16", 16),
+                                    ("rmat10", 4), ("hub", 3), ("path2", 2)])
+def test_initial_partition_matches_oracle(orc, name, k):
+    g = with_weights(named_cases()[name])
+    wp, op = product_initpart(g, k)
+    orc.srand(1)
+    wo, oo = orc.initpart(g.xadj, g.adjncy, g.adjwgt, g.vwgt, k)
+    assert np.array_equal(wp, wo)
+    assert op == oo
+
+
+def test_initial_partition_weighted_vertices(orc):
+    g = with_weights(graphs.grid2d(25, 25, wmax=30, seed=4))
+    g.vwgt[:] = np.random.default_rng(1).integers(1, 6, size=g.n)
+    wp, op = product_initpart(g, 6)
+    orc.srand(1)
+    wo, oo = orc.initpart(g.xadj, g.adjncy, g.adjwgt, g.vwgt, 6)
+    assert np.array_equal(wp, wo) and op == oo
+
+
+def test_read_metis_3elt():
+    lib = _lib.load()
+    n, m = C.c_int64(), C.c_int64()
+    px, pa, pw, pv = C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_void_p()
+    err = C.create_string_buffer(256)
+    rc = lib.cm_read_metis(str(GOLD / "3elt.graph").encode(), C.byref(n), C.byref(m), C.byref(px),
+                           C.byref(pa), C.byref(pw), C.byref(pv), err, 256)
+    assert rc == 0, err.value
+    ref = graphs.read_metis(str(GOLD / "3elt.graph"))
+    xadj = np.ctypeslib.as_array((C.c_int64 * (n.value + 1)).from_address(px.value)).copy()
+    adj = np.ctypeslib.as_array((C.c_int32 * m.value).from_address(pa.value)).copy()
+    for p in (px, pa, pw, pv):
+        lib.cm_free_host(p)
+    assert n.value == 4720 and m.value == 27444
+    assert np.array_equal(xadj, ref.xadj.astype(np.int64)) and np.array_equal(adj, ref.adjncy)
+
+
+@pytest.mark.parametrize("text,msg", [
+    ("3\n2\n1\n", b"number of vertices and edges"),
+    ("0 1\n", b"must be positive"),
+    ("2 1 112\n2\n1\n", b"Cannot read this type"),
+    ("2 1\n3\n1\n", b"out of bounds"),
+    ("2 1\n1\n1\n", b"self loop"),
+    ("2 1 1\n2 0\n1 0\n", b"must be positive"),
+    ("3 1\n2\n1 3\n2\n", b"more edges"),
+    ("3 2\n2\n1\n\n", b"only found"),
+    ("3 1\n2\n1\n", b"Premature end"),
+])
+def test_read_metis_errors(tmp_path, text, msg):
+    lib = _lib.load()
+    f = tmp_path / "bad.graph"
+    f.write_text(text)
+    n, m = C.c_int64(), C.c_int64()
+    px, pa, pw, pv = C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_void_p()
+    err = C.create_string_buffer(512)
+    rc = lib.cm_read_metis(str(f).encode(), C.byref(n), C.byref(m), C.byref(px), C.byref(pa),
+                           C.byref(pw), C.byref(pv), err, 512)
+    assert rc == _lib.CM_ERR_INPUT
+    assert msg in err.value
+
+
+def test_write_partition(tmp_path):
+    lib = _lib.load()
+    part = np.array([3, 0, 2, 1], np.int32)
+    f = tmp_path / "g.graph.part.4"
+    assert lib.cm_write_partition(str(f).encode(), part.ctypes.data, 4) == 0
+    assert f.read_text() == "3\n0\n2\n1\n"
