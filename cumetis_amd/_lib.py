@@ -38,7 +38,8 @@ class LevelInfo(C.Structure):
     _fields_ = [("n", C.c_int64), ("m", C.c_int64), ("offset_type", C.c_int32),
                 ("rounds", C.c_int32), ("xadj", C.c_void_p), ("adjncy", C.c_void_p),
                 ("adjwgt", C.c_void_p), ("vwgt", C.c_void_p), ("match", C.c_void_p),
-                ("cmap", C.c_void_p), ("tvwgt", C.c_int64)]
+                ("cmap", C.c_void_p), ("tvwgt", C.c_int64), ("vertex_base", C.c_int64),
+                ("edge_base", C.c_int64), ("coarse_base", C.c_int64)]
 
 
 class PartReport(C.Structure):

@@ -351,6 +351,15 @@ static Level coarsen_level(cm_ctx *ctx, cm_hier *h, Level &F, int32_t maxvwgt, u
 
 }  // namespace cm
 
+namespace cm {
+SharedBlocks::~SharedBlocks()
+{
+  if (!ctx) return;
+  (void)hipStreamSynchronize(ctx->stream);
+  for (void *q : blocks) ctx->pool.release(q);
+}
+}  // namespace cm
+
 // ------------------------------------------------------------- cm_ctx
 hipEvent_t cm_ctx::ev_get()
 {
@@ -587,6 +596,9 @@ int cm_hier_level(const cm_hier *h, int level, cm_level_info *out)
   out->match = L.match;
   out->cmap = L.cmap;
   out->tvwgt = L.tvwgt;
+  out->vertex_base = L.vbase;
+  out->edge_base = L.ebase;
+  out->coarse_base = L.cbase;
   return CM_OK;
 }
 
@@ -608,6 +620,21 @@ int cm_hier_copy_level(cm_ctx *ctx, const cm_hier *h, int level, void *xadj, int
   cp(match, L.match, 4 * L.n);
   cp(cmap, L.cmap, 4 * L.n);
   cm::sync(ctx);
+  // local values for batch slices
+  if (xadj && L.ebase) {
+    if (L.off64)
+      for (int64_t i = 0; i <= L.n; ++i) ((int64_t *)xadj)[i] -= L.ebase;
+    else
+      for (int64_t i = 0; i <= L.n; ++i) ((int32_t *)xadj)[i] -= (int32_t)L.ebase;
+  }
+  if (L.vbase) {
+    if (adjncy)
+      for (int64_t j = 0; j < L.m; ++j) adjncy[j] -= (int32_t)L.vbase;
+    if (match && L.match)
+      for (int64_t i = 0; i < L.n; ++i) match[i] -= (int32_t)L.vbase;
+  }
+  if (cmap && L.cmap && L.cbase)
+    for (int64_t i = 0; i < L.n; ++i) cmap[i] -= (int32_t)L.cbase;
   CM_API_END(ctx)
 }
 
@@ -616,7 +643,8 @@ int cm_project(cm_ctx *ctx, const cm_hier *h, int level, const int32_t *d_where_
   if (!ctx || !h || level < 0 || level + 1 >= (int)h->levels.size()) return CM_ERR_INVALID;
   CM_API_BEGIN
   const cm::Level &L = h->levels[level];
-  cm::project(ctx, L.n, L.cmap, d_where_c, d_where_f);
+  // stored cmap values are local + cbase: shift the coarse array instead
+  cm::project(ctx, L.n, L.cmap, d_where_c - L.cbase, d_where_f);
   CM_API_END(ctx)
 }
 

@@ -6,6 +6,7 @@
 #include <stdint.h>
 
 #include <map>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -70,13 +71,28 @@ struct Level {
   int64_t tvwgt = 0;
   bool owns_graph = false;  // xadj/adjncy/adjwgt/vwgt come from the pool
   int rounds = 0;           // handshake rounds used at this level
+  // batch slices: stored ids are local id + base (0 for a single graph)
+  int64_t vbase = 0;  // added to adjncy and match values
+  int64_t ebase = 0;  // added to xadj values
+  int64_t cbase = 0;  // added to cmap values
 };
 
+}  // namespace cm
+
+namespace cm {
+// Device blocks shared by the hierarchies of one batch; released when the
+// last of them is released.
+struct SharedBlocks {
+  cm_ctx *ctx = nullptr;
+  std::vector<void *> blocks;
+  ~SharedBlocks();
+};
 }  // namespace cm
 
 struct cm_hier {
   std::vector<cm::Level> levels;
   std::vector<void *> owned;  // pool blocks released with the hierarchy
+  std::shared_ptr<cm::SharedBlocks> shared;
 };
 
 struct cm_ctx {
@@ -138,9 +154,22 @@ void fill_u32(cm_ctx *ctx, void *p, uint32_t v, int64_t count);
 void fill_u64(cm_ctx *ctx, void *p, uint64_t v, int64_t count);
 void scan_excl_i32(cm_ctx *ctx, int64_t n, const int32_t *in, int32_t *out);
 
+// Owning graph of every vertex of a batch (disjoint union); gid == nullptr
+// means one graph with maxv0 / seed0.
+struct MatchOwners {
+  const int32_t *gid = nullptr;    // graph slot of each vertex
+  const int32_t *maxv = nullptr;   // per slot
+  const uint64_t *seed = nullptr;  // per slot
+  const int64_t *vbase = nullptr;  // per slot: first vertex in the union
+  int32_t maxv0 = 0;
+  uint64_t seed0 = 0;
+};
+
 // ---- subsystem entry points (device pointers, ctx stream) ----
 // (a) matching; returns rounds.  match[] ends complete (partner or self).
 int match_handshake(cm_ctx *ctx, const Level &L, int32_t maxvwgt, uint64_t seed, int max_rounds,
+                    int32_t *d_match);
+int match_handshake(cm_ctx *ctx, const Level &L, const MatchOwners &own, int max_rounds,
                     int32_t *d_match);
 // (b) numbering: cmap for every fine vertex; rep/segoff/cvwgt per coarse vertex.
 //     Returns nc; segoff[nc] = m.

@@ -29,6 +29,24 @@ __device__ __forceinline__ bool round_live(const unsigned *cnt, int r)
   return cnt[2 * (r - 1)] != 0 && cnt[2 * (r - 1) + 1] != 0;
 }
 
+// Per-vertex matching parameters.  A single graph uses maxv0/seed0 and
+// base 0; a batch (disjoint union of graphs) looks up the owning graph of v
+// so that every graph keeps its own weight limit, seed and local vertex ids.
+__device__ __forceinline__ void owner_params(const MatchOwners &o, int64_t v, int32_t &maxv,
+                                             uint64_t &seed, int64_t &base)
+{
+  if (o.gid) {
+    const int32_t g = o.gid[v];
+    maxv = o.maxv[g];
+    seed = o.seed[g];
+    base = o.vbase[g];
+  } else {
+    maxv = o.maxv0;
+    seed = o.seed0;
+    base = 0;
+  }
+}
+
 __device__ __forceinline__ bool key_better(uint64_t a1, uint64_t a2, uint64_t b1, uint64_t b2)
 {
   return a1 > b1 || (a1 == b1 && a2 > b2);
@@ -62,8 +80,8 @@ template <typename OffT>
 __global__ __launch_bounds__(256) void k_propose(int64_t n, const OffT *__restrict__ xadj,
                                                  const int32_t *__restrict__ adjncy,
                                                  const int32_t *__restrict__ adjwgt,
-                                                 const int32_t *__restrict__ vwgt, int32_t maxvwgt,
-                                                 uint64_t seed, const int32_t *__restrict__ match,
+                                                 const int32_t *__restrict__ vwgt, MatchOwners own,
+                                                 const int32_t *__restrict__ match,
                                                  int32_t *__restrict__ prop, const unsigned *cnt, int r)
 {
   if (!round_live(cnt, r)) return;
@@ -73,6 +91,10 @@ __global__ __launch_bounds__(256) void k_propose(int64_t n, const OffT *__restri
   if ((int64_t)(e - b) > LONG_DEG) return;
   const int32_t p = prop[v];
   if (p >= 0 && match[p] == -1) return;
+  int32_t maxvwgt;
+  uint64_t seed;
+  int64_t base;
+  owner_params(own, v, maxvwgt, seed, base);
   const int32_t vw = vwgt[v];
   int32_t best = -2;
   uint64_t bk1 = 0, bk2 = 0;
@@ -81,7 +103,7 @@ __global__ __launch_bounds__(256) void k_propose(int64_t n, const OffT *__restri
     const uint32_t w = (uint32_t)adjwgt[j];
     if (best != -2 && w < (uint32_t)(bk1 >> 32)) continue;
     if (u == (int32_t)v || match[u] != -1 || vw + vwgt[u] > maxvwgt) continue;
-    const uint32_t a = (uint32_t)min((int64_t)u, v), bb = (uint32_t)max((int64_t)u, v);
+    const uint32_t a = (uint32_t)(min((int64_t)u, v) - base), bb = (uint32_t)(max((int64_t)u, v) - base);
     const uint64_t k1 = ((uint64_t)w << 32) | edge_hash(seed, a, bb);
     const uint64_t k2 = ((uint64_t)a << 32) | bb;
     if (best == -2 || key_better(k1, k2, bk1, bk2)) {
@@ -100,7 +122,7 @@ __global__ __launch_bounds__(256) void k_propose_long(const int32_t *list, const
                                                       const int32_t *__restrict__ adjncy,
                                                       const int32_t *__restrict__ adjwgt,
                                                       const int32_t *__restrict__ vwgt,
-                                                      int32_t maxvwgt, uint64_t seed,
+                                                      MatchOwners own,
                                                       const int32_t *__restrict__ match,
                                                       int32_t *__restrict__ prop, const unsigned *cnt,
                                                       int r)
@@ -115,6 +137,10 @@ __global__ __launch_bounds__(256) void k_propose_long(const int32_t *list, const
     const int32_t p = prop[v];
     if (p >= 0 && match[p] == -1) continue;
     const OffT b = xadj[v], e = xadj[v + 1];
+    int32_t maxvwgt;
+    uint64_t seed;
+    int64_t base;
+    owner_params(own, v, maxvwgt, seed, base);
     const int32_t vw = vwgt[v];
     int32_t best = -2;
     uint64_t bk1 = 0, bk2 = 0;
@@ -123,7 +149,7 @@ __global__ __launch_bounds__(256) void k_propose_long(const int32_t *list, const
       const uint32_t w = (uint32_t)adjwgt[j];
       if (best != -2 && w < (uint32_t)(bk1 >> 32)) continue;
       if (u == v || match[u] != -1 || vw + vwgt[u] > maxvwgt) continue;
-      const uint32_t a = (uint32_t)min(u, v), bb = (uint32_t)max(u, v);
+      const uint32_t a = (uint32_t)(min(u, v) - base), bb = (uint32_t)(max(u, v) - base);
       const uint64_t k1 = ((uint64_t)w << 32) | edge_hash(seed, a, bb);
       const uint64_t k2 = ((uint64_t)a << 32) | bb;
       if (best == -2 || key_better(k1, k2, bk1, bk2)) {
@@ -192,7 +218,7 @@ __global__ __launch_bounds__(256) void k_match_finish(int64_t n, int32_t *match)
 }
 
 template <typename OffT>
-static int match_impl(cm_ctx *ctx, const Level &L, int32_t maxvwgt, uint64_t seed, int max_rounds,
+static int match_impl(cm_ctx *ctx, const Level &L, const MatchOwners &own, int max_rounds,
                       int32_t *match)
 {
   const int64_t n = L.n;
@@ -216,11 +242,11 @@ static int match_impl(cm_ctx *ctx, const Level &L, int32_t maxvwgt, uint64_t see
     const int r0 = r;
     for (int i = 0; i < ROUND_BATCH && r < max_rounds; ++i, ++r) {
       launch(ctx, "match_propose", r == 0 ? row_bytes : 0.0, k_propose<OffT>, dim3(g), dim3(256),
-             0, n, xadj, L.adjncy, L.adjwgt, L.vwgt, maxvwgt, seed, (const int32_t *)match, prop,
+             0, n, xadj, L.adjncy, L.adjwgt, L.vwgt, own, (const int32_t *)match, prop,
              (const unsigned *)cnt, r);
       launch(ctx, "match_propose_long", 0.0, k_propose_long<OffT>, dim3(glong), dim3(256), 0,
              (const int32_t *)list, (const unsigned *)list_n, xadj, L.adjncy, L.adjwgt, L.vwgt,
-             maxvwgt, seed, (const int32_t *)match, prop, (const unsigned *)cnt, r);
+             own, (const int32_t *)match, prop, (const unsigned *)cnt, r);
       launch(ctx, "match_accept", 12.0 * n, k_accept, dim3(g), dim3(256), 0, n, match,
              (const int32_t *)prop, cnt, r);
     }
@@ -246,12 +272,21 @@ static int match_impl(cm_ctx *ctx, const Level &L, int32_t maxvwgt, uint64_t see
   return rounds;
 }
 
-int match_handshake(cm_ctx *ctx, const Level &L, int32_t maxvwgt, uint64_t seed, int max_rounds,
+int match_handshake(cm_ctx *ctx, const Level &L, const MatchOwners &own, int max_rounds,
                     int32_t *d_match)
 {
   if (max_rounds <= 0) max_rounds = 4096;
-  return L.off64 ? match_impl<int64_t>(ctx, L, maxvwgt, seed, max_rounds, d_match)
-                 : match_impl<int32_t>(ctx, L, maxvwgt, seed, max_rounds, d_match);
+  return L.off64 ? match_impl<int64_t>(ctx, L, own, max_rounds, d_match)
+                 : match_impl<int32_t>(ctx, L, own, max_rounds, d_match);
+}
+
+int match_handshake(cm_ctx *ctx, const Level &L, int32_t maxvwgt, uint64_t seed, int max_rounds,
+                    int32_t *d_match)
+{
+  MatchOwners own{};
+  own.maxv0 = maxvwgt;
+  own.seed0 = seed;
+  return match_handshake(ctx, L, own, max_rounds, d_match);
 }
 
 }  // namespace cm

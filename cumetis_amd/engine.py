@@ -307,3 +307,51 @@ class Engine:
         self._check(self.lib.cm_partition(self.ctx, C.byref(v), C.byref(p), part.ctypes.data,
                                           C.byref(rep)))
         return part[:g.n], rep
+
+
+def concat_graphs(gs):
+    """Concatenate graphs for Engine.coarsen_batch: global offsets, local ids."""
+    n = [g.n for g in gs]
+    m = [g.m for g in gs]
+    voff = np.concatenate([[0], np.cumsum(n)]).astype(np.int64)
+    eoff = np.concatenate([[0], np.cumsum(m)]).astype(np.int64)
+    use64 = any(g.xadj.dtype == np.int64 for g in gs)
+    odt = np.int64 if use64 else np.int32
+    xadj = np.concatenate([np.asarray(g.xadj[:-1], np.int64) + eoff[b] for b, g in enumerate(gs)]
+                          + [[eoff[-1]]]).astype(odt)
+    adjncy = np.concatenate([np.asarray(g.adjncy, np.int32) for g in gs] or [np.zeros(0, np.int32)])
+    adjwgt = np.concatenate([np.ones(g.m, np.int32) if g.adjwgt is None else np.asarray(g.adjwgt, np.int32)
+                             for g in gs] or [np.zeros(0, np.int32)])
+    vwgt = np.concatenate([np.ones(g.n, np.int32) if g.vwgt is None else np.asarray(g.vwgt, np.int32)
+                           for g in gs] or [np.zeros(0, np.int32)])
+    return Graph(xadj, adjncy, adjwgt, vwgt), voff, eoff
+
+
+def _coarsen_batch(self, gs, nparts, seed=1, row_order=L.CM_ROW_FIRST_SEEN, max_rounds=0,
+                   on_device=False, concat=None):
+    """Batched coarsening; returns one Hierarchy per graph."""
+    cg, voff, eoff = concat if concat is not None else concat_graphs(gs)
+    nb = len(voff) - 1
+    ks = list(nparts) if hasattr(nparts, "__len__") else [nparts] * nb
+    seeds = list(seed) if hasattr(seed, "__len__") else [seed] * nb
+    if on_device:
+        v, keep = self.device_graph(cg)
+    else:
+        v, keep = self.host_view(cg)
+    P = (L.CoarsenParams * max(nb, 1))()
+    for b in range(nb):
+        P[b] = L.CoarsenParams(int(ks[b]), 0, 0, row_order, 0, max_rounds, int(seeds[b]), 0, 0)
+    H = (C.c_void_p * max(nb, 1))()
+    vo = np.ascontiguousarray(voff, np.int64)
+    eo = np.ascontiguousarray(eoff, np.int64)
+    self._check(self.lib.cm_coarsen_batch(self.ctx, nb, C.byref(v), vo.ctypes.data_as(C.POINTER(C.c_int64)),
+                                          eo.ctypes.data_as(C.POINTER(C.c_int64)), P, H))
+    out = []
+    for b in range(nb):
+        h = Hierarchy(self, C.c_void_p(H[b]))
+        h._keep = keep
+        out.append(h)
+    return out
+
+
+Engine.coarsen_batch = _coarsen_batch

@@ -78,7 +78,12 @@ typedef struct {
 /* cm_coarsen_params.flags */
 #define CM_FLAG_TRUSTED_INPUT 1 /* skip the device-side validation of the input graph */
 
-/* One level of a hierarchy (device pointers owned by the hierarchy). */
+/* One level of a hierarchy (device pointers owned by the hierarchy).
+ * A level of a batched hierarchy is a slice of the batch's level arrays:
+ * its stored values are local values plus a base, i.e. xadj holds
+ * local + edge_base, adjncy and match hold local + vertex_base, cmap holds
+ * local + coarse_base (all bases are 0 for cm_coarsen).  cm_hier_copy_level
+ * returns local values. */
 typedef struct {
   int64_t n;
   int64_t m;
@@ -91,6 +96,9 @@ typedef struct {
   int32_t *match; /* NULL on the coarsest level */
   int32_t *cmap;  /* NULL on the coarsest level */
   int64_t tvwgt;
+  int64_t vertex_base;
+  int64_t edge_base;
+  int64_t coarse_base;
 } cm_level_info;
 
 /* ---------------------------------------------------------------- context */
@@ -120,10 +128,15 @@ int cm_memcpy_d2h(cm_ctx *ctx, void *dst, const void *src, size_t bytes);
  * nc > threshold && nc < 0.75 n && mc > nc/2. */
 int cm_coarsen(cm_ctx *ctx, const cm_graph_view *g, const cm_coarsen_params *p, cm_hier **out);
 
-/* Batched coarsening: nb graphs concatenated (vertex ids local to each
- * graph); graph b owns vertices [voff[b], voff[b+1]) and entries
- * [eoff[b], eoff[b+1]) of the concatenated arrays.  Each graph gets its own
- * hierarchy in out[b], bit-identical to coarsening it alone. */
+/* Batched coarsening of nb graphs concatenated into one CSR: concat->xadj
+ * holds the offsets of the concatenation (sum n + 1 entries, concat->m =
+ * sum m), adjncy holds vertex ids local to each graph; graph b owns
+ * vertices [voff[b], voff[b+1]) and entries [eoff[b], eoff[b+1]); voff and
+ * eoff are host arrays of nb+1 entries; params[b] applies to graph b.  Every
+ * level is one pass of each kernel over the union of the graphs still
+ * coarsening; graph b stops by its own rule.  out[b] receives a hierarchy
+ * whose levels (after subtracting the bases, see cm_level_info) are
+ * bit-identical to cm_coarsen of graph b alone. */
 int cm_coarsen_batch(cm_ctx *ctx, int32_t nb, const cm_graph_view *concat, const int64_t *voff,
                      const int64_t *eoff, const cm_coarsen_params *params, cm_hier **out);
 

@@ -203,3 +203,25 @@ def coarsen(xadj, adjncy, adjwgt, vwgt, nparts, matcher="cpu", seed=1, T=None, m
         if not coarsen_continue(nc, L["n"], int(cx[-1]), T):
             break
     return levels
+
+
+def partition(xadj, adjncy, adjwgt, vwgt, nparts, matcher="handshake", seed=1):
+    """The multilevel pipeline (cu:3544-3616) on the CPU: coarsen (matcher
+    'cpu' = the reference's CPU HEM, 'handshake' = the engine's matching
+    model), initial partition of the coarsest level (glibc stream from seed
+    1), then refine the coarsest level and, for every finer level, project and
+    refine.  Returns (where, levels)."""
+    n = len(xadj) - 1
+    if nparts == 1 or n == 0:
+        return np.zeros(n, np.int32), []
+    if matcher == "cpu":
+        srand(1)
+    levels = coarsen(xadj, adjncy, adjwgt, vwgt, nparts, matcher=matcher, seed=seed)
+    C = levels[-1]
+    srand(1)
+    where, _ = initpart(C["xadj"], C["adjncy"], C["adjwgt"], C["vwgt"], nparts)
+    where = kway_refine(C["xadj"], C["adjncy"], C["adjwgt"], C["vwgt"], nparts, where)
+    for L in reversed(levels[:-1]):
+        where = project(L["cmap"], where)
+        where = kway_refine(L["xadj"], L["adjncy"], L["adjwgt"], L["vwgt"], nparts, where)
+    return where, levels
