@@ -2,19 +2,26 @@
 """Coarsening benchmark on one MI355X.
 
 A step is one full multilevel coarsening (match -> number -> contract per
-level, until the reference stop rule ends it) of one synthetic graph that is
-already resident in HBM.  Default workload: 725x725 5-point grid, int32 edge
-weights uniform in [1,100], unit vertex weights, k=64, int32 CSR.
-
-Prints ONE JSON line (see the metric/config fields) with the dominant
-kernel's roofline (algorithmic bytes / hipEvent-timed launch duration on the
-engine's stream) and the reference CPU coarsening path timed on this host.
+level until the reference stop rule ends it) of synthetic input already
+resident in HBM.  Legs:
+  grid725   2D 5-point grid 725x725, adjwgt U[1,100], unit vwgt, k=64, int32
+            CSR (the main line);
+  rmat22    R-MAT scale 22, edge factor 16 (.57/.19/.19), symmetrised, unit
+            weights, k=256, int64 offsets;
+  batch512  512 graphs (60% 2D RGG with n log-uniform in [1k,200k], mean
+            degree 8; 40% grids with side U{32..400}; adjwgt U[1,16];
+            k U{2,4,..,64}): one batched coarsening against the same graphs
+            coarsened one by one.
+Prints ONE JSON line: the main leg's metric, its dominant kernel's roofline
+(algorithmic bytes / hipEvent-timed launch duration on the engine's stream)
+and the reference CPU coarsening path timed on this host; the other legs are
+reported under "legs".
 """
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import json
-import os
 import sys
 import time
 from pathlib import Path
@@ -33,15 +40,94 @@ def level_bytes(n, m, nc, mc, w):
     return 2 * w * (n + 1) + 16 * m + 28 * n + w * (nc + 1) + 8 * mc + 4 * nc
 
 
-def make_graph(cfg: str):
-    from cumetis_amd import graphs
-    if cfg == "grid725":
-        return graphs.grid2d(725, 725, wmax=100, seed=1), 64, "grid2d 725x725 5-point, adjwgt U[1,100], k=64, int32 CSR"
-    if cfg == "rmat22":
-        return graphs.rmat(22, 16, seed=1, offsets=np.int64), 256, "R-MAT s22 ef16 (.57/.19/.19), symmetrised, unit weights, k=256, int64 offsets"
-    if cfg == "delaunay4720":
-        return graphs.delaunay2d(4720, seed=1), 8, "Delaunay 4720 uniform points, unit weights, k=8"
-    raise SystemExit(f"unknown config {cfg}")
+def hier_shape(eng, h):
+    from cumetis_amd import _lib as L
+    out = []
+    for i in range(eng.lib.cm_hier_num_levels(h)):
+        li = L.LevelInfo()
+        eng._check(eng.lib.cm_hier_level(h, i, C.byref(li)))
+        out.append((li.n, li.m, li.rounds))
+    return out
+
+
+def shape_metrics(shape, w):
+    edges = sum(m // 2 for (n, m, r) in shape[:-1])
+    algo = sum(level_bytes(shape[i][0], shape[i][1], shape[i + 1][0], shape[i + 1][1], w)
+               for i in range(len(shape) - 1))
+    return edges, algo
+
+
+def timed_steps(eng, step, warmup, steps):
+    for _ in range(warmup):
+        eng.lib.cm_hier_release(eng.ctx, step())
+    eng.sync()
+    times, last = [], None
+    for i in range(steps):
+        t = time.perf_counter()
+        h = step()
+        eng.sync()
+        times.append(time.perf_counter() - t)
+        if last is not None:
+            eng.lib.cm_hier_release(eng.ctx, last)
+        last = h
+    return times, last
+
+
+def kernel_profile(eng, step, reps=3):
+    """Per-kernel hipEvent timing on the engine stream (separate pass)."""
+    eng.set_timing(True)
+    eng.reset_stats()
+    for _ in range(reps):
+        eng.lib.cm_hier_release(eng.ctx, step())
+    eng.sync()
+    ks = eng.kernel_stats()
+    eng.set_timing(False)
+    priced = {n: s for n, s in ks.items() if s["bytes"] > 0}
+    roof = None
+    if priced:
+        name, s = max(priced.items(), key=lambda kv: kv[1]["ms"])
+        avg_s = s["ms"] / s["count"] / 1e3
+        bpl = s["bytes"] / s["count"]
+        ach = bpl / avg_s / 1e9
+        roof = {"bound": "hbm", "kernel": name, "achieved": round(ach, 2), "peak": 8000.0,
+                "unit": "GB/s", "frac": round(ach / 8000.0, 4), "traffic": None,
+                "avg_launch_us": round(avg_s * 1e6, 2), "algo_bytes_per_launch": int(bpl)}
+    per = {n: round(s["ms"] / reps, 4) for n, s in sorted(ks.items(), key=lambda kv: -kv[1]["ms"])[:14]}
+    return roof, per
+
+
+def progress(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def coarsen_leg(eng, g, k, order, warmup, steps, profile=True):
+    from cumetis_amd import _lib as L
+    progress(f"coarsen leg n={g.n} m={g.m} k={k}")
+    view, keep = eng.device_graph(g)  # input resident in HBM before timing
+    eng.sync()
+    p = L.CoarsenParams(k, 0, 0, order, 0, 0, 1, L.CM_FLAG_TRUSTED_INPUT, 0)
+
+    def step():
+        h = C.c_void_p()
+        eng._check(eng.lib.cm_coarsen(eng.ctx, C.byref(view), C.byref(p), C.byref(h)))
+        return h
+
+    times, h = timed_steps(eng, step, warmup, steps)
+    shape = hier_shape(eng, h)
+    eng.lib.cm_hier_release(eng.ctx, h)
+    w = 8 if g.xadj.dtype == np.int64 else 4
+    edges, algo = shape_metrics(shape, w)
+    med = float(np.median(times))
+    out = {"medges_per_s": round(edges / med / 1e6, 2), "ms": round(med * 1e3, 3),
+           "p10_ms": round(float(np.percentile(times, 10)) * 1e3, 3),
+           "p90_ms": round(float(np.percentile(times, 90)) * 1e3, 3),
+           "levels": len(shape), "coarsest_n": int(shape[-1][0]),
+           "rounds": [r for (_, _, r) in shape[:-1]],
+           "hbm_frac_whole_coarsening": round(algo / med / HBM_PEAK, 5), "algo_bytes": int(algo),
+           "input_medges_per_s": round((g.m // 2) / med / 1e6, 2), "edges": int(edges)}
+    roof, per = kernel_profile(eng, step) if profile else (None, {})
+    del keep
+    return out, roof, per
 
 
 def cpu_baseline(g, k, budget_s=12.0):
@@ -61,9 +147,91 @@ def cpu_baseline(g, k, budget_s=12.0):
         if time.perf_counter() - t0 > budget_s or dt * (reps + 1) > budget_s * 1.5:
             break
     el = time.perf_counter() - t0
-    return {"value": edges / el / 1e6, "unit": "Medges/s", "cores": 1, "kind": "port",
+    return {"value": round(edges / el / 1e6, 2), "unit": "Medges/s", "cores": 1, "kind": "port",
             "sample": f"{reps} full CPU coarsening run(s) of the same graph (cpu_match + "
                       f"cpu_create_cgraph per level, k-way threshold), {el:.1f} s"}
+
+
+def batch_graphs(count=512, seed=2024):
+    from cumetis_amd import graphs
+    rng = np.random.default_rng(seed)
+    gs, ks = [], []
+    for i in range(count):
+        if rng.random() < 0.6:
+            n = int(np.exp(rng.uniform(np.log(1000), np.log(200000))))
+            g = graphs.rgg2d_fast(n, 8.0, wmax=16, seed=seed + i)
+        else:
+            s = int(rng.integers(32, 401))
+            g = graphs.grid2d(s, s, wmax=16, seed=seed + i)
+        gs.append(g)
+        ks.append(int(2 * rng.integers(1, 33)))
+    return gs, ks
+
+
+def batch_leg(eng, count, order, steps):
+    from cumetis_amd import _lib as L
+    from cumetis_amd.engine import concat_graphs
+    progress(f"batch leg: generating {count} graphs")
+    gs, ks = batch_graphs(count)
+    cg, voff, eoff = concat_graphs(gs)
+    progress(f"batch leg: sum n={cg.n} sum m={cg.m}")
+    view, keep = eng.device_graph(cg)
+    nb = len(gs)
+    P = (L.CoarsenParams * nb)()
+    for b in range(nb):
+        P[b] = L.CoarsenParams(ks[b], 0, 0, order, 0, 0, 1000 + b, L.CM_FLAG_TRUSTED_INPUT, 0)
+    vo, eo = np.ascontiguousarray(voff, np.int64), np.ascontiguousarray(eoff, np.int64)
+    H = (C.c_void_p * nb)()
+
+    def batched():
+        eng._check(eng.lib.cm_coarsen_batch(eng.ctx, nb, C.byref(view),
+                                            vo.ctypes.data_as(C.POINTER(C.c_int64)),
+                                            eo.ctypes.data_as(C.POINTER(C.c_int64)), P, H))
+        eng.sync()
+        return [H[b] for b in range(nb)]
+
+    def release(hs):
+        for h in hs:
+            eng.lib.cm_hier_release(eng.ctx, h)
+
+    release(batched())
+    bt = []
+    for _ in range(steps):
+        t = time.perf_counter()
+        hs = batched()
+        bt.append(time.perf_counter() - t)
+        if _ < steps - 1:
+            release(hs)
+    bshape = [hier_shape(eng, h) for h in hs]
+    release(hs)
+    progress("batch leg: one by one")
+    # one by one: each graph resident on its own before timing
+    views = [eng.device_graph(g) for g in gs]
+    eng.sync()
+
+    def one_by_one():
+        out = []
+        for b, (v, _) in enumerate(views):
+            h = C.c_void_p()
+            eng._check(eng.lib.cm_coarsen(eng.ctx, C.byref(v), C.byref(P[b]), C.byref(h)))
+            out.append(h)
+        eng.sync()
+        return out
+
+    release(one_by_one())
+    t = time.perf_counter()
+    hs = one_by_one()
+    ot = time.perf_counter() - t
+    oshape = [hier_shape(eng, h) for h in hs]
+    release(hs)
+    # identical hierarchies => identical partitions => equal edge cut
+    same = all([(n, m) for (n, m, _) in a] == [(n, m) for (n, m, _) in b] for a, b in zip(bshape, oshape))
+    med = float(np.median(bt))
+    return {"graphs": nb, "sum_n": int(voff[-1]), "sum_m": int(eoff[-1]),
+            "batched_graphs_per_s": round(nb / med, 1), "batched_ms": round(med * 1e3, 2),
+            "one_by_one_graphs_per_s": round(nb / ot, 1), "one_by_one_ms": round(ot * 1e3, 2),
+            "speedup": round(ot / med, 2), "hierarchies_identical": bool(same),
+            "equal_cut": "by construction: identical hierarchies give identical partitions"}
 
 
 def main():
@@ -71,8 +239,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--config", default="grid725")
     ap.add_argument("--order", default="first_seen", choices=["first_seen", "ascending"])
+    ap.add_argument("--legs", default="grid725", help="comma list of: grid725, rmat22, batch512")
+    ap.add_argument("--batch-count", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
     args = ap.parse_args()
@@ -82,91 +251,42 @@ def main():
         sys.exit(2)
 
     from cumetis_amd import _lib as L
+    from cumetis_amd import graphs
     from cumetis_amd.engine import Engine
 
-    g, k, desc = make_graph(args.config)
     order = L.CM_ROW_FIRST_SEEN if args.order == "first_seen" else L.CM_ROW_ASCENDING
+    legs = [x for x in args.legs.split(",") if x]
     eng = Engine(0)
-    view, keep = eng.device_graph(g)      # input resident in HBM before timing
-    eng.sync()
-
-    import ctypes as C
-    p = L.CoarsenParams(k, 0, 0, order, 0, 0, 1, L.CM_FLAG_TRUSTED_INPUT, 0)
-
-    def step():
-        h = C.c_void_p()
-        eng._check(eng.lib.cm_coarsen(eng.ctx, C.byref(view), C.byref(p), C.byref(h)))
-        return h
-
-    for _ in range(args.warmup):
-        eng.lib.cm_hier_release(eng.ctx, step())
-    eng.sync()
-    times = []
-    hs = None
-    for i in range(args.steps):
-        t = time.perf_counter()
-        h = step()
-        eng.sync()
-        times.append(time.perf_counter() - t)
-        if i == args.steps - 1:
-            hs = h
-        else:
-            eng.lib.cm_hier_release(eng.ctx, h)
-    eng.sync()
-    total = float(np.sum(times))
-    med = float(np.median(times))
-
-    # hierarchy shape -> edges and algorithmic bytes
-    nlev = eng.lib.cm_hier_num_levels(hs)
-    infos = []
-    for i in range(nlev):
-        li = L.LevelInfo()
-        eng._check(eng.lib.cm_hier_level(hs, i, C.byref(li)))
-        infos.append((li.n, li.m, li.rounds))
-    w = 8 if g.xadj.dtype == np.int64 else 4
-    edges = sum(m // 2 for (n, m, r) in infos[:-1])
-    algo = sum(level_bytes(infos[i][0], infos[i][1], infos[i + 1][0], infos[i + 1][1], w)
-               for i in range(nlev - 1))
-    eng.lib.cm_hier_release(eng.ctx, hs)
-
-    # per-kernel hipEvent timing pass (separate from the timed steps)
-    roof = None
-    kstats = {}
-    if not args.no_profile:
-        eng.set_timing(True)
-        eng.reset_stats()
-        for _ in range(3):
-            eng.lib.cm_hier_release(eng.ctx, step())
-        eng.sync()
-        kstats = eng.kernel_stats()
-        eng.set_timing(False)
-        priced = {n: s for n, s in kstats.items() if s["bytes"] > 0}
-        if priced:
-            name, s = max(priced.items(), key=lambda kv: kv[1]["ms"])
-            avg_s = s["ms"] / s["count"] / 1e3
-            bpl = s["bytes"] / s["count"]
-            ach = bpl / avg_s / 1e9
-            roof = {"bound": "hbm", "kernel": name, "achieved": round(ach, 2), "peak": 8000.0,
-                    "unit": "GB/s", "frac": round(ach / 8000.0, 4), "traffic": None,
-                    "avg_launch_us": round(avg_s * 1e6, 2), "algo_bytes_per_launch": int(bpl)}
-
-    cpu = None if args.no_cpu else cpu_baseline(g, k)
-    value = edges / med / 1e6
+    g = graphs.grid2d(725, 725, wmax=100, seed=1)
+    main_out, roof, per = coarsen_leg(eng, g, 64, order, args.warmup, args.steps,
+                                      profile=not args.no_profile)
+    extra = {}
+    if "rmat22" in legs:
+        progress("rmat22: generating")
+        gr = graphs.rmat_fast(22, 16, seed=1, offsets=np.int64)
+        o, rr, pr = coarsen_leg(eng, gr, 256, order, 1, 3, profile=not args.no_profile)
+        o["roofline"] = rr
+        o["kernels_ms_per_step"] = pr
+        o["config"] = "R-MAT s22 ef16 (.57/.19/.19), symmetrised, unit weights, k=256, int64 offsets"
+        extra["rmat22"] = o
+        del gr
+    if "batch512" in legs:
+        extra["batch512"] = batch_leg(eng, args.batch_count, order, 3)
+    progress("cpu baseline")
+    cpu = None if args.no_cpu else cpu_baseline(g, 64)
     out = {
         "metric": "coarsening Medges/s & %HBM peak; batched vs one-by-one graphs/s at equal cut",
-        "value": round(value, 2), "unit": "Medges/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(med * 1e3, 3), "higher_is_better": True,
+        "value": main_out["medges_per_s"], "unit": "Medges/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": main_out["ms"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "int32", "data": "synthetic",
-        "config": {"workload": desc, "levels": nlev, "row_order": args.order,
-                   "n": int(g.n), "m": int(g.m), "coarsest_n": int(infos[-1][0]),
-                   "rounds": [r for (_, _, r) in infos[:-1]]},
-        "hbm_frac_whole_coarsening": round(algo / med / HBM_PEAK, 5),
-        "algo_bytes": int(algo), "input_edges_per_s_M": round((g.m // 2) / med / 1e6, 2),
-        "p10_ms": round(float(np.percentile(times, 10)) * 1e3, 3),
-        "p90_ms": round(float(np.percentile(times, 90)) * 1e3, 3),
-        "roofline": roof, "cpu_baseline": cpu,
-        "kernels_ms_per_step": {n: round(s["ms"] / 3, 4) for n, s in
-                                sorted(kstats.items(), key=lambda kv: -kv[1]["ms"])[:12]},
+        "config": {"workload": "grid2d 725x725 5-point, adjwgt U[1,100], unit vwgt, k=64, int32 CSR",
+                   "row_order": args.order, "n": int(g.n), "m": int(g.m),
+                   "levels": main_out["levels"], "coarsest_n": main_out["coarsest_n"],
+                   "rounds": main_out["rounds"]},
+        "hbm_frac_whole_coarsening": main_out["hbm_frac_whole_coarsening"],
+        "algo_bytes": main_out["algo_bytes"], "input_medges_per_s": main_out["input_medges_per_s"],
+        "p10_ms": main_out["p10_ms"], "p90_ms": main_out["p90_ms"],
+        "roofline": roof, "cpu_baseline": cpu, "kernels_ms_per_step": per, "legs": extra,
     }
     print(json.dumps(out), flush=True)
     eng.close()

@@ -89,6 +89,10 @@ _PROTOS = {
                                 C.c_char_p, C.c_int]),
     "cm_free_host": (None, [_P]),
     "cm_write_partition": (C.c_int, [C.c_char_p, _P, C.c_int64]),
+    "cm_gen_rmat": (C.c_int, [C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_uint64,
+                              C.POINTER(C.c_int64), C.POINTER(_P), C.POINTER(_P), C.POINTER(_P)]),
+    "cm_gen_rgg2d": (C.c_int, [C.c_int64, C.c_double, C.c_int32, C.c_uint64, C.POINTER(_P),
+                               C.POINTER(_P), C.POINTER(_P)]),
     "cm_initial_partition": (C.c_int, [C.c_int64, _P, _P, _P, _P, C.c_int32, _P,
                                        C.POINTER(C.c_int64)]),
 }

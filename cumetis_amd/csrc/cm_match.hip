@@ -247,7 +247,7 @@ static int match_impl(cm_ctx *ctx, const Level &L, const MatchOwners &own, int m
       launch(ctx, "match_propose_long", 0.0, k_propose_long<OffT>, dim3(glong), dim3(256), 0,
              (const int32_t *)list, (const unsigned *)list_n, xadj, L.adjncy, L.adjwgt, L.vwgt,
              own, (const int32_t *)match, prop, (const unsigned *)cnt, r);
-      launch(ctx, "match_accept", 12.0 * n, k_accept, dim3(g), dim3(256), 0, n, match,
+      launch(ctx, "match_accept", r == 0 ? 12.0 * n : 0.0, k_accept, dim3(g), dim3(256), 0, n, match,
              (const int32_t *)prop, cnt, r);
     }
     const int nr = r - r0;

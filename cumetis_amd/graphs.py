@@ -128,6 +128,46 @@ def rmat(scale: int, edge_factor: int = 16, a: float = 0.57, b: float = 0.19, c:
     return Graph(xadj.astype(offsets), dst, None, None)
 
 
+def _take_csr(lib, n, px, pa, pw, offsets, weighted):
+    import ctypes as C
+    m = C.cast(px.value + 8 * n, C.POINTER(C.c_int64))[0]
+    xadj = np.ctypeslib.as_array(C.cast(px.value, C.POINTER(C.c_int64)), shape=(n + 1,)).copy()
+    adj = np.ctypeslib.as_array(C.cast(pa.value, C.POINTER(C.c_int32)), shape=(max(m, 1),))[:m].copy()
+    wgt = np.ctypeslib.as_array(C.cast(pw.value, C.POINTER(C.c_int32)), shape=(max(m, 1),))[:m].copy()
+    for p in (px, pa, pw):
+        lib.cm_free_host(p)
+    return Graph(xadj.astype(offsets), adj, wgt if weighted else None, None)
+
+
+def rmat_fast(scale: int, edge_factor: int = 16, a: float = 0.57, b: float = 0.19, c: float = 0.19,
+              seed: int = 1, offsets=np.int64) -> Graph:
+    """R-MAT from the engine library's multithreaded generator (a different
+    seeded instance than ``rmat``, same distribution)."""
+    import ctypes as C
+    from . import _lib
+    lib = _lib.load()
+    n = C.c_int64()
+    px, pa, pw = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    rc = lib.cm_gen_rmat(scale, edge_factor, a, b, c, seed, C.byref(n), C.byref(px), C.byref(pa),
+                         C.byref(pw))
+    if rc != 0:
+        raise RuntimeError(f"cm_gen_rmat failed ({rc})")
+    return _take_csr(lib, n.value, px, pa, pw, offsets, False)
+
+
+def rgg2d_fast(n: int, mean_degree: float = 8.0, wmax: int = 0, seed: int = 1,
+               offsets=np.int32) -> Graph:
+    """2D random geometric graph from the engine library's generator."""
+    import ctypes as C
+    from . import _lib
+    lib = _lib.load()
+    px, pa, pw = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    rc = lib.cm_gen_rgg2d(n, mean_degree, wmax, seed, C.byref(px), C.byref(pa), C.byref(pw))
+    if rc != 0:
+        raise RuntimeError(f"cm_gen_rgg2d failed ({rc})")
+    return _take_csr(lib, n, px, pa, pw, offsets, wmax > 0)
+
+
 # ------------------------------------------------------------- METIS files
 def read_metis(path: str) -> Graph:
     """METIS .graph reader (format of cu:3712-3851: header n m [fmt], 1-based

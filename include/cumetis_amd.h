@@ -212,6 +212,18 @@ int cm_write_partition(const char *path, const int32_t *part, int64_t n);
 int cm_initial_partition(int64_t n, const int64_t *xadj, const int32_t *adjncy, const int32_t *adjwgt,
                          const int32_t *vwgt, int32_t nparts, int32_t *where, int64_t *objval);
 
+/* Seeded benchmark graphs (host, multithreaded; arrays malloc'd, release
+ * with cm_free_host): symmetric CSR, rows ascending, no self loops or
+ * duplicates, weights per undirected edge mirrored.
+ *   R-MAT: 2^scale vertices, edge_factor * 2^scale generated edges with
+ *          quadrant probabilities a, b, c (d = 1-a-b-c), unit weights.
+ *   RGG:   n uniform points in the unit square, radius sqrt(d/(pi n)) for
+ *          mean degree d, weights uniform in [1, wmax] (1 if wmax <= 0). */
+int cm_gen_rmat(int32_t scale, int32_t edge_factor, double a, double b, double c, uint64_t seed,
+                int64_t *n, int64_t **xadj, int32_t **adjncy, int32_t **adjwgt);
+int cm_gen_rgg2d(int64_t n, double mean_degree, int32_t wmax, uint64_t seed, int64_t **xadj,
+                 int32_t **adjncy, int32_t **adjwgt);
+
 #ifdef __cplusplus
 }
 #endif
