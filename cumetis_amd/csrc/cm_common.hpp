@@ -41,13 +41,14 @@ __device__ __forceinline__ int mask_rank(uint64_t mask)
 }
 
 // ------------------------------------------------------ decoupled look-back
-// One 64-bit status word per tile: bits 62..63 state (0 = not ready,
-// 1 = tile aggregate, 2 = inclusive prefix), bits 0..61 the value.  The value
-// travels in the same atomic word as its state, so relaxed agent-scope atomics
-// suffice and no cache maintenance is needed across XCDs.
-constexpr uint64_t ST_AGG = 1ull << 62;
-constexpr uint64_t ST_INC = 2ull << 62;
-constexpr uint64_t ST_VAL = (1ull << 62) - 1;
+// One 64-bit status word per tile: bits 62..63 state (1 = tile aggregate,
+// 2 = inclusive prefix), bits 42..61 the launch epoch, bits 0..41 the value.
+// The value travels in the same atomic word as its state, so relaxed
+// agent-scope atomics suffice across XCD L2s; the epoch makes words left by
+// earlier launches read as "not ready", so the array never needs clearing.
+constexpr int ST_EPOCH_SHIFT = 42;
+constexpr uint64_t ST_EPOCH_MASK = (1ull << 20) - 1;
+constexpr uint64_t ST_VAL = (1ull << ST_EPOCH_SHIFT) - 1;
 
 __device__ __forceinline__ void st_store(uint64_t *p, uint64_t v)
 {
@@ -57,23 +58,28 @@ __device__ __forceinline__ uint64_t st_load(const uint64_t *p)
 {
   return __hip_atomic_load(const_cast<uint64_t *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+__device__ __forceinline__ uint64_t st_word(uint32_t state, uint32_t epoch, uint64_t v)
+{
+  return ((uint64_t)state << 62) | ((uint64_t)epoch << ST_EPOCH_SHIFT) | (v & ST_VAL);
+}
 
 // Called by all 64 lanes of ONE wave of the tile.  Publishes the tile's
 // aggregate, walks back over predecessors 64 at a time, publishes the
 // inclusive prefix and returns the exclusive prefix (same value in all lanes).
 // Tiles must be numbered in dispatch order (see tile_ticket) so that every
 // predecessor has started and the wait terminates.
-__device__ __forceinline__ uint64_t lookback(uint64_t *status, int tile, uint64_t agg)
+__device__ __forceinline__ uint64_t lookback(uint64_t *status, int tile, uint64_t agg, uint32_t epoch)
 {
   const int l = lane();
-  if (l == 0) st_store(&status[tile], (tile == 0 ? ST_INC : ST_AGG) | agg);
+  if (l == 0) st_store(&status[tile], st_word(tile == 0 ? 2u : 1u, epoch, agg));
   if (tile == 0) return 0;
   uint64_t excl = 0;
   int base = tile - 1;
   for (;;) {
     const int t = base - l;
-    const uint64_t s = t >= 0 ? st_load(&status[t]) : ST_INC;
-    const uint32_t state = (uint32_t)(s >> 62);
+    const uint64_t s = t >= 0 ? st_load(&status[t]) : st_word(2u, epoch, 0);
+    const bool mine = ((s >> ST_EPOCH_SHIFT) & ST_EPOCH_MASK) == epoch;
+    const uint32_t state = mine ? (uint32_t)(s >> 62) : 0u;
     const uint64_t incm = __ballot(state == 2);
     const uint64_t zerom = __ballot(state == 0);
     const int first = incm ? __ffsll((unsigned long long)incm) - 1 : 64;
@@ -83,15 +89,21 @@ __device__ __forceinline__ uint64_t lookback(uint64_t *status, int tile, uint64_
     if (first < 64) break;
     base -= CM_WAVE;
   }
-  if (l == 0) st_store(&status[tile], ST_INC | (excl + agg));
+  if (l == 0) st_store(&status[tile], st_word(2u, epoch, excl + agg));
   return excl;
 }
 
-// Dynamic tile number in dispatch order (one atomic per workgroup).
+// Dynamic tile number in dispatch order (one atomic per workgroup).  The
+// workgroup that draws the last ticket resets the counter for the next
+// launch: every other ticket has been drawn by then.
 __device__ __forceinline__ int tile_ticket(unsigned *ctr)
 {
   __shared__ int s_tile;
-  if (threadIdx.x == 0) s_tile = (int)atomicAdd(ctr, 1u);
+  if (threadIdx.x == 0) {
+    const unsigned t = atomicAdd(ctr, 1u);
+    if (t == gridDim.x - 1) atomicExch(ctr, 0u);
+    s_tile = (int)t;
+  }
   __syncthreads();
   return s_tile;
 }

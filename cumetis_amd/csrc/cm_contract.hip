@@ -20,6 +20,8 @@ namespace cm {
 
 constexpr int GATHER_LONG = 64;   // fine rows longer than this gather by wave
 constexpr int SMALL_SEG = 64;     // coarse segments up to this merge per thread
+constexpr int REG_SEG = 16;       // coarse segments up to this merge in registers
+constexpr int FIRST_SORTED_BIN = 4;  // segsort bins from 17 keys up
 
 // ------------------------------------------------------------- numbering
 template <typename OffT>
@@ -29,7 +31,7 @@ __global__ __launch_bounds__(1024) void k_number(int64_t n, const OffT *__restri
                                                  int32_t *__restrict__ cmap, int32_t *__restrict__ rep,
                                                  OffT *__restrict__ segoff, int32_t *__restrict__ cvwgt,
                                                  uint64_t *st_c, uint64_t *st_l, unsigned *ctr,
-                                                 int64_t *totals)
+                                                 int64_t *totals, uint32_t epoch)
 {
   __shared__ uint64_t sh_c[16], sh_l[16];
   const int tile = tile_ticket(ctr);
@@ -55,8 +57,8 @@ __global__ __launch_bounds__(1024) void k_number(int64_t n, const OffT *__restri
     const uint64_t a = l < 16 ? sh_c[l] : 0, b = l < 16 ? sh_l[l] : 0;
     const uint64_t ia = wave_incl_scan<uint64_t>(a), ib = wave_incl_scan<uint64_t>(b);
     const uint64_t ta = __shfl(ia, 15, CM_WAVE), tb = __shfl(ib, 15, CM_WAVE);
-    const uint64_t ea = lookback(st_c, tile, ta);
-    const uint64_t eb = lookback(st_l, tile, tb);
+    const uint64_t ea = lookback(st_c, tile, ta, epoch);
+    const uint64_t eb = lookback(st_l, tile, tb, epoch);
     if (l < 16) {
       sh_c[l] = ea + ia - a;
       sh_l[l] = eb + ib - b;
@@ -98,12 +100,10 @@ static int64_t number_impl(cm_ctx *ctx, const Level &L, const int32_t *match, in
     return 0;
   }
   const int64_t tiles = (n + 1023) / 1024;
-  ensure_status(ctx, 2 * tiles);
-  fill_u64(ctx, ctx->d_status, 0ull, 2 * tiles);
-  fill_u32(ctx, ctx->d_ctr, 0u, 1);
+  const uint32_t ep = lookback_begin(ctx, 2 * tiles);
   launch(ctx, "number_scan", (double)sizeof(OffT) * (n + 1) * 2 + 16.0 * n, k_number<OffT>,
          dim3((unsigned)tiles), dim3(1024), 0, n, (const OffT *)L.xadj, match, (const int32_t *)L.vwgt,
-         cmap, rep, segoff, cvwgt, ctx->d_status, ctx->d_status + tiles, ctx->d_ctr, ctx->d_small);
+         cmap, rep, segoff, cvwgt, ctx->d_status, ctx->d_status + tiles, ctx->d_ctr, ctx->d_small, ep);
   launch(ctx, "number_fill", 12.0 * n, k_number_fill, dim3(nblocks(n, 256)), dim3(256), 0, n, match,
          cmap);
   CM_HIP(hipMemcpyAsync(ctx->h_small, ctx->d_small, sizeof(int64_t), hipMemcpyDeviceToHost,
@@ -124,7 +124,7 @@ int64_t number(cm_ctx *ctx, const Level &L, const int32_t *d_match, int32_t *d_c
 template <typename OffT>
 __global__ __launch_bounds__(1024) void k_scan_excl(int64_t n, const int32_t *__restrict__ in,
                                                     OffT *__restrict__ out, uint64_t *st,
-                                                    unsigned *ctr, int64_t *total)
+                                                    unsigned *ctr, int64_t *total, uint32_t epoch)
 {
   __shared__ uint64_t sh[16];
   const int tile = tile_ticket(ctr);
@@ -138,7 +138,7 @@ __global__ __launch_bounds__(1024) void k_scan_excl(int64_t n, const int32_t *__
     const uint64_t a = l < 16 ? sh[l] : 0;
     const uint64_t ia = wave_incl_scan<uint64_t>(a);
     const uint64_t ta = __shfl(ia, 15, CM_WAVE);
-    const uint64_t ea = lookback(st, tile, ta);
+    const uint64_t ea = lookback(st, tile, ta, epoch);
     if (l < 16) sh[l] = ea + ia - a;
   }
   __syncthreads();
@@ -160,11 +160,9 @@ static int64_t scan_excl(cm_ctx *ctx, int64_t n, const int32_t *in, OffT *out)
     return 0;
   }
   const int64_t tiles = (n + 1023) / 1024;
-  ensure_status(ctx, tiles);
-  fill_u64(ctx, ctx->d_status, 0ull, tiles);
-  fill_u32(ctx, ctx->d_ctr, 0u, 1);
+  const uint32_t ep = lookback_begin(ctx, tiles);
   launch(ctx, "scan_cxadj", 4.0 * n + sizeof(OffT) * (n + 1), k_scan_excl<OffT>, dim3((unsigned)tiles),
-         dim3(1024), 0, n, in, out, ctx->d_status, ctx->d_ctr, ctx->d_small + 1);
+         dim3(1024), 0, n, in, out, ctx->d_status, ctx->d_ctr, ctx->d_small + 1, ep);
   CM_HIP(hipMemcpyAsync(ctx->h_small + 1, ctx->d_small + 1, sizeof(int64_t), hipMemcpyDeviceToHost,
                         ctx->stream));
   sync(ctx);
@@ -180,11 +178,9 @@ void scan_excl_i32(cm_ctx *ctx, int64_t n, const int32_t *in, int32_t *out)
     return;
   }
   const int64_t tiles = (n + 1023) / 1024;
-  ensure_status(ctx, tiles);
-  fill_u64(ctx, ctx->d_status, 0ull, tiles);
-  fill_u32(ctx, ctx->d_ctr, 0u, 1);
+  const uint32_t ep = lookback_begin(ctx, tiles);
   launch(ctx, "scan_bins", 0.0, k_scan_excl<int32_t>, dim3((unsigned)tiles), dim3(1024), 0, n, in, out,
-         ctx->d_status, ctx->d_ctr, ctx->d_small + 2);
+         ctx->d_status, ctx->d_ctr, ctx->d_small + 2, ep);
 }
 
 // ------------------------------------------------------------------ gather
@@ -205,7 +201,9 @@ __global__ __launch_bounds__(256) void k_gather(int64_t n, const OffT *__restric
   const OffT b = xadj[i], e = xadj[i + 1];
   if ((int64_t)(e - b) > GATHER_LONG) return;
   const int32_t u = match[i];
-  const int64_t s0 = (int64_t)segoff[cmap[i]];
+  const int32_t c = cmap[i];
+  const int64_t s0 = (int64_t)segoff[c];
+  if ((int64_t)segoff[c + 1] - s0 <= REG_SEG) return;  // merged in registers by k_contract_small
   const uint32_t local = u < i ? (uint32_t)(xadj[u + 1] - xadj[u]) : 0u;
   for (OffT j = b; j < e; ++j) {
     const uint32_t pos = local + (uint32_t)(j - b);
@@ -267,18 +265,19 @@ __device__ __forceinline__ uint32_t khi(uint64_t k) { return (uint32_t)(k >> 32)
 __device__ __forceinline__ uint32_t klo(uint64_t k) { return (uint32_t)k; }
 
 template <bool FS, typename OffT>
-__global__ __launch_bounds__(256) void k_count_small(int64_t nc, const OffT *__restrict__ segoff,
+__global__ __launch_bounds__(256) void k_count_small(const int32_t *__restrict__ list, int64_t nlist,
+                                                     const OffT *__restrict__ segoff,
                                                      const uint64_t *__restrict__ keys,
                                                      int32_t *__restrict__ rank,
                                                      int32_t *__restrict__ cdeg,
                                                      int32_t *__restrict__ selfpos,
                                                      int32_t *__restrict__ selfrank)
 {
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= nc) return;
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nlist) return;
+  const int64_t c = list[q];
   const int64_t s0 = (int64_t)segoff[c], s1 = (int64_t)segoff[c + 1];
   const int len = (int)(s1 - s0);
-  if (len > SMALL_SEG) return;
   if (FS)
     for (int p = 0; p < len; ++p) rank[s0 + p] = -1;
   int cnt = 0, ps = -1;
@@ -379,7 +378,8 @@ __device__ __forceinline__ int64_t final_slot(int r, uint32_t p, int ps, int sr,
 }
 
 template <bool FS, typename OffT>
-__global__ __launch_bounds__(256) void k_emit_small(int64_t nc, const OffT *__restrict__ segoff,
+__global__ __launch_bounds__(256) void k_emit_small(const int32_t *__restrict__ list, int64_t nlist,
+                                                    const OffT *__restrict__ segoff,
                                                     const uint64_t *__restrict__ keys,
                                                     const uint32_t *__restrict__ vals,
                                                     const int32_t *__restrict__ rank,
@@ -389,10 +389,10 @@ __global__ __launch_bounds__(256) void k_emit_small(int64_t nc, const OffT *__re
                                                     int32_t *__restrict__ cadjncy,
                                                     int32_t *__restrict__ cadjwgt)
 {
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= nc) return;
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nlist) return;
+  const int64_t c = list[q];
   const int64_t s0 = (int64_t)segoff[c], s1 = (int64_t)segoff[c + 1];
-  if (s1 - s0 > SMALL_SEG) return;
   const int64_t out = (int64_t)cxadj[c], deg = (int64_t)cxadj[c + 1] - out;
   const int ps = FS ? selfpos[c] : -1, sr = FS ? selfrank[c] : -1;
   int r = 0;
@@ -461,10 +461,173 @@ __global__ __launch_bounds__(256) void k_emit_large(const int32_t *__restrict__ 
   }
 }
 
+// ------------------------------------------ short segments: fused, in registers
+// Bitonic network over N register entries (static indices once unrolled).
+template <int N>
+__device__ __forceinline__ void reg_bitonic(uint64_t (&k)[N], uint64_t (&p)[N])
+{
+#pragma unroll
+  for (int sz = 2; sz <= N; sz <<= 1) {
+#pragma unroll
+    for (int j = sz >> 1; j > 0; j >>= 1) {
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        const int l = i ^ j;
+        if (l > i) {
+          const bool up = (i & sz) == 0;
+          if (up ? (k[i] > k[l]) : (k[i] < k[l])) {
+            const uint64_t tk = k[i];
+            k[i] = k[l];
+            k[l] = tk;
+            const uint64_t tp = p[i];
+            p[i] = p[l];
+            p[l] = tp;
+          }
+        }
+      }
+    }
+  }
+}
+
+// Merge the segment of coarse vertex c (row(v) then row(u), at most N
+// entries) entirely in registers.  On return out_key/out_w hold the row in
+// its final order (ascending, or first-seen with the self entry replaced by
+// the last one) and the count of entries.
+template <int N, bool FS, typename OffT>
+__device__ __forceinline__ int merge_regs(int32_t c, int32_t v, int32_t u, const OffT *__restrict__ xadj,
+                                          const int32_t *__restrict__ adjncy,
+                                          const int32_t *__restrict__ adjwgt,
+                                          const int32_t *__restrict__ cmap, uint64_t (&k)[N],
+                                          uint64_t (&p)[N])
+{
+  const int64_t b0 = (int64_t)xadj[v];
+  const int d0 = (int)((int64_t)xadj[v + 1] - b0);
+  const int64_t b1 = u != v ? (int64_t)xadj[u] : 0;
+  const int d1 = u != v ? (int)((int64_t)xadj[u + 1] - b1) : 0;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const bool valid = i < d0 + d1;
+    const int64_t j = i < d0 ? b0 + i : b1 + (i - d0);
+    k[i] = valid ? (((uint64_t)(uint32_t)cmap[adjncy[j]] << 32) | (uint32_t)i) : ~0ull;
+    p[i] = valid ? (uint64_t)(uint32_t)adjwgt[j] : 0ull;
+  }
+  reg_bitonic<N>(k, p);
+  // run sums (runs are adjacent after the sort), heads = first of a run
+  uint64_t sum[N];
+#pragma unroll
+  for (int i = N - 1; i >= 0; --i) {
+    const bool cont = i + 1 < N && (k[i + 1] >> 32) == (k[i] >> 32);
+    sum[i] = p[i] + (cont ? sum[(i + 1) < N ? i + 1 : i] : 0ull);
+  }
+  // order the heads: by first position (first-seen, self included) or by
+  // sorted position without self (ascending); a second network compacts
+  uint64_t k2[N], p2[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const uint32_t key = (uint32_t)(k[i] >> 32);
+    const bool head = k[i] != ~0ull && (i == 0 || (uint32_t)(k[i > 0 ? i - 1 : 0] >> 32) != key);
+    if (FS) k2[i] = head ? (uint64_t)(uint32_t)k[i] : ~0ull;
+    else k2[i] = (head && key != (uint32_t)c) ? (uint64_t)i : ~0ull;
+    p2[i] = ((uint64_t)key << 32) | (uint32_t)sum[i];
+  }
+  reg_bitonic<N>(k2, p2);
+  int len = 0, sidx = -1;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    if (k2[i] != ~0ull) {
+      if (FS && (uint32_t)(p2[i] >> 32) == (uint32_t)c) sidx = i;
+      ++len;
+    }
+  }
+  if (FS && sidx >= 0) {
+    // cu:804-808: the self entry takes the last entry's place
+    uint64_t last = 0;
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+      if (i == len - 1) last = p2[i];
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+      if (i == sidx) p2[i] = last;
+    --len;
+  }
+#pragma unroll
+  for (int i = 0; i < N; ++i) p[i] = p2[i];
+  return len;
+}
+
+// One thread per coarse vertex, tiles of 256 coarse vertices in dispatch
+// order.  Segments of at most REG_SEG entries are merged in registers;
+// longer ones have been merged by the sorted path, which left their degree
+// in cdeg.  A look-back scan of the degrees gives cxadj, and the short rows
+// are written straight to their final place: the fine rows are read once
+// and the coarse rows written once.
+template <bool FS, typename OffT>
+__global__ __launch_bounds__(256) void k_contract_small(int64_t nc, const OffT *__restrict__ segoff,
+                                                        const int32_t *__restrict__ rep,
+                                                        const int32_t *__restrict__ match,
+                                                        const OffT *__restrict__ xadj,
+                                                        const int32_t *__restrict__ adjncy,
+                                                        const int32_t *__restrict__ adjwgt,
+                                                        const int32_t *__restrict__ cmap,
+                                                        const int32_t *__restrict__ cdeg_large,
+                                                        OffT *__restrict__ cxadj, int32_t *__restrict__ cadjncy,
+                                                        int32_t *__restrict__ cadjwgt, uint64_t *st,
+                                                        unsigned *ctr, int64_t *total, uint32_t epoch)
+{
+  __shared__ uint64_t sh[4];
+  __shared__ uint64_t s_excl;
+  const int tile = tile_ticket(ctr);
+  const int64_t c = (int64_t)tile * 256 + threadIdx.x;
+  uint64_t k[REG_SEG], p[REG_SEG];
+  int deg = 0, len = 0;
+  bool small = false;
+  if (c < nc) {
+    len = (int)((int64_t)segoff[c + 1] - (int64_t)segoff[c]);
+    if (len <= REG_SEG) {
+      small = true;
+      const int32_t v = rep[c], u = match[v];
+      if (len <= 8) {
+        uint64_t k8[8], p8[8];
+        deg = merge_regs<8, FS, OffT>((int32_t)c, v, u, xadj, adjncy, adjwgt, cmap, k8, p8);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) p[i] = p8[i];
+      } else {
+        deg = merge_regs<REG_SEG, FS, OffT>((int32_t)c, v, u, xadj, adjncy, adjwgt, cmap, k, p);
+      }
+    } else {
+      deg = cdeg_large[c];
+    }
+  }
+  uint64_t tot;
+  const uint64_t ex = block256_excl_scan<uint64_t>((uint64_t)deg, tot, sh);
+  if (threadIdx.x < CM_WAVE) {
+    const uint64_t e = lookback(st, tile, tot, epoch);
+    if (threadIdx.x == 0) s_excl = e;
+  }
+  __syncthreads();
+  const uint64_t off = s_excl + ex;
+  if (c < nc) {
+    cxadj[c] = (OffT)off;
+    if (c == nc - 1) {
+      cxadj[nc] = (OffT)(off + deg);
+      total[0] = (int64_t)(off + deg);
+    }
+    if (small) {
+#pragma unroll
+      for (int i = 0; i < REG_SEG; ++i) {
+        if (i < deg) {
+          cadjncy[off + i] = (int32_t)(p[i] >> 32);
+          cadjwgt[off + i] = (int32_t)(uint32_t)p[i];
+        }
+      }
+    }
+  }
+}
+
 // ------------------------------------------------------------- host side
 template <typename OffT>
 static int64_t contract_impl(cm_ctx *ctx, const Level &L, const int32_t *match, const int32_t *cmap,
-                             int64_t nc, const OffT *segoff, int row_order, OffT *cxadj,
+                             int64_t nc, const int32_t *rep, const OffT *segoff, int row_order, OffT *cxadj,
                              int32_t *cadjncy, int32_t *cadjwgt)
 {
   const int64_t n = L.n, m = L.m;
@@ -474,74 +637,99 @@ static int64_t contract_impl(cm_ctx *ctx, const Level &L, const int32_t *match, 
     fill_u32(ctx, cxadj, 0u, sizeof(OffT) / 4);
     return 0;
   }
-  uint64_t *keys = (uint64_t *)ctx->pool.alloc(sizeof(uint64_t) * (m > 0 ? m : 1));
-  uint32_t *vals = (uint32_t *)ctx->pool.alloc(sizeof(uint32_t) * (m > 0 ? m : 1));
-  int32_t *list = (int32_t *)ctx->pool.alloc(sizeof(int32_t) * n);
-  int32_t *cdeg = (int32_t *)ctx->pool.alloc(sizeof(int32_t) * nc);
-  int32_t *rank = fs ? (int32_t *)ctx->pool.alloc(sizeof(int32_t) * (m > 0 ? m : 1)) : nullptr;
-  int32_t *selfpos = fs ? (int32_t *)ctx->pool.alloc(sizeof(int32_t) * nc) : nullptr;
-  int32_t *selfrank = fs ? (int32_t *)ctx->pool.alloc(sizeof(int32_t) * nc) : nullptr;
-  unsigned *list_n = ctx->d_ctr + 2;
-  fill_u32(ctx, list_n, 0u, 1);
-  const unsigned gn = nblocks(n, 256), gc = nblocks(nc, 256);
-  const double gather_bytes = (double)sizeof(OffT) * (n + 1) + 8.0 * m + 12.0 * n + 4.0 * m + 12.0 * m;
-  launch(ctx, "contract_select_long", 0.0, k_select_long_rows<OffT>, dim3(gn), dim3(256), 0, n, xadj,
-         list, list_n);
-  launch(ctx, "contract_gather", gather_bytes, k_gather<OffT>, dim3(gn), dim3(256), 0, n, xadj,
-         L.adjncy, L.adjwgt, match, cmap, segoff, keys, vals);
-  launch(ctx, "contract_gather_long", 0.0, k_gather_long<OffT>, dim3(1024), dim3(256), 0,
-         (const int32_t *)list, (const unsigned *)list_n, xadj, L.adjncy, L.adjwgt, match, cmap,
-         segoff, keys, vals);
+  // size classes of the gather segments: only segments above REG_SEG
+  // entries go through the sorted path
   SegBins bins;
   bin_segments(ctx, nc, segoff, sizeof(OffT) == 8, bins);
-  sort_binned(ctx, bins, segoff, sizeof(OffT) == 8, m, keys, vals);
+  const int32_t *mid = bins.list + bins.off[FIRST_SORTED_BIN];
+  const int64_t nmid = bins.off[SEG_LARGE_BIN] - bins.off[FIRST_SORTED_BIN];
   const int32_t *large = bins.list + bins.off[SEG_LARGE_BIN];
   const int64_t nlarge = bins.off[SEG_NBINS] - bins.off[SEG_LARGE_BIN];
+  const bool sorted_path = nmid + nlarge > 0;
+  int32_t *cdeg = (int32_t *)ctx->pool.alloc(sizeof(int32_t) * nc);
+  uint64_t *keys = nullptr;
+  uint32_t *vals = nullptr;
+  int32_t *rank = nullptr, *selfpos = nullptr, *selfrank = nullptr, *list = nullptr;
   const unsigned glarge = (unsigned)std::min<int64_t>(nlarge > 0 ? nlarge : 1, 8192);
-  if (fs) {
-    launch(ctx, "contract_count", 0.0, k_count_small<true, OffT>, dim3(gc), dim3(256), 0, nc, segoff,
-           (const uint64_t *)keys, rank, cdeg, selfpos, selfrank);
-    if (nlarge)
-      launch(ctx, "contract_count_large", 0.0, k_count_large<true, OffT>, dim3(glarge), dim3(256), 0,
-             large, nlarge, segoff, (const uint64_t *)keys, rank, cdeg, selfpos, selfrank);
-  } else {
-    launch(ctx, "contract_count", 0.0, k_count_small<false, OffT>, dim3(gc), dim3(256), 0, nc, segoff,
-           (const uint64_t *)keys, rank, cdeg, selfpos, selfrank);
-    if (nlarge)
-      launch(ctx, "contract_count_large", 0.0, k_count_large<false, OffT>, dim3(glarge), dim3(256), 0,
-             large, nlarge, segoff, (const uint64_t *)keys, rank, cdeg, selfpos, selfrank);
+  if (sorted_path) {
+    keys = (uint64_t *)ctx->pool.alloc(sizeof(uint64_t) * (m > 0 ? m : 1));
+    vals = (uint32_t *)ctx->pool.alloc(sizeof(uint32_t) * (m > 0 ? m : 1));
+    list = (int32_t *)ctx->pool.alloc(sizeof(int32_t) * n);
+    if (fs) {
+      rank = (int32_t *)ctx->pool.alloc(sizeof(int32_t) * (m > 0 ? m : 1));
+      selfpos = (int32_t *)ctx->pool.alloc(sizeof(int32_t) * nc);
+      selfrank = (int32_t *)ctx->pool.alloc(sizeof(int32_t) * nc);
+    }
+    unsigned *list_n = ctx->d_ctr + 2;
+    fill_u32(ctx, list_n, 0u, 1);
+    const unsigned gn = nblocks(n, 256);
+    launch(ctx, "contract_select_long", 0.0, k_select_long_rows<OffT>, dim3(gn), dim3(256), 0, n, xadj,
+           list, list_n);
+    launch(ctx, "contract_gather", 0.0, k_gather<OffT>, dim3(gn), dim3(256), 0, n, xadj, L.adjncy,
+           L.adjwgt, match, cmap, segoff, keys, vals);
+    launch(ctx, "contract_gather_long", 0.0, k_gather_long<OffT>, dim3(1024), dim3(256), 0,
+           (const int32_t *)list, (const unsigned *)list_n, xadj, L.adjncy, L.adjwgt, match, cmap, segoff,
+           keys, vals);
+    sort_binned(ctx, bins, segoff, sizeof(OffT) == 8, m, keys, vals, FIRST_SORTED_BIN);
+#define CM_COUNT(FSV)                                                                                     \
+  if (nmid)                                                                                              \
+    launch(ctx, "contract_count_mid", 0.0, k_count_small<FSV, OffT>, dim3(nblocks(nmid, 256)), dim3(256), \
+           0, mid, nmid, segoff, (const uint64_t *)keys, rank, cdeg, selfpos, selfrank);                  \
+  if (nlarge)                                                                                            \
+    launch(ctx, "contract_count_large", 0.0, k_count_large<FSV, OffT>, dim3(glarge), dim3(256), 0, large, \
+           nlarge, segoff, (const uint64_t *)keys, rank, cdeg, selfpos, selfrank);
+    if (fs) {
+      CM_COUNT(true)
+    } else {
+      CM_COUNT(false)
+    }
+#undef CM_COUNT
   }
-  const int64_t mc = scan_excl<OffT>(ctx, nc, cdeg, cxadj);
-  const double emit_bytes = 12.0 * m + (double)sizeof(OffT) * (nc + 1) + 8.0 * mc;
-  if (fs) {
-    launch(ctx, "contract_emit", emit_bytes, k_emit_small<true, OffT>, dim3(gc), dim3(256), 0, nc,
-           segoff, (const uint64_t *)keys, (const uint32_t *)vals, (const int32_t *)rank,
+  // fused short-segment merge + look-back cxadj + direct row writes
+  const int64_t tiles = (nc + 255) / 256;
+  const uint32_t ep = lookback_begin(ctx, tiles);
+  const double small_bytes = 8.0 * m + 12.0 * n + (double)sizeof(OffT) * (n + 1) + 8.0 * nc +
+                             (double)sizeof(OffT) * (nc + 1);
+  if (fs)
+    launch(ctx, "contract_small", small_bytes, k_contract_small<true, OffT>, dim3((unsigned)tiles), dim3(256),
+           0, nc, segoff, rep, match, xadj, L.adjncy, L.adjwgt, cmap, (const int32_t *)cdeg, cxadj, cadjncy,
+           cadjwgt, ctx->d_status, ctx->d_ctr, ctx->d_small + 1, ep);
+  else
+    launch(ctx, "contract_small", small_bytes, k_contract_small<false, OffT>, dim3((unsigned)tiles),
+           dim3(256), 0, nc, segoff, rep, match, xadj, L.adjncy, L.adjwgt, cmap, (const int32_t *)cdeg, cxadj,
+           cadjncy, cadjwgt, ctx->d_status, ctx->d_ctr, ctx->d_small + 1, ep);
+  if (sorted_path) {
+#define CM_EMIT(FSV)                                                                                      \
+  if (nmid)                                                                                              \
+    launch(ctx, "contract_emit_mid", 0.0, k_emit_small<FSV, OffT>, dim3(nblocks(nmid, 256)), dim3(256), 0, \
+           mid, nmid, segoff, (const uint64_t *)keys, (const uint32_t *)vals, (const int32_t *)rank,       \
+           (const int32_t *)selfpos, (const int32_t *)selfrank, (const OffT *)cxadj, cadjncy, cadjwgt);    \
+  if (nlarge)                                                                                            \
+    launch(ctx, "contract_emit_large", 0.0, k_emit_large<FSV, OffT>, dim3(glarge), dim3(256), 0, large,   \
+           nlarge, segoff, (const uint64_t *)keys, (const uint32_t *)vals, (const int32_t *)rank,          \
            (const int32_t *)selfpos, (const int32_t *)selfrank, (const OffT *)cxadj, cadjncy, cadjwgt);
-    if (nlarge)
-      launch(ctx, "contract_emit_large", 0.0, k_emit_large<true, OffT>, dim3(glarge), dim3(256), 0,
-             large, nlarge, segoff, (const uint64_t *)keys, (const uint32_t *)vals,
-             (const int32_t *)rank, (const int32_t *)selfpos, (const int32_t *)selfrank,
-             (const OffT *)cxadj, cadjncy, cadjwgt);
-  } else {
-    launch(ctx, "contract_emit", emit_bytes, k_emit_small<false, OffT>, dim3(gc), dim3(256), 0, nc,
-           segoff, (const uint64_t *)keys, (const uint32_t *)vals, (const int32_t *)rank,
-           (const int32_t *)selfpos, (const int32_t *)selfrank, (const OffT *)cxadj, cadjncy, cadjwgt);
-    if (nlarge)
-      launch(ctx, "contract_emit_large", 0.0, k_emit_large<false, OffT>, dim3(glarge), dim3(256), 0,
-             large, nlarge, segoff, (const uint64_t *)keys, (const uint32_t *)vals,
-             (const int32_t *)rank, (const int32_t *)selfpos, (const int32_t *)selfrank,
-             (const OffT *)cxadj, cadjncy, cadjwgt);
+    if (fs) {
+      CM_EMIT(true)
+    } else {
+      CM_EMIT(false)
+    }
+#undef CM_EMIT
   }
+  CM_HIP(hipMemcpyAsync(ctx->h_small + 1, ctx->d_small + 1, sizeof(int64_t), hipMemcpyDeviceToHost,
+                        ctx->stream));
   sync(ctx);
+  const int64_t mc = ctx->h_small[1];
   release_bins(ctx, bins);
-  ctx->pool.release(keys);
-  ctx->pool.release(vals);
-  ctx->pool.release(list);
   ctx->pool.release(cdeg);
-  if (fs) {
-    ctx->pool.release(rank);
-    ctx->pool.release(selfpos);
-    ctx->pool.release(selfrank);
+  if (sorted_path) {
+    ctx->pool.release(keys);
+    ctx->pool.release(vals);
+    ctx->pool.release(list);
+    if (fs) {
+      ctx->pool.release(rank);
+      ctx->pool.release(selfpos);
+      ctx->pool.release(selfrank);
+    }
   }
   return mc;
 }
@@ -550,10 +738,9 @@ int64_t contract(cm_ctx *ctx, const Level &L, const int32_t *d_match, const int3
                  int64_t nc, const int32_t *d_rep, const void *d_segoff, int row_order,
                  void *d_cxadj, int32_t *d_cadjncy, int32_t *d_cadjwgt)
 {
-  (void)d_rep;
-  return L.off64 ? contract_impl(ctx, L, d_match, d_cmap, nc, (const int64_t *)d_segoff, row_order,
+  return L.off64 ? contract_impl(ctx, L, d_match, d_cmap, nc, d_rep, (const int64_t *)d_segoff, row_order,
                                  (int64_t *)d_cxadj, d_cadjncy, d_cadjwgt)
-                 : contract_impl(ctx, L, d_match, d_cmap, nc, (const int32_t *)d_segoff, row_order,
+                 : contract_impl(ctx, L, d_match, d_cmap, nc, d_rep, (const int32_t *)d_segoff, row_order,
                                  (int32_t *)d_cxadj, d_cadjncy, d_cadjwgt);
 }
 

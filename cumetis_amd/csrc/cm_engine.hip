@@ -121,6 +121,18 @@ void ensure_status(cm_ctx *ctx, size_t words)
   while (cap < words) cap <<= 1;
   ctx->d_status = (uint64_t *)ctx->pool.alloc(cap * sizeof(uint64_t));
   ctx->status_cap = cap;
+  fill_u64(ctx, ctx->d_status, 0ull, (int64_t)cap);
+  ctx->epoch = 0;
+}
+
+uint32_t lookback_begin(cm_ctx *ctx, size_t words)
+{
+  ensure_status(ctx, words);
+  if (++ctx->epoch > (uint32_t)ST_EPOCH_MASK) {  // wrap: clear and restart
+    fill_u64(ctx, ctx->d_status, 0ull, (int64_t)ctx->status_cap);
+    ctx->epoch = 1;
+  }
+  return ctx->epoch;
 }
 
 void sync(cm_ctx *ctx)

@@ -111,6 +111,7 @@ struct cm_ctx {
   int64_t *h_small = nullptr;     // pinned mirror of d_small
   uint64_t *d_status = nullptr;   // look-back status words
   size_t status_cap = 0;          // in words
+  uint32_t epoch = 0;             // look-back launch epoch (1 .. 2^20-1)
   int64_t last_rounds = 0;
 
   hipEvent_t ev_get();
@@ -123,6 +124,8 @@ constexpr int CTR_SLOTS = 64;
 constexpr int SMALL_SLOTS = 4096;
 
 void ensure_status(cm_ctx *ctx, size_t words);
+// Status words for a look-back launch over `words` tiles; returns the epoch.
+uint32_t lookback_begin(cm_ctx *ctx, size_t words);
 
 template <typename K, typename... Args>
 inline void launch(cm_ctx *ctx, const char *name, double bytes, K kernel, dim3 grid, dim3 block,

@@ -277,12 +277,12 @@ static unsigned cap_grid(int64_t count)
 
 template <typename OffT>
 static void sort_impl(cm_ctx *ctx, const SegBins &b, const OffT *segoff, int64_t total,
-                      uint64_t *keys, uint32_t *vals)
+                      uint64_t *keys, uint32_t *vals, int first_bin)
 {
   const int32_t *L = b.list;
 #define CM_WAVE_BIN(CLS, W)                                                                       \
   {                                                                                               \
-    const int64_t c = b.off[CLS + 1] - b.off[CLS];                                                \
+    const int64_t c = CLS >= first_bin ? b.off[CLS + 1] - b.off[CLS] : 0;                         \
     if (c) launch(ctx, "segsort_wave", 16.0 * c * W, k_sort_wave<W, OffT>,                        \
                   dim3(nblocks(c * W, 256)), dim3(256), 0, L + b.off[CLS], c, segoff, keys, vals); \
   }
@@ -291,7 +291,7 @@ static void sort_impl(cm_ctx *ctx, const SegBins &b, const OffT *segoff, int64_t
 #undef CM_WAVE_BIN
 #define CM_LDS_BIN(CLS, S)                                                                        \
   {                                                                                               \
-    const int64_t c = b.off[CLS + 1] - b.off[CLS];                                                \
+    const int64_t c = CLS >= first_bin ? b.off[CLS + 1] - b.off[CLS] : 0;                         \
     if (c) launch(ctx, "segsort_lds", 0.0, k_sort_lds<S, OffT>, dim3(cap_grid(c)), dim3(256), 0,  \
                   L + b.off[CLS], c, segoff, keys, vals);                                         \
   }
@@ -375,10 +375,10 @@ static void sort_impl(cm_ctx *ctx, const SegBins &b, const OffT *segoff, int64_t
 }
 
 void sort_binned(cm_ctx *ctx, const SegBins &b, const void *segoff, bool off64, int64_t total,
-                 uint64_t *keys, uint32_t *vals)
+                 uint64_t *keys, uint32_t *vals, int first_bin)
 {
-  if (off64) sort_impl(ctx, b, (const int64_t *)segoff, total, keys, vals);
-  else sort_impl(ctx, b, (const int32_t *)segoff, total, keys, vals);
+  if (off64) sort_impl(ctx, b, (const int64_t *)segoff, total, keys, vals, first_bin);
+  else sort_impl(ctx, b, (const int32_t *)segoff, total, keys, vals, first_bin);
 }
 
 void segsort_u64(cm_ctx *ctx, int64_t nseg, const void *d_segoff, bool off64, int64_t total,

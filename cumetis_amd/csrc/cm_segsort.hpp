@@ -28,7 +28,7 @@ struct SegBins {
 void bin_segments(cm_ctx *ctx, int64_t nseg, const void *segoff, bool off64, SegBins &b);
 void release_bins(cm_ctx *ctx, SegBins &b);
 void sort_binned(cm_ctx *ctx, const SegBins &b, const void *segoff, bool off64, int64_t total,
-                 uint64_t *keys, uint32_t *vals);
+                 uint64_t *keys, uint32_t *vals, int first_bin = 0);
 void segsort_i32(cm_ctx *ctx, int32_t *keys, int32_t *vals, int32_t n, const int32_t *segs,
                  int32_t length);
 
