@@ -84,6 +84,14 @@ _PROTOS = {
                                     C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "cm_partition": (C.c_int, [_P, C.POINTER(GraphView), C.POINTER(CoarsenParams), _P,
                                C.POINTER(PartReport)]),
+    "cm_server_create": (C.c_int, [C.c_int, C.c_int64, C.c_int32, C.POINTER(_P)]),
+    "cm_server_submit": (C.c_int, [_P, C.POINTER(GraphView), C.POINTER(CoarsenParams),
+                                   C.POINTER(C.c_uint64)]),
+    "cm_server_wait": (C.c_int, [_P, C.c_uint64, _P, C.POINTER(PartReport)]),
+    "cm_server_stats": (C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_double),
+                                  C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                  C.POINTER(C.c_int32)]),
+    "cm_server_destroy": (C.c_int, [_P]),
     "cm_read_metis": (C.c_int, [C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                 C.POINTER(_P), C.POINTER(_P), C.POINTER(_P), C.POINTER(_P),
                                 C.c_char_p, C.c_int]),

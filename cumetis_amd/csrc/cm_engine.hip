@@ -29,6 +29,7 @@ Pool::~Pool()
 
 void *Pool::alloc(size_t bytes)
 {
+  std::lock_guard<std::recursive_mutex> lk(mu_);
   const size_t c = size_class(bytes ? bytes : 1);
   auto it = free_.find(c);
   void *p = nullptr;
@@ -56,6 +57,7 @@ void *Pool::alloc(size_t bytes)
 void Pool::release(void *p)
 {
   if (!p) return;
+  std::lock_guard<std::recursive_mutex> lk(mu_);
   auto it = live_.find(p);
   if (it == live_.end()) return;
   in_use_ -= it->second;
@@ -65,6 +67,7 @@ void Pool::release(void *p)
 
 void Pool::trim()
 {
+  std::lock_guard<std::recursive_mutex> lk(mu_);
   for (auto &kv : free_) {
     for (void *p : kv.second) {
       (void)hipFree(p);

@@ -7,6 +7,7 @@
 
 #include <map>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -48,6 +49,7 @@ class Pool {
   std::map<size_t, std::vector<void *>> free_;
   std::map<void *, size_t> live_;
   size_t reserved_ = 0, in_use_ = 0;
+  std::recursive_mutex mu_;  // hierarchies may be released from another host thread
 };
 
 struct KStat {
@@ -93,6 +95,7 @@ struct cm_hier {
   std::vector<cm::Level> levels;
   std::vector<void *> owned;  // pool blocks released with the hierarchy
   std::shared_ptr<cm::SharedBlocks> shared;
+  std::vector<std::shared_ptr<cm::SharedBlocks>> shared_more;  // serving: one per union step
 };
 
 struct cm_ctx {
@@ -186,6 +189,11 @@ int64_t contract(cm_ctx *ctx, const Level &L, const int32_t *d_match, const int3
 // segmented sort of 64-bit keys with 32-bit values, segments segoff[0..nseg]
 void segsort_u64(cm_ctx *ctx, int64_t nseg, const void *d_segoff, bool off64, int64_t total,
                  uint64_t *d_keys, uint32_t *d_vals);
+// multilevel partition of an existing hierarchy: initial partition of the
+// coarsest level (host), then per level projection + one refinement pass.
+// part receives level-0 assignments (host).
+void partition_hierarchy(cm_ctx *ctx, const cm_hier *h, int nparts, int32_t *part, double *init_ms,
+                         double *uncoarsen_ms);
 // (d) projection
 void project(cm_ctx *ctx, int64_t n, const int32_t *d_cmap, const int32_t *d_where_c,
              int32_t *d_where_f);

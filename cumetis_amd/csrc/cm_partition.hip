@@ -44,6 +44,10 @@ cmh::HostGraph level_to_host(cm_ctx *ctx, const cm::Level &L)
   }
   if (L.n) CM_HIP(hipMemcpyAsync(g.vwgt.data(), L.vwgt, 4 * L.n, hipMemcpyDeviceToHost, ctx->stream));
   cm::sync(ctx);
+  if (L.ebase)
+    for (auto &x : g.xadj) x -= L.ebase;
+  if (L.vbase)
+    for (auto &a : g.adjncy) a -= (int)L.vbase;
   return g;
 }
 
@@ -62,6 +66,49 @@ int64_t host_cut(const cm_graph_view *g, const int32_t *part)
 
 }  // namespace
 
+namespace cm {
+
+void partition_hierarchy(cm_ctx *ctx, const cm_hier *h, int nparts, int32_t *part, double *init_ms,
+                         double *uncoarsen_ms)
+{
+  const double t1 = now_ms();
+  const int L = (int)h->levels.size();
+  const Level &C = h->levels[L - 1];
+  cmh::HostGraph hc = level_to_host(ctx, C);
+  std::vector<int> where;
+  cmh::initial_partition(hc, nparts, where);
+  const double t2 = now_ms();
+  const int64_t tvwgt = h->levels[0].tvwgt;
+  std::vector<int64_t> pw(nparts, 0);
+  for (int i = 0; i < hc.n; ++i) pw[where[i]] += hc.vwgt[i];
+  int32_t *w0 = (int32_t *)ctx->pool.alloc(4 * (C.n > 0 ? C.n : 1)), *w1 = nullptr;
+  try {
+    if (C.n) CM_HIP(hipMemcpyAsync(w0, where.data(), 4 * C.n, hipMemcpyHostToDevice, ctx->stream));
+    refine_level(ctx, C, nparts, w0, pw, tvwgt);
+    for (int lev = L - 2; lev >= 0; --lev) {
+      const Level &F = h->levels[lev];
+      w1 = (int32_t *)ctx->pool.alloc(4 * (F.n > 0 ? F.n : 1));
+      project(ctx, F.n, F.cmap, w0 - F.cbase, w1);  // stored cmap = local + cbase
+      ctx->pool.release(w0);
+      w0 = w1;
+      w1 = nullptr;
+      refine_level(ctx, F, nparts, w0, pw, tvwgt);
+    }
+    const int64_t n0 = h->levels[0].n;
+    if (n0) CM_HIP(hipMemcpyAsync(part, w0, 4 * n0, hipMemcpyDeviceToHost, ctx->stream));
+    sync(ctx);
+  } catch (...) {
+    ctx->pool.release(w0);
+    if (w1) ctx->pool.release(w1);
+    throw;
+  }
+  ctx->pool.release(w0);
+  if (init_ms) *init_ms = t2 - t1;
+  if (uncoarsen_ms) *uncoarsen_ms = now_ms() - t2;
+}
+
+}  // namespace cm
+
 extern "C" {
 
 int cm_partition(cm_ctx *ctx, const cm_graph_view *g, const cm_coarsen_params *p, int32_t *part,
@@ -69,7 +116,6 @@ int cm_partition(cm_ctx *ctx, const cm_graph_view *g, const cm_coarsen_params *p
 {
   if (!ctx || !g || !p || !part) return CM_ERR_INVALID;
   cm_hier *h = nullptr;
-  int32_t *w0 = nullptr, *w1 = nullptr;
   try {
     CM_REQUIRE(!g->on_device, CM_ERR_INVALID, "cm_partition takes a host graph");
     CM_REQUIRE(p->nparts >= 1, CM_ERR_INVALID, "nparts must be >= 1");
@@ -82,46 +128,21 @@ int cm_partition(cm_ctx *ctx, const cm_graph_view *g, const cm_coarsen_params *p
     int rc = cm_coarsen(ctx, g, p, &h);
     if (rc != CM_OK) return rc;
     const double t1 = now_ms();
-    const int L = (int)h->levels.size();
-    const cm::Level &C = h->levels[L - 1];
-    cmh::HostGraph hc = level_to_host(ctx, C);
-    std::vector<int> where;
-    srand(1);  // the reference draws from an unseeded glibc stream (seed 1)
-    cmh::initial_partition(hc, p->nparts, where);
-    const double t2 = now_ms();
-    const int64_t tvwgt = h->levels[0].tvwgt;
-    std::vector<int64_t> pw(p->nparts, 0);
-    for (int i = 0; i < hc.n; ++i) pw[where[i]] += hc.vwgt[i];
-    w0 = (int32_t *)ctx->pool.alloc(4 * (C.n > 0 ? C.n : 1));
-    if (C.n) CM_HIP(hipMemcpyAsync(w0, where.data(), 4 * C.n, hipMemcpyHostToDevice, ctx->stream));
-    cm::refine_level(ctx, C, p->nparts, w0, pw, tvwgt);
-    for (int lev = L - 2; lev >= 0; --lev) {
-      const cm::Level &F = h->levels[lev];
-      w1 = (int32_t *)ctx->pool.alloc(4 * (F.n > 0 ? F.n : 1));
-      cm::project(ctx, F.n, F.cmap, w0, w1);
-      ctx->pool.release(w0);
-      w0 = w1;
-      w1 = nullptr;
-      cm::refine_level(ctx, F, p->nparts, w0, pw, tvwgt);
-    }
-    if (g->n) CM_HIP(hipMemcpyAsync(part, w0, 4 * g->n, hipMemcpyDeviceToHost, ctx->stream));
-    cm::sync(ctx);
+    double init_ms = 0, unc_ms = 0;
+    cm::partition_hierarchy(ctx, h, p->nparts, part, &init_ms, &unc_ms);
     const double t3 = now_ms();
     if (rep) {
       rep->total_ms = t3 - t0;
       rep->coarsen_ms = t1 - t0;
-      rep->init_ms = t2 - t1;
-      rep->uncoarsen_ms = t3 - t2;
-      rep->levels = L;
-      rep->coarsest_n = (int32_t)C.n;
+      rep->init_ms = init_ms;
+      rep->uncoarsen_ms = unc_ms;
+      rep->levels = (int32_t)h->levels.size();
+      rep->coarsest_n = (int32_t)h->levels.back().n;
       rep->edgecut = host_cut(g, part);
     }
-    ctx->pool.release(w0);
     cm_hier_release(ctx, h);
   } catch (const cm::Error &e) {
     ctx->err = e.what();
-    if (w0) ctx->pool.release(w0);
-    if (w1) ctx->pool.release(w1);
     if (h) cm_hier_release(ctx, h);
     return e.code;
   } catch (const std::exception &e) {

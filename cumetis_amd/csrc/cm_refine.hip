@@ -25,11 +25,14 @@ namespace cm {
 
 constexpr int REFINE_MAX_PARTS = 2048;
 
+// xadj values index adjncy/adjwgt directly; neighbour ids index where_nb
+// (for a batch slice these are the slice pointers shifted back by its bases)
 template <typename OffT>
 __global__ __launch_bounds__(256) void k_refine_candidates(int64_t n, const OffT *__restrict__ xadj,
                                                            const int32_t *__restrict__ adjncy,
                                                            const int32_t *__restrict__ adjwgt,
                                                            const int32_t *__restrict__ where,
+                                                           const int32_t *__restrict__ where_nb,
                                                            int32_t *list, unsigned *count)
 {
   __shared__ unsigned s_cnt, s_base;
@@ -40,7 +43,7 @@ __global__ __launch_bounds__(256) void k_refine_candidates(int64_t n, const OffT
     const int me = where[v];
     int64_t in = 0, ex = 0;
     for (OffT j = xadj[v]; j < xadj[v + 1]; ++j) {
-      if (where[adjncy[j]] == me) in += adjwgt[j];
+      if (where_nb[adjncy[j]] == me) in += adjwgt[j];
       else ex += adjwgt[j];
     }
     cand = ex > in;
@@ -68,6 +71,7 @@ __global__ __launch_bounds__(256) void k_refine_dest(const int32_t *list, const 
                                                      const int32_t *__restrict__ adjncy,
                                                      const int32_t *__restrict__ adjwgt,
                                                      const int32_t *__restrict__ where,
+                                                     const int32_t *__restrict__ where_nb,
                                                      const int32_t *__restrict__ vwgt, int nparts,
                                                      uint64_t *key, uint32_t *idx, int32_t *from,
                                                      int32_t *vw)
@@ -81,7 +85,7 @@ __global__ __launch_bounds__(256) void k_refine_dest(const int32_t *list, const 
   for (int64_t q = (int64_t)blockIdx.x * 4 + w; q < nc; q += nw) {
     const int32_t v = list[q];
     const int me = where[v];
-    for (OffT j = xadj[v] + l; j < xadj[v + 1]; j += CM_WAVE) atomicAdd(&conn[where[adjncy[j]]], adjwgt[j]);
+    for (OffT j = xadj[v] + l; j < xadj[v + 1]; j += CM_WAVE) atomicAdd(&conn[where_nb[adjncy[j]]], adjwgt[j]);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -105,7 +109,7 @@ __global__ __launch_bounds__(256) void k_refine_dest(const int32_t *list, const 
     const int32_t own = conn[me];
     const int to = best > own ? bp : me;
     __builtin_amdgcn_wave_barrier();
-    for (OffT j = xadj[v] + l; j < xadj[v + 1]; j += CM_WAVE) conn[where[adjncy[j]]] = 0;
+    for (OffT j = xadj[v] + l; j < xadj[v + 1]; j += CM_WAVE) conn[where_nb[adjncy[j]]] = 0;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -135,8 +139,10 @@ static int refine_impl(cm_ctx *ctx, const Level &L, int nparts, int32_t *where, 
   int32_t *list = (int32_t *)ctx->pool.alloc(sizeof(int32_t) * n);
   unsigned *count = ctx->d_ctr + 4;
   fill_u32(ctx, count, 0u, 1);
+  const int32_t *adj = L.adjncy - L.ebase, *wgt = L.adjwgt - L.ebase;
+  const int32_t *where_nb = where - L.vbase;
   launch(ctx, "refine_candidates", 0.0, k_refine_candidates<OffT>, dim3(nblocks(n, 256)), dim3(256), 0, n,
-         xadj, L.adjncy, L.adjwgt, (const int32_t *)where, list, count);
+         xadj, adj, wgt, (const int32_t *)where, where_nb, list, count);
   unsigned hc = 0;
   CM_HIP(hipMemcpyAsync(&hc, count, sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
   sync(ctx);
@@ -148,7 +154,7 @@ static int refine_impl(cm_ctx *ctx, const Level &L, int nparts, int32_t *where, 
     int32_t *vw = (int32_t *)ctx->pool.alloc(sizeof(int32_t) * hc);
     const unsigned grid = (unsigned)std::min<int64_t>((hc + 3) / 4, 16384);
     launch(ctx, "refine_dest", 0.0, k_refine_dest<OffT>, dim3(grid), dim3(256), sizeof(int32_t) * 4 * nparts,
-           (const int32_t *)list, (const unsigned *)count, xadj, L.adjncy, L.adjwgt, (const int32_t *)where,
+           (const int32_t *)list, (const unsigned *)count, xadj, adj, wgt, (const int32_t *)where, where_nb,
            (const int32_t *)L.vwgt, nparts, key, idx, from, vw);
     int64_t *segoff = (int64_t *)ctx->pool.alloc(sizeof(int64_t) * 2);
     int64_t hseg[2] = {0, (int64_t)hc};

@@ -1,0 +1,498 @@
+// cm_serve.hip -- continuous-batching partition server.
+//
+// Requests (host graphs) are admitted as slots free up and the HBM footprint
+// budget allows.  One host thread steps coarsening: every step is one level
+// for ALL admitted jobs at once, as one disjoint union (the batched kernels
+// of cm_batch.hip), so a job submitted while others are mid-hierarchy joins
+// the very next step at its level 0 and its work overlaps theirs on the
+// device.  Jobs whose own stop rule ends them move to a second thread (own
+// context and stream) that runs the initial partition and the uncoarsening,
+// again overlapping the coarsening steps.  Every job's hierarchy stays
+// device-resident until its result is produced.  Because the union kernels
+// give each graph exactly its stand-alone result, a job's partition is the
+// same alone and in any mix.
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "cm_common.hpp"
+#include "cm_internal.hpp"
+
+namespace cm {
+
+int ref_threshold(int64_t n, int nparts);
+
+namespace {
+
+double now_ms()
+{
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
+// one slot of a union step: copy a job's current level into the union
+struct SlotSrc {
+  const int64_t *xadj;
+  const int32_t *adj, *wgt, *vw;
+  int64_t sv, se;  // bases of the source values
+  int64_t dv, de;  // first vertex / entry in the union
+};
+
+__device__ __forceinline__ int slot_of(const int64_t *base, int ns, int64_t x)
+{
+  int lo = 0, hi = ns;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (base[mid] <= x) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(256) void k_union_vertices(int64_t n, int ns, const SlotSrc *src,
+                                                        const int64_t *dv, int64_t m_total, int64_t *xadj,
+                                                        int32_t *vw, int32_t *gid)
+{
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v > n) return;
+  if (v == n) {
+    xadj[n] = m_total;
+    return;
+  }
+  const int s = slot_of(dv, ns, v);
+  const SlotSrc q = src[s];
+  const int64_t i = v - q.dv;
+  xadj[v] = q.xadj[i] - q.se + q.de;
+  vw[v] = q.vw[i];
+  gid[v] = s;
+}
+
+__global__ __launch_bounds__(256) void k_union_edges(int64_t m, int ns, const SlotSrc *src, const int64_t *de,
+                                                     int32_t *adj, int32_t *wgt)
+{
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= m) return;
+  const int s = slot_of(de, ns, e);
+  const SlotSrc q = src[s];
+  const int64_t j = e - q.de;
+  adj[e] = (int32_t)(q.adj[j] - q.sv + q.dv);
+  wgt[e] = q.wgt[j];
+}
+
+template <typename OffT>
+__global__ void k_union_bases(int ns, const int64_t *vbase, const int32_t *cmap, const OffT *cxadj, int64_t nc,
+                              int64_t *cvb, int64_t *ceb)
+{
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s > ns) return;
+  const int64_t c = s == ns ? nc : (int64_t)cmap[vbase[s]];
+  cvb[s] = c;
+  ceb[s] = (int64_t)cxadj[c];
+}
+
+}  // namespace
+
+struct Job {
+  uint64_t id = 0;
+  int64_t n = 0, m = 0;
+  std::vector<int64_t> xadj;
+  std::vector<int32_t> adjncy, adjwgt, vwgt;
+  cm_coarsen_params p{};
+  int T = 0;
+  int32_t maxv = 0;
+  int64_t tvwgt = 0;
+  size_t footprint = 0;
+  double t_submit = 0, t_done = 0;
+  cm_hier *h = nullptr;
+  int64_t cur_n = 0;  // vertices at the current level
+  std::vector<int32_t> part;
+  cm_part_report rep{};
+  int status = CM_OK;
+  bool done = false;
+};
+
+}  // namespace cm
+
+struct cm_server {
+  cm_ctx *cctx = nullptr, *fctx = nullptr;
+  std::thread loop, finisher;
+  std::mutex mu;
+  std::condition_variable cv_loop, cv_fin, cv_done;
+  std::deque<cm::Job *> pending, to_finish;
+  std::vector<cm::Job *> active;
+  std::map<uint64_t, std::unique_ptr<cm::Job>> jobs;
+  uint64_t next_id = 1;
+  size_t budget = 0, used = 0;
+  int max_slots = 0;
+  int max_active = 0;
+  bool stop = false;
+  std::vector<double> latencies;
+  double t_first = 0, t_last = 0;
+  int64_t completed = 0;
+};
+
+namespace cm {
+
+static size_t job_footprint(int64_t n, int64_t m)
+{
+  // level 0 + coarser levels (about as much again) + one union copy + the
+  // contraction scratch of its share of a step
+  return (size_t)(64 * m + 64 * n + 4096);
+}
+
+// One coarsening step over the union of the active jobs' current levels.
+static void union_step(cm_server *S, std::vector<Job *> &act)
+{
+  cm_ctx *ctx = S->cctx;
+  const int ns = (int)act.size();
+  auto shared = std::make_shared<SharedBlocks>();
+  shared->ctx = ctx;
+  auto own = [&](size_t bytes) {
+    void *p = ctx->pool.alloc(bytes ? bytes : 1);
+    shared->blocks.push_back(p);
+    return p;
+  };
+  std::vector<void *> tmp_blocks;
+  auto tmp = [&](size_t bytes) {
+    void *p = ctx->pool.alloc(bytes ? bytes : 1);
+    tmp_blocks.push_back(p);
+    return p;
+  };
+  std::vector<SlotSrc> src(ns);
+  std::vector<int64_t> dv(ns + 1), de(ns + 1);
+  std::vector<int32_t> hmaxv(ns);
+  std::vector<uint64_t> hseed(ns);
+  int64_t n = 0, m = 0;
+  for (int s = 0; s < ns; ++s) {
+    const Level &L = act[s]->h->levels.back();
+    src[s] = SlotSrc{(const int64_t *)L.xadj, L.adjncy, L.adjwgt, L.vwgt, L.vbase, L.ebase, n, m};
+    dv[s] = n;
+    de[s] = m;
+    n += L.n;
+    m += L.m;
+    hmaxv[s] = act[s]->maxv;
+    hseed[s] = act[s]->p.seed;
+  }
+  dv[ns] = n;
+  de[ns] = m;
+  SlotSrc *d_src = (SlotSrc *)tmp(sizeof(SlotSrc) * ns);
+  int64_t *d_dv = (int64_t *)tmp(8 * (ns + 1)), *d_de = (int64_t *)tmp(8 * (ns + 1));
+  int32_t *d_maxv = (int32_t *)tmp(4 * ns);
+  uint64_t *d_seed = (uint64_t *)tmp(8 * ns);
+  CM_HIP(hipMemcpyAsync(d_src, src.data(), sizeof(SlotSrc) * ns, hipMemcpyHostToDevice, ctx->stream));
+  CM_HIP(hipMemcpyAsync(d_dv, dv.data(), 8 * (ns + 1), hipMemcpyHostToDevice, ctx->stream));
+  CM_HIP(hipMemcpyAsync(d_de, de.data(), 8 * (ns + 1), hipMemcpyHostToDevice, ctx->stream));
+  CM_HIP(hipMemcpyAsync(d_maxv, hmaxv.data(), 4 * ns, hipMemcpyHostToDevice, ctx->stream));
+  CM_HIP(hipMemcpyAsync(d_seed, hseed.data(), 8 * ns, hipMemcpyHostToDevice, ctx->stream));
+  int64_t *xadj = (int64_t *)own(8 * (n + 1));
+  int32_t *adj = (int32_t *)own(4 * (m > 0 ? m : 1)), *wgt = (int32_t *)own(4 * (m > 0 ? m : 1));
+  int32_t *vw = (int32_t *)own(4 * (n > 0 ? n : 1)), *gid = (int32_t *)tmp(4 * (n > 0 ? n : 1));
+  launch(ctx, "serve_union_v", 0.0, k_union_vertices, dim3(nblocks(n + 1, 256)), dim3(256), 0, n, ns,
+         (const SlotSrc *)d_src, (const int64_t *)d_dv, m, xadj, vw, gid);
+  launch(ctx, "serve_union_e", 0.0, k_union_edges, dim3(nblocks(m, 256)), dim3(256), 0, m, ns,
+         (const SlotSrc *)d_src, (const int64_t *)d_de, adj, wgt);
+  Level U;
+  U.n = n;
+  U.m = m;
+  U.off64 = true;
+  U.xadj = xadj;
+  U.adjncy = adj;
+  U.adjwgt = wgt;
+  U.vwgt = vw;
+  int32_t *match = (int32_t *)own(4 * (n > 0 ? n : 1)), *cmap = (int32_t *)own(4 * (n > 0 ? n : 1));
+  MatchOwners mo;
+  mo.gid = gid;
+  mo.maxv = d_maxv;
+  mo.seed = d_seed;
+  mo.vbase = d_dv;
+  const int rounds = match_handshake(ctx, U, mo, 0, match);
+  int32_t *rep = (int32_t *)tmp(4 * (n > 0 ? n : 1));
+  int64_t *segoff = (int64_t *)tmp(8 * (n + 1));
+  int32_t *cvw = (int32_t *)own(4 * (n > 0 ? n : 1));
+  const int64_t nc = number(ctx, U, match, cmap, rep, segoff, cvw);
+  int64_t *cxadj = (int64_t *)own(8 * (nc + 1));
+  int32_t *cadj = (int32_t *)own(4 * (m > 0 ? m : 1)), *cwgt = (int32_t *)own(4 * (m > 0 ? m : 1));
+  contract(ctx, U, match, cmap, nc, rep, segoff, act[0]->p.row_order, cxadj, cadj, cwgt);
+  int64_t *d_cvb = (int64_t *)tmp(8 * (ns + 1)), *d_ceb = (int64_t *)tmp(8 * (ns + 1));
+  launch(ctx, "serve_bases", 0.0, k_union_bases<int64_t>, dim3(nblocks(ns + 1, 256)), dim3(256), 0, ns,
+         (const int64_t *)d_dv, (const int32_t *)cmap, (const int64_t *)cxadj, nc, d_cvb, d_ceb);
+  std::vector<int64_t> cvb(ns + 1), ceb(ns + 1);
+  CM_HIP(hipMemcpyAsync(cvb.data(), d_cvb, 8 * (ns + 1), hipMemcpyDeviceToHost, ctx->stream));
+  CM_HIP(hipMemcpyAsync(ceb.data(), d_ceb, 8 * (ns + 1), hipMemcpyDeviceToHost, ctx->stream));
+  sync(ctx);
+  for (void *q : tmp_blocks) ctx->pool.release(q);
+  for (int s = 0; s < ns; ++s) {
+    cm_hier *h = act[s]->h;
+    Level &F = h->levels.back();
+    // the level now lives in the union copy, next to its match and cmap
+    F.xadj = xadj + dv[s];
+    F.adjncy = adj + de[s];
+    F.adjwgt = wgt + de[s];
+    F.vwgt = vw + dv[s];
+    F.vbase = dv[s];
+    F.ebase = de[s];
+    F.match = match + dv[s];
+    F.cmap = cmap + dv[s];
+    F.cbase = cvb[s];
+    F.rounds = rounds;
+    Level C;
+    C.n = cvb[s + 1] - cvb[s];
+    C.m = ceb[s + 1] - ceb[s];
+    C.off64 = true;
+    C.xadj = cxadj + cvb[s];
+    C.adjncy = cadj + ceb[s];
+    C.adjwgt = cwgt + ceb[s];
+    C.vwgt = cvw + cvb[s];
+    C.tvwgt = F.tvwgt;
+    C.vbase = cvb[s];
+    C.ebase = ceb[s];
+    h->levels.push_back(C);
+    h->shared_more.push_back(shared);
+  }
+}
+
+static void loop_main(cm_server *S)
+{
+  hipSetDevice(S->cctx->device);
+  for (;;) {
+    std::vector<Job *> act, fresh;
+    {
+      std::unique_lock<std::mutex> lk(S->mu);
+      S->cv_loop.wait(lk, [&] { return S->stop || !S->pending.empty() || !S->active.empty(); });
+      if (S->stop) return;
+      // admission: free slots and HBM budget (one job always fits when idle)
+      while (!S->pending.empty() && (int)S->active.size() < S->max_slots) {
+        Job *j = S->pending.front();
+        if (!S->active.empty() && S->used + j->footprint > S->budget) break;
+        S->pending.pop_front();
+        S->used += j->footprint;
+        S->active.push_back(j);
+        fresh.push_back(j);
+      }
+      S->max_active = std::max(S->max_active, (int)S->active.size());
+      act = S->active;
+    }
+    std::vector<Job *> retire;
+    try {
+      for (Job *j : fresh) {  // level 0 on the device
+        cm_ctx *ctx = S->cctx;
+        j->h = new cm_hier();
+        Level L0;
+        L0.n = j->n;
+        L0.m = j->m;
+        L0.off64 = true;
+        L0.xadj = ctx->pool.alloc(8 * (j->n + 1));
+        L0.adjncy = (int32_t *)ctx->pool.alloc(4 * (j->m > 0 ? j->m : 1));
+        L0.adjwgt = (int32_t *)ctx->pool.alloc(4 * (j->m > 0 ? j->m : 1));
+        L0.vwgt = (int32_t *)ctx->pool.alloc(4 * (j->n > 0 ? j->n : 1));
+        j->h->owned = {L0.xadj, L0.adjncy, L0.adjwgt, L0.vwgt};
+        CM_HIP(hipMemcpyAsync(L0.xadj, j->xadj.data(), 8 * (j->n + 1), hipMemcpyHostToDevice, ctx->stream));
+        if (j->m) {
+          CM_HIP(hipMemcpyAsync(L0.adjncy, j->adjncy.data(), 4 * j->m, hipMemcpyHostToDevice, ctx->stream));
+          CM_HIP(hipMemcpyAsync(L0.adjwgt, j->adjwgt.data(), 4 * j->m, hipMemcpyHostToDevice, ctx->stream));
+        }
+        if (j->n) CM_HIP(hipMemcpyAsync(L0.vwgt, j->vwgt.data(), 4 * j->n, hipMemcpyHostToDevice, ctx->stream));
+        L0.tvwgt = j->tvwgt;
+        j->h->levels.push_back(L0);
+        j->cur_n = j->n;
+      }
+      std::vector<Job *> work;
+      for (Job *j : act)
+        if (j->n > 0) work.push_back(j);
+        else retire.push_back(j);
+      if (!work.empty()) union_step(S, work);
+      for (Job *j : work) {
+        const Level &C = j->h->levels.back();
+        const int64_t nf = j->cur_n;
+        j->cur_n = C.n;
+        const bool more = C.n > j->T && C.n < 0.75 * nf && C.m > C.n / 2;  // cu:1843-1845
+        const bool cap = j->p.max_levels > 0 && (int)j->h->levels.size() > j->p.max_levels;
+        if (!more || cap) retire.push_back(j);
+      }
+    } catch (const Error &e) {
+      for (Job *j : act) j->status = e.code;
+      retire = act;
+    }
+    std::lock_guard<std::mutex> lk(S->mu);
+    for (Job *j : retire) {
+      S->active.erase(std::find(S->active.begin(), S->active.end(), j));
+      S->to_finish.push_back(j);
+    }
+    if (!retire.empty()) S->cv_fin.notify_one();
+  }
+}
+
+static void finish_main(cm_server *S)
+{
+  hipSetDevice(S->fctx->device);
+  for (;;) {
+    Job *j = nullptr;
+    {
+      std::unique_lock<std::mutex> lk(S->mu);
+      S->cv_fin.wait(lk, [&] { return S->stop || !S->to_finish.empty(); });
+      if (S->stop && S->to_finish.empty()) return;
+      j = S->to_finish.front();
+      S->to_finish.pop_front();
+    }
+    if (j->status == CM_OK) {
+      try {
+        j->part.assign(j->n, 0);
+        const double t0 = now_ms();
+        if (j->p.nparts > 1 && j->n > 0)
+          partition_hierarchy(S->fctx, j->h, j->p.nparts, j->part.data(), &j->rep.init_ms, &j->rep.uncoarsen_ms);
+        j->rep.levels = (int32_t)j->h->levels.size();
+        j->rep.coarsest_n = (int32_t)j->h->levels.back().n;
+        int64_t cut = 0;
+        for (int64_t i = 0; i < j->n; ++i)
+          for (int64_t e = j->xadj[i]; e < j->xadj[i + 1]; ++e)
+            if (j->part[i] != j->part[j->adjncy[e]]) cut += j->adjwgt[e];
+        j->rep.edgecut = cut / 2;
+        j->rep.uncoarsen_ms = now_ms() - t0 - j->rep.init_ms;
+      } catch (const Error &e) {
+        j->status = e.code;
+      }
+    }
+    if (j->h) {
+      for (void *q : j->h->owned) S->cctx->pool.release(q);
+      delete j->h;  // drops its references to the union steps
+      j->h = nullptr;
+    }
+    std::lock_guard<std::mutex> lk(S->mu);
+    j->t_done = now_ms();
+    j->rep.total_ms = j->t_done - j->t_submit;
+    j->done = true;
+    S->used -= j->footprint;
+    S->latencies.push_back(j->rep.total_ms);
+    S->t_last = j->t_done;
+    ++S->completed;
+    S->cv_done.notify_all();
+    S->cv_loop.notify_one();  // budget freed
+  }
+}
+
+}  // namespace cm
+
+extern "C" {
+
+int cm_server_create(int device, int64_t hbm_budget, int32_t max_slots, cm_server **out)
+{
+  if (!out || max_slots < 1) return CM_ERR_INVALID;
+  *out = nullptr;
+  cm_server *S = new cm_server();
+  int rc = cm_ctx_create(device, nullptr, &S->cctx);
+  if (rc == CM_OK) rc = cm_ctx_create(device, nullptr, &S->fctx);
+  if (rc != CM_OK) {
+    if (S->cctx) cm_ctx_destroy(S->cctx);
+    delete S;
+    return rc;
+  }
+  S->budget = hbm_budget > 0 ? (size_t)hbm_budget : (size_t)64 << 30;
+  S->max_slots = max_slots;
+  S->loop = std::thread(cm::loop_main, S);
+  S->finisher = std::thread(cm::finish_main, S);
+  *out = S;
+  return CM_OK;
+}
+
+int cm_server_submit(cm_server *S, const cm_graph_view *g, const cm_coarsen_params *p, uint64_t *ticket)
+{
+  if (!S || !g || !p || !ticket || g->on_device || p->nparts < 1 || g->n < 0) return CM_ERR_INVALID;
+  if (p->row_order != CM_ROW_FIRST_SEEN) return CM_ERR_UNSUPPORTED;  // one row order per union step
+  auto j = std::make_unique<cm::Job>();
+  j->n = g->n;
+  j->m = g->m;
+  j->p = *p;
+  j->xadj.resize(g->n + 1);
+  for (int64_t i = 0; i <= g->n; ++i)
+    j->xadj[i] = g->offset_type == CM_OFF64 ? ((const int64_t *)g->xadj)[i] : ((const int32_t *)g->xadj)[i];
+  if (j->xadj[0] != 0 || j->xadj[g->n] != g->m) return CM_ERR_INPUT;
+  j->adjncy.assign(g->adjncy, g->adjncy + g->m);
+  j->adjwgt = g->adjwgt ? std::vector<int32_t>(g->adjwgt, g->adjwgt + g->m) : std::vector<int32_t>(g->m, 1);
+  j->vwgt = g->vwgt ? std::vector<int32_t>(g->vwgt, g->vwgt + g->n) : std::vector<int32_t>(g->n, 1);
+  for (int64_t i = 0; i < g->n; ++i) {
+    if (j->xadj[i] > j->xadj[i + 1]) return CM_ERR_INPUT;
+    for (int64_t e = j->xadj[i]; e < j->xadj[i + 1]; ++e)
+      if (j->adjncy[e] < 0 || j->adjncy[e] >= g->n || j->adjncy[e] == i || j->adjwgt[e] <= 0) return CM_ERR_INPUT;
+  }
+  for (int32_t w : j->vwgt) j->tvwgt += w;
+  j->T = p->threshold > 0 ? p->threshold : cm::ref_threshold(g->n, p->nparts);
+  j->maxv = p->maxvwgt > 0 ? p->maxvwgt : (int32_t)(1.5 * (double)j->tvwgt / j->T);
+  j->footprint = cm::job_footprint(g->n, g->m);
+  j->t_submit = cm::now_ms();
+  std::lock_guard<std::mutex> lk(S->mu);
+  j->id = S->next_id++;
+  *ticket = j->id;
+  if (S->t_first == 0) S->t_first = j->t_submit;
+  S->pending.push_back(j.get());
+  S->jobs[j->id] = std::move(j);
+  S->cv_loop.notify_one();
+  return CM_OK;
+}
+
+int cm_server_wait(cm_server *S, uint64_t ticket, int32_t *part, cm_part_report *rep)
+{
+  if (!S) return CM_ERR_INVALID;
+  std::unique_lock<std::mutex> lk(S->mu);
+  auto it = S->jobs.find(ticket);
+  if (it == S->jobs.end()) return CM_ERR_INVALID;
+  cm::Job *j = it->second.get();
+  S->cv_done.wait(lk, [&] { return j->done; });
+  const int st = j->status;
+  if (st == CM_OK) {
+    if (part) std::copy(j->part.begin(), j->part.end(), part);
+    if (rep) *rep = j->rep;
+  }
+  S->jobs.erase(it);
+  return st;
+}
+
+int cm_server_stats(cm_server *S, int64_t *completed, double *p50_ms, double *p99_ms, double *graphs_per_s,
+                    int32_t *max_active)
+{
+  if (!S) return CM_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(S->mu);
+  std::vector<double> l = S->latencies;
+  std::sort(l.begin(), l.end());
+  auto pct = [&](double q) { return l.empty() ? 0.0 : l[std::min(l.size() - 1, (size_t)(q * (l.size() - 1) + 0.5))]; };
+  if (completed) *completed = S->completed;
+  if (p50_ms) *p50_ms = pct(0.50);
+  if (p99_ms) *p99_ms = pct(0.99);
+  if (graphs_per_s)
+    *graphs_per_s = S->t_last > S->t_first ? S->completed / ((S->t_last - S->t_first) / 1e3) : 0.0;
+  if (max_active) *max_active = S->max_active;
+  return CM_OK;
+}
+
+int cm_server_destroy(cm_server *S)
+{
+  if (!S) return CM_ERR_INVALID;
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->stop = true;
+  }
+  S->cv_loop.notify_all();
+  S->cv_fin.notify_all();
+  if (S->loop.joinable()) S->loop.join();
+  if (S->finisher.joinable()) S->finisher.join();
+  for (auto &kv : S->jobs) {
+    cm::Job *j = kv.second.get();
+    if (j->h) {
+      for (void *q : j->h->owned) S->cctx->pool.release(q);
+      delete j->h;
+    }
+  }
+  S->jobs.clear();
+  cm_ctx_destroy(S->fctx);
+  cm_ctx_destroy(S->cctx);
+  delete S;
+  return CM_OK;
+}
+
+}  // extern "C"

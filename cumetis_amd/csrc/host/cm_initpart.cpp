@@ -17,7 +17,30 @@
 namespace cmh {
 namespace {
 
-int rnd(int max) { return (int)((uint32_t)rand() % (uint32_t)max); }  // cu:287-300
+// cu:287-300 draws from glibc rand() with its default seed 1.  rand() is
+// random() over a TYPE_3 (128-byte) state; random_r over a private state
+// seeded with 1 yields the same sequence without sharing the process-wide
+// state, so concurrent partitions stay deterministic.
+struct RandStream {
+  random_data rd;
+  char state[128];
+  RandStream()
+  {
+    memset(&rd, 0, sizeof(rd));
+    memset(state, 0, sizeof(state));
+    initstate_r(1, state, sizeof(state), &rd);
+    srandom_r(1, &rd);
+  }
+  int32_t next()
+  {
+    int32_t r;
+    random_r(&rd, &r);
+    return r;
+  }
+};
+thread_local RandStream *g_stream = nullptr;
+
+int rnd(int max) { return (int)((uint32_t)g_stream->next() % (uint32_t)max); }
 
 // cu:639-671: identity (flag) then m 4-element block swaps (n >= 10) or n
 // pair swaps (n < 10)
@@ -640,6 +663,8 @@ int initial_partition(const HostGraph &hg, int nparts, std::vector<int> &where)
 {
   where.assign(hg.n, 0);
   if (nparts <= 1 || hg.n == 0) return 0;
+  RandStream stream;  // a fresh stream per call: the reference's first-call sequence
+  g_stream = &stream;
   std::unique_ptr<Part2> g(new Part2());
   g->n = hg.n;
   g->m = hg.m();

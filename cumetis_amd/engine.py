@@ -355,3 +355,58 @@ def _coarsen_batch(self, gs, nparts, seed=1, row_order=L.CM_ROW_FIRST_SEEN, max_
 
 
 Engine.coarsen_batch = _coarsen_batch
+
+
+class Server:
+    """Continuous-batching partition server (cm_server_* of the C ABI)."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 0, max_slots: int = 64, lib_path=None):
+        self.lib = L.load(lib_path)
+        self.s = C.c_void_p()
+        rc = self.lib.cm_server_create(device, int(hbm_budget), int(max_slots), C.byref(self.s))
+        if rc != L.CM_OK:
+            raise L.EngineError(rc, "cm_server_create failed")
+        self._keep = {}
+        self._n = {}
+
+    def submit(self, g: Graph, nparts: int, seed: int = 1) -> int:
+        odt = np.int64 if g.xadj.dtype == np.int64 else np.int32
+        keep = [np.ascontiguousarray(g.xadj, odt), np.ascontiguousarray(g.adjncy, np.int32),
+                None if g.adjwgt is None else np.ascontiguousarray(g.adjwgt, np.int32),
+                None if g.vwgt is None else np.ascontiguousarray(g.vwgt, np.int32)]
+        v = L.GraphView(g.n, g.m, L.CM_OFF64 if odt == np.int64 else L.CM_OFF32, 0, keep[0].ctypes.data,
+                        keep[1].ctypes.data, None if keep[2] is None else keep[2].ctypes.data,
+                        None if keep[3] is None else keep[3].ctypes.data)
+        p = L.CoarsenParams(int(nparts), 0, 0, L.CM_ROW_FIRST_SEEN, 0, 0, int(seed), 0, 0)
+        t = C.c_uint64()
+        rc = self.lib.cm_server_submit(self.s, C.byref(v), C.byref(p), C.byref(t))
+        if rc != L.CM_OK:
+            raise L.EngineError(rc, "cm_server_submit failed")
+        self._n[t.value] = g.n
+        return t.value
+
+    def wait(self, ticket: int):
+        n = self._n.pop(ticket)
+        part = np.empty(max(n, 1), np.int32)
+        rep = L.PartReport()
+        rc = self.lib.cm_server_wait(self.s, C.c_uint64(ticket), part.ctypes.data, C.byref(rep))
+        if rc != L.CM_OK:
+            raise L.EngineError(rc, "job failed")
+        return part[:n], rep
+
+    def stats(self) -> dict:
+        c, p50, p99, gps, ma = C.c_int64(), C.c_double(), C.c_double(), C.c_double(), C.c_int32()
+        self.lib.cm_server_stats(self.s, C.byref(c), C.byref(p50), C.byref(p99), C.byref(gps), C.byref(ma))
+        return {"completed": c.value, "p50_ms": p50.value, "p99_ms": p99.value,
+                "graphs_per_s": gps.value, "max_active": ma.value}
+
+    def close(self):
+        if self.s:
+            self.lib.cm_server_destroy(self.s)
+            self.s = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

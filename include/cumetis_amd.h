@@ -197,6 +197,26 @@ typedef struct {
 int cm_partition(cm_ctx *ctx, const cm_graph_view *g, const cm_coarsen_params *p, int32_t *part,
                  cm_part_report *report);
 
+/* -------------------------------------------------------------- serving */
+/* Continuous-batching partition server.  Submitted host graphs (copied at
+ * submit) are admitted as slots (max_slots) and the HBM budget (bytes, 0 =
+ * 64 GiB; estimated per job from its size) free up.  Each coarsening step
+ * runs one level for all admitted jobs as one batched union, so new jobs
+ * join the next step while others are mid-hierarchy; finished hierarchies
+ * stay device-resident while a second thread runs the initial partition and
+ * uncoarsening of each job, overlapping the coarsening steps.  A job's result
+ * is identical to cm_partition of the same graph alone.  Only
+ * CM_ROW_FIRST_SEEN is accepted. */
+typedef struct cm_server cm_server;
+int cm_server_create(int device, int64_t hbm_budget, int32_t max_slots, cm_server **out);
+int cm_server_submit(cm_server *s, const cm_graph_view *g, const cm_coarsen_params *p, uint64_t *ticket);
+/* Blocks until the job is done; part receives n entries; report total_ms is
+ * the latency from submit to result.  A ticket can be waited once. */
+int cm_server_wait(cm_server *s, uint64_t ticket, int32_t *part, cm_part_report *report);
+int cm_server_stats(cm_server *s, int64_t *completed, double *p50_ms, double *p99_ms, double *graphs_per_s,
+                    int32_t *max_active);
+int cm_server_destroy(cm_server *s);
+
 /* ------------------------------------------------------------ host parts */
 /* METIS .graph reader (cu:3712-3851): arrays are malloc'd, release each
  * with cm_free_host.  Malformed input returns CM_ERR_INPUT with the message
@@ -207,8 +227,9 @@ void cm_free_host(void *p);
 /* "<graph>.part.<k>" writer: one part id per line (cu:3856-3870). */
 int cm_write_partition(const char *path, const int32_t *part, int64_t n);
 /* The host initial partition alone (cuMetis_initialpartition, cu:3091-3108;
- * recursive bisection cu:2964-3025).  Draws glibc rand() from its current
- * state; *objval receives the summed bisection cuts. */
+ * recursive bisection cu:2964-3025).  Draws the glibc rand() sequence of
+ * seed 1 (the reference never seeds) from a private, per-call stream;
+ * *objval receives the summed bisection cuts. */
 int cm_initial_partition(int64_t n, const int64_t *xadj, const int32_t *adjncy, const int32_t *adjwgt,
                          const int32_t *vwgt, int32_t nparts, int32_t *where, int64_t *objval);
 
