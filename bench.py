@@ -234,13 +234,49 @@ def batch_leg(eng, count, order, steps):
             "equal_cut": "by construction: identical hierarchies give identical partitions"}
 
 
+def serve_leg(count=120, rate=20.0, seed=7):
+    """Poisson stream of partition requests: the batch distribution plus 5%
+    3D 7-point grids 160^3 (k=128), at a fixed offered load."""
+    from cumetis_amd import graphs
+    from cumetis_amd.engine import Server
+    progress(f"serve leg: {count} requests at {rate}/s")
+    rng = np.random.default_rng(seed)
+    small, ks = batch_graphs(max(8, count), seed=seed)
+    big = graphs.grid3d(160, 160, 160)
+    reqs = []
+    for i in range(count):
+        if rng.random() < 0.05:
+            reqs.append((big, 128))
+        else:
+            reqs.append((small[i], ks[i]))
+    gaps = rng.exponential(1.0 / rate, size=count)
+    with Server(0, max_slots=64) as srv:
+        t0 = time.perf_counter()
+        tickets = []
+        due = t0
+        for (g, k), gap in zip(reqs, gaps):
+            due += gap
+            d = due - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            tickets.append(srv.submit(g, k, seed=1))
+        progress("serve leg: all submitted")
+        reps = [srv.wait(t)[1] for t in tickets]
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+    return {"requests": count, "offered_per_s": rate, "large_jobs": sum(1 for g, k in reqs if k == 128),
+            "p50_ms": round(st["p50_ms"], 2), "p99_ms": round(st["p99_ms"], 2),
+            "graphs_per_s": round(count / wall, 2), "max_concurrent_jobs": st["max_active"],
+            "mean_coarsen_levels": round(float(np.mean([r.levels for r in reps])), 2)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--order", default="first_seen", choices=["first_seen", "ascending"])
-    ap.add_argument("--legs", default="grid725", help="comma list of: grid725, rmat22, batch512")
+    ap.add_argument("--legs", default="grid725", help="comma list of: grid725, rmat22, batch512, serve")
     ap.add_argument("--batch-count", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
@@ -272,6 +308,8 @@ def main():
         del gr
     if "batch512" in legs:
         extra["batch512"] = batch_leg(eng, args.batch_count, order, 3)
+    if "serve" in legs:
+        extra["serve"] = serve_leg()
     progress("cpu baseline")
     cpu = None if args.no_cpu else cpu_baseline(g, 64)
     out = {

@@ -121,8 +121,10 @@ struct Job {
 }  // namespace cm
 
 struct cm_server {
-  cm_ctx *cctx = nullptr, *fctx = nullptr;
-  std::thread loop, finisher;
+  cm_ctx *cctx = nullptr;
+  std::vector<cm_ctx *> fctx;  // one context (stream) per finisher thread
+  std::thread loop;
+  std::vector<std::thread> finishers;
   std::mutex mu;
   std::condition_variable cv_loop, cv_fin, cv_done;
   std::deque<cm::Job *> pending, to_finish;
@@ -325,13 +327,13 @@ static void loop_main(cm_server *S)
       S->active.erase(std::find(S->active.begin(), S->active.end(), j));
       S->to_finish.push_back(j);
     }
-    if (!retire.empty()) S->cv_fin.notify_one();
+    if (!retire.empty()) S->cv_fin.notify_all();
   }
 }
 
-static void finish_main(cm_server *S)
+static void finish_main(cm_server *S, cm_ctx *fctx)
 {
-  hipSetDevice(S->fctx->device);
+  hipSetDevice(fctx->device);
   for (;;) {
     Job *j = nullptr;
     {
@@ -346,7 +348,7 @@ static void finish_main(cm_server *S)
         j->part.assign(j->n, 0);
         const double t0 = now_ms();
         if (j->p.nparts > 1 && j->n > 0)
-          partition_hierarchy(S->fctx, j->h, j->p.nparts, j->part.data(), &j->rep.init_ms, &j->rep.uncoarsen_ms);
+          partition_hierarchy(fctx, j->h, j->p.nparts, j->part.data(), &j->rep.init_ms, &j->rep.uncoarsen_ms);
         j->rep.levels = (int32_t)j->h->levels.size();
         j->rep.coarsest_n = (int32_t)j->h->levels.back().n;
         int64_t cut = 0;
@@ -387,8 +389,16 @@ int cm_server_create(int device, int64_t hbm_budget, int32_t max_slots, cm_serve
   *out = nullptr;
   cm_server *S = new cm_server();
   int rc = cm_ctx_create(device, nullptr, &S->cctx);
-  if (rc == CM_OK) rc = cm_ctx_create(device, nullptr, &S->fctx);
+  // host initial partitions run in parallel: each finisher has its own stream
+  const unsigned hw = std::thread::hardware_concurrency();
+  const int nfin = (int)std::max(1u, std::min(hw ? hw / 2 : 1u, 4u));
+  for (int i = 0; i < nfin && rc == CM_OK; ++i) {
+    cm_ctx *f = nullptr;
+    rc = cm_ctx_create(device, nullptr, &f);
+    if (rc == CM_OK) S->fctx.push_back(f);
+  }
   if (rc != CM_OK) {
+    for (cm_ctx *f : S->fctx) cm_ctx_destroy(f);
     if (S->cctx) cm_ctx_destroy(S->cctx);
     delete S;
     return rc;
@@ -396,7 +406,7 @@ int cm_server_create(int device, int64_t hbm_budget, int32_t max_slots, cm_serve
   S->budget = hbm_budget > 0 ? (size_t)hbm_budget : (size_t)64 << 30;
   S->max_slots = max_slots;
   S->loop = std::thread(cm::loop_main, S);
-  S->finisher = std::thread(cm::finish_main, S);
+  for (cm_ctx *f : S->fctx) S->finishers.emplace_back(cm::finish_main, S, f);
   *out = S;
   return CM_OK;
 }
@@ -480,7 +490,8 @@ int cm_server_destroy(cm_server *S)
   S->cv_loop.notify_all();
   S->cv_fin.notify_all();
   if (S->loop.joinable()) S->loop.join();
-  if (S->finisher.joinable()) S->finisher.join();
+  for (auto &t : S->finishers)
+    if (t.joinable()) t.join();
   for (auto &kv : S->jobs) {
     cm::Job *j = kv.second.get();
     if (j->h) {
@@ -489,7 +500,7 @@ int cm_server_destroy(cm_server *S)
     }
   }
   S->jobs.clear();
-  cm_ctx_destroy(S->fctx);
+  for (cm_ctx *f : S->fctx) cm_ctx_destroy(f);
   cm_ctx_destroy(S->cctx);
   delete S;
   return CM_OK;
