@@ -20,6 +20,7 @@ CM_ERR_INPUT = -5
 CM_OFF32, CM_OFF64 = 0, 1
 CM_ROW_FIRST_SEEN, CM_ROW_ASCENDING = 0, 1
 CM_FLAG_TRUSTED_INPUT = 1
+CM_FLAG_NO_BALANCE = 2
 
 
 class GraphView(C.Structure):

@@ -193,7 +193,7 @@ void segsort_u64(cm_ctx *ctx, int64_t nseg, const void *d_segoff, bool off64, in
 // coarsest level (host), then per level projection + one refinement pass.
 // part receives level-0 assignments (host).
 void partition_hierarchy(cm_ctx *ctx, const cm_hier *h, int nparts, int32_t *part, double *init_ms,
-                         double *uncoarsen_ms);
+                         double *uncoarsen_ms, bool balance = true);
 // (d) projection
 void project(cm_ctx *ctx, int64_t n, const int32_t *d_cmap, const int32_t *d_where_c,
              int32_t *d_where_f);

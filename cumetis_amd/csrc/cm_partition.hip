@@ -4,6 +4,10 @@
 //   GPU coarsening -> coarsest level to the host -> recursive-bisection
 //   initial partition (host) -> refinement of the coarsest level -> for each
 //   finer level: GPU projection, then one refinement pass.
+// Unless switched off, every level's refinement is preceded by a balancing
+// pass (no counterpart in the reference) that empties parts above the
+// refinement's upper bound int(w/k * 1.03): the refinement never moves a
+// vertex out of such a part, so an overweight initial part stays overweight.
 #include <stdlib.h>
 #include <string.h>
 
@@ -69,7 +73,7 @@ int64_t host_cut(const cm_graph_view *g, const int32_t *part)
 namespace cm {
 
 void partition_hierarchy(cm_ctx *ctx, const cm_hier *h, int nparts, int32_t *part, double *init_ms,
-                         double *uncoarsen_ms)
+                         double *uncoarsen_ms, bool balance)
 {
   const double t1 = now_ms();
   const int L = (int)h->levels.size();
@@ -84,6 +88,7 @@ void partition_hierarchy(cm_ctx *ctx, const cm_hier *h, int nparts, int32_t *par
   int32_t *w0 = (int32_t *)ctx->pool.alloc(4 * (C.n > 0 ? C.n : 1)), *w1 = nullptr;
   try {
     if (C.n) CM_HIP(hipMemcpyAsync(w0, where.data(), 4 * C.n, hipMemcpyHostToDevice, ctx->stream));
+    if (balance) balance_level(ctx, C, nparts, w0, pw, tvwgt);
     refine_level(ctx, C, nparts, w0, pw, tvwgt);
     for (int lev = L - 2; lev >= 0; --lev) {
       const Level &F = h->levels[lev];
@@ -92,6 +97,7 @@ void partition_hierarchy(cm_ctx *ctx, const cm_hier *h, int nparts, int32_t *par
       ctx->pool.release(w0);
       w0 = w1;
       w1 = nullptr;
+      if (balance) balance_level(ctx, F, nparts, w0, pw, tvwgt);
       refine_level(ctx, F, nparts, w0, pw, tvwgt);
     }
     const int64_t n0 = h->levels[0].n;
@@ -129,7 +135,7 @@ int cm_partition(cm_ctx *ctx, const cm_graph_view *g, const cm_coarsen_params *p
     if (rc != CM_OK) return rc;
     const double t1 = now_ms();
     double init_ms = 0, unc_ms = 0;
-    cm::partition_hierarchy(ctx, h, p->nparts, part, &init_ms, &unc_ms);
+    cm::partition_hierarchy(ctx, h, p->nparts, part, &init_ms, &unc_ms, !(p->flags & CM_FLAG_NO_BALANCE));
     const double t3 = now_ms();
     if (rep) {
       rep->total_ms = t3 - t0;

@@ -11,4 +11,9 @@ namespace cm {
 // the number of accepted moves.
 int refine_level(cm_ctx *ctx, const Level &L, int nparts, int32_t *d_where, std::vector<int64_t> &pw,
                  int64_t tvwgt);
+// Moves vertices out of parts above int(w/k * 1.03) (the refinement's upper
+// bound), highest connectivity gain first, into parts with room; no
+// counterpart in the reference, which does not enforce the bound.
+int balance_level(cm_ctx *ctx, const Level &L, int nparts, int32_t *d_where, std::vector<int64_t> &pw,
+                  int64_t tvwgt);
 }  // namespace cm

@@ -348,7 +348,8 @@ static void finish_main(cm_server *S, cm_ctx *fctx)
         j->part.assign(j->n, 0);
         const double t0 = now_ms();
         if (j->p.nparts > 1 && j->n > 0)
-          partition_hierarchy(fctx, j->h, j->p.nparts, j->part.data(), &j->rep.init_ms, &j->rep.uncoarsen_ms);
+          partition_hierarchy(fctx, j->h, j->p.nparts, j->part.data(), &j->rep.init_ms, &j->rep.uncoarsen_ms,
+                              !(j->p.flags & CM_FLAG_NO_BALANCE));
         j->rep.levels = (int32_t)j->h->levels.size();
         j->rep.coarsest_n = (int32_t)j->h->levels.back().n;
         int64_t cut = 0;

@@ -299,9 +299,11 @@ class Engine:
         return hier
 
     def partition(self, g: Graph, nparts: int, seed: int = 1,
-                  row_order: int = L.CM_ROW_FIRST_SEEN):
+                  row_order: int = L.CM_ROW_FIRST_SEEN, balance: bool = True):
+        """cm_partition; balance=False gives the reference's pipeline without
+        the balancing pass."""
         v, keep = self.host_view(g)
-        p = L.CoarsenParams(nparts, 0, 0, row_order, 0, 0, seed, 0, 0)
+        p = L.CoarsenParams(nparts, 0, 0, row_order, 0, 0, seed, 0 if balance else L.CM_FLAG_NO_BALANCE, 0)
         part = np.empty(max(g.n, 1), np.int32)
         rep = L.PartReport()
         self._check(self.lib.cm_partition(self.ctx, C.byref(v), C.byref(p), part.ctypes.data,
@@ -369,7 +371,7 @@ class Server:
         self._keep = {}
         self._n = {}
 
-    def submit(self, g: Graph, nparts: int, seed: int = 1) -> int:
+    def submit(self, g: Graph, nparts: int, seed: int = 1, balance: bool = True) -> int:
         odt = np.int64 if g.xadj.dtype == np.int64 else np.int32
         keep = [np.ascontiguousarray(g.xadj, odt), np.ascontiguousarray(g.adjncy, np.int32),
                 None if g.adjwgt is None else np.ascontiguousarray(g.adjwgt, np.int32),
@@ -377,7 +379,8 @@ class Server:
         v = L.GraphView(g.n, g.m, L.CM_OFF64 if odt == np.int64 else L.CM_OFF32, 0, keep[0].ctypes.data,
                         keep[1].ctypes.data, None if keep[2] is None else keep[2].ctypes.data,
                         None if keep[3] is None else keep[3].ctypes.data)
-        p = L.CoarsenParams(int(nparts), 0, 0, L.CM_ROW_FIRST_SEEN, 0, 0, int(seed), 0, 0)
+        p = L.CoarsenParams(int(nparts), 0, 0, L.CM_ROW_FIRST_SEEN, 0, 0, int(seed),
+                            0 if balance else L.CM_FLAG_NO_BALANCE, 0)
         t = C.c_uint64()
         rc = self.lib.cm_server_submit(self.s, C.byref(v), C.byref(p), C.byref(t))
         if rc != L.CM_OK:

@@ -77,6 +77,11 @@ typedef struct {
 
 /* cm_coarsen_params.flags */
 #define CM_FLAG_TRUSTED_INPUT 1 /* skip the device-side validation of the input graph */
+/* cm_partition / cm_server: skip the balancing pass that moves vertices out
+ * of parts heavier than int(w/k * 1.03) before each level's refinement; with
+ * it set the pipeline is the reference's (cu:3544-3616), which leaves
+ * overweight parts of the initial partition overweight. */
+#define CM_FLAG_NO_BALANCE 2
 
 /* One level of a hierarchy (device pointers owned by the hierarchy).
  * A level of a batched hierarchy is a slice of the batch's level arrays:

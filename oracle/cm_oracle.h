@@ -66,6 +66,9 @@ int orc_initpart(int n, const int64_t *xadj, const int *adjncy, const int *adjwg
                  const int *vwgt, int nparts, int *where);
 void orc_kway_refine(int n, const int64_t *xadj, const int *adjncy, const int *adjwgt,
                      const int *vwgt, int nparts, int *where);
+/* the engine's balancing pass (no reference counterpart); returns moves */
+int orc_kway_balance(int n, const int64_t *xadj, const int *adjncy, const int *adjwgt,
+                     const int *vwgt, int nparts, int *where);
 
 #ifdef __cplusplus
 }

@@ -736,3 +736,75 @@ void orc_kway_refine(int n, const int64_t *xadj, const int *adjncy, const int *a
   }
   free(pw); free(maxw); free(minw); free(cand); free(to); free(conn); free(cnt); free(order);
 }
+
+/* Balancing pass run before each level's refinement by the engine's
+ * pipeline; it has no counterpart in the reference.  Up to four sweeps:
+ *  - parts above maxw = int(tp*tvwgt*1.03f) are "over", parts below it have
+ *    room; the fallback destination is the lightest part (lowest id);
+ *  - every vertex of an over part picks the adjacent part with room and the
+ *    largest connectivity (lowest id on ties), else the fallback; its gain
+ *    is that connectivity (0 for the fallback) minus its own part's;
+ *  - vertices are taken in order of gain descending, then vertex id, and a
+ *    move is accepted while its source part is still over and the
+ *    destination stays within maxw.
+ * A sweep with no accepted move ends the pass. */
+typedef struct { uint64_t key; int to; } bal_t;
+
+static int bal_cmp(const void *a, const void *b)
+{
+  uint64_t x = ((const bal_t *)a)->key, y = ((const bal_t *)b)->key;
+  return x < y ? -1 : x > y;
+}
+
+int orc_kway_balance(int n, const int64_t *xadj, const int *adjncy, const int *adjwgt,
+                     const int *vwgt, int nparts, int *where)
+{
+  int i, p, it, total = 0;
+  int64_t j, tvwgt = 0, maxw;
+  int64_t *pw = (int64_t *)calloc(nparts, sizeof(int64_t));
+  int64_t *conn = (int64_t *)calloc(nparts, sizeof(int64_t));
+  bal_t *cand = (bal_t *)malloc(sizeof(bal_t) * (n > 0 ? n : 1));
+  float tp = (float)(1.0 / nparts);
+  for (i = 0; i < n; i++) { tvwgt += vwgt[i]; pw[where[i]] += vwgt[i]; }
+  maxw = (int64_t)(tp * (float)tvwgt * 1.03f);
+  for (it = 0; it < 4; it++) {
+    int any = 0, fallback = 0, nc = 0, moved = 0;
+    for (p = 0; p < nparts; p++) {
+      if (pw[p] > maxw) any = 1;
+      if (pw[p] < pw[fallback]) fallback = p;
+    }
+    if (!any) break;
+    for (i = 0; i < n; i++) {
+      int me = where[i], bp = -1, to;
+      int64_t best = -1, gain;
+      if (pw[me] <= maxw) continue;
+      for (j = xadj[i]; j < xadj[i + 1]; j++) conn[where[adjncy[j]]] += adjwgt[j];
+      for (p = 0; p < nparts; p++)
+        if (pw[p] < maxw && conn[p] > 0 && conn[p] > best) { best = conn[p]; bp = p; }
+      to = bp >= 0 ? bp : fallback;
+      gain = (bp >= 0 ? best : 0) - conn[me];
+      for (j = xadj[i]; j < xadj[i + 1]; j++) conn[where[adjncy[j]]] = 0;
+      if (to != me) {
+        if (gain > 0x7fffffffLL) gain = 0x7fffffffLL;
+        if (gain < -0x7fffffffLL) gain = -0x7fffffffLL;
+        cand[nc].key = ((uint64_t)(uint32_t)(0x80000000LL - gain) << 32) | (uint32_t)i;
+        cand[nc].to = to;
+        nc++;
+      }
+    }
+    qsort(cand, nc, sizeof(bal_t), bal_cmp);
+    for (i = 0; i < nc; i++) {
+      int v = (int)(uint32_t)cand[i].key, from = where[v], t = cand[i].to;
+      if (pw[from] > maxw && pw[t] + vwgt[v] <= maxw) {
+        pw[from] -= vwgt[v];
+        pw[t] += vwgt[v];
+        where[v] = t;
+        moved++;
+      }
+    }
+    total += moved;
+    if (!moved) break;
+  }
+  free(pw); free(conn); free(cand);
+  return total;
+}

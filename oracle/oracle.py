@@ -47,6 +47,7 @@ def lib():
             "orc_segsort_u64": (None, [C.c_int64, P, P, P]),
             "orc_initpart": (C.c_int, [C.c_int, P, P, P, P, C.c_int, P]),
             "orc_kway_refine": (None, [C.c_int, P, P, P, P, C.c_int, P]),
+            "orc_kway_balance": (C.c_int, [C.c_int, P, P, P, P, C.c_int, P]),
         }
         for k, (r, a) in sig.items():
             f = getattr(_lib, k)
@@ -179,6 +180,15 @@ def kway_refine(xadj, adjncy, adjwgt, vwgt, nparts, where):
     return wh
 
 
+def kway_balance(xadj, adjncy, adjwgt, vwgt, nparts, where):
+    """The engine's balancing pass (no reference counterpart): moves vertices
+    out of parts above int(w/k * 1.03).  Returns (where, moves)."""
+    n, x, a, w, v = _csr(xadj, adjncy, adjwgt, vwgt)
+    wh = np.ascontiguousarray(where, np.int32).copy()
+    moved = lib().orc_kway_balance(n, _p(x), _p(a), _p(w), _p(v), nparts, _p(wh))
+    return wh, moved
+
+
 def coarsen(xadj, adjncy, adjwgt, vwgt, nparts, matcher="cpu", seed=1, T=None, maxv=None):
     """Reference level loop (cu:1824-1850 / 1854-1875) with the k-way threshold
     (cu:3634); matcher 'cpu' = cuMetis_cpu_match, 'handshake' = the engine's
@@ -205,12 +215,14 @@ def coarsen(xadj, adjncy, adjwgt, vwgt, nparts, matcher="cpu", seed=1, T=None, m
     return levels
 
 
-def partition(xadj, adjncy, adjwgt, vwgt, nparts, matcher="handshake", seed=1):
+def partition(xadj, adjncy, adjwgt, vwgt, nparts, matcher="handshake", seed=1, balance=False):
     """The multilevel pipeline (cu:3544-3616) on the CPU: coarsen (matcher
     'cpu' = the reference's CPU HEM, 'handshake' = the engine's matching
     model), initial partition of the coarsest level (glibc stream from seed
     1), then refine the coarsest level and, for every finer level, project and
-    refine.  Returns (where, levels)."""
+    refine.  balance=True runs the engine's balancing pass before each
+    refinement (the engine's default; the reference has none).
+    Returns (where, levels)."""
     n = len(xadj) - 1
     if nparts == 1 or n == 0:
         return np.zeros(n, np.int32), []
@@ -220,8 +232,10 @@ def partition(xadj, adjncy, adjwgt, vwgt, nparts, matcher="handshake", seed=1):
     C = levels[-1]
     srand(1)
     where, _ = initpart(C["xadj"], C["adjncy"], C["adjwgt"], C["vwgt"], nparts)
-    where = kway_refine(C["xadj"], C["adjncy"], C["adjwgt"], C["vwgt"], nparts, where)
-    for L in reversed(levels[:-1]):
-        where = project(L["cmap"], where)
+    for L in [C] + list(reversed(levels[:-1])):
+        if L is not C:
+            where = project(L["cmap"], where)
+        if balance:
+            where, _ = kway_balance(L["xadj"], L["adjncy"], L["adjwgt"], L["vwgt"], nparts, where)
         where = kway_refine(L["xadj"], L["adjncy"], L["adjwgt"], L["vwgt"], nparts, where)
     return where, levels

@@ -71,8 +71,9 @@ def test_batch_device_input_and_int64(engine):
         levels_equal(x, y)
 
 
+@pytest.mark.parametrize("balance", [True, False])
 @pytest.mark.parametrize("name,k", [("delaunay4720", 8), ("grid100", 16), ("rgg", 4), ("hub", 8)])
-def test_partition_bit_exact_with_oracle_pipeline(engine, orc, name, k):
+def test_partition_bit_exact_with_oracle_pipeline(engine, orc, name, k, balance):
     if name == "grid100":
         g = graphs.grid2d(100, 100, wmax=100, seed=2)
     elif name == "rgg":
@@ -82,9 +83,10 @@ def test_partition_bit_exact_with_oracle_pipeline(engine, orc, name, k):
     else:
         g = named_cases()[name]
     g = with_weights(g)
-    say(f"[partition vs oracle pipeline] {name} k={k}")
-    part, rep = engine.partition(g, k, seed=1)
-    where, levels = orc.partition(g.xadj, g.adjncy, g.adjwgt, g.vwgt, k, matcher="handshake", seed=1)
+    say(f"[partition vs oracle pipeline] {name} k={k} balance={balance}")
+    part, rep = engine.partition(g, k, seed=1, balance=balance)
+    where, levels = orc.partition(g.xadj, g.adjncy, g.adjwgt, g.vwgt, k, matcher="handshake", seed=1,
+                                  balance=balance)
     assert rep.levels == len(levels)
     assert np.array_equal(part, where)
     assert rep.edgecut == orc.edgecut(g.xadj, g.adjncy, g.adjwgt, part)
@@ -111,9 +113,11 @@ def test_partition_quality_vs_reference_pipeline(engine, orc, name, k):
     ref, _ = orc.partition(g.xadj, g.adjncy, g.adjwgt, g.vwgt, k, matcher="cpu")
     cut = rep.edgecut
     rcut = orc.edgecut(g.xadj, g.adjncy, g.adjwgt, ref)
-    print(f"cut={cut} ref_cut={rcut} ratio={cut / rcut:.4f} imb={_imbalance(g.vwgt, part, k):.4f} "
+    imb = _imbalance(g.vwgt, part, k)
+    print(f"cut={cut} ref_cut={rcut} ratio={cut / rcut:.4f} imb={imb:.4f} "
           f"ref_imb={_imbalance(g.vwgt, ref, k):.4f}", flush=True)
     assert part.min() >= 0 and part.max() < k
+    assert imb <= 1.03 + 1e-6
 
 
 def test_cli_3elt(tmp_path):

@@ -331,3 +331,58 @@ def test_kway_refine_invariants(orc):
     assert np.all(ext[moved] > inn[moved])
     # parts that started within the bound stay within it
     assert np.all((pw1 <= maxw) | (pw0 > maxw))
+
+
+def _path(n):
+    e = np.array([[i, i + 1] for i in range(n - 1)])
+    return graphs.from_edges(n, e[:, 0], e[:, 1])
+
+
+def test_kway_balance_known_answer(orc):
+    # path 0-1-2-3-4-5, unit weights, k=2, parts [0,0,0,0,0,1]: maxw =
+    # int(0.5*6*1.03) = 3, part 0 is over.  Worked by hand: v4 goes to its
+    # adjacent part 1 with gain 1-1=0; v0..v3 have no adjacent part with room
+    # and fall back to the lightest part 1 with gains -1 (v0) and -2
+    # (v1..v3).  Order v4, v0, v1, ...: v4 and v0 move, then part 0 weighs 3.
+    g = with_weights(_path(6))
+    out, moved = orc.kway_balance(g.xadj, g.adjncy, g.adjwgt, g.vwgt, 2, np.array([0, 0, 0, 0, 0, 1]))
+    assert moved == 2
+    assert list(out) == [1, 0, 0, 0, 1, 1]
+
+
+def test_kway_balance_prefers_connectivity(orc):
+    # 8 unit vertices, k=4: maxw = int(2*1.03) = 2; parts {0,1,2,3} {4} {5,6}
+    # {7} weigh 4,1,2,1, so part 0 is over and parts 1 and 3 have room.
+    # v0: part 2 is its heaviest link (9) but has no room; parts 1 and 3 tie
+    # at 3 -> part 1, gain 3-1=2.  v3: part 3 via 4, gain 4-1=3.  v1, v2 have
+    # no outside link: fallback (lightest part, lowest id: 1), gain -2.
+    # Order v3, v0, v1, v2: v3 and v0 move, then part 0 weighs 2.
+    u = [0, 0, 0, 0, 1, 2, 3]
+    v = [1, 5, 4, 7, 2, 3, 7]
+    w = [1, 9, 3, 3, 1, 1, 4]
+    g = with_weights(graphs.from_edges(8, np.array(u), np.array(v), np.array(w)))
+    out, moved = orc.kway_balance(g.xadj, g.adjncy, g.adjwgt, g.vwgt, 4,
+                                  np.array([0, 0, 0, 0, 1, 2, 2, 3]))
+    assert moved == 2 and list(out) == [1, 0, 0, 3, 1, 2, 2, 3]
+
+
+@pytest.mark.parametrize("k", [2, 5, 16])
+def test_kway_balance_invariants(orc, k):
+    g = with_weights(graphs.grid2d(30, 30))
+    rng = np.random.default_rng(k)
+    # skewed start: part 0 holds a third of the vertices
+    where = rng.integers(0, k, g.n).astype(np.int32)
+    where[rng.random(g.n) < 1 / 3] = 0
+    tv = int(g.vwgt.sum())
+    tp = np.float32(1.0 / k)
+    maxw = int(np.float32(np.float32(tp * np.float32(tv)) * np.float32(1.03)))
+    pw0 = np.bincount(where, weights=g.vwgt, minlength=k)
+    out, moved = orc.kway_balance(g.xadj, g.adjncy, g.adjwgt, g.vwgt, k, where)
+    pw1 = np.bincount(out, weights=g.vwgt, minlength=k)
+    assert moved == int((out != where).sum())
+    assert pw1.max() <= maxw                      # unit weights: always reachable
+    assert np.all(pw0[where[out != where]] > maxw)  # only over parts give vertices
+    assert pw1.sum() == pw0.sum()
+    # a balanced input is a fixed point
+    again, m2 = orc.kway_balance(g.xadj, g.adjncy, g.adjwgt, g.vwgt, k, out)
+    assert m2 == 0 and np.array_equal(again, out)
