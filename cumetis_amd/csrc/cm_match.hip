@@ -19,7 +19,8 @@
 
 namespace cm {
 
-constexpr int LONG_DEG = 64;  // rows longer than this are scanned by a whole wave
+constexpr int LONG_DEG = 64;      // rows longer than this are scanned by whole waves
+constexpr int LONG_CHUNK = 2048;  // entries per wave of a long row
 constexpr int ROUND_BATCH = 9;  // launches queued between host checks
 
 // flags[2r] = 1 if a vertex matched or died in round r, flags[2r+1] = 1 if
@@ -62,19 +63,31 @@ __global__ __launch_bounds__(256) void k_match_init(int64_t n, int32_t *match, i
   }
 }
 
+// Long rows are cut into chunks of up to LONG_CHUNK entries, one wave per
+// chunk.  Chunk descriptor: the row, its first chunk, its chunk count and the
+// chunk's index within the row.
+struct LongChunk {
+  int32_t v, first, nch, k;
+};
+
+// Chunk descriptors of all rows longer than LONG_DEG; the order of rows in
+// the list does not matter (the reduction over a row's chunks is exact).
 template <typename OffT>
-__global__ __launch_bounds__(256) void k_select_long(int64_t n, const OffT *xadj, int32_t *list,
+__global__ __launch_bounds__(256) void k_select_long(int64_t n, const OffT *xadj, LongChunk *chunks,
                                                      unsigned *count)
 {
   const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool is_long = v < n && (int64_t)(xadj[v + 1] - xadj[v]) > LONG_DEG;
-  const uint64_t mk = __ballot(is_long);
-  if (!mk) return;
-  const int leader = __ffsll((unsigned long long)mk) - 1;
+  const int64_t d = v < n ? (int64_t)(xadj[v + 1] - xadj[v]) : 0;
+  const bool is_long = d > LONG_DEG;
+  if (!__ballot(is_long)) return;
+  const unsigned nch = is_long ? (unsigned)((d + LONG_CHUNK - 1) / LONG_CHUNK) : 0u;
+  const unsigned inc = wave_incl_scan<unsigned>(nch);
+  const unsigned tot = __shfl(inc, CM_WAVE - 1, CM_WAVE);
   unsigned base = 0;
-  if (lane() == leader) base = atomicAdd(count, (unsigned)__popcll(mk));
-  base = __shfl(base, leader, CM_WAVE);
-  if (is_long) list[base + mask_rank(mk)] = (int32_t)v;
+  if (lane() == 0) base = atomicAdd(count, tot);
+  base = __shfl(base, 0, CM_WAVE) + inc - nch;
+  for (unsigned k = 0; k < nch; ++k)
+    chunks[base + k] = LongChunk{(int32_t)v, (int32_t)base, (int32_t)nch, (int32_t)k};
 }
 
 // Accepted in round r-1 (from its proposals alone): dead, or mutual.
@@ -114,21 +127,46 @@ __device__ __forceinline__ void scan_row(int64_t v, OffT b, OffT e, int l0, int 
   }
 }
 
+// Wave-wide maximum of (best, bk1, bk2) under key_better; -2 = none.
+__device__ __forceinline__ void wave_best(int32_t &best, uint64_t &bk1, uint64_t &bk2)
+{
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) {
+    const int32_t ob = __shfl_xor(best, d, CM_WAVE);
+    const uint64_t o1 = __shfl_xor(bk1, d, CM_WAVE);
+    const uint64_t o2 = __shfl_xor(bk2, d, CM_WAVE);
+    if (ob != -2 && (best == -2 || key_better(o1, o2, bk1, bk2))) {
+      best = ob;
+      bk1 = o1;
+      bk2 = o2;
+    }
+  }
+}
+
+// Per-chunk partial results of split rows; arrived[first chunk of a row]
+// counts the row's finished chunks and is cleared by the last one.
+struct LongPartial {
+  int32_t *best;
+  uint64_t *k1, *k2;
+  unsigned *arrived;
+};
+
 // Launch r of the handshake: accept of round r-1 (from prev = proposals of
 // round r-1) and proposals of round r (into cur), over the state after round
 // r-1.  match[] is written only for the thread's own vertex; readers treat a
 // neighbour as taken if match[u] != -1 or it was accepted in round r-1, which
 // gives the same answer whether or not the neighbour's own write has landed.
 // Blocks [0, nshort) take one vertex per thread (rows <= LONG_DEG and all
-// accepts); the remaining blocks take long rows from `list`, one wave each.
+// accepts); the remaining blocks take the chunks of long rows, one wave each.
 template <typename OffT>
 __global__ __launch_bounds__(256) void k_round(int64_t n, const OffT *__restrict__ xadj,
                                                const int32_t *__restrict__ adjncy,
                                                const int32_t *__restrict__ adjwgt,
                                                const int32_t *__restrict__ vwgt, MatchOwners own,
                                                int32_t *match, const int32_t *prev, int32_t *cur,
-                                               const int32_t *list, const unsigned *list_n,
-                                               unsigned *flags, int r, unsigned nshort, int propose)
+                                               const LongChunk *chunks, const unsigned *nchunks,
+                                               LongPartial part, unsigned *flags, int r, unsigned nshort,
+                                               int propose)
 {
   if (r >= 2 && !round_live(flags, r - 2)) return;
   const bool have_prev = r >= 1;
@@ -192,16 +230,18 @@ __global__ __launch_bounds__(256) void k_round(int64_t n, const OffT *__restrict
     return;
   }
   if (!propose) return;
-  // long rows: one wave per vertex
-  const unsigned nl = *list_n;
+  // long rows: one wave per chunk.  Whether a row is scanned depends only on
+  // state fixed before this launch (see above), so all chunks of a row agree.
+  const unsigned nq = *nchunks;
   const int l = lane();
   const int64_t nw = (int64_t)(gridDim.x - nshort) * (blockDim.x / CM_WAVE);
-  for (int64_t wi = ((int64_t)(blockIdx.x - nshort) * blockDim.x + threadIdx.x) / CM_WAVE; wi < nl; wi += nw) {
-    const int32_t v = list[wi];
+  for (int64_t q = ((int64_t)(blockIdx.x - nshort) * blockDim.x + threadIdx.x) / CM_WAVE; q < nq; q += nw) {
+    const LongChunk c = chunks[q];
+    const int32_t v = c.v;
     if (match[v] != -1 || (have_prev && accepted_prev(prev, v))) continue;
     const int32_t p = have_prev ? prev[v] : -1;
     if (p >= 0 && match[p] == -1 && !accepted_prev(prev, p)) {
-      if (l == 0) cur[v] = p;
+      if (l == 0 && c.k == 0) cur[v] = p;
       continue;
     }
     int32_t maxv;
@@ -210,19 +250,40 @@ __global__ __launch_bounds__(256) void k_round(int64_t n, const OffT *__restrict
     owner_params(own, v, maxv, seed, base);
     int32_t best = -2;
     uint64_t bk1 = 0, bk2 = 0;
-    scan_row<OffT>(v, xadj[v], xadj[v + 1], l, CM_WAVE, adjncy, adjwgt, vwgt, vwgt[v], maxv, seed, base,
-                   match, prev, have_prev, best, bk1, bk2);
-#pragma unroll
-    for (int d = 32; d > 0; d >>= 1) {
-      const int32_t ob = __shfl_xor(best, d, CM_WAVE);
-      const uint64_t o1 = __shfl_xor(bk1, d, CM_WAVE);
-      const uint64_t o2 = __shfl_xor(bk2, d, CM_WAVE);
+    const OffT b = xadj[v] + (OffT)c.k * LONG_CHUNK, e0 = xadj[v + 1];
+    const OffT e = e0 - b > (OffT)LONG_CHUNK ? b + (OffT)LONG_CHUNK : e0;
+    scan_row<OffT>(v, b, e, l, CM_WAVE, adjncy, adjwgt, vwgt, vwgt[v], maxv, seed, base, match, prev,
+                   have_prev, best, bk1, bk2);
+    wave_best(best, bk1, bk2);
+    if (c.nch == 1) {
+      if (l == 0) cur[v] = best;
+      continue;
+    }
+    // publish the chunk's best; the row's last chunk to finish reduces them
+    unsigned last = 0;
+    if (l == 0) {
+      __hip_atomic_store(&part.best[q], best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&part.k1[q], bk1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&part.k2[q], bk2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const unsigned old = __hip_atomic_fetch_add(&part.arrived[c.first], 1u, __ATOMIC_ACQ_REL,
+                                                  __HIP_MEMORY_SCOPE_AGENT);
+      last = old == (unsigned)c.nch - 1;
+      if (last) __hip_atomic_store(&part.arrived[c.first], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (!__shfl(last, 0, CM_WAVE)) continue;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    best = -2;
+    for (int i = l; i < c.nch; i += CM_WAVE) {
+      const int32_t ob = __hip_atomic_load(&part.best[c.first + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const uint64_t o1 = __hip_atomic_load(&part.k1[c.first + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const uint64_t o2 = __hip_atomic_load(&part.k2[c.first + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       if (ob != -2 && (best == -2 || key_better(o1, o2, bk1, bk2))) {
         best = ob;
         bk1 = o1;
         bk2 = o2;
       }
     }
+    wave_best(best, bk1, bk2);
     if (l == 0) cur[v] = best;
   }
 }
@@ -241,18 +302,25 @@ static int match_impl(cm_ctx *ctx, const Level &L, const MatchOwners &own, int m
   if (n == 0) return 0;
   const OffT *xadj = (const OffT *)L.xadj;
   int32_t *prop = (int32_t *)ctx->pool.alloc(sizeof(int32_t) * 2 * n);  // ping-pong
-  int32_t *list = (int32_t *)ctx->pool.alloc(sizeof(int32_t) * n);
+  // chunks: sum over rows longer than LONG_DEG of ceil(d / LONG_CHUNK)
+  const int64_t cap = L.m / LONG_CHUNK + std::min<int64_t>(n, L.m / (LONG_DEG + 1)) + 1;
+  LongChunk *chunks = (LongChunk *)ctx->pool.alloc(sizeof(LongChunk) * cap);
+  LongPartial part;
+  part.best = (int32_t *)ctx->pool.alloc(sizeof(int32_t) * cap);
+  part.k1 = (uint64_t *)ctx->pool.alloc(sizeof(uint64_t) * cap);
+  part.k2 = (uint64_t *)ctx->pool.alloc(sizeof(uint64_t) * cap);
+  part.arrived = (unsigned *)ctx->pool.alloc(sizeof(unsigned) * cap);
   unsigned *flags = (unsigned *)ctx->pool.alloc(sizeof(unsigned) * (2 * (size_t)max_rounds + 2));
-  unsigned *list_n = flags + 2 * (size_t)max_rounds;
+  unsigned *nchunks = flags + 2 * (size_t)max_rounds;
   fill_u32(ctx, flags, 0u, 2 * (int64_t)max_rounds + 2);
+  fill_u32(ctx, part.arrived, 0u, cap);
   const unsigned g = nblocks(n, 256);
   const double row_bytes = (double)sizeof(OffT) * (n + 1) + 8.0 * L.m + 8.0 * n;
   launch(ctx, "match_init", 8.0 * n, k_match_init, dim3(g), dim3(256), 0, n, match, prop);
   launch(ctx, "match_select_long", (double)sizeof(OffT) * (n + 1), k_select_long<OffT>, dim3(g),
-         dim3(256), 0, n, xadj, list, list_n);
-  const unsigned glong = 256;  // extra blocks whose waves stride over the long rows
-  // launch r: accept of round r-1, proposals of round r (launch max_rounds
-  // only accepts)
+         dim3(256), 0, n, xadj, chunks, nchunks);
+  // extra blocks whose waves stride over the long-row chunks
+  const unsigned glong = (unsigned)std::min<int64_t>(1024, std::max<int64_t>(16, (cap + 3) / 4));
   int r = 0;
   std::vector<unsigned> h(2);
   for (;;) {
@@ -260,8 +328,8 @@ static int match_impl(cm_ctx *ctx, const Level &L, const MatchOwners &own, int m
       int32_t *cur = prop + (size_t)(r & 1) * n;
       const int32_t *prv = prop + (size_t)((r + 1) & 1) * n;
       launch(ctx, "match_round", r == 0 ? row_bytes : 0.0, k_round<OffT>, dim3(g + glong), dim3(256), 0, n,
-             xadj, L.adjncy, L.adjwgt, L.vwgt, own, match, prv, cur, (const int32_t *)list,
-             (const unsigned *)list_n, flags, r, g, r < max_rounds ? 1 : 0);
+             xadj, L.adjncy, L.adjwgt, L.vwgt, own, match, prv, cur, (const LongChunk *)chunks,
+             (const unsigned *)nchunks, part, flags, r, g, r < max_rounds ? 1 : 0);
     }
     if (r < 2) continue;
     // status of the last accepted round (r-2: accepted by launch r-1)
@@ -281,7 +349,11 @@ static int match_impl(cm_ctx *ctx, const Level &L, const MatchOwners &own, int m
   int rounds = 1;
   while (rounds < acc && all[2 * (rounds - 1)] != 0 && all[2 * (rounds - 1) + 1] != 0) ++rounds;
   ctx->pool.release(prop);
-  ctx->pool.release(list);
+  ctx->pool.release(chunks);
+  ctx->pool.release(part.best);
+  ctx->pool.release(part.k1);
+  ctx->pool.release(part.k2);
+  ctx->pool.release(part.arrived);
   ctx->pool.release(flags);
   return rounds;
 }

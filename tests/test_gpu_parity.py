@@ -77,6 +77,24 @@ def test_match_bit_exact(engine, orc, name):
         assert r_gpu == r_orc
 
 
+@pytest.mark.parametrize("unit", [False, True])
+def test_match_split_rows(engine, orc, unit):
+    """Rows cut into many chunks: a hub of 140000 entries (more chunks than a
+    wave has lanes) and rows exactly at and just past the chunk boundary."""
+    from cases import hub_graph
+    g = hub_graph(seed=21, side=400, hub_deg=140000, mids=(2048, 2049, 4096, 4097))
+    if unit:
+        g = Graph(g.xadj, g.adjncy, None, None)
+    g = with_weights(g)
+    say(f"[match split rows] unit={unit} n={g.n} m={g.m}")
+    mv = maxv_for(g, orc=orc)
+    for seed in (1, 99):
+        m_gpu, r_gpu = engine.match(g, mv, seed=seed)
+        m_orc, r_orc = orc.handshake_match(g.xadj, g.adjncy, g.adjwgt, g.vwgt, mv, seed=seed)
+        np.testing.assert_array_equal(m_gpu, m_orc)
+        assert r_gpu == r_orc
+
+
 def test_match_repeatable(engine, orc):
     g = with_weights(CASES["rgg3000w16"])
     say("[match repeat] rgg3000w16")
