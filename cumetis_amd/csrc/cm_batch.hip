@@ -158,7 +158,9 @@ static void batch_impl(cm_ctx *ctx, int nb, const cm_graph_view *g, const int64_
   if (g->vwgt && N) {
     CM_HIP(hipMemcpyAsync(hv.data(), vw, 4 * N, hipMemcpyDeviceToHost, ctx->stream));
   }
+  queue_max_degree(ctx, N, off64, xadj);
   sync(ctx);
+  const int64_t maxdeg0 = ctx->h_small[4];
   // ---- slots (graphs with vertices); empty graphs get two empty levels
   std::vector<Slot> slots;
   for (int b = 0; b < nb; ++b) {
@@ -184,6 +186,7 @@ static void batch_impl(cm_ctx *ctx, int nb, const cm_graph_view *g, const int64_
     L0.tvwgt = tv;
     L0.vbase = voff[b];
     L0.ebase = eoff[b];
+    L0.maxdeg = maxdeg0;
     out[b]->levels.push_back(L0);
     if (n == 0) {
       Level E = L0;
@@ -242,6 +245,11 @@ static void batch_impl(cm_ctx *ctx, int nb, const cm_graph_view *g, const int64_
     U.adjncy = adj;
     U.adjwgt = wgt;
     U.vwgt = vw;
+    U.maxdeg = 0;
+    for (int s = 0; s < ns; ++s) {
+      const int64_t d = out[slots[s].graph]->levels.back().maxdeg;
+      U.maxdeg = (U.maxdeg < 0 || d < 0) ? -1 : std::max(U.maxdeg, d);
+    }
     int32_t *match = (int32_t *)own(4 * n), *cmap = (int32_t *)own(4 * n);
     MatchOwners mo;
     mo.gid = gid;
@@ -255,7 +263,9 @@ static void batch_impl(cm_ctx *ctx, int nb, const cm_graph_view *g, const int64_
     const int64_t nc = number(ctx, U, match, cmap, rep, segoff, cvw);
     OffT *cxadj = (OffT *)own(sizeof(OffT) * (nc + 1));
     int32_t *cadj = (int32_t *)own(4 * (m > 0 ? m : 1)), *cwgt = (int32_t *)own(4 * (m > 0 ? m : 1));
-    const int64_t mc = contract(ctx, U, match, cmap, nc, rep, segoff, params[0].row_order, cxadj, cadj, cwgt);
+    int64_t cmaxdeg = -1;
+    const int64_t mc = contract(ctx, U, match, cmap, nc, rep, segoff, params[0].row_order, cxadj, cadj, cwgt,
+                                &cmaxdeg);
     // per-slot coarse bases
     int64_t *d_cvb = (int64_t *)tmp(8 * (ns + 1)), *d_ceb = (int64_t *)tmp(8 * (ns + 1));
     launch(ctx, "batch_bases", 0.0, k_slot_bases<OffT>, dim3(nblocks(ns + 1, 256)), dim3(256), 0, ns,
@@ -285,6 +295,7 @@ static void batch_impl(cm_ctx *ctx, int nb, const cm_graph_view *g, const int64_
       C.tvwgt = F.tvwgt;
       C.vbase = cvb[s];
       C.ebase = ceb[s];
+      C.maxdeg = cmaxdeg;  // of the union: a bound for each slot
       const int64_t nf = slots[s].n;
       h->levels.push_back(C);
       const bool more = C.n > slots[s].T && C.n < 0.75 * nf && C.m > C.n / 2;

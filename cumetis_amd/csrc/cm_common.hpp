@@ -33,6 +33,17 @@ __device__ __forceinline__ T wave_sum(T v)
   return v;
 }
 
+template <typename T>
+__device__ __forceinline__ T wave_max(T v)
+{
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) {
+    const T o = __shfl_xor(v, d, CM_WAVE);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+
 // Number of set bits of `mask` below this lane.
 __device__ __forceinline__ int mask_rank(uint64_t mask)
 {

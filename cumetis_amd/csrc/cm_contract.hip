@@ -13,6 +13,7 @@
 //     byte for byte: distinct neighbours ranked by first position, self entry
 //     replaced by the last entry.
 #include "cm_common.hpp"
+#include "cm_contract_hash.hpp"
 #include "cm_internal.hpp"
 #include "cm_segsort.hpp"
 
@@ -20,7 +21,6 @@ namespace cm {
 
 constexpr int GATHER_LONG = 64;   // fine rows longer than this gather by wave
 constexpr int SMALL_SEG = 64;     // coarse segments up to this merge per thread
-constexpr int REG_SEG = 16;       // coarse segments up to this merge in registers
 constexpr int FIRST_SORTED_BIN = 4;  // segsort bins from 17 keys up
 
 // ------------------------------------------------------------- numbering
@@ -572,7 +572,8 @@ __global__ __launch_bounds__(256) void k_contract_small(int64_t nc, const OffT *
                                                         const int32_t *__restrict__ cdeg_large,
                                                         OffT *__restrict__ cxadj, int32_t *__restrict__ cadjncy,
                                                         int32_t *__restrict__ cadjwgt, uint64_t *st,
-                                                        unsigned *ctr, int64_t *total, uint32_t epoch)
+                                                        unsigned *ctr, int64_t *total, int64_t *maxdeg,
+                                                        uint32_t epoch)
 {
   __shared__ uint64_t sh[4];
   __shared__ uint64_t s_excl;
@@ -597,6 +598,12 @@ __global__ __launch_bounds__(256) void k_contract_small(int64_t nc, const OffT *
     } else {
       deg = cdeg_large[c];
     }
+  }
+  // longest coarse row: one atomic per wave, only when it raises the maximum
+  {
+    const int64_t dm = wave_max<int64_t>(deg);
+    if (lane() == 0 && dm > __hip_atomic_load(maxdeg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+      atomicMax((unsigned long long *)maxdeg, (unsigned long long)dm);
   }
   uint64_t tot;
   const uint64_t ex = block256_excl_scan<uint64_t>((uint64_t)deg, tot, sh);
@@ -628,14 +635,52 @@ __global__ __launch_bounds__(256) void k_contract_small(int64_t nc, const OffT *
 template <typename OffT>
 static int64_t contract_impl(cm_ctx *ctx, const Level &L, const int32_t *match, const int32_t *cmap,
                              int64_t nc, const int32_t *rep, const OffT *segoff, int row_order, OffT *cxadj,
-                             int32_t *cadjncy, int32_t *cadjwgt)
+                             int32_t *cadjncy, int32_t *cadjwgt, int64_t *maxdeg, bool force_sorted = false)
 {
   const int64_t n = L.n, m = L.m;
   const OffT *xadj = (const OffT *)L.xadj;
   const bool fs = row_order == CM_ROW_FIRST_SEEN;
   if (nc == 0) {
     fill_u32(ctx, cxadj, 0u, sizeof(OffT) / 4);
+    if (maxdeg) *maxdeg = 0;
     return 0;
+  }
+  if (fs && !force_sorted) {
+    // segments above REG_SEG (they exist only if 2 * maxdeg > REG_SEG) by
+    // hashing, the rest in registers; one host synchronisation at the end
+    const bool hashed = L.maxdeg < 0 || 2 * L.maxdeg > REG_SEG;
+    int32_t *cdeg = (int32_t *)ctx->pool.alloc(sizeof(int32_t) * nc);
+    HashContract hc;
+    try {
+      if (hashed) hash_prepare_impl<OffT>(ctx, L, match, cmap, nc, rep, segoff, cdeg, hc);
+      const int64_t tiles = (nc + 255) / 256;
+      const uint32_t ep = lookback_begin(ctx, tiles);
+      fill_u32(ctx, ctx->d_small + 4, 0u, 2);
+      const double small_bytes = 8.0 * m + 12.0 * n + (double)sizeof(OffT) * (n + 1) + 8.0 * nc +
+                                 (double)sizeof(OffT) * (nc + 1);
+      launch(ctx, "contract_small", small_bytes, k_contract_small<true, OffT>, dim3((unsigned)tiles), dim3(256),
+             0, nc, segoff, rep, match, xadj, L.adjncy, L.adjwgt, cmap, (const int32_t *)cdeg, cxadj, cadjncy,
+             cadjwgt, ctx->d_status, ctx->d_ctr, ctx->d_small + 1, ctx->d_small + 4, ep);
+      ctx->h_small[6] = 0;
+      if (hashed) hash_emit_impl<OffT>(ctx, segoff, cdeg, cxadj, cadjncy, cadjwgt, hc);
+      CM_HIP(hipMemcpyAsync(ctx->h_small + 1, ctx->d_small + 1, 4 * sizeof(int64_t), hipMemcpyDeviceToHost,
+                            ctx->stream));
+      sync(ctx);
+    } catch (...) {
+      hash_release(ctx, hc);
+      ctx->pool.release(cdeg);
+      throw;
+    }
+    const int64_t mc = ctx->h_small[1];
+    const int64_t md = ctx->h_small[4];
+    const bool overflow = hashed && (uint32_t)ctx->h_small[6] != 0;
+    hash_release(ctx, hc);
+    ctx->pool.release(cdeg);
+    if (overflow)  // a hub partition held more distinct keys than its table
+      return contract_impl<OffT>(ctx, L, match, cmap, nc, rep, segoff, row_order, cxadj, cadjncy, cadjwgt,
+                                 maxdeg, true);
+    if (maxdeg) *maxdeg = md;
+    return mc;
   }
   // size classes of the gather segments: only segments above REG_SEG
   // entries go through the sorted path
@@ -688,16 +733,17 @@ static int64_t contract_impl(cm_ctx *ctx, const Level &L, const int32_t *match, 
   // fused short-segment merge + look-back cxadj + direct row writes
   const int64_t tiles = (nc + 255) / 256;
   const uint32_t ep = lookback_begin(ctx, tiles);
+  fill_u32(ctx, ctx->d_small + 4, 0u, 2);
   const double small_bytes = 8.0 * m + 12.0 * n + (double)sizeof(OffT) * (n + 1) + 8.0 * nc +
                              (double)sizeof(OffT) * (nc + 1);
   if (fs)
     launch(ctx, "contract_small", small_bytes, k_contract_small<true, OffT>, dim3((unsigned)tiles), dim3(256),
            0, nc, segoff, rep, match, xadj, L.adjncy, L.adjwgt, cmap, (const int32_t *)cdeg, cxadj, cadjncy,
-           cadjwgt, ctx->d_status, ctx->d_ctr, ctx->d_small + 1, ep);
+           cadjwgt, ctx->d_status, ctx->d_ctr, ctx->d_small + 1, ctx->d_small + 4, ep);
   else
     launch(ctx, "contract_small", small_bytes, k_contract_small<false, OffT>, dim3((unsigned)tiles),
            dim3(256), 0, nc, segoff, rep, match, xadj, L.adjncy, L.adjwgt, cmap, (const int32_t *)cdeg, cxadj,
-           cadjncy, cadjwgt, ctx->d_status, ctx->d_ctr, ctx->d_small + 1, ep);
+           cadjncy, cadjwgt, ctx->d_status, ctx->d_ctr, ctx->d_small + 1, ctx->d_small + 4, ep);
   if (sorted_path) {
 #define CM_EMIT(FSV)                                                                                      \
   if (nmid)                                                                                              \
@@ -715,10 +761,11 @@ static int64_t contract_impl(cm_ctx *ctx, const Level &L, const int32_t *match, 
     }
 #undef CM_EMIT
   }
-  CM_HIP(hipMemcpyAsync(ctx->h_small + 1, ctx->d_small + 1, sizeof(int64_t), hipMemcpyDeviceToHost,
+  CM_HIP(hipMemcpyAsync(ctx->h_small + 1, ctx->d_small + 1, 4 * sizeof(int64_t), hipMemcpyDeviceToHost,
                         ctx->stream));
   sync(ctx);
   const int64_t mc = ctx->h_small[1];
+  if (maxdeg) *maxdeg = ctx->h_small[4];
   release_bins(ctx, bins);
   ctx->pool.release(cdeg);
   if (sorted_path) {
@@ -736,12 +783,12 @@ static int64_t contract_impl(cm_ctx *ctx, const Level &L, const int32_t *match, 
 
 int64_t contract(cm_ctx *ctx, const Level &L, const int32_t *d_match, const int32_t *d_cmap,
                  int64_t nc, const int32_t *d_rep, const void *d_segoff, int row_order,
-                 void *d_cxadj, int32_t *d_cadjncy, int32_t *d_cadjwgt)
+                 void *d_cxadj, int32_t *d_cadjncy, int32_t *d_cadjwgt, int64_t *maxdeg)
 {
   return L.off64 ? contract_impl(ctx, L, d_match, d_cmap, nc, d_rep, (const int64_t *)d_segoff, row_order,
-                                 (int64_t *)d_cxadj, d_cadjncy, d_cadjwgt)
+                                 (int64_t *)d_cxadj, d_cadjncy, d_cadjwgt, maxdeg)
                  : contract_impl(ctx, L, d_match, d_cmap, nc, d_rep, (const int32_t *)d_segoff, row_order,
-                                 (int32_t *)d_cxadj, d_cadjncy, d_cadjwgt);
+                                 (int32_t *)d_cxadj, d_cadjncy, d_cadjwgt, maxdeg);
 }
 
 // --------------------------------------------------------- (d) projection

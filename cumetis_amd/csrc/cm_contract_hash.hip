@@ -1,0 +1,633 @@
+// cm_contract_hash.hip -- (c) first-seen contraction of the coarse segments
+// above REG_SEG entries, by hashing instead of sorting.
+//
+// A coarse segment lists the rows of the members of one coarse vertex, lower
+// fine id first, as the reference's contraction walks them (cu:766-808).  Its
+// first-seen row keeps, for every distinct coarse neighbour, the position of
+// the first occurrence and the sum of the weights; the row lists these heads
+// in position order, and the head of the coarse vertex itself (edges inside
+// the pair) is replaced by the last head (cu:804-808).  The sort-based path
+// derives the same rows from a segmented sort; here the heads of a segment
+// come from a hash table in LDS (key, first position, weight sum):
+//   select  segments above REG_SEG: a chunk list (HC_CHUNK entries per chunk)
+//           and items by length -- up to HC_WAVE_MAX one wave, up to
+//           HC_BLOCK_MAX one block, longer ones ("hubs") one block per key
+//           partition of about HC_PART entries;
+//   gather  per chunk: coarse neighbour ids and weights in segment order;
+//   dedup   wave / block items: table, heads ranked by position, the row
+//           written compacted into a staging area and its length to cdeg;
+//           hub partitions: head flag and weight sum per entry;
+//   hubs    count per chunk, scan per hub, compacted write per chunk;
+//   emit    per chunk, once the row offsets are known: staging -> rows.
+// Every item count stays on the device and fixed grids stride over it, so
+// the path needs no host synchronisation.  A hub partition whose table fills
+// (more than HC_SLOTS distinct keys, far beyond the expected HC_PART) sets a
+// flag; the caller then redoes the level on the sort-based path.
+#include "cm_common.hpp"
+#include "cm_contract_hash.hpp"
+#include "cm_internal.hpp"
+
+namespace cm {
+
+constexpr int HC_CHUNK = 2048;
+constexpr int HC_WAVE_MAX = 256;
+constexpr int HC_WAVE_SLOTS = 512;  // >= 2 * HC_WAVE_MAX
+constexpr int HC_BLOCK_MAX = 2048;
+constexpr int HC_SLOTS = 4096;      // >= 2 * HC_BLOCK_MAX
+constexpr int HC_PART = 2048;
+constexpr int HC_SELECT = 16;       // segments per thread in the select pass
+
+struct SegChunk {
+  int32_t c, first, nch, k;  // chunk k of the nch chunks of segment c, from chunk `first`
+};
+struct HubPart {
+  int32_t c, p, P, pad;  // key partition p of P of hub segment c
+};
+
+// item counters
+enum { HC_N_CHUNK = 0, HC_N_WAVE, HC_N_BLOCK, HC_N_PART, HC_N_HUB, HC_N_COUNTERS };
+
+__device__ __forceinline__ void wave_sync()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ int ceil_log2(int x)
+{
+  int l = 0;
+  while ((1 << l) < x) ++l;
+  return l;
+}
+
+// Fibonacci hashing of a coarse id into a table of 1 << lg slots.
+__device__ __forceinline__ uint32_t hc_slot0(int32_t key, int lg)
+{
+  return ((uint32_t)key * 0x9E3779B1u) >> (32 - lg);
+}
+
+// Key partition of a hub entry: a second multiplicative hash scaled to P.
+__device__ __forceinline__ int hc_part(int32_t key, int P)
+{
+  return (int)(((uint64_t)((uint32_t)key * 0x85EBCA77u) * (uint32_t)P) >> 32);
+}
+
+// Slot of `key` (keys >= 0, -1 = empty), inserting it if absent; -1 when
+// the table is full.  Linear probing with compare-and-swap in LDS.
+__device__ __forceinline__ int hc_insert(int32_t *tk, int lg, int32_t key)
+{
+  const uint32_t mask = (1u << lg) - 1;
+  uint32_t h = hc_slot0(key, lg);
+  for (uint32_t probe = 0; probe <= mask; ++probe) {
+    const int32_t old = atomicCAS(&tk[h], -1, key);
+    if (old == -1 || old == key) return (int)h;
+    h = (h + 1) & mask;
+  }
+  return -1;
+}
+
+// Slot of a key already inserted (the table no longer changes); -1 if absent.
+__device__ __forceinline__ int hc_find(const int32_t *tk, int lg, int32_t key)
+{
+  const uint32_t mask = (1u << lg) - 1;
+  uint32_t h = hc_slot0(key, lg);
+  for (uint32_t probe = 0; probe <= mask; ++probe) {
+    const int32_t k = tk[h];
+    if (k == key) return (int)h;
+    if (k == -1) return -1;
+    h = (h + 1) & mask;
+  }
+  return -1;
+}
+
+// ---------------------------------------------------------------- select
+template <typename OffT>
+__global__ __launch_bounds__(256) void k_hc_select(int64_t nc, const OffT *__restrict__ segoff,
+                                                   SegChunk *chunks, int32_t *wave_items, int32_t *block_items,
+                                                   HubPart *parts, int32_t *hubs, unsigned *ctr)
+{
+  __shared__ unsigned s_w[HC_N_COUNTERS][4], s_base[HC_N_COUNTERS];
+  const int64_t c0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * HC_SELECT;
+  unsigned cnt[HC_N_COUNTERS] = {0, 0, 0, 0, 0};
+  for (int t = 0; t < HC_SELECT; ++t) {
+    const int64_t c = c0 + t;
+    if (c >= nc) break;
+    const int64_t len = (int64_t)segoff[c + 1] - (int64_t)segoff[c];
+    if (len <= REG_SEG) continue;
+    cnt[HC_N_CHUNK] += (unsigned)((len + HC_CHUNK - 1) / HC_CHUNK);
+    if (len <= HC_WAVE_MAX) ++cnt[HC_N_WAVE];
+    else if (len <= HC_BLOCK_MAX) ++cnt[HC_N_BLOCK];
+    else {
+      ++cnt[HC_N_HUB];
+      cnt[HC_N_PART] += (unsigned)((len + HC_PART - 1) / HC_PART);
+    }
+  }
+  const int w = threadIdx.x >> 6;
+  unsigned inc[HC_N_COUNTERS];
+#pragma unroll
+  for (int i = 0; i < HC_N_COUNTERS; ++i) {
+    inc[i] = wave_incl_scan<unsigned>(cnt[i]);
+    if (lane() == CM_WAVE - 1) s_w[i][w] = inc[i];
+  }
+  __syncthreads();
+  if (threadIdx.x < HC_N_COUNTERS) {
+    const int i = threadIdx.x;
+    const unsigned tot = s_w[i][0] + s_w[i][1] + s_w[i][2] + s_w[i][3];
+    s_base[i] = tot ? atomicAdd(&ctr[i], tot) : 0u;
+  }
+  __syncthreads();
+  unsigned at[HC_N_COUNTERS];
+#pragma unroll
+  for (int i = 0; i < HC_N_COUNTERS; ++i) {
+    at[i] = s_base[i] + inc[i] - cnt[i];
+    for (int j = 0; j < w; ++j) at[i] += s_w[i][j];
+  }
+  for (int t = 0; t < HC_SELECT; ++t) {
+    const int64_t c = c0 + t;
+    if (c >= nc) break;
+    const int64_t len = (int64_t)segoff[c + 1] - (int64_t)segoff[c];
+    if (len <= REG_SEG) continue;
+    const int nch = (int)((len + HC_CHUNK - 1) / HC_CHUNK);
+    const unsigned first = at[HC_N_CHUNK];
+    for (int k = 0; k < nch; ++k) chunks[first + k] = SegChunk{(int32_t)c, (int32_t)first, nch, k};
+    at[HC_N_CHUNK] += nch;
+    if (len <= HC_WAVE_MAX) {
+      wave_items[at[HC_N_WAVE]++] = (int32_t)c;
+    } else if (len <= HC_BLOCK_MAX) {
+      block_items[at[HC_N_BLOCK]++] = (int32_t)c;
+    } else {
+      hubs[at[HC_N_HUB]++] = (int32_t)first;  // a hub is named by its first chunk
+      const int P = (int)((len + HC_PART - 1) / HC_PART);
+      for (int p = 0; p < P; ++p) parts[at[HC_N_PART] + p] = HubPart{(int32_t)c, p, P, 0};
+      at[HC_N_PART] += P;
+    }
+  }
+}
+
+// ---------------------------------------------------------------- gather
+// One wave per chunk; SCAN entries per lane per step so that the row loads
+// and the cmap gathers behind them overlap.
+template <typename OffT>
+__global__ __launch_bounds__(256) void k_hc_gather(const SegChunk *__restrict__ chunks, const unsigned *ctr,
+                                                   const OffT *__restrict__ segoff, const int32_t *__restrict__ rep,
+                                                   const int32_t *__restrict__ match, const OffT *__restrict__ xadj,
+                                                   const int32_t *__restrict__ adjncy,
+                                                   const int32_t *__restrict__ adjwgt,
+                                                   const int32_t *__restrict__ cmap, int32_t *__restrict__ cval,
+                                                   int32_t *__restrict__ wv)
+{
+  constexpr int SCAN = 4;
+  const unsigned nq = ctr[HC_N_CHUNK];
+  const int l = lane();
+  const int64_t nw = (int64_t)gridDim.x * (blockDim.x / CM_WAVE);
+  for (int64_t q = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / CM_WAVE; q < nq; q += nw) {
+    const SegChunk ch = chunks[q];
+    const int64_t s0 = (int64_t)segoff[ch.c];
+    const int64_t len = (int64_t)segoff[ch.c + 1] - s0;
+    const int32_t v = rep[ch.c], u = match[v];
+    const int64_t b0 = (int64_t)xadj[v], d0 = (int64_t)xadj[v + 1] - b0;
+    const int64_t b1 = u != v ? (int64_t)xadj[u] : 0;
+    const int64_t i0 = (int64_t)ch.k * HC_CHUNK, i1 = len < i0 + HC_CHUNK ? len : i0 + HC_CHUNK;
+    for (int64_t ib = i0 + l; ib < i1; ib += SCAN * CM_WAVE) {
+      int32_t a[SCAN], w[SCAN];
+#pragma unroll
+      for (int t = 0; t < SCAN; ++t) {
+        const int64_t i = ib + (int64_t)t * CM_WAVE;
+        const int64_t j = i < d0 ? b0 + i : b1 + (i - d0);
+        a[t] = i < i1 ? adjncy[j] : 0;
+        w[t] = i < i1 ? adjwgt[j] : 0;
+      }
+#pragma unroll
+      for (int t = 0; t < SCAN; ++t) {
+        const int64_t i = ib + (int64_t)t * CM_WAVE;
+        if (i < i1) {
+          cval[s0 + i] = cmap[a[t]];
+          wv[s0 + i] = w[t];
+        }
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------ dedup: one wave
+// Segments of REG_SEG+1 .. HC_WAVE_MAX entries, one per wave, a table of
+// HC_WAVE_SLOTS slots per wave.
+template <typename OffT>
+__global__ __launch_bounds__(256) void k_hc_dedup_wave(const int32_t *__restrict__ items, const unsigned *ctr,
+                                                       const OffT *__restrict__ segoff,
+                                                       const int32_t *__restrict__ cval,
+                                                       const int32_t *__restrict__ wv, int32_t *__restrict__ tc,
+                                                       int32_t *__restrict__ tw, int32_t *__restrict__ cdeg)
+{
+  constexpr int PER = HC_WAVE_MAX / CM_WAVE;
+  __shared__ int32_t s_key[4][HC_WAVE_SLOTS], s_pos[4][HC_WAVE_SLOTS], s_sum[4][HC_WAVE_SLOTS];
+  const int w = threadIdx.x >> 6, l = lane();
+  int32_t *tk = s_key[w], *tp = s_pos[w], *ts = s_sum[w];
+  const unsigned n = ctr[HC_N_WAVE];
+  const int64_t nw = (int64_t)gridDim.x * 4;
+  for (int64_t q = (int64_t)blockIdx.x * 4 + w; q < n; q += nw) {
+    const int32_t c = items[q];
+    const int64_t s0 = (int64_t)segoff[c];
+    const int len = (int)((int64_t)segoff[c + 1] - s0);
+    int lg = ceil_log2(2 * len);
+    lg = lg < 6 ? 6 : lg;
+    for (int i = l; i < (1 << lg); i += CM_WAVE) {
+      tk[i] = -1;
+      tp[i] = 0x7fffffff;
+      ts[i] = 0;
+    }
+    wave_sync();
+    int32_t cu[PER], sl[PER];
+#pragma unroll
+    for (int t = 0; t < PER; ++t) {
+      const int i = l + t * CM_WAVE;
+      cu[t] = i < len ? cval[s0 + i] : -1;
+      const int32_t wt = i < len ? wv[s0 + i] : 0;
+      sl[t] = -1;
+      if (i < len) {
+        sl[t] = hc_insert(tk, lg, cu[t]);
+        atomicMin(&tp[sl[t]], i);
+        atomicAdd(&ts[sl[t]], wt);
+      }
+    }
+    wave_sync();
+    int running = 0, self_rank = -1, rk[PER];
+    bool hd[PER];
+#pragma unroll
+    for (int t = 0; t < PER; ++t) {
+      const int i = l + t * CM_WAVE;
+      hd[t] = i < len && tp[sl[t]] == i;
+      const uint64_t mk = __ballot(hd[t]);
+      rk[t] = running + mask_rank(mk);
+      const uint64_t sm = __ballot(hd[t] && cu[t] == c);
+      if (sm) self_rank = __shfl(rk[t], __ffsll((unsigned long long)sm) - 1, CM_WAVE);
+      running += (int)__popcll(mk);
+    }
+#pragma unroll
+    for (int t = 0; t < PER; ++t) {
+      if (hd[t] && cu[t] != c) {
+        int r = rk[t];
+        if (self_rank >= 0 && r == running - 1) r = self_rank;
+        tc[s0 + r] = cu[t];
+        tw[s0 + r] = ts[sl[t]];
+      }
+    }
+    if (l == 0) cdeg[c] = running - (self_rank >= 0 ? 1 : 0);
+    wave_sync();
+  }
+}
+
+// ----------------------------------------------------- dedup: one block
+// Segments of HC_WAVE_MAX+1 .. HC_BLOCK_MAX entries, one per block.
+template <typename OffT>
+__global__ __launch_bounds__(256) void k_hc_dedup_block(const int32_t *__restrict__ items, const unsigned *ctr,
+                                                        const OffT *__restrict__ segoff,
+                                                        const int32_t *__restrict__ cval,
+                                                        const int32_t *__restrict__ wv, int32_t *__restrict__ tc,
+                                                        int32_t *__restrict__ tw, int32_t *__restrict__ cdeg)
+{
+  constexpr int PER = HC_BLOCK_MAX / 256;
+  __shared__ int32_t tk[HC_SLOTS], tp[HC_SLOTS], ts[HC_SLOTS];
+  __shared__ int sh[4];
+  __shared__ int s_self;
+  const unsigned n = ctr[HC_N_BLOCK];
+  for (int64_t q = blockIdx.x; q < n; q += gridDim.x) {
+    const int32_t c = items[q];
+    const int64_t s0 = (int64_t)segoff[c];
+    const int len = (int)((int64_t)segoff[c + 1] - s0);
+    const int lg = ceil_log2(2 * len);
+    for (int i = threadIdx.x; i < (1 << lg); i += 256) {
+      tk[i] = -1;
+      tp[i] = 0x7fffffff;
+      ts[i] = 0;
+    }
+    if (threadIdx.x == 0) s_self = -1;
+    __syncthreads();
+    int32_t cu[PER], sl[PER];
+#pragma unroll
+    for (int t = 0; t < PER; ++t) {
+      const int i = threadIdx.x + t * 256;
+      cu[t] = i < len ? cval[s0 + i] : -1;
+      const int32_t wt = i < len ? wv[s0 + i] : 0;
+      sl[t] = -1;
+      if (i < len) {
+        sl[t] = hc_insert(tk, lg, cu[t]);
+        atomicMin(&tp[sl[t]], i);
+        atomicAdd(&ts[sl[t]], wt);
+      }
+    }
+    __syncthreads();
+    int running = 0, rk[PER];
+    bool hd[PER];
+    const int steps = (len + 255) / 256;  // block-uniform
+#pragma unroll
+    for (int t = 0; t < PER; ++t) {
+      hd[t] = false;
+      rk[t] = 0;
+      if (t < steps) {
+        const int i = threadIdx.x + t * 256;
+        hd[t] = i < len && tp[sl[t]] == i;
+        int tot;
+        rk[t] = running + block256_excl_scan<int>(hd[t] ? 1 : 0, tot, sh);
+        if (hd[t] && cu[t] == c) s_self = rk[t];
+        running += tot;
+      }
+    }
+    __syncthreads();
+    const int self_rank = s_self;
+#pragma unroll
+    for (int t = 0; t < PER; ++t) {
+      if (hd[t] && cu[t] != c) {
+        int r = rk[t];
+        if (self_rank >= 0 && r == running - 1) r = self_rank;
+        tc[s0 + r] = cu[t];
+        tw[s0 + r] = ts[sl[t]];
+      }
+    }
+    if (threadIdx.x == 0) cdeg[c] = running - (self_rank >= 0 ? 1 : 0);
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------- dedup: hub partitions
+// One block per key partition of a hub segment: the block reads the whole
+// segment and tables the keys of its partition; each entry of the partition
+// gets its head flag and, for heads, the weight sum.
+template <typename OffT>
+__global__ __launch_bounds__(256) void k_hc_dedup_hub(const HubPart *__restrict__ parts, const unsigned *ctr,
+                                                      const OffT *__restrict__ segoff,
+                                                      const int32_t *__restrict__ cval,
+                                                      const int32_t *__restrict__ wv, uint8_t *__restrict__ head,
+                                                      int32_t *__restrict__ hw, unsigned *overflow)
+{
+  constexpr int lg = 12;  // HC_SLOTS
+  constexpr int SCAN = 4;
+  __shared__ int32_t tk[HC_SLOTS], tp[HC_SLOTS], ts[HC_SLOTS];
+  __shared__ int s_full;
+  const unsigned n = ctr[HC_N_PART];
+  for (int64_t q = blockIdx.x; q < n; q += gridDim.x) {
+    const HubPart hp = parts[q];
+    const int64_t s0 = (int64_t)segoff[hp.c];
+    const int64_t len = (int64_t)segoff[hp.c + 1] - s0;
+    for (int i = threadIdx.x; i < HC_SLOTS; i += 256) {
+      tk[i] = -1;
+      tp[i] = 0x7fffffff;
+      ts[i] = 0;
+    }
+    if (threadIdx.x == 0) s_full = 0;
+    __syncthreads();
+    for (int64_t ib = threadIdx.x; ib < len; ib += SCAN * 256) {
+      int32_t cu[SCAN];
+#pragma unroll
+      for (int t = 0; t < SCAN; ++t) {
+        const int64_t i = ib + (int64_t)t * 256;
+        cu[t] = i < len ? cval[s0 + i] : -1;
+      }
+#pragma unroll
+      for (int t = 0; t < SCAN; ++t) {
+        const int64_t i = ib + (int64_t)t * 256;
+        if (cu[t] < 0 || hc_part(cu[t], hp.P) != hp.p) continue;
+        const int s = hc_insert(tk, lg, cu[t]);
+        if (s < 0) {
+          s_full = 1;
+          continue;
+        }
+        atomicMin(&tp[s], (int32_t)i);
+        atomicAdd(&ts[s], wv[s0 + i]);
+      }
+    }
+    __syncthreads();
+    if (s_full && threadIdx.x == 0) atomicOr(overflow, 1u);
+    for (int64_t i = threadIdx.x; i < len; i += 256) {
+      const int32_t cu = cval[s0 + i];
+      if (hc_part(cu, hp.P) != hp.p) continue;
+      const int s = hc_find(tk, lg, cu);
+      const bool h = s >= 0 && tp[s] == (int32_t)i;
+      head[s0 + i] = h ? 1 : 0;
+      hw[s0 + i] = h ? ts[s] : 0;
+    }
+    __syncthreads();
+  }
+}
+
+// Hub chunks: heads per chunk and the rank of the self head inside it.
+template <typename OffT>
+__global__ __launch_bounds__(256) void k_hc_hub_count(const SegChunk *__restrict__ chunks, const unsigned *ctr,
+                                                      const OffT *__restrict__ segoff,
+                                                      const uint8_t *__restrict__ head,
+                                                      const int32_t *__restrict__ cval, int32_t *__restrict__ ccnt,
+                                                      int32_t *__restrict__ cself)
+{
+  const unsigned nq = ctr[HC_N_CHUNK];
+  const int l = lane();
+  const int64_t nw = (int64_t)gridDim.x * (blockDim.x / CM_WAVE);
+  for (int64_t q = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / CM_WAVE; q < nq; q += nw) {
+    const SegChunk ch = chunks[q];
+    const int64_t s0 = (int64_t)segoff[ch.c];
+    const int64_t len = (int64_t)segoff[ch.c + 1] - s0;
+    if (len <= HC_BLOCK_MAX) continue;
+    const int64_t i0 = (int64_t)ch.k * HC_CHUNK, i1 = len < i0 + HC_CHUNK ? len : i0 + HC_CHUNK;
+    int cnt = 0, self = -1;
+    for (int64_t ib = i0; ib < i1; ib += CM_WAVE) {
+      const int64_t i = ib + l;
+      const bool h = i < i1 && head[s0 + i];
+      const uint64_t mk = __ballot(h);
+      const uint64_t sm = __ballot(h && cval[s0 + i] == ch.c);
+      if (sm) self = cnt + (int)__popcll(mk & ((1ull << (__ffsll((unsigned long long)sm) - 1)) - 1));
+      cnt += (int)__popcll(mk);
+    }
+    if (l == 0) {
+      ccnt[q] = cnt;
+      cself[q] = self;
+    }
+  }
+}
+
+// One wave per hub: chunk offsets, row length, rank of the self head.
+template <typename OffT>
+__global__ __launch_bounds__(256) void k_hc_hub_scan(const int32_t *__restrict__ hubs, const unsigned *ctr,
+                                                     const SegChunk *__restrict__ chunks,
+                                                     const int32_t *__restrict__ ccnt,
+                                                     const int32_t *__restrict__ cself, int32_t *__restrict__ coff,
+                                                     int32_t *__restrict__ hself, int32_t *__restrict__ htot,
+                                                     int32_t *__restrict__ cdeg)
+{
+  const unsigned nh = ctr[HC_N_HUB];
+  const int l = lane();
+  const int64_t nw = (int64_t)gridDim.x * (blockDim.x / CM_WAVE);
+  for (int64_t h = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / CM_WAVE; h < nh; h += nw) {
+    const int32_t first = hubs[h];
+    const SegChunk c0 = chunks[first];
+    int carry = 0, self = -1;
+    for (int k0 = 0; k0 < c0.nch; k0 += CM_WAVE) {
+      const int k = k0 + l;
+      const int x = k < c0.nch ? ccnt[first + k] : 0;
+      const int inc = wave_incl_scan<int>(x);
+      const int off = carry + inc - x;
+      if (k < c0.nch) {
+        coff[first + k] = off;
+        const int sr = cself[first + k];
+        if (sr >= 0) self = off + sr;
+      }
+      carry += __shfl(inc, CM_WAVE - 1, CM_WAVE);
+    }
+    self = wave_max<int>(self);
+    if (l == 0) {
+      hself[first] = self;
+      htot[first] = carry;
+      cdeg[c0.c] = carry - (self >= 0 ? 1 : 0);
+    }
+  }
+}
+
+// Hub chunks: heads written compacted in position order, the self head
+// dropped and replaced by the last head.
+template <typename OffT>
+__global__ __launch_bounds__(256) void k_hc_hub_write(const SegChunk *__restrict__ chunks, const unsigned *ctr,
+                                                      const OffT *__restrict__ segoff,
+                                                      const uint8_t *__restrict__ head,
+                                                      const int32_t *__restrict__ cval, const int32_t *__restrict__ hw,
+                                                      const int32_t *__restrict__ coff,
+                                                      const int32_t *__restrict__ hself,
+                                                      const int32_t *__restrict__ htot, int32_t *__restrict__ tc,
+                                                      int32_t *__restrict__ tw)
+{
+  const unsigned nq = ctr[HC_N_CHUNK];
+  const int l = lane();
+  const int64_t nw = (int64_t)gridDim.x * (blockDim.x / CM_WAVE);
+  for (int64_t q = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / CM_WAVE; q < nq; q += nw) {
+    const SegChunk ch = chunks[q];
+    const int64_t s0 = (int64_t)segoff[ch.c];
+    const int64_t len = (int64_t)segoff[ch.c + 1] - s0;
+    if (len <= HC_BLOCK_MAX) continue;
+    const int self = hself[ch.first], tot = htot[ch.first];
+    const int64_t i0 = (int64_t)ch.k * HC_CHUNK, i1 = len < i0 + HC_CHUNK ? len : i0 + HC_CHUNK;
+    int r0 = coff[q];
+    for (int64_t ib = i0; ib < i1; ib += CM_WAVE) {
+      const int64_t i = ib + l;
+      const bool h = i < i1 && head[s0 + i];
+      const uint64_t mk = __ballot(h);
+      if (h) {
+        const int32_t cu = cval[s0 + i];
+        if (cu != ch.c) {
+          int r = r0 + mask_rank(mk);
+          if (self >= 0 && r == tot - 1) r = self;
+          tc[s0 + r] = cu;
+          tw[s0 + r] = hw[s0 + i];
+        }
+      }
+      r0 += (int)__popcll(mk);
+    }
+  }
+}
+
+// ------------------------------------------------------------------ emit
+template <typename OffT>
+__global__ __launch_bounds__(256) void k_hc_emit(const SegChunk *__restrict__ chunks, const unsigned *ctr,
+                                                 const OffT *__restrict__ segoff, const int32_t *__restrict__ cdeg,
+                                                 const OffT *__restrict__ cxadj, const int32_t *__restrict__ tc,
+                                                 const int32_t *__restrict__ tw, int32_t *__restrict__ cadjncy,
+                                                 int32_t *__restrict__ cadjwgt)
+{
+  const unsigned nq = ctr[HC_N_CHUNK];
+  const int l = lane();
+  const int64_t nw = (int64_t)gridDim.x * (blockDim.x / CM_WAVE);
+  for (int64_t q = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / CM_WAVE; q < nq; q += nw) {
+    const SegChunk ch = chunks[q];
+    const int64_t s0 = (int64_t)segoff[ch.c], o0 = (int64_t)cxadj[ch.c];
+    const int64_t d = cdeg[ch.c];
+    const int64_t i0 = (int64_t)ch.k * HC_CHUNK, i1 = d < i0 + HC_CHUNK ? d : i0 + HC_CHUNK;
+    for (int64_t i = i0 + l; i < i1; i += CM_WAVE) {
+      cadjncy[o0 + i] = tc[s0 + i];
+      cadjwgt[o0 + i] = tw[s0 + i];
+    }
+  }
+}
+
+// ------------------------------------------------------------- host side
+template <typename OffT>
+void hash_prepare_impl(cm_ctx *ctx, const Level &L, const int32_t *match, const int32_t *cmap, int64_t nc,
+                       const int32_t *rep, const OffT *segoff, int32_t *cdeg, HashContract &hc)
+{
+  const int64_t m = L.m;
+  const OffT *xadj = (const OffT *)L.xadj;
+  // bounds of the item counts (segments above REG_SEG hold at most m entries)
+  const int64_t nseg = std::min<int64_t>(nc, m / (REG_SEG + 1)) + 1;
+  const int64_t nchunk = m / HC_CHUNK + nseg;
+  const int64_t nblk = std::min<int64_t>(nc, m / (HC_WAVE_MAX + 1)) + 1;
+  const int64_t nhub = m / (HC_BLOCK_MAX + 1) + 1;
+  const int64_t npart = m / HC_PART + nhub;
+  auto al = [&](size_t bytes) {
+    void *p = ctx->pool.alloc(bytes ? bytes : 1);
+    hc.blocks.push_back(p);
+    return p;
+  };
+  hc.chunks = al(sizeof(SegChunk) * nchunk);
+  int32_t *wave_items = (int32_t *)al(4 * nseg), *block_items = (int32_t *)al(4 * nblk);
+  HubPart *parts = (HubPart *)al(sizeof(HubPart) * npart);
+  int32_t *hubs = (int32_t *)al(4 * nhub);
+  hc.ctr = (unsigned *)al(4 * 8);
+  int32_t *cval = (int32_t *)al(4 * m), *wv = (int32_t *)al(4 * m);
+  hc.tc = (int32_t *)al(4 * m);
+  hc.tw = (int32_t *)al(4 * m);
+  fill_u32(ctx, hc.ctr, 0u, 8);
+  const SegChunk *chunks = (const SegChunk *)hc.chunks;
+  launch(ctx, "hc_select", 0.0, k_hc_select<OffT>, dim3(nblocks(nc, 256 * HC_SELECT)), dim3(256), 0, nc, segoff,
+         (SegChunk *)hc.chunks, wave_items, block_items, parts, hubs, hc.ctr);
+  const unsigned gchunk = (unsigned)std::min<int64_t>(4096, (nchunk + 3) / 4);
+  hc.gchunk = gchunk;
+  launch(ctx, "hc_gather", 0.0, k_hc_gather<OffT>, dim3(gchunk), dim3(256), 0, chunks, (const unsigned *)hc.ctr,
+         segoff, rep, match, xadj, L.adjncy, L.adjwgt, cmap, cval, wv);
+  launch(ctx, "hc_dedup_wave", 0.0, k_hc_dedup_wave<OffT>, dim3((unsigned)std::min<int64_t>(4096, (nseg + 3) / 4)),
+         dim3(256), 0, (const int32_t *)wave_items, (const unsigned *)hc.ctr, segoff, (const int32_t *)cval,
+         (const int32_t *)wv, hc.tc, hc.tw, cdeg);
+  launch(ctx, "hc_dedup_block", 0.0, k_hc_dedup_block<OffT>, dim3((unsigned)std::min<int64_t>(2048, nblk)),
+         dim3(256), 0, (const int32_t *)block_items, (const unsigned *)hc.ctr, segoff, (const int32_t *)cval,
+         (const int32_t *)wv, hc.tc, hc.tw, cdeg);
+  if (m > HC_BLOCK_MAX) {  // a hub needs more than HC_BLOCK_MAX entries
+    uint8_t *head = (uint8_t *)al(m);
+    int32_t *hw = (int32_t *)al(4 * m);
+    int32_t *ccnt = (int32_t *)al(4 * nchunk), *cself = (int32_t *)al(4 * nchunk), *coff = (int32_t *)al(4 * nchunk);
+    int32_t *hself = (int32_t *)al(4 * nchunk), *htot = (int32_t *)al(4 * nchunk);
+    launch(ctx, "hc_dedup_hub", 0.0, k_hc_dedup_hub<OffT>, dim3((unsigned)std::min<int64_t>(2048, npart)),
+           dim3(256), 0, (const HubPart *)parts, (const unsigned *)hc.ctr, segoff, (const int32_t *)cval,
+           (const int32_t *)wv, head, hw, hc.ctr + 6);
+    launch(ctx, "hc_hub_count", 0.0, k_hc_hub_count<OffT>, dim3(gchunk), dim3(256), 0, chunks,
+           (const unsigned *)hc.ctr, segoff, (const uint8_t *)head, (const int32_t *)cval, ccnt, cself);
+    launch(ctx, "hc_hub_scan", 0.0, k_hc_hub_scan<OffT>, dim3((unsigned)std::min<int64_t>(1024, (nhub + 3) / 4)),
+           dim3(256), 0, (const int32_t *)hubs, (const unsigned *)hc.ctr, chunks, (const int32_t *)ccnt,
+           (const int32_t *)cself, coff, hself, htot, cdeg);
+    launch(ctx, "hc_hub_write", 0.0, k_hc_hub_write<OffT>, dim3(gchunk), dim3(256), 0, chunks,
+           (const unsigned *)hc.ctr, segoff, (const uint8_t *)head, (const int32_t *)cval, (const int32_t *)hw,
+           (const int32_t *)coff, (const int32_t *)hself, (const int32_t *)htot, hc.tc, hc.tw);
+  }
+}
+
+template <typename OffT>
+void hash_emit_impl(cm_ctx *ctx, const OffT *segoff, const int32_t *cdeg, const OffT *cxadj, int32_t *cadjncy,
+                    int32_t *cadjwgt, const HashContract &hc)
+{
+  launch(ctx, "hc_emit", 0.0, k_hc_emit<OffT>, dim3(hc.gchunk), dim3(256), 0, (const SegChunk *)hc.chunks,
+         (const unsigned *)hc.ctr, segoff, cdeg, cxadj, (const int32_t *)hc.tc, (const int32_t *)hc.tw, cadjncy,
+         cadjwgt);
+  // hub-table overflow flag -> h_small[6] with the caller's next read
+  CM_HIP(hipMemcpyAsync(ctx->h_small + 6, hc.ctr + 6, sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
+}
+
+template void hash_prepare_impl<int32_t>(cm_ctx *, const Level &, const int32_t *, const int32_t *, int64_t,
+                                         const int32_t *, const int32_t *, int32_t *, HashContract &);
+template void hash_prepare_impl<int64_t>(cm_ctx *, const Level &, const int32_t *, const int32_t *, int64_t,
+                                         const int32_t *, const int64_t *, int32_t *, HashContract &);
+template void hash_emit_impl<int32_t>(cm_ctx *, const int32_t *, const int32_t *, const int32_t *, int32_t *,
+                                      int32_t *, const HashContract &);
+template void hash_emit_impl<int64_t>(cm_ctx *, const int64_t *, const int32_t *, const int64_t *, int32_t *,
+                                      int32_t *, const HashContract &);
+
+void hash_release(cm_ctx *ctx, HashContract &hc)
+{
+  for (void *p : hc.blocks) ctx->pool.release(p);
+  hc.blocks.clear();
+}
+
+}  // namespace cm

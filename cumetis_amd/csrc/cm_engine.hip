@@ -261,6 +261,34 @@ static void validate_view(const cm_graph_view *g)
 
 // Level 0 of a hierarchy from a view: device views are referenced (unit
 // weights are materialised), host views are copied into pool memory.
+// *mx = max(*mx, longest row); per wave one atomic, and only when the wave
+// holds a longer row than the current maximum
+template <typename OffT>
+__global__ __launch_bounds__(256) void k_max_degree(int64_t n, const OffT *xadj, int64_t *mx)
+{
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t d = v < n ? (int64_t)(xadj[v + 1] - xadj[v]) : 0;
+  d = wave_max(d);
+  if (lane() == 0 && d > __hip_atomic_load(mx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+    atomicMax((unsigned long long *)mx, (unsigned long long)d);
+}
+
+// Queues the longest row of a CSR into d_small[4]; the caller reads it back
+// (h_small[4]) with its next synchronisation.
+void queue_max_degree(cm_ctx *ctx, int64_t n, bool off64, const void *xadj)
+{
+  fill_u32(ctx, ctx->d_small + 4, 0u, 2);
+  if (n > 0) {
+    if (off64)
+      launch(ctx, "max_degree", 0.0, k_max_degree<int64_t>, dim3(nblocks(n, 256)), dim3(256), 0, n,
+             (const int64_t *)xadj, ctx->d_small + 4);
+    else
+      launch(ctx, "max_degree", 0.0, k_max_degree<int32_t>, dim3(nblocks(n, 256)), dim3(256), 0, n,
+             (const int32_t *)xadj, ctx->d_small + 4);
+  }
+  CM_HIP(hipMemcpyAsync(ctx->h_small + 4, ctx->d_small + 4, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+}
+
 static Level level0(cm_ctx *ctx, const cm_graph_view *g, cm_hier *h, bool validate = true)
 {
   Level L;
@@ -294,7 +322,10 @@ static Level level0(cm_ctx *ctx, const cm_graph_view *g, cm_hier *h, bool valida
       if (L.n) CM_HIP(hipMemcpyAsync(L.vwgt, g->vwgt, 4 * L.n, hipMemcpyHostToDevice, ctx->stream));
     }
   }
+  queue_max_degree(ctx, L.n, L.off64, L.xadj);
   if (validate) check_graph(ctx, L.n, L.m, L.off64, L.xadj, L.adjncy, L.adjwgt);
+  sync(ctx);
+  L.maxdeg = ctx->h_small[4];
   if (!L.adjwgt) {
     L.adjwgt = (int32_t *)own(4 * (L.m > 0 ? L.m : 1));
     fill_u32(ctx, L.adjwgt, 1u, L.m);
@@ -356,7 +387,8 @@ static Level coarsen_level(cm_ctx *ctx, cm_hier *h, Level &F, int32_t maxvwgt, u
   C.xadj = own(ob * (nc + 1));
   C.adjncy = (int32_t *)own(4 * (m > 0 ? m : 1));
   C.adjwgt = (int32_t *)own(4 * (m > 0 ? m : 1));
-  C.m = contract(ctx, F, F.match, F.cmap, nc, rep, segoff, row_order, C.xadj, C.adjncy, C.adjwgt);
+  C.m = contract(ctx, F, F.match, F.cmap, nc, rep, segoff, row_order, C.xadj, C.adjncy, C.adjwgt,
+                 &C.maxdeg);
   C.tvwgt = F.tvwgt;  // cu:1400
   C.owns_graph = true;
   ctx->pool.release(rep);

@@ -64,6 +64,9 @@ struct Timed {
   double bytes;
 };
 
+// Rows above LONG_ROW entries are scanned by whole waves in the matching.
+constexpr int LONG_ROW = 64;
+
 struct Level {
   int64_t n = 0, m = 0;
   bool off64 = false;
@@ -77,6 +80,7 @@ struct Level {
   int64_t vbase = 0;  // added to adjncy and match values
   int64_t ebase = 0;  // added to xadj values
   int64_t cbase = 0;  // added to cmap values
+  int64_t maxdeg = -1;  // longest row (-1 unknown)
 };
 
 }  // namespace cm
@@ -183,9 +187,10 @@ int64_t number(cm_ctx *ctx, const Level &L, const int32_t *d_match, int32_t *d_c
                int32_t *d_rep, void *d_segoff, int32_t *d_cvwgt);
 // (c) contraction of L into coarse CSR; returns mc.  cxadj/cadjncy/cadjwgt
 //     must hold nc+1 / L.m / L.m entries.
+void queue_max_degree(cm_ctx *ctx, int64_t n, bool off64, const void *xadj);
 int64_t contract(cm_ctx *ctx, const Level &L, const int32_t *d_match, const int32_t *d_cmap,
                  int64_t nc, const int32_t *d_rep, const void *d_segoff, int row_order,
-                 void *d_cxadj, int32_t *d_cadjncy, int32_t *d_cadjwgt);
+                 void *d_cxadj, int32_t *d_cadjncy, int32_t *d_cadjwgt, int64_t *maxdeg = nullptr);
 // segmented sort of 64-bit keys with 32-bit values, segments segoff[0..nseg]
 void segsort_u64(cm_ctx *ctx, int64_t nseg, const void *d_segoff, bool off64, int64_t total,
                  uint64_t *d_keys, uint32_t *d_vals);

@@ -19,8 +19,9 @@
 
 namespace cm {
 
-constexpr int LONG_DEG = 64;      // rows longer than this are scanned by whole waves
+constexpr int LONG_DEG = LONG_ROW;  // rows longer than this are scanned by whole waves
 constexpr int LONG_CHUNK = 2048;  // entries per wave of a long row
+constexpr int SELECT_PER_THREAD = 16;
 constexpr int ROUND_BATCH = 9;  // launches queued between host checks
 
 // flags[2r] = 1 if a vertex matched or died in round r, flags[2r+1] = 1 if
@@ -76,18 +77,35 @@ template <typename OffT>
 __global__ __launch_bounds__(256) void k_select_long(int64_t n, const OffT *xadj, LongChunk *chunks,
                                                      unsigned *count)
 {
-  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t d = v < n ? (int64_t)(xadj[v + 1] - xadj[v]) : 0;
-  const bool is_long = d > LONG_DEG;
-  if (!__ballot(is_long)) return;
-  const unsigned nch = is_long ? (unsigned)((d + LONG_CHUNK - 1) / LONG_CHUNK) : 0u;
-  const unsigned inc = wave_incl_scan<unsigned>(nch);
-  const unsigned tot = __shfl(inc, CM_WAVE - 1, CM_WAVE);
-  unsigned base = 0;
-  if (lane() == 0) base = atomicAdd(count, tot);
-  base = __shfl(base, 0, CM_WAVE) + inc - nch;
-  for (unsigned k = 0; k < nch; ++k)
-    chunks[base + k] = LongChunk{(int32_t)v, (int32_t)base, (int32_t)nch, (int32_t)k};
+  // SELECT_PER_THREAD consecutive vertices per thread and one counter
+  // update per block, so that blocks do not queue on the counter
+  __shared__ unsigned s_w[4], s_base;
+  const int64_t v0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * SELECT_PER_THREAD;
+  unsigned nch[SELECT_PER_THREAD], mine = 0;
+#pragma unroll
+  for (int t = 0; t < SELECT_PER_THREAD; ++t) {
+    const int64_t v = v0 + t;
+    const int64_t d = v < n ? (int64_t)(xadj[v + 1] - xadj[v]) : 0;
+    nch[t] = d > LONG_DEG ? (unsigned)((d + LONG_CHUNK - 1) / LONG_CHUNK) : 0u;
+    mine += nch[t];
+  }
+  const unsigned inc = wave_incl_scan<unsigned>(mine);
+  const int w = threadIdx.x >> 6;
+  if (lane() == CM_WAVE - 1) s_w[w] = inc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned tot = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+    s_base = tot ? atomicAdd(count, tot) : 0u;
+  }
+  __syncthreads();
+  unsigned base = s_base + inc - mine;
+  for (int i = 0; i < w; ++i) base += s_w[i];
+#pragma unroll
+  for (int t = 0; t < SELECT_PER_THREAD; ++t) {
+    for (unsigned k = 0; k < nch[t]; ++k)
+      chunks[base + k] = LongChunk{(int32_t)(v0 + t), (int32_t)base, (int32_t)nch[t], (int32_t)k};
+    base += nch[t];
+  }
 }
 
 // Accepted in round r-1 (from its proposals alone): dead, or mutual.
@@ -109,20 +127,40 @@ __device__ __forceinline__ void scan_row(int64_t v, OffT b, OffT e, int l0, int 
                                          const int32_t *prev, bool have_prev, int32_t &best,
                                          uint64_t &bk1, uint64_t &bk2)
 {
-  for (OffT j = b + l0; j < e; j += step) {
-    const int32_t u = adjncy[j];
-    const uint32_t w = (uint32_t)adjwgt[j];
-    if (best != -2 && w < (uint32_t)(bk1 >> 32)) continue;
-    if (u == (int32_t)v || match[u] != -1 || vw + vwgt[u] > maxv) continue;
-    const uint32_t a = (uint32_t)(min((int64_t)u, v) - base), bb = (uint32_t)(max((int64_t)u, v) - base);
-    const uint64_t k1 = ((uint64_t)w << 32) | edge_hash(seed, a, bb);
-    const uint64_t k2 = ((uint64_t)a << 32) | bb;
-    // the two dependent gathers of accepted_prev only for edges that would
-    // beat the current best (about ln(degree) of them for random keys)
-    if ((best == -2 || key_better(k1, k2, bk1, bk2)) && !(have_prev && accepted_prev(prev, u))) {
-      best = u;
-      bk1 = k1;
-      bk2 = k2;
+  // SCAN_BATCH entries per lane per step: their row loads and the gathers
+  // that depend on them are issued together, so a lane waits once per batch
+  constexpr int SCAN_BATCH = 4;
+  for (OffT j0 = b + l0; j0 < e; j0 += (OffT)SCAN_BATCH * step) {
+    int32_t u[SCAN_BATCH], mu[SCAN_BATCH], wu[SCAN_BATCH];
+    uint32_t w[SCAN_BATCH];
+#pragma unroll
+    for (int t = 0; t < SCAN_BATCH; ++t) {
+      const OffT j = j0 + (OffT)t * step;
+      u[t] = j < e ? adjncy[j] : -1;
+      w[t] = j < e ? (uint32_t)adjwgt[j] : 0u;
+    }
+    const uint32_t wmin = best != -2 ? (uint32_t)(bk1 >> 32) : 0u;
+#pragma unroll
+    for (int t = 0; t < SCAN_BATCH; ++t) {
+      const bool look = u[t] >= 0 && u[t] != (int32_t)v && w[t] >= wmin;
+      mu[t] = look ? match[u[t]] : 0;
+      wu[t] = look ? vwgt[u[t]] : 0;
+      if (!look) u[t] = -1;
+    }
+#pragma unroll
+    for (int t = 0; t < SCAN_BATCH; ++t) {
+      if (u[t] < 0 || mu[t] != -1 || vw + wu[t] > maxv) continue;
+      if (best != -2 && w[t] < (uint32_t)(bk1 >> 32)) continue;
+      const uint32_t a = (uint32_t)(min((int64_t)u[t], v) - base), bb = (uint32_t)(max((int64_t)u[t], v) - base);
+      const uint64_t k1 = ((uint64_t)w[t] << 32) | edge_hash(seed, a, bb);
+      const uint64_t k2 = ((uint64_t)a << 32) | bb;
+      // the two dependent gathers of accepted_prev only for edges that would
+      // beat the current best (about ln(degree) of them for random keys)
+      if ((best == -2 || key_better(k1, k2, bk1, bk2)) && !(have_prev && accepted_prev(prev, u[t]))) {
+        best = u[t];
+        bk1 = k1;
+        bk2 = k2;
+      }
     }
   }
 }
@@ -317,10 +355,14 @@ static int match_impl(cm_ctx *ctx, const Level &L, const MatchOwners &own, int m
   const unsigned g = nblocks(n, 256);
   const double row_bytes = (double)sizeof(OffT) * (n + 1) + 8.0 * L.m + 8.0 * n;
   launch(ctx, "match_init", 8.0 * n, k_match_init, dim3(g), dim3(256), 0, n, match, prop);
-  launch(ctx, "match_select_long", (double)sizeof(OffT) * (n + 1), k_select_long<OffT>, dim3(g),
-         dim3(256), 0, n, xadj, chunks, nchunks);
-  // extra blocks whose waves stride over the long-row chunks
-  const unsigned glong = (unsigned)std::min<int64_t>(1024, std::max<int64_t>(16, (cap + 3) / 4));
+  // extra blocks whose waves stride over the long-row chunks (none when the
+  // level is known to have no long row)
+  unsigned glong = 0;
+  if (L.maxdeg < 0 || L.maxdeg > LONG_DEG) {
+    launch(ctx, "match_select_long", (double)sizeof(OffT) * (n + 1), k_select_long<OffT>,
+           dim3(nblocks(n, 256 * SELECT_PER_THREAD)), dim3(256), 0, n, xadj, chunks, nchunks);
+    glong = (unsigned)std::min<int64_t>(8192, std::max<int64_t>(16, (cap + 3) / 4));
+  }
   int r = 0;
   std::vector<unsigned> h(2);
   for (;;) {
