@@ -92,11 +92,12 @@ __global__ __launch_bounds__(256) void k_number_fill(int64_t n, const int32_t *_
 
 template <typename OffT>
 static int64_t number_impl(cm_ctx *ctx, const Level &L, const int32_t *match, int32_t *cmap,
-                           int32_t *rep, OffT *segoff, int32_t *cvwgt)
+                           int32_t *rep, OffT *segoff, int32_t *cvwgt, bool wait)
 {
   const int64_t n = L.n;
   if (n == 0) {
     fill_u32(ctx, segoff, 0u, sizeof(OffT) / 4);
+    fill_u32(ctx, ctx->d_small, 0u, 2);
     return 0;
   }
   const int64_t tiles = (n + 1023) / 1024;
@@ -106,6 +107,7 @@ static int64_t number_impl(cm_ctx *ctx, const Level &L, const int32_t *match, in
          cmap, rep, segoff, cvwgt, ctx->d_status, ctx->d_status + tiles, ctx->d_ctr, ctx->d_small, ep);
   launch(ctx, "number_fill", 12.0 * n, k_number_fill, dim3(nblocks(n, 256)), dim3(256), 0, n, match,
          cmap);
+  if (!wait) return -1;
   CM_HIP(hipMemcpyAsync(ctx->h_small, ctx->d_small, sizeof(int64_t), hipMemcpyDeviceToHost,
                         ctx->stream));
   sync(ctx);
@@ -113,10 +115,10 @@ static int64_t number_impl(cm_ctx *ctx, const Level &L, const int32_t *match, in
 }
 
 int64_t number(cm_ctx *ctx, const Level &L, const int32_t *d_match, int32_t *d_cmap,
-               int32_t *d_rep, void *d_segoff, int32_t *d_cvwgt)
+               int32_t *d_rep, void *d_segoff, int32_t *d_cvwgt, bool wait)
 {
-  return L.off64 ? number_impl(ctx, L, d_match, d_cmap, d_rep, (int64_t *)d_segoff, d_cvwgt)
-                 : number_impl(ctx, L, d_match, d_cmap, d_rep, (int32_t *)d_segoff, d_cvwgt);
+  return L.off64 ? number_impl(ctx, L, d_match, d_cmap, d_rep, (int64_t *)d_segoff, d_cvwgt, wait)
+                 : number_impl(ctx, L, d_match, d_cmap, d_rep, (int32_t *)d_segoff, d_cvwgt, wait);
 }
 
 // ------------------------------------------------- generic look-back scan
@@ -562,7 +564,8 @@ __device__ __forceinline__ int merge_regs(int32_t c, int32_t v, int32_t u, const
 // are written straight to their final place: the fine rows are read once
 // and the coarse rows written once.
 template <bool FS, typename OffT>
-__global__ __launch_bounds__(256) void k_contract_small(int64_t nc, const OffT *__restrict__ segoff,
+__global__ __launch_bounds__(256) void k_contract_small(const int64_t *__restrict__ ncp,
+                                                        const OffT *__restrict__ segoff,
                                                         const int32_t *__restrict__ rep,
                                                         const int32_t *__restrict__ match,
                                                         const OffT *__restrict__ xadj,
@@ -577,6 +580,7 @@ __global__ __launch_bounds__(256) void k_contract_small(int64_t nc, const OffT *
 {
   __shared__ uint64_t sh[4];
   __shared__ uint64_t s_excl;
+  const int64_t nc = *ncp;
   const int tile = tile_ticket(ctr);
   const int64_t c = (int64_t)tile * 256 + threadIdx.x;
   uint64_t k[REG_SEG], p[REG_SEG];
@@ -635,11 +639,18 @@ __global__ __launch_bounds__(256) void k_contract_small(int64_t nc, const OffT *
 template <typename OffT>
 static int64_t contract_impl(cm_ctx *ctx, const Level &L, const int32_t *match, const int32_t *cmap,
                              int64_t nc, const int32_t *rep, const OffT *segoff, int row_order, OffT *cxadj,
-                             int32_t *cadjncy, int32_t *cadjwgt, int64_t *maxdeg, bool force_sorted = false)
+                             int32_t *cadjncy, int32_t *cadjwgt, int64_t *maxdeg, int64_t *ncp,
+                             bool force_sorted = false)
 {
   const int64_t n = L.n, m = L.m;
   const OffT *xadj = (const OffT *)L.xadj;
   const bool fs = row_order == CM_ROW_FIRST_SEEN;
+  if (nc < 0 && (n == 0 || !fs || force_sorted)) {  // the paths below need nc on the host
+    CM_HIP(hipMemcpyAsync(ctx->h_small, ctx->d_small, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+    sync(ctx);
+    nc = ctx->h_small[0];
+  }
+  if (ncp && nc >= 0) *ncp = nc;
   if (nc == 0) {
     fill_u32(ctx, cxadj, 0u, sizeof(OffT) / 4);
     if (maxdeg) *maxdeg = 0;
@@ -648,29 +659,38 @@ static int64_t contract_impl(cm_ctx *ctx, const Level &L, const int32_t *match, 
   if (fs && !force_sorted) {
     // segments above REG_SEG (they exist only if 2 * maxdeg > REG_SEG) by
     // hashing, the rest in registers; one host synchronisation at the end
+    // nc < 0: the count is on the device (d_small[0]); grids are sized by
+    // the bound n and kernels read the count
     const bool hashed = L.maxdeg < 0 || 2 * L.maxdeg > REG_SEG;
-    int32_t *cdeg = (int32_t *)ctx->pool.alloc(sizeof(int32_t) * nc);
+    const int64_t ncb = nc < 0 ? n : nc;
+    const int64_t *d_nc = ctx->d_small;
+    if (nc >= 0) {
+      fill_u64(ctx, ctx->d_small + 7, (uint64_t)nc, 1);
+      d_nc = ctx->d_small + 7;
+    }
+    int32_t *cdeg = (int32_t *)ctx->pool.alloc(sizeof(int32_t) * ncb);
     HashContract hc;
     try {
-      if (hashed) hash_prepare_impl<OffT>(ctx, L, match, cmap, nc, rep, segoff, cdeg, hc);
-      const int64_t tiles = (nc + 255) / 256;
+      if (hashed) hash_prepare_impl<OffT>(ctx, L, match, cmap, ncb, d_nc, rep, segoff, cdeg, hc);
+      const int64_t tiles = (ncb + 255) / 256;
       const uint32_t ep = lookback_begin(ctx, tiles);
       fill_u32(ctx, ctx->d_small + 4, 0u, 2);
-      const double small_bytes = 8.0 * m + 12.0 * n + (double)sizeof(OffT) * (n + 1) + 8.0 * nc +
-                                 (double)sizeof(OffT) * (nc + 1);
+      const double small_bytes = 8.0 * m + 12.0 * n + (double)sizeof(OffT) * (n + 1) + 8.0 * ncb +
+                                 (double)sizeof(OffT) * (ncb + 1);
       launch(ctx, "contract_small", small_bytes, k_contract_small<true, OffT>, dim3((unsigned)tiles), dim3(256),
-             0, nc, segoff, rep, match, xadj, L.adjncy, L.adjwgt, cmap, (const int32_t *)cdeg, cxadj, cadjncy,
+             0, d_nc, segoff, rep, match, xadj, L.adjncy, L.adjwgt, cmap, (const int32_t *)cdeg, cxadj, cadjncy,
              cadjwgt, ctx->d_status, ctx->d_ctr, ctx->d_small + 1, ctx->d_small + 4, ep);
       ctx->h_small[6] = 0;
       if (hashed) hash_emit_impl<OffT>(ctx, segoff, cdeg, cxadj, cadjncy, cadjwgt, hc);
-      CM_HIP(hipMemcpyAsync(ctx->h_small + 1, ctx->d_small + 1, 4 * sizeof(int64_t), hipMemcpyDeviceToHost,
-                            ctx->stream));
+      CM_HIP(hipMemcpyAsync(ctx->h_small, ctx->d_small, 5 * sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
       sync(ctx);
     } catch (...) {
       hash_release(ctx, hc);
       ctx->pool.release(cdeg);
       throw;
     }
+    if (nc < 0) nc = ctx->h_small[0];
+    if (ncp) *ncp = nc;
     const int64_t mc = ctx->h_small[1];
     const int64_t md = ctx->h_small[4];
     const bool overflow = hashed && (uint32_t)ctx->h_small[6] != 0;
@@ -678,7 +698,7 @@ static int64_t contract_impl(cm_ctx *ctx, const Level &L, const int32_t *match, 
     ctx->pool.release(cdeg);
     if (overflow)  // a hub partition held more distinct keys than its table
       return contract_impl<OffT>(ctx, L, match, cmap, nc, rep, segoff, row_order, cxadj, cadjncy, cadjwgt,
-                                 maxdeg, true);
+                                 maxdeg, ncp, true);
     if (maxdeg) *maxdeg = md;
     return mc;
   }
@@ -731,6 +751,7 @@ static int64_t contract_impl(cm_ctx *ctx, const Level &L, const int32_t *match, 
 #undef CM_COUNT
   }
   // fused short-segment merge + look-back cxadj + direct row writes
+  fill_u64(ctx, ctx->d_small + 7, (uint64_t)nc, 1);
   const int64_t tiles = (nc + 255) / 256;
   const uint32_t ep = lookback_begin(ctx, tiles);
   fill_u32(ctx, ctx->d_small + 4, 0u, 2);
@@ -738,11 +759,11 @@ static int64_t contract_impl(cm_ctx *ctx, const Level &L, const int32_t *match, 
                              (double)sizeof(OffT) * (nc + 1);
   if (fs)
     launch(ctx, "contract_small", small_bytes, k_contract_small<true, OffT>, dim3((unsigned)tiles), dim3(256),
-           0, nc, segoff, rep, match, xadj, L.adjncy, L.adjwgt, cmap, (const int32_t *)cdeg, cxadj, cadjncy,
+           0, (const int64_t *)(ctx->d_small + 7), segoff, rep, match, xadj, L.adjncy, L.adjwgt, cmap, (const int32_t *)cdeg, cxadj, cadjncy,
            cadjwgt, ctx->d_status, ctx->d_ctr, ctx->d_small + 1, ctx->d_small + 4, ep);
   else
     launch(ctx, "contract_small", small_bytes, k_contract_small<false, OffT>, dim3((unsigned)tiles),
-           dim3(256), 0, nc, segoff, rep, match, xadj, L.adjncy, L.adjwgt, cmap, (const int32_t *)cdeg, cxadj,
+           dim3(256), 0, (const int64_t *)(ctx->d_small + 7), segoff, rep, match, xadj, L.adjncy, L.adjwgt, cmap, (const int32_t *)cdeg, cxadj,
            cadjncy, cadjwgt, ctx->d_status, ctx->d_ctr, ctx->d_small + 1, ctx->d_small + 4, ep);
   if (sorted_path) {
 #define CM_EMIT(FSV)                                                                                      \
@@ -783,12 +804,12 @@ static int64_t contract_impl(cm_ctx *ctx, const Level &L, const int32_t *match, 
 
 int64_t contract(cm_ctx *ctx, const Level &L, const int32_t *d_match, const int32_t *d_cmap,
                  int64_t nc, const int32_t *d_rep, const void *d_segoff, int row_order,
-                 void *d_cxadj, int32_t *d_cadjncy, int32_t *d_cadjwgt, int64_t *maxdeg)
+                 void *d_cxadj, int32_t *d_cadjncy, int32_t *d_cadjwgt, int64_t *maxdeg, int64_t *ncp)
 {
   return L.off64 ? contract_impl(ctx, L, d_match, d_cmap, nc, d_rep, (const int64_t *)d_segoff, row_order,
-                                 (int64_t *)d_cxadj, d_cadjncy, d_cadjwgt, maxdeg)
+                                 (int64_t *)d_cxadj, d_cadjncy, d_cadjwgt, maxdeg, ncp)
                  : contract_impl(ctx, L, d_match, d_cmap, nc, d_rep, (const int32_t *)d_segoff, row_order,
-                                 (int32_t *)d_cxadj, d_cadjncy, d_cadjwgt, maxdeg);
+                                 (int32_t *)d_cxadj, d_cadjncy, d_cadjwgt, maxdeg, ncp);
 }
 
 // --------------------------------------------------------- (d) projection

@@ -103,11 +103,12 @@ __device__ __forceinline__ int hc_find(const int32_t *tk, int lg, int32_t key)
 
 // ---------------------------------------------------------------- select
 template <typename OffT>
-__global__ __launch_bounds__(256) void k_hc_select(int64_t nc, const OffT *__restrict__ segoff,
+__global__ __launch_bounds__(256) void k_hc_select(const int64_t *__restrict__ ncp, const OffT *__restrict__ segoff,
                                                    SegChunk *chunks, int32_t *wave_items, int32_t *block_items,
                                                    HubPart *parts, int32_t *hubs, unsigned *ctr)
 {
   __shared__ unsigned s_w[HC_N_COUNTERS][4], s_base[HC_N_COUNTERS];
+  const int64_t nc = *ncp;
   const int64_t c0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * HC_SELECT;
   unsigned cnt[HC_N_COUNTERS] = {0, 0, 0, 0, 0};
   for (int t = 0; t < HC_SELECT; ++t) {
@@ -548,7 +549,8 @@ __global__ __launch_bounds__(256) void k_hc_emit(const SegChunk *__restrict__ ch
 // ------------------------------------------------------------- host side
 template <typename OffT>
 void hash_prepare_impl(cm_ctx *ctx, const Level &L, const int32_t *match, const int32_t *cmap, int64_t nc,
-                       const int32_t *rep, const OffT *segoff, int32_t *cdeg, HashContract &hc)
+                       const int64_t *d_nc, const int32_t *rep, const OffT *segoff, int32_t *cdeg,
+                       HashContract &hc)
 {
   const int64_t m = L.m;
   const OffT *xadj = (const OffT *)L.xadj;
@@ -573,7 +575,7 @@ void hash_prepare_impl(cm_ctx *ctx, const Level &L, const int32_t *match, const 
   hc.tw = (int32_t *)al(4 * m);
   fill_u32(ctx, hc.ctr, 0u, 8);
   const SegChunk *chunks = (const SegChunk *)hc.chunks;
-  launch(ctx, "hc_select", 0.0, k_hc_select<OffT>, dim3(nblocks(nc, 256 * HC_SELECT)), dim3(256), 0, nc, segoff,
+  launch(ctx, "hc_select", 0.0, k_hc_select<OffT>, dim3(nblocks(nc, 256 * HC_SELECT)), dim3(256), 0, d_nc, segoff,
          (SegChunk *)hc.chunks, wave_items, block_items, parts, hubs, hc.ctr);
   const unsigned gchunk = (unsigned)std::min<int64_t>(4096, (nchunk + 3) / 4);
   hc.gchunk = gchunk;
@@ -616,9 +618,11 @@ void hash_emit_impl(cm_ctx *ctx, const OffT *segoff, const int32_t *cdeg, const 
 }
 
 template void hash_prepare_impl<int32_t>(cm_ctx *, const Level &, const int32_t *, const int32_t *, int64_t,
-                                         const int32_t *, const int32_t *, int32_t *, HashContract &);
+                                         const int64_t *, const int32_t *, const int32_t *, int32_t *,
+                                         HashContract &);
 template void hash_prepare_impl<int64_t>(cm_ctx *, const Level &, const int32_t *, const int32_t *, int64_t,
-                                         const int32_t *, const int64_t *, int32_t *, HashContract &);
+                                         const int64_t *, const int32_t *, const int64_t *, int32_t *,
+                                         HashContract &);
 template void hash_emit_impl<int32_t>(cm_ctx *, const int32_t *, const int32_t *, const int32_t *, int32_t *,
                                       int32_t *, const HashContract &);
 template void hash_emit_impl<int64_t>(cm_ctx *, const int64_t *, const int32_t *, const int64_t *, int32_t *,

@@ -382,13 +382,14 @@ static Level coarsen_level(cm_ctx *ctx, cm_hier *h, Level &F, int32_t maxvwgt, u
   Level C;
   C.off64 = F.off64;
   C.vwgt = (int32_t *)own(4 * (n > 0 ? n : 1));
-  const int64_t nc = number(ctx, F, F.match, F.cmap, rep, segoff, C.vwgt);
-  C.n = nc;
-  C.xadj = own(ob * (nc + 1));
+  // nc stays on the device until the contraction's one synchronisation;
+  // the coarse offsets are sized by the bound n
+  number(ctx, F, F.match, F.cmap, rep, segoff, C.vwgt, false);
+  C.xadj = own(ob * (n + 1));
   C.adjncy = (int32_t *)own(4 * (m > 0 ? m : 1));
   C.adjwgt = (int32_t *)own(4 * (m > 0 ? m : 1));
-  C.m = contract(ctx, F, F.match, F.cmap, nc, rep, segoff, row_order, C.xadj, C.adjncy, C.adjwgt,
-                 &C.maxdeg);
+  C.m = contract(ctx, F, F.match, F.cmap, -1, rep, segoff, row_order, C.xadj, C.adjncy, C.adjwgt, &C.maxdeg,
+                 &C.n);
   C.tvwgt = F.tvwgt;  // cu:1400
   C.owns_graph = true;
   ctx->pool.release(rep);

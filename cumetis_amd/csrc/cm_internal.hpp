@@ -183,14 +183,19 @@ int match_handshake(cm_ctx *ctx, const Level &L, const MatchOwners &own, int max
                     int32_t *d_match);
 // (b) numbering: cmap for every fine vertex; rep/segoff/cvwgt per coarse vertex.
 //     Returns nc; segoff[nc] = m.
+// With wait=false nc stays in d_small[0] (read it back with the next
+// synchronisation) and -1 is returned.
 int64_t number(cm_ctx *ctx, const Level &L, const int32_t *d_match, int32_t *d_cmap,
-               int32_t *d_rep, void *d_segoff, int32_t *d_cvwgt);
-// (c) contraction of L into coarse CSR; returns mc.  cxadj/cadjncy/cadjwgt
-//     must hold nc+1 / L.m / L.m entries.
+               int32_t *d_rep, void *d_segoff, int32_t *d_cvwgt, bool wait = true);
 void queue_max_degree(cm_ctx *ctx, int64_t n, bool off64, const void *xadj);
+// (c) contraction of L into coarse CSR; returns mc.  cxadj/cadjncy/cadjwgt
+//     must hold nc+1 / L.m / L.m entries.  nc = -1: the count left on the
+//     device by number(wait=false); cxadj then holds L.n+1 entries and *ncp
+//     receives nc.
 int64_t contract(cm_ctx *ctx, const Level &L, const int32_t *d_match, const int32_t *d_cmap,
                  int64_t nc, const int32_t *d_rep, const void *d_segoff, int row_order,
-                 void *d_cxadj, int32_t *d_cadjncy, int32_t *d_cadjwgt, int64_t *maxdeg = nullptr);
+                 void *d_cxadj, int32_t *d_cadjncy, int32_t *d_cadjwgt, int64_t *maxdeg = nullptr,
+                 int64_t *ncp = nullptr);
 // segmented sort of 64-bit keys with 32-bit values, segments segoff[0..nseg]
 void segsort_u64(cm_ctx *ctx, int64_t nseg, const void *d_segoff, bool off64, int64_t total,
                  uint64_t *d_keys, uint32_t *d_vals);

@@ -341,7 +341,8 @@ static int match_impl(cm_ctx *ctx, const Level &L, const MatchOwners &own, int m
   const OffT *xadj = (const OffT *)L.xadj;
   int32_t *prop = (int32_t *)ctx->pool.alloc(sizeof(int32_t) * 2 * n);  // ping-pong
   // chunks: sum over rows longer than LONG_DEG of ceil(d / LONG_CHUNK)
-  const int64_t cap = L.m / LONG_CHUNK + std::min<int64_t>(n, L.m / (LONG_DEG + 1)) + 1;
+  const bool has_long = L.maxdeg < 0 || L.maxdeg > LONG_DEG;
+  const int64_t cap = has_long ? L.m / LONG_CHUNK + std::min<int64_t>(n, L.m / (LONG_DEG + 1)) + 1 : 1;
   LongChunk *chunks = (LongChunk *)ctx->pool.alloc(sizeof(LongChunk) * cap);
   LongPartial part;
   part.best = (int32_t *)ctx->pool.alloc(sizeof(int32_t) * cap);
@@ -351,20 +352,20 @@ static int match_impl(cm_ctx *ctx, const Level &L, const MatchOwners &own, int m
   unsigned *flags = (unsigned *)ctx->pool.alloc(sizeof(unsigned) * (2 * (size_t)max_rounds + 2));
   unsigned *nchunks = flags + 2 * (size_t)max_rounds;
   fill_u32(ctx, flags, 0u, 2 * (int64_t)max_rounds + 2);
-  fill_u32(ctx, part.arrived, 0u, cap);
+  if (has_long) fill_u32(ctx, part.arrived, 0u, cap);
   const unsigned g = nblocks(n, 256);
   const double row_bytes = (double)sizeof(OffT) * (n + 1) + 8.0 * L.m + 8.0 * n;
   launch(ctx, "match_init", 8.0 * n, k_match_init, dim3(g), dim3(256), 0, n, match, prop);
   // extra blocks whose waves stride over the long-row chunks (none when the
   // level is known to have no long row)
   unsigned glong = 0;
-  if (L.maxdeg < 0 || L.maxdeg > LONG_DEG) {
+  if (has_long) {
     launch(ctx, "match_select_long", (double)sizeof(OffT) * (n + 1), k_select_long<OffT>,
            dim3(nblocks(n, 256 * SELECT_PER_THREAD)), dim3(256), 0, n, xadj, chunks, nchunks);
     glong = (unsigned)std::min<int64_t>(8192, std::max<int64_t>(16, (cap + 3) / 4));
   }
   int r = 0;
-  std::vector<unsigned> h(2);
+  std::vector<unsigned> h(2 * (size_t)max_rounds + 2);
   for (;;) {
     for (int i = 0; i < ROUND_BATCH && r <= max_rounds; ++i, ++r) {
       int32_t *cur = prop + (size_t)(r & 1) * n;
@@ -374,22 +375,18 @@ static int match_impl(cm_ctx *ctx, const Level &L, const MatchOwners &own, int m
              (const unsigned *)nchunks, part, flags, r, g, r < max_rounds ? 1 : 0);
     }
     if (r < 2) continue;
-    // status of the last accepted round (r-2: accepted by launch r-1)
-    CM_HIP(hipMemcpyAsync(h.data(), flags + 2 * (r - 2), sizeof(unsigned) * 2, hipMemcpyDeviceToHost,
-                          ctx->stream));
+    // flags of every accepted round (0 .. r-2: round r-2 is accepted by
+    // launch r-1), one read per batch
+    CM_HIP(hipMemcpyAsync(h.data(), flags, sizeof(unsigned) * 2 * (r - 1), hipMemcpyDeviceToHost, ctx->stream));
     sync(ctx);
-    if (h[0] == 0 || h[1] == 0 || r > max_rounds) break;
+    if (h[2 * (r - 2)] == 0 || h[2 * (r - 2) + 1] == 0 || r > max_rounds) break;
   }
   // rounds that did work: round 0, then every round whose predecessor changed
   // something and left active vertices
   const int acc = std::min(r - 1, max_rounds);  // accepted rounds 0 .. acc-1
-  std::vector<unsigned> all(2 * (size_t)std::max(acc, 1));
-  if (acc > 0)
-    CM_HIP(hipMemcpyAsync(all.data(), flags, sizeof(unsigned) * 2 * acc, hipMemcpyDeviceToHost, ctx->stream));
   launch(ctx, "match_finish", 4.0 * n, k_match_finish, dim3(g), dim3(256), 0, n, match);
-  sync(ctx);
   int rounds = 1;
-  while (rounds < acc && all[2 * (rounds - 1)] != 0 && all[2 * (rounds - 1) + 1] != 0) ++rounds;
+  while (rounds < acc && h[2 * (rounds - 1)] != 0 && h[2 * (rounds - 1) + 1] != 0) ++rounds;
   ctx->pool.release(prop);
   ctx->pool.release(chunks);
   ctx->pool.release(part.best);
