@@ -158,9 +158,9 @@ static void batch_impl(cm_ctx *ctx, int nb, const cm_graph_view *g, const int64_
   if (g->vwgt && N) {
     CM_HIP(hipMemcpyAsync(hv.data(), vw, 4 * N, hipMemcpyDeviceToHost, ctx->stream));
   }
-  queue_max_degree(ctx, N, off64, xadj);
+  queue_max_degree(ctx, N, off64, xadj, vw);
   sync(ctx);
-  const int64_t maxdeg0 = ctx->h_small[4];
+  const int64_t maxdeg0 = ctx->h_small[4], maxvw0 = ctx->h_small[3];
   // ---- slots (graphs with vertices); empty graphs get two empty levels
   std::vector<Slot> slots;
   for (int b = 0; b < nb; ++b) {
@@ -187,6 +187,7 @@ static void batch_impl(cm_ctx *ctx, int nb, const cm_graph_view *g, const int64_
     L0.vbase = voff[b];
     L0.ebase = eoff[b];
     L0.maxdeg = maxdeg0;
+    L0.maxvw = maxvw0;  // of the union: a bound for each graph
     out[b]->levels.push_back(L0);
     if (n == 0) {
       Level E = L0;
@@ -246,9 +247,11 @@ static void batch_impl(cm_ctx *ctx, int nb, const cm_graph_view *g, const int64_
     U.adjwgt = wgt;
     U.vwgt = vw;
     U.maxdeg = 0;
+    U.maxvw = 0;
     for (int s = 0; s < ns; ++s) {
-      const int64_t d = out[slots[s].graph]->levels.back().maxdeg;
-      U.maxdeg = (U.maxdeg < 0 || d < 0) ? -1 : std::max(U.maxdeg, d);
+      const Level &S = out[slots[s].graph]->levels.back();
+      U.maxdeg = (U.maxdeg < 0 || S.maxdeg < 0) ? -1 : std::max(U.maxdeg, S.maxdeg);
+      U.maxvw = (U.maxvw < 0 || S.maxvw < 0) ? -1 : std::max(U.maxvw, S.maxvw);
     }
     int32_t *match = (int32_t *)own(4 * n), *cmap = (int32_t *)own(4 * n);
     MatchOwners mo;
@@ -256,7 +259,7 @@ static void batch_impl(cm_ctx *ctx, int nb, const cm_graph_view *g, const int64_
     mo.maxv = d_maxv;
     mo.seed = d_seed;
     mo.vbase = d_vb;
-    const int rounds = match_handshake(ctx, U, mo, params[0].max_rounds, match);
+    const int rounds = match_handshake(ctx, U, mo, params[0].max_rounds, match, false);  // number() completes it
     int32_t *rep = (int32_t *)tmp(4 * n);
     OffT *segoff = (OffT *)tmp(sizeof(OffT) * (n + 1));
     int32_t *cvw = (int32_t *)own(4 * n);
@@ -266,6 +269,7 @@ static void batch_impl(cm_ctx *ctx, int nb, const cm_graph_view *g, const int64_
     int64_t cmaxdeg = -1;
     const int64_t mc = contract(ctx, U, match, cmap, nc, rep, segoff, params[0].row_order, cxadj, cadj, cwgt,
                                 &cmaxdeg);
+    const int64_t cmaxvw = ctx->h_small[3];
     // per-slot coarse bases
     int64_t *d_cvb = (int64_t *)tmp(8 * (ns + 1)), *d_ceb = (int64_t *)tmp(8 * (ns + 1));
     launch(ctx, "batch_bases", 0.0, k_slot_bases<OffT>, dim3(nblocks(ns + 1, 256)), dim3(256), 0, ns,
@@ -296,6 +300,7 @@ static void batch_impl(cm_ctx *ctx, int nb, const cm_graph_view *g, const int64_
       C.vbase = cvb[s];
       C.ebase = ceb[s];
       C.maxdeg = cmaxdeg;  // of the union: a bound for each slot
+      C.maxvw = cmaxvw;
       const int64_t nf = slots[s].n;
       h->levels.push_back(C);
       const bool more = C.n > slots[s].T && C.n < 0.75 * nf && C.m > C.n / 2;

@@ -74,11 +74,29 @@ __device__ __forceinline__ uint64_t st_word(uint32_t state, uint32_t epoch, uint
   return ((uint64_t)state << 62) | ((uint64_t)epoch << ST_EPOCH_SHIFT) | (v & ST_VAL);
 }
 
+// Combine operators of the look-back chains (values are >= 0, identity 0).
+struct LbSum {
+  static __device__ __forceinline__ uint64_t comb(uint64_t a, uint64_t b) { return a + b; }
+};
+struct LbMax {
+  static __device__ __forceinline__ uint64_t comb(uint64_t a, uint64_t b) { return a > b ? a : b; }
+};
+
+template <typename Op>
+__device__ __forceinline__ uint64_t wave_reduce(uint64_t v)
+{
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) v = Op::comb(v, __shfl_xor(v, d, CM_WAVE));
+  return v;
+}
+
 // Called by all 64 lanes of ONE wave of the tile.  Publishes the tile's
 // aggregate, walks back over predecessors 64 at a time, publishes the
 // inclusive prefix and returns the exclusive prefix (same value in all lanes).
 // Tiles must be numbered in dispatch order (see tile_ticket) so that every
-// predecessor has started and the wait terminates.
+// predecessor has started and the wait terminates.  Op = LbMax gives the
+// running maximum instead of the sum.
+template <typename Op = LbSum>
 __device__ __forceinline__ uint64_t lookback(uint64_t *status, int tile, uint64_t agg, uint32_t epoch)
 {
   const int l = lane();
@@ -96,11 +114,11 @@ __device__ __forceinline__ uint64_t lookback(uint64_t *status, int tile, uint64_
     const int first = incm ? __ffsll((unsigned long long)incm) - 1 : 64;
     const uint64_t consider = first >= 63 ? ~0ull : ((2ull << first) - 1);
     if (zerom & consider) continue;  // a predecessor has not published yet
-    excl += wave_sum<uint64_t>(l <= first ? (s & ST_VAL) : 0);
+    excl = Op::comb(excl, wave_reduce<Op>(l <= first ? (s & ST_VAL) : 0));
     if (first < 64) break;
     base -= CM_WAVE;
   }
-  if (l == 0) st_store(&status[tile], st_word(2u, epoch, excl + agg));
+  if (l == 0) st_store(&status[tile], st_word(2u, epoch, Op::comb(excl, agg)));
   return excl;
 }
 
@@ -117,6 +135,28 @@ __device__ __forceinline__ int tile_ticket(unsigned *ctr)
   }
   __syncthreads();
   return s_tile;
+}
+
+// ------------------------------------------------------ block scan (NW waves)
+// Exclusive block-wide scan for a block of NW waves; returns the exclusive
+// prefix of `v`; `total` receives the block sum.  Uses `sh` (NW entries).
+template <typename T, int NW>
+__device__ __forceinline__ T block_excl_scan(T v, T &total, T *sh)
+{
+  const int w = threadIdx.x >> 6, l = lane();
+  T inc = wave_incl_scan<T>(v);
+  if (l == CM_WAVE - 1) sh[w] = inc;
+  __syncthreads();
+  T before = 0;
+  total = 0;
+#pragma unroll
+  for (int i = 0; i < NW; ++i) {
+    T x = sh[i];
+    if (i < w) before += x;
+    total += x;
+  }
+  __syncthreads();
+  return before + inc - v;
 }
 
 // ------------------------------------------------------ block scan (256 thr)

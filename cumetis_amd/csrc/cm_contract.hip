@@ -24,33 +24,66 @@ constexpr int SMALL_SEG = 64;     // coarse segments up to this merge per thread
 constexpr int FIRST_SORTED_BIN = 4;  // segsort bins from 17 keys up
 
 // ------------------------------------------------------------- numbering
+// NUM_PER consecutive vertices per thread, tiles of 1024 * NUM_PER vertices
+// (fewer tiles = a shorter look-back chain).  A vertex still unmatched
+// (match = -1, the matching left unfinished) becomes its own coarse vertex
+// and its match entry is completed here.  Each tile also stores its largest
+// coarse vertex weight and segment length in tmax[2 * tile .. 2 * tile + 1]
+// (k_number_fill reduces them).
+constexpr int NUM_PER = 2;
+
 template <typename OffT>
 __global__ __launch_bounds__(1024) void k_number(int64_t n, const OffT *__restrict__ xadj,
-                                                 const int32_t *__restrict__ match,
+                                                 int32_t *__restrict__ match,
                                                  const int32_t *__restrict__ vwgt,
                                                  int32_t *__restrict__ cmap, int32_t *__restrict__ rep,
                                                  OffT *__restrict__ segoff, int32_t *__restrict__ cvwgt,
-                                                 uint64_t *st_c, uint64_t *st_l, unsigned *ctr,
+                                                 uint64_t *st_c, uint64_t *st_l, uint64_t *tmax, unsigned *ctr,
                                                  int64_t *totals, uint32_t epoch)
 {
-  __shared__ uint64_t sh_c[16], sh_l[16];
+  __shared__ uint64_t sh_c[16], sh_l[16], sh_m[16], sh_n[16];
   const int tile = tile_ticket(ctr);
-  const int64_t i = (int64_t)tile * 1024 + threadIdx.x;
-  uint64_t flag = 0, len = 0;
-  int32_t u = 0;
-  if (i < n) {
-    u = match[i];
-    flag = (i <= u);
-    if (flag) {
-      len = (uint64_t)(xadj[i + 1] - xadj[i]);
-      if (u != i) len += (uint64_t)(xadj[u + 1] - xadj[u]);
+  const int64_t i0 = ((int64_t)tile * 1024 + threadIdx.x) * NUM_PER;
+  uint64_t fl[NUM_PER], ln[NUM_PER], flag = 0, len = 0, mw = 0, ml = 0;
+  int32_t uu[NUM_PER], cw[NUM_PER];
+#pragma unroll
+  for (int t = 0; t < NUM_PER; ++t) {
+    const int64_t i = i0 + t;
+    fl[t] = 0;
+    ln[t] = 0;
+    uu[t] = 0;
+    cw[t] = 0;
+    if (i < n) {
+      int32_t u = match[i];
+      if (u < 0) {
+        u = (int32_t)i;
+        match[i] = u;
+      }
+      uu[t] = u;
+      fl[t] = (i <= u);
+      if (fl[t]) {
+        ln[t] = (uint64_t)(xadj[i + 1] - xadj[i]);
+        cw[t] = vwgt[i];
+        if (u != i) {
+          ln[t] += (uint64_t)(xadj[u + 1] - xadj[u]);
+          cw[t] += vwgt[u];
+        }
+        mw = (uint64_t)(uint32_t)cw[t] > mw ? (uint64_t)(uint32_t)cw[t] : mw;
+        ml = ln[t] > ml ? ln[t] : ml;
+      }
     }
+    flag += fl[t];
+    len += ln[t];
   }
   const int w = threadIdx.x >> 6, l = lane();
   const uint64_t ic = wave_incl_scan<uint64_t>(flag), il = wave_incl_scan<uint64_t>(len);
+  mw = wave_max<uint64_t>(mw);
+  ml = wave_max<uint64_t>(ml);
   if (l == CM_WAVE - 1) {
     sh_c[w] = ic;
     sh_l[w] = il;
+    sh_m[w] = mw;
+    sh_n[w] = ml;
   }
   __syncthreads();
   if (w == 0) {
@@ -59,30 +92,74 @@ __global__ __launch_bounds__(1024) void k_number(int64_t n, const OffT *__restri
     const uint64_t ta = __shfl(ia, 15, CM_WAVE), tb = __shfl(ib, 15, CM_WAVE);
     const uint64_t ea = lookback(st_c, tile, ta, epoch);
     const uint64_t eb = lookback(st_l, tile, tb, epoch);
+    const uint64_t bm = wave_reduce<LbMax>(l < 16 ? sh_m[l] : 0);
+    const uint64_t bn = wave_reduce<LbMax>(l < 16 ? sh_n[l] : 0);
     if (l < 16) {
       sh_c[l] = ea + ia - a;
       sh_l[l] = eb + ib - b;
     }
+    if (l == 0) {
+      tmax[2 * tile] = bm;
+      tmax[2 * tile + 1] = bn;
+    }
   }
   __syncthreads();
-  if (i < n) {
-    const uint64_t c = sh_c[w] + ic - flag, off = sh_l[w] + il - len;
-    if (flag) {
-      cmap[i] = (int32_t)c;
-      rep[c] = (int32_t)i;
-      segoff[c] = (OffT)off;
-      cvwgt[c] = vwgt[i] + (u != i ? vwgt[u] : 0);
+  uint64_t c = sh_c[w] + ic - flag, off = sh_l[w] + il - len;
+#pragma unroll
+  for (int t = 0; t < NUM_PER; ++t) {
+    const int64_t i = i0 + t;
+    if (i < n) {
+      if (fl[t]) {
+        const int32_t u = uu[t];
+        cmap[i] = (int32_t)c;
+        rep[c] = (int32_t)i;
+        segoff[c] = (OffT)off;
+        cvwgt[c] = cw[t];
+        (void)u;
+      }
+      if (i == n - 1) {
+        totals[0] = (int64_t)(c + fl[t]);
+        segoff[c + fl[t]] = (OffT)(off + ln[t]);
+      }
     }
-    if (i == n - 1) {
-      totals[0] = (int64_t)(c + flag);
-      segoff[c + flag] = (OffT)(off + len);
-    }
+    c += fl[t];
+    off += ln[t];
   }
 }
 
+// Partners take the coarse id of their representative.  Block 0 also
+// reduces the per-tile maxima of k_number: totals[3] = largest coarse vertex
+// weight, totals[4] = longest segment (a bound on the longest coarse row).
 __global__ __launch_bounds__(256) void k_number_fill(int64_t n, const int32_t *__restrict__ match,
-                                                     int32_t *__restrict__ cmap)
+                                                     int32_t *__restrict__ cmap, const uint64_t *__restrict__ tmax,
+                                                     int64_t ntiles, int64_t *totals)
 {
+  if (blockIdx.x == 0) {
+    __shared__ uint64_t s_m[4], s_n[4];
+    uint64_t m0 = 0, n0 = 0;
+    for (int64_t t = threadIdx.x; t < ntiles; t += 256) {
+      const uint64_t a = tmax[2 * t], b = tmax[2 * t + 1];
+      m0 = a > m0 ? a : m0;
+      n0 = b > n0 ? b : n0;
+    }
+    m0 = wave_max<uint64_t>(m0);
+    n0 = wave_max<uint64_t>(n0);
+    if (lane() == 0) {
+      s_m[threadIdx.x >> 6] = m0;
+      s_n[threadIdx.x >> 6] = n0;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int k = 1; k < 4; ++k) {
+        m0 = s_m[k] > s_m[0] ? s_m[k] : s_m[0];
+        s_m[0] = m0;
+        n0 = s_n[k] > s_n[0] ? s_n[k] : s_n[0];
+        s_n[0] = n0;
+      }
+      totals[3] = (int64_t)s_m[0];
+      totals[4] = (int64_t)s_n[0];
+    }
+  }
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {
     const int32_t u = match[i];
@@ -91,22 +168,23 @@ __global__ __launch_bounds__(256) void k_number_fill(int64_t n, const int32_t *_
 }
 
 template <typename OffT>
-static int64_t number_impl(cm_ctx *ctx, const Level &L, const int32_t *match, int32_t *cmap,
+static int64_t number_impl(cm_ctx *ctx, const Level &L, int32_t *match, int32_t *cmap,
                            int32_t *rep, OffT *segoff, int32_t *cvwgt, bool wait)
 {
   const int64_t n = L.n;
   if (n == 0) {
     fill_u32(ctx, segoff, 0u, sizeof(OffT) / 4);
-    fill_u32(ctx, ctx->d_small, 0u, 2);
+    fill_u32(ctx, ctx->d_small, 0u, 10);  // nc and the maxima (slots 0..4)
     return 0;
   }
-  const int64_t tiles = (n + 1023) / 1024;
-  const uint32_t ep = lookback_begin(ctx, 2 * tiles);
+  const int64_t tiles = (n + 1024 * NUM_PER - 1) / (1024 * NUM_PER);
+  const uint32_t ep = lookback_begin(ctx, 4 * tiles);  // 2 chains + tile maxima
   launch(ctx, "number_scan", (double)sizeof(OffT) * (n + 1) * 2 + 16.0 * n, k_number<OffT>,
          dim3((unsigned)tiles), dim3(1024), 0, n, (const OffT *)L.xadj, match, (const int32_t *)L.vwgt,
-         cmap, rep, segoff, cvwgt, ctx->d_status, ctx->d_status + tiles, ctx->d_ctr, ctx->d_small, ep);
+         cmap, rep, segoff, cvwgt, ctx->d_status, ctx->d_status + tiles, ctx->d_status + 2 * tiles, ctx->d_ctr,
+         ctx->d_small, ep);
   launch(ctx, "number_fill", 12.0 * n, k_number_fill, dim3(nblocks(n, 256)), dim3(256), 0, n, match,
-         cmap);
+         cmap, (const uint64_t *)(ctx->d_status + 2 * tiles), tiles, ctx->d_small);
   if (!wait) return -1;
   CM_HIP(hipMemcpyAsync(ctx->h_small, ctx->d_small, sizeof(int64_t), hipMemcpyDeviceToHost,
                         ctx->stream));
@@ -114,7 +192,7 @@ static int64_t number_impl(cm_ctx *ctx, const Level &L, const int32_t *match, in
   return ctx->h_small[0];
 }
 
-int64_t number(cm_ctx *ctx, const Level &L, const int32_t *d_match, int32_t *d_cmap,
+int64_t number(cm_ctx *ctx, const Level &L, int32_t *d_match, int32_t *d_cmap,
                int32_t *d_rep, void *d_segoff, int32_t *d_cvwgt, bool wait)
 {
   return L.off64 ? number_impl(ctx, L, d_match, d_cmap, d_rep, (int64_t *)d_segoff, d_cvwgt, wait)
@@ -502,16 +580,29 @@ __device__ __forceinline__ int merge_regs(int32_t c, int32_t v, int32_t u, const
                                           const int32_t *__restrict__ cmap, uint64_t (&k)[N],
                                           uint64_t (&p)[N])
 {
+  // Loads are unconditional (invalid slots read entry 0 / vertex 0 and are
+  // discarded), so all of them issue before the first wait; conditional
+  // loads would each get a branch and a wait of their own.
   const int64_t b0 = (int64_t)xadj[v];
-  const int d0 = (int)((int64_t)xadj[v + 1] - b0);
-  const int64_t b1 = u != v ? (int64_t)xadj[u] : 0;
-  const int d1 = u != v ? (int)((int64_t)xadj[u + 1] - b1) : 0;
+  const int64_t e0 = (int64_t)xadj[v + 1];
+  const int64_t b1 = (int64_t)xadj[u];
+  const int64_t e1 = (int64_t)xadj[u + 1];
+  const int d0 = (int)(e0 - b0);
+  const int d1 = u != v ? (int)(e1 - b1) : 0;
+  int32_t a[N], wt[N];
 #pragma unroll
   for (int i = 0; i < N; ++i) {
     const bool valid = i < d0 + d1;
-    const int64_t j = i < d0 ? b0 + i : b1 + (i - d0);
-    k[i] = valid ? (((uint64_t)(uint32_t)cmap[adjncy[j]] << 32) | (uint32_t)i) : ~0ull;
-    p[i] = valid ? (uint64_t)(uint32_t)adjwgt[j] : 0ull;
+    const int64_t j = valid ? (i < d0 ? b0 + i : b1 + (i - d0)) : 0;
+    a[i] = adjncy[j];
+    wt[i] = adjwgt[j];
+  }
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const bool valid = i < d0 + d1;
+    const int32_t cu = cmap[valid ? a[i] : 0];
+    k[i] = valid ? (((uint64_t)(uint32_t)cu << 32) | (uint32_t)i) : ~0ull;
+    p[i] = valid ? (uint64_t)(uint32_t)wt[i] : 0ull;
   }
   reg_bitonic<N>(k, p);
   // run sums (runs are adjacent after the sort), heads = first of a run
@@ -563,8 +654,11 @@ __device__ __forceinline__ int merge_regs(int32_t c, int32_t v, int32_t u, const
 // in cdeg.  A look-back scan of the degrees gives cxadj, and the short rows
 // are written straight to their final place: the fine rows are read once
 // and the coarse rows written once.
+constexpr int SMALL_TILE = 1024;   // coarse vertices per tile of k_contract_small
+constexpr int SMALL_STAGE = 8192;  // output entries staged in LDS per window
+
 template <bool FS, typename OffT>
-__global__ __launch_bounds__(256) void k_contract_small(const int64_t *__restrict__ ncp,
+__global__ __launch_bounds__(SMALL_TILE) void k_contract_small(const int64_t *__restrict__ ncp,
                                                         const OffT *__restrict__ segoff,
                                                         const int32_t *__restrict__ rep,
                                                         const int32_t *__restrict__ match,
@@ -575,22 +669,29 @@ __global__ __launch_bounds__(256) void k_contract_small(const int64_t *__restric
                                                         const int32_t *__restrict__ cdeg_large,
                                                         OffT *__restrict__ cxadj, int32_t *__restrict__ cadjncy,
                                                         int32_t *__restrict__ cadjwgt, uint64_t *st,
-                                                        unsigned *ctr, int64_t *total, int64_t *maxdeg,
-                                                        uint32_t epoch)
+                                                        unsigned *ctr, int64_t *total, uint32_t epoch)
 {
-  __shared__ uint64_t sh[4];
+  __shared__ uint64_t sh[SMALL_TILE / CM_WAVE];
   __shared__ uint64_t s_excl;
+  __shared__ int32_t stage_a[SMALL_STAGE], stage_w[SMALL_STAGE];
   const int64_t nc = *ncp;
   const int tile = tile_ticket(ctr);
-  const int64_t c = (int64_t)tile * 256 + threadIdx.x;
+  // the grid may be sized by a bound on nc: tiles past the end take no part
+  // in the look-back chain (no later tile waits for them)
+  if ((int64_t)tile * SMALL_TILE >= nc) return;
+  const int64_t c = (int64_t)tile * SMALL_TILE + threadIdx.x;
   uint64_t k[REG_SEG], p[REG_SEG];
   int deg = 0, len = 0;
   bool small = false;
+  // loads by a clamped index, all issued before the first wait
+  const int64_t cc = c < nc ? c : nc - 1;
+  const int64_t s0 = (int64_t)segoff[cc], s1 = (int64_t)segoff[cc + 1];
+  const int32_t v = rep[cc], dl = cdeg_large[cc];
+  const int32_t u = match[v];
   if (c < nc) {
-    len = (int)((int64_t)segoff[c + 1] - (int64_t)segoff[c]);
+    len = (int)(s1 - s0);
     if (len <= REG_SEG) {
       small = true;
-      const int32_t v = rep[c], u = match[v];
       if (len <= 8) {
         uint64_t k8[8], p8[8];
         deg = merge_regs<8, FS, OffT>((int32_t)c, v, u, xadj, adjncy, adjwgt, cmap, k8, p8);
@@ -600,17 +701,11 @@ __global__ __launch_bounds__(256) void k_contract_small(const int64_t *__restric
         deg = merge_regs<REG_SEG, FS, OffT>((int32_t)c, v, u, xadj, adjncy, adjwgt, cmap, k, p);
       }
     } else {
-      deg = cdeg_large[c];
+      deg = dl;
     }
   }
-  // longest coarse row: one atomic per wave, only when it raises the maximum
-  {
-    const int64_t dm = wave_max<int64_t>(deg);
-    if (lane() == 0 && dm > __hip_atomic_load(maxdeg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-      atomicMax((unsigned long long *)maxdeg, (unsigned long long)dm);
-  }
   uint64_t tot;
-  const uint64_t ex = block256_excl_scan<uint64_t>((uint64_t)deg, tot, sh);
+  const uint64_t ex = block_excl_scan<uint64_t, SMALL_TILE / CM_WAVE>((uint64_t)deg, tot, sh);
   if (threadIdx.x < CM_WAVE) {
     const uint64_t e = lookback(st, tile, tot, epoch);
     if (threadIdx.x == 0) s_excl = e;
@@ -623,15 +718,42 @@ __global__ __launch_bounds__(256) void k_contract_small(const int64_t *__restric
       cxadj[nc] = (OffT)(off + deg);
       total[0] = (int64_t)(off + deg);
     }
-    if (small) {
+  }
+  // Rows go out through LDS: each window of STAGE entries of the tile's
+  // output range is filled by the rows that fall in it, then stored by
+  // consecutive threads.  Slots of longer rows (written later by their own
+  // pass) keep the sentinel and are skipped.  Windows cover the range up to
+  // the end of the last short row.
+  const uint64_t my_end = (small && c < nc) ? ex + (uint64_t)deg : 0;
+  uint64_t span = wave_max<uint64_t>(my_end);
+  if (lane() == 0) sh[threadIdx.x >> 6] = span;
+  __syncthreads();
+  span = 0;
+#pragma unroll
+  for (int i = 0; i < SMALL_TILE / CM_WAVE; ++i) span = sh[i] > span ? sh[i] : span;
+  for (uint64_t w0 = 0; w0 < span; w0 += SMALL_STAGE) {
+    const int wlen = (int)(span - w0 < (uint64_t)SMALL_STAGE ? span - w0 : (uint64_t)SMALL_STAGE);
+    for (int k = threadIdx.x; k < wlen; k += SMALL_TILE) stage_a[k] = -1;
+    __syncthreads();
+    if (small && c < nc) {
 #pragma unroll
       for (int i = 0; i < REG_SEG; ++i) {
-        if (i < deg) {
-          cadjncy[off + i] = (int32_t)(p[i] >> 32);
-          cadjwgt[off + i] = (int32_t)(uint32_t)p[i];
+        const int64_t pos = (int64_t)(ex + i) - (int64_t)w0;
+        if (i < deg && pos >= 0 && pos < wlen) {
+          stage_a[pos] = (int32_t)(p[i] >> 32);
+          stage_w[pos] = (int32_t)(uint32_t)p[i];
         }
       }
     }
+    __syncthreads();
+    for (int k = threadIdx.x; k < wlen; k += SMALL_TILE) {
+      const int32_t a = stage_a[k];
+      if (a != -1) {
+        cadjncy[s_excl + w0 + k] = a;
+        cadjwgt[s_excl + w0 + k] = stage_w[k];
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -671,15 +793,16 @@ static int64_t contract_impl(cm_ctx *ctx, const Level &L, const int32_t *match, 
     int32_t *cdeg = (int32_t *)ctx->pool.alloc(sizeof(int32_t) * ncb);
     HashContract hc;
     try {
-      if (hashed) hash_prepare_impl<OffT>(ctx, L, match, cmap, ncb, d_nc, rep, segoff, cdeg, hc);
-      const int64_t tiles = (ncb + 255) / 256;
+      if (hashed)
+        hash_prepare_impl<OffT>(ctx, L, match, cmap, ncb, d_nc, rep, segoff, cdeg, L.maxdeg < 0 ? -1 : 2 * L.maxdeg,
+                                hc);
+      const int64_t tiles = (ncb + SMALL_TILE - 1) / SMALL_TILE;
       const uint32_t ep = lookback_begin(ctx, tiles);
-      fill_u32(ctx, ctx->d_small + 4, 0u, 2);
       const double small_bytes = 8.0 * m + 12.0 * n + (double)sizeof(OffT) * (n + 1) + 8.0 * ncb +
                                  (double)sizeof(OffT) * (ncb + 1);
-      launch(ctx, "contract_small", small_bytes, k_contract_small<true, OffT>, dim3((unsigned)tiles), dim3(256),
+      launch(ctx, "contract_small", small_bytes, k_contract_small<true, OffT>, dim3((unsigned)tiles), dim3(SMALL_TILE),
              0, d_nc, segoff, rep, match, xadj, L.adjncy, L.adjwgt, cmap, (const int32_t *)cdeg, cxadj, cadjncy,
-             cadjwgt, ctx->d_status, ctx->d_ctr, ctx->d_small + 1, ctx->d_small + 4, ep);
+             cadjwgt, ctx->d_status, ctx->d_ctr, ctx->d_small + 1, ep);
       ctx->h_small[6] = 0;
       if (hashed) hash_emit_impl<OffT>(ctx, segoff, cdeg, cxadj, cadjncy, cadjwgt, hc);
       CM_HIP(hipMemcpyAsync(ctx->h_small, ctx->d_small, 5 * sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
@@ -752,19 +875,18 @@ static int64_t contract_impl(cm_ctx *ctx, const Level &L, const int32_t *match, 
   }
   // fused short-segment merge + look-back cxadj + direct row writes
   fill_u64(ctx, ctx->d_small + 7, (uint64_t)nc, 1);
-  const int64_t tiles = (nc + 255) / 256;
+  const int64_t tiles = (nc + SMALL_TILE - 1) / SMALL_TILE;
   const uint32_t ep = lookback_begin(ctx, tiles);
-  fill_u32(ctx, ctx->d_small + 4, 0u, 2);
   const double small_bytes = 8.0 * m + 12.0 * n + (double)sizeof(OffT) * (n + 1) + 8.0 * nc +
                              (double)sizeof(OffT) * (nc + 1);
   if (fs)
-    launch(ctx, "contract_small", small_bytes, k_contract_small<true, OffT>, dim3((unsigned)tiles), dim3(256),
+    launch(ctx, "contract_small", small_bytes, k_contract_small<true, OffT>, dim3((unsigned)tiles), dim3(SMALL_TILE),
            0, (const int64_t *)(ctx->d_small + 7), segoff, rep, match, xadj, L.adjncy, L.adjwgt, cmap, (const int32_t *)cdeg, cxadj, cadjncy,
-           cadjwgt, ctx->d_status, ctx->d_ctr, ctx->d_small + 1, ctx->d_small + 4, ep);
+           cadjwgt, ctx->d_status, ctx->d_ctr, ctx->d_small + 1, ep);
   else
     launch(ctx, "contract_small", small_bytes, k_contract_small<false, OffT>, dim3((unsigned)tiles),
-           dim3(256), 0, (const int64_t *)(ctx->d_small + 7), segoff, rep, match, xadj, L.adjncy, L.adjwgt, cmap, (const int32_t *)cdeg, cxadj,
-           cadjncy, cadjwgt, ctx->d_status, ctx->d_ctr, ctx->d_small + 1, ctx->d_small + 4, ep);
+           dim3(SMALL_TILE), 0, (const int64_t *)(ctx->d_small + 7), segoff, rep, match, xadj, L.adjncy, L.adjwgt, cmap, (const int32_t *)cdeg, cxadj,
+           cadjncy, cadjwgt, ctx->d_status, ctx->d_ctr, ctx->d_small + 1, ep);
   if (sorted_path) {
 #define CM_EMIT(FSV)                                                                                      \
   if (nmid)                                                                                              \

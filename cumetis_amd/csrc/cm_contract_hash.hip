@@ -111,10 +111,15 @@ __global__ __launch_bounds__(256) void k_hc_select(const int64_t *__restrict__ n
   const int64_t nc = *ncp;
   const int64_t c0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * HC_SELECT;
   unsigned cnt[HC_N_COUNTERS] = {0, 0, 0, 0, 0};
+  // the HC_SELECT + 1 offsets load at once (clamped to nc)
+  int64_t so[HC_SELECT + 1];
+#pragma unroll
+  for (int t = 0; t <= HC_SELECT; ++t) so[t] = (int64_t)segoff[c0 + t < nc ? c0 + t : nc];
+#pragma unroll
   for (int t = 0; t < HC_SELECT; ++t) {
     const int64_t c = c0 + t;
     if (c >= nc) break;
-    const int64_t len = (int64_t)segoff[c + 1] - (int64_t)segoff[c];
+    const int64_t len = so[t + 1] - so[t];
     if (len <= REG_SEG) continue;
     cnt[HC_N_CHUNK] += (unsigned)((len + HC_CHUNK - 1) / HC_CHUNK);
     if (len <= HC_WAVE_MAX) ++cnt[HC_N_WAVE];
@@ -144,10 +149,11 @@ __global__ __launch_bounds__(256) void k_hc_select(const int64_t *__restrict__ n
     at[i] = s_base[i] + inc[i] - cnt[i];
     for (int j = 0; j < w; ++j) at[i] += s_w[i][j];
   }
+#pragma unroll
   for (int t = 0; t < HC_SELECT; ++t) {
     const int64_t c = c0 + t;
     if (c >= nc) break;
-    const int64_t len = (int64_t)segoff[c + 1] - (int64_t)segoff[c];
+    const int64_t len = so[t + 1] - so[t];
     if (len <= REG_SEG) continue;
     const int nch = (int)((len + HC_CHUNK - 1) / HC_CHUNK);
     const unsigned first = at[HC_N_CHUNK];
@@ -191,19 +197,23 @@ __global__ __launch_bounds__(256) void k_hc_gather(const SegChunk *__restrict__ 
     const int64_t b1 = u != v ? (int64_t)xadj[u] : 0;
     const int64_t i0 = (int64_t)ch.k * HC_CHUNK, i1 = len < i0 + HC_CHUNK ? len : i0 + HC_CHUNK;
     for (int64_t ib = i0 + l; ib < i1; ib += SCAN * CM_WAVE) {
-      int32_t a[SCAN], w[SCAN];
+      // clamped, unconditional loads: all issue before the first wait
+      int32_t a[SCAN], w[SCAN], cv[SCAN];
 #pragma unroll
       for (int t = 0; t < SCAN; ++t) {
-        const int64_t i = ib + (int64_t)t * CM_WAVE;
+        int64_t i = ib + (int64_t)t * CM_WAVE;
+        i = i < i1 ? i : i1 - 1;
         const int64_t j = i < d0 ? b0 + i : b1 + (i - d0);
-        a[t] = i < i1 ? adjncy[j] : 0;
-        w[t] = i < i1 ? adjwgt[j] : 0;
+        a[t] = adjncy[j];
+        w[t] = adjwgt[j];
       }
+#pragma unroll
+      for (int t = 0; t < SCAN; ++t) cv[t] = cmap[a[t]];
 #pragma unroll
       for (int t = 0; t < SCAN; ++t) {
         const int64_t i = ib + (int64_t)t * CM_WAVE;
         if (i < i1) {
-          cval[s0 + i] = cmap[a[t]];
+          cval[s0 + i] = cv[t];
           wv[s0 + i] = w[t];
         }
       }
@@ -550,8 +560,9 @@ __global__ __launch_bounds__(256) void k_hc_emit(const SegChunk *__restrict__ ch
 template <typename OffT>
 void hash_prepare_impl(cm_ctx *ctx, const Level &L, const int32_t *match, const int32_t *cmap, int64_t nc,
                        const int64_t *d_nc, const int32_t *rep, const OffT *segoff, int32_t *cdeg,
-                       HashContract &hc)
+                       int64_t seg_max, HashContract &hc)
 {
+  if (seg_max < 0) seg_max = INT64_MAX;
   const int64_t m = L.m;
   const OffT *xadj = (const OffT *)L.xadj;
   // bounds of the item counts (segments above REG_SEG hold at most m entries)
@@ -584,10 +595,11 @@ void hash_prepare_impl(cm_ctx *ctx, const Level &L, const int32_t *match, const 
   launch(ctx, "hc_dedup_wave", 0.0, k_hc_dedup_wave<OffT>, dim3((unsigned)std::min<int64_t>(4096, (nseg + 3) / 4)),
          dim3(256), 0, (const int32_t *)wave_items, (const unsigned *)hc.ctr, segoff, (const int32_t *)cval,
          (const int32_t *)wv, hc.tc, hc.tw, cdeg);
-  launch(ctx, "hc_dedup_block", 0.0, k_hc_dedup_block<OffT>, dim3((unsigned)std::min<int64_t>(2048, nblk)),
-         dim3(256), 0, (const int32_t *)block_items, (const unsigned *)hc.ctr, segoff, (const int32_t *)cval,
-         (const int32_t *)wv, hc.tc, hc.tw, cdeg);
-  if (m > HC_BLOCK_MAX) {  // a hub needs more than HC_BLOCK_MAX entries
+  if (seg_max > HC_WAVE_MAX)
+    launch(ctx, "hc_dedup_block", 0.0, k_hc_dedup_block<OffT>, dim3((unsigned)std::min<int64_t>(2048, nblk)),
+           dim3(256), 0, (const int32_t *)block_items, (const unsigned *)hc.ctr, segoff, (const int32_t *)cval,
+           (const int32_t *)wv, hc.tc, hc.tw, cdeg);
+  if (m > HC_BLOCK_MAX && seg_max > HC_BLOCK_MAX) {  // a hub needs more than HC_BLOCK_MAX entries
     uint8_t *head = (uint8_t *)al(m);
     int32_t *hw = (int32_t *)al(4 * m);
     int32_t *ccnt = (int32_t *)al(4 * nchunk), *cself = (int32_t *)al(4 * nchunk), *coff = (int32_t *)al(4 * nchunk);
@@ -618,10 +630,10 @@ void hash_emit_impl(cm_ctx *ctx, const OffT *segoff, const int32_t *cdeg, const 
 }
 
 template void hash_prepare_impl<int32_t>(cm_ctx *, const Level &, const int32_t *, const int32_t *, int64_t,
-                                         const int64_t *, const int32_t *, const int32_t *, int32_t *,
+                                         const int64_t *, const int32_t *, const int32_t *, int32_t *, int64_t,
                                          HashContract &);
 template void hash_prepare_impl<int64_t>(cm_ctx *, const Level &, const int32_t *, const int32_t *, int64_t,
-                                         const int64_t *, const int32_t *, const int64_t *, int32_t *,
+                                         const int64_t *, const int32_t *, const int64_t *, int32_t *, int64_t,
                                          HashContract &);
 template void hash_emit_impl<int32_t>(cm_ctx *, const int32_t *, const int32_t *, const int32_t *, int32_t *,
                                       int32_t *, const HashContract &);

@@ -23,11 +23,12 @@ struct HashContract {
 };
 
 // Rows of every segment above REG_SEG into the staging area, their lengths
-// into cdeg[c].  Queues work only; no host synchronisation.
+// into cdeg[c].  Queues work only; no host synchronisation.  seg_max bounds
+// the segment length (-1 unknown); passes for longer classes are skipped.
 template <typename OffT>
 void hash_prepare_impl(cm_ctx *ctx, const Level &L, const int32_t *match, const int32_t *cmap, int64_t nc_bound,
                        const int64_t *d_nc, const int32_t *rep, const OffT *segoff, int32_t *cdeg,
-                       HashContract &hc);
+                       int64_t seg_max, HashContract &hc);
 // Staging area -> final rows once cxadj is written; queues the overflow flag
 // into h_small[6] for the caller's next synchronisation.
 template <typename OffT>

@@ -261,32 +261,64 @@ static void validate_view(const cm_graph_view *g)
 
 // Level 0 of a hierarchy from a view: device views are referenced (unit
 // weights are materialised), host views are copied into pool memory.
-// *mx = max(*mx, longest row); per wave one atomic, and only when the wave
-// holds a longer row than the current maximum
+// mx[0] = longest row, mx[-1] = largest vertex weight (vwgt != nullptr).
+// Tiles of 4096 rows pass running maxima along two look-back chains; the
+// last tile writes them (no word updated by two tiles).
 template <typename OffT>
-__global__ __launch_bounds__(256) void k_max_degree(int64_t n, const OffT *xadj, int64_t *mx)
+__global__ __launch_bounds__(1024) void k_max_degree(int64_t n, const OffT *xadj, const int32_t *vwgt, int64_t *mx,
+                                                     uint64_t *st, unsigned *ctr, uint32_t epoch)
 {
-  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int64_t d = v < n ? (int64_t)(xadj[v + 1] - xadj[v]) : 0;
-  d = wave_max(d);
-  if (lane() == 0 && d > __hip_atomic_load(mx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-    atomicMax((unsigned long long *)mx, (unsigned long long)d);
+  __shared__ uint64_t sh[16], shw[16];
+  const int tile = tile_ticket(ctr);
+  uint64_t d = 0, vw = 0;
+  for (int t = 0; t < 4; ++t) {
+    const int64_t v = ((int64_t)tile * 1024 + threadIdx.x) * 4 + t;
+    if (v < n) {
+      const uint64_t x = (uint64_t)(xadj[v + 1] - xadj[v]);
+      d = x > d ? x : d;
+      if (vwgt) {
+        const uint64_t y = (uint64_t)(uint32_t)vwgt[v];
+        vw = y > vw ? y : vw;
+      }
+    }
+  }
+  d = wave_max<uint64_t>(d);
+  vw = wave_max<uint64_t>(vw);
+  if (lane() == 0) {
+    sh[threadIdx.x >> 6] = d;
+    shw[threadIdx.x >> 6] = vw;
+  }
+  __syncthreads();
+  if (threadIdx.x < CM_WAVE) {
+    const uint64_t bm = wave_reduce<LbMax>(threadIdx.x < 16 ? sh[threadIdx.x] : 0);
+    const uint64_t bw = wave_reduce<LbMax>(threadIdx.x < 16 ? shw[threadIdx.x] : 0);
+    const uint64_t pm = lookback<LbMax>(st, tile, bm, epoch);
+    const uint64_t pw = lookback<LbMax>(st + gridDim.x, tile, bw, epoch);
+    if (threadIdx.x == 0 && tile == (int)gridDim.x - 1) {
+      mx[0] = (int64_t)(pm > bm ? pm : bm);
+      if (vwgt) mx[-1] = (int64_t)(pw > bw ? pw : bw);
+    }
+  }
 }
 
-// Queues the longest row of a CSR into d_small[4]; the caller reads it back
-// (h_small[4]) with its next synchronisation.
-void queue_max_degree(cm_ctx *ctx, int64_t n, bool off64, const void *xadj)
+// Queues the longest row of a CSR into d_small[4] and, for vwgt != nullptr,
+// the largest vertex weight into d_small[3]; the caller reads them back
+// (h_small[3..4]) with its next synchronisation.
+void queue_max_degree(cm_ctx *ctx, int64_t n, bool off64, const void *xadj, const int32_t *vwgt)
 {
-  fill_u32(ctx, ctx->d_small + 4, 0u, 2);
+  fill_u32(ctx, ctx->d_small + 3, 0u, 4);
   if (n > 0) {
+    const int64_t tiles = (n + 4095) / 4096;
+    const uint32_t ep = lookback_begin(ctx, 2 * tiles);
     if (off64)
-      launch(ctx, "max_degree", 0.0, k_max_degree<int64_t>, dim3(nblocks(n, 256)), dim3(256), 0, n,
-             (const int64_t *)xadj, ctx->d_small + 4);
+      launch(ctx, "max_degree", 0.0, k_max_degree<int64_t>, dim3((unsigned)tiles), dim3(1024), 0, n,
+             (const int64_t *)xadj, vwgt, ctx->d_small + 4, ctx->d_status, ctx->d_ctr, ep);
     else
-      launch(ctx, "max_degree", 0.0, k_max_degree<int32_t>, dim3(nblocks(n, 256)), dim3(256), 0, n,
-             (const int32_t *)xadj, ctx->d_small + 4);
+      launch(ctx, "max_degree", 0.0, k_max_degree<int32_t>, dim3((unsigned)tiles), dim3(1024), 0, n,
+             (const int32_t *)xadj, vwgt, ctx->d_small + 4, ctx->d_status, ctx->d_ctr, ep);
   }
-  CM_HIP(hipMemcpyAsync(ctx->h_small + 4, ctx->d_small + 4, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+  CM_HIP(hipMemcpyAsync(ctx->h_small + 3, ctx->d_small + 3, 2 * sizeof(int64_t), hipMemcpyDeviceToHost,
+                        ctx->stream));
 }
 
 static Level level0(cm_ctx *ctx, const cm_graph_view *g, cm_hier *h, bool validate = true)
@@ -322,10 +354,11 @@ static Level level0(cm_ctx *ctx, const cm_graph_view *g, cm_hier *h, bool valida
       if (L.n) CM_HIP(hipMemcpyAsync(L.vwgt, g->vwgt, 4 * L.n, hipMemcpyHostToDevice, ctx->stream));
     }
   }
-  queue_max_degree(ctx, L.n, L.off64, L.xadj);
+  queue_max_degree(ctx, L.n, L.off64, L.xadj, L.vwgt);  // vwgt: device copy or nullptr
   if (validate) check_graph(ctx, L.n, L.m, L.off64, L.xadj, L.adjncy, L.adjwgt);
   sync(ctx);
   L.maxdeg = ctx->h_small[4];
+  L.maxvw = L.vwgt ? ctx->h_small[3] : 1;
   if (!L.adjwgt) {
     L.adjwgt = (int32_t *)own(4 * (L.m > 0 ? L.m : 1));
     fill_u32(ctx, L.adjwgt, 1u, L.m);
@@ -376,7 +409,7 @@ static Level coarsen_level(cm_ctx *ctx, cm_hier *h, Level &F, int32_t maxvwgt, u
   const int64_t n = F.n, m = F.m;
   F.match = (int32_t *)own(4 * (n > 0 ? n : 1));
   F.cmap = (int32_t *)own(4 * (n > 0 ? n : 1));
-  F.rounds = match_handshake(ctx, F, maxvwgt, seed, max_rounds, F.match);
+  F.rounds = match_handshake(ctx, F, maxvwgt, seed, max_rounds, F.match, false);  // completed by number()
   int32_t *rep = (int32_t *)ctx->pool.alloc(4 * (n > 0 ? n : 1));
   void *segoff = ctx->pool.alloc(ob * (n + 1));
   Level C;
@@ -390,6 +423,7 @@ static Level coarsen_level(cm_ctx *ctx, cm_hier *h, Level &F, int32_t maxvwgt, u
   C.adjwgt = (int32_t *)own(4 * (m > 0 ? m : 1));
   C.m = contract(ctx, F, F.match, F.cmap, -1, rep, segoff, row_order, C.xadj, C.adjncy, C.adjwgt, &C.maxdeg,
                  &C.n);
+  C.maxvw = ctx->h_small[3];
   C.tvwgt = F.tvwgt;  // cu:1400
   C.owns_graph = true;
   ctx->pool.release(rep);
@@ -745,7 +779,7 @@ int cm_number(cm_ctx *ctx, int64_t n, const void *d_xadj, int32_t offset_type, c
   int32_t *rep = (int32_t *)ctx->pool.alloc(4 * (n > 0 ? n : 1));
   int32_t *cv = (int32_t *)ctx->pool.alloc(4 * (n > 0 ? n : 1));
   void *seg = ctx->pool.alloc(ob * (n + 1));
-  int64_t r = cm::number(ctx, L, d_match, d_cmap, rep, seg, cv);
+  int64_t r = cm::number(ctx, L, const_cast<int32_t *>(d_match), d_cmap, rep, seg, cv);  // complete: not written
   cm::sync(ctx);
   ctx->pool.release(vw);
   ctx->pool.release(rep);
@@ -775,7 +809,7 @@ int cm_contract(cm_ctx *ctx, const cm_graph_view *g, const int32_t *d_match, con
     // numbering recomputed to obtain the segment offsets; it must agree
     // with the caller's cmap, which is what the gather uses
     cm::check_match(ctx, n, d_match);
-    const int64_t nc2 = cm::number(ctx, L, d_match, cm2, rep, seg, d_cvwgt);
+    const int64_t nc2 = cm::number(ctx, L, const_cast<int32_t *>(d_match), cm2, rep, seg, d_cvwgt);  // complete
     CM_REQUIRE(nc2 == nc, CM_ERR_INVALID, "nc does not match the matching");
     cm::check_equal(ctx, n, cm2, d_cmap, "cmap must be the canonical numbering of match");
     out = cm::contract(ctx, L, d_match, d_cmap, nc, rep, seg, row_order, d_cxadj, d_cadjncy, d_cadjwgt);

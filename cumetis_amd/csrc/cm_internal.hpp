@@ -80,7 +80,8 @@ struct Level {
   int64_t vbase = 0;  // added to adjncy and match values
   int64_t ebase = 0;  // added to xadj values
   int64_t cbase = 0;  // added to cmap values
-  int64_t maxdeg = -1;  // longest row (-1 unknown)
+  int64_t maxdeg = -1;  // bound on the longest row (-1 unknown)
+  int64_t maxvw = -1;   // largest vertex weight (-1 unknown)
 };
 
 }  // namespace cm
@@ -177,21 +178,25 @@ struct MatchOwners {
 
 // ---- subsystem entry points (device pointers, ctx stream) ----
 // (a) matching; returns rounds.  match[] ends complete (partner or self).
+// finish=false leaves unmatched vertices at -1 for number() to complete.
 int match_handshake(cm_ctx *ctx, const Level &L, int32_t maxvwgt, uint64_t seed, int max_rounds,
-                    int32_t *d_match);
-int match_handshake(cm_ctx *ctx, const Level &L, const MatchOwners &own, int max_rounds,
-                    int32_t *d_match);
+                    int32_t *d_match, bool finish = true);
+int match_handshake(cm_ctx *ctx, const Level &L, const MatchOwners &own, int max_rounds, int32_t *d_match,
+                    bool finish = true);
 // (b) numbering: cmap for every fine vertex; rep/segoff/cvwgt per coarse vertex.
 //     Returns nc; segoff[nc] = m.
 // With wait=false nc stays in d_small[0] (read it back with the next
 // synchronisation) and -1 is returned.
-int64_t number(cm_ctx *ctx, const Level &L, const int32_t *d_match, int32_t *d_cmap,
+// match entries still -1 (matching run with finish=false) are completed as
+// self matches.
+int64_t number(cm_ctx *ctx, const Level &L, int32_t *d_match, int32_t *d_cmap,
                int32_t *d_rep, void *d_segoff, int32_t *d_cvwgt, bool wait = true);
-void queue_max_degree(cm_ctx *ctx, int64_t n, bool off64, const void *xadj);
+void queue_max_degree(cm_ctx *ctx, int64_t n, bool off64, const void *xadj, const int32_t *vwgt = nullptr);
 // (c) contraction of L into coarse CSR; returns mc.  cxadj/cadjncy/cadjwgt
 //     must hold nc+1 / L.m / L.m entries.  nc = -1: the count left on the
 //     device by number(wait=false); cxadj then holds L.n+1 entries and *ncp
-//     receives nc.
+//     receives nc.  After any number() + contract() pair, h_small[3] holds
+//     the largest coarse vertex weight.
 int64_t contract(cm_ctx *ctx, const Level &L, const int32_t *d_match, const int32_t *d_cmap,
                  int64_t nc, const int32_t *d_rep, const void *d_segoff, int row_order,
                  void *d_cxadj, int32_t *d_cadjncy, int32_t *d_cadjwgt, int64_t *maxdeg = nullptr,

@@ -55,13 +55,16 @@ __device__ __forceinline__ bool key_better(uint64_t a1, uint64_t a2, uint64_t b1
   return a1 > b1 || (a1 == b1 && a2 > b2);
 }
 
-__global__ __launch_bounds__(256) void k_match_init(int64_t n, int32_t *match, int32_t *prop)
+__global__ __launch_bounds__(256) void k_match_init(int64_t n, int32_t *match, int32_t *prop, unsigned *flags,
+                                                    int64_t nflags)
 {
   const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (v < n) {
     match[v] = -1;
     prop[v] = -1;
+    prop[n + v] = -1;  // both halves: an entry never read back is still a valid value
   }
+  for (int64_t i = v; i < nflags; i += (int64_t)gridDim.x * blockDim.x) flags[i] = 0u;
 }
 
 // Long rows are cut into chunks of up to LONG_CHUNK entries, one wave per
@@ -124,11 +127,14 @@ __device__ __forceinline__ void scan_row(int64_t v, OffT b, OffT e, int l0, int 
                                          const int32_t *__restrict__ adjwgt,
                                          const int32_t *__restrict__ vwgt, int32_t vw, int32_t maxv,
                                          uint64_t seed, int64_t base, const int32_t *match,
-                                         const int32_t *prev, bool have_prev, int32_t &best,
+                                         const int32_t *prev, bool have_prev, bool check_w, int32_t &best,
                                          uint64_t &bk1, uint64_t &bk2)
 {
   // SCAN_BATCH entries per lane per step: their row loads and the gathers
   // that depend on them are issued together, so a lane waits once per batch
+  // Loads go through clamped indices (the last entry of the range, or v
+  // itself) and are unconditional: conditional loads compile to a branch and
+  // a wait each, which serialises the batch.
   constexpr int SCAN_BATCH = 4;
   for (OffT j0 = b + l0; j0 < e; j0 += (OffT)SCAN_BATCH * step) {
     int32_t u[SCAN_BATCH], mu[SCAN_BATCH], wu[SCAN_BATCH];
@@ -136,15 +142,19 @@ __device__ __forceinline__ void scan_row(int64_t v, OffT b, OffT e, int l0, int 
 #pragma unroll
     for (int t = 0; t < SCAN_BATCH; ++t) {
       const OffT j = j0 + (OffT)t * step;
-      u[t] = j < e ? adjncy[j] : -1;
-      w[t] = j < e ? (uint32_t)adjwgt[j] : 0u;
+      const OffT jj = j < e ? j : e - 1;
+      const int32_t a = adjncy[jj];
+      const uint32_t x = (uint32_t)adjwgt[jj];
+      u[t] = j < e ? a : -1;
+      w[t] = j < e ? x : 0u;
     }
     const uint32_t wmin = best != -2 ? (uint32_t)(bk1 >> 32) : 0u;
 #pragma unroll
     for (int t = 0; t < SCAN_BATCH; ++t) {
       const bool look = u[t] >= 0 && u[t] != (int32_t)v && w[t] >= wmin;
-      mu[t] = look ? match[u[t]] : 0;
-      wu[t] = look ? vwgt[u[t]] : 0;
+      const int32_t uu = look ? u[t] : (int32_t)v;
+      mu[t] = match[uu];
+      wu[t] = check_w ? vwgt[uu] : 0;  // without check_w no pair can exceed maxv
       if (!look) u[t] = -1;
     }
 #pragma unroll
@@ -204,7 +214,7 @@ __global__ __launch_bounds__(256) void k_round(int64_t n, const OffT *__restrict
                                                int32_t *match, const int32_t *prev, int32_t *cur,
                                                const LongChunk *chunks, const unsigned *nchunks,
                                                LongPartial part, unsigned *flags, int r, unsigned nshort,
-                                               int propose)
+                                               int propose, int64_t maxvw)
 {
   if (r >= 2 && !round_live(flags, r - 2)) return;
   const bool have_prev = r >= 1;
@@ -217,37 +227,45 @@ __global__ __launch_bounds__(256) void k_round(int64_t n, const OffT *__restrict
     __syncthreads();
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     bool changed = false, active = false;
-    if (v < n && match[v] == -1) {
+    // Loads in three dependency levels (by v; by p = prev[v]; by prev[p]),
+    // each through clamped indices so that a level issues as one batch.
+    const int64_t vc = v < n ? v : n - 1;
+    const int32_t mv = match[vc];
+    const int32_t p = have_prev ? prev[vc] : -1;
+    if (v < n && mv == -1) {
+      const int32_t pc = p >= 0 ? p : (int32_t)v;
+      const int32_t pp = have_prev ? prev[pc] : -1;  // prev[p]
+      const int32_t mp = match[pc];
+      const OffT b = xadj[v], e = xadj[v + 1];
+      const int32_t vw = vwgt[v];
+      const int32_t ppp = have_prev ? prev[pp >= 0 ? pp : pc] : -1;  // prev[prev[p]]
       bool done = false;
       if (have_prev) {
-        const int32_t p = prev[v];
         if (p == -2) {
           match[v] = (int32_t)v;
           done = true;
-        } else if (p >= 0 && prev[p] == (int32_t)v) {
+        } else if (p >= 0 && pp == (int32_t)v) {
           match[v] = p;
           done = true;
         }
       }
       changed = done;
       active = !done;
-      if (!done && propose) {
-        const OffT b = xadj[v], e = xadj[v + 1];
-        if ((int64_t)(e - b) <= LONG_DEG) {
-          const int32_t p = have_prev ? prev[v] : -1;
-          if (p >= 0 && match[p] == -1 && !accepted_prev(prev, p)) {
-            cur[v] = p;  // still unmatched, so still the best
-          } else {
-            int32_t maxv;
-            uint64_t seed;
-            int64_t base;
-            owner_params(own, v, maxv, seed, base);
-            int32_t best = -2;
-            uint64_t bk1 = 0, bk2 = 0;
-            scan_row<OffT>(v, b, e, 0, 1, adjncy, adjwgt, vwgt, vwgt[v], maxv, seed, base, match, prev,
-                           have_prev, best, bk1, bk2);
-            cur[v] = best;
-          }
+      if (!done && propose && (int64_t)(e - b) <= LONG_DEG) {
+        // p accepted in round r-1 (from the proposals alone): dead or mutual
+        const bool p_acc = pp == -2 || (pp >= 0 && ppp == p);
+        if (p >= 0 && mp == -1 && !p_acc) {
+          cur[v] = p;  // still unmatched, so still the best
+        } else {
+          int32_t maxv;
+          uint64_t seed;
+          int64_t base;
+          owner_params(own, v, maxv, seed, base);
+          int32_t best = -2;
+          uint64_t bk1 = 0, bk2 = 0;
+          scan_row<OffT>(v, b, e, 0, 1, adjncy, adjwgt, vwgt, vw, maxv, seed, base, match, prev, have_prev,
+                         maxvw < 0 || (int64_t)vw + maxvw > maxv, best, bk1, bk2);
+          cur[v] = best;
         }
       }
     }
@@ -276,9 +294,22 @@ __global__ __launch_bounds__(256) void k_round(int64_t n, const OffT *__restrict
   for (int64_t q = ((int64_t)(blockIdx.x - nshort) * blockDim.x + threadIdx.x) / CM_WAVE; q < nq; q += nw) {
     const LongChunk c = chunks[q];
     const int32_t v = c.v;
-    if (match[v] != -1 || (have_prev && accepted_prev(prev, v))) continue;
+    // loads by dependency level, as in the short path.  A vertex matched
+    // before this launch wrote no proposal in round r-1, so its prev entry
+    // is stale and must not be followed: test match first.
+    const int32_t mv = match[v];
     const int32_t p = have_prev ? prev[v] : -1;
-    if (p >= 0 && match[p] == -1 && !accepted_prev(prev, p)) {
+    if (mv != -1) continue;
+    const int32_t pc = p >= 0 ? p : v;
+    const int32_t pp = have_prev ? prev[pc] : -1;
+    const int32_t mp = match[pc];
+    const OffT xb = xadj[v], xe = xadj[v + 1];
+    const int32_t vw = vwgt[v];
+    const int32_t ppp = have_prev ? prev[pp >= 0 ? pp : pc] : -1;
+    const bool v_acc = have_prev && (p == -2 || (p >= 0 && pp == v));  // accepted_prev(prev, v)
+    if (v_acc) continue;
+    const bool p_acc = pp == -2 || (pp >= 0 && ppp == p);
+    if (p >= 0 && mp == -1 && !p_acc) {
       if (l == 0 && c.k == 0) cur[v] = p;
       continue;
     }
@@ -288,10 +319,10 @@ __global__ __launch_bounds__(256) void k_round(int64_t n, const OffT *__restrict
     owner_params(own, v, maxv, seed, base);
     int32_t best = -2;
     uint64_t bk1 = 0, bk2 = 0;
-    const OffT b = xadj[v] + (OffT)c.k * LONG_CHUNK, e0 = xadj[v + 1];
+    const OffT b = xb + (OffT)c.k * LONG_CHUNK, e0 = xe;
     const OffT e = e0 - b > (OffT)LONG_CHUNK ? b + (OffT)LONG_CHUNK : e0;
-    scan_row<OffT>(v, b, e, l, CM_WAVE, adjncy, adjwgt, vwgt, vwgt[v], maxv, seed, base, match, prev,
-                   have_prev, best, bk1, bk2);
+    scan_row<OffT>(v, b, e, l, CM_WAVE, adjncy, adjwgt, vwgt, vw, maxv, seed, base, match, prev, have_prev,
+                   maxvw < 0 || (int64_t)vw + maxvw > maxv, best, bk1, bk2);
     wave_best(best, bk1, bk2);
     if (c.nch == 1) {
       if (l == 0) cur[v] = best;
@@ -334,7 +365,7 @@ __global__ __launch_bounds__(256) void k_match_finish(int64_t n, int32_t *match)
 
 template <typename OffT>
 static int match_impl(cm_ctx *ctx, const Level &L, const MatchOwners &own, int max_rounds,
-                      int32_t *match)
+                      int32_t *match, bool finish)
 {
   const int64_t n = L.n;
   if (n == 0) return 0;
@@ -351,11 +382,11 @@ static int match_impl(cm_ctx *ctx, const Level &L, const MatchOwners &own, int m
   part.arrived = (unsigned *)ctx->pool.alloc(sizeof(unsigned) * cap);
   unsigned *flags = (unsigned *)ctx->pool.alloc(sizeof(unsigned) * (2 * (size_t)max_rounds + 2));
   unsigned *nchunks = flags + 2 * (size_t)max_rounds;
-  fill_u32(ctx, flags, 0u, 2 * (int64_t)max_rounds + 2);
   if (has_long) fill_u32(ctx, part.arrived, 0u, cap);
   const unsigned g = nblocks(n, 256);
   const double row_bytes = (double)sizeof(OffT) * (n + 1) + 8.0 * L.m + 8.0 * n;
-  launch(ctx, "match_init", 8.0 * n, k_match_init, dim3(g), dim3(256), 0, n, match, prop);
+  launch(ctx, "match_init", 8.0 * n, k_match_init, dim3(g), dim3(256), 0, n, match, prop, flags,
+         2 * (int64_t)max_rounds + 2);
   // extra blocks whose waves stride over the long-row chunks (none when the
   // level is known to have no long row)
   unsigned glong = 0;
@@ -372,7 +403,7 @@ static int match_impl(cm_ctx *ctx, const Level &L, const MatchOwners &own, int m
       const int32_t *prv = prop + (size_t)((r + 1) & 1) * n;
       launch(ctx, "match_round", r == 0 ? row_bytes : 0.0, k_round<OffT>, dim3(g + glong), dim3(256), 0, n,
              xadj, L.adjncy, L.adjwgt, L.vwgt, own, match, prv, cur, (const LongChunk *)chunks,
-             (const unsigned *)nchunks, part, flags, r, g, r < max_rounds ? 1 : 0);
+             (const unsigned *)nchunks, part, flags, r, g, r < max_rounds ? 1 : 0, L.maxvw);
     }
     if (r < 2) continue;
     // flags of every accepted round (0 .. r-2: round r-2 is accepted by
@@ -384,7 +415,7 @@ static int match_impl(cm_ctx *ctx, const Level &L, const MatchOwners &own, int m
   // rounds that did work: round 0, then every round whose predecessor changed
   // something and left active vertices
   const int acc = std::min(r - 1, max_rounds);  // accepted rounds 0 .. acc-1
-  launch(ctx, "match_finish", 4.0 * n, k_match_finish, dim3(g), dim3(256), 0, n, match);
+  if (finish) launch(ctx, "match_finish", 4.0 * n, k_match_finish, dim3(g), dim3(256), 0, n, match);
   int rounds = 1;
   while (rounds < acc && h[2 * (rounds - 1)] != 0 && h[2 * (rounds - 1) + 1] != 0) ++rounds;
   ctx->pool.release(prop);
@@ -398,20 +429,20 @@ static int match_impl(cm_ctx *ctx, const Level &L, const MatchOwners &own, int m
 }
 
 int match_handshake(cm_ctx *ctx, const Level &L, const MatchOwners &own, int max_rounds,
-                    int32_t *d_match)
+                    int32_t *d_match, bool finish)
 {
   if (max_rounds <= 0) max_rounds = 4096;
-  return L.off64 ? match_impl<int64_t>(ctx, L, own, max_rounds, d_match)
-                 : match_impl<int32_t>(ctx, L, own, max_rounds, d_match);
+  return L.off64 ? match_impl<int64_t>(ctx, L, own, max_rounds, d_match, finish)
+                 : match_impl<int32_t>(ctx, L, own, max_rounds, d_match, finish);
 }
 
 int match_handshake(cm_ctx *ctx, const Level &L, int32_t maxvwgt, uint64_t seed, int max_rounds,
-                    int32_t *d_match)
+                    int32_t *d_match, bool finish)
 {
   MatchOwners own{};
   own.maxv0 = maxvwgt;
   own.seed0 = seed;
-  return match_handshake(ctx, L, own, max_rounds, d_match);
+  return match_handshake(ctx, L, own, max_rounds, d_match, finish);
 }
 
 }  // namespace cm

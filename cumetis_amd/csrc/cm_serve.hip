@@ -209,9 +209,11 @@ static void union_step(cm_server *S, std::vector<Job *> &act)
   U.adjwgt = wgt;
   U.vwgt = vw;
   U.maxdeg = 0;
+  U.maxvw = 0;
   for (int s = 0; s < ns; ++s) {
-    const int64_t d = act[s]->h->levels.back().maxdeg;
-    U.maxdeg = (U.maxdeg < 0 || d < 0) ? -1 : std::max(U.maxdeg, d);
+    const Level &S = act[s]->h->levels.back();
+    U.maxdeg = (U.maxdeg < 0 || S.maxdeg < 0) ? -1 : std::max(U.maxdeg, S.maxdeg);
+    U.maxvw = (U.maxvw < 0 || S.maxvw < 0) ? -1 : std::max(U.maxvw, S.maxvw);
   }
   int32_t *match = (int32_t *)own(4 * (n > 0 ? n : 1)), *cmap = (int32_t *)own(4 * (n > 0 ? n : 1));
   MatchOwners mo;
@@ -219,7 +221,7 @@ static void union_step(cm_server *S, std::vector<Job *> &act)
   mo.maxv = d_maxv;
   mo.seed = d_seed;
   mo.vbase = d_dv;
-  const int rounds = match_handshake(ctx, U, mo, 0, match);
+  const int rounds = match_handshake(ctx, U, mo, 0, match, false);  // number() completes it
   int32_t *rep = (int32_t *)tmp(4 * (n > 0 ? n : 1));
   int64_t *segoff = (int64_t *)tmp(8 * (n + 1));
   int32_t *cvw = (int32_t *)own(4 * (n > 0 ? n : 1));
@@ -228,6 +230,7 @@ static void union_step(cm_server *S, std::vector<Job *> &act)
   int32_t *cadj = (int32_t *)own(4 * (m > 0 ? m : 1)), *cwgt = (int32_t *)own(4 * (m > 0 ? m : 1));
   int64_t cmaxdeg = -1;
   contract(ctx, U, match, cmap, nc, rep, segoff, act[0]->p.row_order, cxadj, cadj, cwgt, &cmaxdeg);
+  const int64_t cmaxvw = ctx->h_small[3];
   int64_t *d_cvb = (int64_t *)tmp(8 * (ns + 1)), *d_ceb = (int64_t *)tmp(8 * (ns + 1));
   launch(ctx, "serve_bases", 0.0, k_union_bases<int64_t>, dim3(nblocks(ns + 1, 256)), dim3(256), 0, ns,
          (const int64_t *)d_dv, (const int32_t *)cmap, (const int64_t *)cxadj, nc, d_cvb, d_ceb);
@@ -262,6 +265,7 @@ static void union_step(cm_server *S, std::vector<Job *> &act)
     C.vbase = cvb[s];
     C.ebase = ceb[s];
     C.maxdeg = cmaxdeg;  // of the union: a bound for each job
+    C.maxvw = cmaxvw;
     h->levels.push_back(C);
     h->shared_more.push_back(shared);
   }
@@ -310,7 +314,11 @@ static void loop_main(cm_server *S)
         if (j->n) CM_HIP(hipMemcpyAsync(L0.vwgt, j->vwgt.data(), 4 * j->n, hipMemcpyHostToDevice, ctx->stream));
         L0.tvwgt = j->tvwgt;
         L0.maxdeg = 0;
-        for (int64_t v = 0; v < j->n; ++v) L0.maxdeg = std::max(L0.maxdeg, j->xadj[v + 1] - j->xadj[v]);
+        L0.maxvw = 0;
+        for (int64_t v = 0; v < j->n; ++v) {
+          L0.maxdeg = std::max(L0.maxdeg, j->xadj[v + 1] - j->xadj[v]);
+          L0.maxvw = std::max<int64_t>(L0.maxvw, j->vwgt[v]);
+        }
         j->h->levels.push_back(L0);
         j->cur_n = j->n;
       }
