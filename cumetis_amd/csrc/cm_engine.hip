@@ -128,6 +128,20 @@ void ensure_status(cm_ctx *ctx, size_t words)
   ctx->epoch = 0;
 }
 
+void ensure_host_flags(cm_ctx *ctx, size_t words)
+{
+  if (words <= ctx->h_flags_cap) return;
+  size_t cap = std::max<size_t>(words, 1024);
+  if (ctx->h_flags) {
+    sync(ctx);  // a queued copy may still target the old buffer
+    CM_HIP(hipHostFree(ctx->h_flags));
+    ctx->h_flags = nullptr;
+    ctx->h_flags_cap = 0;
+  }
+  CM_HIP(hipHostMalloc((void **)&ctx->h_flags, sizeof(unsigned) * cap, hipHostMallocDefault));
+  ctx->h_flags_cap = cap;
+}
+
 uint32_t lookback_begin(cm_ctx *ctx, size_t words)
 {
   ensure_status(ctx, words);
@@ -397,6 +411,10 @@ int ref_threshold(int64_t n, int nparts)
 }
 
 // One coarsening level: match -> number -> contract (cu:1699-1806, 1291-1406).
+// Levels with at most this many entries run their matching rounds without a
+// host check before the contraction (see coarsen_level).
+constexpr int64_t SPECULATE_EDGES = 1 << 24;
+
 static Level coarsen_level(cm_ctx *ctx, cm_hier *h, Level &F, int32_t maxvwgt, uint64_t seed,
                            int max_rounds, int row_order)
 {
@@ -409,24 +427,54 @@ static Level coarsen_level(cm_ctx *ctx, cm_hier *h, Level &F, int32_t maxvwgt, u
   const int64_t n = F.n, m = F.m;
   F.match = (int32_t *)own(4 * (n > 0 ? n : 1));
   F.cmap = (int32_t *)own(4 * (n > 0 ? n : 1));
-  F.rounds = match_handshake(ctx, F, maxvwgt, seed, max_rounds, F.match, false);  // completed by number()
   int32_t *rep = (int32_t *)ctx->pool.alloc(4 * (n > 0 ? n : 1));
+  int32_t *mstate = (int32_t *)ctx->pool.alloc(4 * (n > 0 ? n : 1));  // the rounds' working match
   void *segoff = ctx->pool.alloc(ob * (n + 1));
   Level C;
   C.off64 = F.off64;
   C.vwgt = (int32_t *)own(4 * (n > 0 ? n : 1));
-  // nc stays on the device until the contraction's one synchronisation;
-  // the coarse offsets are sized by the bound n
-  number(ctx, F, F.match, F.cmap, rep, segoff, C.vwgt, false);
-  C.xadj = own(ob * (n + 1));
+  C.xadj = own(ob * (n + 1));  // sized by the bound n: nc is read back at the level's end
   C.adjncy = (int32_t *)own(4 * (m > 0 ? m : 1));
   C.adjwgt = (int32_t *)own(4 * (m > 0 ? m : 1));
-  C.m = contract(ctx, F, F.match, F.cmap, -1, rep, segoff, row_order, C.xadj, C.adjncy, C.adjwgt, &C.maxdeg,
-                 &C.n);
-  C.maxvw = ctx->h_small[3];
+  MatchOwners mo{};
+  mo.maxv0 = maxvwgt;
+  mo.seed0 = seed;
+  MatchRun *mr = match_begin(ctx, F, mo, max_rounds, mstate);
+  try {
+    // Small levels queue one batch of rounds and go straight on to the
+    // numbering and contraction of the finished copy of the match: the
+    // contraction's synchronisation also brings back the round flags.  In
+    // the rare case that the batch did not finish, the rounds continue from
+    // their state and the level is redone.  Large levels, where a
+    // synchronisation costs little, check the rounds first.
+    const bool speculate = m <= SPECULATE_EDGES;
+    if (n > 0) {
+      match_queue(ctx, mr, MATCH_BATCH);
+      if (!speculate) sync(ctx);
+    }
+    for (int pass = 0;; ++pass) {
+      if (!speculate || pass > 0) match_finish(ctx, mr);
+      match_copy_finished(ctx, mr, F.match);
+      // nc stays on the device until the contraction's one synchronisation
+      number(ctx, F, F.match, F.cmap, rep, segoff, C.vwgt, false);
+      C.m = contract(ctx, F, F.match, F.cmap, -1, rep, segoff, row_order, C.xadj, C.adjncy, C.adjwgt,
+                     &C.maxdeg, &C.n);
+      C.maxvw = ctx->h_small[3];
+      if (match_done(ctx, mr)) break;
+    }
+    F.rounds = match_finish(ctx, mr);
+  } catch (...) {
+    match_end(ctx, mr);
+    ctx->pool.release(rep);
+    ctx->pool.release(mstate);
+    ctx->pool.release(segoff);
+    throw;
+  }
+  match_end(ctx, mr);
   C.tvwgt = F.tvwgt;  // cu:1400
   C.owns_graph = true;
   ctx->pool.release(rep);
+  ctx->pool.release(mstate);
   ctx->pool.release(segoff);
   return C;
 }
@@ -530,6 +578,7 @@ int cm_ctx_destroy(cm_ctx *ctx)
   }
   for (auto e : ctx->ev_free) (void)hipEventDestroy(e);
   if (ctx->h_small) (void)hipHostFree(ctx->h_small);
+  if (ctx->h_flags) (void)hipHostFree(ctx->h_flags);
   if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return CM_OK;

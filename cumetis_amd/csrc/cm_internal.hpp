@@ -117,6 +117,8 @@ struct cm_ctx {
   unsigned *d_ctr = nullptr;      // tile tickets / counters (CTR_SLOTS)
   int64_t *d_small = nullptr;     // totals written by kernels (SMALL_SLOTS)
   int64_t *h_small = nullptr;     // pinned mirror of d_small
+  unsigned *h_flags = nullptr;    // pinned copy of matching round flags
+  size_t h_flags_cap = 0;         // in words
   uint64_t *d_status = nullptr;   // look-back status words
   size_t status_cap = 0;          // in words
   uint32_t epoch = 0;             // look-back launch epoch (1 .. 2^20-1)
@@ -132,6 +134,7 @@ constexpr int CTR_SLOTS = 64;
 constexpr int SMALL_SLOTS = 4096;
 
 void ensure_status(cm_ctx *ctx, size_t words);
+void ensure_host_flags(cm_ctx *ctx, size_t words);
 // Status words for a look-back launch over `words` tiles; returns the epoch.
 uint32_t lookback_begin(cm_ctx *ctx, size_t words);
 
@@ -178,6 +181,20 @@ struct MatchOwners {
 
 // ---- subsystem entry points (device pointers, ctx stream) ----
 // (a) matching; returns rounds.  match[] ends complete (partner or self).
+// A matching in progress (see cm_match.hip): begin; queue batches of round
+// launches (each also queues a copy of the round flags to ctx->h_flags);
+// after a synchronisation, done() tells whether the queued launches finished
+// it; finish() runs further batches until done and returns the rounds;
+// copy_finished() writes the result with unmatched vertices matched to
+// themselves; end() releases the state.
+struct MatchRun;
+MatchRun *match_begin(cm_ctx *ctx, const Level &L, const MatchOwners &own, int max_rounds, int32_t *d_match);
+void match_queue(cm_ctx *ctx, MatchRun *mr, int count);
+bool match_done(cm_ctx *ctx, MatchRun *mr);
+int match_finish(cm_ctx *ctx, MatchRun *mr);
+void match_copy_finished(cm_ctx *ctx, MatchRun *mr, int32_t *out);
+void match_end(cm_ctx *ctx, MatchRun *mr);
+constexpr int MATCH_BATCH = 9;  // launches per batch
 // finish=false leaves unmatched vertices at -1 for number() to complete.
 int match_handshake(cm_ctx *ctx, const Level &L, int32_t maxvwgt, uint64_t seed, int max_rounds,
                     int32_t *d_match, bool finish = true);

@@ -22,7 +22,7 @@ namespace cm {
 constexpr int LONG_DEG = LONG_ROW;  // rows longer than this are scanned by whole waves
 constexpr int LONG_CHUNK = 2048;  // entries per wave of a long row
 constexpr int SELECT_PER_THREAD = 16;
-constexpr int ROUND_BATCH = 9;  // launches queued between host checks
+constexpr int ROUND_BATCH = MATCH_BATCH;  // launches queued between host checks
 
 // flags[2r] = 1 if a vertex matched or died in round r, flags[2r+1] = 1 if
 // any vertex is still unmatched after round r.  Round r's accept runs in
@@ -357,83 +357,192 @@ __global__ __launch_bounds__(256) void k_round(int64_t n, const OffT *__restrict
   }
 }
 
-__global__ __launch_bounds__(256) void k_match_finish(int64_t n, int32_t *match)
+// out[v] = match[v], with vertices still unmatched (-1) matched to themselves;
+// out may be match itself.
+__global__ __launch_bounds__(256) void k_match_finish(int64_t n, const int32_t *match, int32_t *out)
 {
   const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (v < n && match[v] == -1) match[v] = (int32_t)v;
+  if (v < n) {
+    const int32_t u = match[v];
+    out[v] = u == -1 ? (int32_t)v : u;
+  }
 }
 
+// Device state of one matching between host calls: the round launches can be
+// queued in batches without reading anything back, and resumed if the flags
+// show that the last batch did not finish.
+struct MatchRun {
+  const Level *L = nullptr;
+  MatchOwners own;
+  int max_rounds = 0;
+  int32_t *match = nullptr;
+  int32_t *prop = nullptr;
+  LongChunk *chunks = nullptr;
+  LongPartial part{};
+  unsigned *flags = nullptr, *nchunks = nullptr;
+  unsigned g = 0, glong = 0;
+  int r = 0;  // launches queued so far
+  std::vector<void *> blocks;
+};
+
 template <typename OffT>
-static int match_impl(cm_ctx *ctx, const Level &L, const MatchOwners &own, int max_rounds,
-                      int32_t *match, bool finish)
+static void match_start(cm_ctx *ctx, MatchRun &mr)
 {
+  const Level &L = *mr.L;
   const int64_t n = L.n;
-  if (n == 0) return 0;
   const OffT *xadj = (const OffT *)L.xadj;
-  int32_t *prop = (int32_t *)ctx->pool.alloc(sizeof(int32_t) * 2 * n);  // ping-pong
+  auto al = [&](size_t bytes) {
+    void *p = ctx->pool.alloc(bytes ? bytes : 1);
+    mr.blocks.push_back(p);
+    return p;
+  };
+  mr.prop = (int32_t *)al(sizeof(int32_t) * 2 * n);  // ping-pong
   // chunks: sum over rows longer than LONG_DEG of ceil(d / LONG_CHUNK)
   const bool has_long = L.maxdeg < 0 || L.maxdeg > LONG_DEG;
   const int64_t cap = has_long ? L.m / LONG_CHUNK + std::min<int64_t>(n, L.m / (LONG_DEG + 1)) + 1 : 1;
-  LongChunk *chunks = (LongChunk *)ctx->pool.alloc(sizeof(LongChunk) * cap);
-  LongPartial part;
-  part.best = (int32_t *)ctx->pool.alloc(sizeof(int32_t) * cap);
-  part.k1 = (uint64_t *)ctx->pool.alloc(sizeof(uint64_t) * cap);
-  part.k2 = (uint64_t *)ctx->pool.alloc(sizeof(uint64_t) * cap);
-  part.arrived = (unsigned *)ctx->pool.alloc(sizeof(unsigned) * cap);
-  unsigned *flags = (unsigned *)ctx->pool.alloc(sizeof(unsigned) * (2 * (size_t)max_rounds + 2));
-  unsigned *nchunks = flags + 2 * (size_t)max_rounds;
-  if (has_long) fill_u32(ctx, part.arrived, 0u, cap);
-  const unsigned g = nblocks(n, 256);
-  const double row_bytes = (double)sizeof(OffT) * (n + 1) + 8.0 * L.m + 8.0 * n;
-  launch(ctx, "match_init", 8.0 * n, k_match_init, dim3(g), dim3(256), 0, n, match, prop, flags,
-         2 * (int64_t)max_rounds + 2);
+  mr.chunks = (LongChunk *)al(sizeof(LongChunk) * cap);
+  mr.part.best = (int32_t *)al(sizeof(int32_t) * cap);
+  mr.part.k1 = (uint64_t *)al(sizeof(uint64_t) * cap);
+  mr.part.k2 = (uint64_t *)al(sizeof(uint64_t) * cap);
+  mr.part.arrived = (unsigned *)al(sizeof(unsigned) * cap);
+  mr.flags = (unsigned *)al(sizeof(unsigned) * (2 * (size_t)mr.max_rounds + 2));
+  mr.nchunks = mr.flags + 2 * (size_t)mr.max_rounds;
+  if (has_long) fill_u32(ctx, mr.part.arrived, 0u, cap);
+  mr.g = nblocks(n, 256);
+  launch(ctx, "match_init", 8.0 * n, k_match_init, dim3(mr.g), dim3(256), 0, n, mr.match, mr.prop, mr.flags,
+         2 * (int64_t)mr.max_rounds + 2);
   // extra blocks whose waves stride over the long-row chunks (none when the
   // level is known to have no long row)
-  unsigned glong = 0;
   if (has_long) {
     launch(ctx, "match_select_long", (double)sizeof(OffT) * (n + 1), k_select_long<OffT>,
-           dim3(nblocks(n, 256 * SELECT_PER_THREAD)), dim3(256), 0, n, xadj, chunks, nchunks);
-    glong = (unsigned)std::min<int64_t>(8192, std::max<int64_t>(16, (cap + 3) / 4));
+           dim3(nblocks(n, 256 * SELECT_PER_THREAD)), dim3(256), 0, n, xadj, mr.chunks, mr.nchunks);
+    mr.glong = (unsigned)std::min<int64_t>(8192, std::max<int64_t>(16, (cap + 3) / 4));
   }
-  int r = 0;
-  std::vector<unsigned> h(2 * (size_t)max_rounds + 2);
-  for (;;) {
-    for (int i = 0; i < ROUND_BATCH && r <= max_rounds; ++i, ++r) {
-      int32_t *cur = prop + (size_t)(r & 1) * n;
-      const int32_t *prv = prop + (size_t)((r + 1) & 1) * n;
-      launch(ctx, "match_round", r == 0 ? row_bytes : 0.0, k_round<OffT>, dim3(g + glong), dim3(256), 0, n,
-             xadj, L.adjncy, L.adjwgt, L.vwgt, own, match, prv, cur, (const LongChunk *)chunks,
-             (const unsigned *)nchunks, part, flags, r, g, r < max_rounds ? 1 : 0, L.maxvw);
-    }
-    if (r < 2) continue;
-    // flags of every accepted round (0 .. r-2: round r-2 is accepted by
-    // launch r-1), one read per batch
-    CM_HIP(hipMemcpyAsync(h.data(), flags, sizeof(unsigned) * 2 * (r - 1), hipMemcpyDeviceToHost, ctx->stream));
-    sync(ctx);
-    if (h[2 * (r - 2)] == 0 || h[2 * (r - 2) + 1] == 0 || r > max_rounds) break;
+}
+
+// Queues up to `count` more launches (launch r: accept of round r-1,
+// proposals of round r; launch max_rounds only accepts).
+template <typename OffT>
+static void match_launch(cm_ctx *ctx, MatchRun &mr, int count)
+{
+  const Level &L = *mr.L;
+  const int64_t n = L.n;
+  const double row_bytes = (double)sizeof(OffT) * (n + 1) + 8.0 * L.m + 8.0 * n;
+  for (int i = 0; i < count && mr.r <= mr.max_rounds; ++i, ++mr.r) {
+    const int r = mr.r;
+    int32_t *cur = mr.prop + (size_t)(r & 1) * n;
+    const int32_t *prv = mr.prop + (size_t)((r + 1) & 1) * n;
+    launch(ctx, "match_round", r == 0 ? row_bytes : 0.0, k_round<OffT>, dim3(mr.g + mr.glong), dim3(256), 0, n,
+           (const OffT *)L.xadj, L.adjncy, L.adjwgt, L.vwgt, mr.own, mr.match, prv, cur,
+           (const LongChunk *)mr.chunks, (const unsigned *)mr.nchunks, mr.part, mr.flags, r, mr.g,
+           r < mr.max_rounds ? 1 : 0, L.maxvw);
   }
-  // rounds that did work: round 0, then every round whose predecessor changed
-  // something and left active vertices
-  const int acc = std::min(r - 1, max_rounds);  // accepted rounds 0 .. acc-1
-  if (finish) launch(ctx, "match_finish", 4.0 * n, k_match_finish, dim3(g), dim3(256), 0, n, match);
+}
+
+// Queues the flags of every accepted round (0 .. r-2: round r-2 is accepted
+// by launch r-1) into the pinned host buffer.
+static void match_queue_flags(cm_ctx *ctx, MatchRun &mr)
+{
+  const size_t words = 2 * (size_t)std::max(mr.r - 1, 1);
+  ensure_host_flags(ctx, words);
+  CM_HIP(hipMemcpyAsync(ctx->h_flags, mr.flags, sizeof(unsigned) * words, hipMemcpyDeviceToHost, ctx->stream));
+}
+
+// After a synchronisation that follows match_queue_flags: did the queued
+// launches finish the matching?
+static bool match_complete(cm_ctx *ctx, const MatchRun &mr)
+{
+  if (mr.r > mr.max_rounds) return true;
+  if (mr.r < 2) return false;
+  const unsigned *h = ctx->h_flags;
+  return h[2 * (mr.r - 2)] == 0 || h[2 * (mr.r - 2) + 1] == 0;
+}
+
+// Rounds that did work: round 0, then every round whose predecessor changed
+// something and left active vertices.
+static int match_rounds(cm_ctx *ctx, const MatchRun &mr)
+{
+  const int acc = std::min(mr.r - 1, mr.max_rounds);  // accepted rounds 0 .. acc-1
+  const unsigned *h = ctx->h_flags;
   int rounds = 1;
   while (rounds < acc && h[2 * (rounds - 1)] != 0 && h[2 * (rounds - 1) + 1] != 0) ++rounds;
-  ctx->pool.release(prop);
-  ctx->pool.release(chunks);
-  ctx->pool.release(part.best);
-  ctx->pool.release(part.k1);
-  ctx->pool.release(part.k2);
-  ctx->pool.release(part.arrived);
-  ctx->pool.release(flags);
   return rounds;
+}
+
+MatchRun *match_begin(cm_ctx *ctx, const Level &L, const MatchOwners &own, int max_rounds, int32_t *d_match)
+{
+  MatchRun *mr = new MatchRun();
+  mr->L = &L;
+  mr->own = own;
+  mr->max_rounds = max_rounds > 0 ? max_rounds : 4096;
+  mr->match = d_match;
+  if (L.n > 0) {
+    try {
+      if (L.off64) match_start<int64_t>(ctx, *mr);
+      else match_start<int32_t>(ctx, *mr);
+    } catch (...) {
+      match_end(ctx, mr);
+      throw;
+    }
+  }
+  return mr;
+}
+
+void match_queue(cm_ctx *ctx, MatchRun *mr, int count)
+{
+  if (mr->L->n == 0) return;
+  if (mr->L->off64) match_launch<int64_t>(ctx, *mr, count);
+  else match_launch<int32_t>(ctx, *mr, count);
+  match_queue_flags(ctx, *mr);
+}
+
+bool match_done(cm_ctx *ctx, MatchRun *mr)
+{
+  return mr->L->n == 0 || match_complete(ctx, *mr);
+}
+
+int match_finish(cm_ctx *ctx, MatchRun *mr)
+{
+  for (;;) {
+    if (match_done(ctx, mr)) break;
+    match_queue(ctx, mr, ROUND_BATCH);
+    sync(ctx);
+  }
+  return mr->L->n == 0 ? 0 : match_rounds(ctx, *mr);
+}
+
+void match_copy_finished(cm_ctx *ctx, MatchRun *mr, int32_t *out)
+{
+  const int64_t n = mr->L->n;
+  if (n > 0)
+    launch(ctx, "match_finish", 8.0 * n, k_match_finish, dim3(nblocks(n, 256)), dim3(256), 0, n,
+           (const int32_t *)mr->match, out);
+}
+
+void match_end(cm_ctx *ctx, MatchRun *mr)
+{
+  for (void *p : mr->blocks) ctx->pool.release(p);
+  delete mr;
 }
 
 int match_handshake(cm_ctx *ctx, const Level &L, const MatchOwners &own, int max_rounds,
                     int32_t *d_match, bool finish)
 {
-  if (max_rounds <= 0) max_rounds = 4096;
-  return L.off64 ? match_impl<int64_t>(ctx, L, own, max_rounds, d_match, finish)
-                 : match_impl<int32_t>(ctx, L, own, max_rounds, d_match, finish);
+  MatchRun *mr = match_begin(ctx, L, own, max_rounds, d_match);
+  int rounds = 0;
+  try {
+    if (L.n > 0) {
+      match_queue(ctx, mr, ROUND_BATCH);
+      sync(ctx);
+    }
+    rounds = match_finish(ctx, mr);
+    if (finish) match_copy_finished(ctx, mr, d_match);
+  } catch (...) {
+    match_end(ctx, mr);
+    throw;
+  }
+  match_end(ctx, mr);
+  return rounds;
 }
 
 int match_handshake(cm_ctx *ctx, const Level &L, int32_t maxvwgt, uint64_t seed, int max_rounds,
