@@ -143,12 +143,19 @@ def test_project(engine, orc):
 
 
 @pytest.mark.parametrize("order", [CM_ROW_FIRST_SEEN, CM_ROW_ASCENDING])
-@pytest.mark.parametrize("name,k", [("grid100w100", 8), ("delaunay4720", 8), ("rmat12", 16), ("hub", 4)])
+@pytest.mark.parametrize("name,k", [("grid100w100", 8), ("delaunay4720", 8), ("rmat12", 16), ("hub", 4),
+                                    ("path_increasing", 2)])
 def test_hierarchy(engine, orc, name, k, order):
     if name == "grid100w100":
         g = graphs.grid2d(100, 100, wmax=100, seed=9)
     elif name == "rmat12":
         g = graphs.rmat(12, 16, seed=4, offsets=np.int32)
+    elif name == "path_increasing":
+        # weights rise along the path: each round matches only the heaviest
+        # remaining edge, so level 0 needs 200 rounds, far more than one batch
+        # of launches (the contraction is redone after the rounds resume)
+        n = 400
+        g = graphs.from_edges(n, np.arange(n - 1), np.arange(1, n), np.arange(1, n))
     else:
         g = CASES[name]
     g = with_weights(g)
@@ -169,6 +176,9 @@ def test_hierarchy(engine, orc, name, k, order):
         if i < len(ref) - 1:
             np.testing.assert_array_equal(L.match, R["match"])
             np.testing.assert_array_equal(L.cmap, R["cmap"])
+            _, rounds = orc.handshake_match(R["xadj"], R["adjncy"], R["adjwgt"], R["vwgt"],
+                                            maxv_for(g, k, orc), seed=5)
+            assert L.rounds == rounds, i
     h.release()
 
 
