@@ -21,7 +21,8 @@ namespace cm {
 
 constexpr int GATHER_LONG = 64;   // fine rows longer than this gather by wave
 constexpr int SMALL_SEG = 64;     // coarse segments up to this merge per thread
-constexpr int FIRST_SORTED_BIN = 4;  // segsort bins from 17 keys up
+constexpr int FIRST_SORTED_BIN = 4;  // segsort bins from REG_SEG + 1 = 17 keys up
+static_assert(REG_SEG == 16, "FIRST_SORTED_BIN must be the first bin above REG_SEG");
 
 // ------------------------------------------------------------- numbering
 // NUM_PER consecutive vertices per thread, tiles of 1024 * NUM_PER vertices
@@ -34,7 +35,7 @@ constexpr int NUM_PER = 2;
 
 template <typename OffT>
 __global__ __launch_bounds__(1024) void k_number(int64_t n, const OffT *__restrict__ xadj,
-                                                 int32_t *__restrict__ match,
+                                                 const int32_t *match_in, int32_t *match,
                                                  const int32_t *__restrict__ vwgt,
                                                  int32_t *__restrict__ cmap, int32_t *__restrict__ rep,
                                                  OffT *__restrict__ segoff, int32_t *__restrict__ cvwgt,
@@ -54,11 +55,9 @@ __global__ __launch_bounds__(1024) void k_number(int64_t n, const OffT *__restri
     uu[t] = 0;
     cw[t] = 0;
     if (i < n) {
-      int32_t u = match[i];
-      if (u < 0) {
-        u = (int32_t)i;
-        match[i] = u;
-      }
+      const int32_t u0 = match_in[i];
+      const int32_t u = u0 < 0 ? (int32_t)i : u0;
+      if (u0 < 0 || match_in != match) match[i] = u;
       uu[t] = u;
       fl[t] = (i <= u);
       if (fl[t]) {
@@ -168,7 +167,7 @@ __global__ __launch_bounds__(256) void k_number_fill(int64_t n, const int32_t *_
 }
 
 template <typename OffT>
-static int64_t number_impl(cm_ctx *ctx, const Level &L, int32_t *match, int32_t *cmap,
+static int64_t number_impl(cm_ctx *ctx, const Level &L, const int32_t *match_in, int32_t *match, int32_t *cmap,
                            int32_t *rep, OffT *segoff, int32_t *cvwgt, bool wait)
 {
   const int64_t n = L.n;
@@ -180,7 +179,7 @@ static int64_t number_impl(cm_ctx *ctx, const Level &L, int32_t *match, int32_t 
   const int64_t tiles = (n + 1024 * NUM_PER - 1) / (1024 * NUM_PER);
   const uint32_t ep = lookback_begin(ctx, 4 * tiles);  // 2 chains + tile maxima
   launch(ctx, "number_scan", (double)sizeof(OffT) * (n + 1) * 2 + 16.0 * n, k_number<OffT>,
-         dim3((unsigned)tiles), dim3(1024), 0, n, (const OffT *)L.xadj, match, (const int32_t *)L.vwgt,
+         dim3((unsigned)tiles), dim3(1024), 0, n, (const OffT *)L.xadj, match_in, match, (const int32_t *)L.vwgt,
          cmap, rep, segoff, cvwgt, ctx->d_status, ctx->d_status + tiles, ctx->d_status + 2 * tiles, ctx->d_ctr,
          ctx->d_small, ep);
   launch(ctx, "number_fill", 12.0 * n, k_number_fill, dim3(nblocks(n, 256)), dim3(256), 0, n, match,
@@ -193,10 +192,11 @@ static int64_t number_impl(cm_ctx *ctx, const Level &L, int32_t *match, int32_t 
 }
 
 int64_t number(cm_ctx *ctx, const Level &L, int32_t *d_match, int32_t *d_cmap,
-               int32_t *d_rep, void *d_segoff, int32_t *d_cvwgt, bool wait)
+               int32_t *d_rep, void *d_segoff, int32_t *d_cvwgt, bool wait, const int32_t *d_match_in)
 {
-  return L.off64 ? number_impl(ctx, L, d_match, d_cmap, d_rep, (int64_t *)d_segoff, d_cvwgt, wait)
-                 : number_impl(ctx, L, d_match, d_cmap, d_rep, (int32_t *)d_segoff, d_cvwgt, wait);
+  const int32_t *in = d_match_in ? d_match_in : d_match;
+  return L.off64 ? number_impl(ctx, L, in, d_match, d_cmap, d_rep, (int64_t *)d_segoff, d_cvwgt, wait)
+                 : number_impl(ctx, L, in, d_match, d_cmap, d_rep, (int32_t *)d_segoff, d_cvwgt, wait);
 }
 
 // ------------------------------------------------- generic look-back scan

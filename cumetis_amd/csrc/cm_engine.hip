@@ -454,9 +454,10 @@ static Level coarsen_level(cm_ctx *ctx, cm_hier *h, Level &F, int32_t maxvwgt, u
     }
     for (int pass = 0;; ++pass) {
       if (!speculate || pass > 0) match_finish(ctx, mr);
-      match_copy_finished(ctx, mr, F.match);
-      // nc stays on the device until the contraction's one synchronisation
-      number(ctx, F, F.match, F.cmap, rep, segoff, C.vwgt, false);
+      // the numbering writes the completed copy of the rounds' match to
+      // F.match; nc stays on the device until the contraction's one
+      // synchronisation
+      number(ctx, F, F.match, F.cmap, rep, segoff, C.vwgt, false, mstate);
       C.m = contract(ctx, F, F.match, F.cmap, -1, rep, segoff, row_order, C.xadj, C.adjncy, C.adjwgt,
                      &C.maxdeg, &C.n);
       C.maxvw = ctx->h_small[3];

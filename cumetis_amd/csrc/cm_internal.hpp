@@ -205,9 +205,11 @@ int match_handshake(cm_ctx *ctx, const Level &L, const MatchOwners &own, int max
 // With wait=false nc stays in d_small[0] (read it back with the next
 // synchronisation) and -1 is returned.
 // match entries still -1 (matching run with finish=false) are completed as
-// self matches.
+// self matches.  With d_match_in, the matching is read from there and its
+// completed copy written to d_match.
 int64_t number(cm_ctx *ctx, const Level &L, int32_t *d_match, int32_t *d_cmap,
-               int32_t *d_rep, void *d_segoff, int32_t *d_cvwgt, bool wait = true);
+               int32_t *d_rep, void *d_segoff, int32_t *d_cvwgt, bool wait = true,
+               const int32_t *d_match_in = nullptr);
 void queue_max_degree(cm_ctx *ctx, int64_t n, bool off64, const void *xadj, const int32_t *vwgt = nullptr);
 // (c) contraction of L into coarse CSR; returns mc.  cxadj/cadjncy/cadjwgt
 //     must hold nc+1 / L.m / L.m entries.  nc = -1: the count left on the
