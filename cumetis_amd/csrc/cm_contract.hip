@@ -654,6 +654,62 @@ __device__ __forceinline__ int merge_regs(int32_t c, int32_t v, int32_t u, const
 // in cdeg.  A look-back scan of the degrees gives cxadj, and the short rows
 // are written straight to their final place: the fine rows are read once
 // and the coarse rows written once.
+// First-seen merge of a short segment without sorting (row(v) then row(u),
+// cu:766-808): entry i is a head when no earlier entry has its coarse id, a
+// head's weight is its own plus that of its later duplicates, heads keep
+// their position order, and the head of the coarse vertex itself gives its
+// slot to the last head (cu:804-808).  dst[i] = row slot of entry i (-1 for
+// non-heads and the self head); returns the row length.  All comparisons use
+// static indices, so the arrays stay in registers.
+template <int N, typename OffT>
+__device__ __forceinline__ int merge_fs(int32_t c, int32_t v, int32_t u, const OffT *__restrict__ xadj,
+                                        const int32_t *__restrict__ adjncy,
+                                        const int32_t *__restrict__ adjwgt,
+                                        const int32_t *__restrict__ cmap, int32_t (&cu)[N], int32_t (&sum)[N],
+                                        int32_t (&dst)[N])
+{
+  const int64_t b0 = (int64_t)xadj[v];
+  const int64_t e0 = (int64_t)xadj[v + 1];
+  const int64_t b1 = (int64_t)xadj[u];
+  const int64_t e1 = (int64_t)xadj[u + 1];
+  const int d0 = (int)(e0 - b0);
+  const int cnt = d0 + (u != v ? (int)(e1 - b1) : 0);
+  int32_t a[N], wt[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) {  // unconditional loads (see merge_regs)
+    const int64_t j = i < cnt ? (i < d0 ? b0 + i : b1 + (i - d0)) : 0;
+    a[i] = adjncy[j];
+    wt[i] = adjwgt[j];
+  }
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const int32_t x = cmap[i < cnt ? a[i] : 0];
+    cu[i] = i < cnt ? x : -1 - i;  // distinct negative ids for empty slots never match
+  }
+  int r = 0, sidx = -1;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    bool h = i < cnt;
+#pragma unroll
+    for (int j = 0; j < i; ++j) h = h && cu[j] != cu[i];
+    int32_t s = wt[i];
+#pragma unroll
+    for (int j = i + 1; j < N; ++j) s += cu[j] == cu[i] ? wt[j] : 0;
+    sum[i] = s;
+    dst[i] = h ? r : -1;
+    if (h && cu[i] == c) sidx = r;
+    r += h ? 1 : 0;
+  }
+  if (sidx < 0) return r;
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+    if (dst[i] == sidx) dst[i] = -1;  // the self head
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+    if (dst[i] == r - 1) dst[i] = sidx;  // the last head takes its slot
+  return r - 1;
+}
+
 constexpr int SMALL_TILE = 1024;   // coarse vertices per tile of k_contract_small
 constexpr int SMALL_STAGE = 8192;  // output entries staged in LDS per window
 
@@ -680,7 +736,8 @@ __global__ __launch_bounds__(SMALL_TILE) void k_contract_small(const int64_t *__
   // in the look-back chain (no later tile waits for them)
   if ((int64_t)tile * SMALL_TILE >= nc) return;
   const int64_t c = (int64_t)tile * SMALL_TILE + threadIdx.x;
-  uint64_t k[REG_SEG], p[REG_SEG];
+  uint64_t k[REG_SEG], p[REG_SEG];            // ascending order: sorted (id, weight) pairs
+  int32_t fcu[REG_SEG], fsum[REG_SEG], fdst[REG_SEG];  // first-seen order: entries and row slots
   int deg = 0, len = 0;
   bool small = false;
   // loads by a clamped index, all issued before the first wait
@@ -692,7 +749,9 @@ __global__ __launch_bounds__(SMALL_TILE) void k_contract_small(const int64_t *__
     len = (int)(s1 - s0);
     if (len <= REG_SEG) {
       small = true;
-      if (len <= 8) {
+      if constexpr (FS) {
+        deg = merge_fs<REG_SEG, OffT>((int32_t)c, v, u, xadj, adjncy, adjwgt, cmap, fcu, fsum, fdst);
+      } else if (len <= 8) {
         uint64_t k8[8], p8[8];
         deg = merge_regs<8, FS, OffT>((int32_t)c, v, u, xadj, adjncy, adjwgt, cmap, k8, p8);
 #pragma unroll
@@ -738,10 +797,18 @@ __global__ __launch_bounds__(SMALL_TILE) void k_contract_small(const int64_t *__
     if (small && c < nc) {
 #pragma unroll
       for (int i = 0; i < REG_SEG; ++i) {
-        const int64_t pos = (int64_t)(ex + i) - (int64_t)w0;
-        if (i < deg && pos >= 0 && pos < wlen) {
-          stage_a[pos] = (int32_t)(p[i] >> 32);
-          stage_w[pos] = (int32_t)(uint32_t)p[i];
+        if constexpr (FS) {
+          const int64_t pos = (int64_t)ex + fdst[i] - (int64_t)w0;
+          if (fdst[i] >= 0 && pos >= 0 && pos < wlen) {
+            stage_a[pos] = fcu[i];
+            stage_w[pos] = fsum[i];
+          }
+        } else {
+          const int64_t pos = (int64_t)(ex + i) - (int64_t)w0;
+          if (i < deg && pos >= 0 && pos < wlen) {
+            stage_a[pos] = (int32_t)(p[i] >> 32);
+            stage_w[pos] = (int32_t)(uint32_t)p[i];
+          }
         }
       }
     }
