@@ -13,7 +13,8 @@
 //           and items by length -- up to HC_WAVE_MAX one wave, up to
 //           HC_BLOCK_MAX one block, longer ones ("hubs") one block per key
 //           partition of about HC_PART entries;
-//   gather  per chunk: coarse neighbour ids and weights in segment order;
+//   gather  per chunk of a hub: coarse neighbour ids and weights in segment
+//           order (wave and block items read their rows directly);
 //   dedup   wave / block items: table, heads ranked by position, the row
 //           written compacted into a staging area and its length to cdeg;
 //           hub partitions: head flag and weight sum per entry;
@@ -99,6 +100,26 @@ __device__ __forceinline__ int hc_find(const int32_t *tk, int lg, int32_t key)
     h = (h + 1) & mask;
   }
   return -1;
+}
+
+// Entry i of segment c (row(v) then row(u), v = rep[c]): its coarse
+// neighbour id and weight.  `at` holds the segment's row starts and the
+// length of row(v).
+struct SegRows {
+  int64_t b0, b1;
+  int32_t d0;
+};
+
+template <typename OffT>
+__device__ __forceinline__ SegRows seg_rows(int32_t c, const int32_t *__restrict__ rep,
+                                            const int32_t *__restrict__ match, const OffT *__restrict__ xadj)
+{
+  const int32_t v = rep[c], u = match[v];
+  SegRows r;
+  r.b0 = (int64_t)xadj[v];
+  r.d0 = (int32_t)((int64_t)xadj[v + 1] - r.b0);
+  r.b1 = (int64_t)xadj[u];  // used only when u != v (then the segment is longer than row(v))
+  return r;
 }
 
 // ---------------------------------------------------------------- select
@@ -192,6 +213,7 @@ __global__ __launch_bounds__(256) void k_hc_gather(const SegChunk *__restrict__ 
     const SegChunk ch = chunks[q];
     const int64_t s0 = (int64_t)segoff[ch.c];
     const int64_t len = (int64_t)segoff[ch.c + 1] - s0;
+    if (len <= HC_BLOCK_MAX) continue;  // wave and block items read their rows themselves
     const int32_t v = rep[ch.c], u = match[v];
     const int64_t b0 = (int64_t)xadj[v], d0 = (int64_t)xadj[v + 1] - b0;
     const int64_t b1 = u != v ? (int64_t)xadj[u] : 0;
@@ -227,8 +249,12 @@ __global__ __launch_bounds__(256) void k_hc_gather(const SegChunk *__restrict__ 
 template <typename OffT>
 __global__ __launch_bounds__(256) void k_hc_dedup_wave(const int32_t *__restrict__ items, const unsigned *ctr,
                                                        const OffT *__restrict__ segoff,
-                                                       const int32_t *__restrict__ cval,
-                                                       const int32_t *__restrict__ wv, int32_t *__restrict__ tc,
+                                                       const int32_t *__restrict__ rep,
+                                                       const int32_t *__restrict__ match,
+                                                       const OffT *__restrict__ xadj,
+                                                       const int32_t *__restrict__ adjncy,
+                                                       const int32_t *__restrict__ adjwgt,
+                                                       const int32_t *__restrict__ cmap, int32_t *__restrict__ tc,
                                                        int32_t *__restrict__ tw, int32_t *__restrict__ cdeg)
 {
   constexpr int PER = HC_WAVE_MAX / CM_WAVE;
@@ -249,17 +275,25 @@ __global__ __launch_bounds__(256) void k_hc_dedup_wave(const int32_t *__restrict
       ts[i] = 0;
     }
     wave_sync();
-    int32_t cu[PER], sl[PER];
+    const SegRows sr = seg_rows<OffT>(c, rep, match, xadj);
+    int32_t cu[PER], sl[PER], wt[PER], a[PER];
+#pragma unroll
+    for (int t = 0; t < PER; ++t) {  // clamped, unconditional loads
+      const int i = l + t * CM_WAVE < len ? l + t * CM_WAVE : len - 1;
+      const int64_t j = i < sr.d0 ? sr.b0 + i : sr.b1 + (i - sr.d0);
+      a[t] = adjncy[j];
+      wt[t] = adjwgt[j];
+    }
 #pragma unroll
     for (int t = 0; t < PER; ++t) {
       const int i = l + t * CM_WAVE;
-      cu[t] = i < len ? cval[s0 + i] : -1;
-      const int32_t wt = i < len ? wv[s0 + i] : 0;
+      const int32_t x = cmap[a[t]];
+      cu[t] = i < len ? x : -1;
       sl[t] = -1;
       if (i < len) {
         sl[t] = hc_insert(tk, lg, cu[t]);
         atomicMin(&tp[sl[t]], i);
-        atomicAdd(&ts[sl[t]], wt);
+        atomicAdd(&ts[sl[t]], wt[t]);
       }
     }
     wave_sync();
@@ -294,8 +328,12 @@ __global__ __launch_bounds__(256) void k_hc_dedup_wave(const int32_t *__restrict
 template <typename OffT>
 __global__ __launch_bounds__(256) void k_hc_dedup_block(const int32_t *__restrict__ items, const unsigned *ctr,
                                                         const OffT *__restrict__ segoff,
-                                                        const int32_t *__restrict__ cval,
-                                                        const int32_t *__restrict__ wv, int32_t *__restrict__ tc,
+                                                        const int32_t *__restrict__ rep,
+                                                        const int32_t *__restrict__ match,
+                                                        const OffT *__restrict__ xadj,
+                                                        const int32_t *__restrict__ adjncy,
+                                                        const int32_t *__restrict__ adjwgt,
+                                                        const int32_t *__restrict__ cmap, int32_t *__restrict__ tc,
                                                         int32_t *__restrict__ tw, int32_t *__restrict__ cdeg)
 {
   constexpr int PER = HC_BLOCK_MAX / 256;
@@ -315,17 +353,25 @@ __global__ __launch_bounds__(256) void k_hc_dedup_block(const int32_t *__restric
     }
     if (threadIdx.x == 0) s_self = -1;
     __syncthreads();
-    int32_t cu[PER], sl[PER];
+    const SegRows sr = seg_rows<OffT>(c, rep, match, xadj);
+    int32_t cu[PER], sl[PER], wt[PER], a[PER];
+#pragma unroll
+    for (int t = 0; t < PER; ++t) {  // clamped, unconditional loads
+      const int i = (int)threadIdx.x + t * 256 < len ? (int)threadIdx.x + t * 256 : len - 1;
+      const int64_t j = i < sr.d0 ? sr.b0 + i : sr.b1 + (i - sr.d0);
+      a[t] = adjncy[j];
+      wt[t] = adjwgt[j];
+    }
 #pragma unroll
     for (int t = 0; t < PER; ++t) {
       const int i = threadIdx.x + t * 256;
-      cu[t] = i < len ? cval[s0 + i] : -1;
-      const int32_t wt = i < len ? wv[s0 + i] : 0;
+      const int32_t x = cmap[a[t]];
+      cu[t] = i < len ? x : -1;
       sl[t] = -1;
       if (i < len) {
         sl[t] = hc_insert(tk, lg, cu[t]);
         atomicMin(&tp[sl[t]], i);
-        atomicAdd(&ts[sl[t]], wt);
+        atomicAdd(&ts[sl[t]], wt[t]);
       }
     }
     __syncthreads();
@@ -364,7 +410,10 @@ __global__ __launch_bounds__(256) void k_hc_dedup_block(const int32_t *__restric
 // ------------------------------------------------- dedup: hub partitions
 // One block per key partition of a hub segment: the block reads the whole
 // segment and tables the keys of its partition; each entry of the partition
-// gets its head flag and, for heads, the weight sum.
+// gets its head flag and, for heads, the weight sum.  The first pass also
+// lists the partition's entries (position << 12 | slot) in LDS, so the second
+// pass walks that list instead of the segment (a full second pass remains for
+// lists that overflow and for segments of 2^19 entries or more).
 template <typename OffT>
 __global__ __launch_bounds__(256) void k_hc_dedup_hub(const HubPart *__restrict__ parts, const unsigned *ctr,
                                                       const OffT *__restrict__ segoff,
@@ -374,8 +423,11 @@ __global__ __launch_bounds__(256) void k_hc_dedup_hub(const HubPart *__restrict_
 {
   constexpr int lg = 12;  // HC_SLOTS
   constexpr int SCAN = 4;
+  constexpr int LIST = 3072;
   __shared__ int32_t tk[HC_SLOTS], tp[HC_SLOTS], ts[HC_SLOTS];
+  __shared__ uint32_t lst[LIST];
   __shared__ int s_full;
+  __shared__ unsigned s_cnt;
   const unsigned n = ctr[HC_N_PART];
   for (int64_t q = blockIdx.x; q < n; q += gridDim.x) {
     const HubPart hp = parts[q];
@@ -386,8 +438,12 @@ __global__ __launch_bounds__(256) void k_hc_dedup_hub(const HubPart *__restrict_
       tp[i] = 0x7fffffff;
       ts[i] = 0;
     }
-    if (threadIdx.x == 0) s_full = 0;
+    if (threadIdx.x == 0) {
+      s_full = 0;
+      s_cnt = 0;
+    }
     __syncthreads();
+    const bool listable = len < (1 << 19);
     for (int64_t ib = threadIdx.x; ib < len; ib += SCAN * 256) {
       int32_t cu[SCAN];
 #pragma unroll
@@ -398,25 +454,45 @@ __global__ __launch_bounds__(256) void k_hc_dedup_hub(const HubPart *__restrict_
 #pragma unroll
       for (int t = 0; t < SCAN; ++t) {
         const int64_t i = ib + (int64_t)t * 256;
-        if (cu[t] < 0 || hc_part(cu[t], hp.P) != hp.p) continue;
-        const int s = hc_insert(tk, lg, cu[t]);
-        if (s < 0) {
-          s_full = 1;
-          continue;
+        int s = -1;
+        if (cu[t] >= 0 && hc_part(cu[t], hp.P) == hp.p) {
+          s = hc_insert(tk, lg, cu[t]);
+          if (s < 0) {
+            s_full = 1;
+          } else {
+            atomicMin(&tp[s], (int32_t)i);
+            atomicAdd(&ts[s], wv[s0 + i]);
+          }
         }
-        atomicMin(&tp[s], (int32_t)i);
-        atomicAdd(&ts[s], wv[s0 + i]);
+        // one LDS counter update per wave
+        const uint64_t mk = __ballot(s >= 0);
+        unsigned base = 0;
+        if (mk && lane() == 0) base = atomicAdd(&s_cnt, (unsigned)__popcll(mk));
+        base = __shfl(base, 0, CM_WAVE);
+        const unsigned k = base + (unsigned)mask_rank(mk);
+        if (s >= 0 && k < (unsigned)LIST) lst[k] = ((uint32_t)i << 12) | (uint32_t)s;
       }
     }
     __syncthreads();
     if (s_full && threadIdx.x == 0) atomicOr(overflow, 1u);
-    for (int64_t i = threadIdx.x; i < len; i += 256) {
-      const int32_t cu = cval[s0 + i];
-      if (hc_part(cu, hp.P) != hp.p) continue;
-      const int s = hc_find(tk, lg, cu);
-      const bool h = s >= 0 && tp[s] == (int32_t)i;
-      head[s0 + i] = h ? 1 : 0;
-      hw[s0 + i] = h ? ts[s] : 0;
+    if (listable && s_cnt <= (unsigned)LIST) {
+      for (unsigned k = threadIdx.x; k < s_cnt; k += 256) {
+        const uint32_t e = lst[k];
+        const int32_t i = (int32_t)(e >> 12);
+        const int s = (int)(e & 4095u);
+        const bool h = tp[s] == i;
+        head[s0 + i] = h ? 1 : 0;
+        hw[s0 + i] = h ? ts[s] : 0;
+      }
+    } else {
+      for (int64_t i = threadIdx.x; i < len; i += 256) {
+        const int32_t cu = cval[s0 + i];
+        if (hc_part(cu, hp.P) != hp.p) continue;
+        const int s = hc_find(tk, lg, cu);
+        const bool h = s >= 0 && tp[s] == (int32_t)i;
+        head[s0 + i] = h ? 1 : 0;
+        hw[s0 + i] = h ? ts[s] : 0;
+      }
     }
     __syncthreads();
   }
@@ -581,7 +657,6 @@ void hash_prepare_impl(cm_ctx *ctx, const Level &L, const int32_t *match, const 
   HubPart *parts = (HubPart *)al(sizeof(HubPart) * npart);
   int32_t *hubs = (int32_t *)al(4 * nhub);
   hc.ctr = (unsigned *)al(4 * 8);
-  int32_t *cval = (int32_t *)al(4 * m), *wv = (int32_t *)al(4 * m);
   hc.tc = (int32_t *)al(4 * m);
   hc.tw = (int32_t *)al(4 * m);
   fill_u32(ctx, hc.ctr, 0u, 8);
@@ -590,16 +665,17 @@ void hash_prepare_impl(cm_ctx *ctx, const Level &L, const int32_t *match, const 
          (SegChunk *)hc.chunks, wave_items, block_items, parts, hubs, hc.ctr);
   const unsigned gchunk = (unsigned)std::min<int64_t>(4096, (nchunk + 3) / 4);
   hc.gchunk = gchunk;
-  launch(ctx, "hc_gather", 0.0, k_hc_gather<OffT>, dim3(gchunk), dim3(256), 0, chunks, (const unsigned *)hc.ctr,
-         segoff, rep, match, xadj, L.adjncy, L.adjwgt, cmap, cval, wv);
   launch(ctx, "hc_dedup_wave", 0.0, k_hc_dedup_wave<OffT>, dim3((unsigned)std::min<int64_t>(4096, (nseg + 3) / 4)),
-         dim3(256), 0, (const int32_t *)wave_items, (const unsigned *)hc.ctr, segoff, (const int32_t *)cval,
-         (const int32_t *)wv, hc.tc, hc.tw, cdeg);
+         dim3(256), 0, (const int32_t *)wave_items, (const unsigned *)hc.ctr, segoff, rep, match, xadj, L.adjncy,
+         L.adjwgt, cmap, hc.tc, hc.tw, cdeg);
   if (seg_max > HC_WAVE_MAX)
     launch(ctx, "hc_dedup_block", 0.0, k_hc_dedup_block<OffT>, dim3((unsigned)std::min<int64_t>(2048, nblk)),
-           dim3(256), 0, (const int32_t *)block_items, (const unsigned *)hc.ctr, segoff, (const int32_t *)cval,
-           (const int32_t *)wv, hc.tc, hc.tw, cdeg);
+           dim3(256), 0, (const int32_t *)block_items, (const unsigned *)hc.ctr, segoff, rep, match, xadj,
+           L.adjncy, L.adjwgt, cmap, hc.tc, hc.tw, cdeg);
   if (m > HC_BLOCK_MAX && seg_max > HC_BLOCK_MAX) {  // a hub needs more than HC_BLOCK_MAX entries
+    int32_t *cval = (int32_t *)al(4 * m), *wv = (int32_t *)al(4 * m);
+    launch(ctx, "hc_gather", 0.0, k_hc_gather<OffT>, dim3(gchunk), dim3(256), 0, chunks, (const unsigned *)hc.ctr,
+           segoff, rep, match, xadj, L.adjncy, L.adjwgt, cmap, cval, wv);
     uint8_t *head = (uint8_t *)al(m);
     int32_t *hw = (int32_t *)al(4 * m);
     int32_t *ccnt = (int32_t *)al(4 * nchunk), *cself = (int32_t *)al(4 * nchunk), *coff = (int32_t *)al(4 * nchunk);
