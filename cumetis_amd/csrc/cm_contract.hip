@@ -46,30 +46,29 @@ __global__ __launch_bounds__(1024) void k_number(int64_t n, const OffT *__restri
   const int tile = tile_ticket(ctr);
   const int64_t i0 = ((int64_t)tile * 1024 + threadIdx.x) * NUM_PER;
   uint64_t fl[NUM_PER], ln[NUM_PER], flag = 0, len = 0, mw = 0, ml = 0;
-  int32_t uu[NUM_PER], cw[NUM_PER];
+  int32_t uu[NUM_PER], cw[NUM_PER], u0[NUM_PER];
+  // two dependency levels (match, then rows and weights of both ends), each
+  // loaded unconditionally through indices clamped to n - 1
+#pragma unroll
+  for (int t = 0; t < NUM_PER; ++t) {
+    const int64_t ic = i0 + t < n ? i0 + t : n - 1;
+    u0[t] = match_in[ic];
+  }
 #pragma unroll
   for (int t = 0; t < NUM_PER; ++t) {
     const int64_t i = i0 + t;
-    fl[t] = 0;
-    ln[t] = 0;
-    uu[t] = 0;
-    cw[t] = 0;
-    if (i < n) {
-      const int32_t u0 = match_in[i];
-      const int32_t u = u0 < 0 ? (int32_t)i : u0;
-      if (u0 < 0 || match_in != match) match[i] = u;
-      uu[t] = u;
-      fl[t] = (i <= u);
-      if (fl[t]) {
-        ln[t] = (uint64_t)(xadj[i + 1] - xadj[i]);
-        cw[t] = vwgt[i];
-        if (u != i) {
-          ln[t] += (uint64_t)(xadj[u + 1] - xadj[u]);
-          cw[t] += vwgt[u];
-        }
-        mw = (uint64_t)(uint32_t)cw[t] > mw ? (uint64_t)(uint32_t)cw[t] : mw;
-        ml = ln[t] > ml ? ln[t] : ml;
-      }
+    const int64_t ic = i < n ? i : n - 1;
+    const int32_t u = u0[t] < 0 ? (int32_t)ic : u0[t];
+    const OffT xa = xadj[ic], xb = xadj[ic + 1], ya = xadj[u], yb = xadj[u + 1];
+    const int32_t wa = vwgt[ic], wb = vwgt[u];
+    uu[t] = u;
+    fl[t] = (i < n && i <= u) ? 1 : 0;
+    ln[t] = fl[t] ? (uint64_t)(xb - xa) + (u != i ? (uint64_t)(yb - ya) : 0) : 0;
+    cw[t] = fl[t] ? wa + (u != i ? wb : 0) : 0;
+    if (i < n && (u0[t] < 0 || match_in != match)) match[i] = u;
+    if (fl[t]) {
+      mw = (uint64_t)(uint32_t)cw[t] > mw ? (uint64_t)(uint32_t)cw[t] : mw;
+      ml = ln[t] > ml ? ln[t] : ml;
     }
     flag += fl[t];
     len += ln[t];
