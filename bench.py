@@ -156,26 +156,66 @@ def coarsen_leg(eng, g, k, order, warmup, steps, profile=True, workload=None):
     return out, roof, per
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(g, k, budget_s=12.0):
     """Reference CPU coarsening path (cuMetis_cpu_coarsen restated in C,
-    single thread), repeated within a time budget."""
+    single thread, pinned to one core of this host), repeated within a time
+    budget."""
+    import os
     from oracle import oracle as O
     x = g.xadj.astype(np.int64)
     w = g.adjwgt if g.adjwgt is not None else np.ones(g.m, np.int32)
-    reps, edges, t0 = 0, 0, time.perf_counter()
-    while True:
-        O.srand(1)
-        t = time.perf_counter()
-        lv = O.coarsen(x, g.adjncy, w, None, k, matcher="cpu")
-        dt = time.perf_counter() - t
-        edges += sum(int(L["xadj"][-1]) // 2 for L in lv[:-1])
-        reps += 1
-        if time.perf_counter() - t0 > budget_s or dt * (reps + 1) > budget_s * 1.5:
-            break
-    el = time.perf_counter() - t0
+    old = os.sched_getaffinity(0)
+    core = min(old)
+    os.sched_setaffinity(0, {core})
+    try:
+        reps, edges, t0 = 0, 0, time.perf_counter()
+        while True:
+            O.srand(1)
+            t = time.perf_counter()
+            lv = O.coarsen(x, g.adjncy, w, None, k, matcher="cpu")
+            dt = time.perf_counter() - t
+            edges += sum(int(L["xadj"][-1]) // 2 for L in lv[:-1])
+            reps += 1
+            if time.perf_counter() - t0 > budget_s or dt * (reps + 1) > budget_s * 1.5:
+                break
+        el = time.perf_counter() - t0
+    finally:
+        os.sched_setaffinity(0, old)
     return {"value": round(edges / el / 1e6, 2), "unit": "Medges/s", "cores": 1, "kind": "port",
             "sample": f"{reps} full CPU coarsening run(s) of the same graph (cpu_match + "
-                      f"cpu_create_cgraph per level, k-way threshold), {el:.1f} s"}
+                      f"cpu_create_cgraph per level, k-way threshold), {el:.1f} s, pinned to core {core}",
+            "cpu_model": cpu_model(), "host_cpus": len(old)}
+
+
+def stream_copy_gbs(eng, reps=20, nbytes=1 << 30):
+    """Measured device-to-device copy bandwidth on this GPU (read + write
+    bytes per second of a 1 GiB copy through the engine's stream), the
+    on-box ceiling for streaming kernels."""
+    a, b = C.c_void_p(), C.c_void_p()
+    eng._check(eng.lib.cm_dmalloc(eng.ctx, nbytes, C.byref(a)))
+    eng._check(eng.lib.cm_dmalloc(eng.ctx, nbytes, C.byref(b)))
+    try:
+        eng._check(eng.lib.cm_memcpy_d2d(eng.ctx, b, a, nbytes))
+        eng.sync()
+        t = time.perf_counter()
+        for _ in range(reps):
+            eng._check(eng.lib.cm_memcpy_d2d(eng.ctx, b, a, nbytes))
+        eng.sync()
+        s = (time.perf_counter() - t) / reps
+    finally:
+        eng.lib.cm_dfree(eng.ctx, a)
+        eng.lib.cm_dfree(eng.ctx, b)
+    return round(2 * nbytes / s / 1e9, 1)
 
 
 def batch_graphs(count=512, seed=2024):
@@ -302,7 +342,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--order", default="first_seen", choices=["first_seen", "ascending"])
-    ap.add_argument("--legs", default="grid725", help="comma list of: grid725, rmat22, batch512, serve")
+    ap.add_argument("--legs", default="grid725,rmat22,batch512,serve",
+                    help="comma list of: grid725 (always run), rmat22, batch512, serve")
     ap.add_argument("--batch-count", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
@@ -337,6 +378,11 @@ def main():
         extra["batch512"] = batch_leg(eng, args.batch_count, order, 3)
     if "serve" in legs:
         extra["serve"] = serve_leg()
+    progress("stream copy")
+    copy_gbs = stream_copy_gbs(eng)
+    if roof is not None:
+        roof["measured_copy_gbs"] = copy_gbs
+        roof["frac_of_copy"] = round(roof["achieved"] / copy_gbs, 4)
     progress("cpu baseline")
     cpu = None if args.no_cpu else cpu_baseline(g, 64)
     out = {
@@ -349,6 +395,7 @@ def main():
                    "levels": main_out["levels"], "coarsest_n": main_out["coarsest_n"],
                    "rounds": main_out["rounds"]},
         "hbm_frac_whole_coarsening": main_out["hbm_frac_whole_coarsening"],
+        "copy_frac_whole_coarsening": round(main_out["hbm_frac_whole_coarsening"] * 8000.0 / copy_gbs, 5),
         "algo_bytes": main_out["algo_bytes"], "input_medges_per_s": main_out["input_medges_per_s"],
         "p10_ms": main_out["p10_ms"], "p90_ms": main_out["p90_ms"],
         "pcie_inclusive": main_out["pcie_inclusive"],

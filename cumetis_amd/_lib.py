@@ -65,6 +65,7 @@ _PROTOS = {
     "cm_dfree": (C.c_int, [_P, _P]),
     "cm_memcpy_h2d": (C.c_int, [_P, _P, _P, C.c_size_t]),
     "cm_memcpy_d2h": (C.c_int, [_P, _P, _P, C.c_size_t]),
+    "cm_memcpy_d2d": (C.c_int, [_P, _P, _P, C.c_size_t]),
     "cm_coarsen": (C.c_int, [_P, C.POINTER(GraphView), C.POINTER(CoarsenParams), C.POINTER(_P)]),
     "cm_coarsen_batch": (C.c_int, [_P, C.c_int32, C.POINTER(GraphView), C.POINTER(C.c_int64),
                                    C.POINTER(C.c_int64), C.POINTER(CoarsenParams), C.POINTER(_P)]),

@@ -676,6 +676,14 @@ int cm_memcpy_d2h(cm_ctx *ctx, void *dst, const void *src, size_t bytes)
   CM_API_END(ctx)
 }
 
+int cm_memcpy_d2d(cm_ctx *ctx, void *dst, const void *src, size_t bytes)
+{
+  if (!ctx) return CM_ERR_INVALID;
+  CM_API_BEGIN
+  if (bytes) CM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+  CM_API_END(ctx)
+}
+
 int cm_coarsen(cm_ctx *ctx, const cm_graph_view *g, const cm_coarsen_params *p, cm_hier **out)
 {
   if (!ctx || !out || !p) return CM_ERR_INVALID;

@@ -126,6 +126,9 @@ int cm_dmalloc(cm_ctx *ctx, size_t bytes, void **out);
 int cm_dfree(cm_ctx *ctx, void *p);
 int cm_memcpy_h2d(cm_ctx *ctx, void *dst, const void *src, size_t bytes);
 int cm_memcpy_d2h(cm_ctx *ctx, void *dst, const void *src, size_t bytes);
+/* Device-to-device copy queued on the context stream (returns without
+ * waiting); with cm_ctx_sync it measures the on-box copy bandwidth. */
+int cm_memcpy_d2d(cm_ctx *ctx, void *dst, const void *src, size_t bytes);
 
 /* ------------------------------------------------------------- hierarchy */
 /* Multilevel coarsening: replaces cuMetis_coarsen (cu:1824-1850).  Levels are
