@@ -647,25 +647,21 @@ __device__ __forceinline__ int merge_regs(int32_t c, int32_t v, int32_t u, const
   return len;
 }
 
-// One thread per coarse vertex, tiles of 256 coarse vertices in dispatch
-// order.  Segments of at most REG_SEG entries are merged in registers;
-// longer ones have been merged by the sorted path, which left their degree
-// in cdeg.  A look-back scan of the degrees gives cxadj, and the short rows
-// are written straight to their final place: the fine rows are read once
-// and the coarse rows written once.
 // First-seen merge of a short segment without sorting (row(v) then row(u),
 // cu:766-808): entry i is a head when no earlier entry has its coarse id, a
 // head's weight is its own plus that of its later duplicates, heads keep
 // their position order, and the head of the coarse vertex itself gives its
 // slot to the last head (cu:804-808).  dst[i] = row slot of entry i (-1 for
 // non-heads and the self head); returns the row length.  All comparisons use
-// static indices, so the arrays stay in registers.
+// static indices, so the arrays stay in registers, and each pair (j < i) is
+// compared once and used at once (a later duplicate adds its weight to the
+// earlier entry and marks itself), so no comparison result stays live.
 template <int N, typename OffT>
 __device__ __forceinline__ int merge_fs(int32_t c, int32_t v, int32_t u, const OffT *__restrict__ xadj,
                                         const int32_t *__restrict__ adjncy,
                                         const int32_t *__restrict__ adjwgt,
-                                        const int32_t *__restrict__ cmap, int32_t (&cu)[N], int32_t (&sum)[N],
-                                        int32_t (&dst)[N])
+                                        const int32_t *__restrict__ cmap, int32_t *cu, int32_t *sum,
+                                        int32_t *dst)
 {
   const int64_t b0 = (int64_t)xadj[v];
   const int64_t e0 = (int64_t)xadj[v + 1];
@@ -673,28 +669,32 @@ __device__ __forceinline__ int merge_fs(int32_t c, int32_t v, int32_t u, const O
   const int64_t e1 = (int64_t)xadj[u + 1];
   const int d0 = (int)(e0 - b0);
   const int cnt = d0 + (u != v ? (int)(e1 - b1) : 0);
-  int32_t a[N], wt[N];
+  int32_t a[N];
 #pragma unroll
   for (int i = 0; i < N; ++i) {  // unconditional loads (see merge_regs)
     const int64_t j = i < cnt ? (i < d0 ? b0 + i : b1 + (i - d0)) : 0;
     a[i] = adjncy[j];
-    wt[i] = adjwgt[j];
+    sum[i] = adjwgt[j];
   }
 #pragma unroll
   for (int i = 0; i < N; ++i) {
     const int32_t x = cmap[i < cnt ? a[i] : 0];
     cu[i] = i < cnt ? x : -1 - i;  // distinct negative ids for empty slots never match
   }
+  uint32_t dup = 0;  // bit i: entry i repeats an earlier one
+#pragma unroll
+  for (int i = 1; i < N; ++i) {
+#pragma unroll
+    for (int j = 0; j < i; ++j) {
+      const bool eq = cu[j] == cu[i];
+      sum[j] += eq ? sum[i] : 0;
+      dup |= eq ? (1u << i) : 0u;
+    }
+  }
   int r = 0, sidx = -1;
 #pragma unroll
   for (int i = 0; i < N; ++i) {
-    bool h = i < cnt;
-#pragma unroll
-    for (int j = 0; j < i; ++j) h = h && cu[j] != cu[i];
-    int32_t s = wt[i];
-#pragma unroll
-    for (int j = i + 1; j < N; ++j) s += cu[j] == cu[i] ? wt[j] : 0;
-    sum[i] = s;
+    const bool h = i < cnt && !((dup >> i) & 1u);
     dst[i] = h ? r : -1;
     if (h && cu[i] == c) sidx = r;
     r += h ? 1 : 0;
@@ -709,8 +709,17 @@ __device__ __forceinline__ int merge_fs(int32_t c, int32_t v, int32_t u, const O
   return r - 1;
 }
 
-constexpr int SMALL_TILE = 1024;   // coarse vertices per tile of k_contract_small
-constexpr int SMALL_STAGE = 8192;  // output entries staged in LDS per window
+// One thread per coarse vertex, tiles of SMALL_TILE coarse vertices in
+// dispatch order.  Segments of at most REG_SEG entries are merged in
+// registers; longer ones have been merged by the sorted or hashed path,
+// which left their degree in cdeg.  A look-back scan of the degrees gives
+// cxadj, and the short rows are written straight to their final place: the
+// fine rows are read once and the coarse rows written once.
+// Tiles of 256 keep every CU busy on small levels and balance the large
+// ones (the merge is VALU work per coarse vertex, so a tile's time is set by
+// its waves per SIMD, not by memory).
+constexpr int SMALL_TILE = 256;    // coarse vertices per tile of k_contract_small
+constexpr int SMALL_STAGE = 4096;  // output entries staged in LDS per window
 
 template <bool FS, typename OffT>
 __global__ __launch_bounds__(SMALL_TILE) void k_contract_small(const int64_t *__restrict__ ncp,
@@ -737,20 +746,31 @@ __global__ __launch_bounds__(SMALL_TILE) void k_contract_small(const int64_t *__
   const int64_t c = (int64_t)tile * SMALL_TILE + threadIdx.x;
   uint64_t k[REG_SEG], p[REG_SEG];            // ascending order: sorted (id, weight) pairs
   int32_t fcu[REG_SEG], fsum[REG_SEG], fdst[REG_SEG];  // first-seen order: entries and row slots
-  int deg = 0, len = 0;
-  bool small = false;
+  int deg = 0;
   // loads by a clamped index, all issued before the first wait
   const int64_t cc = c < nc ? c : nc - 1;
   const int64_t s0 = (int64_t)segoff[cc], s1 = (int64_t)segoff[cc + 1];
   const int32_t v = rep[cc], dl = cdeg_large[cc];
   const int32_t u = match[v];
-  if (c < nc) {
-    len = (int)(s1 - s0);
-    if (len <= REG_SEG) {
-      small = true;
-      if constexpr (FS) {
+  const int len = (int)(s1 - s0);
+  const bool small = c < nc && len <= REG_SEG;
+  if constexpr (FS) {
+    // the merge width is chosen per wave: 8 slots cost a quarter of 16
+    const bool wide = __ballot(small && len > 8) != 0;
+    if (small) {
+      if (wide) {
         deg = merge_fs<REG_SEG, OffT>((int32_t)c, v, u, xadj, adjncy, adjwgt, cmap, fcu, fsum, fdst);
-      } else if (len <= 8) {
+      } else {
+        deg = merge_fs<8, OffT>((int32_t)c, v, u, xadj, adjncy, adjwgt, cmap, fcu, fsum, fdst);
+#pragma unroll
+        for (int i = 8; i < REG_SEG; ++i) fdst[i] = -1;
+      }
+    } else if (c < nc) {
+      deg = dl;
+    }
+  } else if (c < nc) {
+    if (small) {
+      if (len <= 8) {
         uint64_t k8[8], p8[8];
         deg = merge_regs<8, FS, OffT>((int32_t)c, v, u, xadj, adjncy, adjwgt, cmap, k8, p8);
 #pragma unroll
@@ -782,7 +802,7 @@ __global__ __launch_bounds__(SMALL_TILE) void k_contract_small(const int64_t *__
   // consecutive threads.  Slots of longer rows (written later by their own
   // pass) keep the sentinel and are skipped.  Windows cover the range up to
   // the end of the last short row.
-  const uint64_t my_end = (small && c < nc) ? ex + (uint64_t)deg : 0;
+  const uint64_t my_end = small ? ex + (uint64_t)deg : 0;
   uint64_t span = wave_max<uint64_t>(my_end);
   if (lane() == 0) sh[threadIdx.x >> 6] = span;
   __syncthreads();
@@ -793,18 +813,22 @@ __global__ __launch_bounds__(SMALL_TILE) void k_contract_small(const int64_t *__
     const int wlen = (int)(span - w0 < (uint64_t)SMALL_STAGE ? span - w0 : (uint64_t)SMALL_STAGE);
     for (int k = threadIdx.x; k < wlen; k += SMALL_TILE) stage_a[k] = -1;
     __syncthreads();
-    if (small && c < nc) {
+    // the row's offset inside the window, in 32 bits once the row is known
+    // to overlap it
+    const int64_t rel = (int64_t)ex - (int64_t)w0;
+    if (small && rel < wlen && rel + deg > 0) {
+      const int r32 = (int)rel;
 #pragma unroll
       for (int i = 0; i < REG_SEG; ++i) {
         if constexpr (FS) {
-          const int64_t pos = (int64_t)ex + fdst[i] - (int64_t)w0;
-          if (fdst[i] >= 0 && pos >= 0 && pos < wlen) {
+          const int pos = r32 + fdst[i];
+          if (fdst[i] >= 0 && (unsigned)pos < (unsigned)wlen) {
             stage_a[pos] = fcu[i];
             stage_w[pos] = fsum[i];
           }
         } else {
-          const int64_t pos = (int64_t)(ex + i) - (int64_t)w0;
-          if (i < deg && pos >= 0 && pos < wlen) {
+          const int pos = r32 + i;
+          if (i < deg && (unsigned)pos < (unsigned)wlen) {
             stage_a[pos] = (int32_t)(p[i] >> 32);
             stage_w[pos] = (int32_t)(uint32_t)p[i];
           }
