@@ -22,6 +22,7 @@ namespace cm {
 constexpr int LONG_DEG = LONG_ROW;  // rows longer than this are scanned by whole waves
 constexpr int LONG_CHUNK = 2048;  // entries per wave of a long row
 constexpr int SELECT_PER_THREAD = 16;
+constexpr int MID_GROUP = 16;  // mid rows per wave of the long-row path
 constexpr int ROUND_BATCH = MATCH_BATCH;  // launches queued between host checks
 
 // flags[2r] = 1 if a vertex matched or died in round r, flags[2r+1] = 1 if
@@ -67,49 +68,72 @@ __global__ __launch_bounds__(256) void k_match_init(int64_t n, int32_t *match, i
   for (int64_t i = v; i < nflags; i += (int64_t)gridDim.x * blockDim.x) flags[i] = 0u;
 }
 
-// Long rows are cut into chunks of up to LONG_CHUNK entries, one wave per
-// chunk.  Chunk descriptor: the row, its first chunk, its chunk count and the
-// chunk's index within the row.
+// Rows longer than LONG_DEG are scanned by whole waves.  Rows of at most
+// LONG_CHUNK entries ("mid" rows) are listed by vertex; longer rows ("hubs")
+// are cut into chunks of LONG_CHUNK entries, one wave per chunk.  Chunk
+// descriptor: the row, its first chunk, its chunk count and the chunk's index
+// within the row.
 struct LongChunk {
   int32_t v, first, nch, k;
 };
 
-// Chunk descriptors of all rows longer than LONG_DEG; the order of rows in
-// the list does not matter (the reduction over a row's chunks is exact).
+// count[0] = hub chunks, count[1] = mid rows.  The order of the lists does
+// not matter (the reduction over a row's chunks is exact).
 template <typename OffT>
 __global__ __launch_bounds__(256) void k_select_long(int64_t n, const OffT *xadj, LongChunk *chunks,
-                                                     unsigned *count)
+                                                     int32_t *mids, unsigned *count)
 {
-  // SELECT_PER_THREAD consecutive vertices per thread and one counter
-  // update per block, so that blocks do not queue on the counter
-  __shared__ unsigned s_w[4], s_base;
+  // SELECT_PER_THREAD consecutive vertices per thread and one update of each
+  // counter per block, so that blocks do not queue on the counters
+  __shared__ unsigned s_h[4], s_m[4], s_hb, s_mb;
   const int64_t v0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * SELECT_PER_THREAD;
-  unsigned nch[SELECT_PER_THREAD], mine = 0;
+  unsigned nch[SELECT_PER_THREAD], hub = 0, mid = 0;
 #pragma unroll
   for (int t = 0; t < SELECT_PER_THREAD; ++t) {
     const int64_t v = v0 + t;
     const int64_t d = v < n ? (int64_t)(xadj[v + 1] - xadj[v]) : 0;
     nch[t] = d > LONG_DEG ? (unsigned)((d + LONG_CHUNK - 1) / LONG_CHUNK) : 0u;
-    mine += nch[t];
+    hub += nch[t] > 1 ? nch[t] : 0u;
+    mid += nch[t] == 1 ? 1u : 0u;
   }
-  const unsigned inc = wave_incl_scan<unsigned>(mine);
+  const unsigned hinc = wave_incl_scan<unsigned>(hub), minc = wave_incl_scan<unsigned>(mid);
   const int w = threadIdx.x >> 6;
-  if (lane() == CM_WAVE - 1) s_w[w] = inc;
+  if (lane() == CM_WAVE - 1) {
+    s_h[w] = hinc;
+    s_m[w] = minc;
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
-    const unsigned tot = s_w[0] + s_w[1] + s_w[2] + s_w[3];
-    s_base = tot ? atomicAdd(count, tot) : 0u;
+    const unsigned ht = s_h[0] + s_h[1] + s_h[2] + s_h[3], mt = s_m[0] + s_m[1] + s_m[2] + s_m[3];
+    s_hb = ht ? atomicAdd(&count[0], ht) : 0u;
+    s_mb = mt ? atomicAdd(&count[1], mt) : 0u;
   }
   __syncthreads();
-  unsigned base = s_base + inc - mine;
-  for (int i = 0; i < w; ++i) base += s_w[i];
+  unsigned hb = s_hb + hinc - hub, mb = s_mb + minc - mid;
+  for (int i = 0; i < w; ++i) {
+    hb += s_h[i];
+    mb += s_m[i];
+  }
 #pragma unroll
   for (int t = 0; t < SELECT_PER_THREAD; ++t) {
-    for (unsigned k = 0; k < nch[t]; ++k)
-      chunks[base + k] = LongChunk{(int32_t)(v0 + t), (int32_t)base, (int32_t)nch[t], (int32_t)k};
-    base += nch[t];
+    if (nch[t] == 1) mids[mb++] = (int32_t)(v0 + t);
+    if (nch[t] > 1) {
+      for (unsigned k = 0; k < nch[t]; ++k)
+        chunks[hb + k] = LongChunk{(int32_t)(v0 + t), (int32_t)hb, (int32_t)nch[t], (int32_t)k};
+      hb += nch[t];
+    }
   }
 }
+
+// Wave-uniform copies of lane i's values (i uniform).
+__device__ __forceinline__ int32_t bcast(int32_t x, int i) { return __builtin_amdgcn_readlane(x, i); }
+__device__ __forceinline__ int64_t bcast(int64_t x, int i)
+{
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int32_t)(uint32_t)x, i);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int32_t)(uint32_t)((uint64_t)x >> 32), i);
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+__device__ __forceinline__ uint64_t bcast(uint64_t x, int i) { return (uint64_t)bcast((int64_t)x, i); }
 
 // Accepted in round r-1 (from its proposals alone): dead, or mutual.
 __device__ __forceinline__ bool accepted_prev(const int32_t *prev, int32_t u)
@@ -153,7 +177,9 @@ __device__ __forceinline__ void scan_row(int64_t v, OffT b, OffT e, int l0, int 
     for (int t = 0; t < SCAN_BATCH; ++t) {
       const bool look = u[t] >= 0 && u[t] != (int32_t)v && w[t] >= wmin;
       const int32_t uu = look ? u[t] : (int32_t)v;
-      mu[t] = match[uu];
+      // round 0 runs before any accept, so every neighbour is unmatched: all
+      // lanes read one shared word instead of gathering
+      mu[t] = match[have_prev ? uu : 0];
       wu[t] = check_w ? vwgt[uu] : 0;  // without check_w no pair can exceed maxv
       if (!look) u[t] = -1;
     }
@@ -191,13 +217,145 @@ __device__ __forceinline__ void wave_best(int32_t &best, uint64_t &bk1, uint64_t
   }
 }
 
-// Per-chunk partial results of split rows; arrived[first chunk of a row]
-// counts the row's finished chunks and is cleared by the last one.
+// Per-chunk partial results of hub rows.  scanned[first chunk of a row] =
+// the launch (r + 1) in which the row's chunks were scanned; k_hub_reduce
+// reduces them after that launch.
 struct LongPartial {
   int32_t *best;
   uint64_t *k1, *k2;
-  unsigned *arrived;
+  unsigned *scanned;
 };
+
+// One chunk of a hub row, scanned by the whole wave.  The chunk's best goes
+// to part[q] with plain stores; the next kernel on the stream reduces the
+// row's chunks (a hand-off inside the launch would need an L2 writeback and
+// invalidate per chunk for its release/acquire pair).
+template <typename OffT>
+__device__ __forceinline__ void hub_chunk(int64_t q, const LongChunk *chunks, const OffT *__restrict__ xadj,
+                                          const int32_t *__restrict__ adjncy, const int32_t *__restrict__ adjwgt,
+                                          const int32_t *__restrict__ vwgt, const MatchOwners &own,
+                                          const int32_t *match, const int32_t *prev, int32_t *cur,
+                                          const LongPartial &part, bool have_prev, int64_t maxvw,
+                                          unsigned epoch)
+{
+  const int l = lane();
+  const LongChunk c = chunks[q];
+  const int32_t v = c.v;
+  // loads by dependency level, as in the short path.  A vertex matched
+  // before this launch wrote no proposal in round r-1, so its prev entry
+  // is stale and must not be followed: test match first.
+  const int32_t mv = match[v];
+  const int32_t p = have_prev ? prev[v] : -1;
+  if (mv != -1) return;
+  const int32_t pc = p >= 0 ? p : v;
+  const int32_t pp = have_prev ? prev[pc] : -1;
+  const int32_t mp = match[pc];
+  const OffT xb = xadj[v], xe = xadj[v + 1];
+  const int32_t vw = vwgt[v];
+  const int32_t ppp = have_prev ? prev[pp >= 0 ? pp : pc] : -1;
+  const bool v_acc = have_prev && (p == -2 || (p >= 0 && pp == v));  // accepted_prev(prev, v)
+  if (v_acc) return;
+  const bool p_acc = pp == -2 || (pp >= 0 && ppp == p);
+  if (p >= 0 && mp == -1 && !p_acc) {
+    if (l == 0 && c.k == 0) cur[v] = p;
+    return;
+  }
+  int32_t maxv;
+  uint64_t seed;
+  int64_t base;
+  owner_params(own, v, maxv, seed, base);
+  int32_t best = -2;
+  uint64_t bk1 = 0, bk2 = 0;
+  const OffT b = xb + (OffT)c.k * LONG_CHUNK, e0 = xe;
+  const OffT e = e0 - b > (OffT)LONG_CHUNK ? b + (OffT)LONG_CHUNK : e0;
+  scan_row<OffT>(v, b, e, l, CM_WAVE, adjncy, adjwgt, vwgt, vw, maxv, seed, base, match, prev, have_prev,
+                 maxvw < 0 || (int64_t)vw + maxvw > maxv, best, bk1, bk2);
+  wave_best(best, bk1, bk2);
+  if (l == 0) {
+    part.best[q] = best;
+    part.k1[q] = bk1;
+    part.k2[q] = bk2;
+    if (c.k == 0) part.scanned[q] = epoch;
+  }
+}
+
+// After launch r: cur[v] of every hub row scanned in it = the best of its
+// chunks' partial results.  One wave per row, found at its first chunk.
+__global__ __launch_bounds__(256) void k_hub_reduce(const LongChunk *chunks, const unsigned *nchunks,
+                                                    LongPartial part, int32_t *cur, const unsigned *flags, int r)
+{
+  if (r >= 2 && !round_live(flags, r - 2)) return;  // launch r did nothing
+  const unsigned nh = nchunks[0];
+  const int l = lane();
+  const int64_t nw = (int64_t)gridDim.x * (blockDim.x / CM_WAVE);
+  for (int64_t q = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / CM_WAVE; q < nh; q += nw) {
+    const LongChunk c = chunks[q];
+    if (c.k != 0 || part.scanned[q] != (unsigned)r + 1u) continue;
+    int32_t best = -2;
+    uint64_t bk1 = 0, bk2 = 0;
+    for (int i = l; i < c.nch; i += CM_WAVE) {
+      const int32_t ob = part.best[q + i];
+      const uint64_t o1 = part.k1[q + i], o2 = part.k2[q + i];
+      if (ob != -2 && (best == -2 || key_better(o1, o2, bk1, bk2))) {
+        best = ob;
+        bk1 = o1;
+        bk2 = o2;
+      }
+    }
+    wave_best(best, bk1, bk2);
+    if (l == 0) cur[c.v] = best;
+  }
+}
+
+// Mid rows mids[m0 .. m0+MID_GROUP): lane i loads row i's vertex state (the
+// dependency levels of the short path, for all rows of the group at once),
+// then the wave scans the rows that need it one after another, with each
+// row's parameters read from its lane.  A wave thus waits for the state
+// loads once per group instead of once per row.
+template <typename OffT>
+__device__ __forceinline__ void mid_group(int64_t m0, unsigned nm, const int32_t *mids,
+                                          const OffT *__restrict__ xadj, const int32_t *__restrict__ adjncy,
+                                          const int32_t *__restrict__ adjwgt, const int32_t *__restrict__ vwgt,
+                                          const MatchOwners &own, const int32_t *match, const int32_t *prev,
+                                          int32_t *cur, bool have_prev, int64_t maxvw)
+{
+  const int l = lane();
+  const int64_t mi = m0 + l;
+  const bool has = l < MID_GROUP && mi < (int64_t)nm;
+  const int32_t v = mids[has ? mi : m0];
+  const int32_t mv = match[v];
+  const int32_t p = have_prev ? prev[v] : -1;
+  const OffT xb = xadj[v], xe = xadj[v + 1];
+  const int32_t vw = vwgt[v];
+  int32_t maxv;
+  uint64_t seed;
+  int64_t base;
+  owner_params(own, v, maxv, seed, base);
+  // v's own prev entry is stale once v is matched: it is not followed
+  const int32_t pc = mv == -1 && p >= 0 ? p : v;
+  const int32_t pp = have_prev ? prev[pc] : -1;
+  const int32_t mp = match[pc];
+  const int32_t ppp = have_prev ? prev[pp >= 0 ? pp : pc] : -1;
+  const bool v_acc = have_prev && (p == -2 || (p >= 0 && pp == v));  // accepted_prev(prev, v)
+  const bool p_acc = pp == -2 || (pp >= 0 && ppp == p);
+  const bool live = has && mv == -1 && !v_acc;
+  const bool reuse = p >= 0 && mp == -1 && !p_acc;
+  if (live && reuse) cur[v] = p;  // still unmatched, so still the best
+  const int32_t cw = maxvw < 0 || (int64_t)vw + maxvw > maxv ? 1 : 0;
+  uint64_t todo = __ballot(live && !reuse);
+  while (todo) {
+    const int i = __ffsll((unsigned long long)todo) - 1;
+    todo &= todo - 1;
+    const int32_t vi = bcast(v, i);
+    const OffT bi = (OffT)bcast((int64_t)xb, i), ei = (OffT)bcast((int64_t)xe, i);
+    int32_t best = -2;
+    uint64_t bk1 = 0, bk2 = 0;
+    scan_row<OffT>(vi, bi, ei, l, CM_WAVE, adjncy, adjwgt, vwgt, bcast(vw, i), bcast(maxv, i), bcast(seed, i),
+                   bcast(base, i), match, prev, have_prev, bcast(cw, i) != 0, best, bk1, bk2);
+    wave_best(best, bk1, bk2);
+    if (l == 0) cur[vi] = best;
+  }
+}
 
 // Launch r of the handshake: accept of round r-1 (from prev = proposals of
 // round r-1) and proposals of round r (into cur), over the state after round
@@ -212,7 +370,8 @@ __global__ __launch_bounds__(256) void k_round(int64_t n, const OffT *__restrict
                                                const int32_t *__restrict__ adjwgt,
                                                const int32_t *__restrict__ vwgt, MatchOwners own,
                                                int32_t *match, const int32_t *prev, int32_t *cur,
-                                               const LongChunk *chunks, const unsigned *nchunks,
+                                               const LongChunk *chunks, const int32_t *mids,
+                                               const unsigned *nchunks,
                                                LongPartial part, unsigned *flags, int r, unsigned nshort,
                                                int propose, int64_t maxvw)
 {
@@ -286,74 +445,20 @@ __global__ __launch_bounds__(256) void k_round(int64_t n, const OffT *__restrict
     return;
   }
   if (!propose) return;
-  // long rows: one wave per chunk.  Whether a row is scanned depends only on
-  // state fixed before this launch (see above), so all chunks of a row agree.
-  const unsigned nq = *nchunks;
-  const int l = lane();
+  // Long rows.  Work items in order: the hub chunks (one wave each), then
+  // groups of MID_GROUP mid rows (one wave per group).  Whether a row is
+  // scanned depends only on state fixed before this launch (see above), so
+  // all chunks of a row agree.
+  const unsigned nh = nchunks[0], nm = nchunks[1];
+  const int64_t items = (int64_t)nh + ((int64_t)nm + MID_GROUP - 1) / MID_GROUP;
   const int64_t nw = (int64_t)(gridDim.x - nshort) * (blockDim.x / CM_WAVE);
-  for (int64_t q = ((int64_t)(blockIdx.x - nshort) * blockDim.x + threadIdx.x) / CM_WAVE; q < nq; q += nw) {
-    const LongChunk c = chunks[q];
-    const int32_t v = c.v;
-    // loads by dependency level, as in the short path.  A vertex matched
-    // before this launch wrote no proposal in round r-1, so its prev entry
-    // is stale and must not be followed: test match first.
-    const int32_t mv = match[v];
-    const int32_t p = have_prev ? prev[v] : -1;
-    if (mv != -1) continue;
-    const int32_t pc = p >= 0 ? p : v;
-    const int32_t pp = have_prev ? prev[pc] : -1;
-    const int32_t mp = match[pc];
-    const OffT xb = xadj[v], xe = xadj[v + 1];
-    const int32_t vw = vwgt[v];
-    const int32_t ppp = have_prev ? prev[pp >= 0 ? pp : pc] : -1;
-    const bool v_acc = have_prev && (p == -2 || (p >= 0 && pp == v));  // accepted_prev(prev, v)
-    if (v_acc) continue;
-    const bool p_acc = pp == -2 || (pp >= 0 && ppp == p);
-    if (p >= 0 && mp == -1 && !p_acc) {
-      if (l == 0 && c.k == 0) cur[v] = p;
-      continue;
-    }
-    int32_t maxv;
-    uint64_t seed;
-    int64_t base;
-    owner_params(own, v, maxv, seed, base);
-    int32_t best = -2;
-    uint64_t bk1 = 0, bk2 = 0;
-    const OffT b = xb + (OffT)c.k * LONG_CHUNK, e0 = xe;
-    const OffT e = e0 - b > (OffT)LONG_CHUNK ? b + (OffT)LONG_CHUNK : e0;
-    scan_row<OffT>(v, b, e, l, CM_WAVE, adjncy, adjwgt, vwgt, vw, maxv, seed, base, match, prev, have_prev,
-                   maxvw < 0 || (int64_t)vw + maxvw > maxv, best, bk1, bk2);
-    wave_best(best, bk1, bk2);
-    if (c.nch == 1) {
-      if (l == 0) cur[v] = best;
-      continue;
-    }
-    // publish the chunk's best; the row's last chunk to finish reduces them
-    unsigned last = 0;
-    if (l == 0) {
-      __hip_atomic_store(&part.best[q], best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(&part.k1[q], bk1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(&part.k2[q], bk2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      const unsigned old = __hip_atomic_fetch_add(&part.arrived[c.first], 1u, __ATOMIC_ACQ_REL,
-                                                  __HIP_MEMORY_SCOPE_AGENT);
-      last = old == (unsigned)c.nch - 1;
-      if (last) __hip_atomic_store(&part.arrived[c.first], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    if (!__shfl(last, 0, CM_WAVE)) continue;
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    best = -2;
-    for (int i = l; i < c.nch; i += CM_WAVE) {
-      const int32_t ob = __hip_atomic_load(&part.best[c.first + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      const uint64_t o1 = __hip_atomic_load(&part.k1[c.first + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      const uint64_t o2 = __hip_atomic_load(&part.k2[c.first + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (ob != -2 && (best == -2 || key_better(o1, o2, bk1, bk2))) {
-        best = ob;
-        bk1 = o1;
-        bk2 = o2;
-      }
-    }
-    wave_best(best, bk1, bk2);
-    if (l == 0) cur[v] = best;
+  for (int64_t it = ((int64_t)(blockIdx.x - nshort) * blockDim.x + threadIdx.x) / CM_WAVE; it < items; it += nw) {
+    if (it < nh)
+      hub_chunk<OffT>(it, chunks, xadj, adjncy, adjwgt, vwgt, own, match, prev, cur, part, have_prev, maxvw,
+                      (unsigned)r + 1u);
+    else
+      mid_group<OffT>((it - nh) * MID_GROUP, nm, mids, xadj, adjncy, adjwgt, vwgt, own, match, prev, cur,
+                      have_prev, maxvw);
   }
 }
 
@@ -377,10 +482,12 @@ struct MatchRun {
   int max_rounds = 0;
   int32_t *match = nullptr;
   int32_t *prop = nullptr;
-  LongChunk *chunks = nullptr;
+  LongChunk *chunks = nullptr;  // hub chunks
+  int32_t *mids = nullptr;      // mid rows
   LongPartial part{};
   unsigned *flags = nullptr, *nchunks = nullptr;
   unsigned g = 0, glong = 0;
+  unsigned ghub = 0;  // blocks of k_hub_reduce (0: the level has no hub row)
   int r = 0;  // launches queued so far
   std::vector<void *> blocks;
 };
@@ -399,15 +506,19 @@ static void match_start(cm_ctx *ctx, MatchRun &mr)
   mr.prop = (int32_t *)al(sizeof(int32_t) * 2 * n);  // ping-pong
   // chunks: sum over rows longer than LONG_DEG of ceil(d / LONG_CHUNK)
   const bool has_long = L.maxdeg < 0 || L.maxdeg > LONG_DEG;
-  const int64_t cap = has_long ? L.m / LONG_CHUNK + std::min<int64_t>(n, L.m / (LONG_DEG + 1)) + 1 : 1;
+  // hub chunks: at most m / LONG_CHUNK full ones plus one partial per hub
+  // row (each hub row holds more than LONG_CHUNK entries)
+  const int64_t cap = has_long ? 2 * (L.m / LONG_CHUNK) + 1 : 1;
+  const int64_t cap_mid = has_long ? std::min<int64_t>(n, L.m / (LONG_DEG + 1)) + 1 : 1;
   mr.chunks = (LongChunk *)al(sizeof(LongChunk) * cap);
+  mr.mids = (int32_t *)al(sizeof(int32_t) * cap_mid);
   mr.part.best = (int32_t *)al(sizeof(int32_t) * cap);
   mr.part.k1 = (uint64_t *)al(sizeof(uint64_t) * cap);
   mr.part.k2 = (uint64_t *)al(sizeof(uint64_t) * cap);
-  mr.part.arrived = (unsigned *)al(sizeof(unsigned) * cap);
+  mr.part.scanned = (unsigned *)al(sizeof(unsigned) * cap);
   mr.flags = (unsigned *)al(sizeof(unsigned) * (2 * (size_t)mr.max_rounds + 2));
-  mr.nchunks = mr.flags + 2 * (size_t)mr.max_rounds;
-  if (has_long) fill_u32(ctx, mr.part.arrived, 0u, cap);
+  mr.nchunks = mr.flags + 2 * (size_t)mr.max_rounds;  // [0] hub chunks, [1] mid rows
+  if (has_long) fill_u32(ctx, mr.part.scanned, 0u, cap);  // no launch number is 0
   mr.g = nblocks(n, 256);
   launch(ctx, "match_init", 8.0 * n, k_match_init, dim3(mr.g), dim3(256), 0, n, mr.match, mr.prop, mr.flags,
          2 * (int64_t)mr.max_rounds + 2);
@@ -415,8 +526,11 @@ static void match_start(cm_ctx *ctx, MatchRun &mr)
   // level is known to have no long row)
   if (has_long) {
     launch(ctx, "match_select_long", (double)sizeof(OffT) * (n + 1), k_select_long<OffT>,
-           dim3(nblocks(n, 256 * SELECT_PER_THREAD)), dim3(256), 0, n, xadj, mr.chunks, mr.nchunks);
-    mr.glong = (unsigned)std::min<int64_t>(8192, std::max<int64_t>(16, (cap + 3) / 4));
+           dim3(nblocks(n, 256 * SELECT_PER_THREAD)), dim3(256), 0, n, xadj, mr.chunks, mr.mids,
+           mr.nchunks);
+    const int64_t items = cap + (cap_mid + MID_GROUP - 1) / MID_GROUP;  // a wave each
+    mr.glong = (unsigned)std::min<int64_t>(8192, std::max<int64_t>(16, (items + 3) / 4));
+    if (L.maxdeg < 0 || L.maxdeg > LONG_CHUNK) mr.ghub = (unsigned)std::min<int64_t>(4096, (cap + 3) / 4);
   }
 }
 
@@ -434,8 +548,13 @@ static void match_launch(cm_ctx *ctx, MatchRun &mr, int count)
     const int32_t *prv = mr.prop + (size_t)((r + 1) & 1) * n;
     launch(ctx, "match_round", r == 0 ? row_bytes : 0.0, k_round<OffT>, dim3(mr.g + mr.glong), dim3(256), 0, n,
            (const OffT *)L.xadj, L.adjncy, L.adjwgt, L.vwgt, mr.own, mr.match, prv, cur,
-           (const LongChunk *)mr.chunks, (const unsigned *)mr.nchunks, mr.part, mr.flags, r, mr.g,
+           (const LongChunk *)mr.chunks, (const int32_t *)mr.mids, (const unsigned *)mr.nchunks, mr.part,
+           mr.flags, r, mr.g,
            r < mr.max_rounds ? 1 : 0, L.maxvw);
+    if (mr.ghub && r < mr.max_rounds)
+      launch(ctx, "match_hub_reduce", 0.0, k_hub_reduce, dim3(mr.ghub), dim3(256), 0,
+             (const LongChunk *)mr.chunks, (const unsigned *)mr.nchunks, mr.part, cur, (const unsigned *)mr.flags,
+             r);
   }
 }
 
