@@ -22,7 +22,7 @@
 //   emit    per chunk, once the row offsets are known: staging -> rows.
 // Every item count stays on the device and fixed grids stride over it, so
 // the path needs no host synchronisation.  A hub partition whose table fills
-// (more than HC_SLOTS distinct keys, far beyond the expected HC_PART) sets a
+// (more than HC_HUB_SLOTS distinct keys, far beyond the expected HC_PART) sets a
 // flag; the caller then redoes the level on the sort-based path.
 #include "cm_common.hpp"
 #include "cm_contract_hash.hpp"
@@ -35,7 +35,15 @@ constexpr int HC_WAVE_MAX = 256;
 constexpr int HC_WAVE_SLOTS = 512;  // >= 2 * HC_WAVE_MAX
 constexpr int HC_BLOCK_MAX = 2048;
 constexpr int HC_SLOTS = 4096;      // >= 2 * HC_BLOCK_MAX
-constexpr int HC_PART = 2048;
+// Hub partitions: one 1024-thread block with a table of HC_HUB_SLOTS slots
+// (96 KB of LDS) per partition of about HC_PART entries.  Every partition
+// block reads the whole segment, so the segment is read len / HC_PART times:
+// the larger the table, the fewer the passes.
+constexpr int HC_PART = 4096;
+constexpr int HC_HUB_LG = 13;
+constexpr int HC_HUB_SLOTS = 1 << HC_HUB_LG;  // >= 2 * HC_PART
+constexpr int HC_HUB_THREADS = 1024;
+constexpr int HC_HUB_LIST = 6144;  // entries of the partition listed for the second pass
 constexpr int HC_SELECT = 16;       // segments per thread in the select pass
 
 struct SegChunk {
@@ -415,17 +423,19 @@ __global__ __launch_bounds__(256) void k_hc_dedup_block(const int32_t *__restric
 // pass walks that list instead of the segment (a full second pass remains for
 // lists that overflow and for segments of 2^19 entries or more).
 template <typename OffT>
-__global__ __launch_bounds__(256) void k_hc_dedup_hub(const HubPart *__restrict__ parts, const unsigned *ctr,
-                                                      const OffT *__restrict__ segoff,
-                                                      const int32_t *__restrict__ cval,
-                                                      const int32_t *__restrict__ wv, uint8_t *__restrict__ head,
-                                                      int32_t *__restrict__ hw, unsigned *overflow)
+__global__ __launch_bounds__(HC_HUB_THREADS) void k_hc_dedup_hub(const HubPart *__restrict__ parts,
+                                                                 const unsigned *ctr,
+                                                                 const OffT *__restrict__ segoff,
+                                                                 const int32_t *__restrict__ cval,
+                                                                 const int32_t *__restrict__ wv,
+                                                                 uint8_t *__restrict__ head,
+                                                                 int32_t *__restrict__ hw, unsigned *overflow)
 {
-  constexpr int lg = 12;  // HC_SLOTS
+  constexpr int lg = HC_HUB_LG;
   constexpr int SCAN = 4;
-  constexpr int LIST = 3072;
-  __shared__ int32_t tk[HC_SLOTS], tp[HC_SLOTS], ts[HC_SLOTS];
-  __shared__ uint32_t lst[LIST];
+  constexpr int T = HC_HUB_THREADS;
+  __shared__ int32_t tk[HC_HUB_SLOTS], tp[HC_HUB_SLOTS], ts[HC_HUB_SLOTS];
+  __shared__ uint32_t lst[HC_HUB_LIST];
   __shared__ int s_full;
   __shared__ unsigned s_cnt;
   const unsigned n = ctr[HC_N_PART];
@@ -433,7 +443,7 @@ __global__ __launch_bounds__(256) void k_hc_dedup_hub(const HubPart *__restrict_
     const HubPart hp = parts[q];
     const int64_t s0 = (int64_t)segoff[hp.c];
     const int64_t len = (int64_t)segoff[hp.c + 1] - s0;
-    for (int i = threadIdx.x; i < HC_SLOTS; i += 256) {
+    for (int i = threadIdx.x; i < HC_HUB_SLOTS; i += T) {
       tk[i] = -1;
       tp[i] = 0x7fffffff;
       ts[i] = 0;
@@ -443,17 +453,18 @@ __global__ __launch_bounds__(256) void k_hc_dedup_hub(const HubPart *__restrict_
       s_cnt = 0;
     }
     __syncthreads();
-    const bool listable = len < (1 << 19);
-    for (int64_t ib = threadIdx.x; ib < len; ib += SCAN * 256) {
+    // list entries pack (position << lg) | slot into 32 bits
+    const bool listable = len < (1ll << (32 - lg));
+    for (int64_t ib = threadIdx.x; ib < len; ib += SCAN * T) {
       int32_t cu[SCAN];
 #pragma unroll
       for (int t = 0; t < SCAN; ++t) {
-        const int64_t i = ib + (int64_t)t * 256;
+        const int64_t i = ib + (int64_t)t * T;
         cu[t] = i < len ? cval[s0 + i] : -1;
       }
 #pragma unroll
       for (int t = 0; t < SCAN; ++t) {
-        const int64_t i = ib + (int64_t)t * 256;
+        const int64_t i = ib + (int64_t)t * T;
         int s = -1;
         if (cu[t] >= 0 && hc_part(cu[t], hp.P) == hp.p) {
           s = hc_insert(tk, lg, cu[t]);
@@ -470,22 +481,22 @@ __global__ __launch_bounds__(256) void k_hc_dedup_hub(const HubPart *__restrict_
         if (mk && lane() == 0) base = atomicAdd(&s_cnt, (unsigned)__popcll(mk));
         base = __shfl(base, 0, CM_WAVE);
         const unsigned k = base + (unsigned)mask_rank(mk);
-        if (s >= 0 && k < (unsigned)LIST) lst[k] = ((uint32_t)i << 12) | (uint32_t)s;
+        if (s >= 0 && k < (unsigned)HC_HUB_LIST) lst[k] = ((uint32_t)i << lg) | (uint32_t)s;
       }
     }
     __syncthreads();
     if (s_full && threadIdx.x == 0) atomicOr(overflow, 1u);
-    if (listable && s_cnt <= (unsigned)LIST) {
-      for (unsigned k = threadIdx.x; k < s_cnt; k += 256) {
+    if (listable && s_cnt <= (unsigned)HC_HUB_LIST) {
+      for (unsigned k = threadIdx.x; k < s_cnt; k += T) {
         const uint32_t e = lst[k];
-        const int32_t i = (int32_t)(e >> 12);
-        const int s = (int)(e & 4095u);
+        const int32_t i = (int32_t)(e >> lg);
+        const int s = (int)(e & (uint32_t)(HC_HUB_SLOTS - 1));
         const bool h = tp[s] == i;
         head[s0 + i] = h ? 1 : 0;
         hw[s0 + i] = h ? ts[s] : 0;
       }
     } else {
-      for (int64_t i = threadIdx.x; i < len; i += 256) {
+      for (int64_t i = threadIdx.x; i < len; i += T) {
         const int32_t cu = cval[s0 + i];
         if (hc_part(cu, hp.P) != hp.p) continue;
         const int s = hc_find(tk, lg, cu);
@@ -680,8 +691,8 @@ void hash_prepare_impl(cm_ctx *ctx, const Level &L, const int32_t *match, const 
     int32_t *hw = (int32_t *)al(4 * m);
     int32_t *ccnt = (int32_t *)al(4 * nchunk), *cself = (int32_t *)al(4 * nchunk), *coff = (int32_t *)al(4 * nchunk);
     int32_t *hself = (int32_t *)al(4 * nchunk), *htot = (int32_t *)al(4 * nchunk);
-    launch(ctx, "hc_dedup_hub", 0.0, k_hc_dedup_hub<OffT>, dim3((unsigned)std::min<int64_t>(2048, npart)),
-           dim3(256), 0, (const HubPart *)parts, (const unsigned *)hc.ctr, segoff, (const int32_t *)cval,
+    launch(ctx, "hc_dedup_hub", 0.0, k_hc_dedup_hub<OffT>, dim3((unsigned)std::min<int64_t>(1024, npart)),
+           dim3(HC_HUB_THREADS), 0, (const HubPart *)parts, (const unsigned *)hc.ctr, segoff, (const int32_t *)cval,
            (const int32_t *)wv, head, hw, hc.ctr + 6);
     launch(ctx, "hc_hub_count", 0.0, k_hc_hub_count<OffT>, dim3(gchunk), dim3(256), 0, chunks,
            (const unsigned *)hc.ctr, segoff, (const uint8_t *)head, (const int32_t *)cval, ccnt, cself);

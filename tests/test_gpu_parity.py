@@ -95,6 +95,25 @@ def test_match_split_rows(engine, orc, unit):
         assert r_gpu == r_orc
 
 
+@pytest.mark.parametrize("hub_deg,side", [(140000, 400), (600000, 800)])
+def test_contract_hub_partitions(engine, orc, hub_deg, side):
+    """Hub segments cut into many key partitions (35 and 147 blocks each
+    re-reading the segment), the larger one past the length whose entries
+    can be listed in LDS (2^19), so the second pass rescans the segment."""
+    from cases import hub_graph
+    g = with_weights(hub_graph(seed=23, side=side, hub_deg=hub_deg, mids=(5000, 2100)))
+    say(f"[contract hub partitions] hub_deg={hub_deg} n={g.n} m={g.m}")
+    mv = maxv_for(g, orc=orc)
+    match, _ = orc.handshake_match(g.xadj, g.adjncy, g.adjwgt, g.vwgt, mv, seed=3)
+    cmap, nc = orc.number(match)
+    cx, ca, cw, cv = engine.contract(g, match, cmap, nc, CM_ROW_FIRST_SEEN)
+    ex, ea, ew, ev = orc.cpu_contract(g.xadj, g.adjncy, g.adjwgt, g.vwgt, match, cmap, nc)
+    np.testing.assert_array_equal(cx.astype(np.int64), ex)
+    np.testing.assert_array_equal(cv, ev)
+    np.testing.assert_array_equal(ca, ea)
+    np.testing.assert_array_equal(cw, ew)
+
+
 def test_match_repeatable(engine, orc):
     g = with_weights(CASES["rgg3000w16"])
     say("[match repeat] rgg3000w16")
