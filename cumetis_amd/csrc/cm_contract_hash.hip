@@ -11,7 +11,9 @@
 // come from a hash table in LDS (key, first position, weight sum):
 //   select  segments above REG_SEG: a chunk list (HC_CHUNK entries per chunk)
 //           and items by length -- up to HC_WAVE_MAX one wave, up to
-//           HC_BLOCK_MAX one block, longer ones ("hubs") one block per key
+//           HC_BLOCK_MAX one block (a half-size table up to HC_BLOCK_SPLIT,
+//           so that twice as many blocks fit a CU), longer ones ("hubs") one
+//           block per key
 //           partition of about HC_PART entries;
 //   gather  per chunk of a hub: coarse neighbour ids and weights in segment
 //           order (wave and block items read their rows directly);
@@ -34,7 +36,7 @@ constexpr int HC_CHUNK = 2048;
 constexpr int HC_WAVE_MAX = 256;
 constexpr int HC_WAVE_SLOTS = 512;  // >= 2 * HC_WAVE_MAX
 constexpr int HC_BLOCK_MAX = 2048;
-constexpr int HC_SLOTS = 4096;      // >= 2 * HC_BLOCK_MAX
+constexpr int HC_BLOCK_SPLIT = 1024;  // block items up to this use a table of 2 * HC_BLOCK_SPLIT slots
 // Hub partitions: one 1024-thread block with a table of HC_HUB_SLOTS slots
 // (96 KB of LDS) per partition of about HC_PART entries.  Every partition
 // block reads the whole segment, so the segment is read len / HC_PART times:
@@ -54,7 +56,8 @@ struct HubPart {
 };
 
 // item counters
-enum { HC_N_CHUNK = 0, HC_N_WAVE, HC_N_BLOCK, HC_N_PART, HC_N_HUB, HC_N_COUNTERS };
+// (ctr[6] is the hub-table overflow flag)
+enum { HC_N_CHUNK = 0, HC_N_WAVE, HC_N_BLOCK, HC_N_PART, HC_N_HUB, HC_N_BLOCKL, HC_N_COUNTERS };
 
 __device__ __forceinline__ void wave_sync()
 {
@@ -134,12 +137,13 @@ __device__ __forceinline__ SegRows seg_rows(int32_t c, const int32_t *__restrict
 template <typename OffT>
 __global__ __launch_bounds__(256) void k_hc_select(const int64_t *__restrict__ ncp, const OffT *__restrict__ segoff,
                                                    SegChunk *chunks, int32_t *wave_items, int32_t *block_items,
+                                                   int32_t *blockl_items,
                                                    HubPart *parts, int32_t *hubs, unsigned *ctr)
 {
   __shared__ unsigned s_w[HC_N_COUNTERS][4], s_base[HC_N_COUNTERS];
   const int64_t nc = *ncp;
   const int64_t c0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * HC_SELECT;
-  unsigned cnt[HC_N_COUNTERS] = {0, 0, 0, 0, 0};
+  unsigned cnt[HC_N_COUNTERS] = {0, 0, 0, 0, 0, 0};
   // the HC_SELECT + 1 offsets load at once (clamped to nc)
   int64_t so[HC_SELECT + 1];
 #pragma unroll
@@ -152,7 +156,8 @@ __global__ __launch_bounds__(256) void k_hc_select(const int64_t *__restrict__ n
     if (len <= REG_SEG) continue;
     cnt[HC_N_CHUNK] += (unsigned)((len + HC_CHUNK - 1) / HC_CHUNK);
     if (len <= HC_WAVE_MAX) ++cnt[HC_N_WAVE];
-    else if (len <= HC_BLOCK_MAX) ++cnt[HC_N_BLOCK];
+    else if (len <= HC_BLOCK_SPLIT) ++cnt[HC_N_BLOCK];
+    else if (len <= HC_BLOCK_MAX) ++cnt[HC_N_BLOCKL];
     else {
       ++cnt[HC_N_HUB];
       cnt[HC_N_PART] += (unsigned)((len + HC_PART - 1) / HC_PART);
@@ -190,8 +195,10 @@ __global__ __launch_bounds__(256) void k_hc_select(const int64_t *__restrict__ n
     at[HC_N_CHUNK] += nch;
     if (len <= HC_WAVE_MAX) {
       wave_items[at[HC_N_WAVE]++] = (int32_t)c;
-    } else if (len <= HC_BLOCK_MAX) {
+    } else if (len <= HC_BLOCK_SPLIT) {
       block_items[at[HC_N_BLOCK]++] = (int32_t)c;
+    } else if (len <= HC_BLOCK_MAX) {
+      blockl_items[at[HC_N_BLOCKL]++] = (int32_t)c;
     } else {
       hubs[at[HC_N_HUB]++] = (int32_t)first;  // a hub is named by its first chunk
       const int P = (int)((len + HC_PART - 1) / HC_PART);
@@ -332,9 +339,10 @@ __global__ __launch_bounds__(256) void k_hc_dedup_wave(const int32_t *__restrict
 }
 
 // ----------------------------------------------------- dedup: one block
-// Segments of HC_WAVE_MAX+1 .. HC_BLOCK_MAX entries, one per block.
-template <typename OffT>
-__global__ __launch_bounds__(256) void k_hc_dedup_block(const int32_t *__restrict__ items, const unsigned *ctr,
+// Segments of up to 2^(LG-1) entries (above HC_WAVE_MAX), one per block, a
+// table of 2^LG slots; *count items.
+template <typename OffT, int LG>
+__global__ __launch_bounds__(256) void k_hc_dedup_block(const int32_t *__restrict__ items, const unsigned *count,
                                                         const OffT *__restrict__ segoff,
                                                         const int32_t *__restrict__ rep,
                                                         const int32_t *__restrict__ match,
@@ -344,11 +352,12 @@ __global__ __launch_bounds__(256) void k_hc_dedup_block(const int32_t *__restric
                                                         const int32_t *__restrict__ cmap, int32_t *__restrict__ tc,
                                                         int32_t *__restrict__ tw, int32_t *__restrict__ cdeg)
 {
-  constexpr int PER = HC_BLOCK_MAX / 256;
-  __shared__ int32_t tk[HC_SLOTS], tp[HC_SLOTS], ts[HC_SLOTS];
+  constexpr int SLOTS = 1 << LG;
+  constexpr int PER = SLOTS / 2 / 256;
+  __shared__ int32_t tk[SLOTS], tp[SLOTS], ts[SLOTS];
   __shared__ int sh[4];
   __shared__ int s_self;
-  const unsigned n = ctr[HC_N_BLOCK];
+  const unsigned n = *count;
   for (int64_t q = blockIdx.x; q < n; q += gridDim.x) {
     const int32_t c = items[q];
     const int64_t s0 = (int64_t)segoff[c];
@@ -656,6 +665,7 @@ void hash_prepare_impl(cm_ctx *ctx, const Level &L, const int32_t *match, const 
   const int64_t nseg = std::min<int64_t>(nc, m / (REG_SEG + 1)) + 1;
   const int64_t nchunk = m / HC_CHUNK + nseg;
   const int64_t nblk = std::min<int64_t>(nc, m / (HC_WAVE_MAX + 1)) + 1;
+  const int64_t nblkl = std::min<int64_t>(nc, m / (HC_BLOCK_SPLIT + 1)) + 1;
   const int64_t nhub = m / (HC_BLOCK_MAX + 1) + 1;
   const int64_t npart = m / HC_PART + nhub;
   auto al = [&](size_t bytes) {
@@ -665,6 +675,7 @@ void hash_prepare_impl(cm_ctx *ctx, const Level &L, const int32_t *match, const 
   };
   hc.chunks = al(sizeof(SegChunk) * nchunk);
   int32_t *wave_items = (int32_t *)al(4 * nseg), *block_items = (int32_t *)al(4 * nblk);
+  int32_t *blockl_items = (int32_t *)al(4 * nblkl);
   HubPart *parts = (HubPart *)al(sizeof(HubPart) * npart);
   int32_t *hubs = (int32_t *)al(4 * nhub);
   hc.ctr = (unsigned *)al(4 * 8);
@@ -673,16 +684,22 @@ void hash_prepare_impl(cm_ctx *ctx, const Level &L, const int32_t *match, const 
   fill_u32(ctx, hc.ctr, 0u, 8);
   const SegChunk *chunks = (const SegChunk *)hc.chunks;
   launch(ctx, "hc_select", 0.0, k_hc_select<OffT>, dim3(nblocks(nc, 256 * HC_SELECT)), dim3(256), 0, d_nc, segoff,
-         (SegChunk *)hc.chunks, wave_items, block_items, parts, hubs, hc.ctr);
+         (SegChunk *)hc.chunks, wave_items, block_items, blockl_items, parts, hubs, hc.ctr);
   const unsigned gchunk = (unsigned)std::min<int64_t>(4096, (nchunk + 3) / 4);
   hc.gchunk = gchunk;
   launch(ctx, "hc_dedup_wave", 0.0, k_hc_dedup_wave<OffT>, dim3((unsigned)std::min<int64_t>(4096, (nseg + 3) / 4)),
          dim3(256), 0, (const int32_t *)wave_items, (const unsigned *)hc.ctr, segoff, rep, match, xadj, L.adjncy,
          L.adjwgt, cmap, hc.tc, hc.tw, cdeg);
+  static_assert(2 * HC_BLOCK_SPLIT == 1 << 11 && 2 * HC_BLOCK_MAX == 1 << 12, "block table sizes");
   if (seg_max > HC_WAVE_MAX)
-    launch(ctx, "hc_dedup_block", 0.0, k_hc_dedup_block<OffT>, dim3((unsigned)std::min<int64_t>(2048, nblk)),
-           dim3(256), 0, (const int32_t *)block_items, (const unsigned *)hc.ctr, segoff, rep, match, xadj,
-           L.adjncy, L.adjwgt, cmap, hc.tc, hc.tw, cdeg);
+    launch(ctx, "hc_dedup_block", 0.0, k_hc_dedup_block<OffT, 11>, dim3((unsigned)std::min<int64_t>(4096, nblk)),
+           dim3(256), 0, (const int32_t *)block_items, (const unsigned *)hc.ctr + HC_N_BLOCK, segoff, rep, match,
+           xadj, L.adjncy, L.adjwgt, cmap, hc.tc, hc.tw, cdeg);
+  if (seg_max > HC_BLOCK_SPLIT)
+    launch(ctx, "hc_dedup_block_l", 0.0, k_hc_dedup_block<OffT, 12>,
+           dim3((unsigned)std::min<int64_t>(2048, nblkl)), dim3(256), 0, (const int32_t *)blockl_items,
+           (const unsigned *)hc.ctr + HC_N_BLOCKL, segoff, rep, match, xadj, L.adjncy, L.adjwgt, cmap, hc.tc, hc.tw,
+           cdeg);
   if (m > HC_BLOCK_MAX && seg_max > HC_BLOCK_MAX) {  // a hub needs more than HC_BLOCK_MAX entries
     int32_t *cval = (int32_t *)al(4 * m), *wv = (int32_t *)al(4 * m);
     launch(ctx, "hc_gather", 0.0, k_hc_gather<OffT>, dim3(gchunk), dim3(256), 0, chunks, (const unsigned *)hc.ctr,

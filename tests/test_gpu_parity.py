@@ -101,7 +101,7 @@ def test_contract_hub_partitions(engine, orc, hub_deg, side):
     re-reading the segment), the larger one past the length whose entries
     can be listed in LDS (2^19), so the second pass rescans the segment."""
     from cases import hub_graph
-    g = with_weights(hub_graph(seed=23, side=side, hub_deg=hub_deg, mids=(5000, 2100)))
+    g = with_weights(hub_graph(seed=23, side=side, hub_deg=hub_deg, mids=(5000, 2100, 1500, 600)))
     say(f"[contract hub partitions] hub_deg={hub_deg} n={g.n} m={g.m}")
     mv = maxv_for(g, orc=orc)
     match, _ = orc.handshake_match(g.xadj, g.adjncy, g.adjwgt, g.vwgt, mv, seed=3)
