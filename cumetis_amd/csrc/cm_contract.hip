@@ -755,15 +755,20 @@ __global__ __launch_bounds__(SMALL_TILE) void k_contract_small(const int64_t *__
   const int len = (int)(s1 - s0);
   const bool small = c < nc && len <= REG_SEG;
   if constexpr (FS) {
-    // the merge width is chosen per wave: 8 slots cost a quarter of 16
+    // the merge width is chosen per wave (pairs compared: 120, 28 or 6)
     const bool wide = __ballot(small && len > 8) != 0;
+    const bool narrow = __ballot(small && len > 4) == 0;
     if (small) {
       if (wide) {
         deg = merge_fs<REG_SEG, OffT>((int32_t)c, v, u, xadj, adjncy, adjwgt, cmap, fcu, fsum, fdst);
-      } else {
+      } else if (!narrow) {
         deg = merge_fs<8, OffT>((int32_t)c, v, u, xadj, adjncy, adjwgt, cmap, fcu, fsum, fdst);
 #pragma unroll
         for (int i = 8; i < REG_SEG; ++i) fdst[i] = -1;
+      } else {
+        deg = merge_fs<4, OffT>((int32_t)c, v, u, xadj, adjncy, adjwgt, cmap, fcu, fsum, fdst);
+#pragma unroll
+        for (int i = 4; i < REG_SEG; ++i) fdst[i] = -1;
       }
     } else if (c < nc) {
       deg = dl;

@@ -34,7 +34,7 @@ namespace cm {
 
 constexpr int HC_CHUNK = 2048;
 constexpr int HC_WAVE_MAX = 256;
-constexpr int HC_WAVE_SLOTS = 512;  // >= 2 * HC_WAVE_MAX
+constexpr int HC_WAVE_SPLIT = 64;  // wave items up to this use a table of 2 * HC_WAVE_SPLIT slots
 constexpr int HC_BLOCK_MAX = 2048;
 constexpr int HC_BLOCK_SPLIT = 1024;  // block items up to this use a table of 2 * HC_BLOCK_SPLIT slots
 // Hub partitions: one 1024-thread block with a table of HC_HUB_SLOTS slots
@@ -56,8 +56,9 @@ struct HubPart {
 };
 
 // item counters
-// (ctr[6] is the hub-table overflow flag)
-enum { HC_N_CHUNK = 0, HC_N_WAVE, HC_N_BLOCK, HC_N_PART, HC_N_HUB, HC_N_BLOCKL, HC_N_COUNTERS };
+enum { HC_N_CHUNK = 0, HC_N_WAVE, HC_N_BLOCK, HC_N_PART, HC_N_HUB, HC_N_BLOCKL, HC_N_WAVEL, HC_N_COUNTERS };
+constexpr int HC_OVERFLOW = 8;  // ctr[HC_OVERFLOW]: a hub table filled
+constexpr int HC_CTR_WORDS = 16;
 
 __device__ __forceinline__ void wave_sync()
 {
@@ -136,14 +137,14 @@ __device__ __forceinline__ SegRows seg_rows(int32_t c, const int32_t *__restrict
 // ---------------------------------------------------------------- select
 template <typename OffT>
 __global__ __launch_bounds__(256) void k_hc_select(const int64_t *__restrict__ ncp, const OffT *__restrict__ segoff,
-                                                   SegChunk *chunks, int32_t *wave_items, int32_t *block_items,
-                                                   int32_t *blockl_items,
+                                                   SegChunk *chunks, int32_t *wave_items, int32_t *wavel_items,
+                                                   int32_t *block_items, int32_t *blockl_items,
                                                    HubPart *parts, int32_t *hubs, unsigned *ctr)
 {
   __shared__ unsigned s_w[HC_N_COUNTERS][4], s_base[HC_N_COUNTERS];
   const int64_t nc = *ncp;
   const int64_t c0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * HC_SELECT;
-  unsigned cnt[HC_N_COUNTERS] = {0, 0, 0, 0, 0, 0};
+  unsigned cnt[HC_N_COUNTERS] = {};
   // the HC_SELECT + 1 offsets load at once (clamped to nc)
   int64_t so[HC_SELECT + 1];
 #pragma unroll
@@ -155,7 +156,8 @@ __global__ __launch_bounds__(256) void k_hc_select(const int64_t *__restrict__ n
     const int64_t len = so[t + 1] - so[t];
     if (len <= REG_SEG) continue;
     cnt[HC_N_CHUNK] += (unsigned)((len + HC_CHUNK - 1) / HC_CHUNK);
-    if (len <= HC_WAVE_MAX) ++cnt[HC_N_WAVE];
+    if (len <= HC_WAVE_SPLIT) ++cnt[HC_N_WAVE];
+    else if (len <= HC_WAVE_MAX) ++cnt[HC_N_WAVEL];
     else if (len <= HC_BLOCK_SPLIT) ++cnt[HC_N_BLOCK];
     else if (len <= HC_BLOCK_MAX) ++cnt[HC_N_BLOCKL];
     else {
@@ -194,7 +196,8 @@ __global__ __launch_bounds__(256) void k_hc_select(const int64_t *__restrict__ n
     for (int k = 0; k < nch; ++k) chunks[first + k] = SegChunk{(int32_t)c, (int32_t)first, nch, k};
     at[HC_N_CHUNK] += nch;
     if (len <= HC_WAVE_MAX) {
-      wave_items[at[HC_N_WAVE]++] = (int32_t)c;
+      if (len <= HC_WAVE_SPLIT) wave_items[at[HC_N_WAVE]++] = (int32_t)c;
+      else wavel_items[at[HC_N_WAVEL]++] = (int32_t)c;
     } else if (len <= HC_BLOCK_SPLIT) {
       block_items[at[HC_N_BLOCK]++] = (int32_t)c;
     } else if (len <= HC_BLOCK_MAX) {
@@ -259,10 +262,10 @@ __global__ __launch_bounds__(256) void k_hc_gather(const SegChunk *__restrict__ 
 }
 
 // ------------------------------------------------------ dedup: one wave
-// Segments of REG_SEG+1 .. HC_WAVE_MAX entries, one per wave, a table of
-// HC_WAVE_SLOTS slots per wave.
-template <typename OffT>
-__global__ __launch_bounds__(256) void k_hc_dedup_wave(const int32_t *__restrict__ items, const unsigned *ctr,
+// Segments of up to 2^(LG-1) entries (above REG_SEG), one per wave, a table
+// of up to 2^LG slots per wave; *count items.
+template <typename OffT, int LG>
+__global__ __launch_bounds__(256) void k_hc_dedup_wave(const int32_t *__restrict__ items, const unsigned *count,
                                                        const OffT *__restrict__ segoff,
                                                        const int32_t *__restrict__ rep,
                                                        const int32_t *__restrict__ match,
@@ -272,11 +275,12 @@ __global__ __launch_bounds__(256) void k_hc_dedup_wave(const int32_t *__restrict
                                                        const int32_t *__restrict__ cmap, int32_t *__restrict__ tc,
                                                        int32_t *__restrict__ tw, int32_t *__restrict__ cdeg)
 {
-  constexpr int PER = HC_WAVE_MAX / CM_WAVE;
-  __shared__ int32_t s_key[4][HC_WAVE_SLOTS], s_pos[4][HC_WAVE_SLOTS], s_sum[4][HC_WAVE_SLOTS];
+  constexpr int SLOTS = 1 << LG;
+  constexpr int PER = SLOTS / 2 / CM_WAVE;
+  __shared__ int32_t s_key[4][SLOTS], s_pos[4][SLOTS], s_sum[4][SLOTS];
   const int w = threadIdx.x >> 6, l = lane();
   int32_t *tk = s_key[w], *tp = s_pos[w], *ts = s_sum[w];
-  const unsigned n = ctr[HC_N_WAVE];
+  const unsigned n = *count;
   const int64_t nw = (int64_t)gridDim.x * 4;
   for (int64_t q = (int64_t)blockIdx.x * 4 + w; q < n; q += nw) {
     const int32_t c = items[q];
@@ -675,21 +679,30 @@ void hash_prepare_impl(cm_ctx *ctx, const Level &L, const int32_t *match, const 
   };
   hc.chunks = al(sizeof(SegChunk) * nchunk);
   int32_t *wave_items = (int32_t *)al(4 * nseg), *block_items = (int32_t *)al(4 * nblk);
+  const int64_t nwavel = std::min<int64_t>(nc, m / (HC_WAVE_SPLIT + 1)) + 1;
+  int32_t *wavel_items = (int32_t *)al(4 * nwavel);
   int32_t *blockl_items = (int32_t *)al(4 * nblkl);
   HubPart *parts = (HubPart *)al(sizeof(HubPart) * npart);
   int32_t *hubs = (int32_t *)al(4 * nhub);
-  hc.ctr = (unsigned *)al(4 * 8);
+  hc.ctr = (unsigned *)al(4 * HC_CTR_WORDS);
   hc.tc = (int32_t *)al(4 * m);
   hc.tw = (int32_t *)al(4 * m);
-  fill_u32(ctx, hc.ctr, 0u, 8);
+  fill_u32(ctx, hc.ctr, 0u, HC_CTR_WORDS);
   const SegChunk *chunks = (const SegChunk *)hc.chunks;
   launch(ctx, "hc_select", 0.0, k_hc_select<OffT>, dim3(nblocks(nc, 256 * HC_SELECT)), dim3(256), 0, d_nc, segoff,
-         (SegChunk *)hc.chunks, wave_items, block_items, blockl_items, parts, hubs, hc.ctr);
+         (SegChunk *)hc.chunks, wave_items, wavel_items, block_items, blockl_items, parts, hubs, hc.ctr);
   const unsigned gchunk = (unsigned)std::min<int64_t>(4096, (nchunk + 3) / 4);
   hc.gchunk = gchunk;
-  launch(ctx, "hc_dedup_wave", 0.0, k_hc_dedup_wave<OffT>, dim3((unsigned)std::min<int64_t>(4096, (nseg + 3) / 4)),
-         dim3(256), 0, (const int32_t *)wave_items, (const unsigned *)hc.ctr, segoff, rep, match, xadj, L.adjncy,
-         L.adjwgt, cmap, hc.tc, hc.tw, cdeg);
+  static_assert(2 * HC_WAVE_SPLIT == 1 << 7 && 2 * HC_WAVE_MAX == 1 << 9, "wave table sizes");
+  launch(ctx, "hc_dedup_wave", 0.0, k_hc_dedup_wave<OffT, 7>,
+         dim3((unsigned)std::min<int64_t>(4096, (nseg + 3) / 4)), dim3(256), 0, (const int32_t *)wave_items,
+         (const unsigned *)hc.ctr + HC_N_WAVE, segoff, rep, match, xadj, L.adjncy, L.adjwgt, cmap, hc.tc, hc.tw,
+         cdeg);
+  if (seg_max > HC_WAVE_SPLIT)
+    launch(ctx, "hc_dedup_wave_l", 0.0, k_hc_dedup_wave<OffT, 9>,
+           dim3((unsigned)std::min<int64_t>(4096, (nwavel + 3) / 4)), dim3(256), 0, (const int32_t *)wavel_items,
+           (const unsigned *)hc.ctr + HC_N_WAVEL, segoff, rep, match, xadj, L.adjncy, L.adjwgt, cmap, hc.tc, hc.tw,
+           cdeg);
   static_assert(2 * HC_BLOCK_SPLIT == 1 << 11 && 2 * HC_BLOCK_MAX == 1 << 12, "block table sizes");
   if (seg_max > HC_WAVE_MAX)
     launch(ctx, "hc_dedup_block", 0.0, k_hc_dedup_block<OffT, 11>, dim3((unsigned)std::min<int64_t>(4096, nblk)),
@@ -710,7 +723,7 @@ void hash_prepare_impl(cm_ctx *ctx, const Level &L, const int32_t *match, const 
     int32_t *hself = (int32_t *)al(4 * nchunk), *htot = (int32_t *)al(4 * nchunk);
     launch(ctx, "hc_dedup_hub", 0.0, k_hc_dedup_hub<OffT>, dim3((unsigned)std::min<int64_t>(1024, npart)),
            dim3(HC_HUB_THREADS), 0, (const HubPart *)parts, (const unsigned *)hc.ctr, segoff, (const int32_t *)cval,
-           (const int32_t *)wv, head, hw, hc.ctr + 6);
+           (const int32_t *)wv, head, hw, hc.ctr + HC_OVERFLOW);
     launch(ctx, "hc_hub_count", 0.0, k_hc_hub_count<OffT>, dim3(gchunk), dim3(256), 0, chunks,
            (const unsigned *)hc.ctr, segoff, (const uint8_t *)head, (const int32_t *)cval, ccnt, cself);
     launch(ctx, "hc_hub_scan", 0.0, k_hc_hub_scan<OffT>, dim3((unsigned)std::min<int64_t>(1024, (nhub + 3) / 4)),
@@ -730,7 +743,8 @@ void hash_emit_impl(cm_ctx *ctx, const OffT *segoff, const int32_t *cdeg, const 
          (const unsigned *)hc.ctr, segoff, cdeg, cxadj, (const int32_t *)hc.tc, (const int32_t *)hc.tw, cadjncy,
          cadjwgt);
   // hub-table overflow flag -> h_small[6] with the caller's next read
-  CM_HIP(hipMemcpyAsync(ctx->h_small + 6, hc.ctr + 6, sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
+  CM_HIP(hipMemcpyAsync(ctx->h_small + 6, hc.ctr + HC_OVERFLOW, sizeof(unsigned), hipMemcpyDeviceToHost,
+                        ctx->stream));
 }
 
 template void hash_prepare_impl<int32_t>(cm_ctx *, const Level &, const int32_t *, const int32_t *, int64_t,

@@ -17,7 +17,7 @@ constexpr int REG_SEG = 16;  // coarse segments up to this merge in registers
 struct HashContract {
   std::vector<void *> blocks;  // pool blocks to release
   void *chunks = nullptr;
-  unsigned *ctr = nullptr;     // item counters; ctr[6] = hub-table overflow
+  unsigned *ctr = nullptr;     // item counters and the hub-table overflow flag
   int32_t *tc = nullptr, *tw = nullptr;  // compacted rows, at the segment offsets
   unsigned gchunk = 1;
 };
