@@ -47,6 +47,7 @@ constexpr int HC_HUB_SLOTS = 1 << HC_HUB_LG;  // >= 2 * HC_PART
 constexpr int HC_HUB_THREADS = 1024;
 constexpr int HC_HUB_LIST = 6144;  // entries of the partition listed for the second pass
 constexpr int HC_SELECT = 16;       // segments per thread in the select pass
+constexpr int EMIT_BATCH = 8;       // entries per lane per step of the chunk copy passes
 
 struct SegChunk {
   int32_t c, first, nch, k;  // chunk k of the nch chunks of segment c, from chunk `first`
@@ -540,13 +541,23 @@ __global__ __launch_bounds__(256) void k_hc_hub_count(const SegChunk *__restrict
     if (len <= HC_BLOCK_MAX) continue;
     const int64_t i0 = (int64_t)ch.k * HC_CHUNK, i1 = len < i0 + HC_CHUNK ? len : i0 + HC_CHUNK;
     int cnt = 0, self = -1;
-    for (int64_t ib = i0; ib < i1; ib += CM_WAVE) {
-      const int64_t i = ib + l;
-      const bool h = i < i1 && head[s0 + i];
-      const uint64_t mk = __ballot(h);
-      const uint64_t sm = __ballot(h && cval[s0 + i] == ch.c);
-      if (sm) self = cnt + (int)__popcll(mk & ((1ull << (__ffsll((unsigned long long)sm) - 1)) - 1));
-      cnt += (int)__popcll(mk);
+    for (int64_t ib = i0; ib < i1; ib += EMIT_BATCH * CM_WAVE) {
+      bool hh[EMIT_BATCH];
+      int32_t cv[EMIT_BATCH];
+#pragma unroll
+      for (int t = 0; t < EMIT_BATCH; ++t) {  // clamped loads, issued together
+        const int64_t i = ib + (int64_t)t * CM_WAVE + l, ii = i < i1 ? i : i1 - 1;
+        const uint8_t hb = head[s0 + ii];
+        hh[t] = i < i1 && hb;
+        cv[t] = cval[s0 + ii];
+      }
+#pragma unroll
+      for (int t = 0; t < EMIT_BATCH; ++t) {
+        const uint64_t mk = __ballot(hh[t]);
+        const uint64_t sm = __ballot(hh[t] && cv[t] == ch.c);
+        if (sm) self = cnt + (int)__popcll(mk & ((1ull << (__ffsll((unsigned long long)sm) - 1)) - 1));
+        cnt += (int)__popcll(mk);
+      }
     }
     if (l == 0) {
       ccnt[q] = cnt;
@@ -615,20 +626,28 @@ __global__ __launch_bounds__(256) void k_hc_hub_write(const SegChunk *__restrict
     const int self = hself[ch.first], tot = htot[ch.first];
     const int64_t i0 = (int64_t)ch.k * HC_CHUNK, i1 = len < i0 + HC_CHUNK ? len : i0 + HC_CHUNK;
     int r0 = coff[q];
-    for (int64_t ib = i0; ib < i1; ib += CM_WAVE) {
-      const int64_t i = ib + l;
-      const bool h = i < i1 && head[s0 + i];
-      const uint64_t mk = __ballot(h);
-      if (h) {
-        const int32_t cu = cval[s0 + i];
-        if (cu != ch.c) {
+    for (int64_t ib = i0; ib < i1; ib += EMIT_BATCH * CM_WAVE) {
+      bool hh[EMIT_BATCH];
+      int32_t cv[EMIT_BATCH], hv[EMIT_BATCH];
+#pragma unroll
+      for (int t = 0; t < EMIT_BATCH; ++t) {  // clamped loads, issued together
+        const int64_t i = ib + (int64_t)t * CM_WAVE + l, ii = i < i1 ? i : i1 - 1;
+        const uint8_t hb = head[s0 + ii];
+        hh[t] = i < i1 && hb;
+        cv[t] = cval[s0 + ii];
+        hv[t] = hw[s0 + ii];
+      }
+#pragma unroll
+      for (int t = 0; t < EMIT_BATCH; ++t) {
+        const uint64_t mk = __ballot(hh[t]);
+        if (hh[t] && cv[t] != ch.c) {
           int r = r0 + mask_rank(mk);
           if (self >= 0 && r == tot - 1) r = self;
-          tc[s0 + r] = cu;
-          tw[s0 + r] = hw[s0 + i];
+          tc[s0 + r] = cv[t];
+          tw[s0 + r] = hv[t];
         }
+        r0 += (int)__popcll(mk);
       }
-      r0 += (int)__popcll(mk);
     }
   }
 }
@@ -649,9 +668,23 @@ __global__ __launch_bounds__(256) void k_hc_emit(const SegChunk *__restrict__ ch
     const int64_t s0 = (int64_t)segoff[ch.c], o0 = (int64_t)cxadj[ch.c];
     const int64_t d = cdeg[ch.c];
     const int64_t i0 = (int64_t)ch.k * HC_CHUNK, i1 = d < i0 + HC_CHUNK ? d : i0 + HC_CHUNK;
-    for (int64_t i = i0 + l; i < i1; i += CM_WAVE) {
-      cadjncy[o0 + i] = tc[s0 + i];
-      cadjwgt[o0 + i] = tw[s0 + i];
+    // EMIT_BATCH entries per lane per step: all loads issue before the stores
+    for (int64_t ib = i0 + l; ib < i1; ib += EMIT_BATCH * CM_WAVE) {
+      int32_t a[EMIT_BATCH], w[EMIT_BATCH];
+#pragma unroll
+      for (int t = 0; t < EMIT_BATCH; ++t) {
+        const int64_t i = ib + (int64_t)t * CM_WAVE, ii = i < i1 ? i : i1 - 1;
+        a[t] = tc[s0 + ii];
+        w[t] = tw[s0 + ii];
+      }
+#pragma unroll
+      for (int t = 0; t < EMIT_BATCH; ++t) {
+        const int64_t i = ib + (int64_t)t * CM_WAVE;
+        if (i < i1) {
+          cadjncy[o0 + i] = a[t];
+          cadjwgt[o0 + i] = w[t];
+        }
+      }
     }
   }
 }
