@@ -28,9 +28,12 @@ static_assert(REG_SEG == 16, "FIRST_SORTED_BIN must be the first bin above REG_S
 // NUM_PER consecutive vertices per thread, tiles of 1024 * NUM_PER vertices
 // (fewer tiles = a shorter look-back chain).  A vertex still unmatched
 // (match = -1, the matching left unfinished) becomes its own coarse vertex
-// and its match entry is completed here.  Each tile also stores its largest
-// coarse vertex weight and segment length in tmax[2 * tile .. 2 * tile + 1]
-// (k_number_fill reduces them).
+// and its match entry is completed here; a representative also gives its
+// partner the coarse id.  Four look-back chains: coarse count and segment
+// length (sums, walked by wave 0), largest coarse vertex weight and longest
+// segment (maxima, walked by wave 1); the last tile stores the totals:
+// totals[0] = nc, totals[3] = largest coarse vertex weight, totals[4] =
+// longest segment (a bound on the longest coarse row).
 constexpr int NUM_PER = 2;
 
 template <typename OffT>
@@ -39,8 +42,8 @@ __global__ __launch_bounds__(1024) void k_number(int64_t n, const OffT *__restri
                                                  const int32_t *__restrict__ vwgt,
                                                  int32_t *__restrict__ cmap, int32_t *__restrict__ rep,
                                                  OffT *__restrict__ segoff, int32_t *__restrict__ cvwgt,
-                                                 uint64_t *st_c, uint64_t *st_l, uint64_t *tmax, unsigned *ctr,
-                                                 int64_t *totals, uint32_t epoch)
+                                                 uint64_t *st_c, uint64_t *st_l, uint64_t *st_m, uint64_t *st_n,
+                                                 unsigned *ctr, int64_t *totals, uint32_t epoch)
 {
   __shared__ uint64_t sh_c[16], sh_l[16], sh_m[16], sh_n[16];
   const int tile = tile_ticket(ctr);
@@ -90,15 +93,18 @@ __global__ __launch_bounds__(1024) void k_number(int64_t n, const OffT *__restri
     const uint64_t ta = __shfl(ia, 15, CM_WAVE), tb = __shfl(ib, 15, CM_WAVE);
     const uint64_t ea = lookback(st_c, tile, ta, epoch);
     const uint64_t eb = lookback(st_l, tile, tb, epoch);
-    const uint64_t bm = wave_reduce<LbMax>(l < 16 ? sh_m[l] : 0);
-    const uint64_t bn = wave_reduce<LbMax>(l < 16 ? sh_n[l] : 0);
     if (l < 16) {
       sh_c[l] = ea + ia - a;
       sh_l[l] = eb + ib - b;
     }
-    if (l == 0) {
-      tmax[2 * tile] = bm;
-      tmax[2 * tile + 1] = bn;
+  } else if (w == 1) {
+    const uint64_t bm = wave_reduce<LbMax>(l < 16 ? sh_m[l] : 0);
+    const uint64_t bn = wave_reduce<LbMax>(l < 16 ? sh_n[l] : 0);
+    const uint64_t em = lookback<LbMax>(st_m, tile, bm, epoch);
+    const uint64_t en = lookback<LbMax>(st_n, tile, bn, epoch);
+    if (l == 0 && (int64_t)(tile + 1) * 1024 * NUM_PER >= n) {  // the last tile
+      totals[3] = (int64_t)(em > bm ? em : bm);
+      totals[4] = (int64_t)(en > bn ? en : bn);
     }
   }
   __syncthreads();
@@ -110,10 +116,10 @@ __global__ __launch_bounds__(1024) void k_number(int64_t n, const OffT *__restri
       if (fl[t]) {
         const int32_t u = uu[t];
         cmap[i] = (int32_t)c;
+        if (u != (int32_t)i) cmap[u] = (int32_t)c;  // the partner (u > i)
         rep[c] = (int32_t)i;
         segoff[c] = (OffT)off;
         cvwgt[c] = cw[t];
-        (void)u;
       }
       if (i == n - 1) {
         totals[0] = (int64_t)(c + fl[t]);
@@ -122,46 +128,6 @@ __global__ __launch_bounds__(1024) void k_number(int64_t n, const OffT *__restri
     }
     c += fl[t];
     off += ln[t];
-  }
-}
-
-// Partners take the coarse id of their representative.  Block 0 also
-// reduces the per-tile maxima of k_number: totals[3] = largest coarse vertex
-// weight, totals[4] = longest segment (a bound on the longest coarse row).
-__global__ __launch_bounds__(256) void k_number_fill(int64_t n, const int32_t *__restrict__ match,
-                                                     int32_t *__restrict__ cmap, const uint64_t *__restrict__ tmax,
-                                                     int64_t ntiles, int64_t *totals)
-{
-  if (blockIdx.x == 0) {
-    __shared__ uint64_t s_m[4], s_n[4];
-    uint64_t m0 = 0, n0 = 0;
-    for (int64_t t = threadIdx.x; t < ntiles; t += 256) {
-      const uint64_t a = tmax[2 * t], b = tmax[2 * t + 1];
-      m0 = a > m0 ? a : m0;
-      n0 = b > n0 ? b : n0;
-    }
-    m0 = wave_max<uint64_t>(m0);
-    n0 = wave_max<uint64_t>(n0);
-    if (lane() == 0) {
-      s_m[threadIdx.x >> 6] = m0;
-      s_n[threadIdx.x >> 6] = n0;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      for (int k = 1; k < 4; ++k) {
-        m0 = s_m[k] > s_m[0] ? s_m[k] : s_m[0];
-        s_m[0] = m0;
-        n0 = s_n[k] > s_n[0] ? s_n[k] : s_n[0];
-        s_n[0] = n0;
-      }
-      totals[3] = (int64_t)s_m[0];
-      totals[4] = (int64_t)s_n[0];
-    }
-  }
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) {
-    const int32_t u = match[i];
-    if (u < i) cmap[i] = cmap[u];
   }
 }
 
@@ -176,13 +142,11 @@ static int64_t number_impl(cm_ctx *ctx, const Level &L, const int32_t *match_in,
     return 0;
   }
   const int64_t tiles = (n + 1024 * NUM_PER - 1) / (1024 * NUM_PER);
-  const uint32_t ep = lookback_begin(ctx, 4 * tiles);  // 2 chains + tile maxima
+  const uint32_t ep = lookback_begin(ctx, 4 * tiles);  // 4 chains
   launch(ctx, "number_scan", (double)sizeof(OffT) * (n + 1) * 2 + 16.0 * n, k_number<OffT>,
          dim3((unsigned)tiles), dim3(1024), 0, n, (const OffT *)L.xadj, match_in, match, (const int32_t *)L.vwgt,
-         cmap, rep, segoff, cvwgt, ctx->d_status, ctx->d_status + tiles, ctx->d_status + 2 * tiles, ctx->d_ctr,
-         ctx->d_small, ep);
-  launch(ctx, "number_fill", 12.0 * n, k_number_fill, dim3(nblocks(n, 256)), dim3(256), 0, n, match,
-         cmap, (const uint64_t *)(ctx->d_status + 2 * tiles), tiles, ctx->d_small);
+         cmap, rep, segoff, cvwgt, ctx->d_status, ctx->d_status + tiles, ctx->d_status + 2 * tiles,
+         ctx->d_status + 3 * tiles, ctx->d_ctr, ctx->d_small, ep);
   if (!wait) return -1;
   CM_HIP(hipMemcpyAsync(ctx->h_small, ctx->d_small, sizeof(int64_t), hipMemcpyDeviceToHost,
                         ctx->stream));
