@@ -684,6 +684,7 @@ __device__ __forceinline__ int merge_fs(int32_t c, int32_t v, int32_t u, const O
 // its waves per SIMD, not by memory).
 constexpr int SMALL_TILE = 256;    // coarse vertices per tile of k_contract_small
 constexpr int SMALL_STAGE = 4096;  // output entries staged in LDS per window
+constexpr int COPY_BATCH = 8;      // staged entries per step of a tile-copied hashed row
 
 template <bool FS, typename OffT>
 __global__ __launch_bounds__(SMALL_TILE) void k_contract_small(const int64_t *__restrict__ ncp,
@@ -695,6 +696,8 @@ __global__ __launch_bounds__(SMALL_TILE) void k_contract_small(const int64_t *__
                                                         const int32_t *__restrict__ adjwgt,
                                                         const int32_t *__restrict__ cmap,
                                                         const int32_t *__restrict__ cdeg_large,
+                                                        const int32_t *__restrict__ tc,
+                                                        const int32_t *__restrict__ tw,
                                                         OffT *__restrict__ cxadj, int32_t *__restrict__ cadjncy,
                                                         int32_t *__restrict__ cadjwgt, uint64_t *st,
                                                         unsigned *ctr, int64_t *total, uint32_t epoch)
@@ -768,10 +771,12 @@ __global__ __launch_bounds__(SMALL_TILE) void k_contract_small(const int64_t *__
   }
   // Rows go out through LDS: each window of STAGE entries of the tile's
   // output range is filled by the rows that fall in it, then stored by
-  // consecutive threads.  Slots of longer rows (written later by their own
-  // pass) keep the sentinel and are skipped.  Windows cover the range up to
-  // the end of the last short row.
-  const uint64_t my_end = small ? ex + (uint64_t)deg : 0;
+  // consecutive threads.  Hashed rows of segments up to TILE_COPY_SEG
+  // entries are copied in from the staging area (tc, tw); slots of longer
+  // rows (written later by their own pass) keep the sentinel and are
+  // skipped.  Windows cover the range up to the end of the last row written.
+  const bool copied = tc != nullptr && c < nc && !small && len <= TILE_COPY_SEG;
+  const uint64_t my_end = (small || copied) ? ex + (uint64_t)deg : 0;
   uint64_t span = wave_max<uint64_t>(my_end);
   if (lane() == 0) sh[threadIdx.x >> 6] = span;
   __syncthreads();
@@ -802,6 +807,25 @@ __global__ __launch_bounds__(SMALL_TILE) void k_contract_small(const int64_t *__
             stage_w[pos] = (int32_t)(uint32_t)p[i];
           }
         }
+      }
+    }
+    if (copied && rel < wlen && rel + deg > 0) {
+      const int r32 = (int)rel;
+      const int i0 = r32 < 0 ? -r32 : 0, i1 = deg < wlen - r32 ? deg : wlen - r32;
+      for (int i = i0; i < i1; i += COPY_BATCH) {
+        int32_t a[COPY_BATCH], w[COPY_BATCH];
+#pragma unroll
+        for (int t = 0; t < COPY_BATCH; ++t) {  // clamped loads, issued together
+          const int j = i + t < i1 ? i + t : i1 - 1;
+          a[t] = tc[s0 + j];
+          w[t] = tw[s0 + j];
+        }
+#pragma unroll
+        for (int t = 0; t < COPY_BATCH; ++t)
+          if (i + t < i1) {
+            stage_a[r32 + i + t] = a[t];
+            stage_w[r32 + i + t] = w[t];
+          }
       }
     }
     __syncthreads();
@@ -860,8 +884,9 @@ static int64_t contract_impl(cm_ctx *ctx, const Level &L, const int32_t *match, 
       const double small_bytes = 8.0 * m + 12.0 * n + (double)sizeof(OffT) * (n + 1) + 8.0 * ncb +
                                  (double)sizeof(OffT) * (ncb + 1);
       launch(ctx, "contract_small", small_bytes, k_contract_small<true, OffT>, dim3((unsigned)tiles), dim3(SMALL_TILE),
-             0, d_nc, segoff, rep, match, xadj, L.adjncy, L.adjwgt, cmap, (const int32_t *)cdeg, cxadj, cadjncy,
-             cadjwgt, ctx->d_status, ctx->d_ctr, ctx->d_small + 1, ep);
+             0, d_nc, segoff, rep, match, xadj, L.adjncy, L.adjwgt, cmap, (const int32_t *)cdeg,
+             (const int32_t *)hc.tc, (const int32_t *)hc.tw, cxadj, cadjncy, cadjwgt, ctx->d_status, ctx->d_ctr,
+             ctx->d_small + 1, ep);
       ctx->h_small[6] = 0;
       if (hashed) hash_emit_impl<OffT>(ctx, segoff, cdeg, cxadj, cadjncy, cadjwgt, hc);
       CM_HIP(hipMemcpyAsync(ctx->h_small, ctx->d_small, 5 * sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
@@ -940,11 +965,13 @@ static int64_t contract_impl(cm_ctx *ctx, const Level &L, const int32_t *match, 
                              (double)sizeof(OffT) * (nc + 1);
   if (fs)
     launch(ctx, "contract_small", small_bytes, k_contract_small<true, OffT>, dim3((unsigned)tiles), dim3(SMALL_TILE),
-           0, (const int64_t *)(ctx->d_small + 7), segoff, rep, match, xadj, L.adjncy, L.adjwgt, cmap, (const int32_t *)cdeg, cxadj, cadjncy,
-           cadjwgt, ctx->d_status, ctx->d_ctr, ctx->d_small + 1, ep);
+           0, (const int64_t *)(ctx->d_small + 7), segoff, rep, match, xadj, L.adjncy, L.adjwgt, cmap,
+           (const int32_t *)cdeg, (const int32_t *)nullptr, (const int32_t *)nullptr, cxadj, cadjncy, cadjwgt,
+           ctx->d_status, ctx->d_ctr, ctx->d_small + 1, ep);
   else
     launch(ctx, "contract_small", small_bytes, k_contract_small<false, OffT>, dim3((unsigned)tiles),
-           dim3(SMALL_TILE), 0, (const int64_t *)(ctx->d_small + 7), segoff, rep, match, xadj, L.adjncy, L.adjwgt, cmap, (const int32_t *)cdeg, cxadj,
+           dim3(SMALL_TILE), 0, (const int64_t *)(ctx->d_small + 7), segoff, rep, match, xadj, L.adjncy,
+           L.adjwgt, cmap, (const int32_t *)cdeg, (const int32_t *)nullptr, (const int32_t *)nullptr, cxadj,
            cadjncy, cadjwgt, ctx->d_status, ctx->d_ctr, ctx->d_small + 1, ep);
   if (sorted_path) {
 #define CM_EMIT(FSV)                                                                                      \

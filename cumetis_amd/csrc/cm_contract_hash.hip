@@ -666,6 +666,7 @@ __global__ __launch_bounds__(256) void k_hc_emit(const SegChunk *__restrict__ ch
   for (int64_t q = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / CM_WAVE; q < nq; q += nw) {
     const SegChunk ch = chunks[q];
     const int64_t s0 = (int64_t)segoff[ch.c], o0 = (int64_t)cxadj[ch.c];
+    if ((int64_t)segoff[ch.c + 1] - s0 <= TILE_COPY_SEG) continue;  // copied by k_contract_small
     const int64_t d = cdeg[ch.c];
     const int64_t i0 = (int64_t)ch.k * HC_CHUNK, i1 = d < i0 + HC_CHUNK ? d : i0 + HC_CHUNK;
     // EMIT_BATCH entries per lane per step: all loads issue before the stores
@@ -695,6 +696,7 @@ void hash_prepare_impl(cm_ctx *ctx, const Level &L, const int32_t *match, const 
                        const int64_t *d_nc, const int32_t *rep, const OffT *segoff, int32_t *cdeg,
                        int64_t seg_max, HashContract &hc)
 {
+  hc.seg_max = seg_max;
   if (seg_max < 0) seg_max = INT64_MAX;
   const int64_t m = L.m;
   const OffT *xadj = (const OffT *)L.xadj;
@@ -772,12 +774,15 @@ template <typename OffT>
 void hash_emit_impl(cm_ctx *ctx, const OffT *segoff, const int32_t *cdeg, const OffT *cxadj, int32_t *cadjncy,
                     int32_t *cadjwgt, const HashContract &hc)
 {
-  launch(ctx, "hc_emit", 0.0, k_hc_emit<OffT>, dim3(hc.gchunk), dim3(256), 0, (const SegChunk *)hc.chunks,
+  if (hc.seg_max < 0 || hc.seg_max > TILE_COPY_SEG)
+    launch(ctx, "hc_emit", 0.0, k_hc_emit<OffT>, dim3(hc.gchunk), dim3(256), 0, (const SegChunk *)hc.chunks,
          (const unsigned *)hc.ctr, segoff, cdeg, cxadj, (const int32_t *)hc.tc, (const int32_t *)hc.tw, cadjncy,
          cadjwgt);
-  // hub-table overflow flag -> h_small[6] with the caller's next read
-  CM_HIP(hipMemcpyAsync(ctx->h_small + 6, hc.ctr + HC_OVERFLOW, sizeof(unsigned), hipMemcpyDeviceToHost,
-                        ctx->stream));
+  // hub-table overflow flag -> h_small[6] with the caller's next read (only
+  // hubs can overflow; the caller cleared h_small[6])
+  if (hc.seg_max < 0 || hc.seg_max > HC_BLOCK_MAX)
+    CM_HIP(hipMemcpyAsync(ctx->h_small + 6, hc.ctr + HC_OVERFLOW, sizeof(unsigned), hipMemcpyDeviceToHost,
+                          ctx->stream));
 }
 
 template void hash_prepare_impl<int32_t>(cm_ctx *, const Level &, const int32_t *, const int32_t *, int64_t,

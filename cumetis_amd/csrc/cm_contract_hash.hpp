@@ -12,6 +12,10 @@ namespace cm {
 struct Level;
 
 constexpr int REG_SEG = 16;  // coarse segments up to this merge in registers
+// Hashed segments up to this length have their staged rows copied to the
+// final place by k_contract_small (with the short rows of its tile); only
+// longer ones need the emit pass.
+constexpr int TILE_COPY_SEG = 64;
 
 // Device state between the two halves: everything the emit step reads.
 struct HashContract {
@@ -20,6 +24,7 @@ struct HashContract {
   unsigned *ctr = nullptr;     // item counters and the hub-table overflow flag
   int32_t *tc = nullptr, *tw = nullptr;  // compacted rows, at the segment offsets
   unsigned gchunk = 1;
+  int64_t seg_max = -1;  // bound on the segment length (-1 unknown)
 };
 
 // Rows of every segment above REG_SEG into the staging area, their lengths

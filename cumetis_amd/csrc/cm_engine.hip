@@ -416,7 +416,7 @@ int ref_threshold(int64_t n, int nparts)
 constexpr int64_t SPECULATE_EDGES = 1 << 24;
 
 static Level coarsen_level(cm_ctx *ctx, cm_hier *h, Level &F, int32_t maxvwgt, uint64_t seed,
-                           int max_rounds, int row_order)
+                           int max_rounds, int row_order, int batch)
 {
   const size_t ob = F.off64 ? 8 : 4;
   auto own = [&](size_t bytes) {
@@ -449,7 +449,7 @@ static Level coarsen_level(cm_ctx *ctx, cm_hier *h, Level &F, int32_t maxvwgt, u
     // synchronisation costs little, check the rounds first.
     const bool speculate = m <= SPECULATE_EDGES;
     if (n > 0) {
-      match_queue(ctx, mr, MATCH_BATCH);
+      match_queue(ctx, mr, batch);
       if (!speculate) sync(ctx);
     }
     for (int pass = 0;; ++pass) {
@@ -699,9 +699,15 @@ int cm_coarsen(cm_ctx *ctx, const cm_graph_view *g, const cm_coarsen_params *p, 
     const int T = p->threshold > 0 ? p->threshold : cm::ref_threshold(g->n, p->nparts);
     const int32_t maxvwgt =
         p->maxvwgt > 0 ? p->maxvwgt : (int32_t)(1.5 * (double)h->levels[0].tvwgt / T);
+    int batch = cm::MATCH_BATCH;
     for (;;) {
       cm::Level C = cm::coarsen_level(ctx, h, h->levels.back(), maxvwgt, p->seed, p->max_rounds,
-                                      p->row_order);
+                                      p->row_order, batch);
+      // the next level's first batch: launches for the rounds this level
+      // took plus one (launch r accepts round r-1, so r rounds need r + 1
+      // launches to show that they finished); a short batch only costs a
+      // resumption
+      batch = std::min(cm::MATCH_BATCH, std::max(5, h->levels.back().rounds + 2));
       const int64_t nf = h->levels.back().n;
       h->levels.push_back(C);
       const bool more = C.n > T && C.n < 0.75 * nf && C.m > C.n / 2;  // cu:1843-1845
