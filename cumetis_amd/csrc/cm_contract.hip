@@ -43,10 +43,11 @@ __global__ __launch_bounds__(1024) void k_number(int64_t n, const OffT *__restri
                                                  int32_t *__restrict__ cmap, int32_t *__restrict__ rep,
                                                  OffT *__restrict__ segoff, int32_t *__restrict__ cvwgt,
                                                  uint64_t *st_c, uint64_t *st_l, uint64_t *st_m, uint64_t *st_n,
-                                                 unsigned *ctr, int64_t *totals, uint32_t epoch)
+                                                 unsigned *ctr, int64_t *totals, unsigned *zero, uint32_t epoch)
 {
   __shared__ uint64_t sh_c[16], sh_l[16], sh_m[16], sh_n[16];
   const int tile = tile_ticket(ctr);
+  if (tile == 0 && threadIdx.x < 16) zero[threadIdx.x] = 0u;  // the hash contraction's counters
   const int64_t i0 = ((int64_t)tile * 1024 + threadIdx.x) * NUM_PER;
   uint64_t fl[NUM_PER], ln[NUM_PER], flag = 0, len = 0, mw = 0, ml = 0;
   int32_t uu[NUM_PER], cw[NUM_PER], u0[NUM_PER];
@@ -146,7 +147,8 @@ static int64_t number_impl(cm_ctx *ctx, const Level &L, const int32_t *match_in,
   launch(ctx, "number_scan", (double)sizeof(OffT) * (n + 1) * 2 + 16.0 * n, k_number<OffT>,
          dim3((unsigned)tiles), dim3(1024), 0, n, (const OffT *)L.xadj, match_in, match, (const int32_t *)L.vwgt,
          cmap, rep, segoff, cvwgt, ctx->d_status, ctx->d_status + tiles, ctx->d_status + 2 * tiles,
-         ctx->d_status + 3 * tiles, ctx->d_ctr, ctx->d_small, ep);
+         ctx->d_status + 3 * tiles, ctx->d_ctr, ctx->d_small, ctx->d_ctr + HASH_CTR_SLOT, ep);
+  ctx->hash_ctr_zero = true;
   if (!wait) return -1;
   CM_HIP(hipMemcpyAsync(ctx->h_small, ctx->d_small, sizeof(int64_t), hipMemcpyDeviceToHost,
                         ctx->stream));

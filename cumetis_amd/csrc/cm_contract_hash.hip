@@ -719,10 +719,12 @@ void hash_prepare_impl(cm_ctx *ctx, const Level &L, const int32_t *match, const 
   int32_t *blockl_items = (int32_t *)al(4 * nblkl);
   HubPart *parts = (HubPart *)al(sizeof(HubPart) * npart);
   int32_t *hubs = (int32_t *)al(4 * nhub);
-  hc.ctr = (unsigned *)al(4 * HC_CTR_WORDS);
+  static_assert(HASH_CTR_SLOT + HC_CTR_WORDS <= CTR_SLOTS, "hash counters");
+  hc.ctr = ctx->d_ctr + HASH_CTR_SLOT;
   hc.tc = (int32_t *)al(4 * m);
   hc.tw = (int32_t *)al(4 * m);
-  fill_u32(ctx, hc.ctr, 0u, HC_CTR_WORDS);
+  if (!ctx->hash_ctr_zero) fill_u32(ctx, hc.ctr, 0u, HC_CTR_WORDS);
+  ctx->hash_ctr_zero = false;
   const SegChunk *chunks = (const SegChunk *)hc.chunks;
   launch(ctx, "hc_select", 0.0, k_hc_select<OffT>, dim3(nblocks(nc, 256 * HC_SELECT)), dim3(256), 0, d_nc, segoff,
          (SegChunk *)hc.chunks, wave_items, wavel_items, block_items, blockl_items, parts, hubs, hc.ctr);

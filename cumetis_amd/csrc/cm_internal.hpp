@@ -123,6 +123,9 @@ struct cm_ctx {
   size_t status_cap = 0;          // in words
   uint32_t epoch = 0;             // look-back launch epoch (1 .. 2^20-1)
   int64_t last_rounds = 0;
+  // the hash-contraction item counters (d_ctr + HASH_CTR_SLOT) are zero at
+  // this point of the stream: the numbering kernel clears them on its way
+  bool hash_ctr_zero = false;
 
   hipEvent_t ev_get();
   void collect_timing();
@@ -131,6 +134,7 @@ struct cm_ctx {
 namespace cm {
 
 constexpr int CTR_SLOTS = 64;
+constexpr int HASH_CTR_SLOT = 32;  // 16 counters of the hash contraction
 constexpr int SMALL_SLOTS = 4096;
 
 void ensure_status(cm_ctx *ctx, size_t words);
