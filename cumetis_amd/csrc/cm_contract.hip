@@ -43,11 +43,10 @@ __global__ __launch_bounds__(1024) void k_number(int64_t n, const OffT *__restri
                                                  int32_t *__restrict__ cmap, int32_t *__restrict__ rep,
                                                  OffT *__restrict__ segoff, int32_t *__restrict__ cvwgt,
                                                  uint64_t *st_c, uint64_t *st_l, uint64_t *st_m, uint64_t *st_n,
-                                                 unsigned *ctr, int64_t *totals, unsigned *zero, uint32_t epoch)
+                                                 unsigned *ctr, int64_t *totals, HcLists sel, uint32_t epoch)
 {
   __shared__ uint64_t sh_c[16], sh_l[16], sh_m[16], sh_n[16];
   const int tile = tile_ticket(ctr);
-  if (tile == 0 && threadIdx.x < 16) zero[threadIdx.x] = 0u;  // the hash contraction's counters
   const int64_t i0 = ((int64_t)tile * 1024 + threadIdx.x) * NUM_PER;
   uint64_t fl[NUM_PER], ln[NUM_PER], flag = 0, len = 0, mw = 0, ml = 0;
   int32_t uu[NUM_PER], cw[NUM_PER], u0[NUM_PER];
@@ -110,9 +109,11 @@ __global__ __launch_bounds__(1024) void k_number(int64_t n, const OffT *__restri
   }
   __syncthreads();
   uint64_t c = sh_c[w] + ic - flag, off = sh_l[w] + il - len;
+  int32_t cid[NUM_PER];
 #pragma unroll
   for (int t = 0; t < NUM_PER; ++t) {
     const int64_t i = i0 + t;
+    cid[t] = (int32_t)c;
     if (i < n) {
       if (fl[t]) {
         const int32_t u = uu[t];
@@ -130,11 +131,23 @@ __global__ __launch_bounds__(1024) void k_number(int64_t n, const OffT *__restri
     c += fl[t];
     off += ln[t];
   }
+  // the hash contraction's select step: segments above REG_SEG into the
+  // item lists (all threads take part in the block-wide reservation)
+  if (sel.ctr) {
+    unsigned cnt[HC_N_COUNTERS] = {}, at[HC_N_COUNTERS];
+#pragma unroll
+    for (int t = 0; t < NUM_PER; ++t)
+      if (i0 + t < n && fl[t]) hc_count((int64_t)ln[t], cnt);
+    hc_reserve<16>(cnt, at, sel.ctr);
+#pragma unroll
+    for (int t = 0; t < NUM_PER; ++t)
+      if (i0 + t < n && fl[t]) hc_write(cid[t], (int64_t)ln[t], at, sel);
+  }
 }
 
 template <typename OffT>
 static int64_t number_impl(cm_ctx *ctx, const Level &L, const int32_t *match_in, int32_t *match, int32_t *cmap,
-                           int32_t *rep, OffT *segoff, int32_t *cvwgt, bool wait)
+                           int32_t *rep, OffT *segoff, int32_t *cvwgt, bool wait, HashContract *hc)
 {
   const int64_t n = L.n;
   if (n == 0) {
@@ -144,11 +157,17 @@ static int64_t number_impl(cm_ctx *ctx, const Level &L, const int32_t *match_in,
   }
   const int64_t tiles = (n + 1024 * NUM_PER - 1) / (1024 * NUM_PER);
   const uint32_t ep = lookback_begin(ctx, 4 * tiles);  // 4 chains
+  HcLists sel;
+  if (hc) {  // the select step into hc's lists, whose counters start at zero
+    sel = hc->lists;
+    if (!ctx->hash_ctr_zero) fill_u32(ctx, sel.ctr, 0u, HC_CTR_WORDS);
+    ctx->hash_ctr_zero = false;
+    hc->selected = true;
+  }
   launch(ctx, "number_scan", (double)sizeof(OffT) * (n + 1) * 2 + 16.0 * n, k_number<OffT>,
          dim3((unsigned)tiles), dim3(1024), 0, n, (const OffT *)L.xadj, match_in, match, (const int32_t *)L.vwgt,
          cmap, rep, segoff, cvwgt, ctx->d_status, ctx->d_status + tiles, ctx->d_status + 2 * tiles,
-         ctx->d_status + 3 * tiles, ctx->d_ctr, ctx->d_small, ctx->d_ctr + HASH_CTR_SLOT, ep);
-  ctx->hash_ctr_zero = true;
+         ctx->d_status + 3 * tiles, ctx->d_ctr, ctx->d_small, sel, ep);
   if (!wait) return -1;
   CM_HIP(hipMemcpyAsync(ctx->h_small, ctx->d_small, sizeof(int64_t), hipMemcpyDeviceToHost,
                         ctx->stream));
@@ -157,11 +176,12 @@ static int64_t number_impl(cm_ctx *ctx, const Level &L, const int32_t *match_in,
 }
 
 int64_t number(cm_ctx *ctx, const Level &L, int32_t *d_match, int32_t *d_cmap,
-               int32_t *d_rep, void *d_segoff, int32_t *d_cvwgt, bool wait, const int32_t *d_match_in)
+               int32_t *d_rep, void *d_segoff, int32_t *d_cvwgt, bool wait, const int32_t *d_match_in,
+               HashContract *hc)
 {
   const int32_t *in = d_match_in ? d_match_in : d_match;
-  return L.off64 ? number_impl(ctx, L, in, d_match, d_cmap, d_rep, (int64_t *)d_segoff, d_cvwgt, wait)
-                 : number_impl(ctx, L, in, d_match, d_cmap, d_rep, (int32_t *)d_segoff, d_cvwgt, wait);
+  return L.off64 ? number_impl(ctx, L, in, d_match, d_cmap, d_rep, (int64_t *)d_segoff, d_cvwgt, wait, hc)
+                 : number_impl(ctx, L, in, d_match, d_cmap, d_rep, (int32_t *)d_segoff, d_cvwgt, wait, hc);
 }
 
 // ------------------------------------------------- generic look-back scan
@@ -773,10 +793,11 @@ __global__ __launch_bounds__(SMALL_TILE) void k_contract_small(const int64_t *__
   }
   // Rows go out through LDS: each window of STAGE entries of the tile's
   // output range is filled by the rows that fall in it, then stored by
-  // consecutive threads.  Hashed rows of segments up to TILE_COPY_SEG
-  // entries are copied in from the staging area (tc, tw); slots of longer
-  // rows (written later by their own pass) keep the sentinel and are
-  // skipped.  Windows cover the range up to the end of the last row written.
+  // consecutive threads.  With tc/tw, the hashed rows (all of them then
+  // come from segments of at most TILE_COPY_SEG entries) are copied in from
+  // the staging area; otherwise their slots (written later by the emit
+  // pass) keep the sentinel and are skipped.  Windows cover the range up to
+  // the end of the last row written.
   const bool copied = tc != nullptr && c < nc && !small && len <= TILE_COPY_SEG;
   const uint64_t my_end = (small || copied) ? ex + (uint64_t)deg : 0;
   uint64_t span = wave_max<uint64_t>(my_end);
@@ -847,11 +868,21 @@ template <typename OffT>
 static int64_t contract_impl(cm_ctx *ctx, const Level &L, const int32_t *match, const int32_t *cmap,
                              int64_t nc, const int32_t *rep, const OffT *segoff, int row_order, OffT *cxadj,
                              int32_t *cadjncy, int32_t *cadjwgt, int64_t *maxdeg, int64_t *ncp,
-                             bool force_sorted = false)
+                             bool force_sorted = false, HashContract *pre = nullptr)
 {
   const int64_t n = L.n, m = L.m;
   const OffT *xadj = (const OffT *)L.xadj;
   const bool fs = row_order == CM_ROW_FIRST_SEEN;
+  // pre: the hash lists set up before the numbering (and filled by it); they
+  // are used by the first-seen hash path and released here on every path
+  struct PreGuard {
+    cm_ctx *ctx;
+    HashContract *hc;
+    ~PreGuard()
+    {
+      if (hc) hash_release(ctx, *hc);
+    }
+  } pre_guard{ctx, pre};
   if (nc < 0 && (n == 0 || !fs || force_sorted)) {  // the paths below need nc on the host
     CM_HIP(hipMemcpyAsync(ctx->h_small, ctx->d_small, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
     sync(ctx);
@@ -876,7 +907,8 @@ static int64_t contract_impl(cm_ctx *ctx, const Level &L, const int32_t *match, 
       d_nc = ctx->d_small + 7;
     }
     int32_t *cdeg = (int32_t *)ctx->pool.alloc(sizeof(int32_t) * ncb);
-    HashContract hc;
+    HashContract hc_local;
+    HashContract &hc = pre ? *pre : hc_local;
     try {
       if (hashed)
         hash_prepare_impl<OffT>(ctx, L, match, cmap, ncb, d_nc, rep, segoff, cdeg, L.maxdeg < 0 ? -1 : 2 * L.maxdeg,
@@ -885,10 +917,12 @@ static int64_t contract_impl(cm_ctx *ctx, const Level &L, const int32_t *match, 
       const uint32_t ep = lookback_begin(ctx, tiles);
       const double small_bytes = 8.0 * m + 12.0 * n + (double)sizeof(OffT) * (n + 1) + 8.0 * ncb +
                                  (double)sizeof(OffT) * (ncb + 1);
+      // hashed rows copied here when the emit pass is not needed
+      const bool tile_copy = hashed && hc.seg_max >= 0 && hc.seg_max <= TILE_COPY_SEG;
       launch(ctx, "contract_small", small_bytes, k_contract_small<true, OffT>, dim3((unsigned)tiles), dim3(SMALL_TILE),
              0, d_nc, segoff, rep, match, xadj, L.adjncy, L.adjwgt, cmap, (const int32_t *)cdeg,
-             (const int32_t *)hc.tc, (const int32_t *)hc.tw, cxadj, cadjncy, cadjwgt, ctx->d_status, ctx->d_ctr,
-             ctx->d_small + 1, ep);
+             tile_copy ? (const int32_t *)hc.tc : nullptr, tile_copy ? (const int32_t *)hc.tw : nullptr, cxadj,
+             cadjncy, cadjwgt, ctx->d_status, ctx->d_ctr, ctx->d_small + 1, ep);
       ctx->h_small[6] = 0;
       if (hashed) hash_emit_impl<OffT>(ctx, segoff, cdeg, cxadj, cadjncy, cadjwgt, hc);
       CM_HIP(hipMemcpyAsync(ctx->h_small, ctx->d_small, 5 * sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
@@ -1014,12 +1048,13 @@ static int64_t contract_impl(cm_ctx *ctx, const Level &L, const int32_t *match, 
 
 int64_t contract(cm_ctx *ctx, const Level &L, const int32_t *d_match, const int32_t *d_cmap,
                  int64_t nc, const int32_t *d_rep, const void *d_segoff, int row_order,
-                 void *d_cxadj, int32_t *d_cadjncy, int32_t *d_cadjwgt, int64_t *maxdeg, int64_t *ncp)
+                 void *d_cxadj, int32_t *d_cadjncy, int32_t *d_cadjwgt, int64_t *maxdeg, int64_t *ncp,
+                 HashContract *pre)
 {
   return L.off64 ? contract_impl(ctx, L, d_match, d_cmap, nc, d_rep, (const int64_t *)d_segoff, row_order,
-                                 (int64_t *)d_cxadj, d_cadjncy, d_cadjwgt, maxdeg, ncp)
+                                 (int64_t *)d_cxadj, d_cadjncy, d_cadjwgt, maxdeg, ncp, false, pre)
                  : contract_impl(ctx, L, d_match, d_cmap, nc, d_rep, (const int32_t *)d_segoff, row_order,
-                                 (int32_t *)d_cxadj, d_cadjncy, d_cadjwgt, maxdeg, ncp);
+                                 (int32_t *)d_cxadj, d_cadjncy, d_cadjwgt, maxdeg, ncp, false, pre);
 }
 
 // --------------------------------------------------------- (d) projection

@@ -32,34 +32,16 @@
 
 namespace cm {
 
-constexpr int HC_CHUNK = 2048;
-constexpr int HC_WAVE_MAX = 256;
-constexpr int HC_WAVE_SPLIT = 64;  // wave items up to this use a table of 2 * HC_WAVE_SPLIT slots
-constexpr int HC_BLOCK_MAX = 2048;
-constexpr int HC_BLOCK_SPLIT = 1024;  // block items up to this use a table of 2 * HC_BLOCK_SPLIT slots
 // Hub partitions: one 1024-thread block with a table of HC_HUB_SLOTS slots
 // (96 KB of LDS) per partition of about HC_PART entries.  Every partition
 // block reads the whole segment, so the segment is read len / HC_PART times:
 // the larger the table, the fewer the passes.
-constexpr int HC_PART = 4096;
 constexpr int HC_HUB_LG = 13;
 constexpr int HC_HUB_SLOTS = 1 << HC_HUB_LG;  // >= 2 * HC_PART
 constexpr int HC_HUB_THREADS = 1024;
 constexpr int HC_HUB_LIST = 6144;  // entries of the partition listed for the second pass
 constexpr int HC_SELECT = 16;       // segments per thread in the select pass
 constexpr int EMIT_BATCH = 8;       // entries per lane per step of the chunk copy passes
-
-struct SegChunk {
-  int32_t c, first, nch, k;  // chunk k of the nch chunks of segment c, from chunk `first`
-};
-struct HubPart {
-  int32_t c, p, P, pad;  // key partition p of P of hub segment c
-};
-
-// item counters
-enum { HC_N_CHUNK = 0, HC_N_WAVE, HC_N_BLOCK, HC_N_PART, HC_N_HUB, HC_N_BLOCKL, HC_N_WAVEL, HC_N_COUNTERS };
-constexpr int HC_OVERFLOW = 8;  // ctr[HC_OVERFLOW]: a hub table filled
-constexpr int HC_CTR_WORDS = 16;
 
 __device__ __forceinline__ void wave_sync()
 {
@@ -138,78 +120,22 @@ __device__ __forceinline__ SegRows seg_rows(int32_t c, const int32_t *__restrict
 // ---------------------------------------------------------------- select
 template <typename OffT>
 __global__ __launch_bounds__(256) void k_hc_select(const int64_t *__restrict__ ncp, const OffT *__restrict__ segoff,
-                                                   SegChunk *chunks, int32_t *wave_items, int32_t *wavel_items,
-                                                   int32_t *block_items, int32_t *blockl_items,
-                                                   HubPart *parts, int32_t *hubs, unsigned *ctr)
+                                                   HcLists hl)
 {
-  __shared__ unsigned s_w[HC_N_COUNTERS][4], s_base[HC_N_COUNTERS];
   const int64_t nc = *ncp;
   const int64_t c0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * HC_SELECT;
-  unsigned cnt[HC_N_COUNTERS] = {};
   // the HC_SELECT + 1 offsets load at once (clamped to nc)
   int64_t so[HC_SELECT + 1];
 #pragma unroll
   for (int t = 0; t <= HC_SELECT; ++t) so[t] = (int64_t)segoff[c0 + t < nc ? c0 + t : nc];
+  unsigned cnt[HC_N_COUNTERS] = {}, at[HC_N_COUNTERS];
 #pragma unroll
-  for (int t = 0; t < HC_SELECT; ++t) {
-    const int64_t c = c0 + t;
-    if (c >= nc) break;
-    const int64_t len = so[t + 1] - so[t];
-    if (len <= REG_SEG) continue;
-    cnt[HC_N_CHUNK] += (unsigned)((len + HC_CHUNK - 1) / HC_CHUNK);
-    if (len <= HC_WAVE_SPLIT) ++cnt[HC_N_WAVE];
-    else if (len <= HC_WAVE_MAX) ++cnt[HC_N_WAVEL];
-    else if (len <= HC_BLOCK_SPLIT) ++cnt[HC_N_BLOCK];
-    else if (len <= HC_BLOCK_MAX) ++cnt[HC_N_BLOCKL];
-    else {
-      ++cnt[HC_N_HUB];
-      cnt[HC_N_PART] += (unsigned)((len + HC_PART - 1) / HC_PART);
-    }
-  }
-  const int w = threadIdx.x >> 6;
-  unsigned inc[HC_N_COUNTERS];
+  for (int t = 0; t < HC_SELECT; ++t)
+    if (c0 + t < nc) hc_count(so[t + 1] - so[t], cnt);
+  hc_reserve<4>(cnt, at, hl.ctr);
 #pragma unroll
-  for (int i = 0; i < HC_N_COUNTERS; ++i) {
-    inc[i] = wave_incl_scan<unsigned>(cnt[i]);
-    if (lane() == CM_WAVE - 1) s_w[i][w] = inc[i];
-  }
-  __syncthreads();
-  if (threadIdx.x < HC_N_COUNTERS) {
-    const int i = threadIdx.x;
-    const unsigned tot = s_w[i][0] + s_w[i][1] + s_w[i][2] + s_w[i][3];
-    s_base[i] = tot ? atomicAdd(&ctr[i], tot) : 0u;
-  }
-  __syncthreads();
-  unsigned at[HC_N_COUNTERS];
-#pragma unroll
-  for (int i = 0; i < HC_N_COUNTERS; ++i) {
-    at[i] = s_base[i] + inc[i] - cnt[i];
-    for (int j = 0; j < w; ++j) at[i] += s_w[i][j];
-  }
-#pragma unroll
-  for (int t = 0; t < HC_SELECT; ++t) {
-    const int64_t c = c0 + t;
-    if (c >= nc) break;
-    const int64_t len = so[t + 1] - so[t];
-    if (len <= REG_SEG) continue;
-    const int nch = (int)((len + HC_CHUNK - 1) / HC_CHUNK);
-    const unsigned first = at[HC_N_CHUNK];
-    for (int k = 0; k < nch; ++k) chunks[first + k] = SegChunk{(int32_t)c, (int32_t)first, nch, k};
-    at[HC_N_CHUNK] += nch;
-    if (len <= HC_WAVE_MAX) {
-      if (len <= HC_WAVE_SPLIT) wave_items[at[HC_N_WAVE]++] = (int32_t)c;
-      else wavel_items[at[HC_N_WAVEL]++] = (int32_t)c;
-    } else if (len <= HC_BLOCK_SPLIT) {
-      block_items[at[HC_N_BLOCK]++] = (int32_t)c;
-    } else if (len <= HC_BLOCK_MAX) {
-      blockl_items[at[HC_N_BLOCKL]++] = (int32_t)c;
-    } else {
-      hubs[at[HC_N_HUB]++] = (int32_t)first;  // a hub is named by its first chunk
-      const int P = (int)((len + HC_PART - 1) / HC_PART);
-      for (int p = 0; p < P; ++p) parts[at[HC_N_PART] + p] = HubPart{(int32_t)c, p, P, 0};
-      at[HC_N_PART] += P;
-    }
-  }
+  for (int t = 0; t < HC_SELECT; ++t)
+    if (c0 + t < nc) hc_write((int32_t)(c0 + t), so[t + 1] - so[t], at, hl);
 }
 
 // ---------------------------------------------------------------- gather
@@ -666,7 +592,6 @@ __global__ __launch_bounds__(256) void k_hc_emit(const SegChunk *__restrict__ ch
   for (int64_t q = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / CM_WAVE; q < nq; q += nw) {
     const SegChunk ch = chunks[q];
     const int64_t s0 = (int64_t)segoff[ch.c], o0 = (int64_t)cxadj[ch.c];
-    if ((int64_t)segoff[ch.c + 1] - s0 <= TILE_COPY_SEG) continue;  // copied by k_contract_small
     const int64_t d = cdeg[ch.c];
     const int64_t i0 = (int64_t)ch.k * HC_CHUNK, i1 = d < i0 + HC_CHUNK ? d : i0 + HC_CHUNK;
     // EMIT_BATCH entries per lane per step: all loads issue before the stores
@@ -691,6 +616,47 @@ __global__ __launch_bounds__(256) void k_hc_emit(const SegChunk *__restrict__ ch
 }
 
 // ------------------------------------------------------------- host side
+// bounds of the item counts for nc coarse vertices (segments above REG_SEG
+// hold at most m entries)
+struct HcBounds {
+  int64_t nseg, nchunk, nwavel, nblk, nblkl, nhub, npart;
+  HcBounds(int64_t nc, int64_t m)
+  {
+    nseg = std::min<int64_t>(nc, m / (REG_SEG + 1)) + 1;
+    nchunk = m / HC_CHUNK + nseg;
+    nwavel = std::min<int64_t>(nc, m / (HC_WAVE_SPLIT + 1)) + 1;
+    nblk = std::min<int64_t>(nc, m / (HC_WAVE_MAX + 1)) + 1;
+    nblkl = std::min<int64_t>(nc, m / (HC_BLOCK_SPLIT + 1)) + 1;
+    nhub = m / (HC_BLOCK_MAX + 1) + 1;
+    npart = m / HC_PART + nhub;
+  }
+};
+
+bool hash_needed(const Level &L) { return L.maxdeg < 0 || 2 * L.maxdeg > REG_SEG; }
+
+void hash_alloc(cm_ctx *ctx, const Level &L, int64_t nc, HashContract &hc)
+{
+  const int64_t m = L.m;
+  const HcBounds b(nc, m);
+  auto al = [&](size_t bytes) {
+    void *p = ctx->pool.alloc(bytes ? bytes : 1);
+    hc.blocks.push_back(p);
+    return p;
+  };
+  hc.nc_bound = nc;
+  hc.lists.chunks = (SegChunk *)al(sizeof(SegChunk) * b.nchunk);
+  hc.lists.wave = (int32_t *)al(4 * b.nseg);
+  hc.lists.wavel = (int32_t *)al(4 * b.nwavel);
+  hc.lists.block = (int32_t *)al(4 * b.nblk);
+  hc.lists.blockl = (int32_t *)al(4 * b.nblkl);
+  hc.lists.parts = (HubPart *)al(sizeof(HubPart) * b.npart);
+  hc.lists.hubs = (int32_t *)al(4 * b.nhub);
+  static_assert(HASH_CTR_SLOT + HC_CTR_WORDS <= CTR_SLOTS, "hash counters");
+  hc.lists.ctr = ctx->d_ctr + HASH_CTR_SLOT;
+  hc.tc = (int32_t *)al(4 * m);
+  hc.tw = (int32_t *)al(4 * m);
+}
+
 template <typename OffT>
 void hash_prepare_impl(cm_ctx *ctx, const Level &L, const int32_t *match, const int32_t *cmap, int64_t nc,
                        const int64_t *d_nc, const int32_t *rep, const OffT *segoff, int32_t *cdeg,
@@ -700,74 +666,68 @@ void hash_prepare_impl(cm_ctx *ctx, const Level &L, const int32_t *match, const 
   if (seg_max < 0) seg_max = INT64_MAX;
   const int64_t m = L.m;
   const OffT *xadj = (const OffT *)L.xadj;
-  // bounds of the item counts (segments above REG_SEG hold at most m entries)
-  const int64_t nseg = std::min<int64_t>(nc, m / (REG_SEG + 1)) + 1;
-  const int64_t nchunk = m / HC_CHUNK + nseg;
-  const int64_t nblk = std::min<int64_t>(nc, m / (HC_WAVE_MAX + 1)) + 1;
-  const int64_t nblkl = std::min<int64_t>(nc, m / (HC_BLOCK_SPLIT + 1)) + 1;
-  const int64_t nhub = m / (HC_BLOCK_MAX + 1) + 1;
-  const int64_t npart = m / HC_PART + nhub;
+  if (!hc.lists.chunks) hash_alloc(ctx, L, nc, hc);
+  nc = hc.nc_bound;
+  const HcBounds b(nc, m);
+  const int64_t nseg = b.nseg, nchunk = b.nchunk, nwavel = b.nwavel, nblk = b.nblk, nblkl = b.nblkl,
+                nhub = b.nhub, npart = b.npart;
   auto al = [&](size_t bytes) {
     void *p = ctx->pool.alloc(bytes ? bytes : 1);
     hc.blocks.push_back(p);
     return p;
   };
-  hc.chunks = al(sizeof(SegChunk) * nchunk);
-  int32_t *wave_items = (int32_t *)al(4 * nseg), *block_items = (int32_t *)al(4 * nblk);
-  const int64_t nwavel = std::min<int64_t>(nc, m / (HC_WAVE_SPLIT + 1)) + 1;
-  int32_t *wavel_items = (int32_t *)al(4 * nwavel);
-  int32_t *blockl_items = (int32_t *)al(4 * nblkl);
-  HubPart *parts = (HubPart *)al(sizeof(HubPart) * npart);
-  int32_t *hubs = (int32_t *)al(4 * nhub);
-  static_assert(HASH_CTR_SLOT + HC_CTR_WORDS <= CTR_SLOTS, "hash counters");
-  hc.ctr = ctx->d_ctr + HASH_CTR_SLOT;
-  hc.tc = (int32_t *)al(4 * m);
-  hc.tw = (int32_t *)al(4 * m);
-  if (!ctx->hash_ctr_zero) fill_u32(ctx, hc.ctr, 0u, HC_CTR_WORDS);
-  ctx->hash_ctr_zero = false;
-  const SegChunk *chunks = (const SegChunk *)hc.chunks;
-  launch(ctx, "hc_select", 0.0, k_hc_select<OffT>, dim3(nblocks(nc, 256 * HC_SELECT)), dim3(256), 0, d_nc, segoff,
-         (SegChunk *)hc.chunks, wave_items, wavel_items, block_items, blockl_items, parts, hubs, hc.ctr);
+  const HcLists &hl = hc.lists;
+  unsigned *ctr = hl.ctr;
+  if (!hc.selected) {
+    if (!ctx->hash_ctr_zero) fill_u32(ctx, ctr, 0u, HC_CTR_WORDS);
+    ctx->hash_ctr_zero = false;
+    launch(ctx, "hc_select", 0.0, k_hc_select<OffT>, dim3(nblocks(nc, 256 * HC_SELECT)), dim3(256), 0, d_nc,
+           segoff, hl);
+  }
+  const SegChunk *chunks = hl.chunks;
+  const int32_t *wave_items = hl.wave, *wavel_items = hl.wavel, *block_items = hl.block, *blockl_items = hl.blockl,
+                *hubs = hl.hubs;
+  const HubPart *parts = hl.parts;
   const unsigned gchunk = (unsigned)std::min<int64_t>(4096, (nchunk + 3) / 4);
   hc.gchunk = gchunk;
   static_assert(2 * HC_WAVE_SPLIT == 1 << 7 && 2 * HC_WAVE_MAX == 1 << 9, "wave table sizes");
   launch(ctx, "hc_dedup_wave", 0.0, k_hc_dedup_wave<OffT, 7>,
          dim3((unsigned)std::min<int64_t>(4096, (nseg + 3) / 4)), dim3(256), 0, (const int32_t *)wave_items,
-         (const unsigned *)hc.ctr + HC_N_WAVE, segoff, rep, match, xadj, L.adjncy, L.adjwgt, cmap, hc.tc, hc.tw,
+         (const unsigned *)ctr + HC_N_WAVE, segoff, rep, match, xadj, L.adjncy, L.adjwgt, cmap, hc.tc, hc.tw,
          cdeg);
   if (seg_max > HC_WAVE_SPLIT)
     launch(ctx, "hc_dedup_wave_l", 0.0, k_hc_dedup_wave<OffT, 9>,
            dim3((unsigned)std::min<int64_t>(4096, (nwavel + 3) / 4)), dim3(256), 0, (const int32_t *)wavel_items,
-           (const unsigned *)hc.ctr + HC_N_WAVEL, segoff, rep, match, xadj, L.adjncy, L.adjwgt, cmap, hc.tc, hc.tw,
+           (const unsigned *)ctr + HC_N_WAVEL, segoff, rep, match, xadj, L.adjncy, L.adjwgt, cmap, hc.tc, hc.tw,
            cdeg);
   static_assert(2 * HC_BLOCK_SPLIT == 1 << 11 && 2 * HC_BLOCK_MAX == 1 << 12, "block table sizes");
   if (seg_max > HC_WAVE_MAX)
     launch(ctx, "hc_dedup_block", 0.0, k_hc_dedup_block<OffT, 11>, dim3((unsigned)std::min<int64_t>(4096, nblk)),
-           dim3(256), 0, (const int32_t *)block_items, (const unsigned *)hc.ctr + HC_N_BLOCK, segoff, rep, match,
+           dim3(256), 0, (const int32_t *)block_items, (const unsigned *)ctr + HC_N_BLOCK, segoff, rep, match,
            xadj, L.adjncy, L.adjwgt, cmap, hc.tc, hc.tw, cdeg);
   if (seg_max > HC_BLOCK_SPLIT)
     launch(ctx, "hc_dedup_block_l", 0.0, k_hc_dedup_block<OffT, 12>,
            dim3((unsigned)std::min<int64_t>(2048, nblkl)), dim3(256), 0, (const int32_t *)blockl_items,
-           (const unsigned *)hc.ctr + HC_N_BLOCKL, segoff, rep, match, xadj, L.adjncy, L.adjwgt, cmap, hc.tc, hc.tw,
+           (const unsigned *)ctr + HC_N_BLOCKL, segoff, rep, match, xadj, L.adjncy, L.adjwgt, cmap, hc.tc, hc.tw,
            cdeg);
   if (m > HC_BLOCK_MAX && seg_max > HC_BLOCK_MAX) {  // a hub needs more than HC_BLOCK_MAX entries
     int32_t *cval = (int32_t *)al(4 * m), *wv = (int32_t *)al(4 * m);
-    launch(ctx, "hc_gather", 0.0, k_hc_gather<OffT>, dim3(gchunk), dim3(256), 0, chunks, (const unsigned *)hc.ctr,
+    launch(ctx, "hc_gather", 0.0, k_hc_gather<OffT>, dim3(gchunk), dim3(256), 0, chunks, (const unsigned *)ctr,
            segoff, rep, match, xadj, L.adjncy, L.adjwgt, cmap, cval, wv);
     uint8_t *head = (uint8_t *)al(m);
     int32_t *hw = (int32_t *)al(4 * m);
     int32_t *ccnt = (int32_t *)al(4 * nchunk), *cself = (int32_t *)al(4 * nchunk), *coff = (int32_t *)al(4 * nchunk);
     int32_t *hself = (int32_t *)al(4 * nchunk), *htot = (int32_t *)al(4 * nchunk);
     launch(ctx, "hc_dedup_hub", 0.0, k_hc_dedup_hub<OffT>, dim3((unsigned)std::min<int64_t>(1024, npart)),
-           dim3(HC_HUB_THREADS), 0, (const HubPart *)parts, (const unsigned *)hc.ctr, segoff, (const int32_t *)cval,
-           (const int32_t *)wv, head, hw, hc.ctr + HC_OVERFLOW);
+           dim3(HC_HUB_THREADS), 0, (const HubPart *)parts, (const unsigned *)ctr, segoff, (const int32_t *)cval,
+           (const int32_t *)wv, head, hw, ctr + HC_OVERFLOW);
     launch(ctx, "hc_hub_count", 0.0, k_hc_hub_count<OffT>, dim3(gchunk), dim3(256), 0, chunks,
-           (const unsigned *)hc.ctr, segoff, (const uint8_t *)head, (const int32_t *)cval, ccnt, cself);
+           (const unsigned *)ctr, segoff, (const uint8_t *)head, (const int32_t *)cval, ccnt, cself);
     launch(ctx, "hc_hub_scan", 0.0, k_hc_hub_scan<OffT>, dim3((unsigned)std::min<int64_t>(1024, (nhub + 3) / 4)),
-           dim3(256), 0, (const int32_t *)hubs, (const unsigned *)hc.ctr, chunks, (const int32_t *)ccnt,
+           dim3(256), 0, (const int32_t *)hubs, (const unsigned *)ctr, chunks, (const int32_t *)ccnt,
            (const int32_t *)cself, coff, hself, htot, cdeg);
     launch(ctx, "hc_hub_write", 0.0, k_hc_hub_write<OffT>, dim3(gchunk), dim3(256), 0, chunks,
-           (const unsigned *)hc.ctr, segoff, (const uint8_t *)head, (const int32_t *)cval, (const int32_t *)hw,
+           (const unsigned *)ctr, segoff, (const uint8_t *)head, (const int32_t *)cval, (const int32_t *)hw,
            (const int32_t *)coff, (const int32_t *)hself, (const int32_t *)htot, hc.tc, hc.tw);
   }
 }
@@ -777,13 +737,13 @@ void hash_emit_impl(cm_ctx *ctx, const OffT *segoff, const int32_t *cdeg, const 
                     int32_t *cadjwgt, const HashContract &hc)
 {
   if (hc.seg_max < 0 || hc.seg_max > TILE_COPY_SEG)
-    launch(ctx, "hc_emit", 0.0, k_hc_emit<OffT>, dim3(hc.gchunk), dim3(256), 0, (const SegChunk *)hc.chunks,
-         (const unsigned *)hc.ctr, segoff, cdeg, cxadj, (const int32_t *)hc.tc, (const int32_t *)hc.tw, cadjncy,
+    launch(ctx, "hc_emit", 0.0, k_hc_emit<OffT>, dim3(hc.gchunk), dim3(256), 0, (const SegChunk *)hc.lists.chunks,
+         (const unsigned *)hc.lists.ctr, segoff, cdeg, cxadj, (const int32_t *)hc.tc, (const int32_t *)hc.tw, cadjncy,
          cadjwgt);
   // hub-table overflow flag -> h_small[6] with the caller's next read (only
   // hubs can overflow; the caller cleared h_small[6])
   if (hc.seg_max < 0 || hc.seg_max > HC_BLOCK_MAX)
-    CM_HIP(hipMemcpyAsync(ctx->h_small + 6, hc.ctr + HC_OVERFLOW, sizeof(unsigned), hipMemcpyDeviceToHost,
+    CM_HIP(hipMemcpyAsync(ctx->h_small + 6, hc.lists.ctr + HC_OVERFLOW, sizeof(unsigned), hipMemcpyDeviceToHost,
                           ctx->stream));
 }
 

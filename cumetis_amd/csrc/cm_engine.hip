@@ -8,6 +8,7 @@
 
 #include "cm_common.hpp"
 #include "cm_internal.hpp"
+#include "cm_contract_hash.hpp"
 #include "cm_segsort.hpp"
 
 namespace cm {
@@ -457,9 +458,19 @@ static Level coarsen_level(cm_ctx *ctx, cm_hier *h, Level &F, int32_t maxvwgt, u
       // the numbering writes the completed copy of the rounds' match to
       // F.match; nc stays on the device until the contraction's one
       // synchronisation
-      number(ctx, F, F.match, F.cmap, rep, segoff, C.vwgt, false, mstate);
+      // first-seen levels with hashed segments: the numbering also lists
+      // them for the hash contraction (one select pass fewer)
+      struct Lists {
+        cm_ctx *ctx;
+        HashContract hc;
+        ~Lists() { hash_release(ctx, hc); }  // after contract() this is a no-op
+      } lists{ctx, {}};
+      HashContract &hc = lists.hc;
+      const bool pre = row_order == CM_ROW_FIRST_SEEN && n > 0 && hash_needed(F);
+      if (pre) hash_alloc(ctx, F, n, hc);
+      number(ctx, F, F.match, F.cmap, rep, segoff, C.vwgt, false, mstate, pre ? &hc : nullptr);
       C.m = contract(ctx, F, F.match, F.cmap, -1, rep, segoff, row_order, C.xadj, C.adjncy, C.adjwgt,
-                     &C.maxdeg, &C.n);
+                     &C.maxdeg, &C.n, pre ? &hc : nullptr);
       C.maxvw = ctx->h_small[3];
       if (match_done(ctx, mr)) break;
     }

@@ -124,7 +124,7 @@ struct cm_ctx {
   uint32_t epoch = 0;             // look-back launch epoch (1 .. 2^20-1)
   int64_t last_rounds = 0;
   // the hash-contraction item counters (d_ctr + HASH_CTR_SLOT) are zero at
-  // this point of the stream: the numbering kernel clears them on its way
+  // this point of the stream: the matching's init kernel clears them
   bool hash_ctr_zero = false;
 
   hipEvent_t ev_get();
@@ -211,9 +211,12 @@ int match_handshake(cm_ctx *ctx, const Level &L, const MatchOwners &own, int max
 // match entries still -1 (matching run with finish=false) are completed as
 // self matches.  With d_match_in, the matching is read from there and its
 // completed copy written to d_match.
+// With hc (from hash_alloc), the numbering also fills hc's item lists for
+// the hash contraction (hand hc to contract() then).
+struct HashContract;
 int64_t number(cm_ctx *ctx, const Level &L, int32_t *d_match, int32_t *d_cmap,
                int32_t *d_rep, void *d_segoff, int32_t *d_cvwgt, bool wait = true,
-               const int32_t *d_match_in = nullptr);
+               const int32_t *d_match_in = nullptr, HashContract *hc = nullptr);
 void queue_max_degree(cm_ctx *ctx, int64_t n, bool off64, const void *xadj, const int32_t *vwgt = nullptr);
 // (c) contraction of L into coarse CSR; returns mc.  cxadj/cadjncy/cadjwgt
 //     must hold nc+1 / L.m / L.m entries.  nc = -1: the count left on the
@@ -223,7 +226,7 @@ void queue_max_degree(cm_ctx *ctx, int64_t n, bool off64, const void *xadj, cons
 int64_t contract(cm_ctx *ctx, const Level &L, const int32_t *d_match, const int32_t *d_cmap,
                  int64_t nc, const int32_t *d_rep, const void *d_segoff, int row_order,
                  void *d_cxadj, int32_t *d_cadjncy, int32_t *d_cadjwgt, int64_t *maxdeg = nullptr,
-                 int64_t *ncp = nullptr);
+                 int64_t *ncp = nullptr, HashContract *pre = nullptr);
 // segmented sort of 64-bit keys with 32-bit values, segments segoff[0..nseg]
 void segsort_u64(cm_ctx *ctx, int64_t nseg, const void *d_segoff, bool off64, int64_t total,
                  uint64_t *d_keys, uint32_t *d_vals);

@@ -56,10 +56,12 @@ __device__ __forceinline__ bool key_better(uint64_t a1, uint64_t a2, uint64_t b1
   return a1 > b1 || (a1 == b1 && a2 > b2);
 }
 
+// Also clears the hash contraction's item counters (hzero, 16 words).
 __global__ __launch_bounds__(256) void k_match_init(int64_t n, int32_t *match, int32_t *prop, unsigned *flags,
-                                                    int64_t nflags)
+                                                    int64_t nflags, unsigned *hzero)
 {
   const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v < 16) hzero[v] = 0u;
   if (v < n) {
     match[v] = -1;
     prop[v] = -1;
@@ -521,7 +523,8 @@ static void match_start(cm_ctx *ctx, MatchRun &mr)
   if (has_long) fill_u32(ctx, mr.part.scanned, 0u, cap);  // no launch number is 0
   mr.g = nblocks(n, 256);
   launch(ctx, "match_init", 8.0 * n, k_match_init, dim3(mr.g), dim3(256), 0, n, mr.match, mr.prop, mr.flags,
-         2 * (int64_t)mr.max_rounds + 2);
+         2 * (int64_t)mr.max_rounds + 2, ctx->d_ctr + HASH_CTR_SLOT);
+  ctx->hash_ctr_zero = true;
   // extra blocks whose waves stride over the long-row chunks (none when the
   // level is known to have no long row)
   if (has_long) {
