@@ -181,6 +181,53 @@ __device__ __forceinline__ T block256_excl_scan(T v, T &total, T *sh)
   return before + inc - v;
 }
 
+// -------------------------------------------------------------- read-back
+// Words a level's last kernel writes straight into pinned host memory (the
+// small totals and the matching's round flags), so that the host's one
+// synchronisation per level needs no copy launches: dst[first + i] =
+// src[first + i] (slot `own_slot` takes the writer's own value), dst[6] =
+// *ovf, dst_flags[i] = flags[i], and last dst[RB_TAG_SLOT] = tag.  Passed by
+// value as a kernel argument, so it is kept small.
+constexpr int RB_TAG_SLOT = 15;
+
+struct Readback {
+  int64_t *dst = nullptr;
+  unsigned *dst_flags = nullptr;
+  const unsigned *flags = nullptr;
+  const unsigned *ovf = nullptr;
+  int64_t tag = 0;
+  int32_t nflags = 0;
+  int16_t first = 0, count = 0;
+};
+
+// System-scope stores write through to host memory.
+template <typename T>
+__device__ __forceinline__ void store_host(T *p, T v)
+{
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// Called by all lanes of one wave once every value it copies is final (src
+// is the device copy of the slots): lane i stores word i (the slots, then the
+// flags, then the overflow word), so the stores leave together.
+__device__ __forceinline__ void readback_write(const Readback &rb, const int64_t *src, int own_slot, int64_t own)
+{
+  if (!rb.dst) return;
+  const int64_t words = (int64_t)rb.count + rb.nflags + (rb.ovf ? 1 : 0);
+  for (int64_t i = lane(); i < words; i += CM_WAVE) {
+    if (i < rb.count) {
+      const int slot = rb.first + (int)i;
+      store_host(&rb.dst[slot], slot == own_slot ? own : src[slot]);
+    } else if (i < (int64_t)rb.count + rb.nflags) {
+      const int64_t j = i - rb.count;
+      store_host(&rb.dst_flags[j], rb.flags[j]);
+    } else {
+      store_host(&rb.dst[6], (int64_t)*rb.ovf);
+    }
+  }
+  if (lane() == 0) store_host(&rb.dst[RB_TAG_SLOT], rb.tag);
+}
+
 // ------------------------------------------------------------- edge order
 // splitmix64 finaliser (Steele, Lea, Flood 2014), top 32 bits.
 __device__ __forceinline__ uint32_t edge_hash(uint64_t seed, uint32_t a, uint32_t b)

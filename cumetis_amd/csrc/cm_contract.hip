@@ -722,7 +722,8 @@ __global__ __launch_bounds__(SMALL_TILE) void k_contract_small(const int64_t *__
                                                         const int32_t *__restrict__ tw,
                                                         OffT *__restrict__ cxadj, int32_t *__restrict__ cadjncy,
                                                         int32_t *__restrict__ cadjwgt, uint64_t *st,
-                                                        unsigned *ctr, int64_t *total, uint32_t epoch)
+                                                        unsigned *ctr, int64_t *total, uint32_t epoch,
+                                                        Readback rb)
 {
   __shared__ uint64_t sh[SMALL_TILE / CM_WAVE];
   __shared__ uint64_t s_excl;
@@ -783,6 +784,10 @@ __global__ __launch_bounds__(SMALL_TILE) void k_contract_small(const int64_t *__
     if (threadIdx.x == 0) s_excl = e;
   }
   __syncthreads();
+  // the last tile's wave 0 sends the read-back of the small totals (total =
+  // d_small + 1, the coarse entry count, is its own slot)
+  if ((int64_t)(tile + 1) * SMALL_TILE >= nc && threadIdx.x < CM_WAVE)
+    readback_write(rb, total - 1, 1, (int64_t)(s_excl + tot));
   const uint64_t off = s_excl + ex;
   if (c < nc) {
     cxadj[c] = (OffT)off;
@@ -919,14 +924,15 @@ static int64_t contract_impl(cm_ctx *ctx, const Level &L, const int32_t *match, 
                                  (double)sizeof(OffT) * (ncb + 1);
       // hashed rows copied here when the emit pass is not needed
       const bool tile_copy = hashed && hc.seg_max >= 0 && hc.seg_max <= TILE_COPY_SEG;
+      // the totals come back with this kernel (the emit pass only copies rows)
+      ctx->h_small[6] = 0;
+      const Readback rb = readback_begin(ctx, 0, 5, hashed ? hash_overflow_flag(hc) : nullptr);
       launch(ctx, "contract_small", small_bytes, k_contract_small<true, OffT>, dim3((unsigned)tiles), dim3(SMALL_TILE),
              0, d_nc, segoff, rep, match, xadj, L.adjncy, L.adjwgt, cmap, (const int32_t *)cdeg,
              tile_copy ? (const int32_t *)hc.tc : nullptr, tile_copy ? (const int32_t *)hc.tw : nullptr, cxadj,
-             cadjncy, cadjwgt, ctx->d_status, ctx->d_ctr, ctx->d_small + 1, ep);
-      ctx->h_small[6] = 0;
+             cadjncy, cadjwgt, ctx->d_status, ctx->d_ctr, ctx->d_small + 1, ep, rb);
       if (hashed) hash_emit_impl<OffT>(ctx, segoff, cdeg, cxadj, cadjncy, cadjwgt, hc);
-      CM_HIP(hipMemcpyAsync(ctx->h_small, ctx->d_small, 5 * sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
-      sync(ctx);
+      readback_end(ctx, rb);
     } catch (...) {
       hash_release(ctx, hc);
       ctx->pool.release(cdeg);
@@ -999,16 +1005,17 @@ static int64_t contract_impl(cm_ctx *ctx, const Level &L, const int32_t *match, 
   const uint32_t ep = lookback_begin(ctx, tiles);
   const double small_bytes = 8.0 * m + 12.0 * n + (double)sizeof(OffT) * (n + 1) + 8.0 * nc +
                              (double)sizeof(OffT) * (nc + 1);
+  const Readback rb = readback_begin(ctx, 1, 4);
   if (fs)
     launch(ctx, "contract_small", small_bytes, k_contract_small<true, OffT>, dim3((unsigned)tiles), dim3(SMALL_TILE),
            0, (const int64_t *)(ctx->d_small + 7), segoff, rep, match, xadj, L.adjncy, L.adjwgt, cmap,
            (const int32_t *)cdeg, (const int32_t *)nullptr, (const int32_t *)nullptr, cxadj, cadjncy, cadjwgt,
-           ctx->d_status, ctx->d_ctr, ctx->d_small + 1, ep);
+           ctx->d_status, ctx->d_ctr, ctx->d_small + 1, ep, rb);
   else
     launch(ctx, "contract_small", small_bytes, k_contract_small<false, OffT>, dim3((unsigned)tiles),
            dim3(SMALL_TILE), 0, (const int64_t *)(ctx->d_small + 7), segoff, rep, match, xadj, L.adjncy,
            L.adjwgt, cmap, (const int32_t *)cdeg, (const int32_t *)nullptr, (const int32_t *)nullptr, cxadj,
-           cadjncy, cadjwgt, ctx->d_status, ctx->d_ctr, ctx->d_small + 1, ep);
+           cadjncy, cadjwgt, ctx->d_status, ctx->d_ctr, ctx->d_small + 1, ep, rb);
   if (sorted_path) {
 #define CM_EMIT(FSV)                                                                                      \
   if (nmid)                                                                                              \
@@ -1026,9 +1033,7 @@ static int64_t contract_impl(cm_ctx *ctx, const Level &L, const int32_t *match, 
     }
 #undef CM_EMIT
   }
-  CM_HIP(hipMemcpyAsync(ctx->h_small + 1, ctx->d_small + 1, 4 * sizeof(int64_t), hipMemcpyDeviceToHost,
-                        ctx->stream));
-  sync(ctx);
+  readback_end(ctx, rb);
   const int64_t mc = ctx->h_small[1];
   if (maxdeg) *maxdeg = ctx->h_small[4];
   release_bins(ctx, bins);

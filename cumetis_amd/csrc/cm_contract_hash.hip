@@ -740,11 +740,11 @@ void hash_emit_impl(cm_ctx *ctx, const OffT *segoff, const int32_t *cdeg, const 
     launch(ctx, "hc_emit", 0.0, k_hc_emit<OffT>, dim3(hc.gchunk), dim3(256), 0, (const SegChunk *)hc.lists.chunks,
          (const unsigned *)hc.lists.ctr, segoff, cdeg, cxadj, (const int32_t *)hc.tc, (const int32_t *)hc.tw, cadjncy,
          cadjwgt);
-  // hub-table overflow flag -> h_small[6] with the caller's next read (only
-  // hubs can overflow; the caller cleared h_small[6])
-  if (hc.seg_max < 0 || hc.seg_max > HC_BLOCK_MAX)
-    CM_HIP(hipMemcpyAsync(ctx->h_small + 6, hc.lists.ctr + HC_OVERFLOW, sizeof(unsigned), hipMemcpyDeviceToHost,
-                          ctx->stream));
+}
+
+const unsigned *hash_overflow_flag(const HashContract &hc)
+{
+  return hc.seg_max < 0 || hc.seg_max > HC_BLOCK_MAX ? hc.lists.ctr + HC_OVERFLOW : nullptr;
 }
 
 template void hash_prepare_impl<int32_t>(cm_ctx *, const Level &, const int32_t *, const int32_t *, int64_t,

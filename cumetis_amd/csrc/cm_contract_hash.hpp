@@ -54,19 +54,20 @@ struct HcLists {
   unsigned *ctr = nullptr;
 };
 
-// Items a segment of len entries adds to each list.
+// Items a segment of len entries adds to each list.  Every counter is
+// updated by its own predicated add (a class chosen by branches would merge
+// into one dynamically indexed update, which puts the array in scratch).
 __device__ __forceinline__ void hc_count(int64_t len, unsigned (&cnt)[HC_N_COUNTERS])
 {
   if (len <= REG_SEG) return;
+  const bool hub = len > HC_BLOCK_MAX;
   cnt[HC_N_CHUNK] += (unsigned)((len + HC_CHUNK - 1) / HC_CHUNK);
-  if (len <= HC_WAVE_SPLIT) ++cnt[HC_N_WAVE];
-  else if (len <= HC_WAVE_MAX) ++cnt[HC_N_WAVEL];
-  else if (len <= HC_BLOCK_SPLIT) ++cnt[HC_N_BLOCK];
-  else if (len <= HC_BLOCK_MAX) ++cnt[HC_N_BLOCKL];
-  else {
-    ++cnt[HC_N_HUB];
-    cnt[HC_N_PART] += (unsigned)((len + HC_PART - 1) / HC_PART);
-  }
+  cnt[HC_N_WAVE] += len <= HC_WAVE_SPLIT ? 1u : 0u;
+  cnt[HC_N_WAVEL] += len > HC_WAVE_SPLIT && len <= HC_WAVE_MAX ? 1u : 0u;
+  cnt[HC_N_BLOCK] += len > HC_WAVE_MAX && len <= HC_BLOCK_SPLIT ? 1u : 0u;
+  cnt[HC_N_BLOCKL] += len > HC_BLOCK_SPLIT && !hub ? 1u : 0u;
+  cnt[HC_N_HUB] += hub ? 1u : 0u;
+  cnt[HC_N_PART] += hub ? (unsigned)((len + HC_PART - 1) / HC_PART) : 0u;
 }
 
 // The items of segment c at the positions at[] (advanced past them).
@@ -77,20 +78,24 @@ __device__ __forceinline__ void hc_write(int32_t c, int64_t len, unsigned (&at)[
   const unsigned first = at[HC_N_CHUNK];
   for (int k = 0; k < nch; ++k) hl.chunks[first + k] = SegChunk{c, (int32_t)first, nch, k};
   at[HC_N_CHUNK] += nch;
-  if (len <= HC_WAVE_SPLIT) {
-    hl.wave[at[HC_N_WAVE]++] = c;
-  } else if (len <= HC_WAVE_MAX) {
-    hl.wavel[at[HC_N_WAVEL]++] = c;
-  } else if (len <= HC_BLOCK_SPLIT) {
-    hl.block[at[HC_N_BLOCK]++] = c;
-  } else if (len <= HC_BLOCK_MAX) {
-    hl.blockl[at[HC_N_BLOCKL]++] = c;
-  } else {
-    hl.hubs[at[HC_N_HUB]++] = (int32_t)first;  // a hub is named by its first chunk
+  const bool hub = len > HC_BLOCK_MAX;
+  const bool cw = len <= HC_WAVE_SPLIT, cwl = !cw && len <= HC_WAVE_MAX;
+  const bool cb = len > HC_WAVE_MAX && len <= HC_BLOCK_SPLIT, cbl = len > HC_BLOCK_SPLIT && !hub;
+  if (cw) hl.wave[at[HC_N_WAVE]] = c;
+  if (cwl) hl.wavel[at[HC_N_WAVEL]] = c;
+  if (cb) hl.block[at[HC_N_BLOCK]] = c;
+  if (cbl) hl.blockl[at[HC_N_BLOCKL]] = c;
+  at[HC_N_WAVE] += cw ? 1u : 0u;
+  at[HC_N_WAVEL] += cwl ? 1u : 0u;
+  at[HC_N_BLOCK] += cb ? 1u : 0u;
+  at[HC_N_BLOCKL] += cbl ? 1u : 0u;
+  if (hub) {
+    hl.hubs[at[HC_N_HUB]] = (int32_t)first;  // a hub is named by its first chunk
     const int P = (int)((len + HC_PART - 1) / HC_PART);
     for (int p = 0; p < P; ++p) hl.parts[at[HC_N_PART] + p] = HubPart{c, p, P, 0};
-    at[HC_N_PART] += P;
   }
+  at[HC_N_HUB] += hub ? 1u : 0u;
+  at[HC_N_PART] += hub ? (unsigned)((len + HC_PART - 1) / HC_PART) : 0u;
 }
 
 // Block of NW waves, called by all its threads: reserves every thread's
@@ -149,11 +154,13 @@ template <typename OffT>
 void hash_prepare_impl(cm_ctx *ctx, const Level &L, const int32_t *match, const int32_t *cmap, int64_t nc_bound,
                        const int64_t *d_nc, const int32_t *rep, const OffT *segoff, int32_t *cdeg,
                        int64_t seg_max, HashContract &hc);
-// Staging area -> final rows once cxadj is written; queues the overflow flag
-// into h_small[6] for the caller's next synchronisation.
+// Staging area -> final rows once cxadj is written.
 template <typename OffT>
 void hash_emit_impl(cm_ctx *ctx, const OffT *segoff, const int32_t *cdeg, const OffT *cxadj, int32_t *cadjncy,
                     int32_t *cadjwgt, const HashContract &hc);
+// The device word that is nonzero if a hub table filled (only hubs can
+// overflow), or nullptr when no hub pass ran.
+const unsigned *hash_overflow_flag(const HashContract &hc);
 void hash_release(cm_ctx *ctx, HashContract &hc);
 
 }  // namespace cm

@@ -141,6 +141,42 @@ void ensure_host_flags(cm_ctx *ctx, size_t words)
   }
   CM_HIP(hipHostMalloc((void **)&ctx->h_flags, sizeof(unsigned) * cap, hipHostMallocDefault));
   ctx->h_flags_cap = cap;
+  CM_HIP(hipHostGetDevicePointer((void **)&ctx->h_flags_dev, ctx->h_flags, 0));
+}
+
+Readback readback_begin(cm_ctx *ctx, int first, int count, const unsigned *ovf)
+{
+  Readback rb;
+  rb.dst = ctx->h_small_dev;
+  rb.dst_flags = ctx->h_flags_dev;
+  rb.flags = ctx->rb_flags;
+  rb.nflags = ctx->rb_flags ? (int32_t)ctx->rb_nflags : 0;
+  rb.ovf = ovf;
+  rb.first = (int16_t)first;
+  rb.count = (int16_t)count;
+  rb.tag = ++ctx->rb_tag;
+  ctx->rb_flags = nullptr;
+  ctx->rb_nflags = 0;
+  return rb;
+}
+
+void readback_end(cm_ctx *ctx, const Readback &rb)
+{
+  sync(ctx);
+  // the stream's completion makes the stores visible; a stale tag would
+  // mean they were not, so the words are then copied the ordinary way
+  if (ctx->h_small[RB_TAG_SLOT] != rb.tag) {
+    CM_HIP(hipMemcpyAsync(ctx->h_small + rb.first, ctx->d_small + rb.first, sizeof(int64_t) * rb.count,
+                          hipMemcpyDeviceToHost, ctx->stream));
+    if (rb.ovf)
+      CM_HIP(hipMemcpyAsync(ctx->h_small + 6, rb.ovf, sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
+    if (rb.nflags)
+      CM_HIP(hipMemcpyAsync(ctx->h_flags, rb.flags, sizeof(unsigned) * rb.nflags, hipMemcpyDeviceToHost,
+                            ctx->stream));
+    sync(ctx);
+    if (rb.ovf) ctx->h_small[6] = (int64_t)(uint32_t)ctx->h_small[6];
+    ctx->stats["readback_fallback"].count += 1;
+  }
 }
 
 uint32_t lookback_begin(cm_ctx *ctx, size_t words)
@@ -450,7 +486,7 @@ static Level coarsen_level(cm_ctx *ctx, cm_hier *h, Level &F, int32_t maxvwgt, u
     // synchronisation costs little, check the rounds first.
     const bool speculate = m <= SPECULATE_EDGES;
     if (n > 0) {
-      match_queue(ctx, mr, batch);
+      match_queue(ctx, mr, batch, !speculate);  // speculative: flags come with the contraction's read-back
       if (!speculate) sync(ctx);
     }
     for (int pass = 0;; ++pass) {
@@ -472,10 +508,17 @@ static Level coarsen_level(cm_ctx *ctx, cm_hier *h, Level &F, int32_t maxvwgt, u
       C.m = contract(ctx, F, F.match, F.cmap, -1, rep, segoff, row_order, C.xadj, C.adjncy, C.adjwgt,
                      &C.maxdeg, &C.n, pre ? &hc : nullptr);
       C.maxvw = ctx->h_small[3];
+      if (ctx->rb_flags) {  // a contraction that read nothing back left the flags
+        CM_HIP(hipMemcpyAsync(ctx->h_flags, ctx->rb_flags, sizeof(unsigned) * ctx->rb_nflags,
+                              hipMemcpyDeviceToHost, ctx->stream));
+        ctx->rb_flags = nullptr;
+        sync(ctx);
+      }
       if (match_done(ctx, mr)) break;
     }
     F.rounds = match_finish(ctx, mr);
   } catch (...) {
+    ctx->rb_flags = nullptr;
     match_end(ctx, mr);
     ctx->pool.release(rep);
     ctx->pool.release(mstate);
@@ -568,6 +611,7 @@ int cm_ctx_create(int device, void *stream, cm_ctx **out)
     ctx->d_ctr = (unsigned *)ctx->pool.alloc(sizeof(unsigned) * cm::CTR_SLOTS);
     ctx->d_small = (int64_t *)ctx->pool.alloc(sizeof(int64_t) * cm::SMALL_SLOTS);
     CM_HIP(hipHostMalloc((void **)&ctx->h_small, sizeof(int64_t) * cm::SMALL_SLOTS, hipHostMallocDefault));
+    CM_HIP(hipHostGetDevicePointer((void **)&ctx->h_small_dev, ctx->h_small, 0));
     cm::fill_u32(ctx, ctx->d_ctr, 0u, cm::CTR_SLOTS);
     cm::sync(ctx);
   } catch (const cm::Error &e) {

@@ -119,6 +119,13 @@ struct cm_ctx {
   int64_t *h_small = nullptr;     // pinned mirror of d_small
   unsigned *h_flags = nullptr;    // pinned copy of matching round flags
   size_t h_flags_cap = 0;         // in words
+  int64_t *h_small_dev = nullptr;   // device addresses of h_small and h_flags
+  unsigned *h_flags_dev = nullptr;  // (written by a level's last kernel)
+  // round flags for the next read-back (readback_begin): the matching's
+  // flags are then copied with the level's totals instead of on their own
+  const unsigned *rb_flags = nullptr;
+  int64_t rb_nflags = 0;
+  int64_t rb_tag = 0;
   uint64_t *d_status = nullptr;   // look-back status words
   size_t status_cap = 0;          // in words
   uint32_t epoch = 0;             // look-back launch epoch (1 .. 2^20-1)
@@ -168,6 +175,13 @@ inline unsigned nblocks(int64_t work, int per_block)
 }
 
 void sync(cm_ctx *ctx);
+// The read-back of a contraction (see Readback): d_small[first, first +
+// count) to h_small, *ovf (if given) to h_small[6], and the registered round
+// flags (ctx->rb_flags, consumed) to h_flags.  begin() describes it for the
+// kernel that sends it; end() synchronises and checks that it arrived.
+struct Readback;  // cm_common.hpp
+Readback readback_begin(cm_ctx *ctx, int first, int count, const unsigned *ovf = nullptr);
+void readback_end(cm_ctx *ctx, const Readback &rb);
 void fill_u32(cm_ctx *ctx, void *p, uint32_t v, int64_t count);
 void fill_u64(cm_ctx *ctx, void *p, uint64_t v, int64_t count);
 void scan_excl_i32(cm_ctx *ctx, int64_t n, const int32_t *in, int32_t *out);
@@ -193,7 +207,8 @@ struct MatchOwners {
 // themselves; end() releases the state.
 struct MatchRun;
 MatchRun *match_begin(cm_ctx *ctx, const Level &L, const MatchOwners &own, int max_rounds, int32_t *d_match);
-void match_queue(cm_ctx *ctx, MatchRun *mr, int count);
+// copy_flags = false leaves the flags to the next read-back (readback_begin).
+void match_queue(cm_ctx *ctx, MatchRun *mr, int count, bool copy_flags = true);
 bool match_done(cm_ctx *ctx, MatchRun *mr);
 int match_finish(cm_ctx *ctx, MatchRun *mr);
 void match_copy_finished(cm_ctx *ctx, MatchRun *mr, int32_t *out);

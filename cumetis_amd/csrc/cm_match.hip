@@ -610,12 +610,19 @@ MatchRun *match_begin(cm_ctx *ctx, const Level &L, const MatchOwners &own, int m
   return mr;
 }
 
-void match_queue(cm_ctx *ctx, MatchRun *mr, int count)
+void match_queue(cm_ctx *ctx, MatchRun *mr, int count, bool copy_flags)
 {
   if (mr->L->n == 0) return;
   if (mr->L->off64) match_launch<int64_t>(ctx, *mr, count);
   else match_launch<int32_t>(ctx, *mr, count);
-  match_queue_flags(ctx, *mr);
+  if (copy_flags) {
+    match_queue_flags(ctx, *mr);
+  } else {  // the next read-back (readback_begin) copies them
+    const size_t words = 2 * (size_t)std::max(mr->r - 1, 1);
+    ensure_host_flags(ctx, words);
+    ctx->rb_flags = mr->flags;
+    ctx->rb_nflags = (int64_t)words;
+  }
 }
 
 bool match_done(cm_ctx *ctx, MatchRun *mr)
