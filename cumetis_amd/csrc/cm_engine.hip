@@ -144,6 +144,18 @@ void ensure_host_flags(cm_ctx *ctx, size_t words)
   CM_HIP(hipHostGetDevicePointer((void **)&ctx->h_flags_dev, ctx->h_flags, 0));
 }
 
+const int32_t *unit_weights(cm_ctx *ctx, int64_t count)
+{
+  if (count > ctx->ones_cap) {  // the pool keeps the old block until the context dies
+    const int64_t cap = std::max<int64_t>(count, std::max<int64_t>(2 * ctx->ones_cap, 1 << 16));
+    int32_t *p = (int32_t *)ctx->pool.alloc(sizeof(int32_t) * cap);
+    fill_u32(ctx, p, 1u, cap);
+    ctx->ones = p;
+    ctx->ones_cap = cap;
+  }
+  return ctx->ones;
+}
+
 Readback readback_begin(cm_ctx *ctx, int first, int count, const unsigned *ovf)
 {
   Readback rb;
@@ -410,14 +422,8 @@ static Level level0(cm_ctx *ctx, const cm_graph_view *g, cm_hier *h, bool valida
   sync(ctx);
   L.maxdeg = ctx->h_small[4];
   L.maxvw = L.vwgt ? ctx->h_small[3] : 1;
-  if (!L.adjwgt) {
-    L.adjwgt = (int32_t *)own(4 * (L.m > 0 ? L.m : 1));
-    fill_u32(ctx, L.adjwgt, 1u, L.m);
-  }
-  if (!L.vwgt) {
-    L.vwgt = (int32_t *)own(4 * (L.n > 0 ? L.n : 1));
-    fill_u32(ctx, L.vwgt, 1u, L.n);
-  }
+  if (!L.adjwgt) L.adjwgt = const_cast<int32_t *>(unit_weights(ctx, L.m));
+  if (!L.vwgt) L.vwgt = const_cast<int32_t *>(unit_weights(ctx, L.n));
   if (!g->on_device && g->vwgt) {
     int64_t s = 0;
     for (int64_t i = 0; i < L.n; ++i) s += g->vwgt[i];

@@ -126,6 +126,10 @@ struct cm_ctx {
   const unsigned *rb_flags = nullptr;
   int64_t rb_nflags = 0;
   int64_t rb_tag = 0;
+  // unit weights shared by every graph given without weights (never
+  // written; earlier, shorter blocks stay valid until the context dies)
+  int32_t *ones = nullptr;
+  int64_t ones_cap = 0;
   uint64_t *d_status = nullptr;   // look-back status words
   size_t status_cap = 0;          // in words
   uint32_t epoch = 0;             // look-back launch epoch (1 .. 2^20-1)
@@ -183,6 +187,9 @@ struct Readback;  // cm_common.hpp
 Readback readback_begin(cm_ctx *ctx, int first, int count, const unsigned *ovf = nullptr);
 void readback_end(cm_ctx *ctx, const Readback &rb);
 void fill_u32(cm_ctx *ctx, void *p, uint32_t v, int64_t count);
+// count unit weights (read-only, valid for the context's lifetime): graphs
+// without vertex or edge weights share them instead of filling their own.
+const int32_t *unit_weights(cm_ctx *ctx, int64_t count);
 void fill_u64(cm_ctx *ctx, void *p, uint64_t v, int64_t count);
 void scan_excl_i32(cm_ctx *ctx, int64_t n, const int32_t *in, int32_t *out);
 

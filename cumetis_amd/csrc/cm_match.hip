@@ -20,6 +20,13 @@
 namespace cm {
 
 constexpr int LONG_DEG = LONG_ROW;  // rows longer than this are scanned by whole waves
+// On levels with rows longer than LONG_DEG, rows of SHORT_DEG+1 .. LONG_DEG
+// entries ("small mid" rows) are listed too and scanned by 16-lane groups, 4
+// rows at a time: in a thread-per-row wave one such row keeps all 64 lanes
+// waiting through up to LONG_DEG / 4 dependent steps.  Levels without long
+// rows (meshes) keep every row on the thread-per-row path and need no list.
+constexpr int SHORT_DEG = 16;
+constexpr int SUB_LANES = 16;  // lanes per small mid row
 constexpr int LONG_CHUNK = 2048;  // entries per wave of a long row
 constexpr int SELECT_PER_THREAD = 16;
 constexpr int MID_GROUP = 16;  // mid rows per wave of the long-row path
@@ -79,45 +86,55 @@ struct LongChunk {
   int32_t v, first, nch, k;
 };
 
-// count[0] = hub chunks, count[1] = mid rows.  The order of the lists does
-// not matter (the reduction over a row's chunks is exact).
+// count[0] = hub chunks, count[1] = mid rows, count[2] = small mid rows.
+// The order of the lists does not matter (the reduction over a row's chunks
+// is exact).
 template <typename OffT>
 __global__ __launch_bounds__(256) void k_select_long(int64_t n, const OffT *xadj, LongChunk *chunks,
-                                                     int32_t *mids, unsigned *count)
+                                                     int32_t *mids, int32_t *smids, unsigned *count)
 {
   // SELECT_PER_THREAD consecutive vertices per thread and one update of each
   // counter per block, so that blocks do not queue on the counters
-  __shared__ unsigned s_h[4], s_m[4], s_hb, s_mb;
+  __shared__ unsigned s_h[4], s_m[4], s_s[4], s_hb, s_mb, s_sb;
   const int64_t v0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * SELECT_PER_THREAD;
-  unsigned nch[SELECT_PER_THREAD], hub = 0, mid = 0;
+  unsigned nch[SELECT_PER_THREAD], hub = 0, mid = 0, sm = 0;
+  bool small[SELECT_PER_THREAD];
 #pragma unroll
   for (int t = 0; t < SELECT_PER_THREAD; ++t) {
     const int64_t v = v0 + t;
     const int64_t d = v < n ? (int64_t)(xadj[v + 1] - xadj[v]) : 0;
     nch[t] = d > LONG_DEG ? (unsigned)((d + LONG_CHUNK - 1) / LONG_CHUNK) : 0u;
+    small[t] = d > SHORT_DEG && d <= LONG_DEG;
     hub += nch[t] > 1 ? nch[t] : 0u;
     mid += nch[t] == 1 ? 1u : 0u;
+    sm += small[t] ? 1u : 0u;
   }
   const unsigned hinc = wave_incl_scan<unsigned>(hub), minc = wave_incl_scan<unsigned>(mid);
+  const unsigned sinc = wave_incl_scan<unsigned>(sm);
   const int w = threadIdx.x >> 6;
   if (lane() == CM_WAVE - 1) {
     s_h[w] = hinc;
     s_m[w] = minc;
+    s_s[w] = sinc;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
     const unsigned ht = s_h[0] + s_h[1] + s_h[2] + s_h[3], mt = s_m[0] + s_m[1] + s_m[2] + s_m[3];
+    const unsigned st = s_s[0] + s_s[1] + s_s[2] + s_s[3];
     s_hb = ht ? atomicAdd(&count[0], ht) : 0u;
     s_mb = mt ? atomicAdd(&count[1], mt) : 0u;
+    s_sb = st ? atomicAdd(&count[2], st) : 0u;
   }
   __syncthreads();
-  unsigned hb = s_hb + hinc - hub, mb = s_mb + minc - mid;
+  unsigned hb = s_hb + hinc - hub, mb = s_mb + minc - mid, sb = s_sb + sinc - sm;
   for (int i = 0; i < w; ++i) {
     hb += s_h[i];
     mb += s_m[i];
+    sb += s_s[i];
   }
 #pragma unroll
   for (int t = 0; t < SELECT_PER_THREAD; ++t) {
+    if (small[t]) smids[sb++] = (int32_t)(v0 + t);
     if (nch[t] == 1) mids[mb++] = (int32_t)(v0 + t);
     if (nch[t] > 1) {
       for (unsigned k = 0; k < nch[t]; ++k)
@@ -359,13 +376,88 @@ __device__ __forceinline__ void mid_group(int64_t m0, unsigned nm, const int32_t
   }
 }
 
+// Best of lanes within each aligned group of SUB_LANES lanes.
+__device__ __forceinline__ void group_best(int32_t &best, uint64_t &bk1, uint64_t &bk2)
+{
+#pragma unroll
+  for (int d = SUB_LANES / 2; d > 0; d >>= 1) {
+    const int32_t ob = __shfl_xor(best, d, CM_WAVE);
+    const uint64_t o1 = __shfl_xor(bk1, d, CM_WAVE);
+    const uint64_t o2 = __shfl_xor(bk2, d, CM_WAVE);
+    if (ob != -2 && (best == -2 || key_better(o1, o2, bk1, bk2))) {
+      best = ob;
+      bk1 = o1;
+      bk2 = o2;
+    }
+  }
+}
+
+// Small mid rows smids[m0 .. m0+MID_GROUP): the vertex state as in
+// mid_group (lane i for row i), then the rows to scan go CM_WAVE / SUB_LANES
+// at a time, one per group of SUB_LANES lanes (a row of at most LONG_DEG
+// entries is one step of SCAN_BATCH entries per lane).
+template <typename OffT>
+__device__ __forceinline__ void small_group(int64_t m0, unsigned ns, const int32_t *smids,
+                                            const OffT *__restrict__ xadj, const int32_t *__restrict__ adjncy,
+                                            const int32_t *__restrict__ adjwgt, const int32_t *__restrict__ vwgt,
+                                            const MatchOwners &own, const int32_t *match, const int32_t *prev,
+                                            int32_t *cur, bool have_prev, int64_t maxvw)
+{
+  const int l = lane();
+  const int64_t mi = m0 + l;
+  const bool has = l < MID_GROUP && mi < (int64_t)ns;
+  const int32_t v = smids[has ? mi : m0];
+  const int32_t mv = match[v];
+  const int32_t p = have_prev ? prev[v] : -1;
+  const OffT xb = xadj[v], xe = xadj[v + 1];
+  const int32_t vw = vwgt[v];
+  int32_t maxv;
+  uint64_t seed;
+  int64_t base;
+  owner_params(own, v, maxv, seed, base);
+  const int32_t pc = mv == -1 && p >= 0 ? p : v;
+  const int32_t pp = have_prev ? prev[pc] : -1;
+  const int32_t mp = match[pc];
+  const int32_t ppp = have_prev ? prev[pp >= 0 ? pp : pc] : -1;
+  const bool v_acc = have_prev && (p == -2 || (p >= 0 && pp == v));  // accepted_prev(prev, v)
+  const bool p_acc = pp == -2 || (pp >= 0 && ppp == p);
+  const bool live = has && mv == -1 && !v_acc;
+  const bool reuse = p >= 0 && mp == -1 && !p_acc;
+  if (live && reuse) cur[v] = p;  // still unmatched, so still the best
+  const int32_t cw = maxvw < 0 || (int64_t)vw + maxvw > maxv ? 1 : 0;
+  uint64_t todo = __ballot(live && !reuse);
+  const int g = l / SUB_LANES, gl = l % SUB_LANES;
+  while (todo) {
+    // group g takes the g-th row still to do (none if fewer remain)
+    uint64_t t = todo;
+    for (int k = 0; k < g; ++k) t &= t - 1;
+    const int i = t ? __ffsll((unsigned long long)t) - 1 : -1;
+    for (int k = 0; k < CM_WAVE / SUB_LANES; ++k) todo &= todo - 1;
+    const int src = i >= 0 ? i : 0;
+    const int32_t vi = __shfl(v, src, CM_WAVE);
+    const int64_t bi = __shfl((int64_t)xb, src, CM_WAVE), ei = __shfl((int64_t)xe, src, CM_WAVE);
+    const int32_t vwi = __shfl(vw, src, CM_WAVE), maxvi = __shfl(maxv, src, CM_WAVE);
+    const uint64_t seedi = __shfl(seed, src, CM_WAVE);
+    const int64_t basei = __shfl(base, src, CM_WAVE);
+    const int32_t cwi = __shfl(cw, src, CM_WAVE);
+    int32_t best = -2;
+    uint64_t bk1 = 0, bk2 = 0;
+    if (i >= 0)
+      scan_row<OffT>(vi, (OffT)bi, (OffT)ei, gl, SUB_LANES, adjncy, adjwgt, vwgt, vwi, maxvi, seedi, basei, match,
+                     prev, have_prev, cwi != 0, best, bk1, bk2);
+    group_best(best, bk1, bk2);
+    if (i >= 0 && gl == 0) cur[vi] = best;
+  }
+}
+
 // Launch r of the handshake: accept of round r-1 (from prev = proposals of
 // round r-1) and proposals of round r (into cur), over the state after round
 // r-1.  match[] is written only for the thread's own vertex; readers treat a
 // neighbour as taken if match[u] != -1 or it was accepted in round r-1, which
 // gives the same answer whether or not the neighbour's own write has landed.
-// Blocks [0, nshort) take one vertex per thread (rows <= LONG_DEG and all
-// accepts); the remaining blocks take the chunks of long rows, one wave each.
+// Blocks [0, nshort) take one vertex per thread (rows <= short_deg and all
+// accepts); the remaining blocks take the chunks of long rows, one wave each,
+// and groups of mid and small mid rows.
 template <typename OffT>
 __global__ __launch_bounds__(256) void k_round(int64_t n, const OffT *__restrict__ xadj,
                                                const int32_t *__restrict__ adjncy,
@@ -374,8 +466,8 @@ __global__ __launch_bounds__(256) void k_round(int64_t n, const OffT *__restrict
                                                int32_t *match, const int32_t *prev, int32_t *cur,
                                                const LongChunk *chunks, const int32_t *mids,
                                                const unsigned *nchunks,
-                                               LongPartial part, unsigned *flags, int r, unsigned nshort,
-                                               int propose, int64_t maxvw)
+                                               const int32_t *smids, LongPartial part, unsigned *flags, int r,
+                                               unsigned nshort, int propose, int64_t maxvw, int short_deg)
 {
   if (r >= 2 && !round_live(flags, r - 2)) return;
   const bool have_prev = r >= 1;
@@ -412,7 +504,7 @@ __global__ __launch_bounds__(256) void k_round(int64_t n, const OffT *__restrict
       }
       changed = done;
       active = !done;
-      if (!done && propose && (int64_t)(e - b) <= LONG_DEG) {
+      if (!done && propose && (int64_t)(e - b) <= short_deg) {
         // p accepted in round r-1 (from the proposals alone): dead or mutual
         const bool p_acc = pp == -2 || (pp >= 0 && ppp == p);
         if (p >= 0 && mp == -1 && !p_acc) {
@@ -451,16 +543,20 @@ __global__ __launch_bounds__(256) void k_round(int64_t n, const OffT *__restrict
   // groups of MID_GROUP mid rows (one wave per group).  Whether a row is
   // scanned depends only on state fixed before this launch (see above), so
   // all chunks of a row agree.
-  const unsigned nh = nchunks[0], nm = nchunks[1];
-  const int64_t items = (int64_t)nh + ((int64_t)nm + MID_GROUP - 1) / MID_GROUP;
+  const unsigned nh = nchunks[0], nm = nchunks[1], ns = nchunks[2];
+  const int64_t gm = ((int64_t)nm + MID_GROUP - 1) / MID_GROUP;
+  const int64_t items = (int64_t)nh + gm + ((int64_t)ns + MID_GROUP - 1) / MID_GROUP;
   const int64_t nw = (int64_t)(gridDim.x - nshort) * (blockDim.x / CM_WAVE);
   for (int64_t it = ((int64_t)(blockIdx.x - nshort) * blockDim.x + threadIdx.x) / CM_WAVE; it < items; it += nw) {
     if (it < nh)
       hub_chunk<OffT>(it, chunks, xadj, adjncy, adjwgt, vwgt, own, match, prev, cur, part, have_prev, maxvw,
                       (unsigned)r + 1u);
-    else
+    else if (it < nh + gm)
       mid_group<OffT>((it - nh) * MID_GROUP, nm, mids, xadj, adjncy, adjwgt, vwgt, own, match, prev, cur,
                       have_prev, maxvw);
+    else
+      small_group<OffT>((it - nh - gm) * MID_GROUP, ns, smids, xadj, adjncy, adjwgt, vwgt, own, match, prev, cur,
+                        have_prev, maxvw);
   }
 }
 
@@ -486,6 +582,8 @@ struct MatchRun {
   int32_t *prop = nullptr;
   LongChunk *chunks = nullptr;  // hub chunks
   int32_t *mids = nullptr;      // mid rows
+  int32_t *smids = nullptr;     // small mid rows
+  int short_deg = LONG_DEG;     // longest row on the thread-per-row path
   LongPartial part{};
   unsigned *flags = nullptr, *nchunks = nullptr;
   unsigned g = 0, glong = 0;
@@ -512,28 +610,32 @@ static void match_start(cm_ctx *ctx, MatchRun &mr)
   // row (each hub row holds more than LONG_CHUNK entries)
   const int64_t cap = has_long ? 2 * (L.m / LONG_CHUNK) + 1 : 1;
   const int64_t cap_mid = has_long ? std::min<int64_t>(n, L.m / (LONG_DEG + 1)) + 1 : 1;
+  const int64_t cap_small = has_long ? std::min<int64_t>(n, L.m / (SHORT_DEG + 1)) + 1 : 1;
+  mr.short_deg = has_long ? SHORT_DEG : LONG_DEG;
   mr.chunks = (LongChunk *)al(sizeof(LongChunk) * cap);
   mr.mids = (int32_t *)al(sizeof(int32_t) * cap_mid);
+  mr.smids = (int32_t *)al(sizeof(int32_t) * cap_small);
   mr.part.best = (int32_t *)al(sizeof(int32_t) * cap);
   mr.part.k1 = (uint64_t *)al(sizeof(uint64_t) * cap);
   mr.part.k2 = (uint64_t *)al(sizeof(uint64_t) * cap);
   mr.part.scanned = (unsigned *)al(sizeof(unsigned) * cap);
-  mr.flags = (unsigned *)al(sizeof(unsigned) * (2 * (size_t)mr.max_rounds + 2));
-  mr.nchunks = mr.flags + 2 * (size_t)mr.max_rounds;  // [0] hub chunks, [1] mid rows
-  if (has_long) fill_u32(ctx, mr.part.scanned, 0u, cap);  // no launch number is 0
+  mr.flags = (unsigned *)al(sizeof(unsigned) * (2 * (size_t)mr.max_rounds + 4));
+  mr.nchunks = mr.flags + 2 * (size_t)mr.max_rounds;  // [0] hub chunks, [1] mid rows, [2] small mid rows
+  const bool has_hub = L.maxdeg < 0 || L.maxdeg > LONG_CHUNK;
+  if (has_hub) fill_u32(ctx, mr.part.scanned, 0u, cap);  // no launch number is 0
   mr.g = nblocks(n, 256);
   launch(ctx, "match_init", 8.0 * n, k_match_init, dim3(mr.g), dim3(256), 0, n, mr.match, mr.prop, mr.flags,
-         2 * (int64_t)mr.max_rounds + 2, ctx->d_ctr + HASH_CTR_SLOT);
+         2 * (int64_t)mr.max_rounds + 4, ctx->d_ctr + HASH_CTR_SLOT);
   ctx->hash_ctr_zero = true;
   // extra blocks whose waves stride over the long-row chunks (none when the
   // level is known to have no long row)
   if (has_long) {
     launch(ctx, "match_select_long", (double)sizeof(OffT) * (n + 1), k_select_long<OffT>,
-           dim3(nblocks(n, 256 * SELECT_PER_THREAD)), dim3(256), 0, n, xadj, mr.chunks, mr.mids,
+           dim3(nblocks(n, 256 * SELECT_PER_THREAD)), dim3(256), 0, n, xadj, mr.chunks, mr.mids, mr.smids,
            mr.nchunks);
-    const int64_t items = cap + (cap_mid + MID_GROUP - 1) / MID_GROUP;  // a wave each
-    mr.glong = (unsigned)std::min<int64_t>(8192, std::max<int64_t>(16, (items + 3) / 4));
-    if (L.maxdeg < 0 || L.maxdeg > LONG_CHUNK) mr.ghub = (unsigned)std::min<int64_t>(4096, (cap + 3) / 4);
+    const int64_t items = cap + (cap_mid + MID_GROUP - 1) / MID_GROUP + (cap_small + MID_GROUP - 1) / MID_GROUP;
+    mr.glong = (unsigned)std::min<int64_t>(8192, std::max<int64_t>(16, (items + 3) / 4));  // a wave each
+    if (has_hub) mr.ghub = (unsigned)std::min<int64_t>(4096, (cap + 3) / 4);
   }
 }
 
@@ -551,9 +653,8 @@ static void match_launch(cm_ctx *ctx, MatchRun &mr, int count)
     const int32_t *prv = mr.prop + (size_t)((r + 1) & 1) * n;
     launch(ctx, "match_round", r == 0 ? row_bytes : 0.0, k_round<OffT>, dim3(mr.g + mr.glong), dim3(256), 0, n,
            (const OffT *)L.xadj, L.adjncy, L.adjwgt, L.vwgt, mr.own, mr.match, prv, cur,
-           (const LongChunk *)mr.chunks, (const int32_t *)mr.mids, (const unsigned *)mr.nchunks, mr.part,
-           mr.flags, r, mr.g,
-           r < mr.max_rounds ? 1 : 0, L.maxvw);
+           (const LongChunk *)mr.chunks, (const int32_t *)mr.mids, (const unsigned *)mr.nchunks,
+           (const int32_t *)mr.smids, mr.part, mr.flags, r, mr.g, r < mr.max_rounds ? 1 : 0, L.maxvw, mr.short_deg);
     if (mr.ghub && r < mr.max_rounds)
       launch(ctx, "match_hub_reduce", 0.0, k_hub_reduce, dim3(mr.ghub), dim3(256), 0,
              (const LongChunk *)mr.chunks, (const unsigned *)mr.nchunks, mr.part, cur, (const unsigned *)mr.flags,

@@ -95,6 +95,34 @@ def test_match_split_rows(engine, orc, unit):
         assert r_gpu == r_orc
 
 
+def _symmetric_weights(g, wmax, seed):
+    """Edge weights that depend only on the unordered pair, vertex weights 1..3."""
+    src = np.repeat(np.arange(g.n, dtype=np.int64), np.diff(g.xadj.astype(np.int64)))
+    a = np.minimum(src, g.adjncy.astype(np.int64))
+    b = np.maximum(src, g.adjncy.astype(np.int64))
+    aw = (1 + (a * 7919 + b * 104729 + seed) % wmax).astype(np.int32)
+    vw = np.random.default_rng(seed).integers(1, 4, size=g.n).astype(np.int32)
+    return Graph(g.xadj, g.adjncy, aw, vw)
+
+
+@pytest.mark.parametrize("wmax", [1, 7, 1000])
+def test_match_small_mid_rows(engine, orc, wmax):
+    """A skewed graph (rows above 64 entries exist), so rows of 17..64
+    entries go to the 16-lane group path, with edge weights that prune by
+    weight (wmax 7: many ties broken by the hash; 1000: few ties) and vertex
+    weights that make some pairs too heavy."""
+    g = _symmetric_weights(graphs.rmat(12, 16, seed=5, offsets=np.int32), wmax, seed=wmax)
+    deg = np.diff(g.xadj)
+    assert deg.max() > 64 and ((deg > 16) & (deg <= 64)).sum() > 100
+    say(f"[match small mid rows] wmax={wmax} n={g.n} m={g.m}")
+    mv = maxv_for(g, orc=orc)
+    for seed in (2, 31):
+        m_gpu, r_gpu = engine.match(g, mv, seed=seed)
+        m_orc, r_orc = orc.handshake_match(g.xadj, g.adjncy, g.adjwgt, g.vwgt, mv, seed=seed)
+        np.testing.assert_array_equal(m_gpu, m_orc)
+        assert r_gpu == r_orc
+
+
 @pytest.mark.parametrize("hub_deg,side", [(140000, 400), (600000, 800)])
 def test_contract_hub_partitions(engine, orc, hub_deg, side):
     """Hub segments cut into many key partitions (35 and 147 blocks each
