@@ -26,7 +26,9 @@ constexpr int LONG_DEG = LONG_ROW;  // rows longer than this are scanned by whol
 // waiting through up to LONG_DEG / 4 dependent steps.  Levels without long
 // rows (meshes) keep every row on the thread-per-row path and need no list.
 constexpr int SHORT_DEG = 16;
-constexpr int SUB_LANES = 16;  // lanes per small mid row
+// Likewise rows of LONG_DEG+1 .. HALF_DEG entries go to 32-lane groups, 2
+// rows at a time; longer ones are scanned by whole waves.
+constexpr int HALF_DEG = 128;
 constexpr int LONG_CHUNK = 2048;  // entries per wave of a long row
 constexpr int SELECT_PER_THREAD = 16;
 constexpr int MID_GROUP = 16;  // mid rows per wave of the long-row path
@@ -77,65 +79,71 @@ __global__ __launch_bounds__(256) void k_match_init(int64_t n, int32_t *match, i
   for (int64_t i = v; i < nflags; i += (int64_t)gridDim.x * blockDim.x) flags[i] = 0u;
 }
 
-// Rows longer than LONG_DEG are scanned by whole waves.  Rows of at most
-// LONG_CHUNK entries ("mid" rows) are listed by vertex; longer rows ("hubs")
-// are cut into chunks of LONG_CHUNK entries, one wave per chunk.  Chunk
-// descriptor: the row, its first chunk, its chunk count and the chunk's index
-// within the row.
+// Rows longer than LONG_DEG are scanned by lane groups (up to HALF_DEG
+// entries) or whole waves.  Rows of at most LONG_CHUNK entries ("mid" rows)
+// are listed by vertex; longer rows ("hubs") are cut into chunks of
+// LONG_CHUNK entries, one wave per chunk.  Chunk descriptor: the row, its
+// first chunk, its chunk count and the chunk's index within the row.
 struct LongChunk {
   int32_t v, first, nch, k;
 };
 
-// count[0] = hub chunks, count[1] = mid rows, count[2] = small mid rows.
-// The order of the lists does not matter (the reduction over a row's chunks
-// is exact).
+// count[0] = hub chunks, count[1] = mid rows, count[2] = small mid rows
+// (16-lane groups), count[3] = rows for 32-lane groups.  The order of the
+// lists does not matter (the reduction over a row's chunks is exact).
 template <typename OffT>
 __global__ __launch_bounds__(256) void k_select_long(int64_t n, const OffT *xadj, LongChunk *chunks,
-                                                     int32_t *mids, int32_t *smids, unsigned *count)
+                                                     int32_t *mids, int32_t *smids, int32_t *hmids,
+                                                     unsigned *count)
 {
   // SELECT_PER_THREAD consecutive vertices per thread and one update of each
   // counter per block, so that blocks do not queue on the counters
-  __shared__ unsigned s_h[4], s_m[4], s_s[4], s_hb, s_mb, s_sb;
+  __shared__ unsigned s_h[4], s_m[4], s_s[4], s_f[4], s_hb, s_mb, s_sb, s_fb;
   const int64_t v0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * SELECT_PER_THREAD;
-  unsigned nch[SELECT_PER_THREAD], hub = 0, mid = 0, sm = 0;
-  bool small[SELECT_PER_THREAD];
+  unsigned nch[SELECT_PER_THREAD], hub = 0, mid = 0, sm = 0, hm = 0;
+  uint8_t cls[SELECT_PER_THREAD];  // 1: 16-lane group, 2: 32-lane group, 3: whole wave
 #pragma unroll
   for (int t = 0; t < SELECT_PER_THREAD; ++t) {
     const int64_t v = v0 + t;
     const int64_t d = v < n ? (int64_t)(xadj[v + 1] - xadj[v]) : 0;
     nch[t] = d > LONG_DEG ? (unsigned)((d + LONG_CHUNK - 1) / LONG_CHUNK) : 0u;
-    small[t] = d > SHORT_DEG && d <= LONG_DEG;
+    cls[t] = d <= SHORT_DEG ? 0 : d <= LONG_DEG ? 1 : d <= HALF_DEG ? 2 : nch[t] == 1 ? 3 : 0;
     hub += nch[t] > 1 ? nch[t] : 0u;
-    mid += nch[t] == 1 ? 1u : 0u;
-    sm += small[t] ? 1u : 0u;
+    mid += cls[t] == 3 ? 1u : 0u;
+    sm += cls[t] == 1 ? 1u : 0u;
+    hm += cls[t] == 2 ? 1u : 0u;
   }
   const unsigned hinc = wave_incl_scan<unsigned>(hub), minc = wave_incl_scan<unsigned>(mid);
-  const unsigned sinc = wave_incl_scan<unsigned>(sm);
+  const unsigned sinc = wave_incl_scan<unsigned>(sm), finc = wave_incl_scan<unsigned>(hm);
   const int w = threadIdx.x >> 6;
   if (lane() == CM_WAVE - 1) {
     s_h[w] = hinc;
     s_m[w] = minc;
     s_s[w] = sinc;
+    s_f[w] = finc;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
     const unsigned ht = s_h[0] + s_h[1] + s_h[2] + s_h[3], mt = s_m[0] + s_m[1] + s_m[2] + s_m[3];
-    const unsigned st = s_s[0] + s_s[1] + s_s[2] + s_s[3];
+    const unsigned st = s_s[0] + s_s[1] + s_s[2] + s_s[3], ft = s_f[0] + s_f[1] + s_f[2] + s_f[3];
     s_hb = ht ? atomicAdd(&count[0], ht) : 0u;
     s_mb = mt ? atomicAdd(&count[1], mt) : 0u;
     s_sb = st ? atomicAdd(&count[2], st) : 0u;
+    s_fb = ft ? atomicAdd(&count[3], ft) : 0u;
   }
   __syncthreads();
-  unsigned hb = s_hb + hinc - hub, mb = s_mb + minc - mid, sb = s_sb + sinc - sm;
+  unsigned hb = s_hb + hinc - hub, mb = s_mb + minc - mid, sb = s_sb + sinc - sm, fb = s_fb + finc - hm;
   for (int i = 0; i < w; ++i) {
     hb += s_h[i];
     mb += s_m[i];
     sb += s_s[i];
+    fb += s_f[i];
   }
 #pragma unroll
   for (int t = 0; t < SELECT_PER_THREAD; ++t) {
-    if (small[t]) smids[sb++] = (int32_t)(v0 + t);
-    if (nch[t] == 1) mids[mb++] = (int32_t)(v0 + t);
+    if (cls[t] == 1) smids[sb++] = (int32_t)(v0 + t);
+    if (cls[t] == 2) hmids[fb++] = (int32_t)(v0 + t);
+    if (cls[t] == 3) mids[mb++] = (int32_t)(v0 + t);
     if (nch[t] > 1) {
       for (unsigned k = 0; k < nch[t]; ++k)
         chunks[hb + k] = LongChunk{(int32_t)(v0 + t), (int32_t)hb, (int32_t)nch[t], (int32_t)k};
@@ -376,11 +384,12 @@ __device__ __forceinline__ void mid_group(int64_t m0, unsigned nm, const int32_t
   }
 }
 
-// Best of lanes within each aligned group of SUB_LANES lanes.
+// Best of lanes within each aligned group of SUB lanes.
+template <int SUB>
 __device__ __forceinline__ void group_best(int32_t &best, uint64_t &bk1, uint64_t &bk2)
 {
 #pragma unroll
-  for (int d = SUB_LANES / 2; d > 0; d >>= 1) {
+  for (int d = SUB / 2; d > 0; d >>= 1) {
     const int32_t ob = __shfl_xor(best, d, CM_WAVE);
     const uint64_t o1 = __shfl_xor(bk1, d, CM_WAVE);
     const uint64_t o2 = __shfl_xor(bk2, d, CM_WAVE);
@@ -392,11 +401,11 @@ __device__ __forceinline__ void group_best(int32_t &best, uint64_t &bk1, uint64_
   }
 }
 
-// Small mid rows smids[m0 .. m0+MID_GROUP): the vertex state as in
-// mid_group (lane i for row i), then the rows to scan go CM_WAVE / SUB_LANES
-// at a time, one per group of SUB_LANES lanes (a row of at most LONG_DEG
-// entries is one step of SCAN_BATCH entries per lane).
-template <typename OffT>
+// Rows smids[m0 .. m0+MID_GROUP) of at most 4 * SUB entries: the vertex
+// state as in mid_group (lane i for row i), then the rows to scan go
+// CM_WAVE / SUB at a time, one per group of SUB lanes (such a row is one step
+// of SCAN_BATCH entries per lane).
+template <int SUB, typename OffT>
 __device__ __forceinline__ void small_group(int64_t m0, unsigned ns, const int32_t *smids,
                                             const OffT *__restrict__ xadj, const int32_t *__restrict__ adjncy,
                                             const int32_t *__restrict__ adjwgt, const int32_t *__restrict__ vwgt,
@@ -426,13 +435,13 @@ __device__ __forceinline__ void small_group(int64_t m0, unsigned ns, const int32
   if (live && reuse) cur[v] = p;  // still unmatched, so still the best
   const int32_t cw = maxvw < 0 || (int64_t)vw + maxvw > maxv ? 1 : 0;
   uint64_t todo = __ballot(live && !reuse);
-  const int g = l / SUB_LANES, gl = l % SUB_LANES;
+  const int g = l / SUB, gl = l % SUB;
   while (todo) {
     // group g takes the g-th row still to do (none if fewer remain)
     uint64_t t = todo;
     for (int k = 0; k < g; ++k) t &= t - 1;
     const int i = t ? __ffsll((unsigned long long)t) - 1 : -1;
-    for (int k = 0; k < CM_WAVE / SUB_LANES; ++k) todo &= todo - 1;
+    for (int k = 0; k < CM_WAVE / SUB; ++k) todo &= todo - 1;
     const int src = i >= 0 ? i : 0;
     const int32_t vi = __shfl(v, src, CM_WAVE);
     const int64_t bi = __shfl((int64_t)xb, src, CM_WAVE), ei = __shfl((int64_t)xe, src, CM_WAVE);
@@ -443,9 +452,9 @@ __device__ __forceinline__ void small_group(int64_t m0, unsigned ns, const int32
     int32_t best = -2;
     uint64_t bk1 = 0, bk2 = 0;
     if (i >= 0)
-      scan_row<OffT>(vi, (OffT)bi, (OffT)ei, gl, SUB_LANES, adjncy, adjwgt, vwgt, vwi, maxvi, seedi, basei, match,
-                     prev, have_prev, cwi != 0, best, bk1, bk2);
-    group_best(best, bk1, bk2);
+      scan_row<OffT>(vi, (OffT)bi, (OffT)ei, gl, SUB, adjncy, adjwgt, vwgt, vwi, maxvi, seedi, basei, match, prev,
+                     have_prev, cwi != 0, best, bk1, bk2);
+    group_best<SUB>(best, bk1, bk2);
     if (i >= 0 && gl == 0) cur[vi] = best;
   }
 }
@@ -466,7 +475,8 @@ __global__ __launch_bounds__(256) void k_round(int64_t n, const OffT *__restrict
                                                int32_t *match, const int32_t *prev, int32_t *cur,
                                                const LongChunk *chunks, const int32_t *mids,
                                                const unsigned *nchunks,
-                                               const int32_t *smids, LongPartial part, unsigned *flags, int r,
+                                               const int32_t *smids, const int32_t *hmids, LongPartial part,
+                                               unsigned *flags, int r,
                                                unsigned nshort, int propose, int64_t maxvw, int short_deg)
 {
   if (r >= 2 && !round_live(flags, r - 2)) return;
@@ -543,9 +553,9 @@ __global__ __launch_bounds__(256) void k_round(int64_t n, const OffT *__restrict
   // groups of MID_GROUP mid rows (one wave per group).  Whether a row is
   // scanned depends only on state fixed before this launch (see above), so
   // all chunks of a row agree.
-  const unsigned nh = nchunks[0], nm = nchunks[1], ns = nchunks[2];
-  const int64_t gm = ((int64_t)nm + MID_GROUP - 1) / MID_GROUP;
-  const int64_t items = (int64_t)nh + gm + ((int64_t)ns + MID_GROUP - 1) / MID_GROUP;
+  const unsigned nh = nchunks[0], nm = nchunks[1], ns = nchunks[2], nf = nchunks[3];
+  const int64_t gm = ((int64_t)nm + MID_GROUP - 1) / MID_GROUP, gs = ((int64_t)ns + MID_GROUP - 1) / MID_GROUP;
+  const int64_t items = (int64_t)nh + gm + gs + ((int64_t)nf + MID_GROUP - 1) / MID_GROUP;
   const int64_t nw = (int64_t)(gridDim.x - nshort) * (blockDim.x / CM_WAVE);
   for (int64_t it = ((int64_t)(blockIdx.x - nshort) * blockDim.x + threadIdx.x) / CM_WAVE; it < items; it += nw) {
     if (it < nh)
@@ -554,9 +564,12 @@ __global__ __launch_bounds__(256) void k_round(int64_t n, const OffT *__restrict
     else if (it < nh + gm)
       mid_group<OffT>((it - nh) * MID_GROUP, nm, mids, xadj, adjncy, adjwgt, vwgt, own, match, prev, cur,
                       have_prev, maxvw);
+    else if (it < nh + gm + gs)
+      small_group<16, OffT>((it - nh - gm) * MID_GROUP, ns, smids, xadj, adjncy, adjwgt, vwgt, own, match, prev,
+                            cur, have_prev, maxvw);
     else
-      small_group<OffT>((it - nh - gm) * MID_GROUP, ns, smids, xadj, adjncy, adjwgt, vwgt, own, match, prev, cur,
-                        have_prev, maxvw);
+      small_group<32, OffT>((it - nh - gm - gs) * MID_GROUP, nf, hmids, xadj, adjncy, adjwgt, vwgt, own, match,
+                            prev, cur, have_prev, maxvw);
   }
 }
 
@@ -582,7 +595,8 @@ struct MatchRun {
   int32_t *prop = nullptr;
   LongChunk *chunks = nullptr;  // hub chunks
   int32_t *mids = nullptr;      // mid rows
-  int32_t *smids = nullptr;     // small mid rows
+  int32_t *smids = nullptr;     // small mid rows (16-lane groups)
+  int32_t *hmids = nullptr;     // rows for 32-lane groups
   int short_deg = LONG_DEG;     // longest row on the thread-per-row path
   LongPartial part{};
   unsigned *flags = nullptr, *nchunks = nullptr;
@@ -611,16 +625,18 @@ static void match_start(cm_ctx *ctx, MatchRun &mr)
   const int64_t cap = has_long ? 2 * (L.m / LONG_CHUNK) + 1 : 1;
   const int64_t cap_mid = has_long ? std::min<int64_t>(n, L.m / (LONG_DEG + 1)) + 1 : 1;
   const int64_t cap_small = has_long ? std::min<int64_t>(n, L.m / (SHORT_DEG + 1)) + 1 : 1;
+  const int64_t cap_half = has_long ? std::min<int64_t>(n, L.m / (LONG_DEG + 1)) + 1 : 1;
   mr.short_deg = has_long ? SHORT_DEG : LONG_DEG;
   mr.chunks = (LongChunk *)al(sizeof(LongChunk) * cap);
   mr.mids = (int32_t *)al(sizeof(int32_t) * cap_mid);
   mr.smids = (int32_t *)al(sizeof(int32_t) * cap_small);
+  mr.hmids = (int32_t *)al(sizeof(int32_t) * cap_half);
   mr.part.best = (int32_t *)al(sizeof(int32_t) * cap);
   mr.part.k1 = (uint64_t *)al(sizeof(uint64_t) * cap);
   mr.part.k2 = (uint64_t *)al(sizeof(uint64_t) * cap);
   mr.part.scanned = (unsigned *)al(sizeof(unsigned) * cap);
   mr.flags = (unsigned *)al(sizeof(unsigned) * (2 * (size_t)mr.max_rounds + 4));
-  mr.nchunks = mr.flags + 2 * (size_t)mr.max_rounds;  // [0] hub chunks, [1] mid rows, [2] small mid rows
+  mr.nchunks = mr.flags + 2 * (size_t)mr.max_rounds;  // [0] hub chunks, [1..3] mid rows by group width
   const bool has_hub = L.maxdeg < 0 || L.maxdeg > LONG_CHUNK;
   if (has_hub) fill_u32(ctx, mr.part.scanned, 0u, cap);  // no launch number is 0
   mr.g = nblocks(n, 256);
@@ -632,8 +648,9 @@ static void match_start(cm_ctx *ctx, MatchRun &mr)
   if (has_long) {
     launch(ctx, "match_select_long", (double)sizeof(OffT) * (n + 1), k_select_long<OffT>,
            dim3(nblocks(n, 256 * SELECT_PER_THREAD)), dim3(256), 0, n, xadj, mr.chunks, mr.mids, mr.smids,
-           mr.nchunks);
-    const int64_t items = cap + (cap_mid + MID_GROUP - 1) / MID_GROUP + (cap_small + MID_GROUP - 1) / MID_GROUP;
+           mr.hmids, mr.nchunks);
+    const int64_t items = cap + (cap_mid + MID_GROUP - 1) / MID_GROUP + (cap_small + MID_GROUP - 1) / MID_GROUP +
+                          (cap_half + MID_GROUP - 1) / MID_GROUP;
     mr.glong = (unsigned)std::min<int64_t>(8192, std::max<int64_t>(16, (items + 3) / 4));  // a wave each
     if (has_hub) mr.ghub = (unsigned)std::min<int64_t>(4096, (cap + 3) / 4);
   }
@@ -654,7 +671,8 @@ static void match_launch(cm_ctx *ctx, MatchRun &mr, int count)
     launch(ctx, "match_round", r == 0 ? row_bytes : 0.0, k_round<OffT>, dim3(mr.g + mr.glong), dim3(256), 0, n,
            (const OffT *)L.xadj, L.adjncy, L.adjwgt, L.vwgt, mr.own, mr.match, prv, cur,
            (const LongChunk *)mr.chunks, (const int32_t *)mr.mids, (const unsigned *)mr.nchunks,
-           (const int32_t *)mr.smids, mr.part, mr.flags, r, mr.g, r < mr.max_rounds ? 1 : 0, L.maxvw, mr.short_deg);
+           (const int32_t *)mr.smids, (const int32_t *)mr.hmids, mr.part, mr.flags, r, mr.g, r < mr.max_rounds ? 1 : 0,
+           L.maxvw, mr.short_deg);
     if (mr.ghub && r < mr.max_rounds)
       launch(ctx, "match_hub_reduce", 0.0, k_hub_reduce, dim3(mr.ghub), dim3(256), 0,
              (const LongChunk *)mr.chunks, (const unsigned *)mr.nchunks, mr.part, cur, (const unsigned *)mr.flags,

@@ -108,12 +108,13 @@ def _symmetric_weights(g, wmax, seed):
 @pytest.mark.parametrize("wmax", [1, 7, 1000])
 def test_match_small_mid_rows(engine, orc, wmax):
     """A skewed graph (rows above 64 entries exist), so rows of 17..64
-    entries go to the 16-lane group path, with edge weights that prune by
-    weight (wmax 7: many ties broken by the hash; 1000: few ties) and vertex
-    weights that make some pairs too heavy."""
+    entries go to the 16-lane group path and rows of 65..128 to the 32-lane
+    one, with edge weights that prune by weight (wmax 7: many ties broken by
+    the hash; 1000: few ties) and vertex weights that make some pairs too
+    heavy."""
     g = _symmetric_weights(graphs.rmat(12, 16, seed=5, offsets=np.int32), wmax, seed=wmax)
     deg = np.diff(g.xadj)
-    assert deg.max() > 64 and ((deg > 16) & (deg <= 64)).sum() > 100
+    assert deg.max() > 128 and ((deg > 16) & (deg <= 64)).sum() > 100 and ((deg > 64) & (deg <= 128)).sum() > 50
     say(f"[match small mid rows] wmax={wmax} n={g.n} m={g.m}")
     mv = maxv_for(g, orc=orc)
     for seed in (2, 31):
