@@ -19,17 +19,30 @@
 
 namespace cm {
 
-constexpr int LONG_DEG = LONG_ROW;  // rows longer than this are scanned by whole waves
-// On levels with rows longer than LONG_DEG, rows of SHORT_DEG+1 .. LONG_DEG
-// entries ("small mid" rows) are listed too and scanned by 16-lane groups, 4
-// rows at a time: in a thread-per-row wave one such row keeps all 64 lanes
-// waiting through up to LONG_DEG / 4 dependent steps.  Levels without long
-// rows (meshes) keep every row on the thread-per-row path and need no list.
-constexpr int SHORT_DEG = 16;
-// Likewise rows of LONG_DEG+1 .. HALF_DEG entries go to 32-lane groups, 2
-// rows at a time; longer ones are scanned by whole waves.
-constexpr int HALF_DEG = 128;
-constexpr int LONG_CHUNK = 2048;  // entries per wave of a long row
+constexpr int LONG_DEG = LONG_ROW;  // levels with longer rows list their rows by class
+constexpr int LONG_CHUNK = 2048;    // entries per wave of a hub row
+// On levels with rows longer than LONG_DEG, every row above SHORT_DEG
+// entries is listed by class and scanned by a group of lanes: in a
+// thread-per-row wave one long row keeps all 64 lanes waiting through its
+// dependent steps.  Class k takes rows of (class_lo(k), class_hi(k)] entries,
+// class_lanes(k) lanes per row: classes 0..2 scan a row in one step of
+// SCAN_BATCH entries per lane, CM_WAVE / class_lanes(k) rows at a time;
+// class 3 ("mid" rows, up to LONG_CHUNK entries) scans one row per wave.
+// Levels without long rows (meshes) keep every row on the thread-per-row
+// path and need no lists.
+constexpr int SHORT_DEG = 8;
+constexpr int NCLASS = 4;
+constexpr int MID_CLASS = 3;
+__host__ __device__ constexpr int class_lanes(int k) { return k == 0 ? 4 : k == 1 ? 16 : k == 2 ? 32 : CM_WAVE; }
+__host__ __device__ constexpr int class_hi(int k) { return k == MID_CLASS ? LONG_CHUNK : 4 * class_lanes(k); }
+__host__ __device__ constexpr int class_lo(int k) { return k == 0 ? SHORT_DEG : class_hi(k - 1); }
+static_assert(class_hi(0) == 16 && class_hi(1) == 64 && class_hi(2) == 128, "row classes");
+static_assert(class_hi(1) == LONG_DEG, "classes 0..1 cover the rows up to LONG_DEG");
+
+// The row lists of the classes (vertex ids).
+struct RowLists {
+  int32_t *rows[NCLASS];
+};
 constexpr int SELECT_PER_THREAD = 16;
 constexpr int MID_GROUP = 16;  // mid rows per wave of the long-row path
 constexpr int ROUND_BATCH = MATCH_BATCH;  // launches queued between host checks
@@ -88,66 +101,63 @@ struct LongChunk {
   int32_t v, first, nch, k;
 };
 
-// count[0] = hub chunks, count[1] = mid rows, count[2] = small mid rows
-// (16-lane groups), count[3] = rows for 32-lane groups.  The order of the
+// count[0] = hub chunks, count[1 + k] = rows of class k.  The order of the
 // lists does not matter (the reduction over a row's chunks is exact).
 template <typename OffT>
-__global__ __launch_bounds__(256) void k_select_long(int64_t n, const OffT *xadj, LongChunk *chunks,
-                                                     int32_t *mids, int32_t *smids, int32_t *hmids,
+__global__ __launch_bounds__(256) void k_select_long(int64_t n, const OffT *xadj, LongChunk *chunks, RowLists rl,
                                                      unsigned *count)
 {
   // SELECT_PER_THREAD consecutive vertices per thread and one update of each
-  // counter per block, so that blocks do not queue on the counters
-  __shared__ unsigned s_h[4], s_m[4], s_s[4], s_f[4], s_hb, s_mb, s_sb, s_fb;
+  // counter per block, so that blocks do not queue on the counters.  Every
+  // per-class array is indexed by unrolled loop counters only.
+  constexpr int NC = NCLASS + 1;  // [0] hub chunks, [1 + k] class k
+  __shared__ unsigned s_w[NC][4], s_b[NC];
   const int64_t v0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * SELECT_PER_THREAD;
-  unsigned nch[SELECT_PER_THREAD], hub = 0, mid = 0, sm = 0, hm = 0;
-  uint8_t cls[SELECT_PER_THREAD];  // 1: 16-lane group, 2: 32-lane group, 3: whole wave
+  unsigned nch[SELECT_PER_THREAD], cnt[NC] = {};
+  int cls[SELECT_PER_THREAD];  // -1: thread-per-row or hub
 #pragma unroll
   for (int t = 0; t < SELECT_PER_THREAD; ++t) {
     const int64_t v = v0 + t;
     const int64_t d = v < n ? (int64_t)(xadj[v + 1] - xadj[v]) : 0;
-    nch[t] = d > LONG_DEG ? (unsigned)((d + LONG_CHUNK - 1) / LONG_CHUNK) : 0u;
-    cls[t] = d <= SHORT_DEG ? 0 : d <= LONG_DEG ? 1 : d <= HALF_DEG ? 2 : nch[t] == 1 ? 3 : 0;
-    hub += nch[t] > 1 ? nch[t] : 0u;
-    mid += cls[t] == 3 ? 1u : 0u;
-    sm += cls[t] == 1 ? 1u : 0u;
-    hm += cls[t] == 2 ? 1u : 0u;
+    nch[t] = d > LONG_CHUNK ? (unsigned)((d + LONG_CHUNK - 1) / LONG_CHUNK) : 0u;
+    cls[t] = -1;
+#pragma unroll
+    for (int k = NCLASS - 1; k >= 0; --k)
+      if (d > class_lo(k) && d <= class_hi(k)) cls[t] = k;
+    cnt[0] += nch[t];
+#pragma unroll
+    for (int k = 0; k < NCLASS; ++k) cnt[1 + k] += cls[t] == k ? 1u : 0u;
   }
-  const unsigned hinc = wave_incl_scan<unsigned>(hub), minc = wave_incl_scan<unsigned>(mid);
-  const unsigned sinc = wave_incl_scan<unsigned>(sm), finc = wave_incl_scan<unsigned>(hm);
+  unsigned inc[NC];
   const int w = threadIdx.x >> 6;
-  if (lane() == CM_WAVE - 1) {
-    s_h[w] = hinc;
-    s_m[w] = minc;
-    s_s[w] = sinc;
-    s_f[w] = finc;
+#pragma unroll
+  for (int i = 0; i < NC; ++i) {
+    inc[i] = wave_incl_scan<unsigned>(cnt[i]);
+    if (lane() == CM_WAVE - 1) s_w[i][w] = inc[i];
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    const unsigned ht = s_h[0] + s_h[1] + s_h[2] + s_h[3], mt = s_m[0] + s_m[1] + s_m[2] + s_m[3];
-    const unsigned st = s_s[0] + s_s[1] + s_s[2] + s_s[3], ft = s_f[0] + s_f[1] + s_f[2] + s_f[3];
-    s_hb = ht ? atomicAdd(&count[0], ht) : 0u;
-    s_mb = mt ? atomicAdd(&count[1], mt) : 0u;
-    s_sb = st ? atomicAdd(&count[2], st) : 0u;
-    s_fb = ft ? atomicAdd(&count[3], ft) : 0u;
+  if (threadIdx.x < NC) {
+    const int i = threadIdx.x;
+    const unsigned tot = s_w[i][0] + s_w[i][1] + s_w[i][2] + s_w[i][3];
+    s_b[i] = tot ? atomicAdd(&count[i], tot) : 0u;
   }
   __syncthreads();
-  unsigned hb = s_hb + hinc - hub, mb = s_mb + minc - mid, sb = s_sb + sinc - sm, fb = s_fb + finc - hm;
-  for (int i = 0; i < w; ++i) {
-    hb += s_h[i];
-    mb += s_m[i];
-    sb += s_s[i];
-    fb += s_f[i];
+  unsigned at[NC];
+#pragma unroll
+  for (int i = 0; i < NC; ++i) {
+    at[i] = s_b[i] + inc[i] - cnt[i];
+    for (int j = 0; j < w; ++j) at[i] += s_w[i][j];
   }
 #pragma unroll
   for (int t = 0; t < SELECT_PER_THREAD; ++t) {
-    if (cls[t] == 1) smids[sb++] = (int32_t)(v0 + t);
-    if (cls[t] == 2) hmids[fb++] = (int32_t)(v0 + t);
-    if (cls[t] == 3) mids[mb++] = (int32_t)(v0 + t);
-    if (nch[t] > 1) {
-      for (unsigned k = 0; k < nch[t]; ++k)
-        chunks[hb + k] = LongChunk{(int32_t)(v0 + t), (int32_t)hb, (int32_t)nch[t], (int32_t)k};
-      hb += nch[t];
+    const int32_t v = (int32_t)(v0 + t);
+#pragma unroll
+    for (int k = 0; k < NCLASS; ++k)
+      if (cls[t] == k) rl.rows[k][at[1 + k]++] = v;
+    if (nch[t] > 0) {
+      const unsigned hb = at[0];
+      for (unsigned q = 0; q < nch[t]; ++q) chunks[hb + q] = LongChunk{v, (int32_t)hb, (int32_t)nch[t], (int32_t)q};
+      at[0] += nch[t];
     }
   }
 }
@@ -473,11 +483,9 @@ __global__ __launch_bounds__(256) void k_round(int64_t n, const OffT *__restrict
                                                const int32_t *__restrict__ adjwgt,
                                                const int32_t *__restrict__ vwgt, MatchOwners own,
                                                int32_t *match, const int32_t *prev, int32_t *cur,
-                                               const LongChunk *chunks, const int32_t *mids,
-                                               const unsigned *nchunks,
-                                               const int32_t *smids, const int32_t *hmids, LongPartial part,
-                                               unsigned *flags, int r,
-                                               unsigned nshort, int propose, int64_t maxvw, int short_deg)
+                                               const LongChunk *chunks, RowLists rl, const unsigned *nchunks,
+                                               LongPartial part, unsigned *flags, int r, unsigned nshort,
+                                               int propose, int64_t maxvw, int short_deg)
 {
   if (r >= 2 && !round_live(flags, r - 2)) return;
   const bool have_prev = r >= 1;
@@ -553,23 +561,32 @@ __global__ __launch_bounds__(256) void k_round(int64_t n, const OffT *__restrict
   // groups of MID_GROUP mid rows (one wave per group).  Whether a row is
   // scanned depends only on state fixed before this launch (see above), so
   // all chunks of a row agree.
-  const unsigned nh = nchunks[0], nm = nchunks[1], ns = nchunks[2], nf = nchunks[3];
-  const int64_t gm = ((int64_t)nm + MID_GROUP - 1) / MID_GROUP, gs = ((int64_t)ns + MID_GROUP - 1) / MID_GROUP;
-  const int64_t items = (int64_t)nh + gm + gs + ((int64_t)nf + MID_GROUP - 1) / MID_GROUP;
+  // groups of each class, in class order after the hub chunks
+  const unsigned nh = nchunks[0];
+  int64_t gend[NCLASS];
+  int64_t items = nh;
+#pragma unroll
+  for (int k = 0; k < NCLASS; ++k) {
+    items += ((int64_t)nchunks[1 + k] + MID_GROUP - 1) / MID_GROUP;
+    gend[k] = items;
+  }
   const int64_t nw = (int64_t)(gridDim.x - nshort) * (blockDim.x / CM_WAVE);
   for (int64_t it = ((int64_t)(blockIdx.x - nshort) * blockDim.x + threadIdx.x) / CM_WAVE; it < items; it += nw) {
     if (it < nh)
       hub_chunk<OffT>(it, chunks, xadj, adjncy, adjwgt, vwgt, own, match, prev, cur, part, have_prev, maxvw,
                       (unsigned)r + 1u);
-    else if (it < nh + gm)
-      mid_group<OffT>((it - nh) * MID_GROUP, nm, mids, xadj, adjncy, adjwgt, vwgt, own, match, prev, cur,
-                      have_prev, maxvw);
-    else if (it < nh + gm + gs)
-      small_group<16, OffT>((it - nh - gm) * MID_GROUP, ns, smids, xadj, adjncy, adjwgt, vwgt, own, match, prev,
-                            cur, have_prev, maxvw);
+    else if (it < gend[0])
+      small_group<class_lanes(0), OffT>((it - nh) * MID_GROUP, nchunks[1], rl.rows[0], xadj, adjncy, adjwgt, vwgt,
+                                        own, match, prev, cur, have_prev, maxvw);
+    else if (it < gend[1])
+      small_group<class_lanes(1), OffT>((it - gend[0]) * MID_GROUP, nchunks[2], rl.rows[1], xadj, adjncy, adjwgt,
+                                        vwgt, own, match, prev, cur, have_prev, maxvw);
+    else if (it < gend[2])
+      small_group<class_lanes(2), OffT>((it - gend[1]) * MID_GROUP, nchunks[3], rl.rows[2], xadj, adjncy, adjwgt,
+                                        vwgt, own, match, prev, cur, have_prev, maxvw);
     else
-      small_group<32, OffT>((it - nh - gm - gs) * MID_GROUP, nf, hmids, xadj, adjncy, adjwgt, vwgt, own, match,
-                            prev, cur, have_prev, maxvw);
+      mid_group<OffT>((it - gend[2]) * MID_GROUP, nchunks[1 + MID_CLASS], rl.rows[MID_CLASS], xadj, adjncy, adjwgt,
+                      vwgt, own, match, prev, cur, have_prev, maxvw);
   }
 }
 
@@ -594,9 +611,7 @@ struct MatchRun {
   int32_t *match = nullptr;
   int32_t *prop = nullptr;
   LongChunk *chunks = nullptr;  // hub chunks
-  int32_t *mids = nullptr;      // mid rows
-  int32_t *smids = nullptr;     // small mid rows (16-lane groups)
-  int32_t *hmids = nullptr;     // rows for 32-lane groups
+  RowLists rl{};                // rows by class
   int short_deg = LONG_DEG;     // longest row on the thread-per-row path
   LongPartial part{};
   unsigned *flags = nullptr, *nchunks = nullptr;
@@ -618,39 +633,37 @@ static void match_start(cm_ctx *ctx, MatchRun &mr)
     return p;
   };
   mr.prop = (int32_t *)al(sizeof(int32_t) * 2 * n);  // ping-pong
-  // chunks: sum over rows longer than LONG_DEG of ceil(d / LONG_CHUNK)
+  // chunks: sum over rows longer than LONG_CHUNK of ceil(d / LONG_CHUNK)
   const bool has_long = L.maxdeg < 0 || L.maxdeg > LONG_DEG;
   // hub chunks: at most m / LONG_CHUNK full ones plus one partial per hub
   // row (each hub row holds more than LONG_CHUNK entries)
   const int64_t cap = has_long ? 2 * (L.m / LONG_CHUNK) + 1 : 1;
-  const int64_t cap_mid = has_long ? std::min<int64_t>(n, L.m / (LONG_DEG + 1)) + 1 : 1;
-  const int64_t cap_small = has_long ? std::min<int64_t>(n, L.m / (SHORT_DEG + 1)) + 1 : 1;
-  const int64_t cap_half = has_long ? std::min<int64_t>(n, L.m / (LONG_DEG + 1)) + 1 : 1;
   mr.short_deg = has_long ? SHORT_DEG : LONG_DEG;
   mr.chunks = (LongChunk *)al(sizeof(LongChunk) * cap);
-  mr.mids = (int32_t *)al(sizeof(int32_t) * cap_mid);
-  mr.smids = (int32_t *)al(sizeof(int32_t) * cap_small);
-  mr.hmids = (int32_t *)al(sizeof(int32_t) * cap_half);
+  int64_t groups = 0;  // wave items of the class lists, at most
+  for (int k = 0; k < NCLASS; ++k) {
+    const int64_t ck = has_long ? std::min<int64_t>(n, L.m / (class_lo(k) + 1)) + 1 : 1;
+    mr.rl.rows[k] = (int32_t *)al(sizeof(int32_t) * ck);
+    groups += (ck + MID_GROUP - 1) / MID_GROUP;
+  }
   mr.part.best = (int32_t *)al(sizeof(int32_t) * cap);
   mr.part.k1 = (uint64_t *)al(sizeof(uint64_t) * cap);
   mr.part.k2 = (uint64_t *)al(sizeof(uint64_t) * cap);
   mr.part.scanned = (unsigned *)al(sizeof(unsigned) * cap);
-  mr.flags = (unsigned *)al(sizeof(unsigned) * (2 * (size_t)mr.max_rounds + 4));
-  mr.nchunks = mr.flags + 2 * (size_t)mr.max_rounds;  // [0] hub chunks, [1..3] mid rows by group width
+  mr.flags = (unsigned *)al(sizeof(unsigned) * (2 * (size_t)mr.max_rounds + 8));
+  mr.nchunks = mr.flags + 2 * (size_t)mr.max_rounds;  // [0] hub chunks, [1 + k] rows of class k
   const bool has_hub = L.maxdeg < 0 || L.maxdeg > LONG_CHUNK;
   if (has_hub) fill_u32(ctx, mr.part.scanned, 0u, cap);  // no launch number is 0
   mr.g = nblocks(n, 256);
   launch(ctx, "match_init", 8.0 * n, k_match_init, dim3(mr.g), dim3(256), 0, n, mr.match, mr.prop, mr.flags,
-         2 * (int64_t)mr.max_rounds + 4, ctx->d_ctr + HASH_CTR_SLOT);
+         2 * (int64_t)mr.max_rounds + 8, ctx->d_ctr + HASH_CTR_SLOT);
   ctx->hash_ctr_zero = true;
   // extra blocks whose waves stride over the long-row chunks (none when the
   // level is known to have no long row)
   if (has_long) {
     launch(ctx, "match_select_long", (double)sizeof(OffT) * (n + 1), k_select_long<OffT>,
-           dim3(nblocks(n, 256 * SELECT_PER_THREAD)), dim3(256), 0, n, xadj, mr.chunks, mr.mids, mr.smids,
-           mr.hmids, mr.nchunks);
-    const int64_t items = cap + (cap_mid + MID_GROUP - 1) / MID_GROUP + (cap_small + MID_GROUP - 1) / MID_GROUP +
-                          (cap_half + MID_GROUP - 1) / MID_GROUP;
+           dim3(nblocks(n, 256 * SELECT_PER_THREAD)), dim3(256), 0, n, xadj, mr.chunks, mr.rl, mr.nchunks);
+    const int64_t items = cap + groups;
     mr.glong = (unsigned)std::min<int64_t>(8192, std::max<int64_t>(16, (items + 3) / 4));  // a wave each
     if (has_hub) mr.ghub = (unsigned)std::min<int64_t>(4096, (cap + 3) / 4);
   }
@@ -670,9 +683,8 @@ static void match_launch(cm_ctx *ctx, MatchRun &mr, int count)
     const int32_t *prv = mr.prop + (size_t)((r + 1) & 1) * n;
     launch(ctx, "match_round", r == 0 ? row_bytes : 0.0, k_round<OffT>, dim3(mr.g + mr.glong), dim3(256), 0, n,
            (const OffT *)L.xadj, L.adjncy, L.adjwgt, L.vwgt, mr.own, mr.match, prv, cur,
-           (const LongChunk *)mr.chunks, (const int32_t *)mr.mids, (const unsigned *)mr.nchunks,
-           (const int32_t *)mr.smids, (const int32_t *)mr.hmids, mr.part, mr.flags, r, mr.g, r < mr.max_rounds ? 1 : 0,
-           L.maxvw, mr.short_deg);
+           (const LongChunk *)mr.chunks, mr.rl, (const unsigned *)mr.nchunks, mr.part, mr.flags, r, mr.g,
+           r < mr.max_rounds ? 1 : 0, L.maxvw, mr.short_deg);
     if (mr.ghub && r < mr.max_rounds)
       launch(ctx, "match_hub_reduce", 0.0, k_hub_reduce, dim3(mr.ghub), dim3(256), 0,
              (const LongChunk *)mr.chunks, (const unsigned *)mr.nchunks, mr.part, cur, (const unsigned *)mr.flags,
