@@ -44,7 +44,8 @@ struct RowLists {
   int32_t *rows[NCLASS];
 };
 constexpr int SELECT_PER_THREAD = 16;
-constexpr int MID_GROUP = 16;  // mid rows per wave of the long-row path
+constexpr int MID_GROUP = 16;  // rows per wave of the lane-group classes
+constexpr int WAVE_GROUP = 16;  // rows per wave of the whole-wave (mid) class
 constexpr int ROUND_BATCH = MATCH_BATCH;  // launches queued between host checks
 
 // flags[2r] = 1 if a vertex matched or died in round r, flags[2r+1] = 1 if
@@ -344,7 +345,7 @@ __global__ __launch_bounds__(256) void k_hub_reduce(const LongChunk *chunks, con
   }
 }
 
-// Mid rows mids[m0 .. m0+MID_GROUP): lane i loads row i's vertex state (the
+// Mid rows mids[m0 .. m0+WAVE_GROUP): lane i loads row i's vertex state (the
 // dependency levels of the short path, for all rows of the group at once),
 // then the wave scans the rows that need it one after another, with each
 // row's parameters read from its lane.  A wave thus waits for the state
@@ -358,7 +359,7 @@ __device__ __forceinline__ void mid_group(int64_t m0, unsigned nm, const int32_t
 {
   const int l = lane();
   const int64_t mi = m0 + l;
-  const bool has = l < MID_GROUP && mi < (int64_t)nm;
+  const bool has = l < WAVE_GROUP && mi < (int64_t)nm;
   const int32_t v = mids[has ? mi : m0];
   const int32_t mv = match[v];
   const int32_t p = have_prev ? prev[v] : -1;
@@ -558,16 +559,16 @@ __global__ __launch_bounds__(256) void k_round(int64_t n, const OffT *__restrict
   }
   if (!propose) return;
   // Long rows.  Work items in order: the hub chunks (one wave each), then
-  // groups of MID_GROUP mid rows (one wave per group).  Whether a row is
-  // scanned depends only on state fixed before this launch (see above), so
-  // all chunks of a row agree.
-  // groups of each class, in class order after the hub chunks
+  // the row groups of each class in class order (one wave per group).
+  // Whether a row is scanned depends only on state fixed before this launch
+  // (see above), so all chunks of a row agree.
   const unsigned nh = nchunks[0];
   int64_t gend[NCLASS];
   int64_t items = nh;
 #pragma unroll
   for (int k = 0; k < NCLASS; ++k) {
-    items += ((int64_t)nchunks[1 + k] + MID_GROUP - 1) / MID_GROUP;
+    const int rows = k == MID_CLASS ? WAVE_GROUP : MID_GROUP;
+    items += ((int64_t)nchunks[1 + k] + rows - 1) / rows;
     gend[k] = items;
   }
   const int64_t nw = (int64_t)(gridDim.x - nshort) * (blockDim.x / CM_WAVE);
@@ -585,7 +586,7 @@ __global__ __launch_bounds__(256) void k_round(int64_t n, const OffT *__restrict
       small_group<class_lanes(2), OffT>((it - gend[1]) * MID_GROUP, nchunks[3], rl.rows[2], xadj, adjncy, adjwgt,
                                         vwgt, own, match, prev, cur, have_prev, maxvw);
     else
-      mid_group<OffT>((it - gend[2]) * MID_GROUP, nchunks[1 + MID_CLASS], rl.rows[MID_CLASS], xadj, adjncy, adjwgt,
+      mid_group<OffT>((it - gend[2]) * WAVE_GROUP, nchunks[1 + MID_CLASS], rl.rows[MID_CLASS], xadj, adjncy, adjwgt,
                       vwgt, own, match, prev, cur, have_prev, maxvw);
   }
 }
@@ -644,7 +645,8 @@ static void match_start(cm_ctx *ctx, MatchRun &mr)
   for (int k = 0; k < NCLASS; ++k) {
     const int64_t ck = has_long ? std::min<int64_t>(n, L.m / (class_lo(k) + 1)) + 1 : 1;
     mr.rl.rows[k] = (int32_t *)al(sizeof(int32_t) * ck);
-    groups += (ck + MID_GROUP - 1) / MID_GROUP;
+    const int rows = k == MID_CLASS ? WAVE_GROUP : MID_GROUP;
+    groups += (ck + rows - 1) / rows;
   }
   mr.part.best = (int32_t *)al(sizeof(int32_t) * cap);
   mr.part.k1 = (uint64_t *)al(sizeof(uint64_t) * cap);
