@@ -83,7 +83,9 @@ typedef struct {
  * overweight parts of the initial partition overweight. */
 #define CM_FLAG_NO_BALANCE 2
 
-/* One level of a hierarchy (device pointers owned by the hierarchy).
+/* One level of a hierarchy (device pointers owned by the hierarchy; they are
+ * read-only for the caller: level 0 of a graph given without weights points
+ * at unit weights the context shares between graphs).
  * A level of a batched hierarchy is a slice of the batch's level arrays:
  * its stored values are local values plus a base, i.e. xadj holds
  * local + edge_base, adjncy and match hold local + vertex_base, cmap holds
