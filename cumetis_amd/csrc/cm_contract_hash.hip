@@ -15,11 +15,14 @@
 //           so that twice as many blocks fit a CU), longer ones ("hubs") one
 //           block per key
 //           partition of about HC_PART entries;
-//   gather  per chunk of a hub: coarse neighbour ids and weights in segment
-//           order (wave and block items read their rows directly);
+//   gather  per chunk of a hub: coarse neighbour ids in segment order, and
+//           the chunk's entries (position, id, weight) bucketed by key
+//           partition with a counting sort in LDS (wave and block items read
+//           their rows directly);
 //   dedup   wave / block items: table, heads ranked by position, the row
 //           written compacted into a staging area and its length to cdeg;
-//           hub partitions: head flag and weight sum per entry;
+//           hub partitions: head flag and weight sum per entry, from the
+//           partition's bucket slice of every chunk of the hub;
 //   hubs    count per chunk, scan per hub, compacted write per chunk;
 //   emit    per chunk, once the row offsets are known: staging -> rows.
 // Every item count stays on the device and fixed grids stride over it, so
@@ -139,52 +142,93 @@ __global__ __launch_bounds__(256) void k_hc_select(const int64_t *__restrict__ n
 }
 
 // ---------------------------------------------------------------- gather
-// One wave per chunk; SCAN entries per lane per step so that the row loads
-// and the cmap gathers behind them overlap.
+// One block per hub chunk (HC_CHUNK entries, 8 per thread).  Every entry's
+// coarse neighbour id goes to cval in segment order.  For hubs of at most
+// HC_PMAX partitions the chunk's entries are also counting-sorted by key
+// partition into bpos/bkey/bw (at the chunk's own segment positions), with
+// the partition starts in coff[q * (HC_PMAX + 1) + p] (coff[.. + P] = the
+// chunk's length); the order inside a partition's slice is arbitrary (the
+// partition pass takes minima and sums).  Longer hubs keep their weights in
+// segment order (wv) for the whole-segment pass.
 template <typename OffT>
-__global__ __launch_bounds__(256) void k_hc_gather(const SegChunk *__restrict__ chunks, const unsigned *ctr,
+__global__ __launch_bounds__(256) void k_hc_gather(const SegChunk *__restrict__ hchunks, const unsigned *ctr,
                                                    const OffT *__restrict__ segoff, const int32_t *__restrict__ rep,
                                                    const int32_t *__restrict__ match, const OffT *__restrict__ xadj,
                                                    const int32_t *__restrict__ adjncy,
                                                    const int32_t *__restrict__ adjwgt,
                                                    const int32_t *__restrict__ cmap, int32_t *__restrict__ cval,
-                                                   int32_t *__restrict__ wv)
+                                                   int32_t *__restrict__ wv, int32_t *__restrict__ bpos,
+                                                   int32_t *__restrict__ bkey, int32_t *__restrict__ bw,
+                                                   uint16_t *__restrict__ coff)
 {
-  constexpr int SCAN = 4;
-  const unsigned nq = ctr[HC_N_CHUNK];
-  const int l = lane();
-  const int64_t nw = (int64_t)gridDim.x * (blockDim.x / CM_WAVE);
-  for (int64_t q = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / CM_WAVE; q < nq; q += nw) {
-    const SegChunk ch = chunks[q];
+  constexpr int PER = HC_CHUNK / 256;
+  __shared__ unsigned s_cnt[HC_PMAX + 1];
+  __shared__ unsigned s_sc[4];
+  const unsigned nq = ctr[HC_N_HCHUNK];
+  const int tid = threadIdx.x;
+  for (unsigned q = blockIdx.x; q < nq; q += gridDim.x) {
+    const SegChunk ch = hchunks[q];
     const int64_t s0 = (int64_t)segoff[ch.c];
     const int64_t len = (int64_t)segoff[ch.c + 1] - s0;
-    if (len <= HC_BLOCK_MAX) continue;  // wave and block items read their rows themselves
+    const int P = (int)((len + HC_PART - 1) / HC_PART);
+    const bool bucket = P <= HC_PMAX;
     const int32_t v = rep[ch.c], u = match[v];
     const int64_t b0 = (int64_t)xadj[v], d0 = (int64_t)xadj[v + 1] - b0;
     const int64_t b1 = u != v ? (int64_t)xadj[u] : 0;
-    const int64_t i0 = (int64_t)ch.k * HC_CHUNK, i1 = len < i0 + HC_CHUNK ? len : i0 + HC_CHUNK;
-    for (int64_t ib = i0 + l; ib < i1; ib += SCAN * CM_WAVE) {
-      // clamped, unconditional loads: all issue before the first wait
-      int32_t a[SCAN], w[SCAN], cv[SCAN];
+    const int64_t i0 = (int64_t)ch.k * HC_CHUNK;
+    const int cn = (int)(len - i0 < HC_CHUNK ? len - i0 : HC_CHUNK);
+    // clamped, unconditional loads: all issue before the first wait
+    int32_t a[PER], w[PER], cv[PER];
 #pragma unroll
-      for (int t = 0; t < SCAN; ++t) {
-        int64_t i = ib + (int64_t)t * CM_WAVE;
-        i = i < i1 ? i : i1 - 1;
-        const int64_t j = i < d0 ? b0 + i : b1 + (i - d0);
-        a[t] = adjncy[j];
-        w[t] = adjwgt[j];
-      }
+    for (int t = 0; t < PER; ++t) {
+      const int i = t * 256 + tid, ii = i < cn ? i : cn - 1;
+      const int64_t e = i0 + ii;
+      const int64_t j = e < d0 ? b0 + e : b1 + (e - d0);
+      a[t] = adjncy[j];
+      w[t] = adjwgt[j];
+    }
 #pragma unroll
-      for (int t = 0; t < SCAN; ++t) cv[t] = cmap[a[t]];
+    for (int t = 0; t < PER; ++t) cv[t] = cmap[a[t]];
 #pragma unroll
-      for (int t = 0; t < SCAN; ++t) {
-        const int64_t i = ib + (int64_t)t * CM_WAVE;
-        if (i < i1) {
-          cval[s0 + i] = cv[t];
-          wv[s0 + i] = w[t];
-        }
+    for (int t = 0; t < PER; ++t) {
+      const int i = t * 256 + tid;
+      if (i < cn) {
+        cval[s0 + i0 + i] = cv[t];
+        if (!bucket) wv[s0 + i0 + i] = w[t];
       }
     }
+    if (!bucket) continue;  // uniform over the block
+    for (int p = tid; p <= P; p += 256) s_cnt[p] = 0u;
+    __syncthreads();
+    int pp[PER];
+    unsigned rk[PER];
+#pragma unroll
+    for (int t = 0; t < PER; ++t) {
+      const int i = t * 256 + tid;
+      pp[t] = hc_part(cv[t], P);
+      rk[t] = i < cn ? atomicAdd(&s_cnt[pp[t]], 1u) : 0u;
+    }
+    __syncthreads();
+    unsigned tot;
+    const unsigned ex = block_excl_scan<unsigned, 4>(tid < P ? s_cnt[tid] : 0u, tot, s_sc);
+    uint16_t *co = coff + (size_t)q * (HC_PMAX + 1);
+    if (tid < P) {
+      co[tid] = (uint16_t)ex;
+      s_cnt[tid] = ex;
+    }
+    if (tid == 0) co[P] = (uint16_t)cn;
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < PER; ++t) {
+      const int i = t * 256 + tid;
+      if (i < cn) {
+        const int64_t o = s0 + i0 + (int64_t)(s_cnt[pp[t]] + rk[t]);
+        bpos[o] = (int32_t)(i0 + i);
+        bkey[o] = cv[t];
+        bw[o] = w[t];
+      }
+    }
+    __syncthreads();  // s_cnt is reused by the next chunk
   }
 }
 
@@ -356,33 +400,50 @@ __global__ __launch_bounds__(256) void k_hc_dedup_block(const int32_t *__restric
 }
 
 // ------------------------------------------------- dedup: hub partitions
-// One block per key partition of a hub segment: the block reads the whole
-// segment and tables the keys of its partition; each entry of the partition
-// gets its head flag and, for heads, the weight sum.  The first pass also
-// lists the partition's entries (position << 12 | slot) in LDS, so the second
-// pass walks that list instead of the segment (a full second pass remains for
-// lists that overflow and for segments of 2^19 entries or more).
+// One block per key partition of a hub segment: the block tables the keys of
+// its partition; each entry of the partition gets its head flag and, for
+// heads, the weight sum.  Bucketed hubs (P <= HC_PMAX) give the block its
+// partition's slice of every chunk; longer hubs make it read the whole
+// segment.  The first pass also lists the partition's entries (position <<
+// 12 | slot) in LDS, so the second pass walks that list instead (lists that
+// overflow, and segments of 2^19 entries or more, read the slices or the
+// segment again).
 template <typename OffT>
 __global__ __launch_bounds__(HC_HUB_THREADS) void k_hc_dedup_hub(const HubPart *__restrict__ parts,
                                                                  const unsigned *ctr,
                                                                  const OffT *__restrict__ segoff,
                                                                  const int32_t *__restrict__ cval,
                                                                  const int32_t *__restrict__ wv,
+                                                                 const int32_t *__restrict__ bpos,
+                                                                 const int32_t *__restrict__ bkey,
+                                                                 const int32_t *__restrict__ bw,
+                                                                 const uint16_t *__restrict__ coff,
                                                                  uint8_t *__restrict__ head,
                                                                  int32_t *__restrict__ hw, unsigned *overflow)
 {
   constexpr int lg = HC_HUB_LG;
-  constexpr int SCAN = 4;
+  constexpr int SCAN = 8;
   constexpr int T = HC_HUB_THREADS;
+  constexpr int NWV = T / CM_WAVE;
   __shared__ int32_t tk[HC_HUB_SLOTS], tp[HC_HUB_SLOTS], ts[HC_HUB_SLOTS];
   __shared__ uint32_t lst[HC_HUB_LIST];
+  // bucketed hubs: the partition's slice of each chunk (start, and the
+  // running total of the slice lengths before it)
+  constexpr int MAXCH = HC_PMAX * HC_PART / HC_CHUNK + 1;
+  __shared__ uint16_t s_sb[MAXCH];
+  __shared__ uint32_t s_so[MAXCH + 1];
+  __shared__ uint32_t s_sc[NWV];
   __shared__ int s_full;
   __shared__ unsigned s_cnt;
+  static_assert(MAXCH <= T, "one chunk per thread in the slice scan");
   const unsigned n = ctr[HC_N_PART];
+  const int l = lane(), wv_id = threadIdx.x / CM_WAVE;
   for (int64_t q = blockIdx.x; q < n; q += gridDim.x) {
     const HubPart hp = parts[q];
     const int64_t s0 = (int64_t)segoff[hp.c];
     const int64_t len = (int64_t)segoff[hp.c + 1] - s0;
+    const bool bucket = hp.P <= HC_PMAX;
+    const int nch = (int)((len + HC_CHUNK - 1) / HC_CHUNK);
     for (int i = threadIdx.x; i < HC_HUB_SLOTS; i += T) {
       tk[i] = -1;
       tp[i] = 0x7fffffff;
@@ -395,33 +456,70 @@ __global__ __launch_bounds__(HC_HUB_THREADS) void k_hc_dedup_hub(const HubPart *
     __syncthreads();
     // list entries pack (position << lg) | slot into 32 bits
     const bool listable = len < (1ll << (32 - lg));
-    for (int64_t ib = threadIdx.x; ib < len; ib += SCAN * T) {
-      int32_t cu[SCAN];
-#pragma unroll
-      for (int t = 0; t < SCAN; ++t) {
-        const int64_t i = ib + (int64_t)t * T;
-        cu[t] = i < len ? cval[s0 + i] : -1;
-      }
-#pragma unroll
-      for (int t = 0; t < SCAN; ++t) {
-        const int64_t i = ib + (int64_t)t * T;
-        int s = -1;
-        if (cu[t] >= 0 && hc_part(cu[t], hp.P) == hp.p) {
-          s = hc_insert(tk, lg, cu[t]);
-          if (s < 0) {
-            s_full = 1;
-          } else {
-            atomicMin(&tp[s], (int32_t)i);
-            atomicAdd(&ts[s], wv[s0 + i]);
-          }
+    // one entry of the partition: table it and list it (called by whole
+    // waves; lanes without an entry pass ok = false)
+    auto take = [&](bool ok, int32_t key, int32_t pos, int32_t wt) {
+      int s = -1;
+      if (ok) {
+        s = hc_insert(tk, lg, key);
+        if (s < 0) {
+          s_full = 1;
+        } else {
+          atomicMin(&tp[s], pos);
+          atomicAdd(&ts[s], wt);
         }
-        // one LDS counter update per wave
-        const uint64_t mk = __ballot(s >= 0);
-        unsigned base = 0;
-        if (mk && lane() == 0) base = atomicAdd(&s_cnt, (unsigned)__popcll(mk));
-        base = __shfl(base, 0, CM_WAVE);
-        const unsigned k = base + (unsigned)mask_rank(mk);
-        if (s >= 0 && k < (unsigned)HC_HUB_LIST) lst[k] = ((uint32_t)i << lg) | (uint32_t)s;
+      }
+      // one LDS counter update per wave
+      const uint64_t mk = __ballot(s >= 0);
+      unsigned base = 0;
+      if (mk && l == 0) base = atomicAdd(&s_cnt, (unsigned)__popcll(mk));
+      base = __shfl(base, 0, CM_WAVE);
+      const unsigned k = base + (unsigned)mask_rank(mk);
+      if (s >= 0 && k < (unsigned)HC_HUB_LIST) lst[k] = ((uint32_t)pos << lg) | (uint32_t)s;
+    };
+    if (bucket) {
+      // the slices laid end to end, so that all threads share them whatever
+      // their number and lengths (a hub of a few chunks has slices of up to
+      // HC_CHUNK entries)
+      uint32_t sl = 0;
+      if (threadIdx.x < nch) {
+        const uint16_t *co = coff + (size_t)(hp.hfirst + threadIdx.x) * (HC_PMAX + 1);
+        const uint16_t b = co[hp.p];
+        s_sb[threadIdx.x] = b;
+        sl = (uint32_t)co[hp.p + 1] - b;
+      }
+      uint32_t total;
+      const uint32_t ex = block_excl_scan<uint32_t, NWV>(sl, total, s_sc);
+      if (threadIdx.x < nch) s_so[threadIdx.x] = ex;
+      if (threadIdx.x == 0) s_so[nch] = total;
+      __syncthreads();
+      for (uint32_t f0 = 0; f0 < total; f0 += T) {
+        const uint32_t f = f0 + threadIdx.x;
+        const bool ok = f < total;
+        // the chunk whose slice holds entry f: the last j with s_so[j] <= f
+        int lo = 0, hi = nch - 1;
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (s_so[mid] <= f) lo = mid;
+          else hi = mid - 1;
+        }
+        const int64_t o = ok ? s0 + (int64_t)lo * HC_CHUNK + s_sb[lo] + (f - s_so[lo]) : s0;
+        take(ok, bkey[o], bpos[o], bw[o]);
+      }
+    } else {
+      for (int64_t ib = threadIdx.x; ib < len; ib += SCAN * T) {
+        int32_t cu[SCAN];
+#pragma unroll
+        for (int t = 0; t < SCAN; ++t) {
+          const int64_t i = ib + (int64_t)t * T;
+          cu[t] = i < len ? cval[s0 + i] : -1;
+        }
+#pragma unroll
+        for (int t = 0; t < SCAN; ++t) {
+          const int64_t i = ib + (int64_t)t * T;
+          const bool ok = cu[t] >= 0 && hc_part(cu[t], hp.P) == hp.p;
+          take(ok, cu[t], (int32_t)i, ok ? wv[s0 + i] : 0);
+        }
       }
     }
     __syncthreads();
@@ -434,6 +532,19 @@ __global__ __launch_bounds__(HC_HUB_THREADS) void k_hc_dedup_hub(const HubPart *
         const bool h = tp[s] == i;
         head[s0 + i] = h ? 1 : 0;
         hw[s0 + i] = h ? ts[s] : 0;
+      }
+    } else if (bucket) {
+      for (int j = wv_id; j < nch; j += NWV) {
+        const uint16_t *co = coff + (size_t)(hp.hfirst + j) * (HC_PMAX + 1);
+        const int b = co[hp.p], e = co[hp.p + 1];
+        const int64_t cb = s0 + (int64_t)j * HC_CHUNK;
+        for (int i = b + l; i < e; i += CM_WAVE) {
+          const int32_t key = bkey[cb + i], pos = bpos[cb + i];
+          const int s = hc_find(tk, lg, key);
+          const bool h = s >= 0 && tp[s] == pos;
+          head[s0 + pos] = h ? 1 : 0;
+          hw[s0 + pos] = h ? ts[s] : 0;
+        }
       }
     } else {
       for (int64_t i = threadIdx.x; i < len; i += T) {
@@ -619,7 +730,7 @@ __global__ __launch_bounds__(256) void k_hc_emit(const SegChunk *__restrict__ ch
 // bounds of the item counts for nc coarse vertices (segments above REG_SEG
 // hold at most m entries)
 struct HcBounds {
-  int64_t nseg, nchunk, nwavel, nblk, nblkl, nhub, npart;
+  int64_t nseg, nchunk, nwavel, nblk, nblkl, nhub, npart, nhchunk;
   HcBounds(int64_t nc, int64_t m)
   {
     nseg = std::min<int64_t>(nc, m / (REG_SEG + 1)) + 1;
@@ -629,6 +740,7 @@ struct HcBounds {
     nblkl = std::min<int64_t>(nc, m / (HC_BLOCK_SPLIT + 1)) + 1;
     nhub = m / (HC_BLOCK_MAX + 1) + 1;
     npart = m / HC_PART + nhub;
+    nhchunk = m / HC_CHUNK + nhub;
   }
 };
 
@@ -651,6 +763,7 @@ void hash_alloc(cm_ctx *ctx, const Level &L, int64_t nc, HashContract &hc)
   hc.lists.blockl = (int32_t *)al(4 * b.nblkl);
   hc.lists.parts = (HubPart *)al(sizeof(HubPart) * b.npart);
   hc.lists.hubs = (int32_t *)al(4 * b.nhub);
+  hc.lists.hchunks = (SegChunk *)al(sizeof(SegChunk) * b.nhchunk);
   static_assert(HASH_CTR_SLOT + HC_CTR_WORDS <= CTR_SLOTS, "hash counters");
   hc.lists.ctr = ctx->d_ctr + HASH_CTR_SLOT;
   hc.tc = (int32_t *)al(4 * m);
@@ -712,15 +825,19 @@ void hash_prepare_impl(cm_ctx *ctx, const Level &L, const int32_t *match, const 
            cdeg);
   if (m > HC_BLOCK_MAX && seg_max > HC_BLOCK_MAX) {  // a hub needs more than HC_BLOCK_MAX entries
     int32_t *cval = (int32_t *)al(4 * m), *wv = (int32_t *)al(4 * m);
-    launch(ctx, "hc_gather", 0.0, k_hc_gather<OffT>, dim3(gchunk), dim3(256), 0, chunks, (const unsigned *)ctr,
-           segoff, rep, match, xadj, L.adjncy, L.adjwgt, cmap, cval, wv);
+    int32_t *bpos = (int32_t *)al(4 * m), *bkey = (int32_t *)al(4 * m), *bw = (int32_t *)al(4 * m);
+    uint16_t *coff_b = (uint16_t *)al(sizeof(uint16_t) * (HC_PMAX + 1) * b.nhchunk);
+    launch(ctx, "hc_gather", 0.0, k_hc_gather<OffT>, dim3((unsigned)std::min<int64_t>(8192, b.nhchunk)), dim3(256),
+           0, (const SegChunk *)hl.hchunks, (const unsigned *)ctr, segoff, rep, match, xadj, L.adjncy, L.adjwgt, cmap,
+           cval, wv, bpos, bkey, bw, coff_b);
     uint8_t *head = (uint8_t *)al(m);
     int32_t *hw = (int32_t *)al(4 * m);
     int32_t *ccnt = (int32_t *)al(4 * nchunk), *cself = (int32_t *)al(4 * nchunk), *coff = (int32_t *)al(4 * nchunk);
     int32_t *hself = (int32_t *)al(4 * nchunk), *htot = (int32_t *)al(4 * nchunk);
     launch(ctx, "hc_dedup_hub", 0.0, k_hc_dedup_hub<OffT>, dim3((unsigned)std::min<int64_t>(1024, npart)),
            dim3(HC_HUB_THREADS), 0, (const HubPart *)parts, (const unsigned *)ctr, segoff, (const int32_t *)cval,
-           (const int32_t *)wv, head, hw, ctr + HC_OVERFLOW);
+           (const int32_t *)wv, (const int32_t *)bpos, (const int32_t *)bkey, (const int32_t *)bw,
+           (const uint16_t *)coff_b, head, hw, ctr + HC_OVERFLOW);
     launch(ctx, "hc_hub_count", 0.0, k_hc_hub_count<OffT>, dim3(gchunk), dim3(256), 0, chunks,
            (const unsigned *)ctr, segoff, (const uint8_t *)head, (const int32_t *)cval, ccnt, cself);
     launch(ctx, "hc_hub_scan", 0.0, k_hc_hub_scan<OffT>, dim3((unsigned)std::min<int64_t>(1024, (nhub + 3) / 4)),

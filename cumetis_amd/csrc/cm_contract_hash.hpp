@@ -33,22 +33,28 @@ constexpr int HC_WAVE_MAX = 256;
 constexpr int HC_BLOCK_SPLIT = 1024;
 constexpr int HC_BLOCK_MAX = 2048;
 constexpr int HC_PART = 4096;
+// Hubs of at most HC_PMAX key partitions have their entries bucketed by
+// partition inside each chunk, so a partition block reads only its own
+// slices; longer ones are read whole by every partition block.
+constexpr int HC_PMAX = 256;
 
 struct SegChunk {
   int32_t c, first, nch, k;  // chunk k of the nch chunks of segment c, from chunk `first`
 };
 struct HubPart {
-  int32_t c, p, P, pad;  // key partition p of P of hub segment c
+  int32_t c, p, P, hfirst;  // key partition p of P of hub segment c, whose chunks start at hfirst
 };
 
-// item counters
-enum { HC_N_CHUNK = 0, HC_N_WAVE, HC_N_BLOCK, HC_N_PART, HC_N_HUB, HC_N_BLOCKL, HC_N_WAVEL, HC_N_COUNTERS };
+// item counters (HC_N_HCHUNK: chunks of hub segments, listed again densely)
+enum { HC_N_CHUNK = 0, HC_N_WAVE, HC_N_BLOCK, HC_N_PART, HC_N_HUB, HC_N_BLOCKL, HC_N_WAVEL, HC_N_HCHUNK, HC_N_COUNTERS };
 constexpr int HC_OVERFLOW = 8;  // ctr[HC_OVERFLOW]: a hub table filled
+static_assert(HC_N_COUNTERS <= HC_OVERFLOW, "counters below the overflow word");
 constexpr int HC_CTR_WORDS = 16;
 
 // The item lists (device pointers) and their counters.
 struct HcLists {
   SegChunk *chunks = nullptr;
+  SegChunk *hchunks = nullptr;  // the hub chunks again, one dense run per hub
   int32_t *wave = nullptr, *wavel = nullptr, *block = nullptr, *blockl = nullptr, *hubs = nullptr;
   HubPart *parts = nullptr;
   unsigned *ctr = nullptr;
@@ -68,6 +74,7 @@ __device__ __forceinline__ void hc_count(int64_t len, unsigned (&cnt)[HC_N_COUNT
   cnt[HC_N_BLOCKL] += len > HC_BLOCK_SPLIT && !hub ? 1u : 0u;
   cnt[HC_N_HUB] += hub ? 1u : 0u;
   cnt[HC_N_PART] += hub ? (unsigned)((len + HC_PART - 1) / HC_PART) : 0u;
+  cnt[HC_N_HCHUNK] += hub ? (unsigned)((len + HC_CHUNK - 1) / HC_CHUNK) : 0u;
 }
 
 // The items of segment c at the positions at[] (advanced past them).
@@ -91,11 +98,14 @@ __device__ __forceinline__ void hc_write(int32_t c, int64_t len, unsigned (&at)[
   at[HC_N_BLOCKL] += cbl ? 1u : 0u;
   if (hub) {
     hl.hubs[at[HC_N_HUB]] = (int32_t)first;  // a hub is named by its first chunk
+    const unsigned hf = at[HC_N_HCHUNK];
+    for (int k = 0; k < nch; ++k) hl.hchunks[hf + k] = SegChunk{c, (int32_t)hf, nch, k};
     const int P = (int)((len + HC_PART - 1) / HC_PART);
-    for (int p = 0; p < P; ++p) hl.parts[at[HC_N_PART] + p] = HubPart{c, p, P, 0};
+    for (int p = 0; p < P; ++p) hl.parts[at[HC_N_PART] + p] = HubPart{c, p, P, (int32_t)hf};
   }
   at[HC_N_HUB] += hub ? 1u : 0u;
   at[HC_N_PART] += hub ? (unsigned)((len + HC_PART - 1) / HC_PART) : 0u;
+  at[HC_N_HCHUNK] += hub ? (unsigned)nch : 0u;
 }
 
 // Block of NW waves, called by all its threads: reserves every thread's

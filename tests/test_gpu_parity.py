@@ -124,11 +124,13 @@ def test_match_small_mid_rows(engine, orc, wmax):
         assert r_gpu == r_orc
 
 
-@pytest.mark.parametrize("hub_deg,side", [(140000, 400), (600000, 800)])
+@pytest.mark.parametrize("hub_deg,side", [(140000, 400), (600000, 800), (1100000, 1100)])
 def test_contract_hub_partitions(engine, orc, hub_deg, side):
-    """Hub segments cut into many key partitions (35 and 147 blocks each
-    re-reading the segment), the larger one past the length whose entries
-    can be listed in LDS (2^19), so the second pass rescans the segment."""
+    """Hub segments cut into many key partitions: 35 and 147 partitions
+    (entries bucketed by partition in each chunk), the second past the length
+    whose entries can be listed in LDS (2^19), so the second pass reads the
+    bucket slices again; and 269 partitions, more than are bucketed, so every
+    partition block reads the whole segment."""
     from cases import hub_graph
     g = with_weights(hub_graph(seed=23, side=side, hub_deg=hub_deg, mids=(5000, 2100, 1500, 600)))
     say(f"[contract hub partitions] hub_deg={hub_deg} n={g.n} m={g.m}")
