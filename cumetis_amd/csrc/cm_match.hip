@@ -470,6 +470,9 @@ __device__ __forceinline__ void small_group(int64_t m0, unsigned ns, const int32
   }
 }
 
+// At most 7 waves per SIMD: the register bound keeps the lane-group paths
+// from taking the kernel down to 6 waves, which the latency-bound row scans
+// feel (R-MAT rounds -1.5%).
 // Launch r of the handshake: accept of round r-1 (from prev = proposals of
 // round r-1) and proposals of round r (into cur), over the state after round
 // r-1.  match[] is written only for the thread's own vertex; readers treat a
@@ -479,7 +482,7 @@ __device__ __forceinline__ void small_group(int64_t m0, unsigned ns, const int32
 // accepts); the remaining blocks take the chunks of long rows, one wave each,
 // and groups of mid and small mid rows.
 template <typename OffT>
-__global__ __launch_bounds__(256) void k_round(int64_t n, const OffT *__restrict__ xadj,
+__global__ __launch_bounds__(256, 7) void k_round(int64_t n, const OffT *__restrict__ xadj,
                                                const int32_t *__restrict__ adjncy,
                                                const int32_t *__restrict__ adjwgt,
                                                const int32_t *__restrict__ vwgt, MatchOwners own,
