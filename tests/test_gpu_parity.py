@@ -145,6 +145,41 @@ def test_contract_hub_partitions(engine, orc, hub_deg, side):
     np.testing.assert_array_equal(cw, ew)
 
 
+def test_contract_hub_table_overflow(engine, orc):
+    """A hub whose coarse neighbour ids all fall into one key partition
+    (9000 distinct ids against an 8192-slot table): the partition's table
+    fills, and the level is redone on the sorted path, still bit-exact.
+    Isolated vertices keep cmap[v] = v below the hub, so the ids are chosen
+    through the partition hash of the contraction (a multiplicative hash of
+    the id, scaled to the partition count)."""
+    P = 3  # partitions of a 9001-entry segment of 4096-entry partitions
+    ids = np.arange(30000, dtype=np.uint64)
+    part = ((((ids * np.uint64(0x85EBCA77)) & np.uint64(0xFFFFFFFF)) * np.uint64(P)) >> np.uint64(32))
+    S = np.nonzero(part == 0)[0][:9000]
+    assert len(S) == 9000
+    hub = 30000
+    g = graphs.from_edges(hub + 1, np.full(len(S), hub), S, np.random.default_rng(4).integers(1, 50, size=len(S)))
+    g = with_weights(g)
+    say(f"[contract hub overflow] n={g.n} m={g.m}")
+    mv = maxv_for(g, orc=orc)
+    match, _ = orc.handshake_match(g.xadj, g.adjncy, g.adjwgt, g.vwgt, mv, seed=3)
+    cmap, nc = orc.number(match)
+    assert np.array_equal(cmap[:hub], np.arange(hub))  # the ids above are the coarse ids
+    engine.set_timing(True)
+    engine.reset_stats()
+    try:
+        cx, ca, cw, cv = engine.contract(g, match, cmap, nc, CM_ROW_FIRST_SEEN)
+        stats = engine.kernel_stats()
+    finally:
+        engine.set_timing(False)
+    assert "hc_dedup_hub" in stats and "contract_gather" in stats  # the hub pass ran, then the sorted path
+    ex, ea, ew, ev = orc.cpu_contract(g.xadj, g.adjncy, g.adjwgt, g.vwgt, match, cmap, nc)
+    np.testing.assert_array_equal(cx.astype(np.int64), ex)
+    np.testing.assert_array_equal(cv, ev)
+    np.testing.assert_array_equal(ca, ea)
+    np.testing.assert_array_equal(cw, ew)
+
+
 def test_match_repeatable(engine, orc):
     g = with_weights(CASES["rgg3000w16"])
     say("[match repeat] rgg3000w16")
