@@ -93,11 +93,10 @@ __global__ __launch_bounds__(256) void k_match_init(int64_t n, int32_t *match, i
   for (int64_t i = v; i < nflags; i += (int64_t)gridDim.x * blockDim.x) flags[i] = 0u;
 }
 
-// Rows longer than LONG_DEG are scanned by lane groups (up to HALF_DEG
-// entries) or whole waves.  Rows of at most LONG_CHUNK entries ("mid" rows)
-// are listed by vertex; longer rows ("hubs") are cut into chunks of
-// LONG_CHUNK entries, one wave per chunk.  Chunk descriptor: the row, its
-// first chunk, its chunk count and the chunk's index within the row.
+// Rows of at most LONG_CHUNK entries are listed by vertex in their class
+// (above); longer rows ("hubs") are cut into chunks of LONG_CHUNK entries,
+// one wave per chunk.  Chunk descriptor: the row, its first chunk, its chunk
+// count and the chunk's index within the row.
 struct LongChunk {
   int32_t v, first, nch, k;
 };
@@ -479,8 +478,8 @@ __device__ __forceinline__ void small_group(int64_t m0, unsigned ns, const int32
 // neighbour as taken if match[u] != -1 or it was accepted in round r-1, which
 // gives the same answer whether or not the neighbour's own write has landed.
 // Blocks [0, nshort) take one vertex per thread (rows <= short_deg and all
-// accepts); the remaining blocks take the chunks of long rows, one wave each,
-// and groups of mid and small mid rows.
+// accepts); the remaining blocks take the chunks of hub rows and the row
+// groups of the classes, one wave per item.
 template <typename OffT>
 __global__ __launch_bounds__(256, 7) void k_round(int64_t n, const OffT *__restrict__ xadj,
                                                const int32_t *__restrict__ adjncy,
