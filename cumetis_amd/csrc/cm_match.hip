@@ -547,14 +547,15 @@ __global__ __launch_bounds__(256, 7) void k_round(int64_t n, const OffT *__restr
       if (__ballot(changed) && lane() == 0) s_changed = 1;
       if (__ballot(active) && lane() == 0) s_active = 1;
       __syncthreads();
-      // round status words are flags: one plain store per block, and only
-      // while still clear, so blocks do not serialise on one address
+      // round status words are flags, read only by later launches: one
+      // plain store of 1 per block.  Stores of one value from many blocks
+      // merge in the L2s and reach memory at the end of the launch; an atomic
+      // (or a check-then-store through the memory-side coherence point) would
+      // add a round trip to the end of every block.
       if (threadIdx.x == 0) {
         unsigned *f = flags + 2 * (r - 1);
-        if (s_changed && __hip_atomic_load(&f[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
-          __hip_atomic_store(&f[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (s_active && __hip_atomic_load(&f[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
-          __hip_atomic_store(&f[1], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (s_changed) f[0] = 1u;
+        if (s_active) f[1] = 1u;
       }
     }
     return;
