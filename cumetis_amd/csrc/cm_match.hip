@@ -162,6 +162,116 @@ __global__ __launch_bounds__(256) void k_select_long(int64_t n, const OffT *xadj
   }
 }
 
+// The frontier.  After launch COMPACT_ROUND of a large level most vertices
+// are matched, yet every later launch would still visit all n vertices, all
+// class rows and all hub chunks to find that out (its time then goes to
+// dispatch and one load per item).  k_compact lists again what is still
+// unmatched after that launch: the vertices (for the accepts and the
+// thread-per-row proposals), the class rows, and the hub chunks of unmatched
+// rows (each row's chunks kept together, `first` re-pointed).  A later launch
+// that skips a matched vertex skips only work that returns at once, so the
+// rounds, flags and matching are those of the full launches.  The lists are
+// in no particular order (a launch's result does not depend on its order).
+constexpr int COMPACT_ROUND = 5;
+constexpr int64_t COMPACT_MIN_N = int64_t(1) << 20;  // smaller levels' rounds cost about a launch
+// Vertex regions: compaction block b lists the unmatched vertices of
+// [b * VREGION, (b + 1) * VREGION) at slist[b * VREGION ..] with their count
+// in vcount[b], and block b of a list launch takes region b.  No atomics: a
+// counter shared by many blocks serialises its updates.
+constexpr int VREGION = 1024;
+constexpr int VREGION_PER_THREAD = VREGION / 256;
+// The class lists and the hub chunks: LIST_BLOCKS blocks per list, each
+// over a contiguous range of it, with one atomic per block and list.
+constexpr int LIST_BLOCKS = 64;
+
+// Exclusive position of this thread's c entries in the list counted by
+// *count: one atomic per block.  Block-uniform call (256 threads).
+__device__ __forceinline__ unsigned block_append(unsigned c, unsigned *count, unsigned *s_w, unsigned &s_b)
+{
+  unsigned tot;
+  const unsigned before = block256_excl_scan<unsigned>(c, tot, s_w);
+  if (threadIdx.x == 0) s_b = tot ? atomicAdd(count, tot) : 0u;
+  __syncthreads();
+  const unsigned at = s_b + before;
+  __syncthreads();  // s_b is reused by the next call
+  return at;
+}
+
+// Entry i of list L (0: hub chunks, 1 + k: rows of class k) as the number
+// of slots it keeps: a class row of an unmatched vertex keeps 1, the first
+// chunk of an unmatched hub row keeps the row's chunks.
+__device__ __forceinline__ unsigned compact_keep(int L, int64_t i, const int32_t *match, const LongChunk *chunks,
+                                                 const RowLists &rl, LongChunk &c)
+{
+  if (L == 0) {
+    c = chunks[i];
+    return c.k == 0 && match[c.v] == -1 ? (unsigned)c.nch : 0u;
+  }
+  int32_t v = 0;
+#pragma unroll
+  for (int k = 0; k < NCLASS; ++k)
+    if (L == 1 + k) v = rl.rows[k][i];
+  c.v = v;
+  return match[v] == -1 ? 1u : 0u;
+}
+
+// Blocks [0, gv): the vertex regions.  Blocks gv + L * LIST_BLOCKS + j: range
+// j of list L.  cnt2: [0] hub chunks, [1 + k] rows of class k (zero before
+// the launch).
+__global__ __launch_bounds__(256) void k_compact(int64_t n, const int32_t *match, const unsigned *flags, int r0,
+                                                 const LongChunk *chunks, RowLists rl, const unsigned *cnt,
+                                                 LongChunk *chunks2, RowLists rl2, unsigned *cnt2, int32_t *slist,
+                                                 unsigned *vcount, unsigned gv)
+{
+  // launch r0 + 1 and all later ones return at once if round r0 - 1 did
+  // nothing: no list is read
+  if (!round_live(flags, r0 - 1)) return;
+  __shared__ unsigned s_w[4], s_b;
+  if (blockIdx.x < gv) {
+    const int64_t v0 = (int64_t)blockIdx.x * VREGION + threadIdx.x * VREGION_PER_THREAD;
+    int32_t mv[VREGION_PER_THREAD];
+    unsigned c = 0;
+#pragma unroll
+    for (int t = 0; t < VREGION_PER_THREAD; ++t) {
+      mv[t] = v0 + t < n ? match[v0 + t] : 0;
+      c += mv[t] == -1 ? 1u : 0u;
+    }
+    unsigned tot;
+    unsigned at = block256_excl_scan<unsigned>(c, tot, s_w);
+    int32_t *out = slist + (int64_t)blockIdx.x * VREGION;
+#pragma unroll
+    for (int t = 0; t < VREGION_PER_THREAD; ++t)
+      if (mv[t] == -1) out[at++] = (int32_t)(v0 + t);
+    if (threadIdx.x == 0) vcount[blockIdx.x] = tot;
+    return;
+  }
+  const int L = (int)((blockIdx.x - gv) / LIST_BLOCKS), j = (int)((blockIdx.x - gv) % LIST_BLOCKS);
+  const int64_t nl = cnt[L];
+  const int64_t per = (nl + LIST_BLOCKS - 1) / LIST_BLOCKS;
+  const int64_t lo = std::min<int64_t>(nl, j * per), hi = std::min<int64_t>(nl, lo + per);
+  // two passes over the range: count, one atomic, then write (the second
+  // pass reads again what the first brought into the caches)
+  unsigned c = 0;
+  for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+    LongChunk ch;
+    c += compact_keep(L, i, match, chunks, rl, ch);
+  }
+  unsigned at = block_append(c, cnt2 + L, s_w, s_b);
+  for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+    LongChunk ch;
+    const unsigned keep = compact_keep(L, i, match, chunks, rl, ch);
+    if (!keep) continue;
+    if (L == 0) {
+      for (int q = 0; q < ch.nch; ++q) chunks2[at + q] = LongChunk{ch.v, (int32_t)at, ch.nch, q};
+    } else {
+#pragma unroll
+      for (int k = 0; k < NCLASS; ++k)
+        if (L == 1 + k) rl2.rows[k][at] = ch.v;
+    }
+    at += keep;
+  }
+}
+
 // Wave-uniform copies of lane i's values (i uniform).
 __device__ __forceinline__ int32_t bcast(int32_t x, int i) { return __builtin_amdgcn_readlane(x, i); }
 __device__ __forceinline__ int64_t bcast(int64_t x, int i)
@@ -477,18 +587,73 @@ __device__ __forceinline__ void small_group(int64_t m0, unsigned ns, const int32
 // r-1.  match[] is written only for the thread's own vertex; readers treat a
 // neighbour as taken if match[u] != -1 or it was accepted in round r-1, which
 // gives the same answer whether or not the neighbour's own write has landed.
-// Blocks [0, nshort) take one vertex per thread (rows <= short_deg and all
-// accepts); the remaining blocks take the chunks of hub rows and the row
-// groups of the classes, one wave per item.
+// One vertex of the thread-per-row path: the accept of round r-1 for every
+// vertex, the proposal of round r for rows of at most short_deg entries.
+// changed / active are or-ed with this vertex's status.
 template <typename OffT>
-__global__ __launch_bounds__(256, 7) void k_round(int64_t n, const OffT *__restrict__ xadj,
-                                               const int32_t *__restrict__ adjncy,
-                                               const int32_t *__restrict__ adjwgt,
-                                               const int32_t *__restrict__ vwgt, MatchOwners own,
-                                               int32_t *match, const int32_t *prev, int32_t *cur,
-                                               const LongChunk *chunks, RowLists rl, const unsigned *nchunks,
-                                               LongPartial part, unsigned *flags, int r, unsigned nshort,
-                                               int propose, int64_t maxvw, int short_deg)
+__device__ __forceinline__ void short_row(int64_t v, int64_t n, const OffT *__restrict__ xadj,
+                                          const int32_t *__restrict__ adjncy, const int32_t *__restrict__ adjwgt,
+                                          const int32_t *__restrict__ vwgt, const MatchOwners &own, int32_t *match,
+                                          const int32_t *prev, int32_t *cur, bool have_prev, int propose,
+                                          int64_t maxvw, int short_deg, bool &changed, bool &active)
+{
+  // Loads in three dependency levels (by v; by p = prev[v]; by prev[p]),
+  // each through clamped indices so that a level issues as one batch.
+  const int64_t vc = v < n ? v : n - 1;
+  const int32_t mv = match[vc];
+  const int32_t p = have_prev ? prev[vc] : -1;
+  if (v < n && mv == -1) {
+    const int32_t pc = p >= 0 ? p : (int32_t)v;
+    const int32_t pp = have_prev ? prev[pc] : -1;  // prev[p]
+    const int32_t mp = match[pc];
+    const OffT b = xadj[v], e = xadj[v + 1];
+    const int32_t vw = vwgt[v];
+    const int32_t ppp = have_prev ? prev[pp >= 0 ? pp : pc] : -1;  // prev[prev[p]]
+    bool done = false;
+    if (have_prev) {
+      if (p == -2) {
+        match[v] = (int32_t)v;
+        done = true;
+      } else if (p >= 0 && pp == (int32_t)v) {
+        match[v] = p;
+        done = true;
+      }
+    }
+    changed |= done;
+    active |= !done;
+    if (!done && propose && (int64_t)(e - b) <= short_deg) {
+      // p accepted in round r-1 (from the proposals alone): dead or mutual
+      const bool p_acc = pp == -2 || (pp >= 0 && ppp == p);
+      if (p >= 0 && mp == -1 && !p_acc) {
+        cur[v] = p;  // still unmatched, so still the best
+      } else {
+        int32_t maxv;
+        uint64_t seed;
+        int64_t base;
+        owner_params(own, v, maxv, seed, base);
+        int32_t best = -2;
+        uint64_t bk1 = 0, bk2 = 0;
+        scan_row<OffT>(v, b, e, 0, 1, adjncy, adjwgt, vwgt, vw, maxv, seed, base, match, prev, have_prev,
+                       maxvw < 0 || (int64_t)vw + maxvw > maxv, best, bk1, bk2);
+        cur[v] = best;
+      }
+    }
+  }
+}
+
+// Blocks [0, nshort) take one vertex per thread (rows <= short_deg and all
+// accepts), or (LIST) take vertex region blockIdx.x of the frontier:
+// slist[blockIdx.x * VREGION ..] with vcount[blockIdx.x] entries;
+// the remaining blocks take the chunks of hub rows and the row groups of the
+// classes, one wave per item.
+template <typename OffT, bool LIST>
+__device__ __forceinline__ void round_body(int64_t n, const OffT *__restrict__ xadj,
+                                           const int32_t *__restrict__ adjncy, const int32_t *__restrict__ adjwgt,
+                                           const int32_t *__restrict__ vwgt, const MatchOwners &own, int32_t *match,
+                                           const int32_t *prev, int32_t *cur, const LongChunk *chunks,
+                                           const RowLists &rl, const unsigned *nchunks, const LongPartial &part,
+                                           unsigned *flags, int r, unsigned nshort, int propose, int64_t maxvw,
+                                           int short_deg, const int32_t *slist, const unsigned *vcount)
 {
   if (r >= 2 && !round_live(flags, r - 2)) return;
   const bool have_prev = r >= 1;
@@ -499,49 +664,16 @@ __global__ __launch_bounds__(256, 7) void k_round(int64_t n, const OffT *__restr
       s_active = 0;
     }
     __syncthreads();
-    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     bool changed = false, active = false;
-    // Loads in three dependency levels (by v; by p = prev[v]; by prev[p]),
-    // each through clamped indices so that a level issues as one batch.
-    const int64_t vc = v < n ? v : n - 1;
-    const int32_t mv = match[vc];
-    const int32_t p = have_prev ? prev[vc] : -1;
-    if (v < n && mv == -1) {
-      const int32_t pc = p >= 0 ? p : (int32_t)v;
-      const int32_t pp = have_prev ? prev[pc] : -1;  // prev[p]
-      const int32_t mp = match[pc];
-      const OffT b = xadj[v], e = xadj[v + 1];
-      const int32_t vw = vwgt[v];
-      const int32_t ppp = have_prev ? prev[pp >= 0 ? pp : pc] : -1;  // prev[prev[p]]
-      bool done = false;
-      if (have_prev) {
-        if (p == -2) {
-          match[v] = (int32_t)v;
-          done = true;
-        } else if (p >= 0 && pp == (int32_t)v) {
-          match[v] = p;
-          done = true;
-        }
-      }
-      changed = done;
-      active = !done;
-      if (!done && propose && (int64_t)(e - b) <= short_deg) {
-        // p accepted in round r-1 (from the proposals alone): dead or mutual
-        const bool p_acc = pp == -2 || (pp >= 0 && ppp == p);
-        if (p >= 0 && mp == -1 && !p_acc) {
-          cur[v] = p;  // still unmatched, so still the best
-        } else {
-          int32_t maxv;
-          uint64_t seed;
-          int64_t base;
-          owner_params(own, v, maxv, seed, base);
-          int32_t best = -2;
-          uint64_t bk1 = 0, bk2 = 0;
-          scan_row<OffT>(v, b, e, 0, 1, adjncy, adjwgt, vwgt, vw, maxv, seed, base, match, prev, have_prev,
-                         maxvw < 0 || (int64_t)vw + maxvw > maxv, best, bk1, bk2);
-          cur[v] = best;
-        }
-      }
+    if constexpr (LIST) {  // the frontier: only vertices still unmatched are listed
+      const unsigned cnt = vcount[blockIdx.x];
+      const int32_t *list = slist + (int64_t)blockIdx.x * VREGION;
+      for (unsigned i = threadIdx.x; i < cnt; i += blockDim.x)
+        short_row<OffT>(list[i], n, xadj, adjncy, adjwgt, vwgt, own, match, prev, cur, have_prev, propose,
+                        maxvw, short_deg, changed, active);
+    } else {
+      short_row<OffT>((int64_t)blockIdx.x * blockDim.x + threadIdx.x, n, xadj, adjncy, adjwgt, vwgt, own, match,
+                      prev, cur, have_prev, propose, maxvw, short_deg, changed, active);
     }
     if (have_prev) {
       if (__ballot(changed) && lane() == 0) s_changed = 1;
@@ -594,6 +726,37 @@ __global__ __launch_bounds__(256, 7) void k_round(int64_t n, const OffT *__restr
   }
 }
 
+// Launch r of the handshake over all vertices and the level's long-row lists.
+template <typename OffT>
+__global__ __launch_bounds__(256, 7) void k_round(int64_t n, const OffT *__restrict__ xadj,
+                                               const int32_t *__restrict__ adjncy,
+                                               const int32_t *__restrict__ adjwgt,
+                                               const int32_t *__restrict__ vwgt, MatchOwners own,
+                                               int32_t *match, const int32_t *prev, int32_t *cur,
+                                               const LongChunk *chunks, RowLists rl, const unsigned *nchunks,
+                                               LongPartial part, unsigned *flags, int r, unsigned nshort,
+                                               int propose, int64_t maxvw, int short_deg)
+{
+  round_body<OffT, false>(n, xadj, adjncy, adjwgt, vwgt, own, match, prev, cur, chunks, rl, nchunks, part, flags, r,
+                          nshort, propose, maxvw, short_deg, nullptr, nullptr);
+}
+
+// Launch r over the frontier (vertex regions and compacted lists).
+template <typename OffT>
+__global__ __launch_bounds__(256, 7) void k_round_list(int64_t n, const OffT *__restrict__ xadj,
+                                                    const int32_t *__restrict__ adjncy,
+                                                    const int32_t *__restrict__ adjwgt,
+                                                    const int32_t *__restrict__ vwgt, MatchOwners own,
+                                                    int32_t *match, const int32_t *prev, int32_t *cur,
+                                                    const LongChunk *chunks, RowLists rl, const unsigned *nchunks,
+                                                    LongPartial part, unsigned *flags, int r, unsigned nshort,
+                                                    int propose, int64_t maxvw, int short_deg,
+                                                    const int32_t *slist, const unsigned *vcount)
+{
+  round_body<OffT, true>(n, xadj, adjncy, adjwgt, vwgt, own, match, prev, cur, chunks, rl, nchunks, part, flags, r,
+                         nshort, propose, maxvw, short_deg, slist, vcount);
+}
+
 // out[v] = match[v], with vertices still unmatched (-1) matched to themselves;
 // out may be match itself.
 __global__ __launch_bounds__(256) void k_match_finish(int64_t n, const int32_t *match, int32_t *out)
@@ -621,6 +784,14 @@ struct MatchRun {
   unsigned *flags = nullptr, *nchunks = nullptr;
   unsigned g = 0, glong = 0;
   unsigned ghub = 0;  // blocks of k_hub_reduce (0: the level has no hub row)
+  // the frontier lists from launch COMPACT_ROUND + 1 on (large levels only)
+  bool compact = false;
+  LongChunk *chunks2 = nullptr;
+  RowLists rl2{};
+  int32_t *slist = nullptr;
+  unsigned *vcount = nullptr;    // vertices listed per region
+  unsigned *nchunks2 = nullptr;  // [0] hub chunks, [1 + k] class k
+  unsigned gs = 0;               // short blocks of the list launches: one per vertex region
   int r = 0;  // launches queued so far
   std::vector<void *> blocks;
 };
@@ -655,13 +826,25 @@ static void match_start(cm_ctx *ctx, MatchRun &mr)
   mr.part.k1 = (uint64_t *)al(sizeof(uint64_t) * cap);
   mr.part.k2 = (uint64_t *)al(sizeof(uint64_t) * cap);
   mr.part.scanned = (unsigned *)al(sizeof(unsigned) * cap);
-  mr.flags = (unsigned *)al(sizeof(unsigned) * (2 * (size_t)mr.max_rounds + 8));
+  mr.flags = (unsigned *)al(sizeof(unsigned) * (2 * (size_t)mr.max_rounds + 16));
   mr.nchunks = mr.flags + 2 * (size_t)mr.max_rounds;  // [0] hub chunks, [1 + k] rows of class k
+  mr.nchunks2 = mr.nchunks + 8;
+  static_assert(NCLASS + 1 <= 8, "list counts");
+  mr.compact = n >= COMPACT_MIN_N && mr.max_rounds > COMPACT_ROUND + 1;
+  if (mr.compact) {
+    mr.slist = (int32_t *)al(sizeof(int32_t) * (size_t)nblocks(n, VREGION) * VREGION);
+    mr.vcount = (unsigned *)al(sizeof(unsigned) * nblocks(n, VREGION));
+    mr.chunks2 = (LongChunk *)al(sizeof(LongChunk) * cap);
+    for (int k = 0; k < NCLASS; ++k) {
+      const int64_t ck = has_long ? std::min<int64_t>(n, L.m / (class_lo(k) + 1)) + 1 : 1;
+      mr.rl2.rows[k] = (int32_t *)al(sizeof(int32_t) * ck);
+    }
+  }
   const bool has_hub = L.maxdeg < 0 || L.maxdeg > LONG_CHUNK;
   if (has_hub) fill_u32(ctx, mr.part.scanned, 0u, cap);  // no launch number is 0
   mr.g = nblocks(n, 256);
   launch(ctx, "match_init", 8.0 * n, k_match_init, dim3(mr.g), dim3(256), 0, n, mr.match, mr.prop, mr.flags,
-         2 * (int64_t)mr.max_rounds + 8, ctx->d_ctr + HASH_CTR_SLOT);
+         2 * (int64_t)mr.max_rounds + 16, ctx->d_ctr + HASH_CTR_SLOT);
   ctx->hash_ctr_zero = true;
   // extra blocks whose waves stride over the long-row chunks (none when the
   // level is known to have no long row)
@@ -672,6 +855,7 @@ static void match_start(cm_ctx *ctx, MatchRun &mr)
     mr.glong = (unsigned)std::min<int64_t>(8192, std::max<int64_t>(16, (items + 3) / 4));  // a wave each
     if (has_hub) mr.ghub = (unsigned)std::min<int64_t>(4096, (cap + 3) / 4);
   }
+  mr.gs = nblocks(n, VREGION);
 }
 
 // Queues up to `count` more launches (launch r: accept of round r-1,
@@ -686,14 +870,26 @@ static void match_launch(cm_ctx *ctx, MatchRun &mr, int count)
     const int r = mr.r;
     int32_t *cur = mr.prop + (size_t)(r & 1) * n;
     const int32_t *prv = mr.prop + (size_t)((r + 1) & 1) * n;
-    launch(ctx, "match_round", r == 0 ? row_bytes : 0.0, k_round<OffT>, dim3(mr.g + mr.glong), dim3(256), 0, n,
-           (const OffT *)L.xadj, L.adjncy, L.adjwgt, L.vwgt, mr.own, mr.match, prv, cur,
-           (const LongChunk *)mr.chunks, mr.rl, (const unsigned *)mr.nchunks, mr.part, mr.flags, r, mr.g,
-           r < mr.max_rounds ? 1 : 0, L.maxvw, mr.short_deg);
+    const bool lst = mr.compact && r > COMPACT_ROUND;
+    const LongChunk *chunks = lst ? mr.chunks2 : mr.chunks;
+    const unsigned *counts = lst ? mr.nchunks2 : mr.nchunks;
+    const unsigned gshort = lst ? mr.gs : mr.g, glong = mr.glong;
+    if (lst)
+      launch(ctx, "match_round", 0.0, k_round_list<OffT>, dim3(gshort + glong), dim3(256), 0, n, (const OffT *)L.xadj,
+             L.adjncy, L.adjwgt, L.vwgt, mr.own, mr.match, prv, cur, chunks, mr.rl2, counts, mr.part, mr.flags, r,
+             gshort, r < mr.max_rounds ? 1 : 0, L.maxvw, mr.short_deg, (const int32_t *)mr.slist,
+             (const unsigned *)mr.vcount);
+    else
+      launch(ctx, "match_round", r == 0 ? row_bytes : 0.0, k_round<OffT>, dim3(gshort + glong), dim3(256), 0, n,
+             (const OffT *)L.xadj, L.adjncy, L.adjwgt, L.vwgt, mr.own, mr.match, prv, cur, chunks, mr.rl, counts,
+             mr.part, mr.flags, r, gshort, r < mr.max_rounds ? 1 : 0, L.maxvw, mr.short_deg);
     if (mr.ghub && r < mr.max_rounds)
-      launch(ctx, "match_hub_reduce", 0.0, k_hub_reduce, dim3(mr.ghub), dim3(256), 0,
-             (const LongChunk *)mr.chunks, (const unsigned *)mr.nchunks, mr.part, cur, (const unsigned *)mr.flags,
-             r);
+      launch(ctx, "match_hub_reduce", 0.0, k_hub_reduce, dim3(mr.ghub), dim3(256), 0, chunks, counts, mr.part,
+             cur, (const unsigned *)mr.flags, r);
+    if (mr.compact && r == COMPACT_ROUND)
+      launch(ctx, "match_compact", 4.0 * n, k_compact, dim3(mr.gs + (1 + NCLASS) * LIST_BLOCKS), dim3(256), 0, n,
+             (const int32_t *)mr.match, (const unsigned *)mr.flags, r, (const LongChunk *)mr.chunks, mr.rl,
+             (const unsigned *)mr.nchunks, mr.chunks2, mr.rl2, mr.nchunks2, mr.slist, mr.vcount, mr.gs);
   }
 }
 

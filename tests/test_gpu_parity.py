@@ -124,6 +124,34 @@ def test_match_small_mid_rows(engine, orc, wmax):
         assert r_gpu == r_orc
 
 
+@pytest.mark.parametrize("name", ["rmat20", "grid1024"])
+def test_match_frontier(engine, orc, name):
+    """Levels of 2^20 vertices or more list what is still unmatched after
+    round 5 (vertices, class rows, hub chunks) and run the later rounds over
+    those lists: the matching and the round count must be those of the
+    oracle.  R-MAT scale 20 has every row class and hubs; the 1024^2 grid
+    has only thread-per-row rows."""
+    if name == "rmat20":
+        g = _symmetric_weights(graphs.rmat_fast(20, 8, seed=9, offsets=np.int32), 7, seed=3)
+    else:
+        g = with_weights(graphs.grid2d(1024, 1024, wmax=100, seed=2))
+    assert g.n >= 1 << 20
+    say(f"[match frontier] case={name} n={g.n} m={g.m}")
+    mv = maxv_for(g, orc=orc)
+    for seed in (5, 77):
+        engine.set_timing(True)
+        engine.reset_stats()
+        try:
+            m_gpu, r_gpu = engine.match(g, mv, seed=seed)
+            stats = engine.kernel_stats()
+        finally:
+            engine.set_timing(False)
+        m_orc, r_orc = orc.handshake_match(g.xadj, g.adjncy, g.adjwgt, g.vwgt, mv, seed=seed)
+        assert r_orc >= 7 and "match_compact" in stats  # list launches did accepts
+        np.testing.assert_array_equal(m_gpu, m_orc)
+        assert r_gpu == r_orc
+
+
 @pytest.mark.parametrize("hub_deg,side", [(140000, 400), (600000, 800), (1100000, 1100)])
 def test_contract_hub_partitions(engine, orc, hub_deg, side):
     """Hub segments cut into many key partitions: 35 and 147 partitions
