@@ -457,6 +457,10 @@ int ref_threshold(int64_t n, int nparts)
 // Levels with at most this many entries run their matching rounds without a
 // host check before the contraction (see coarsen_level).
 constexpr int64_t SPECULATE_EDGES = 1 << 24;
+// Launches in the first batch of a larger level.  A launch after the last
+// live round returns at once (4-7 us on such a level), while a second batch
+// costs a host synchronisation (about 40 us) first: R-MAT s22 needs 13.
+constexpr int LARGE_LEVEL_BATCH = 13;
 
 static Level coarsen_level(cm_ctx *ctx, cm_hier *h, Level &F, int32_t maxvwgt, uint64_t seed,
                            int max_rounds, int row_order, int batch)
@@ -492,7 +496,8 @@ static Level coarsen_level(cm_ctx *ctx, cm_hier *h, Level &F, int32_t maxvwgt, u
     // synchronisation costs little, check the rounds first.
     const bool speculate = m <= SPECULATE_EDGES;
     if (n > 0) {
-      match_queue(ctx, mr, batch, !speculate);  // speculative: flags come with the contraction's read-back
+      // speculative: flags come with the contraction's read-back
+      match_queue(ctx, mr, speculate ? batch : std::max(batch, LARGE_LEVEL_BATCH), !speculate);
       if (!speculate) sync(ctx);
     }
     for (int pass = 0;; ++pass) {
