@@ -202,6 +202,7 @@ struct MatchOwners {
   const int64_t *vbase = nullptr;  // per slot: first vertex in the union
   int32_t maxv0 = 0;
   uint64_t seed0 = 0;
+  bool wunit = false;  // every edge weight is 1 (the level uses the shared unit array)
 };
 
 // ---- subsystem entry points (device pointers, ctx stream) ----

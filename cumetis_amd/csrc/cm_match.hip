@@ -298,8 +298,8 @@ __device__ __forceinline__ void scan_row(int64_t v, OffT b, OffT e, int l0, int 
                                          const int32_t *__restrict__ adjwgt,
                                          const int32_t *__restrict__ vwgt, int32_t vw, int32_t maxv,
                                          uint64_t seed, int64_t base, const int32_t *match,
-                                         const int32_t *prev, bool have_prev, bool check_w, int32_t &best,
-                                         uint64_t &bk1, uint64_t &bk2)
+                                         const int32_t *prev, bool have_prev, bool check_w, bool wunit,
+                                         int32_t &best, uint64_t &bk1, uint64_t &bk2)
 {
   // SCAN_BATCH entries per lane per step: their row loads and the gathers
   // that depend on them are issued together, so a lane waits once per batch
@@ -315,7 +315,9 @@ __device__ __forceinline__ void scan_row(int64_t v, OffT b, OffT e, int l0, int 
       const OffT j = j0 + (OffT)t * step;
       const OffT jj = j < e ? j : e - 1;
       const int32_t a = adjncy[jj];
-      const uint32_t x = (uint32_t)adjwgt[jj];
+      // unit weights: every lane reads the first word (one cache line)
+      // instead of streaming a second array of ones
+      const uint32_t x = (uint32_t)adjwgt[wunit ? 0 : jj];
       u[t] = j < e ? a : -1;
       w[t] = j < e ? x : 0u;
     }
@@ -416,7 +418,7 @@ __device__ __forceinline__ void hub_chunk(int64_t q, const LongChunk *chunks, co
   const OffT b = xb + (OffT)c.k * LONG_CHUNK, e0 = xe;
   const OffT e = e0 - b > (OffT)LONG_CHUNK ? b + (OffT)LONG_CHUNK : e0;
   scan_row<OffT>(v, b, e, l, CM_WAVE, adjncy, adjwgt, vwgt, vw, maxv, seed, base, match, prev, have_prev,
-                 maxvw < 0 || (int64_t)vw + maxvw > maxv, best, bk1, bk2);
+                 maxvw < 0 || (int64_t)vw + maxvw > maxv, own.wunit, best, bk1, bk2);
   wave_best(best, bk1, bk2);
   if (l == 0) {
     part.best[q] = best;
@@ -498,7 +500,7 @@ __device__ __forceinline__ void mid_group(int64_t m0, unsigned nm, const int32_t
     int32_t best = -2;
     uint64_t bk1 = 0, bk2 = 0;
     scan_row<OffT>(vi, bi, ei, l, CM_WAVE, adjncy, adjwgt, vwgt, bcast(vw, i), bcast(maxv, i), bcast(seed, i),
-                   bcast(base, i), match, prev, have_prev, bcast(cw, i) != 0, best, bk1, bk2);
+                   bcast(base, i), match, prev, have_prev, bcast(cw, i) != 0, own.wunit, best, bk1, bk2);
     wave_best(best, bk1, bk2);
     if (l == 0) cur[vi] = best;
   }
@@ -573,7 +575,7 @@ __device__ __forceinline__ void small_group(int64_t m0, unsigned ns, const int32
     uint64_t bk1 = 0, bk2 = 0;
     if (i >= 0)
       scan_row<OffT>(vi, (OffT)bi, (OffT)ei, gl, SUB, adjncy, adjwgt, vwgt, vwi, maxvi, seedi, basei, match, prev,
-                     have_prev, cwi != 0, best, bk1, bk2);
+                     have_prev, cwi != 0, own.wunit, best, bk1, bk2);
     group_best<SUB>(best, bk1, bk2);
     if (i >= 0 && gl == 0) cur[vi] = best;
   }
@@ -634,7 +636,7 @@ __device__ __forceinline__ void short_row(int64_t v, int64_t n, const OffT *__re
         int32_t best = -2;
         uint64_t bk1 = 0, bk2 = 0;
         scan_row<OffT>(v, b, e, 0, 1, adjncy, adjwgt, vwgt, vw, maxv, seed, base, match, prev, have_prev,
-                       maxvw < 0 || (int64_t)vw + maxvw > maxv, best, bk1, bk2);
+                       maxvw < 0 || (int64_t)vw + maxvw > maxv, own.wunit, best, bk1, bk2);
         cur[v] = best;
       }
     }
@@ -808,6 +810,9 @@ static void match_start(cm_ctx *ctx, MatchRun &mr)
     return p;
   };
   mr.prop = (int32_t *)al(sizeof(int32_t) * 2 * n);  // ping-pong
+  // the shared unit array is never written, so a level that points to it has
+  // unit edge weights
+  mr.own.wunit = L.m > 0 && L.adjwgt == ctx->ones;
   // chunks: sum over rows longer than LONG_CHUNK of ceil(d / LONG_CHUNK)
   const bool has_long = L.maxdeg < 0 || L.maxdeg > LONG_DEG;
   // hub chunks: at most m / LONG_CHUNK full ones plus one partial per hub

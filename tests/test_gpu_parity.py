@@ -124,6 +124,30 @@ def test_match_small_mid_rows(engine, orc, wmax):
         assert r_gpu == r_orc
 
 
+@pytest.mark.parametrize("name", ["rmat13", "hub", "rmat20"])
+def test_match_unit_weights(engine, orc, name):
+    """Graphs given without weights use the shared array of ones, and the row
+    scans then read its first word instead of streaming it: the matching must
+    be the oracle's with explicit unit weights (rmat20 also runs the frontier
+    rounds)."""
+    from cases import hub_graph
+    if name == "rmat13":
+        g = graphs.rmat(13, 16, seed=8, offsets=np.int32)
+    elif name == "hub":
+        g = hub_graph(seed=22, side=300, hub_deg=50000, mids=(2048, 2049, 300, 100))
+    else:
+        g = graphs.rmat_fast(20, 8, seed=10, offsets=np.int32)
+    g = Graph(g.xadj, g.adjncy, None, None)
+    say(f"[match unit weights] case={name} n={g.n} m={g.m}")
+    gw = with_weights(g)
+    mv = maxv_for(gw, orc=orc)
+    for seed in (3, 41):
+        m_gpu, r_gpu = engine.match(g, mv, seed=seed)
+        m_orc, r_orc = orc.handshake_match(gw.xadj, gw.adjncy, gw.adjwgt, gw.vwgt, mv, seed=seed)
+        np.testing.assert_array_equal(m_gpu, m_orc)
+        assert r_gpu == r_orc
+
+
 @pytest.mark.parametrize("name", ["rmat20", "grid1024"])
 def test_match_frontier(engine, orc, name):
     """Levels of 2^20 vertices or more list what is still unmatched after
