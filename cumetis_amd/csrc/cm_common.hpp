@@ -137,6 +137,14 @@ __device__ __forceinline__ int tile_ticket(unsigned *ctr)
   return s_tile;
 }
 
+// Edge weight j; adjwgt == nullptr stands for unit weights (the contraction
+// is given no array when the level uses the shared array of ones, so it does
+// not stream that array).
+__device__ __forceinline__ int32_t edge_w(const int32_t *__restrict__ adjwgt, int64_t j)
+{
+  return adjwgt ? adjwgt[j] : 1;
+}
+
 // ------------------------------------------------------ block scan (NW waves)
 // Exclusive block-wide scan for a block of NW waves; returns the exclusive
 // prefix of `v`; `total` receives the block sum.  Uses `sh` (NW entries).

@@ -273,7 +273,7 @@ __global__ __launch_bounds__(256) void k_gather(int64_t n, const OffT *__restric
   for (OffT j = b; j < e; ++j) {
     const uint32_t pos = local + (uint32_t)(j - b);
     keys[s0 + pos] = ((uint64_t)(uint32_t)cmap[adjncy[j]] << 32) | pos;
-    vals[s0 + pos] = (uint32_t)adjwgt[j];
+    vals[s0 + pos] = (uint32_t)edge_w(adjwgt, j);
   }
 }
 
@@ -300,7 +300,7 @@ __global__ __launch_bounds__(256) void k_gather_long(const int32_t *list, const 
     for (OffT j = b + l; j < e; j += CM_WAVE) {
       const uint32_t pos = local + (uint32_t)(j - b);
       keys[s0 + pos] = ((uint64_t)(uint32_t)cmap[adjncy[j]] << 32) | pos;
-      vals[s0 + pos] = (uint32_t)adjwgt[j];
+      vals[s0 + pos] = (uint32_t)edge_w(adjwgt, j);
     }
   }
 }
@@ -580,7 +580,7 @@ __device__ __forceinline__ int merge_regs(int32_t c, int32_t v, int32_t u, const
     const bool valid = i < d0 + d1;
     const int64_t j = valid ? (i < d0 ? b0 + i : b1 + (i - d0)) : 0;
     a[i] = adjncy[j];
-    wt[i] = adjwgt[j];
+    wt[i] = edge_w(adjwgt, j);
   }
 #pragma unroll
   for (int i = 0; i < N; ++i) {
@@ -660,7 +660,7 @@ __device__ __forceinline__ int merge_fs(int32_t c, int32_t v, int32_t u, const O
   for (int i = 0; i < N; ++i) {  // unconditional loads (see merge_regs)
     const int64_t j = i < cnt ? (i < d0 ? b0 + i : b1 + (i - d0)) : 0;
     a[i] = adjncy[j];
-    sum[i] = adjwgt[j];
+    sum[i] = edge_w(adjwgt, j);
   }
 #pragma unroll
   for (int i = 0; i < N; ++i) {
@@ -1056,9 +1056,14 @@ int64_t contract(cm_ctx *ctx, const Level &L, const int32_t *d_match, const int3
                  void *d_cxadj, int32_t *d_cadjncy, int32_t *d_cadjwgt, int64_t *maxdeg, int64_t *ncp,
                  HashContract *pre)
 {
-  return L.off64 ? contract_impl(ctx, L, d_match, d_cmap, nc, d_rep, (const int64_t *)d_segoff, row_order,
+  // a level on the shared array of ones (never written) has unit edge
+  // weights: the kernels take no array then (edge_w) and stream one array
+  // less
+  Level W = L;
+  if (W.m > 0 && W.adjwgt == ctx->ones) W.adjwgt = nullptr;
+  return W.off64 ? contract_impl(ctx, W, d_match, d_cmap, nc, d_rep, (const int64_t *)d_segoff, row_order,
                                  (int64_t *)d_cxadj, d_cadjncy, d_cadjwgt, maxdeg, ncp, false, pre)
-                 : contract_impl(ctx, L, d_match, d_cmap, nc, d_rep, (const int32_t *)d_segoff, row_order,
+                 : contract_impl(ctx, W, d_match, d_cmap, nc, d_rep, (const int32_t *)d_segoff, row_order,
                                  (int32_t *)d_cxadj, d_cadjncy, d_cadjwgt, maxdeg, ncp, false, pre);
 }
 

@@ -185,7 +185,7 @@ __global__ __launch_bounds__(256) void k_hc_gather(const SegChunk *__restrict__ 
       const int64_t e = i0 + ii;
       const int64_t j = e < d0 ? b0 + e : b1 + (e - d0);
       a[t] = adjncy[j];
-      w[t] = adjwgt[j];
+      w[t] = edge_w(adjwgt, j);
     }
 #pragma unroll
     for (int t = 0; t < PER; ++t) cv[t] = cmap[a[t]];
@@ -272,7 +272,7 @@ __global__ __launch_bounds__(256) void k_hc_dedup_wave(const int32_t *__restrict
       const int i = l + t * CM_WAVE < len ? l + t * CM_WAVE : len - 1;
       const int64_t j = i < sr.d0 ? sr.b0 + i : sr.b1 + (i - sr.d0);
       a[t] = adjncy[j];
-      wt[t] = adjwgt[j];
+      wt[t] = edge_w(adjwgt, j);
     }
 #pragma unroll
     for (int t = 0; t < PER; ++t) {
@@ -352,7 +352,7 @@ __global__ __launch_bounds__(256) void k_hc_dedup_block(const int32_t *__restric
       const int i = (int)threadIdx.x + t * 256 < len ? (int)threadIdx.x + t * 256 : len - 1;
       const int64_t j = i < sr.d0 ? sr.b0 + i : sr.b1 + (i - sr.d0);
       a[t] = adjncy[j];
-      wt[t] = adjwgt[j];
+      wt[t] = edge_w(adjwgt, j);
     }
 #pragma unroll
     for (int t = 0; t < PER; ++t) {

@@ -148,6 +148,35 @@ def test_match_unit_weights(engine, orc, name):
         assert r_gpu == r_orc
 
 
+@pytest.mark.parametrize("order", [CM_ROW_FIRST_SEEN, CM_ROW_ASCENDING])
+@pytest.mark.parametrize("name", ["rmat13", "hub"])
+def test_contract_unit_weights(engine, orc, name, order):
+    """A graph given without weights: the contraction kernels get no weight
+    array and count every edge as 1.  The coarse graph must be the oracle's
+    with explicit unit weights, on the register, hash (segments of all
+    classes, a hub segment) and sorted paths."""
+    from cases import hub_graph
+    if name == "rmat13":
+        g = graphs.rmat(13, 16, seed=12, offsets=np.int32)
+    else:
+        g = hub_graph(seed=24, side=300, hub_deg=50000, mids=(2048, 2049, 300, 100))
+    g = Graph(g.xadj, g.adjncy, None, None)
+    gw = with_weights(g)
+    say(f"[contract unit weights] case={name} order={order} n={g.n} m={g.m}")
+    mv = maxv_for(gw, orc=orc)
+    match, _ = orc.handshake_match(gw.xadj, gw.adjncy, gw.adjwgt, gw.vwgt, mv, seed=13)
+    cmap, nc = orc.number(match)
+    cx, ca, cw, cv = engine.contract(g, match, cmap, nc, order)
+    ex, ea, ew, ev = orc.cpu_contract(gw.xadj, gw.adjncy, gw.adjwgt, gw.vwgt, match, cmap, nc)
+    np.testing.assert_array_equal(cx.astype(np.int64), ex)
+    np.testing.assert_array_equal(cv, ev)
+    if order == CM_ROW_ASCENDING:
+        ea, ew = sort_rows(ex, ea, ew)
+    np.testing.assert_array_equal(ca, ea)
+    np.testing.assert_array_equal(cw, ew)
+    assert cw.sum() == g.m - int((cmap[np.repeat(np.arange(g.n), np.diff(g.xadj))] == cmap[g.adjncy]).sum())
+
+
 @pytest.mark.parametrize("name", ["rmat20", "grid1024"])
 def test_match_frontier(engine, orc, name):
     """Levels of 2^20 vertices or more list what is still unmatched after
