@@ -47,7 +47,9 @@ typedef struct cm_ctx cm_ctx;
 typedef struct cm_hier cm_hier;
 
 /* A CSR graph.  xadj has n+1 entries of the offset type; adjwgt/vwgt may be
- * NULL for unit weights (cu:479-503).  on_device selects device or host
+ * NULL for unit weights (cu:479-503); with adjwgt NULL the first level's
+ * matching and contraction read no weight array, so pass NULL rather than an
+ * array of ones.  on_device selects device or host
  * pointers.  The graph must be symmetric, without self loops, with positive
  * edge weights. */
 typedef struct {
