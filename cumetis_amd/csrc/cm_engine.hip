@@ -325,8 +325,10 @@ static void validate_view(const cm_graph_view *g)
 // Level 0 of a hierarchy from a view: device views are referenced (unit
 // weights are materialised), host views are copied into pool memory.
 // mx[0] = longest row, mx[-1] = largest vertex weight (vwgt != nullptr).
-// Tiles of 4096 rows pass running maxima along two look-back chains; the
+// Tiles of 1024 * MAXDEG_PER_THREAD rows pass running maxima along two look-back chains; the
 // last tile writes them (no word updated by two tiles).
+constexpr int MAXDEG_PER_THREAD = 8;  // vertices per thread: half the tiles of 4 (16: no gain)
+
 template <typename OffT>
 __global__ __launch_bounds__(1024) void k_max_degree(int64_t n, const OffT *xadj, const int32_t *vwgt, int64_t *mx,
                                                      uint64_t *st, unsigned *ctr, uint32_t epoch)
@@ -334,8 +336,9 @@ __global__ __launch_bounds__(1024) void k_max_degree(int64_t n, const OffT *xadj
   __shared__ uint64_t sh[16], shw[16];
   const int tile = tile_ticket(ctr);
   uint64_t d = 0, vw = 0;
-  for (int t = 0; t < 4; ++t) {
-    const int64_t v = ((int64_t)tile * 1024 + threadIdx.x) * 4 + t;
+#pragma unroll
+  for (int t = 0; t < MAXDEG_PER_THREAD; ++t) {
+    const int64_t v = ((int64_t)tile * 1024 + threadIdx.x) * MAXDEG_PER_THREAD + t;
     if (v < n) {
       const uint64_t x = (uint64_t)(xadj[v + 1] - xadj[v]);
       d = x > d ? x : d;
@@ -371,7 +374,7 @@ void queue_max_degree(cm_ctx *ctx, int64_t n, bool off64, const void *xadj, cons
 {
   fill_u32(ctx, ctx->d_small + 3, 0u, 4);
   if (n > 0) {
-    const int64_t tiles = (n + 4095) / 4096;
+    const int64_t tiles = (n + 1024 * MAXDEG_PER_THREAD - 1) / (1024 * MAXDEG_PER_THREAD);
     const uint32_t ep = lookback_begin(ctx, 2 * tiles);
     if (off64)
       launch(ctx, "max_degree", 0.0, k_max_degree<int64_t>, dim3((unsigned)tiles), dim3(1024), 0, n,
