@@ -327,7 +327,7 @@ static void validate_view(const cm_graph_view *g)
 // mx[0] = longest row, mx[-1] = largest vertex weight (vwgt != nullptr).
 // Tiles of 1024 * MAXDEG_PER_THREAD rows pass running maxima along two look-back chains; the
 // last tile writes them (no word updated by two tiles).
-constexpr int MAXDEG_PER_THREAD = 8;  // vertices per thread: half the tiles of 4 (16: no gain)
+constexpr int MAXDEG_PER_THREAD = 8;  // vertices per thread (faster than 4 or 16 on grid and R-MAT)
 
 template <typename OffT>
 __global__ __launch_bounds__(1024) void k_max_degree(int64_t n, const OffT *xadj, const int32_t *vwgt, int64_t *mx,
