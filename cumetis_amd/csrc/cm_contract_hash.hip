@@ -149,7 +149,8 @@ __global__ __launch_bounds__(256) void k_hc_select(const int64_t *__restrict__ n
 // the partition starts in coff[q * (HC_PMAX + 1) + p] (coff[.. + P] = the
 // chunk's length); the order inside a partition's slice is arbitrary (the
 // partition pass takes minima and sums).  Longer hubs keep their weights in
-// segment order (wv) for the whole-segment pass.
+// segment order (wv) for the whole-segment pass.  With unit weights
+// (adjwgt == nullptr) neither weight copy exists.
 template <typename OffT>
 __global__ __launch_bounds__(256) void k_hc_gather(const SegChunk *__restrict__ hchunks, const unsigned *ctr,
                                                    const OffT *__restrict__ segoff, const int32_t *__restrict__ rep,
@@ -194,7 +195,7 @@ __global__ __launch_bounds__(256) void k_hc_gather(const SegChunk *__restrict__ 
       const int i = t * 256 + tid;
       if (i < cn) {
         cval[s0 + i0 + i] = cv[t];
-        if (!bucket) wv[s0 + i0 + i] = w[t];
+        if (!bucket && wv) wv[s0 + i0 + i] = w[t];
       }
     }
     if (!bucket) continue;  // uniform over the block
@@ -225,7 +226,7 @@ __global__ __launch_bounds__(256) void k_hc_gather(const SegChunk *__restrict__ 
         const int64_t o = s0 + i0 + (int64_t)(s_cnt[pp[t]] + rk[t]);
         bpos[o] = (int32_t)(i0 + i);
         bkey[o] = cv[t];
-        bw[o] = w[t];
+        if (bw) bw[o] = w[t];  // none for unit weights
       }
     }
     __syncthreads();  // s_cnt is reused by the next chunk
@@ -504,7 +505,7 @@ __global__ __launch_bounds__(HC_HUB_THREADS) void k_hc_dedup_hub(const HubPart *
           else hi = mid - 1;
         }
         const int64_t o = ok ? s0 + (int64_t)lo * HC_CHUNK + s_sb[lo] + (f - s_so[lo]) : s0;
-        take(ok, bkey[o], bpos[o], bw[o]);
+        take(ok, bkey[o], bpos[o], bw ? bw[o] : 1);
       }
     } else {
       for (int64_t ib = threadIdx.x; ib < len; ib += SCAN * T) {
@@ -518,7 +519,7 @@ __global__ __launch_bounds__(HC_HUB_THREADS) void k_hc_dedup_hub(const HubPart *
         for (int t = 0; t < SCAN; ++t) {
           const int64_t i = ib + (int64_t)t * T;
           const bool ok = cu[t] >= 0 && hc_part(cu[t], hp.P) == hp.p;
-          take(ok, cu[t], (int32_t)i, ok ? wv[s0 + i] : 0);
+          take(ok, cu[t], (int32_t)i, ok ? (wv ? wv[s0 + i] : 1) : 0);
         }
       }
     }
@@ -824,8 +825,10 @@ void hash_prepare_impl(cm_ctx *ctx, const Level &L, const int32_t *match, const 
            (const unsigned *)ctr + HC_N_BLOCKL, segoff, rep, match, xadj, L.adjncy, L.adjwgt, cmap, hc.tc, hc.tw,
            cdeg);
   if (m > HC_BLOCK_MAX && seg_max > HC_BLOCK_MAX) {  // a hub needs more than HC_BLOCK_MAX entries
-    int32_t *cval = (int32_t *)al(4 * m), *wv = (int32_t *)al(4 * m);
-    int32_t *bpos = (int32_t *)al(4 * m), *bkey = (int32_t *)al(4 * m), *bw = (int32_t *)al(4 * m);
+    // unit weights (adjwgt == nullptr): no weight copies; the dedup counts 1 per entry
+    int32_t *cval = (int32_t *)al(4 * m), *wv = L.adjwgt ? (int32_t *)al(4 * m) : nullptr;
+    int32_t *bpos = (int32_t *)al(4 * m), *bkey = (int32_t *)al(4 * m);
+    int32_t *bw = L.adjwgt ? (int32_t *)al(4 * m) : nullptr;
     uint16_t *coff_b = (uint16_t *)al(sizeof(uint16_t) * (HC_PMAX + 1) * b.nhchunk);
     launch(ctx, "hc_gather", 0.0, k_hc_gather<OffT>, dim3((unsigned)std::min<int64_t>(8192, b.nhchunk)), dim3(256),
            0, (const SegChunk *)hl.hchunks, (const unsigned *)ctr, segoff, rep, match, xadj, L.adjncy, L.adjwgt, cmap,

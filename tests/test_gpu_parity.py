@@ -205,20 +205,25 @@ def test_match_frontier(engine, orc, name):
         assert r_gpu == r_orc
 
 
+@pytest.mark.parametrize("unit", [False, True])
 @pytest.mark.parametrize("hub_deg,side", [(140000, 400), (600000, 800), (1100000, 1100)])
-def test_contract_hub_partitions(engine, orc, hub_deg, side):
+def test_contract_hub_partitions(engine, orc, hub_deg, side, unit):
     """Hub segments cut into many key partitions: 35 and 147 partitions
     (entries bucketed by partition in each chunk), the second past the length
     whose entries can be listed in LDS (2^19), so the second pass reads the
     bucket slices again; and 269 partitions, more than are bucketed, so every
-    partition block reads the whole segment."""
+    partition block reads the whole segment.  Unit: the graph is given
+    without weights, so the hub pass keeps no weight copies."""
     from cases import hub_graph
-    g = with_weights(hub_graph(seed=23, side=side, hub_deg=hub_deg, mids=(5000, 2100, 1500, 600)))
-    say(f"[contract hub partitions] hub_deg={hub_deg} n={g.n} m={g.m}")
+    g0 = hub_graph(seed=23, side=side, hub_deg=hub_deg, mids=(5000, 2100, 1500, 600))
+    if unit:
+        g0 = Graph(g0.xadj, g0.adjncy, None, None)
+    g = with_weights(g0)
+    say(f"[contract hub partitions] hub_deg={hub_deg} unit={unit} n={g.n} m={g.m}")
     mv = maxv_for(g, orc=orc)
     match, _ = orc.handshake_match(g.xadj, g.adjncy, g.adjwgt, g.vwgt, mv, seed=3)
     cmap, nc = orc.number(match)
-    cx, ca, cw, cv = engine.contract(g, match, cmap, nc, CM_ROW_FIRST_SEEN)
+    cx, ca, cw, cv = engine.contract(g0, match, cmap, nc, CM_ROW_FIRST_SEEN)
     ex, ea, ew, ev = orc.cpu_contract(g.xadj, g.adjncy, g.adjwgt, g.vwgt, match, cmap, nc)
     np.testing.assert_array_equal(cx.astype(np.int64), ex)
     np.testing.assert_array_equal(cv, ev)
