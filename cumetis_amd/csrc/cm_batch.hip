@@ -36,6 +36,43 @@ __device__ __forceinline__ int find_slot(const int64_t *base, int nslots, int64_
   return lo;
 }
 
+// Validation of a concatenated input before any id is made global: for
+// every vertex v of graph s (bad |= 1 offsets, 2 id outside [0, n_s), 4 self
+// loop, 8 weight <= 0): the row lies inside the graph's entries
+// [eoff[s], eoff[s+1]) and ascends, the graph's first row starts at eoff[s],
+// and every local id is a vertex of the graph other than v itself.  Thread N
+// checks xadj[N] == M.  Empty graphs own no vertex: find_slot gives the
+// graph that does.
+template <typename OffT>
+__global__ __launch_bounds__(256) void k_check_batch(int64_t N, int64_t M, const OffT *xadj, const int32_t *adj_local,
+                                                     const int32_t *wgt, const int64_t *voff, const int64_t *eoff,
+                                                     int nb, unsigned *bad)
+{
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v > N) return;
+  unsigned f = 0;
+  if (v == N) {
+    if ((int64_t)xadj[N] != M) f |= 1;
+  } else {
+    const int s = find_slot(voff, nb, v);
+    const int64_t b = (int64_t)xadj[v], e = (int64_t)xadj[v + 1];
+    const int64_t lo = eoff[s], hi = eoff[s + 1];
+    if (v == voff[s] && b != lo) f |= 1;
+    if (b > e || b < lo || e > hi) {
+      f |= 1;
+    } else {
+      const int64_t nv = voff[s + 1] - voff[s], self = v - voff[s];
+      for (int64_t j = b; j < e; ++j) {
+        const int32_t u = adj_local[j];
+        if (u < 0 || u >= nv) f |= 2;
+        else if (u == self) f |= 4;
+        if (wgt && wgt[j] <= 0) f |= 8;
+      }
+    }
+  }
+  if (f) atomicOr(bad, f);
+}
+
 // level 0: local -> global neighbour ids, and the graph slot of each vertex
 __global__ __launch_bounds__(256) void k_globalize(int64_t m, const int32_t *adj_local,
                                                    const int64_t *eoff, const int64_t *voff, int nb,
@@ -150,6 +187,22 @@ static void batch_impl(cm_ctx *ctx, int nb, const cm_graph_view *g, const int64_
     if (N) CM_HIP(hipMemcpyAsync(vw, g->vwgt, 4 * N, kind, ctx->stream));
   } else {
     fill_u32(ctx, vw, 1u, N);
+  }
+  bool trusted = true;
+  for (int b = 0; b < nb; ++b) trusted = trusted && (params[b].flags & CM_FLAG_TRUSTED_INPUT);
+  if (!trusted) {
+    unsigned *bad = ctx->d_ctr + 3;
+    fill_u32(ctx, bad, 0u, 1);
+    launch(ctx, "validate_batch", 0.0, k_check_batch<OffT>, dim3(nblocks(N + 1, 256)), dim3(256), 0, N, M,
+           (const OffT *)xadj, (const int32_t *)adj_local, (const int32_t *)(g->adjwgt ? wgt : nullptr),
+           (const int64_t *)d_voff, (const int64_t *)d_eoff, nb, bad);
+    unsigned h = 0;
+    CM_HIP(hipMemcpyAsync(&h, bad, sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
+    sync(ctx);
+    CM_REQUIRE(!(h & 1), CM_ERR_INPUT, "batch offsets: a graph's rows must lie in [eoff[b], eoff[b+1]) and ascend");
+    CM_REQUIRE(!(h & 2), CM_ERR_INPUT, "batch adjncy holds a local vertex id outside [0, n_b)");
+    CM_REQUIRE(!(h & 4), CM_ERR_INPUT, "a graph of the batch has a self loop");
+    CM_REQUIRE(!(h & 8), CM_ERR_INPUT, "edge weights must be positive");
   }
   launch(ctx, "batch_globalize", 0.0, k_globalize, dim3(nblocks(M, 256)), dim3(256), 0, M,
          (const int32_t *)adj_local, (const int64_t *)d_eoff, (const int64_t *)d_voff, nb, adj);
@@ -394,13 +447,20 @@ extern "C" int cm_coarsen_batch(cm_ctx *ctx, int32_t nb, const cm_graph_view *co
     CM_HIP(hipSetDevice(ctx->device));
     CM_REQUIRE(concat->offset_type == CM_OFF32 || concat->offset_type == CM_OFF64, CM_ERR_INVALID,
                "offset_type must be CM_OFF32 or CM_OFF64");
+    cm::validate_view(concat);
     CM_REQUIRE(voff[0] == 0 && eoff[0] == 0 && voff[nb] == concat->n && eoff[nb] == concat->m,
                CM_ERR_INVALID, "voff/eoff must start at 0 and end at n/m");
     for (int b = 0; b < nb; ++b) {
       CM_REQUIRE(voff[b] <= voff[b + 1] && eoff[b] <= eoff[b + 1], CM_ERR_INVALID, "voff/eoff must ascend");
+      CM_REQUIRE(voff[b] < voff[b + 1] || eoff[b] == eoff[b + 1], CM_ERR_INVALID,
+                 "a graph without vertices must have no entries");
       CM_REQUIRE(params[b].nparts >= 1, CM_ERR_INVALID, "nparts must be >= 1");
+      // the union's kernels run once per level for all graphs: one row
+      // order, one round cap and one level cap
       CM_REQUIRE(params[b].row_order == params[0].row_order, CM_ERR_INVALID,
                  "all graphs of a batch use one row order");
+      CM_REQUIRE(params[b].max_rounds == params[0].max_rounds && params[b].max_levels == params[0].max_levels,
+                 CM_ERR_INVALID, "all graphs of a batch use one max_rounds and one max_levels");
     }
     CM_REQUIRE(concat->n < (int64_t)INT32_MAX, CM_ERR_UNSUPPORTED, "batch vertex count must fit int32");
     if (concat->offset_type == CM_OFF64)

@@ -308,7 +308,7 @@ static void check_equal(cm_ctx *ctx, int64_t n, const int32_t *a, const int32_t 
 // ----------------------------------------------------------- graph inputs
 static int offset_bytes(int t) { return t == CM_OFF64 ? 8 : 4; }
 
-static void validate_view(const cm_graph_view *g)
+void validate_view(const cm_graph_view *g)
 {
   CM_REQUIRE(g != nullptr, CM_ERR_INVALID, "graph view is NULL");
   CM_REQUIRE(g->n >= 0 && g->m >= 0, CM_ERR_INVALID, "negative graph size");

@@ -148,6 +148,8 @@ constexpr int CTR_SLOTS = 64;
 constexpr int HASH_CTR_SLOT = 32;  // 16 counters of the hash contraction
 constexpr int SMALL_SLOTS = 4096;
 
+// Host-side checks of a graph view (sizes, offset type, NULL arrays).
+void validate_view(const cm_graph_view *g);
 void ensure_status(cm_ctx *ctx, size_t words);
 void ensure_host_flags(cm_ctx *ctx, size_t words);
 // Status words for a look-back launch over `words` tiles; returns the epoch.

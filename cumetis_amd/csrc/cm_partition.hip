@@ -12,6 +12,7 @@
 #include <string.h>
 
 #include <chrono>
+#include <mutex>
 #include <vector>
 
 #include "cm_internal.hpp"
@@ -159,14 +160,19 @@ int cm_partition(cm_ctx *ctx, const cm_graph_view *g, const cm_coarsen_params *p
   return CM_OK;
 }
 
-// cu:3622-3670.  One process-wide context on device 0.
+// cu:3622-3670.  One process-wide context on device 0, created by the first
+// call.  Calls are serialised: the context (its stream, pool and read-back
+// buffers) serves one host thread at a time, and the reference itself is not
+// reentrant (its globals, cu:4-144).
 int cuMetis_PartGraph(int *nvtxs, int *xadj, int *adjncy, int *vwgt, int *adjwgt, int *nparts,
                       float *tpwgts, float *ubvec, int *objval, int *part)
 {
   (void)tpwgts;
   (void)ubvec;
+  static std::mutex mu;
   static cm_ctx *ctx = nullptr;
   if (!nvtxs || !xadj || !nparts || !part) return CM_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(mu);
   if (!ctx) {
     int rc = cm_ctx_create(0, nullptr, &ctx);
     if (rc != CM_OK) return rc;

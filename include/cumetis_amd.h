@@ -144,11 +144,16 @@ int cm_coarsen(cm_ctx *ctx, const cm_graph_view *g, const cm_coarsen_params *p, 
  * holds the offsets of the concatenation (sum n + 1 entries, concat->m =
  * sum m), adjncy holds vertex ids local to each graph; graph b owns
  * vertices [voff[b], voff[b+1]) and entries [eoff[b], eoff[b+1]); voff and
- * eoff are host arrays of nb+1 entries; params[b] applies to graph b.  Every
- * level is one pass of each kernel over the union of the graphs still
- * coarsening; graph b stops by its own rule.  out[b] receives a hierarchy
- * whose levels (after subtracting the bases, see cm_level_info) are
- * bit-identical to cm_coarsen of graph b alone. */
+ * eoff are host arrays of nb+1 entries; params[b] applies to graph b, except
+ * that row_order, max_rounds and max_levels must agree across the batch
+ * (CM_ERR_INVALID otherwise).  Unless every params[b].flags holds
+ * CM_FLAG_TRUSTED_INPUT, each graph is checked on the device before use
+ * (rows inside its entries, local ids in [0, n_b), no self loop, positive
+ * weights; CM_ERR_INPUT otherwise).  Every level is one pass of each kernel
+ * over the union of the graphs still coarsening; graph b stops by its own
+ * rule.  out[b] receives a hierarchy whose levels (after subtracting the
+ * bases, see cm_level_info) are bit-identical to cm_coarsen of graph b
+ * alone. */
 int cm_coarsen_batch(cm_ctx *ctx, int32_t nb, const cm_graph_view *concat, const int64_t *voff,
                      const int64_t *eoff, const cm_coarsen_params *params, cm_hier **out);
 

@@ -8,7 +8,8 @@ from cumetis_amd import graphs
 from cumetis_amd.engine import Graph
 
 
-def hub_graph(seed: int = 5, side: int = 90, hub_deg: int = 6000, mids=(90, 700, 3000)) -> Graph:
+def hub_graph(seed: int = 5, side: int = 90, hub_deg: int = 6000, mids=(90, 700, 3000),
+              offsets=np.int32) -> Graph:
     """Grid plus one hub row above 4096 entries and medium rows of the given
     degrees, so that every sort bin of the contraction is exercised."""
     rng = np.random.default_rng(seed)
@@ -25,7 +26,7 @@ def hub_graph(seed: int = 5, side: int = 90, hub_deg: int = 6000, mids=(90, 700,
     u = np.concatenate(u)
     v = np.concatenate(v)
     w = rng.integers(1, 50, size=len(u))
-    return graphs.from_edges(n, u, v, w)
+    return graphs.from_edges(n, u, v, w, offsets)
 
 
 def isolated(n: int = 10) -> Graph:

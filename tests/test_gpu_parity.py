@@ -77,16 +77,18 @@ def test_match_bit_exact(engine, orc, name):
         assert r_gpu == r_orc
 
 
+@pytest.mark.parametrize("offsets", [np.int32, np.int64])
 @pytest.mark.parametrize("unit", [False, True])
-def test_match_split_rows(engine, orc, unit):
+def test_match_split_rows(engine, orc, unit, offsets):
     """Rows cut into many chunks: a hub of 140000 entries (more chunks than a
-    wave has lanes) and rows exactly at and just past the chunk boundary."""
+    wave has lanes) and rows exactly at and just past the chunk boundary,
+    with 32- and 64-bit offsets."""
     from cases import hub_graph
-    g = hub_graph(seed=21, side=400, hub_deg=140000, mids=(2048, 2049, 4096, 4097))
+    g = hub_graph(seed=21, side=400, hub_deg=140000, mids=(2048, 2049, 4096, 4097), offsets=offsets)
     if unit:
         g = Graph(g.xadj, g.adjncy, None, None)
     g = with_weights(g)
-    say(f"[match split rows] unit={unit} n={g.n} m={g.m}")
+    say(f"[match split rows] unit={unit} offsets={np.dtype(offsets).name} n={g.n} m={g.m}")
     mv = maxv_for(g, orc=orc)
     for seed in (1, 99):
         m_gpu, r_gpu = engine.match(g, mv, seed=seed)
@@ -124,7 +126,7 @@ def test_match_small_mid_rows(engine, orc, wmax):
         assert r_gpu == r_orc
 
 
-@pytest.mark.parametrize("name", ["rmat13", "hub", "rmat20"])
+@pytest.mark.parametrize("name", ["rmat13", "hub", "rmat20", "rmat20_off64"])
 def test_match_unit_weights(engine, orc, name):
     """Graphs given without weights use the shared array of ones, and the row
     scans then read its first word instead of streaming it: the matching must
@@ -136,9 +138,9 @@ def test_match_unit_weights(engine, orc, name):
     elif name == "hub":
         g = hub_graph(seed=22, side=300, hub_deg=50000, mids=(2048, 2049, 300, 100))
     else:
-        g = graphs.rmat_fast(20, 8, seed=10, offsets=np.int32)
+        g = graphs.rmat_fast(20, 8, seed=10, offsets=np.int64 if name.endswith("off64") else np.int32)
     g = Graph(g.xadj, g.adjncy, None, None)
-    say(f"[match unit weights] case={name} n={g.n} m={g.m}")
+    say(f"[match unit weights] case={name} n={g.n} m={g.m} offsets={g.xadj.dtype}")
     gw = with_weights(g)
     mv = maxv_for(gw, orc=orc)
     for seed in (3, 41):
@@ -177,15 +179,16 @@ def test_contract_unit_weights(engine, orc, name, order):
     assert cw.sum() == g.m - int((cmap[np.repeat(np.arange(g.n), np.diff(g.xadj))] == cmap[g.adjncy]).sum())
 
 
-@pytest.mark.parametrize("name", ["rmat20", "grid1024"])
+@pytest.mark.parametrize("name", ["rmat20", "rmat20_off64", "grid1024"])
 def test_match_frontier(engine, orc, name):
     """Levels of 2^20 vertices or more list what is still unmatched after
     round 5 (vertices, class rows, hub chunks) and run the later rounds over
     those lists: the matching and the round count must be those of the
     oracle.  R-MAT scale 20 has every row class and hubs; the 1024^2 grid
     has only thread-per-row rows."""
-    if name == "rmat20":
-        g = _symmetric_weights(graphs.rmat_fast(20, 8, seed=9, offsets=np.int32), 7, seed=3)
+    if name.startswith("rmat20"):
+        off = np.int64 if name.endswith("off64") else np.int32
+        g = _symmetric_weights(graphs.rmat_fast(20, 8, seed=9, offsets=off), 7, seed=3)
     else:
         g = with_weights(graphs.grid2d(1024, 1024, wmax=100, seed=2))
     assert g.n >= 1 << 20
@@ -205,9 +208,10 @@ def test_match_frontier(engine, orc, name):
         assert r_gpu == r_orc
 
 
+@pytest.mark.parametrize("offsets", [np.int32, np.int64])
 @pytest.mark.parametrize("unit", [False, True])
 @pytest.mark.parametrize("hub_deg,side", [(140000, 400), (600000, 800), (1100000, 1100)])
-def test_contract_hub_partitions(engine, orc, hub_deg, side, unit):
+def test_contract_hub_partitions(engine, orc, hub_deg, side, unit, offsets):
     """Hub segments cut into many key partitions: 35 and 147 partitions
     (entries bucketed by partition in each chunk), the second past the length
     whose entries can be listed in LDS (2^19), so the second pass reads the
@@ -215,11 +219,12 @@ def test_contract_hub_partitions(engine, orc, hub_deg, side, unit):
     partition block reads the whole segment.  Unit: the graph is given
     without weights, so the hub pass keeps no weight copies."""
     from cases import hub_graph
-    g0 = hub_graph(seed=23, side=side, hub_deg=hub_deg, mids=(5000, 2100, 1500, 600))
+    g0 = hub_graph(seed=23, side=side, hub_deg=hub_deg, mids=(5000, 2100, 1500, 600), offsets=offsets)
     if unit:
         g0 = Graph(g0.xadj, g0.adjncy, None, None)
     g = with_weights(g0)
-    say(f"[contract hub partitions] hub_deg={hub_deg} unit={unit} n={g.n} m={g.m}")
+    say(f"[contract hub partitions] hub_deg={hub_deg} unit={unit} offsets={np.dtype(offsets).name} "
+        f"n={g.n} m={g.m}")
     mv = maxv_for(g, orc=orc)
     match, _ = orc.handshake_match(g.xadj, g.adjncy, g.adjwgt, g.vwgt, mv, seed=3)
     cmap, nc = orc.number(match)
@@ -315,12 +320,15 @@ def test_project(engine, orc):
 
 @pytest.mark.parametrize("order", [CM_ROW_FIRST_SEEN, CM_ROW_ASCENDING])
 @pytest.mark.parametrize("name,k", [("grid100w100", 8), ("delaunay4720", 8), ("rmat12", 16), ("hub", 4),
-                                    ("path_increasing", 2)])
+                                    ("path_increasing", 2), ("rmat12_off64", 16), ("hub_off64", 4)])
 def test_hierarchy(engine, orc, name, k, order):
     if name == "grid100w100":
         g = graphs.grid2d(100, 100, wmax=100, seed=9)
-    elif name == "rmat12":
-        g = graphs.rmat(12, 16, seed=4, offsets=np.int32)
+    elif name.startswith("rmat12"):
+        g = graphs.rmat(12, 16, seed=4, offsets=np.int64 if name.endswith("off64") else np.int32)
+    elif name == "hub_off64":
+        from cases import hub_graph
+        g = hub_graph(offsets=np.int64)
     elif name == "path_increasing":
         # weights rise along the path: each round matches only the heaviest
         # remaining edge, so level 0 needs 200 rounds, far more than one batch
@@ -337,6 +345,7 @@ def test_hierarchy(engine, orc, name, k, order):
     for i, R in enumerate(ref):
         L = h.level(i)
         assert L.n == R["n"]
+        assert L.xadj.dtype == g.xadj.dtype
         np.testing.assert_array_equal(L.xadj.astype(np.int64), R["xadj"])
         np.testing.assert_array_equal(L.vwgt, R["vwgt"])
         ea, ew = R["adjncy"], R["adjwgt"]

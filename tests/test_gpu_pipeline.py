@@ -71,6 +71,60 @@ def test_batch_device_input_and_int64(engine):
         levels_equal(x, y)
 
 
+def _batch_call(engine, cg, voff, eoff, params):
+    import ctypes as C
+    from cumetis_amd import _lib as L
+    v, keep = engine.host_view(cg)
+    nb = len(voff) - 1
+    P = (L.CoarsenParams * nb)(*params)
+    H = (C.c_void_p * nb)()
+    vo = np.ascontiguousarray(voff, np.int64)
+    eo = np.ascontiguousarray(eoff, np.int64)
+    rc = engine.lib.cm_coarsen_batch(engine.ctx, nb, C.byref(v), vo.ctypes.data_as(C.POINTER(C.c_int64)),
+                                     eo.ctypes.data_as(C.POINTER(C.c_int64)), P, H)
+    for b in range(nb):
+        if H[b]:
+            engine.lib.cm_hier_release(engine.ctx, C.c_void_p(H[b]))
+    return rc
+
+
+@pytest.mark.parametrize("fault", ["id_past_graph", "self_loop", "zero_weight", "graph_start", "last_offset",
+                                   "mixed_rounds", "mixed_levels"])
+def test_batch_rejects_bad_input(engine, fault):
+    """A concatenated batch is checked per graph before its ids are made
+    global: a local id that names a vertex of the next graph, a self loop, a
+    weight <= 0, a graph whose first row does not start at eoff[b] or a wrong
+    last offset give CM_ERR_INPUT; per-graph round or level caps that differ
+    give CM_ERR_INVALID.  Nothing runs on the bad arrays."""
+    from cumetis_amd import _lib as L
+    gs = batch_graphs()[:4]
+    cg, voff, eoff = concat_graphs(gs)
+    x, a, w = cg.xadj.copy(), cg.adjncy.copy(), cg.adjwgt.copy()
+    params = [L.CoarsenParams(4, 0, 0, 0, 0, 0, 1, 0, 0) for _ in gs]
+    want = L.CM_ERR_INPUT
+    if fault == "id_past_graph":
+        a[eoff[1] + 3] = gs[1].n  # a valid id of the union, not of graph 1
+    elif fault == "self_loop":
+        a[x[voff[1] + 5]] = 5
+    elif fault == "zero_weight":
+        w[eoff[3] - 1] = 0
+    elif fault == "graph_start":
+        x[voff[1]] += 1
+    elif fault == "last_offset":
+        x[-1] -= 1
+    elif fault == "mixed_rounds":
+        params[2].max_rounds = 50
+        want = L.CM_ERR_INVALID
+    else:
+        params[1].max_levels = 3
+        want = L.CM_ERR_INVALID
+    say(f"[batch rejects] {fault}")
+    rc = _batch_call(engine, Graph(x, a, w, cg.vwgt), voff, eoff, params)
+    assert rc == want, (fault, rc)
+    # the context still works
+    assert _batch_call(engine, cg, voff, eoff, [L.CoarsenParams(4, 0, 0, 0, 0, 0, 1, 0, 0) for _ in gs]) == L.CM_OK
+
+
 @pytest.mark.parametrize("balance", [True, False])
 @pytest.mark.parametrize("name,k", [("delaunay4720", 8), ("grid100", 16), ("rgg", 4), ("hub", 8)])
 def test_partition_bit_exact_with_oracle_pipeline(engine, orc, name, k, balance):
@@ -134,3 +188,75 @@ def test_cli_3elt(tmp_path):
     gg = graphs.read_metis(str(g))
     src = np.repeat(np.arange(gg.n), np.diff(gg.xadj))
     assert cut == int((part[src] != part[gg.adjncy]).sum()) // 2
+
+
+def _partgraph(lib, g, k):
+    """cuMetis_PartGraph through ctypes with the reference's argument list
+    (include/cuMetis.h:18-19): int pointers, NULL weights allowed."""
+    import ctypes as C
+    ip = C.POINTER(C.c_int)
+    n = C.c_int(g.n)
+    nparts = C.c_int(k)
+    objval = C.c_int(-1)
+    xadj = np.ascontiguousarray(g.xadj, np.int32)
+    adjncy = np.ascontiguousarray(g.adjncy, np.int32)
+    vw = None if g.vwgt is None else np.ascontiguousarray(g.vwgt, np.int32)
+    aw = None if g.adjwgt is None else np.ascontiguousarray(g.adjwgt, np.int32)
+    part = np.full(g.n, -1, np.int32)
+    cast = lambda a: None if a is None else a.ctypes.data_as(ip)
+    rc = lib.cuMetis_PartGraph(C.byref(n), cast(xadj), cast(adjncy), cast(vw), cast(aw), C.byref(nparts),
+                               None, None, C.byref(objval), cast(part))
+    return rc, objval.value, part
+
+
+@pytest.mark.parametrize("name,k", [("3elt", 8), ("grid60x50w100", 16)])
+def test_partgraph_public_entry(engine, orc, name, k):
+    """The drop-in entry (cu:3622-3670): returns 1, writes part[n], and here
+    *objval holds the edge cut of part.  The partition is cm_partition's and
+    the oracle pipeline's (handshake coarsening, glibc seed-1 initial
+    partition, balancing + refinement per level).  3elt is passed with NULL
+    vertex and edge weights (unit weights, cu:479-503)."""
+    if name == "3elt":
+        g = graphs.read_metis(str(ROOT / "tests" / "golden" / "3elt.graph"))
+        g = Graph(g.xadj, g.adjncy, None, None)
+    else:
+        g = graphs.grid2d(60, 50, wmax=100, seed=4)
+        g = Graph(g.xadj, g.adjncy, g.adjwgt, np.random.default_rng(4).integers(1, 4, size=g.n).astype(np.int32))
+    say(f"[cuMetis_PartGraph] {name} k={k} n={g.n}")
+    rc, obj, part = _partgraph(engine.lib, g, k)
+    assert rc == 1
+    gw = with_weights(g)
+    assert obj == orc.edgecut(gw.xadj, gw.adjncy, gw.adjwgt, part)
+    p2, rep = engine.partition(g, k, seed=1)
+    np.testing.assert_array_equal(part, p2)
+    assert rep.edgecut == obj
+    where, _ = orc.partition(gw.xadj, gw.adjncy, gw.adjwgt, gw.vwgt, k, matcher="handshake", seed=1, balance=True)
+    np.testing.assert_array_equal(part, where)
+    # a second call in the same process gives the same partition
+    rc2, obj2, part2 = _partgraph(engine.lib, g, k)
+    assert rc2 == 1 and obj2 == obj
+    np.testing.assert_array_equal(part2, part)
+
+
+def test_partgraph_concurrent_callers(engine):
+    """Calls from several host threads at once (ctypes releases the GIL) are
+    serialised on the one process-wide context and each gets the result of a
+    call made alone."""
+    import threading
+    g = graphs.grid2d(48, 40, wmax=50, seed=6)
+    say("[cuMetis_PartGraph threads] grid48x40 k=8")
+    rc, obj, part = _partgraph(engine.lib, g, 8)
+    assert rc == 1
+    out = [None] * 6
+
+    def run(i):
+        out[i] = _partgraph(engine.lib, g, 8)
+
+    ts = [threading.Thread(target=run, args=(i,)) for i in range(len(out))]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join(timeout=120)
+    for r in out:
+        assert r is not None and r[0] == 1 and r[1] == obj
+        np.testing.assert_array_equal(r[2], part)
