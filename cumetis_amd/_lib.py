@@ -21,6 +21,7 @@ CM_OFF32, CM_OFF64 = 0, 1
 CM_ROW_FIRST_SEEN, CM_ROW_ASCENDING = 0, 1
 CM_FLAG_TRUSTED_INPUT = 1
 CM_FLAG_NO_BALANCE = 2
+CM_DEFAULT_TRIALS = 4
 
 
 class GraphView(C.Structure):
@@ -32,7 +33,7 @@ class GraphView(C.Structure):
 class CoarsenParams(C.Structure):
     _fields_ = [("nparts", C.c_int32), ("threshold", C.c_int32), ("maxvwgt", C.c_int32),
                 ("row_order", C.c_int32), ("max_levels", C.c_int32), ("max_rounds", C.c_int32),
-                ("seed", C.c_uint64), ("flags", C.c_int32), ("reserved", C.c_int32)]
+                ("seed", C.c_uint64), ("flags", C.c_int32), ("trials", C.c_int32)]
 
 
 class LevelInfo(C.Structure):
@@ -46,7 +47,7 @@ class LevelInfo(C.Structure):
 class PartReport(C.Structure):
     _fields_ = [("total_ms", C.c_double), ("coarsen_ms", C.c_double), ("init_ms", C.c_double),
                 ("uncoarsen_ms", C.c_double), ("levels", C.c_int32), ("coarsest_n", C.c_int32),
-                ("edgecut", C.c_int64)]
+                ("edgecut", C.c_int64), ("trials", C.c_int32), ("best_trial", C.c_int32)]
 
 
 # name -> (restype, argtypes); every symbol declared in include/cumetis_amd.h

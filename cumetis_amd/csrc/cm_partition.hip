@@ -11,6 +11,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <chrono>
 #include <mutex>
 #include <vector>
@@ -118,43 +119,99 @@ void partition_hierarchy(cm_ctx *ctx, const cm_hier *h, int nparts, int32_t *par
 
 extern "C" {
 
+// Trials: the graph goes to the device once; trial t coarsens it with seed
+// + t and runs the same initial partition (glibc seed-1 stream) and
+// uncoarsening; the partition with the lowest cut is kept, the earlier trial
+// on ties.  Single multilevel runs spread widely in cut (one refinement pass
+// per level, a coarsest graph that depends on the matching), so the best of
+// a few seeds is the standard remedy (METIS's ncuts).
 int cm_partition(cm_ctx *ctx, const cm_graph_view *g, const cm_coarsen_params *p, int32_t *part,
                  cm_part_report *rep)
 {
   if (!ctx || !g || !p || !part) return CM_ERR_INVALID;
   cm_hier *h = nullptr;
+  std::vector<void *> dev;
+  auto release_dev = [&]() {
+    for (void *q : dev) ctx->pool.release(q);
+    dev.clear();
+  };
   try {
     CM_REQUIRE(!g->on_device, CM_ERR_INVALID, "cm_partition takes a host graph");
     CM_REQUIRE(p->nparts >= 1, CM_ERR_INVALID, "nparts must be >= 1");
+    CM_REQUIRE(p->trials >= 0, CM_ERR_INVALID, "trials must be >= 0");
+    const int trials = p->trials > 0 ? p->trials : CM_DEFAULT_TRIALS;
     const double t0 = now_ms();
     if (p->nparts == 1 || g->n == 0) {
       for (int64_t i = 0; i < g->n; ++i) part[i] = 0;
-      if (rep) *rep = cm_part_report{0, 0, 0, 0, 1, (int32_t)g->n, 0};
+      if (rep) *rep = cm_part_report{0, 0, 0, 0, 1, (int32_t)g->n, 0, 0, 0};
       return CM_OK;
     }
-    int rc = cm_coarsen(ctx, g, p, &h);
-    if (rc != CM_OK) return rc;
-    const double t1 = now_ms();
-    double init_ms = 0, unc_ms = 0;
-    cm::partition_hierarchy(ctx, h, p->nparts, part, &init_ms, &unc_ms, !(p->flags & CM_FLAG_NO_BALANCE));
-    const double t3 = now_ms();
-    if (rep) {
-      rep->total_ms = t3 - t0;
-      rep->coarsen_ms = t1 - t0;
-      rep->init_ms = init_ms;
-      rep->uncoarsen_ms = unc_ms;
-      rep->levels = (int32_t)h->levels.size();
-      rep->coarsest_n = (int32_t)h->levels.back().n;
-      rep->edgecut = host_cut(g, part);
+    CM_HIP(hipSetDevice(ctx->device));
+    cm::validate_view(g);
+    // the graph on the device once, shared by the trials (weights stay
+    // implicit when not given)
+    const size_t ob = g->offset_type == CM_OFF64 ? 8 : 4;
+    auto up = [&](const void *src, size_t bytes) -> void * {
+      if (!src) return nullptr;
+      void *d = ctx->pool.alloc(bytes ? bytes : 1);
+      dev.push_back(d);
+      if (bytes) CM_HIP(hipMemcpyAsync(d, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+      return d;
+    };
+    cm_graph_view dv = *g;
+    dv.on_device = 1;
+    dv.xadj = up(g->xadj, ob * (g->n + 1));
+    dv.adjncy = (const int32_t *)up(g->adjncy, 4 * g->m);
+    dv.adjwgt = (const int32_t *)up(g->adjwgt, 4 * g->m);
+    dv.vwgt = (const int32_t *)up(g->vwgt, 4 * g->n);
+    std::vector<int32_t> cur(g->n);
+    int64_t best = -1;
+    cm_part_report r{};
+    r.trials = trials;
+    for (int t = 0; t < trials; ++t) {
+      cm_coarsen_params q = *p;
+      q.seed = p->seed + (uint64_t)t;
+      if (t > 0) q.flags |= CM_FLAG_TRUSTED_INPUT;  // validated by the first trial
+      const double c0 = now_ms();
+      int rc = cm_coarsen(ctx, &dv, &q, &h);
+      if (rc != CM_OK) {
+        release_dev();
+        return rc;
+      }
+      const double c1 = now_ms();
+      double init_ms = 0, unc_ms = 0;
+      int32_t *dst = t == 0 ? part : cur.data();
+      cm::partition_hierarchy(ctx, h, p->nparts, dst, &init_ms, &unc_ms, !(p->flags & CM_FLAG_NO_BALANCE));
+      const int64_t cut = host_cut(g, dst);
+      r.coarsen_ms += c1 - c0;
+      r.init_ms += init_ms;
+      r.uncoarsen_ms += unc_ms;
+      if (best < 0 || cut < best) {
+        best = cut;
+        r.best_trial = t;
+        r.levels = (int32_t)h->levels.size();
+        r.coarsest_n = (int32_t)h->levels.back().n;
+        r.edgecut = cut;
+        if (dst != part) std::copy(cur.begin(), cur.end(), part);
+      }
+      cm_hier_release(ctx, h);
+      h = nullptr;
     }
-    cm_hier_release(ctx, h);
+    cm::sync(ctx);
+    release_dev();
+    r.total_ms = now_ms() - t0;
+    if (rep) *rep = r;
   } catch (const cm::Error &e) {
     ctx->err = e.what();
     if (h) cm_hier_release(ctx, h);
+    (void)hipStreamSynchronize(ctx->stream);
+    release_dev();
     return e.code;
   } catch (const std::exception &e) {
     ctx->err = e.what();
     if (h) cm_hier_release(ctx, h);
+    (void)hipStreamSynchronize(ctx->stream);
+    release_dev();
     return CM_ERR_INVALID;
   }
   return CM_OK;

@@ -99,23 +99,45 @@ __global__ void k_union_bases(int ns, const int64_t *vbase, const int32_t *cmap,
 
 }  // namespace
 
-struct Job {
-  uint64_t id = 0;
-  int64_t n = 0, m = 0;
+// A job's host graph, shared by its trials.
+struct JobGraph {
   std::vector<int64_t> xadj;
   std::vector<int32_t> adjncy, adjwgt, vwgt;
+};
+
+struct Request;
+
+// One trial of a request: one hierarchy in the union steps, coarsened with
+// the request's seed + trial.
+struct Job {
+  Request *req = nullptr;
+  int trial = 0;
+  std::shared_ptr<const JobGraph> gr;
+  int64_t n = 0, m = 0;
   cm_coarsen_params p{};
   int T = 0;
   int32_t maxv = 0;
   int64_t tvwgt = 0;
   size_t footprint = 0;
-  double t_submit = 0, t_done = 0;
   cm_hier *h = nullptr;
   int64_t cur_n = 0;  // vertices at the current level
+  int status = CM_OK;
+};
+
+// A submitted graph: its trials run as separate jobs; the request is done
+// when all have finished, with the lowest cut kept (the earlier trial on
+// ties), as cm_partition does.
+struct Request {
+  uint64_t id = 0;
+  int trials = 1;
+  int finished = 0;
+  double t_submit = 0, t_done = 0;
   std::vector<int32_t> part;
   cm_part_report rep{};
+  int64_t best_cut = -1;
   int status = CM_OK;
   bool done = false;
+  std::vector<std::unique_ptr<Job>> jobs;
 };
 
 }  // namespace cm
@@ -129,7 +151,7 @@ struct cm_server {
   std::condition_variable cv_loop, cv_fin, cv_done;
   std::deque<cm::Job *> pending, to_finish;
   std::vector<cm::Job *> active;
-  std::map<uint64_t, std::unique_ptr<cm::Job>> jobs;
+  std::map<uint64_t, std::unique_ptr<cm::Request>> jobs;
   uint64_t next_id = 1;
   size_t budget = 0, used = 0;
   int max_slots = 0;
@@ -306,18 +328,19 @@ static void loop_main(cm_server *S)
         L0.adjwgt = (int32_t *)ctx->pool.alloc(4 * (j->m > 0 ? j->m : 1));
         L0.vwgt = (int32_t *)ctx->pool.alloc(4 * (j->n > 0 ? j->n : 1));
         j->h->owned = {L0.xadj, L0.adjncy, L0.adjwgt, L0.vwgt};
-        CM_HIP(hipMemcpyAsync(L0.xadj, j->xadj.data(), 8 * (j->n + 1), hipMemcpyHostToDevice, ctx->stream));
+        const JobGraph &G = *j->gr;
+        CM_HIP(hipMemcpyAsync(L0.xadj, G.xadj.data(), 8 * (j->n + 1), hipMemcpyHostToDevice, ctx->stream));
         if (j->m) {
-          CM_HIP(hipMemcpyAsync(L0.adjncy, j->adjncy.data(), 4 * j->m, hipMemcpyHostToDevice, ctx->stream));
-          CM_HIP(hipMemcpyAsync(L0.adjwgt, j->adjwgt.data(), 4 * j->m, hipMemcpyHostToDevice, ctx->stream));
+          CM_HIP(hipMemcpyAsync(L0.adjncy, G.adjncy.data(), 4 * j->m, hipMemcpyHostToDevice, ctx->stream));
+          CM_HIP(hipMemcpyAsync(L0.adjwgt, G.adjwgt.data(), 4 * j->m, hipMemcpyHostToDevice, ctx->stream));
         }
-        if (j->n) CM_HIP(hipMemcpyAsync(L0.vwgt, j->vwgt.data(), 4 * j->n, hipMemcpyHostToDevice, ctx->stream));
+        if (j->n) CM_HIP(hipMemcpyAsync(L0.vwgt, G.vwgt.data(), 4 * j->n, hipMemcpyHostToDevice, ctx->stream));
         L0.tvwgt = j->tvwgt;
         L0.maxdeg = 0;
         L0.maxvw = 0;
         for (int64_t v = 0; v < j->n; ++v) {
-          L0.maxdeg = std::max(L0.maxdeg, j->xadj[v + 1] - j->xadj[v]);
-          L0.maxvw = std::max<int64_t>(L0.maxvw, j->vwgt[v]);
+          L0.maxdeg = std::max(L0.maxdeg, G.xadj[v + 1] - G.xadj[v]);
+          L0.maxvw = std::max<int64_t>(L0.maxvw, G.vwgt[v]);
         }
         j->h->levels.push_back(L0);
         j->cur_n = j->n;
@@ -360,21 +383,24 @@ static void finish_main(cm_server *S, cm_ctx *fctx)
       j = S->to_finish.front();
       S->to_finish.pop_front();
     }
+    std::vector<int32_t> part;
+    cm_part_report rep{};
+    int64_t cut = 0;
     if (j->status == CM_OK) {
       try {
-        j->part.assign(j->n, 0);
+        part.assign(j->n, 0);
         const double t0 = now_ms();
         if (j->p.nparts > 1 && j->n > 0)
-          partition_hierarchy(fctx, j->h, j->p.nparts, j->part.data(), &j->rep.init_ms, &j->rep.uncoarsen_ms,
+          partition_hierarchy(fctx, j->h, j->p.nparts, part.data(), &rep.init_ms, &rep.uncoarsen_ms,
                               !(j->p.flags & CM_FLAG_NO_BALANCE));
-        j->rep.levels = (int32_t)j->h->levels.size();
-        j->rep.coarsest_n = (int32_t)j->h->levels.back().n;
-        int64_t cut = 0;
+        rep.levels = (int32_t)j->h->levels.size();
+        rep.coarsest_n = (int32_t)j->h->levels.back().n;
+        const JobGraph &G = *j->gr;
         for (int64_t i = 0; i < j->n; ++i)
-          for (int64_t e = j->xadj[i]; e < j->xadj[i + 1]; ++e)
-            if (j->part[i] != j->part[j->adjncy[e]]) cut += j->adjwgt[e];
-        j->rep.edgecut = cut / 2;
-        j->rep.uncoarsen_ms = now_ms() - t0 - j->rep.init_ms;
+          for (int64_t e = G.xadj[i]; e < G.xadj[i + 1]; ++e)
+            if (part[i] != part[G.adjncy[e]]) cut += G.adjwgt[e];
+        cut /= 2;
+        rep.uncoarsen_ms = now_ms() - t0 - rep.init_ms;
       } catch (const Error &e) {
         j->status = e.code;
       }
@@ -385,14 +411,32 @@ static void finish_main(cm_server *S, cm_ctx *fctx)
       j->h = nullptr;
     }
     std::lock_guard<std::mutex> lk(S->mu);
-    j->t_done = now_ms();
-    j->rep.total_ms = j->t_done - j->t_submit;
-    j->done = true;
+    Request *r = j->req;
     S->used -= j->footprint;
-    S->latencies.push_back(j->rep.total_ms);
-    S->t_last = j->t_done;
-    ++S->completed;
-    S->cv_done.notify_all();
+    if (j->status != CM_OK && r->status == CM_OK) r->status = j->status;
+    if (j->status == CM_OK) {
+      r->rep.init_ms += rep.init_ms;
+      r->rep.uncoarsen_ms += rep.uncoarsen_ms;
+      // lowest cut, then the earlier trial (trials finish in any order)
+      if (r->best_cut < 0 || cut < r->best_cut || (cut == r->best_cut && j->trial < r->rep.best_trial)) {
+        r->best_cut = cut;
+        r->part.swap(part);
+        r->rep.best_trial = j->trial;
+        r->rep.levels = rep.levels;
+        r->rep.coarsest_n = rep.coarsest_n;
+        r->rep.edgecut = cut;
+      }
+    }
+    if (++r->finished == r->trials) {
+      r->t_done = now_ms();
+      r->rep.total_ms = r->t_done - r->t_submit;
+      r->rep.trials = r->trials;
+      r->done = true;
+      S->latencies.push_back(r->rep.total_ms);
+      S->t_last = r->t_done;
+      ++S->completed;
+      S->cv_done.notify_all();
+    }
     S->cv_loop.notify_one();  // budget freed
   }
 }
@@ -431,35 +475,53 @@ int cm_server_create(int device, int64_t hbm_budget, int32_t max_slots, cm_serve
 
 int cm_server_submit(cm_server *S, const cm_graph_view *g, const cm_coarsen_params *p, uint64_t *ticket)
 {
-  if (!S || !g || !p || !ticket || g->on_device || p->nparts < 1 || g->n < 0) return CM_ERR_INVALID;
+  if (!S || !g || !p || !ticket || g->on_device || p->nparts < 1 || g->n < 0 || p->trials < 0)
+    return CM_ERR_INVALID;
   if (p->row_order != CM_ROW_FIRST_SEEN) return CM_ERR_UNSUPPORTED;  // one row order per union step
-  auto j = std::make_unique<cm::Job>();
-  j->n = g->n;
-  j->m = g->m;
-  j->p = *p;
-  j->xadj.resize(g->n + 1);
+  if (g->offset_type != CM_OFF32 && g->offset_type != CM_OFF64) return CM_ERR_INVALID;
+  if (g->n >= (int64_t)INT32_MAX || (g->m > 0 && !g->adjncy) || !g->xadj) return CM_ERR_INVALID;
+  auto gr = std::make_shared<cm::JobGraph>();
+  gr->xadj.resize(g->n + 1);
   for (int64_t i = 0; i <= g->n; ++i)
-    j->xadj[i] = g->offset_type == CM_OFF64 ? ((const int64_t *)g->xadj)[i] : ((const int32_t *)g->xadj)[i];
-  if (j->xadj[0] != 0 || j->xadj[g->n] != g->m) return CM_ERR_INPUT;
-  j->adjncy.assign(g->adjncy, g->adjncy + g->m);
-  j->adjwgt = g->adjwgt ? std::vector<int32_t>(g->adjwgt, g->adjwgt + g->m) : std::vector<int32_t>(g->m, 1);
-  j->vwgt = g->vwgt ? std::vector<int32_t>(g->vwgt, g->vwgt + g->n) : std::vector<int32_t>(g->n, 1);
+    gr->xadj[i] = g->offset_type == CM_OFF64 ? ((const int64_t *)g->xadj)[i] : ((const int32_t *)g->xadj)[i];
+  if (gr->xadj[0] != 0 || gr->xadj[g->n] != g->m) return CM_ERR_INPUT;
+  gr->adjncy.assign(g->adjncy, g->adjncy + g->m);
+  gr->adjwgt = g->adjwgt ? std::vector<int32_t>(g->adjwgt, g->adjwgt + g->m) : std::vector<int32_t>(g->m, 1);
+  gr->vwgt = g->vwgt ? std::vector<int32_t>(g->vwgt, g->vwgt + g->n) : std::vector<int32_t>(g->n, 1);
   for (int64_t i = 0; i < g->n; ++i) {
-    if (j->xadj[i] > j->xadj[i + 1]) return CM_ERR_INPUT;
-    for (int64_t e = j->xadj[i]; e < j->xadj[i + 1]; ++e)
-      if (j->adjncy[e] < 0 || j->adjncy[e] >= g->n || j->adjncy[e] == i || j->adjwgt[e] <= 0) return CM_ERR_INPUT;
+    if (gr->xadj[i] > gr->xadj[i + 1]) return CM_ERR_INPUT;
+    for (int64_t e = gr->xadj[i]; e < gr->xadj[i + 1]; ++e)
+      if (gr->adjncy[e] < 0 || gr->adjncy[e] >= g->n || gr->adjncy[e] == i || gr->adjwgt[e] <= 0)
+        return CM_ERR_INPUT;
   }
-  for (int32_t w : j->vwgt) j->tvwgt += w;
-  j->T = p->threshold > 0 ? p->threshold : cm::ref_threshold(g->n, p->nparts);
-  j->maxv = p->maxvwgt > 0 ? p->maxvwgt : (int32_t)(1.5 * (double)j->tvwgt / j->T);
-  j->footprint = cm::job_footprint(g->n, g->m);
-  j->t_submit = cm::now_ms();
+  int64_t tvwgt = 0;
+  for (int32_t w : gr->vwgt) tvwgt += w;
+  const int T = p->threshold > 0 ? p->threshold : cm::ref_threshold(g->n, p->nparts);
+  const int32_t maxv = p->maxvwgt > 0 ? p->maxvwgt : (int32_t)(1.5 * (double)tvwgt / T);
+  auto r = std::make_unique<cm::Request>();
+  r->trials = p->nparts > 1 && g->n > 0 ? (p->trials > 0 ? p->trials : CM_DEFAULT_TRIALS) : 1;
+  r->t_submit = cm::now_ms();
+  for (int t = 0; t < r->trials; ++t) {
+    auto j = std::make_unique<cm::Job>();
+    j->req = r.get();
+    j->trial = t;
+    j->gr = gr;
+    j->n = g->n;
+    j->m = g->m;
+    j->p = *p;
+    j->p.seed = p->seed + (uint64_t)t;
+    j->tvwgt = tvwgt;
+    j->T = T;
+    j->maxv = maxv;
+    j->footprint = cm::job_footprint(g->n, g->m);
+    r->jobs.push_back(std::move(j));
+  }
   std::lock_guard<std::mutex> lk(S->mu);
-  j->id = S->next_id++;
-  *ticket = j->id;
-  if (S->t_first == 0) S->t_first = j->t_submit;
-  S->pending.push_back(j.get());
-  S->jobs[j->id] = std::move(j);
+  r->id = S->next_id++;
+  *ticket = r->id;
+  if (S->t_first == 0) S->t_first = r->t_submit;
+  for (auto &j : r->jobs) S->pending.push_back(j.get());
+  S->jobs[r->id] = std::move(r);
   S->cv_loop.notify_one();
   return CM_OK;
 }
@@ -470,12 +532,12 @@ int cm_server_wait(cm_server *S, uint64_t ticket, int32_t *part, cm_part_report 
   std::unique_lock<std::mutex> lk(S->mu);
   auto it = S->jobs.find(ticket);
   if (it == S->jobs.end()) return CM_ERR_INVALID;
-  cm::Job *j = it->second.get();
-  S->cv_done.wait(lk, [&] { return j->done; });
-  const int st = j->status;
+  cm::Request *r = it->second.get();
+  S->cv_done.wait(lk, [&] { return r->done; });
+  const int st = r->status;
   if (st == CM_OK) {
-    if (part) std::copy(j->part.begin(), j->part.end(), part);
-    if (rep) *rep = j->rep;
+    if (part) std::copy(r->part.begin(), r->part.end(), part);
+    if (rep) *rep = r->rep;
   }
   S->jobs.erase(it);
   return st;
@@ -510,13 +572,12 @@ int cm_server_destroy(cm_server *S)
   if (S->loop.joinable()) S->loop.join();
   for (auto &t : S->finishers)
     if (t.joinable()) t.join();
-  for (auto &kv : S->jobs) {
-    cm::Job *j = kv.second.get();
-    if (j->h) {
-      for (void *q : j->h->owned) S->cctx->pool.release(q);
-      delete j->h;
-    }
-  }
+  for (auto &kv : S->jobs)
+    for (auto &j : kv.second->jobs)
+      if (j->h) {
+        for (void *q : j->h->owned) S->cctx->pool.release(q);
+        delete j->h;
+      }
   S->jobs.clear();
   for (cm_ctx *f : S->fctx) cm_ctx_destroy(f);
   cm_ctx_destroy(S->cctx);

@@ -299,11 +299,11 @@ class Engine:
         return hier
 
     def partition(self, g: Graph, nparts: int, seed: int = 1,
-                  row_order: int = L.CM_ROW_FIRST_SEEN, balance: bool = True):
-        """cm_partition; balance=False gives the reference's pipeline without
-        the balancing pass."""
+                  row_order: int = L.CM_ROW_FIRST_SEEN, balance: bool = True, trials: int = 0):
+        """cm_partition; balance=False leaves out the balancing pass (the
+        reference's pipeline has none); trials 0 = CM_DEFAULT_TRIALS."""
         v, keep = self.host_view(g)
-        p = L.CoarsenParams(nparts, 0, 0, row_order, 0, 0, seed, 0 if balance else L.CM_FLAG_NO_BALANCE, 0)
+        p = L.CoarsenParams(nparts, 0, 0, row_order, 0, 0, seed, 0 if balance else L.CM_FLAG_NO_BALANCE, trials)
         part = np.empty(max(g.n, 1), np.int32)
         rep = L.PartReport()
         self._check(self.lib.cm_partition(self.ctx, C.byref(v), C.byref(p), part.ctypes.data,
@@ -371,7 +371,7 @@ class Server:
         self._keep = {}
         self._n = {}
 
-    def submit(self, g: Graph, nparts: int, seed: int = 1, balance: bool = True) -> int:
+    def submit(self, g: Graph, nparts: int, seed: int = 1, balance: bool = True, trials: int = 0) -> int:
         odt = np.int64 if g.xadj.dtype == np.int64 else np.int32
         keep = [np.ascontiguousarray(g.xadj, odt), np.ascontiguousarray(g.adjncy, np.int32),
                 None if g.adjwgt is None else np.ascontiguousarray(g.adjwgt, np.int32),
@@ -380,7 +380,7 @@ class Server:
                         keep[1].ctypes.data, None if keep[2] is None else keep[2].ctypes.data,
                         None if keep[3] is None else keep[3].ctypes.data)
         p = L.CoarsenParams(int(nparts), 0, 0, L.CM_ROW_FIRST_SEEN, 0, 0, int(seed),
-                            0 if balance else L.CM_FLAG_NO_BALANCE, 0)
+                            0 if balance else L.CM_FLAG_NO_BALANCE, int(trials))
         t = C.c_uint64()
         rc = self.lib.cm_server_submit(self.s, C.byref(v), C.byref(p), C.byref(t))
         if rc != L.CM_OK:

@@ -74,8 +74,13 @@ typedef struct {
   int32_t max_rounds; /* handshake round cap, 0 = default (4096) */
   uint64_t seed;
   int32_t flags;      /* CM_FLAG_* */
-  int32_t reserved;
+  int32_t trials;     /* cm_partition / cuMetis_PartGraph / cm_server: partition
+                         trials, 0 = CM_DEFAULT_TRIALS; trial t coarsens with
+                         seed + t, the lowest cut is kept (the earlier trial on
+                         ties); cm_coarsen ignores it */
 } cm_coarsen_params;
+
+#define CM_DEFAULT_TRIALS 4
 
 /* cm_coarsen_params.flags */
 #define CM_FLAG_TRUSTED_INPUT 1 /* skip the device-side validation of the input graph */
@@ -204,12 +209,15 @@ int cm_project_raw(cm_ctx *ctx, int64_t n, const int32_t *d_cmap, const int32_t 
 int cuMetis_PartGraph(int *nvtxs, int *xadj, int *adjncy, int *vwgt, int *adjwgt, int *nparts,
                       float *tpwgts, float *ubvec, int *objval, int *part);
 
-/* Same pipeline with explicit options and per-phase timings (ms). */
+/* Same pipeline with explicit options and per-phase timings (ms, summed
+ * over the trials; levels and coarsest_n are the kept trial's). */
 typedef struct {
   double total_ms, coarsen_ms, init_ms, uncoarsen_ms;
   int32_t levels;
   int32_t coarsest_n;
   int64_t edgecut;
+  int32_t trials;     /* trials run */
+  int32_t best_trial; /* the kept trial (its seed is params.seed + best_trial) */
 } cm_part_report;
 int cm_partition(cm_ctx *ctx, const cm_graph_view *g, const cm_coarsen_params *p, int32_t *part,
                  cm_part_report *report);

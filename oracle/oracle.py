@@ -215,27 +215,35 @@ def coarsen(xadj, adjncy, adjwgt, vwgt, nparts, matcher="cpu", seed=1, T=None, m
     return levels
 
 
-def partition(xadj, adjncy, adjwgt, vwgt, nparts, matcher="handshake", seed=1, balance=False):
+def partition(xadj, adjncy, adjwgt, vwgt, nparts, matcher="handshake", seed=1, balance=False, trials=1):
     """The multilevel pipeline (cu:3544-3616) on the CPU: coarsen (matcher
     'cpu' = the reference's CPU HEM, 'handshake' = the engine's matching
     model), initial partition of the coarsest level (glibc stream from seed
     1), then refine the coarsest level and, for every finer level, project and
     refine.  balance=True runs the engine's balancing pass before each
-    refinement (the engine's default; the reference has none).
-    Returns (where, levels)."""
+    refinement (the engine's default; the reference has none).  trials > 1
+    (handshake only) repeats the pipeline with seeds seed .. seed+trials-1
+    and keeps the lowest edge cut, the earlier trial on ties.
+    Returns (where, levels) of the kept trial."""
     n = len(xadj) - 1
     if nparts == 1 or n == 0:
         return np.zeros(n, np.int32), []
-    if matcher == "cpu":
+    assert trials == 1 or matcher == "handshake"
+    best = None
+    for t in range(trials):
+        if matcher == "cpu":
+            srand(1)
+        levels = coarsen(xadj, adjncy, adjwgt, vwgt, nparts, matcher=matcher, seed=seed + t)
+        C = levels[-1]
         srand(1)
-    levels = coarsen(xadj, adjncy, adjwgt, vwgt, nparts, matcher=matcher, seed=seed)
-    C = levels[-1]
-    srand(1)
-    where, _ = initpart(C["xadj"], C["adjncy"], C["adjwgt"], C["vwgt"], nparts)
-    for L in [C] + list(reversed(levels[:-1])):
-        if L is not C:
-            where = project(L["cmap"], where)
-        if balance:
-            where, _ = kway_balance(L["xadj"], L["adjncy"], L["adjwgt"], L["vwgt"], nparts, where)
-        where = kway_refine(L["xadj"], L["adjncy"], L["adjwgt"], L["vwgt"], nparts, where)
-    return where, levels
+        where, _ = initpart(C["xadj"], C["adjncy"], C["adjwgt"], C["vwgt"], nparts)
+        for L in [C] + list(reversed(levels[:-1])):
+            if L is not C:
+                where = project(L["cmap"], where)
+            if balance:
+                where, _ = kway_balance(L["xadj"], L["adjncy"], L["adjwgt"], L["vwgt"], nparts, where)
+            where = kway_refine(L["xadj"], L["adjncy"], L["adjwgt"], L["vwgt"], nparts, where)
+        cut = edgecut(xadj, adjncy, adjwgt, where)
+        if best is None or cut < best[0]:
+            best = (cut, where, levels)
+    return best[1], best[2]
