@@ -10,7 +10,9 @@
  * which gives the same values for every graph it can represent.
  *
  * Pinning (what checks each function independently of its own formula):
- *   orc_rand / orc_randarrayofp  glibc rand() known-answer sequence (seed 1)
+ *   orc_rand / orc_randarrayofp  glibc rand() known-answer sequence (seed 1);
+ *                                hand-worked permutations of both branches
+ *                                (n < 10 swaps, n >= 10 block swaps)
  *   orc_number                   closed form: rank of representatives
  *   orc_cpu_contract             closed form P^T A P (diagonal dropped),
  *                                brute force over all matchings of tiny graphs,
@@ -21,10 +23,15 @@
  *                                local dominance), brute force on tiny graphs
  *   orc_project                  closed form where_f = where_c[cmap]
  *   orc_edgecut / orc_pwgts      closed forms
- *   orc_initpart / orc_refine    parity unpinned against the reference
- *                                (its output depends on the reference's own
- *                                rand() stream, which cannot be run here);
- *                                checked by invariants only.
+ *   orc_initpart                 a bisection traced by hand through
+ *                                cu:1854-1875, 2536-2661, 2367-2531 with the
+ *                                glibc seed-1 stream (BFS start, FM heap ties,
+ *                                rollback), plus invariants
+ *   orc_kway_refine              a 3-part pass traced by hand through
+ *                                cu:3244-3309, 3401-3488 (bounds, tie to the
+ *                                lower part, the two passes), plus invariants
+ *   orc_kway_balance             hand-worked known answers (no reference
+ *                                counterpart: the engine's own pass)
  */
 #ifndef CM_ORACLE_H
 #define CM_ORACLE_H

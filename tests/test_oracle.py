@@ -29,6 +29,22 @@ def test_randarrayofp_small_known_answer(orc):
     assert list(orc.randarrayofp(5, 0)) == [1, 4, 3, 2, 0]
 
 
+@pytest.mark.parametrize("m,expect", [(2, [1, 2, 3, 4, 8, 5, 6, 10, 7, 0, 9, 11]),
+                                      (3, [1, 2, 3, 4, 8, 0, 9, 10, 7, 5, 6, 11])])
+def test_randarrayofp_block_swaps_known_answer(orc, m, expect):
+    # n >= 10: m steps of v = rand%(n-3), u = rand%(n-3) and the four swaps
+    # p[v]<->p[u+2], p[v+1]<->p[u+3], p[v+2]<->p[u], p[v+3]<->p[u+1]
+    # (cu:658-668), worked by hand for n=12 from GLIBC_SEED1:
+    #   step 1: 1804289383%9 = 1, 846930886%9 = 7:
+    #           swaps (1,9) (2,10) (3,7) (4,8) -> [0,9,10,7,8,5,6,3,4,1,2,11]
+    #   step 2: 1681692777%9 = 0, 1714636915%9 = 7:
+    #           swaps (0,9) (1,10) (2,7) (3,8) -> [1,2,3,4,8,5,6,10,7,0,9,11]
+    #   step 3: 1957747793%9 = 5, 424238335%9 = 7: swaps (5,9) (6,10) and
+    #           two self swaps (7,7) (8,8) -> [1,2,3,4,8,0,9,10,7,5,6,11]
+    orc.srand(1)
+    assert list(orc.randarrayofp(12, m)) == expect
+
+
 def test_randarrayofp_is_permutation(orc):
     orc.srand(1)
     p = orc.randarrayofp(1000, 125)
@@ -310,6 +326,68 @@ def test_initpart_invariants(orc, k):
     assert pw.min() > 0.7 * g.n / k and pw.max() < 1.3 * g.n / k
     if k == 2:  # the bisection objective is the cut of the 2-way split
         assert obj1 == orc.edgecut(g.xadj, g.adjncy, g.adjwgt, w1)
+
+
+def test_initpart_bisection_known_answer(orc):
+    """cuMetis_initialpartition for k=2 (cu:3091-3108 -> cu:2964-3025 ->
+    cu:2806-2829), traced by hand on the 6-cycle 0-1-5-4-3-2-0 with edge
+    weights 0-1:5 3-4:5 and 1 elsewhere, unit vertex weights, glibc seed 1:
+    - the CPU coarsening (T=200, maxvwgt=int(1.5*6/200)=0, cu:1856) matches
+      nothing but draws its visit permutation: 12 rands (n<10 branch of
+      cu:649-656);
+    - growbisection (cu:2536-2661), onemaxpwgt=3, oneminpwgt=2.  Iteration
+      0: BFS from 783368690%6 = 2 takes 2, 0, 3: parts {0,2,3}|{1,4,5}, cut
+      10, boundary list [0,1,3,4].  FM pass 0 (cu:2367-2531): the
+      permutation of the 4 boundary vertices from the next 8 rands is
+      [3,1,2,0], so 4 (key 4) enters queue 1 before 1 (key 4) and stays at
+      the heap top on the tie; parts are equal, so from=1: 4 moves (cut 6),
+      3 leaves queue 0, then from=0: 0 moves (cut 2, balanced: mincutorder
+      1), then 5, 2, 1, 3 move without a better cut and are rolled back.
+      Parts {2,3,4}|{0,1,5}, cut 2; pass 1 finds nothing (all moves rolled
+      back).  Iterations 1..4 reach cut 2 at best and do not replace it
+      (strict '>' at cu:2649).
+    - the refinement (cu:2769-2802) and the projection keep it.
+    Had the heap tie gone the other way (1 before 4), FM would end at
+    {0,1,2}|{3,4,5} = [0,0,0,1,1,1]."""
+    g = graphs.from_edges(6, np.array([0, 0, 2, 3, 4, 1]), np.array([1, 2, 3, 4, 5, 5]),
+                          np.array([5, 1, 1, 5, 1, 1]))
+    g = with_weights(g)
+    assert list(g.adjncy) == [1, 2, 0, 5, 0, 3, 2, 4, 3, 5, 1, 4]
+    orc.srand(1)
+    where, obj = orc.initpart(g.xadj, g.adjncy, g.adjwgt, g.vwgt, 2)
+    assert list(where) == [1, 1, 0, 0, 0, 1]
+    assert obj == 2
+
+
+def test_kway_refine_known_answer(orc):
+    """One refinement pass (candidates cu:3244-3267, destinations
+    cu:3272-3309, bounds cu:3401-3412, moves cu:3417-3488 with destinations
+    taken in ascending order and each destination's candidates by vertex),
+    worked by hand.  k=3, parts {0,1,2} {3,4,5} {6,7,8}, vertex weights
+    [2,95,3,2,96,3,2,95,2] (part weights 100,101,99 of 300), so the bounds
+    are int(100*1.03)=103 and int(100/1.03)=97.
+    Candidates (external > internal) and destinations (first part with the
+    strictly largest connectivity above the own part's):
+      v0: conn [1,2,2] -> 1 (tie of parts 1 and 2: the lower id)
+      v2: [1,5,0] -> 1;  v3: [5,1,0] -> 0;  v5: [0,1,8] -> 2
+      v4: [2,2,1] -> stays 1 (external 3 > internal 2, no part beats 2)
+      v6: [0,4,1] -> 1;  v8: [0,5,1] -> 1;  v1, v7: not candidates.
+    Sorted by (destination, vertex): (0,v3) (1,v0) (1,v2) (1,v4) (1,v6)
+    (1,v8) (2,v5).
+    Pass from <= to: v0 1->... 0->1: 101+2=103 ok, 100-2=98 ok, accept;
+      v2: 103+3=106 > 103 reject; v4: 103+96 reject; v5 1->2: 99+3=102,
+      103-3=100, accept.  Part weights 98,100,102.
+    Pass from > to: v3 1->0: 100 / 98 accept; v6 2->1: 100 / 100 accept;
+      v8 2->1: 102 / 98 accept.  Part weights 100,102,98.
+    Taking destinations in descending order instead leaves v6 in part 2."""
+    u = np.array([0, 1, 3, 4, 6, 7, 2, 5, 5, 0, 0, 4])
+    v = np.array([1, 2, 4, 5, 7, 8, 3, 6, 8, 7, 4, 8])
+    w = np.array([1, 1, 1, 1, 1, 1, 5, 4, 4, 2, 2, 1])
+    g = graphs.from_edges(9, u, v, w)
+    vw = np.array([2, 95, 3, 2, 96, 3, 2, 95, 2], np.int32)
+    where = np.array([0, 0, 0, 1, 1, 1, 2, 2, 2], np.int32)
+    out = orc.kway_refine(g.xadj, g.adjncy, g.adjwgt, vw, 3, where)
+    assert list(out) == [1, 0, 0, 0, 1, 2, 1, 2, 1]
 
 
 def test_kway_refine_invariants(orc):
