@@ -81,6 +81,7 @@ _PROTOS = {
     "cm_contract": (C.c_int, [_P, C.POINTER(GraphView), _P, _P, C.c_int64, C.c_int32, _P, _P, _P,
                               _P, C.POINTER(C.c_int64)]),
     "cm_segsort_i32": (C.c_int, [_P, _P, _P, C.c_int32, _P, C.c_int32]),
+    "cm_segsort_i32_noalloc": (C.c_int, [_P, _P, _P, C.c_int32, _P, C.c_int32]),
     "cm_project_raw": (C.c_int, [_P, C.c_int64, _P, _P, _P]),
     "cuMetis_PartGraph": (C.c_int, [C.POINTER(C.c_int)] * 5 + [C.POINTER(C.c_int),
                                     C.POINTER(C.c_float), C.POINTER(C.c_float),

@@ -244,8 +244,19 @@ void scan_excl_i32(cm_ctx *ctx, int64_t n, const int32_t *in, int32_t *out)
   }
   const int64_t tiles = (n + 1023) / 1024;
   const uint32_t ep = lookback_begin(ctx, tiles);
-  launch(ctx, "scan_bins", 0.0, k_scan_excl<int32_t>, dim3((unsigned)tiles), dim3(1024), 0, n, in, out,
-         ctx->d_status, ctx->d_ctr, ctx->d_small + 2, ep);
+  scan_excl_i32_status(ctx, n, in, out, ctx->d_status, ep);
+}
+
+void scan_excl_i32_status(cm_ctx *ctx, int64_t n, const int32_t *in, int32_t *out, uint64_t *status,
+                          uint32_t epoch)
+{
+  if (n == 0) {
+    fill_u32(ctx, out, 0u, 1);
+    return;
+  }
+  const int64_t tiles = (n + 1023) / 1024;
+  launch(ctx, "scan_bins", 0.0, k_scan_excl<int32_t>, dim3((unsigned)tiles), dim3(1024), 0, n, in, out, status,
+         ctx->d_ctr, ctx->d_small + 2, epoch);
 }
 
 // ------------------------------------------------------------------ gather

@@ -627,6 +627,7 @@ int cm_ctx_create(int device, void *stream, cm_ctx **out)
     CM_HIP(hipHostMalloc((void **)&ctx->h_small, sizeof(int64_t) * cm::SMALL_SLOTS, hipHostMallocDefault));
     CM_HIP(hipHostGetDevicePointer((void **)&ctx->h_small_dev, ctx->h_small, 0));
     cm::fill_u32(ctx, ctx->d_ctr, 0u, cm::CTR_SLOTS);
+    ctx->noalloc_scratch = cm::noalloc_init(device);
     cm::sync(ctx);
   } catch (const cm::Error &e) {
     int code = e.code;

@@ -137,6 +137,8 @@ struct cm_ctx {
   // the hash-contraction item counters (d_ctr + HASH_CTR_SLOT) are zero at
   // this point of the stream: the matching's init kernel clears them
   bool hash_ctr_zero = false;
+  // device address of this device's static scratch array (cm_noalloc.hip)
+  void *noalloc_scratch = nullptr;
 
   hipEvent_t ev_get();
   void collect_timing();
@@ -148,6 +150,9 @@ constexpr int CTR_SLOTS = 64;
 constexpr int HASH_CTR_SLOT = 32;  // 16 counters of the hash contraction
 constexpr int SMALL_SLOTS = 4096;
 
+// Device address of the allocation-free entry point's static scratch array
+// on `device` (which must be current); looked up once per device.
+void *noalloc_init(int device);
 // Host-side checks of a graph view (sizes, offset type, NULL arrays).
 void validate_view(const cm_graph_view *g);
 void ensure_status(cm_ctx *ctx, size_t words);
@@ -194,6 +199,10 @@ void fill_u32(cm_ctx *ctx, void *p, uint32_t v, int64_t count);
 const int32_t *unit_weights(cm_ctx *ctx, int64_t count);
 void fill_u64(cm_ctx *ctx, void *p, uint64_t v, int64_t count);
 void scan_excl_i32(cm_ctx *ctx, int64_t n, const int32_t *in, int32_t *out);
+// The same scan over caller-provided look-back status words (ceil(n/1024) of
+// them, none holding a word of `epoch` from an earlier launch).
+void scan_excl_i32_status(cm_ctx *ctx, int64_t n, const int32_t *in, int32_t *out, uint64_t *status,
+                          uint32_t epoch);
 
 // Owning graph of every vertex of a batch (disjoint union); gid == nullptr
 // means one graph with maxv0 / seed0.

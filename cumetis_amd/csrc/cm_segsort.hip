@@ -237,16 +237,73 @@ __global__ __launch_bounds__(256) void k_hub_info(const int32_t *list, int64_t c
 }
 
 // ---------------------------------------------------------------- host side
-template <typename OffT>
-static void bin_impl(cm_ctx *ctx, int64_t nseg, const OffT *segoff, SegBins &b)
+constexpr size_t ARENA_ALIGN = 256;
+
+static size_t arena_round(size_t bytes) { return (bytes + ARENA_ALIGN - 1) / ARENA_ALIGN * ARENA_ALIGN; }
+
+void *seg_alloc(cm_ctx *ctx, SegArena *ar, size_t bytes)
 {
-  b.list = (int32_t *)ctx->pool.alloc(sizeof(int32_t) * (nseg > 0 ? nseg : 1));
+  if (!ar) return ctx->pool.alloc(bytes);
+  const size_t b = arena_round(bytes ? bytes : 1);
+  CM_REQUIRE(ar->used + b <= ar->cap, CM_ERR_UNSUPPORTED, "segmented sort scratch exhausted");
+  void *p = ar->base + ar->used;
+  ar->used += b;
+  return p;
+}
+
+void seg_release(cm_ctx *ctx, SegArena *ar, void *p)
+{
+  if (!ar) ctx->pool.release(p);
+}
+
+// Status words of a look-back scan over n entries (1024-entry tiles): the
+// context's (pool mode), or cleared arena words with epoch 1 (arena memory
+// holds whatever the last call left there).
+static void scan_excl_arena(cm_ctx *ctx, SegArena *ar, int64_t n, const int32_t *in, int32_t *out)
+{
+  if (!ar) {
+    scan_excl_i32(ctx, n, in, out);
+    return;
+  }
+  const int64_t tiles = n > 0 ? (n + 1023) / 1024 : 1;
+  uint64_t *st = (uint64_t *)seg_alloc(ctx, ar, sizeof(uint64_t) * (size_t)tiles);
+  fill_u64(ctx, st, 0ull, tiles);
+  scan_excl_i32_status(ctx, n, in, out, st, 1u);
+}
+
+// Largest number of hub merge descriptors one pass can need for `total` keys:
+// per hub row of len > HUB_TILE keys, at most len / HUB_TILE + 1 tiles per
+// run pair plus one per pair, with fewer than total / HUB_TILE rows.
+static int64_t desc_bound(int64_t total) { return 2 * (total / HUB_TILE) + 16; }
+
+size_t segsort_i32_arena_bytes(int64_t n, int64_t length)
+{
+  const int64_t nb = (length + 255) / 256 > 0 ? (length + 255) / 256 : 1;
+  const int64_t nh = (int64_t)SEG_NBINS * nb;
+  size_t b = 0;
+  b += arena_round(4 * (size_t)(length + 1));                 // segoff
+  b += arena_round(8 * (size_t)std::max<int64_t>(n, 1));      // packed keys
+  b += arena_round(4 * (size_t)std::max<int64_t>(n, 1));      // values
+  b += arena_round(4 * (size_t)std::max<int64_t>(length, 1)); // bin lists
+  b += 2 * arena_round(4 * (size_t)(nh + 1));                 // histogram, bases
+  b += arena_round(8 * (size_t)((nh + 1023) / 1024 + 1));     // scan status words
+  b += arena_round(16 * (size_t)(n / (HUB_TILE + 1) + 1));    // hub (start, len)
+  b += arena_round(sizeof(MergeDesc) * (size_t)desc_bound(n));  // merge descriptors
+  b += arena_round(8 * (size_t)std::max<int64_t>(n, 1));      // merge ping-pong keys
+  b += arena_round(4 * (size_t)std::max<int64_t>(n, 1));      // and values
+  return b;
+}
+
+template <typename OffT>
+static void bin_impl(cm_ctx *ctx, int64_t nseg, const OffT *segoff, SegBins &b, SegArena *ar)
+{
+  b.list = (int32_t *)seg_alloc(ctx, ar, sizeof(int32_t) * (nseg > 0 ? nseg : 1));
   const unsigned nb = nblocks(nseg, 256);
   const int64_t nh = (int64_t)SEG_NBINS * nb;
-  int32_t *hist = (int32_t *)ctx->pool.alloc(sizeof(int32_t) * (nh + 1));
-  int32_t *base = (int32_t *)ctx->pool.alloc(sizeof(int32_t) * (nh + 1));
+  int32_t *hist = (int32_t *)seg_alloc(ctx, ar, sizeof(int32_t) * (nh + 1));
+  int32_t *base = (int32_t *)seg_alloc(ctx, ar, sizeof(int32_t) * (nh + 1));
   launch(ctx, "seg_bin_hist", 0.0, k_bin_hist<OffT>, dim3(nb), dim3(256), 0, nseg, segoff, nb, hist);
-  scan_excl_i32(ctx, nh, hist, base);  // base[nh] = number of listed segments
+  scan_excl_arena(ctx, ar, nh, hist, base);  // base[nh] = number of listed segments
   int64_t *d_off = ctx->d_small + 16;
   launch(ctx, "seg_bin_scatter", 0.0, k_bin_scatter<OffT>, dim3(nb), dim3(256), 0, nseg, segoff, nb,
          (const int32_t *)base, b.list, d_off);
@@ -254,19 +311,19 @@ static void bin_impl(cm_ctx *ctx, int64_t nseg, const OffT *segoff, SegBins &b)
                         hipMemcpyDeviceToHost, ctx->stream));
   sync(ctx);
   for (int c = 0; c <= SEG_NBINS; ++c) b.off[c] = ctx->h_small[16 + c];
-  ctx->pool.release(hist);
-  ctx->pool.release(base);
+  seg_release(ctx, ar, hist);
+  seg_release(ctx, ar, base);
 }
 
-void bin_segments(cm_ctx *ctx, int64_t nseg, const void *segoff, bool off64, SegBins &b)
+void bin_segments(cm_ctx *ctx, int64_t nseg, const void *segoff, bool off64, SegBins &b, SegArena *ar)
 {
-  if (off64) bin_impl(ctx, nseg, (const int64_t *)segoff, b);
-  else bin_impl(ctx, nseg, (const int32_t *)segoff, b);
+  if (off64) bin_impl(ctx, nseg, (const int64_t *)segoff, b, ar);
+  else bin_impl(ctx, nseg, (const int32_t *)segoff, b, ar);
 }
 
-void release_bins(cm_ctx *ctx, SegBins &b)
+void release_bins(cm_ctx *ctx, SegBins &b, SegArena *ar)
 {
-  if (b.list) ctx->pool.release(b.list);
+  if (b.list) seg_release(ctx, ar, b.list);
   b.list = nullptr;
 }
 
@@ -277,7 +334,7 @@ static unsigned cap_grid(int64_t count)
 
 template <typename OffT>
 static void sort_impl(cm_ctx *ctx, const SegBins &b, const OffT *segoff, int64_t total,
-                      uint64_t *keys, uint32_t *vals, int first_bin)
+                      uint64_t *keys, uint32_t *vals, int first_bin, SegArena *ar)
 {
   const int32_t *L = b.list;
 #define CM_WAVE_BIN(CLS, W)                                                                       \
@@ -302,14 +359,14 @@ static void sort_impl(cm_ctx *ctx, const SegBins &b, const OffT *segoff, int64_t
   const int64_t nhub = b.off[13] - b.off[12];
   if (!nhub) return;
   // hub rows: fetch (start, len), build tile and merge descriptors on the host
-  int64_t *d_info = (int64_t *)ctx->pool.alloc(sizeof(int64_t) * 2 * nhub);
+  int64_t *d_info = (int64_t *)seg_alloc(ctx, ar, sizeof(int64_t) * 2 * nhub);
   launch(ctx, "hub_info", 0.0, k_hub_info<OffT>, dim3(nblocks(nhub, 256)), dim3(256), 0,
          L + b.off[12], nhub, segoff, d_info);
   std::vector<int64_t> info(2 * nhub);
   CM_HIP(hipMemcpyAsync(info.data(), d_info, sizeof(int64_t) * 2 * nhub, hipMemcpyDeviceToHost,
                         ctx->stream));
   sync(ctx);
-  ctx->pool.release(d_info);
+  seg_release(ctx, ar, d_info);
   int64_t maxlen = 0, hubtotal = 0;
   std::vector<MergeDesc> descs;
   for (int64_t h = 0; h < nhub; ++h) {
@@ -321,8 +378,13 @@ static void sort_impl(cm_ctx *ctx, const SegBins &b, const OffT *segoff, int64_t
   }
   MergeDesc *d_desc = nullptr;
   size_t desc_cap = 0;
+  if (ar) {  // one block of the largest size a pass can need
+    desc_cap = (size_t)desc_bound(total);
+    d_desc = (MergeDesc *)seg_alloc(ctx, ar, sizeof(MergeDesc) * desc_cap);
+  }
   auto upload = [&](const std::vector<MergeDesc> &v) {
     if (v.size() > desc_cap) {
+      CM_REQUIRE(!ar, CM_ERR_UNSUPPORTED, "segmented sort: merge descriptors exceed the scratch bound");
       if (d_desc) ctx->pool.release(d_desc);
       desc_cap = v.size();
       d_desc = (MergeDesc *)ctx->pool.alloc(sizeof(MergeDesc) * desc_cap);
@@ -334,8 +396,8 @@ static void sort_impl(cm_ctx *ctx, const SegBins &b, const OffT *segoff, int64_t
   upload(descs);
   launch(ctx, "segsort_hub_tile", 0.0, k_sort_tiles, dim3(cap_grid((int64_t)descs.size())),
          dim3(256), 0, (const MergeDesc *)d_desc, (int64_t)descs.size(), keys, vals);
-  uint64_t *k2 = (uint64_t *)ctx->pool.alloc(sizeof(uint64_t) * (size_t)total);
-  uint32_t *v2 = (uint32_t *)ctx->pool.alloc(sizeof(uint32_t) * (size_t)total);
+  uint64_t *k2 = (uint64_t *)seg_alloc(ctx, ar, sizeof(uint64_t) * (size_t)total);
+  uint32_t *v2 = (uint32_t *)seg_alloc(ctx, ar, sizeof(uint32_t) * (size_t)total);
   uint64_t *sk = keys, *dk = k2;
   uint32_t *sv = vals, *dv = v2;
   for (int64_t w = HUB_TILE; w < maxlen; w *= 2) {
@@ -369,16 +431,16 @@ static void sort_impl(cm_ctx *ctx, const SegBins &b, const OffT *segoff, int64_t
            (const uint32_t *)sv, keys, vals);
   }
   sync(ctx);
-  ctx->pool.release(k2);
-  ctx->pool.release(v2);
-  if (d_desc) ctx->pool.release(d_desc);
+  seg_release(ctx, ar, k2);
+  seg_release(ctx, ar, v2);
+  if (d_desc) seg_release(ctx, ar, d_desc);
 }
 
 void sort_binned(cm_ctx *ctx, const SegBins &b, const void *segoff, bool off64, int64_t total,
-                 uint64_t *keys, uint32_t *vals, int first_bin)
+                 uint64_t *keys, uint32_t *vals, int first_bin, SegArena *ar)
 {
-  if (off64) sort_impl(ctx, b, (const int64_t *)segoff, total, keys, vals, first_bin);
-  else sort_impl(ctx, b, (const int32_t *)segoff, total, keys, vals, first_bin);
+  if (off64) sort_impl(ctx, b, (const int64_t *)segoff, total, keys, vals, first_bin, ar);
+  else sort_impl(ctx, b, (const int32_t *)segoff, total, keys, vals, first_bin, ar);
 }
 
 void segsort_u64(cm_ctx *ctx, int64_t nseg, const void *d_segoff, bool off64, int64_t total,
@@ -427,23 +489,26 @@ __global__ void k_make_segoff(int32_t n, const int32_t *segs, int32_t len, int32
 }
 
 void segsort_i32(cm_ctx *ctx, int32_t *keys, int32_t *vals, int32_t n, const int32_t *segs,
-                 int32_t length)
+                 int32_t length, SegArena *ar)
 {
   if (n <= 0 || length <= 0) return;
-  int32_t *segoff = (int32_t *)ctx->pool.alloc(sizeof(int32_t) * (length + 1));
-  uint64_t *ck = (uint64_t *)ctx->pool.alloc(sizeof(uint64_t) * n);
-  uint32_t *cv = (uint32_t *)ctx->pool.alloc(sizeof(uint32_t) * n);
+  int32_t *segoff = (int32_t *)seg_alloc(ctx, ar, sizeof(int32_t) * (length + 1));
+  uint64_t *ck = (uint64_t *)seg_alloc(ctx, ar, sizeof(uint64_t) * n);
+  uint32_t *cv = (uint32_t *)seg_alloc(ctx, ar, sizeof(uint32_t) * n);
   launch(ctx, "segsort_i32_prep", 0.0, k_make_segoff, dim3(nblocks(length + 1, 256)), dim3(256), 0,
          n, segs, length, segoff);
   launch(ctx, "segsort_i32_pack", 0.0, k_pack_i32, dim3(nblocks(n, 256)), dim3(256), 0, n,
          (const int32_t *)segoff, length, (const int32_t *)keys, (const int32_t *)vals, ck, cv);
-  segsort_u64(ctx, length, segoff, false, n, ck, cv);
+  SegBins b;
+  bin_segments(ctx, length, segoff, false, b, ar);
+  sort_binned(ctx, b, segoff, false, n, ck, cv, 0, ar);
+  release_bins(ctx, b, ar);
   launch(ctx, "segsort_i32_unpack", 0.0, k_unpack_i32, dim3(nblocks(n, 256)), dim3(256), 0, n,
          (const uint64_t *)ck, (const uint32_t *)cv, keys, vals);
   sync(ctx);
-  ctx->pool.release(segoff);
-  ctx->pool.release(ck);
-  ctx->pool.release(cv);
+  seg_release(ctx, ar, segoff);
+  seg_release(ctx, ar, ck);
+  seg_release(ctx, ar, cv);
 }
 
 }  // namespace cm

@@ -196,6 +196,20 @@ int cm_contract(cm_ctx *ctx, const cm_graph_view *g, const int32_t *d_match,
  * fixed (by original position). */
 int cm_segsort_i32(cm_ctx *ctx, int32_t *d_keys, int32_t *d_vals, int32_t n,
                    const int32_t *d_segs, int32_t length);
+/* cm_segsort_i32 without allocating: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid it).  Same arguments and the
+ * same result as cm_segsort_i32; all scratch comes from one static device
+ * array of CM_NOALLOC_SCRATCH_BYTES per device (held until the process
+ * ends), so the call performs no hipMalloc/hipFree.  It needs about
+ * 24 n + 9 length + 64 KiB bytes of that array (n keys in `length`
+ * segments: about one million keys in as many segments, 1.3 million in
+ * long segments): larger calls return CM_ERR_UNSUPPORTED.  Every call on one device shares that device's
+ * array and it keeps the last call's contents: calls that overlap on one
+ * device (other contexts, streams or host threads) race. */
+#define CM_NOALLOC_SCRATCH_BYTES (32u << 20)
+int cm_segsort_i32_noalloc(cm_ctx *ctx, int32_t *d_keys, int32_t *d_vals, int32_t n,
+                           const int32_t *d_segs, int32_t length);
 /* Projection on raw arrays (cu:3067-3075). */
 int cm_project_raw(cm_ctx *ctx, int64_t n, const int32_t *d_cmap, const int32_t *d_where_c,
                    int32_t *d_where_f);
