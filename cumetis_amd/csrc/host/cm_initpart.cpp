@@ -68,7 +68,16 @@ void rand_perm(int n, int *p, int m, bool init)
 // Max-priority queue with float keys and a position index (cu:1976-2242).
 class MaxQueue {
  public:
+  MaxQueue() = default;
   explicit MaxQueue(int cap) : heap_(cap > 0 ? cap : 1), pos_(cap > 0 ? cap : 1, -1) {}
+  // room for vertex ids below cap; an empty queue stays empty
+  void reserve(int cap)
+  {
+    if ((size_t)cap > heap_.size()) {
+      heap_.resize(cap);
+      pos_.resize(cap, -1);
+    }
+  }
   void clear()
   {
     for (long i = size_ - 1; i >= 0; --i) pos_[heap_[i].val] = -1;
@@ -160,6 +169,27 @@ class MaxQueue {
     put(i, k, node);
   }
 };
+
+// Per-thread FM scratch kept between calls.  The reference creates two
+// queues with an n-entry position index per FM call (cu:1976-1990, 2393-
+// 2394), an O(n) cost that dominated the initial partition of large graphs;
+// here the queues and the `moved` marks live on, and every FM call leaves
+// them as it found them (queues empty, marks -1).
+struct FmWork {
+  MaxQueue q[2];
+  std::vector<int> moved, swaps, perm;  // swaps and perm are written before read
+  void reserve(int n)
+  {
+    q[0].reserve(n);
+    q[1].reserve(n);
+    if ((size_t)n > moved.size()) moved.resize(n, -1);
+    if ((size_t)n + 1 > swaps.size()) {
+      swaps.resize(n + 1);
+      perm.resize(n + 1);
+    }
+  }
+};
+thread_local FmWork tl_fm;
 
 // Graph of one recursion step plus its 2-way state.
 struct Part2 {
@@ -410,9 +440,19 @@ void fm_refine(Part2 &g, const float *ntp, int niter)  // cu:2367-2531
   const int limit = (int)lim;
   const int tot = g.pwgts[0] + g.pwgts[1];
   const int avgvwgt = std::min(tot / 20, 2 * tot / n);
-  std::vector<int> moved(n, -1), swaps(std::max(n, 1)), perm(std::max(n, 1));
-  MaxQueue q0(n), q1(n);
+  tl_fm.reserve(n);
+  std::vector<int> &moved = tl_fm.moved;  // all -1 between passes
+  std::vector<int> &swaps = tl_fm.swaps, &perm = tl_fm.perm;
+  MaxQueue &q0 = tl_fm.q[0], &q1 = tl_fm.q[1];
   MaxQueue *qs[2] = {&q0, &q1};
+  struct Empty {  // the queues go back empty, also on an early return
+    MaxQueue &a, &b;
+    ~Empty()
+    {
+      a.clear();
+      b.clear();
+    }
+  } empty_on_exit{q0, q1};
   const int origdiff = abs(tp[0] - g.pwgts[0]);
   for (int pass = 0; pass < niter; ++pass) {
     q0.clear();
@@ -483,33 +523,65 @@ void fm_refine(Part2 &g, const float *ntp, int niter)  // cu:2367-2531
   }
 }
 
+// Unseen vertices for the BFS restarts of grow_bisection: the reference picks
+// the k-th unseen vertex in index order with a linear scan (cu:2585-2595),
+// which is quadratic on graphs with many components (R-MAT's isolated
+// vertices: millions of restarts).  A Fenwick tree over the unseen flags
+// gives the same vertex in O(log n).
+struct Unseen {
+  std::vector<int> t;  // 1-based partial sums
+  int n = 0, top = 1;
+  void reset(int nn)
+  {
+    n = nn;
+    t.assign(n + 1, 0);
+    for (int i = 1; i <= n; ++i) t[i] = i & -i;  // all vertices unseen
+    top = 1;
+    while (top * 2 <= n) top *= 2;
+  }
+  void mark(int v)
+  {
+    for (int i = v + 1; i <= n; i += i & -i) --t[i];
+  }
+  // index of the unseen vertex with k unseen vertices before it
+  int kth(int k) const
+  {
+    int pos = 0;
+    for (int step = top; step > 0; step >>= 1)
+      if (pos + step <= n && t[pos + step] <= k) {
+        pos += step;
+        k -= t[pos];
+      }
+    return pos;
+  }
+};
+
 void grow_bisection(Part2 &g, Settings &s, const float *ntp, int niparts)  // cu:2536-2661
 {
   const int n = g.n;
   g.alloc2way();
   std::vector<int> best(n), queue(std::max(n, 1)), seen(n);
+  Unseen unseen;
   const int onemax = (int)(s.ub * g.tvwgt * ntp[1]);
   const int onemin = (int)((1.0 / s.ub) * g.tvwgt * ntp[1]);
   int bestcut = 0;
   for (int it = 0; it < niparts; ++it) {
     std::fill(g.where.begin(), g.where.end(), 1);
     std::fill(seen.begin(), seen.end(), 0);
+    unseen.reset(n);
     int pw0 = 0, pw1 = g.tvwgt;
     queue[0] = rnd(n);
     seen[queue[0]] = 1;
+    unseen.mark(queue[0]);
     int first = 0, last = 1, nleft = n - 1;
     bool stalled = false;
     for (;;) {
       if (first == last) {  // component exhausted: restart from a random unseen vertex
         if (nleft == 0 || stalled) break;
-        int k = rnd(nleft), i = 0;
-        for (; i < n; ++i)
-          if (seen[i] == 0) {
-            if (k == 0) break;
-            --k;
-          }
+        const int i = unseen.kth(rnd(nleft));
         queue[0] = i;
         seen[i] = 1;
+        unseen.mark(i);
         first = 0;
         last = 1;
         --nleft;
@@ -529,6 +601,7 @@ void grow_bisection(Part2 &g, Settings &s, const float *ntp, int niparts)  // cu
         if (!seen[k]) {
           queue[last++] = k;
           seen[k] = 1;
+          unseen.mark(k);
           --nleft;
         }
       }

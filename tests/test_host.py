@@ -39,6 +39,20 @@ def test_initial_partition_matches_oracle(orc, name, k):
     assert op == oo
 
 
+@pytest.mark.parametrize("k", [2, 8, 64])
+def test_initial_partition_many_components(orc, k):
+    """R-MAT scale 13 keeps a third of its vertices isolated, so every BFS of
+    the bisection restarts thousands of times from the k-th unseen vertex
+    (cu:2585-2595); the product finds it with a Fenwick tree instead of the
+    reference's scan, and must pick the same vertex every time."""
+    g = with_weights(graphs.rmat(13, 8, seed=6, offsets=np.int64))
+    assert (np.diff(g.xadj) == 0).sum() > g.n // 5
+    wp, op = product_initpart(g, k)
+    orc.srand(1)
+    wo, oo = orc.initpart(g.xadj, g.adjncy, g.adjwgt, g.vwgt, k)
+    assert np.array_equal(wp, wo) and op == oo
+
+
 def test_initial_partition_weighted_vertices(orc):
     g = with_weights(graphs.grid2d(25, 25, wmax=30, seed=4))
     g.vwgt[:] = np.random.default_rng(1).integers(1, 6, size=g.n)
