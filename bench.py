@@ -84,15 +84,29 @@ def measured_traffic(workload, kernel):
     return None if b is None else int(b)
 
 
-def kernel_profile(eng, step, reps=3, workload=None):
-    """Per-kernel hipEvent timing on the engine stream (separate pass)."""
+def kernel_profile(eng, step, reps=3, workload=None, plain_s=None):
+    """Per-kernel hipEvent timing on the engine stream (separate pass).
+    Events around every launch keep the stream from starting a launch while
+    the previous one drains, so each event-timed launch carries an extra:
+    the step time with events minus the plain step time (plain_s, from the
+    timed steps), spread over the step's launches.  The roofline gives the
+    achieved rate with and without that extra."""
     eng.set_timing(True)
     eng.reset_stats()
+    wall = []
     for _ in range(reps):
-        eng.lib.cm_hier_release(eng.ctx, step())
+        t = time.perf_counter()
+        h = step()
+        eng.sync()
+        wall.append(time.perf_counter() - t)
+        eng.lib.cm_hier_release(eng.ctx, h)
     eng.sync()
     ks = eng.kernel_stats()
     eng.set_timing(False)
+    launches = sum(s["count"] for s in ks.values()) / reps
+    overhead_us = None
+    if plain_s is not None and launches > 0:
+        overhead_us = max(0.0, (float(np.median(wall)) - plain_s) / launches * 1e6)
     priced = {n: s for n, s in ks.items() if s["bytes"] > 0}
     roof = None
     if priced:
@@ -104,6 +118,12 @@ def kernel_profile(eng, step, reps=3, workload=None):
                 "unit": "GB/s", "frac": round(ach / 8000.0, 4),
                 "traffic": measured_traffic(workload, name) if workload else None,
                 "avg_launch_us": round(avg_s * 1e6, 2), "algo_bytes_per_launch": int(bpl)}
+        if overhead_us is not None and avg_s * 1e6 > overhead_us:
+            net = avg_s - overhead_us / 1e6
+            roof["event_overhead_us"] = round(overhead_us, 2)
+            roof["launches_per_step"] = round(launches, 1)
+            roof["achieved_net_of_event_overhead"] = round(bpl / net / 1e9, 2)
+            roof["frac_net_of_event_overhead"] = round(bpl / net / 1e9 / 8000.0, 4)
     per = {n: round(s["ms"] / reps, 4) for n, s in sorted(ks.items(), key=lambda kv: -kv[1]["ms"])[:14]}
     return roof, per
 
@@ -137,7 +157,7 @@ def coarsen_leg(eng, g, k, order, warmup, steps, profile=True, workload=None):
            "rounds": [r for (_, _, r) in shape[:-1]],
            "hbm_frac_whole_coarsening": round(algo / med / HBM_PEAK, 5), "algo_bytes": int(algo),
            "input_medges_per_s": round((g.m // 2) / med / 1e6, 2), "edges": int(edges)}
-    roof, per = kernel_profile(eng, step, workload=workload) if profile else (None, {})
+    roof, per = kernel_profile(eng, step, workload=workload, plain_s=med) if profile else (None, {})
     # the same coarsening from host arrays (the cuMetis_PartGraph boundary):
     # includes the host-to-device copies of the graph; reported beside the
     # device-resident value, never as it
@@ -168,8 +188,8 @@ def cpu_model():
 
 def cpu_baseline(g, k, budget_s=12.0):
     """Reference CPU coarsening path (cuMetis_cpu_coarsen restated in C,
-    single thread, pinned to one core of this host), repeated within a time
-    budget."""
+    built with -O3, single thread, pinned to one core of this host),
+    repeated within a time budget (at least one full run)."""
     import os
     from oracle import oracle as O
     x = g.xadj.astype(np.int64)
@@ -234,7 +254,7 @@ def batch_graphs(count=512, seed=2024):
     return gs, ks
 
 
-def batch_leg(eng, count, order, steps):
+def batch_leg(eng, count, order, steps, cpu=True):
     from cumetis_amd import _lib as L
     from cumetis_amd.engine import concat_graphs
     progress(f"batch leg: generating {count} graphs")
@@ -290,22 +310,115 @@ def batch_leg(eng, count, order, steps):
     ot = time.perf_counter() - t
     oshape = [hier_shape(eng, h) for h in hs]
     release(hs)
-    # identical hierarchies => identical partitions => equal edge cut
-    same = all([(n, m) for (n, m, _) in a] == [(n, m) for (n, m, _) in b] for a, b in zip(bshape, oshape))
+    same_shape = all([(n, m) for (n, m, _) in a] == [(n, m) for (n, m, _) in b] for a, b in zip(bshape, oshape))
+    # every level array of a sample of graphs, batched against alone
+    progress("batch leg: level arrays of a sample")
+    sample = list(range(0, nb, max(1, nb // 16)))[:16]
+    hb = batched()
+    arrays_equal = True
+    for b in sample:
+        h = C.c_void_p()
+        eng._check(eng.lib.cm_coarsen(eng.ctx, C.byref(views[b][0]), C.byref(P[b]), C.byref(h)))
+        arrays_equal &= hier_arrays_equal(eng, C.c_void_p(hb[b]), h)
+        eng.lib.cm_hier_release(eng.ctx, h)
+    release(hb)
+    del views
+    cuts = batch_cuts(gs, ks, sample[:8])
     med = float(np.median(bt))
-    return {"graphs": nb, "sum_n": int(voff[-1]), "sum_m": int(eoff[-1]),
-            "batched_graphs_per_s": round(nb / med, 1), "batched_ms": round(med * 1e3, 2),
-            "one_by_one_graphs_per_s": round(nb / ot, 1), "one_by_one_ms": round(ot * 1e3, 2),
-            "speedup": round(ot / med, 2), "hierarchies_identical": bool(same),
-            "equal_cut": "by construction: identical hierarchies give identical partitions"}
+    out = {"graphs": nb, "sum_n": int(voff[-1]), "sum_m": int(eoff[-1]),
+           "batched_graphs_per_s": round(nb / med, 1), "batched_ms": round(med * 1e3, 2),
+           "one_by_one_graphs_per_s": round(nb / ot, 1), "one_by_one_ms": round(ot * 1e3, 2),
+           "speedup": round(ot / med, 2), "level_shapes_identical": bool(same_shape),
+           "level_arrays_identical_sample": {"graphs": len(sample), "identical": bool(arrays_equal)},
+           "cuts": cuts}
+    if cpu:
+        out["cpu_all_cores"] = batch_cpu_all_cores(gs, ks)
+    return out
 
 
-def serve_leg(count=120, rate=20.0, seed=7):
+def hier_arrays_equal(eng, ha, hb):
+    """Every level's xadj/adjncy/adjwgt/vwgt/match/cmap of two hierarchies,
+    local values (bases subtracted by cm_hier_copy_level)."""
+    from cumetis_amd.engine import Hierarchy
+    a, b = Hierarchy(eng, ha), Hierarchy(eng, hb)
+    try:
+        if a.num_levels != b.num_levels:
+            return False
+        for i in range(a.num_levels):
+            x, y = a.level(i), b.level(i)
+            for f in ("xadj", "adjncy", "adjwgt", "vwgt", "match", "cmap"):
+                u, v = getattr(x, f), getattr(y, f)
+                if (u is None) != (v is None):
+                    return False
+                if u is not None and not np.array_equal(u.astype(np.int64), v.astype(np.int64)):
+                    return False
+        return True
+    finally:
+        a.h = b.h = None  # released by the caller
+
+
+def batch_cuts(gs, ks, sample):
+    """Edge cuts of a sample of the batch's graphs partitioned alone
+    (cm_partition) and through the server, whose coarsening runs every
+    admitted job's level as one batched union per step."""
+    from cumetis_amd.engine import Engine, Server
+    alone = []
+    with Engine(0) as e2:
+        for b in sample:
+            alone.append(int(e2.partition(gs[b], ks[b], seed=1)[1].edgecut))
+    with Server(0, max_slots=64) as srv:
+        tickets = [srv.submit(gs[b], ks[b], seed=1) for b in sample]
+        served = [int(srv.wait(t)[1].edgecut) for t in tickets]
+    return {"graphs": len(sample), "alone": alone, "batched_union": served, "equal": alone == served}
+
+
+def _cpu_coarsen_edges(g, k):
+    from oracle import oracle as O
+    x = g.xadj.astype(np.int64)
+    w = g.adjwgt if g.adjwgt is not None else np.ones(g.m, np.int32)
+    lv = O.coarsen(x, g.adjncy, w, None, k, matcher="cpu")
+    return sum(int(L["xadj"][-1]) // 2 for L in lv[:-1])
+
+
+def batch_cpu_all_cores(gs, ks):
+    """The reference CPU coarsening of the whole batch with one graph per
+    host core (threads calling the C restatement, which runs without the
+    GIL); its rand() stream is shared, so the matchings differ from a
+    one-at-a-time run, the work does not."""
+    import concurrent.futures as cf
+    import os
+    cores = len(os.sched_getaffinity(0))
+    t = time.perf_counter()
+    with cf.ThreadPoolExecutor(cores) as ex:
+        edges = sum(ex.map(lambda b: _cpu_coarsen_edges(gs[b], ks[b]), range(len(gs))))
+    el = time.perf_counter() - t
+    return {"graphs_per_s": round(len(gs) / el, 1), "medges_per_s": round(edges / el / 1e6, 2),
+            "cores": cores, "seconds": round(el, 2), "kind": "port",
+            "sample": f"all {len(gs)} graphs, one per core"}
+
+
+def e2e_leg(eng, g, k, trials=0, reps=3):
+    """The whole pipeline (cm_partition: coarsening, host initial partition,
+    uncoarsening with balancing and refinement) from host arrays, with the
+    phase times of the reference's report lines (cu:3901-3919)."""
+    reps_out = []
+    for _ in range(reps):
+        part, rep = eng.partition(g, k, seed=1, trials=trials)
+        reps_out.append(rep)
+    r = min(reps_out, key=lambda x: x.total_ms)
+    return {"total_ms": round(r.total_ms, 2), "coarsen_ms": round(r.coarsen_ms, 2),
+            "init_ms": round(r.init_ms, 2), "uncoarsen_ms": round(r.uncoarsen_ms, 2),
+            "trials": int(r.trials), "edgecut": int(r.edgecut), "levels": int(r.levels),
+            "coarsest_n": int(r.coarsest_n), "runs": reps}
+
+
+def serve_leg(count=120, rate=20.0, seed=7, trials=0):
     """Poisson stream of partition requests: the batch distribution plus 5%
     3D 7-point grids 160^3 (k=128), at a fixed offered load."""
     from cumetis_amd import graphs
+    from cumetis_amd import _lib as L
     from cumetis_amd.engine import Server
-    progress(f"serve leg: {count} requests at {rate}/s")
+    progress(f"serve leg: {count} requests at {rate}/s, trials {trials or 'default'}")
     rng = np.random.default_rng(seed)
     small, ks = batch_graphs(max(8, count), seed=seed)
     big = graphs.grid3d(160, 160, 160)
@@ -325,12 +438,13 @@ def serve_leg(count=120, rate=20.0, seed=7):
             d = due - time.perf_counter()
             if d > 0:
                 time.sleep(d)
-            tickets.append(srv.submit(g, k, seed=1))
+            tickets.append(srv.submit(g, k, seed=1, trials=trials))
         progress("serve leg: all submitted")
         reps = [srv.wait(t)[1] for t in tickets]
         wall = time.perf_counter() - t0
         st = srv.stats()
-    return {"requests": count, "offered_per_s": rate, "large_jobs": sum(1 for g, k in reqs if k == 128),
+    return {"requests": count, "offered_per_s": rate, "trials_per_request": trials or L.CM_DEFAULT_TRIALS,
+            "large_jobs": sum(1 for g, k in reqs if k == 128),
             "p50_ms": round(st["p50_ms"], 2), "p99_ms": round(st["p99_ms"], 2),
             "graphs_per_s": round(count / wall, 2), "max_concurrent_jobs": st["max_active"],
             "mean_coarsen_levels": round(float(np.mean([r.levels for r in reps])), 2)}
@@ -342,8 +456,10 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--order", default="first_seen", choices=["first_seen", "ascending"])
-    ap.add_argument("--legs", default="grid725,rmat22,batch512,serve",
-                    help="comma list of: grid725 (always run), rmat22, batch512, serve")
+    ap.add_argument("--legs", default="grid725,delaunay4720,rmat22,batch512,serve,e2e",
+                    help="comma list of: grid725 (always run), delaunay4720, rmat22, batch512, serve, "
+                         "e2e (grid 725^2 k=64 through cm_partition), e2e_rmat22 (R-MAT s22 k=256, one "
+                         "trial: minutes of host initial partition)")
     ap.add_argument("--batch-count", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
@@ -365,19 +481,45 @@ def main():
                                       profile=not args.no_profile,
                                       workload="grid725" if args.order == "first_seen" else None)
     extra = {}
+    if "delaunay4720" in legs:
+        progress("delaunay4720")
+        gd = graphs.delaunay2d(4720, seed=1)
+        o, rd, pd = coarsen_leg(eng, gd, 8, order, args.warmup, args.steps, profile=not args.no_profile)
+        o["roofline"] = rd
+        o["kernels_ms_per_step"] = pd
+        o["config"] = "2D Delaunay triangulation of 4720 uniform points (seed 1), unit weights, k=8, int32 CSR"
+        if not args.no_cpu:
+            o["cpu_baseline"] = cpu_baseline(gd, 8, budget_s=5.0)
+        extra["delaunay4720"] = o
     if "rmat22" in legs:
         progress("rmat22: generating")
         gr = graphs.rmat_fast(22, 16, seed=1, offsets=np.int64)
-        o, rr, pr = coarsen_leg(eng, gr, 256, order, 1, 3, profile=not args.no_profile)
+        o, rr, pr = coarsen_leg(eng, gr, 256, order, args.warmup, args.steps, profile=not args.no_profile,
+                                workload="rmat22" if args.order == "first_seen" else None)
         o["roofline"] = rr
         o["kernels_ms_per_step"] = pr
         o["config"] = "R-MAT s22 ef16 (.57/.19/.19), symmetrised, unit weights, k=256, int64 offsets"
+        # actually required bytes: with no edge weights given, the level-0
+        # kernels read no adjwgt (8m of the 16m per level are unit weights)
+        w0 = 4 * gr.m  # one int32 per entry, read twice (matching, contraction)
+        o["algo_bytes_without_unit_adjwgt"] = int(o["algo_bytes"] - 2 * w0)
+        o["hbm_frac_without_unit_adjwgt"] = round(o["algo_bytes_without_unit_adjwgt"] / (o["ms"] / 1e3) / HBM_PEAK, 5)
+        if not args.no_cpu:
+            progress("rmat22: cpu baseline")
+            o["cpu_baseline"] = cpu_baseline(gr, 256, budget_s=10.0)
         extra["rmat22"] = o
+        if "e2e_rmat22" in legs:
+            progress("rmat22: end to end (one trial)")
+            extra["e2e_rmat22"] = e2e_leg(eng, gr, 256, trials=1, reps=1)
         del gr
     if "batch512" in legs:
-        extra["batch512"] = batch_leg(eng, args.batch_count, order, 3)
+        extra["batch512"] = batch_leg(eng, args.batch_count, order, 3, cpu=not args.no_cpu)
     if "serve" in legs:
         extra["serve"] = serve_leg()
+        extra["serve_one_trial"] = serve_leg(trials=1)
+    if "e2e" in legs:
+        progress("e2e grid725")
+        extra["e2e_grid725"] = e2e_leg(eng, g, 64)
     progress("stream copy")
     copy_gbs = stream_copy_gbs(eng)
     if roof is not None:

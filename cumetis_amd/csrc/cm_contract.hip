@@ -35,8 +35,12 @@ static_assert(REG_SEG == 16, "FIRST_SORTED_BIN must be the first bin above REG_S
 // totals[0] = nc, totals[3] = largest coarse vertex weight, totals[4] =
 // longest segment (a bound on the longest coarse row).
 constexpr int NUM_PER = 2;
+// Levels of at most 1024 * NUM_PER_ONE vertices are numbered by one tile:
+// no look-back at all (each chain step goes through the memory-side
+// coherence point, a few us on the small levels of a hierarchy).
+constexpr int NUM_PER_ONE = 8;
 
-template <typename OffT>
+template <typename OffT, int NUM_PER>
 __global__ __launch_bounds__(1024) void k_number(int64_t n, const OffT *__restrict__ xadj,
                                                  const int32_t *match_in, int32_t *match,
                                                  const int32_t *__restrict__ vwgt,
@@ -155,7 +159,9 @@ static int64_t number_impl(cm_ctx *ctx, const Level &L, const int32_t *match_in,
     fill_u32(ctx, ctx->d_small, 0u, 10);  // nc and the maxima (slots 0..4)
     return 0;
   }
-  const int64_t tiles = (n + 1024 * NUM_PER - 1) / (1024 * NUM_PER);
+  const bool one = n <= 1024 * NUM_PER_ONE;
+  const int per = one ? NUM_PER_ONE : NUM_PER;
+  const int64_t tiles = (n + 1024 * per - 1) / (1024 * per);
   const uint32_t ep = lookback_begin(ctx, 4 * tiles);  // 4 chains
   HcLists sel;
   if (hc) {  // the select step into hc's lists, whose counters start at zero
@@ -164,10 +170,11 @@ static int64_t number_impl(cm_ctx *ctx, const Level &L, const int32_t *match_in,
     ctx->hash_ctr_zero = false;
     hc->selected = true;
   }
-  launch(ctx, "number_scan", (double)sizeof(OffT) * (n + 1) * 2 + 16.0 * n, k_number<OffT>,
-         dim3((unsigned)tiles), dim3(1024), 0, n, (const OffT *)L.xadj, match_in, match, (const int32_t *)L.vwgt,
-         cmap, rep, segoff, cvwgt, ctx->d_status, ctx->d_status + tiles, ctx->d_status + 2 * tiles,
-         ctx->d_status + 3 * tiles, ctx->d_ctr, ctx->d_small, sel, ep);
+  launch(ctx, "number_scan", (double)sizeof(OffT) * (n + 1) * 2 + 16.0 * n,
+         one ? k_number<OffT, NUM_PER_ONE> : k_number<OffT, NUM_PER>, dim3((unsigned)tiles), dim3(1024), 0, n,
+         (const OffT *)L.xadj, match_in, match, (const int32_t *)L.vwgt, cmap, rep, segoff, cvwgt, ctx->d_status,
+         ctx->d_status + tiles, ctx->d_status + 2 * tiles, ctx->d_status + 3 * tiles, ctx->d_ctr, ctx->d_small, sel,
+         ep);
   if (!wait) return -1;
   CM_HIP(hipMemcpyAsync(ctx->h_small, ctx->d_small, sizeof(int64_t), hipMemcpyDeviceToHost,
                         ctx->stream));
