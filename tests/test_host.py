@@ -103,6 +103,82 @@ def test_read_metis_errors(tmp_path, text, msg):
     assert msg in err.value
 
 
+def _read(path, threads):
+    import os
+    lib = _lib.load()
+    os.environ["CM_READ_THREADS"] = str(threads)
+    try:
+        n, m = C.c_int64(), C.c_int64()
+        px, pa, pw, pv = C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_void_p()
+        err = C.create_string_buffer(512)
+        rc = lib.cm_read_metis(str(path).encode(), C.byref(n), C.byref(m), C.byref(px), C.byref(pa),
+                               C.byref(pw), C.byref(pv), err, 512)
+    finally:
+        del os.environ["CM_READ_THREADS"]
+    if rc != 0:
+        return rc, err.value
+    get = lambda p, t, k: np.ctypeslib.as_array((t * k).from_address(p.value)).copy() if k else np.zeros(0)
+    out = (get(px, C.c_int64, n.value + 1), get(pa, C.c_int32, m.value), get(pw, C.c_int32, m.value),
+           get(pv, C.c_int32, n.value))
+    for p_ in (px, pa, pw, pv):
+        lib.cm_free_host(p_)
+    return rc, out
+
+
+@pytest.mark.parametrize("weights", ["none", "edges", "both"])
+def test_read_metis_parallel_equals_sequential(tmp_path, weights):
+    """The chunked reader gives the same arrays with 1, 7 and 16 threads, and
+    the arrays of an independent line-by-line reader, on a file with comment
+    lines and empty rows (isolated vertices) spread through it."""
+    g = graphs.rgg2d(20000, 6.0, wmax=16, seed=8)
+    vw = np.random.default_rng(2).integers(0, 9, size=g.n).astype(np.int32)
+    gg = graphs.Graph(g.xadj, g.adjncy, g.adjwgt if weights != "none" else None, vw if weights == "both" else None)
+    f = tmp_path / "g.graph"
+    graphs.write_metis(str(f), gg)
+    lines = f.read_text().split("\n")
+    rng = np.random.default_rng(3)
+    for at in sorted(rng.integers(1, len(lines) - 1, size=40), reverse=True):
+        lines.insert(int(at), "% a comment line")
+    f.write_text("\n".join(lines))
+    ref = graphs.read_metis(str(f))
+    assert (np.diff(ref.xadj) == 0).sum() > 0
+    outs = [_read(f, t) for t in (1, 7, 16)]
+    for rc, (x, a, w, v) in outs:
+        assert rc == 0
+        np.testing.assert_array_equal(x, ref.xadj.astype(np.int64))
+        np.testing.assert_array_equal(a, ref.adjncy)
+        np.testing.assert_array_equal(w, ref.adjwgt)
+        np.testing.assert_array_equal(v, vw if weights == "both" else np.ones(g.n, np.int32))
+
+
+@pytest.mark.parametrize("threads", [1, 5, 16])
+def test_read_metis_parallel_first_error_wins(tmp_path, threads):
+    """Errors in several chunks: the one first in file order is reported,
+    as the sequential reader would; a line that passes the declared edge
+    count before its own bad token reports the overflow."""
+    g = graphs.grid2d(60, 60, seed=1)
+    f = tmp_path / "g.graph"
+    graphs.write_metis(str(f), g)
+    lines = f.read_text().split("\n")
+    n = g.n
+    bad = lines.copy()
+    bad[1 + 3000] = bad[1 + 3000] + " 9999"   # out of bounds, late
+    bad[1 + 1200] = bad[1 + 1200] + " 1201"   # self loop of vertex 1201, earlier
+    f.write_text("\n".join(bad))
+    rc, msg = _read(f, threads)
+    assert rc == _lib.CM_ERR_INPUT and msg == b"Vertex 1201 has a self loop"
+    # one edge too many on the last line, followed by an out-of-range id
+    over = lines.copy()
+    over[n] = over[n] + f" {n - 60} 99999"
+    f.write_text("\n".join(over))
+    rc, msg = _read(f, threads)
+    assert rc == _lib.CM_ERR_INPUT and b"more edges in the file than" in msg
+    # fewer vertex lines than declared
+    f.write_text("\n".join(lines[:n - 5]))
+    rc, msg = _read(f, threads)
+    assert rc == _lib.CM_ERR_INPUT and msg == f"Premature end of input file while reading vertex {n - 5}.".encode()
+
+
 def test_write_partition(tmp_path):
     lib = _lib.load()
     part = np.array([3, 0, 2, 1], np.int32)
