@@ -10,16 +10,16 @@
 //      beats it (cu:3272-3309);
 //   3. candidates are sorted by (destination, vertex) with the segmented
 //      sort (the reference sorts by destination, cu:3387);
-//   4. the host walks the sorted list once and accepts moves that keep both
-//      parts inside [int(w/1.03), int(w*1.03)] of their target
-//      (cu:3401-3488), destinations ascending; accepted moves are applied on
-//      the device.
+//   4. one wave walks the sorted list on the device and accepts the moves
+//      that keep both parts inside [int(w/1.03), int(w*1.03)] of their
+//      target (cu:3401-3488): first the candidates with from <= to, then
+//      those with from > to, destinations ascending, each destination's
+//      candidates by vertex (k_accept_moves).
 #include <vector>
 
 #include "cm_common.hpp"
 #include "cm_internal.hpp"
 #include "cm_refine.hpp"
-#include "host/cm_host.hpp"
 
 namespace cm {
 
@@ -120,6 +120,79 @@ __global__ __launch_bounds__(256) void k_refine_dest(const int32_t *list, const 
       vw[q] = vwgt[v];
     }
   }
+}
+
+// The serial move walk of one refinement pass, done by one wave.  The
+// candidates, sorted by (destination, vertex), are taken 64 at a time: lane i
+// checks its move against the part weights as they would be if every earlier
+// candidate of the chunk was accepted (their weights summed over the earlier
+// lanes).  Up to the first lane that fails, that assumption holds, so those
+// lanes are accepted and the first failing lane is rejected exactly as in
+// the serial walk; the next chunk starts after it.  A chunk with no
+// rejection advances by 64.  Candidates of the other pass take no part (the
+// two passes, from <= to and from > to, hold disjoint candidates).  Moves
+// within one part (no connectivity beats the own part) leave the weights
+// unchanged and are not applied.  pw_io: part weights in and out; out[0]:
+// moves applied.
+__global__ __launch_bounds__(CM_WAVE) void k_accept_moves(const uint64_t *__restrict__ key,
+                                                          const uint32_t *__restrict__ idx,
+                                                          const int32_t *__restrict__ from,
+                                                          const int32_t *__restrict__ vw, unsigned nc, int nparts,
+                                                          int64_t maxw, int64_t minw, int64_t *pw_io,
+                                                          int32_t *where, unsigned *out)
+{
+  extern __shared__ unsigned long long pw[];  // nparts part weights (two's complement)
+  __shared__ int s_to[CM_WAVE], s_me[CM_WAVE];
+  __shared__ long long s_w[CM_WAVE];
+  const int l = lane();
+  for (int p = l; p < nparts; p += CM_WAVE) pw[p] = (unsigned long long)pw_io[p];
+  __syncthreads();
+  unsigned applied = 0;
+  for (int pass = 0; pass < 2; ++pass) {
+    unsigned q0 = 0;
+    while (q0 < nc) {
+      const unsigned q = q0 + (unsigned)l;
+      bool has = q < nc;
+      int to = 0, me = 0, v = 0;
+      long long w = 0;
+      if (has) {
+        const uint64_t k = key[q];
+        to = (int)(k >> 32);
+        v = (int)(uint32_t)k;
+        const uint32_t i = idx[q];
+        me = from[i];
+        w = vw[i];
+        has = pass == 0 ? me <= to : me > to;
+      }
+      s_to[l] = has ? to : -1;
+      s_me[l] = has ? me : -1;
+      s_w[l] = has ? w : 0;
+      __syncthreads();
+      long long dto = 0, dme = 0;
+      for (int j = 0; j < l; ++j) {
+        const int tj = s_to[j], mj = s_me[j];
+        const long long wj = s_w[j];
+        dto += (tj == to ? wj : 0) - (mj == to ? wj : 0);
+        dme += (tj == me ? wj : 0) - (mj == me ? wj : 0);
+      }
+      const long long a = (long long)pw[to] + dto + w, b = (long long)pw[me] + dme - w;
+      const bool ok = a >= minw && a <= maxw && b >= minw && b <= maxw;
+      const uint64_t fail = __ballot(has && !ok);
+      const int f = fail ? __ffsll((unsigned long long)fail) - 1 : CM_WAVE;
+      const bool apply = has && l < f && me != to;
+      __syncthreads();  // every lane has read the weights
+      if (apply) {
+        atomicAdd(&pw[to], (unsigned long long)w);
+        atomicAdd(&pw[me], (unsigned long long)(-w));
+        where[v] = to;
+      }
+      applied += (unsigned)__popcll(__ballot(apply));
+      __syncthreads();
+      q0 += f < CM_WAVE ? (unsigned)f + 1u : (unsigned)CM_WAVE;
+    }
+  }
+  for (int p = l; p < nparts; p += CM_WAVE) pw_io[p] = (int64_t)pw[p];
+  if (l == 0) out[0] = applied;
 }
 
 __global__ __launch_bounds__(256) void k_apply_moves(int64_t nm, const int32_t *moves, int32_t *where)
@@ -317,28 +390,22 @@ static int refine_impl(cm_ctx *ctx, const Level &L, int nparts, int32_t *where, 
     int64_t hseg[2] = {0, (int64_t)hc};
     CM_HIP(hipMemcpyAsync(segoff, hseg, sizeof(hseg), hipMemcpyHostToDevice, ctx->stream));
     segsort_u64(ctx, 1, segoff, true, hc, key, idx);
-    std::vector<uint64_t> hkey(hc);
-    std::vector<uint32_t> hidx(hc);
-    std::vector<int> hfrom(hc), hvw(hc), sfrom(hc), svw(hc);
-    CM_HIP(hipMemcpyAsync(hkey.data(), key, 8 * hc, hipMemcpyDeviceToHost, ctx->stream));
-    CM_HIP(hipMemcpyAsync(hidx.data(), idx, 4 * hc, hipMemcpyDeviceToHost, ctx->stream));
-    CM_HIP(hipMemcpyAsync(hfrom.data(), from, 4 * hc, hipMemcpyDeviceToHost, ctx->stream));
-    CM_HIP(hipMemcpyAsync(hvw.data(), vw, 4 * hc, hipMemcpyDeviceToHost, ctx->stream));
+    // cu:3401-3412: limits from float targets 1/k and ub 1.03
+    const float tp = (float)(1.0 / nparts);
+    const int64_t maxw = (int)(tp * (float)tvwgt * 1.03f);
+    const int64_t minw = (int)(tp * (float)tvwgt / 1.03f);
+    int64_t *d_pw = (int64_t *)ctx->pool.alloc(sizeof(int64_t) * nparts);
+    CM_HIP(hipMemcpyAsync(d_pw, pw.data(), sizeof(int64_t) * nparts, hipMemcpyHostToDevice, ctx->stream));
+    unsigned *d_out = ctx->d_ctr + 6;
+    launch(ctx, "refine_accept", 0.0, k_accept_moves, dim3(1), dim3(CM_WAVE), sizeof(int64_t) * nparts,
+           (const uint64_t *)key, (const uint32_t *)idx, (const int32_t *)from, (const int32_t *)vw, hc, nparts,
+           maxw, minw, d_pw, where, d_out);
+    unsigned hacc = 0;
+    CM_HIP(hipMemcpyAsync(pw.data(), d_pw, sizeof(int64_t) * nparts, hipMemcpyDeviceToHost, ctx->stream));
+    CM_HIP(hipMemcpyAsync(&hacc, d_out, sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
     sync(ctx);
-    for (unsigned q = 0; q < hc; ++q) {
-      sfrom[q] = hfrom[hidx[q]];
-      svw[q] = hvw[hidx[q]];
-    }
-    std::vector<int> moves;
-    accepted = cmh::schedule_moves(nparts, tvwgt, hkey, sfrom, svw, pw, moves);
-    if (accepted > 0) {
-      int32_t *dm = (int32_t *)ctx->pool.alloc(sizeof(int32_t) * moves.size());
-      CM_HIP(hipMemcpyAsync(dm, moves.data(), 4 * moves.size(), hipMemcpyHostToDevice, ctx->stream));
-      launch(ctx, "refine_apply", 0.0, k_apply_moves, dim3(nblocks(accepted, 256)), dim3(256), 0,
-             (int64_t)accepted, (const int32_t *)dm, where);
-      sync(ctx);
-      ctx->pool.release(dm);
-    }
+    accepted = (int)hacc;
+    ctx->pool.release(d_pw);
     ctx->pool.release(key);
     ctx->pool.release(idx);
     ctx->pool.release(from);

@@ -27,13 +27,4 @@ void write_partition(const std::string &path, const int *part, int n);
 // order.  Returns the summed bisection cuts.
 int initial_partition(const HostGraph &g, int nparts, std::vector<int> &where);
 
-// Move scheduler of one k-way refinement pass (cu:3401-3488), serialised:
-// destinations ascending, candidates of one destination in ascending vertex
-// order, first moves with from <= to, then from > to.  cand is sorted by
-// (to, v); pw holds part weights and is updated; returns accepted count and
-// writes the accepted (v, to) pairs to moves.
-int schedule_moves(int nparts, int64_t tvwgt, const std::vector<uint64_t> &cand,
-                   const std::vector<int> &from, const std::vector<int> &vw,
-                   std::vector<int64_t> &pw, std::vector<int> &moves);
-
 }  // namespace cmh

@@ -753,34 +753,4 @@ int initial_partition(const HostGraph &hg, int nparts, std::vector<int> &where)
   return recursive_bisect(std::move(g), s, nparts, where.data(), tp.data(), 0);
 }
 
-int schedule_moves(int nparts, int64_t tvwgt, const std::vector<uint64_t> &cand,
-                   const std::vector<int> &from, const std::vector<int> &vw,
-                   std::vector<int64_t> &pw, std::vector<int> &moves)
-{
-  // cu:3401-3412: limits from float targets 1/k and ub 1.03
-  const float tp = (float)(1.0 / nparts);
-  const int maxw = (int)(tp * (float)tvwgt * 1.03f);
-  const int minw = (int)(tp * (float)tvwgt / 1.03f);
-  moves.clear();
-  std::vector<char> done(cand.size(), 0);
-  for (int pass = 0; pass < 2; ++pass) {
-    for (size_t q = 0; q < cand.size(); ++q) {
-      const int to = (int)(cand[q] >> 32), v = (int)(uint32_t)cand[q];
-      const int me = from[q];
-      if (done[q] || (pass == 0 ? me > to : me <= to)) continue;
-      const int64_t a = pw[to] + vw[q], b = pw[me] - vw[q];
-      if (a >= minw && a <= maxw && b >= minw && b <= maxw) {
-        if (me != to) {  // a move within one part leaves the weights unchanged
-          pw[to] = a;
-          pw[me] = b;
-          moves.push_back(v);
-          moves.push_back(to);
-        }
-        done[q] = 1;
-      }
-    }
-  }
-  return (int)moves.size() / 2;
-}
-
 }  // namespace cmh

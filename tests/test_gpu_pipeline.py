@@ -131,10 +131,19 @@ def test_batch_rejects_bad_input(engine, fault):
 
 @pytest.mark.parametrize("trials", [1, 4])
 @pytest.mark.parametrize("balance", [True, False])
-@pytest.mark.parametrize("name,k", [("delaunay4720", 8), ("grid100", 16), ("rgg", 4), ("hub", 8)])
+@pytest.mark.parametrize("name,k", [("delaunay4720", 8), ("grid100", 16), ("rgg", 4), ("hub", 8),
+                                    ("grid60vw", 48)])
 def test_partition_bit_exact_with_oracle_pipeline(engine, orc, name, k, balance, trials):
+    """Every refinement pass accepts its moves on the device (one wave, 64
+    candidates a step, restarting after each rejection); grid60vw, with
+    vertex weights 1..60 and 48 parts, makes the part bounds reject many
+    moves."""
     if name == "grid100":
         g = graphs.grid2d(100, 100, wmax=100, seed=2)
+    elif name == "grid60vw":
+        g0 = graphs.grid2d(60, 60, wmax=30, seed=5)
+        g = Graph(g0.xadj, g0.adjncy, g0.adjwgt,
+                  np.random.default_rng(5).integers(1, 61, size=g0.n).astype(np.int32))
     elif name == "rgg":
         g = graphs.rgg2d(8000, 8.0, wmax=16, seed=3)
     elif name == "hub":
