@@ -281,6 +281,15 @@ def batch_leg(eng, count, order, steps, cpu=True):
             eng.lib.cm_hier_release(eng.ctx, h)
 
     release(batched())
+    # the same batch with every round batch replayed from a cached hipGraph
+    old_tuning = eng.set_tuning(graph_rounds=1)
+    release(batched())  # captures
+    gt = []
+    for _ in range(steps):
+        t = time.perf_counter()
+        release(batched())
+        gt.append(time.perf_counter() - t)
+    eng.set_tuning(**old_tuning)
     bt = []
     for _ in range(steps):
         t = time.perf_counter()
@@ -328,7 +337,9 @@ def batch_leg(eng, count, order, steps, cpu=True):
     out = {"graphs": nb, "sum_n": int(voff[-1]), "sum_m": int(eoff[-1]),
            "batched_graphs_per_s": round(nb / med, 1), "batched_ms": round(med * 1e3, 2),
            "one_by_one_graphs_per_s": round(nb / ot, 1), "one_by_one_ms": round(ot * 1e3, 2),
-           "speedup": round(ot / med, 2), "level_shapes_identical": bool(same_shape),
+           "speedup": round(ot / med, 2),
+           "batched_ms_graph_replay": round(float(np.median(gt)) * 1e3, 2),
+           "tuning": eng.tuning(), "level_shapes_identical": bool(same_shape),
            "level_arrays_identical_sample": {"graphs": len(sample), "identical": bool(arrays_equal)},
            "cuts": cuts}
     if cpu:

@@ -44,6 +44,14 @@ class LevelInfo(C.Structure):
                 ("edge_base", C.c_int64), ("coarse_base", C.c_int64)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("first_round_batch", C.c_int32), ("level_batch_margin", C.c_int32),
+                ("large_level_batch", C.c_int32), ("compact_round", C.c_int32),
+                ("compact_min_n", C.c_int64), ("speculate_edges", C.c_int64),
+                ("number_one_tile_max", C.c_int32), ("seg_wave_bins", C.c_int32),
+                ("graph_rounds", C.c_int32), ("reserved", C.c_int32 * 7)]
+
+
 class PartReport(C.Structure):
     _fields_ = [("total_ms", C.c_double), ("coarsen_ms", C.c_double), ("init_ms", C.c_double),
                 ("uncoarsen_ms", C.c_double), ("levels", C.c_int32), ("coarsest_n", C.c_int32),
@@ -61,6 +69,8 @@ _PROTOS = {
     "cm_ctx_reset_stats": (C.c_int, [_P]),
     "cm_ctx_kernel_stat": (C.c_int, [_P, C.c_int, C.c_char_p, C.c_int, C.POINTER(C.c_int64),
                                      C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "cm_ctx_get_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
+    "cm_ctx_set_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
     "cm_ctx_pool_bytes": (C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "cm_dmalloc": (C.c_int, [_P, C.c_size_t, C.POINTER(_P)]),
     "cm_dfree": (C.c_int, [_P, _P]),

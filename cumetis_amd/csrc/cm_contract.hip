@@ -35,9 +35,10 @@ static_assert(REG_SEG == 16, "FIRST_SORTED_BIN must be the first bin above REG_S
 // totals[0] = nc, totals[3] = largest coarse vertex weight, totals[4] =
 // longest segment (a bound on the longest coarse row).
 constexpr int NUM_PER = 2;
-// Levels of at most 1024 * NUM_PER_ONE vertices are numbered by one tile:
-// no look-back at all (each chain step goes through the memory-side
-// coherence point, a few us on the small levels of a hierarchy).
+// Levels of at most tune.number_one_tile_max (<= 1024 * NUM_PER_ONE)
+// vertices are numbered by one tile: no look-back at all (each chain step
+// goes through the memory-side coherence point, a few us on the small
+// levels of a hierarchy).
 constexpr int NUM_PER_ONE = 8;
 
 template <typename OffT, int NUM_PER>
@@ -159,7 +160,7 @@ static int64_t number_impl(cm_ctx *ctx, const Level &L, const int32_t *match_in,
     fill_u32(ctx, ctx->d_small, 0u, 10);  // nc and the maxima (slots 0..4)
     return 0;
   }
-  const bool one = n <= 1024 * NUM_PER_ONE;
+  const bool one = n <= std::min<int64_t>(ctx->tune.number_one_tile_max, 1024 * NUM_PER_ONE);
   const int per = one ? NUM_PER_ONE : NUM_PER;
   const int64_t tiles = (n + 1024 * per - 1) / (1024 * per);
   const uint32_t ep = lookback_begin(ctx, 4 * tiles);  // 4 chains

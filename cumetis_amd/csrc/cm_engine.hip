@@ -10,6 +10,7 @@
 #include "cm_internal.hpp"
 #include "cm_contract_hash.hpp"
 #include "cm_segsort.hpp"
+#include "cm_tuned.hpp"
 
 namespace cm {
 
@@ -35,8 +36,8 @@ void *Pool::alloc(size_t bytes)
   auto it = free_.find(c);
   void *p = nullptr;
   if (it != free_.end() && !it->second.empty()) {
-    p = it->second.back();
-    it->second.pop_back();
+    p = *it->second.begin();
+    it->second.erase(it->second.begin());
   } else {
     hipError_t e = hipMalloc(&p, c);
     if (e != hipSuccess) {
@@ -62,7 +63,7 @@ void Pool::release(void *p)
   auto it = live_.find(p);
   if (it == live_.end()) return;
   in_use_ -= it->second;
-  free_[it->second].push_back(p);
+  free_[it->second].insert(p);
   live_.erase(it);
 }
 
@@ -457,13 +458,11 @@ int ref_threshold(int64_t n, int nparts)
 }
 
 // One coarsening level: match -> number -> contract (cu:1699-1806, 1291-1406).
-// Levels with at most this many entries run their matching rounds without a
-// host check before the contraction (see coarsen_level).
-constexpr int64_t SPECULATE_EDGES = 1 << 24;
-// Launches in the first batch of a larger level.  A launch after the last
-// live round returns at once (4-7 us on such a level), while a second batch
-// costs a host synchronisation (about 40 us) first: R-MAT s22 needs 13.
-constexpr int LARGE_LEVEL_BATCH = 13;
+// Levels with at most tune.speculate_edges entries run their matching rounds
+// without a host check before the contraction (see coarsen_level).  Larger
+// levels queue at least tune.large_level_batch launches first: a launch
+// after the last live round returns at once (4-7 us on such a level), while
+// a second batch costs a host synchronisation (about 40 us) first.
 
 static Level coarsen_level(cm_ctx *ctx, cm_hier *h, Level &F, int32_t maxvwgt, uint64_t seed,
                            int max_rounds, int row_order, int batch)
@@ -497,10 +496,10 @@ static Level coarsen_level(cm_ctx *ctx, cm_hier *h, Level &F, int32_t maxvwgt, u
     // the rare case that the batch did not finish, the rounds continue from
     // their state and the level is redone.  Large levels, where a
     // synchronisation costs little, check the rounds first.
-    const bool speculate = m <= SPECULATE_EDGES;
+    const bool speculate = m <= ctx->tune.speculate_edges;
     if (n > 0) {
       // speculative: flags come with the contraction's read-back
-      match_queue(ctx, mr, speculate ? batch : std::max(batch, LARGE_LEVEL_BATCH), !speculate);
+      match_queue(ctx, mr, speculate ? batch : std::max(batch, (int)ctx->tune.large_level_batch), !speculate);
       if (!speculate) sync(ctx);
     }
     for (int pass = 0;; ++pass) {
@@ -556,6 +555,23 @@ SharedBlocks::~SharedBlocks()
   if (!ctx) return;
   (void)hipStreamSynchronize(ctx->stream);
   for (void *q : blocks) ctx->pool.release(q);
+}
+}  // namespace cm
+
+namespace cm {
+cm_tuning tuned_defaults()
+{
+  cm_tuning t{};
+  t.first_round_batch = TUNED_FIRST_ROUND_BATCH;
+  t.level_batch_margin = TUNED_LEVEL_BATCH_MARGIN;
+  t.large_level_batch = TUNED_LARGE_LEVEL_BATCH;
+  t.compact_round = TUNED_COMPACT_ROUND;
+  t.compact_min_n = TUNED_COMPACT_MIN_N;
+  t.speculate_edges = TUNED_SPECULATE_EDGES;
+  t.number_one_tile_max = TUNED_NUMBER_ONE_TILE_MAX;
+  t.seg_wave_bins = TUNED_SEG_WAVE_BINS;
+  t.graph_rounds = TUNED_GRAPH_ROUNDS;
+  return t;
 }
 }  // namespace cm
 
@@ -628,6 +644,7 @@ int cm_ctx_create(int device, void *stream, cm_ctx **out)
     CM_HIP(hipHostGetDevicePointer((void **)&ctx->h_small_dev, ctx->h_small, 0));
     cm::fill_u32(ctx, ctx->d_ctr, 0u, cm::CTR_SLOTS);
     ctx->noalloc_scratch = cm::noalloc_init(device);
+    ctx->tune = cm::tuned_defaults();
     cm::sync(ctx);
   } catch (const cm::Error &e) {
     int code = e.code;
@@ -648,6 +665,7 @@ int cm_ctx_destroy(cm_ctx *ctx)
     (void)hipEventDestroy(t.b);
   }
   for (auto e : ctx->ev_free) (void)hipEventDestroy(e);
+  for (auto &kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second);
   if (ctx->h_small) (void)hipHostFree(ctx->h_small);
   if (ctx->h_flags) (void)hipHostFree(ctx->h_flags);
   if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
@@ -699,6 +717,32 @@ int cm_ctx_kernel_stat(cm_ctx *ctx, int i, char *name, int name_cap, int64_t *co
   if (count) *count = it->second.count;
   if (total_ms) *total_ms = it->second.total_ms;
   if (algo_bytes) *algo_bytes = it->second.algo_bytes;
+  CM_API_END(ctx)
+}
+
+int cm_ctx_get_tuning(const cm_ctx *ctx, cm_tuning *out)
+{
+  if (!ctx || !out) return CM_ERR_INVALID;
+  *out = ctx->tune;
+  return CM_OK;
+}
+
+int cm_ctx_set_tuning(cm_ctx *ctx, const cm_tuning *t)
+{
+  if (!ctx || !t) return CM_ERR_INVALID;
+  CM_API_BEGIN
+  CM_REQUIRE(t->first_round_batch >= 2 && t->first_round_batch <= 64, CM_ERR_INVALID, "first_round_batch in 2..64");
+  CM_REQUIRE(t->level_batch_margin >= 1 && t->level_batch_margin <= 8, CM_ERR_INVALID, "level_batch_margin in 1..8");
+  CM_REQUIRE(t->large_level_batch >= 2 && t->large_level_batch <= 64, CM_ERR_INVALID, "large_level_batch in 2..64");
+  CM_REQUIRE(t->compact_round >= 2 && t->compact_round <= 32, CM_ERR_INVALID, "compact_round in 2..32");
+  CM_REQUIRE(t->compact_min_n >= 1, CM_ERR_INVALID, "compact_min_n >= 1");
+  CM_REQUIRE(t->speculate_edges >= 0, CM_ERR_INVALID, "speculate_edges >= 0");
+  CM_REQUIRE(t->number_one_tile_max >= 0 && t->number_one_tile_max <= 8192, CM_ERR_INVALID,
+             "number_one_tile_max in 0..8192");
+  CM_REQUIRE(t->seg_wave_bins >= 1 && t->seg_wave_bins <= 6, CM_ERR_INVALID, "seg_wave_bins in 1..6");
+  CM_REQUIRE(t->graph_rounds == 0 || t->graph_rounds == 1, CM_ERR_INVALID, "graph_rounds is 0 or 1");
+  cm::sync(ctx);
+  ctx->tune = *t;
   CM_API_END(ctx)
 }
 
@@ -769,7 +813,7 @@ int cm_coarsen(cm_ctx *ctx, const cm_graph_view *g, const cm_coarsen_params *p, 
     const int T = p->threshold > 0 ? p->threshold : cm::ref_threshold(g->n, p->nparts);
     const int32_t maxvwgt =
         p->maxvwgt > 0 ? p->maxvwgt : (int32_t)(1.5 * (double)h->levels[0].tvwgt / T);
-    int batch = cm::MATCH_BATCH;
+    int batch = ctx->tune.first_round_batch;
     for (;;) {
       cm::Level C = cm::coarsen_level(ctx, h, h->levels.back(), maxvwgt, p->seed, p->max_rounds,
                                       p->row_order, batch);
@@ -777,7 +821,8 @@ int cm_coarsen(cm_ctx *ctx, const cm_graph_view *g, const cm_coarsen_params *p, 
       // took plus one (launch r accepts round r-1, so r rounds need r + 1
       // launches to show that they finished); a short batch only costs a
       // resumption
-      batch = std::min(cm::MATCH_BATCH, std::max(5, h->levels.back().rounds + 2));
+      batch = std::min((int)ctx->tune.first_round_batch,
+                       std::max(5, h->levels.back().rounds + (int)ctx->tune.level_batch_margin));
       const int64_t nf = h->levels.back().n;
       h->levels.push_back(C);
       const bool more = C.n > T && C.n < 0.75 * nf && C.m > C.n / 2;  // cu:1843-1845

@@ -7,7 +7,9 @@
 
 #include <map>
 #include <memory>
+#include <unordered_map>
 #include <mutex>
+#include <set>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -35,7 +37,9 @@ struct Error : std::runtime_error {
 
 // Caching device allocator: power-of-two size classes, never returns memory
 // to the runtime before the context dies, so steady-state coarsening calls
-// allocate nothing.
+// allocate nothing.  A size class hands out its lowest free address, so a
+// call repeated from the same free state gets the same blocks (and its
+// kernels the same arguments, which lets replayed launch graphs match).
 class Pool {
  public:
   ~Pool();
@@ -46,7 +50,7 @@ class Pool {
   void trim();
 
  private:
-  std::map<size_t, std::vector<void *>> free_;
+  std::map<size_t, std::set<void *>> free_;
   std::map<void *, size_t> live_;
   size_t reserved_ = 0, in_use_ = 0;
   std::recursive_mutex mu_;  // hierarchies may be released from another host thread
@@ -139,6 +143,10 @@ struct cm_ctx {
   bool hash_ctr_zero = false;
   // device address of this device's static scratch array (cm_noalloc.hip)
   void *noalloc_scratch = nullptr;
+  // launch-shape parameters (cm_tuning; defaults from cm_tuned.hpp)
+  cm_tuning tune{};
+  // replayable round batches, keyed by every parameter of their launches
+  std::unordered_map<std::string, hipGraphExec_t> graphs;
 
   hipEvent_t ev_get();
   void collect_timing();
@@ -153,6 +161,8 @@ constexpr int SMALL_SLOTS = 4096;
 // Device address of the allocation-free entry point's static scratch array
 // on `device` (which must be current); looked up once per device.
 void *noalloc_init(int device);
+// The launch-shape defaults of a new context (cm_tuned.hpp).
+cm_tuning tuned_defaults();
 // Host-side checks of a graph view (sizes, offset type, NULL arrays).
 void validate_view(const cm_graph_view *g);
 void ensure_status(cm_ctx *ctx, size_t words);
@@ -232,7 +242,7 @@ bool match_done(cm_ctx *ctx, MatchRun *mr);
 int match_finish(cm_ctx *ctx, MatchRun *mr);
 void match_copy_finished(cm_ctx *ctx, MatchRun *mr, int32_t *out);
 void match_end(cm_ctx *ctx, MatchRun *mr);
-constexpr int MATCH_BATCH = 9;  // launches per batch
+constexpr int MATCH_BATCH = 9;  // launches per batch of a resumed matching
 // finish=false leaves unmatched vertices at -1 for number() to complete.
 int match_handshake(cm_ctx *ctx, const Level &L, int32_t maxvwgt, uint64_t seed, int max_rounds,
                     int32_t *d_match, bool finish = true);

@@ -162,7 +162,7 @@ __global__ __launch_bounds__(256) void k_select_long(int64_t n, const OffT *xadj
   }
 }
 
-// The frontier.  After launch COMPACT_ROUND of a large level most vertices
+// The frontier.  After launch tune.compact_round of a large level most vertices
 // are matched, yet every later launch would still visit all n vertices, all
 // class rows and all hub chunks to find that out (its time then goes to
 // dispatch and one load per item).  k_compact lists again what is still
@@ -172,8 +172,9 @@ __global__ __launch_bounds__(256) void k_select_long(int64_t n, const OffT *xadj
 // that skips a matched vertex skips only work that returns at once, so the
 // rounds, flags and matching are those of the full launches.  The lists are
 // in no particular order (a launch's result does not depend on its order).
-constexpr int COMPACT_ROUND = 5;
-constexpr int64_t COMPACT_MIN_N = int64_t(1) << 20;  // smaller levels' rounds cost about a launch
+// The round after which the lists are built and the smallest level that
+// builds them are launch-shape parameters of the context (tune.compact_round,
+// tune.compact_min_n; smaller levels' rounds cost about a launch).
 // Vertex regions: compaction block b lists the unmatched vertices of
 // [b * VREGION, (b + 1) * VREGION) at slist[b * VREGION ..] with their count
 // in vcount[b], and block b of a list launch takes region b.  No atomics: a
@@ -786,8 +787,9 @@ struct MatchRun {
   unsigned *flags = nullptr, *nchunks = nullptr;
   unsigned g = 0, glong = 0;
   unsigned ghub = 0;  // blocks of k_hub_reduce (0: the level has no hub row)
-  // the frontier lists from launch COMPACT_ROUND + 1 on (large levels only)
+  // the frontier lists from launch compact_round + 1 on (large levels only)
   bool compact = false;
+  int compact_round = 0;
   LongChunk *chunks2 = nullptr;
   RowLists rl2{};
   int32_t *slist = nullptr;
@@ -835,7 +837,8 @@ static void match_start(cm_ctx *ctx, MatchRun &mr)
   mr.nchunks = mr.flags + 2 * (size_t)mr.max_rounds;  // [0] hub chunks, [1 + k] rows of class k
   mr.nchunks2 = mr.nchunks + 8;
   static_assert(NCLASS + 1 <= 8, "list counts");
-  mr.compact = n >= COMPACT_MIN_N && mr.max_rounds > COMPACT_ROUND + 1;
+  mr.compact_round = ctx->tune.compact_round;
+  mr.compact = n >= ctx->tune.compact_min_n && mr.max_rounds > mr.compact_round + 1;
   if (mr.compact) {
     mr.slist = (int32_t *)al(sizeof(int32_t) * (size_t)nblocks(n, VREGION) * VREGION);
     mr.vcount = (unsigned *)al(sizeof(unsigned) * nblocks(n, VREGION));
@@ -866,7 +869,7 @@ static void match_start(cm_ctx *ctx, MatchRun &mr)
 // Queues up to `count` more launches (launch r: accept of round r-1,
 // proposals of round r; launch max_rounds only accepts).
 template <typename OffT>
-static void match_launch(cm_ctx *ctx, MatchRun &mr, int count)
+static void match_launch_direct(cm_ctx *ctx, MatchRun &mr, int count)
 {
   const Level &L = *mr.L;
   const int64_t n = L.n;
@@ -875,7 +878,7 @@ static void match_launch(cm_ctx *ctx, MatchRun &mr, int count)
     const int r = mr.r;
     int32_t *cur = mr.prop + (size_t)(r & 1) * n;
     const int32_t *prv = mr.prop + (size_t)((r + 1) & 1) * n;
-    const bool lst = mr.compact && r > COMPACT_ROUND;
+    const bool lst = mr.compact && r > mr.compact_round;
     const LongChunk *chunks = lst ? mr.chunks2 : mr.chunks;
     const unsigned *counts = lst ? mr.nchunks2 : mr.nchunks;
     const unsigned gshort = lst ? mr.gs : mr.g, glong = mr.glong;
@@ -891,11 +894,101 @@ static void match_launch(cm_ctx *ctx, MatchRun &mr, int count)
     if (mr.ghub && r < mr.max_rounds)
       launch(ctx, "match_hub_reduce", 0.0, k_hub_reduce, dim3(mr.ghub), dim3(256), 0, chunks, counts, mr.part,
              cur, (const unsigned *)mr.flags, r);
-    if (mr.compact && r == COMPACT_ROUND)
+    if (mr.compact && r == mr.compact_round)
       launch(ctx, "match_compact", 4.0 * n, k_compact, dim3(mr.gs + (1 + NCLASS) * LIST_BLOCKS), dim3(256), 0, n,
              (const int32_t *)mr.match, (const unsigned *)mr.flags, r, (const LongChunk *)mr.chunks, mr.rl,
              (const unsigned *)mr.nchunks, mr.chunks2, mr.rl2, mr.nchunks2, mr.slist, mr.vcount, mr.gs);
   }
+}
+
+// Every value the launches of match_launch_direct(count) take, as bytes: two
+// batches with equal keys launch the same kernels with the same arguments.
+template <typename OffT>
+static std::string batch_key(const MatchRun &mr, int count)
+{
+  const Level &L = *mr.L;
+  std::string k;
+  auto put = [&](const auto &v) { k.append((const char *)&v, sizeof(v)); };
+  put(sizeof(OffT));
+  put(L.n);
+  put(L.m);
+  put(L.xadj);
+  put(L.adjncy);
+  put(L.adjwgt);
+  put(L.vwgt);
+  put(L.maxvw);
+  put(mr.own);
+  put(mr.max_rounds);
+  put(mr.match);
+  put(mr.prop);
+  put(mr.chunks);
+  put(mr.rl);
+  put(mr.short_deg);
+  put(mr.part);
+  put(mr.flags);
+  put(mr.nchunks);
+  put(mr.g);
+  put(mr.glong);
+  put(mr.ghub);
+  put(mr.compact);
+  put(mr.compact_round);
+  put(mr.chunks2);
+  put(mr.rl2);
+  put(mr.slist);
+  put(mr.vcount);
+  put(mr.nchunks2);
+  put(mr.gs);
+  put(mr.r);
+  put(count);
+  return k;
+}
+
+// A batch of round launches: directly, or (tune.graph_rounds, no per-kernel
+// timing) as a replay of the hipGraph captured the first time a batch with
+// the same key was queued.  A replay runs the same kernels with the same
+// arguments over the buffers' current contents, and a graph of dependent
+// nodes starts each kernel sooner than separate launches do.
+constexpr size_t GRAPH_CACHE_MAX = 256;
+
+template <typename OffT>
+static void match_launch(cm_ctx *ctx, MatchRun &mr, int count)
+{
+  if (!ctx->tune.graph_rounds || ctx->timing || count < 2) {
+    match_launch_direct<OffT>(ctx, mr, count);
+    return;
+  }
+  const int r0 = mr.r;
+  const std::string key = batch_key<OffT>(mr, count);
+  auto it = ctx->graphs.find(key);
+  if (it == ctx->graphs.end()) {
+    if (ctx->graphs.size() >= GRAPH_CACHE_MAX) {
+      sync(ctx);  // no replay may still be queued
+      for (auto &kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second);
+      ctx->graphs.clear();
+    }
+    hipGraph_t g = nullptr;
+    CM_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    try {
+      match_launch_direct<OffT>(ctx, mr, count);
+    } catch (...) {
+      (void)hipStreamEndCapture(ctx->stream, &g);
+      if (g) (void)hipGraphDestroy(g);
+      mr.r = r0;
+      throw;
+    }
+    CM_HIP(hipStreamEndCapture(ctx->stream, &g));
+    hipGraphExec_t ge = nullptr;
+    const hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) throw Error(CM_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    it = ctx->graphs.emplace(key, ge).first;
+    ctx->stats["round_graph_captures"].count += 1;
+  } else {
+    ctx->stats["round_graph_replays"].count += 1;
+    // the launches are in the graph: advance the run as they would have
+    for (int i = 0; i < count && mr.r <= mr.max_rounds; ++i) ++mr.r;
+  }
+  CM_HIP(hipGraphLaunch(it->second, ctx->stream));
 }
 
 // Queues the flags of every accepted round (0 .. r-2: round r-2 is accepted

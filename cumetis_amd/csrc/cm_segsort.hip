@@ -337,11 +337,18 @@ static void sort_impl(cm_ctx *ctx, const SegBins &b, const OffT *segoff, int64_t
                       uint64_t *keys, uint32_t *vals, int first_bin, SegArena *ar)
 {
   const int32_t *L = b.list;
+  // bins from tune.seg_wave_bins up to the last wave bin go to the 128-key
+  // LDS kernel instead (same result, other launch shape)
+  const int wave_bins = ctx->tune.seg_wave_bins;
 #define CM_WAVE_BIN(CLS, W)                                                                       \
   {                                                                                               \
     const int64_t c = CLS >= first_bin ? b.off[CLS + 1] - b.off[CLS] : 0;                         \
-    if (c) launch(ctx, "segsort_wave", 16.0 * c * W, k_sort_wave<W, OffT>,                        \
-                  dim3(nblocks(c * W, 256)), dim3(256), 0, L + b.off[CLS], c, segoff, keys, vals); \
+    if (c && CLS < wave_bins)                                                                     \
+      launch(ctx, "segsort_wave", 16.0 * c * W, k_sort_wave<W, OffT>, dim3(nblocks(c * W, 256)),  \
+             dim3(256), 0, L + b.off[CLS], c, segoff, keys, vals);                                \
+    else if (c)                                                                                   \
+      launch(ctx, "segsort_lds", 0.0, k_sort_lds<128, OffT>, dim3(cap_grid(c)), dim3(256), 0,     \
+             L + b.off[CLS], c, segoff, keys, vals);                                              \
   }
   CM_WAVE_BIN(0, 2) CM_WAVE_BIN(1, 4) CM_WAVE_BIN(2, 8) CM_WAVE_BIN(3, 16) CM_WAVE_BIN(4, 32)
   CM_WAVE_BIN(5, 64)

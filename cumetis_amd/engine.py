@@ -188,6 +188,21 @@ class Engine:
             i += 1
         return out
 
+    def tuning(self) -> dict:
+        t = L.Tuning()
+        self._check(self.lib.cm_ctx_get_tuning(self.ctx, C.byref(t)))
+        return {f: getattr(t, f) for f, _ in L.Tuning._fields_ if f != "reserved"}
+
+    def set_tuning(self, **kw):
+        """Change launch-shape parameters (cm_tuning); returns the old values."""
+        old = self.tuning()
+        t = L.Tuning()
+        self._check(self.lib.cm_ctx_get_tuning(self.ctx, C.byref(t)))
+        for k, v in kw.items():
+            setattr(t, k, v)
+        self._check(self.lib.cm_ctx_set_tuning(self.ctx, C.byref(t)))
+        return old
+
     def pool_bytes(self):
         r, u = C.c_int64(), C.c_int64()
         self._check(self.lib.cm_ctx_pool_bytes(self.ctx, C.byref(r), C.byref(u)))

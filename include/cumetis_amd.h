@@ -130,6 +130,29 @@ int cm_ctx_kernel_stat(cm_ctx *ctx, int i, char *name, int name_cap, int64_t *co
                        double *total_ms, double *algo_bytes);
 int cm_ctx_pool_bytes(cm_ctx *ctx, int64_t *reserved, int64_t *in_use);
 
+/* Launch-shape parameters of a context.  None of them changes a result:
+ * they decide how many round launches are queued before a host check, when
+ * the matching switches to its frontier lists, which levels are numbered by
+ * one tile, which segment lengths the segmented sort keeps in registers,
+ * and whether round batches run as replayed hipGraphs.  A new context takes
+ * the values tuned for gfx950 (cumetis_amd/tune.py writes them into the
+ * build); set them to run the tuner's candidates. */
+typedef struct {
+  int32_t first_round_batch;   /* round launches queued at level 0 (2..64) */
+  int32_t level_batch_margin;  /* later levels: previous rounds + this (1..8) */
+  int32_t large_level_batch;   /* at least this many on levels above speculate_edges (2..64) */
+  int32_t compact_round;       /* frontier lists after this launch (2..32) */
+  int64_t compact_min_n;       /* ... on levels with at least this many vertices */
+  int64_t speculate_edges;     /* levels up to this many entries contract before their round check */
+  int32_t number_one_tile_max; /* levels up to this many vertices (0..8192) numbered by one tile */
+  int32_t seg_wave_bins;       /* segment-length bins 0..this-1 (2..2^this keys) sorted in registers (1..6) */
+  int32_t graph_rounds;        /* 1: round batches replay cached hipGraphs */
+  int32_t reserved[7];
+} cm_tuning;
+int cm_ctx_get_tuning(const cm_ctx *ctx, cm_tuning *out);
+/* CM_ERR_INVALID if a value is out of range (the context keeps its values). */
+int cm_ctx_set_tuning(cm_ctx *ctx, const cm_tuning *t);
+
 /* Device memory helpers for callers without an allocator of their own. */
 int cm_dmalloc(cm_ctx *ctx, size_t bytes, void **out);
 int cm_dfree(cm_ctx *ctx, void *p);
