@@ -102,7 +102,9 @@ typedef struct {
   int64_t n;
   int64_t m;
   int32_t offset_type;
-  int32_t rounds; /* matching rounds spent on this level (0 on the coarsest) */
+  int32_t rounds; /* matching rounds spent on this level (0 on the coarsest;
+                     for a batched hierarchy, the rounds of the union step,
+                     which ends when its slowest graph has finished) */
   void *xadj;
   int32_t *adjncy;
   int32_t *adjwgt;

@@ -60,6 +60,59 @@ def test_batch_equals_alone(engine, order):
         h.release()
 
 
+def test_batch_levels_equal_oracle(engine, orc):
+    """Batched hierarchies against the oracle directly (not only against the
+    same graphs coarsened alone on the GPU)."""
+    gs = batch_graphs()
+    ks = [8, 4, 2, 2, 2, 16, 8, 4, 4, 32]
+    seeds = list(range(3, 3 + len(gs)))
+    say(f"[batch vs oracle] graphs={len(gs)}")
+    hs = engine.coarsen_batch(gs, ks, seed=seeds)
+    for b, g in enumerate(gs):
+        if g.n == 0:
+            continue
+        ref = orc.coarsen(g.xadj, g.adjncy, g.adjwgt, g.vwgt, ks[b], matcher="handshake", seed=seeds[b])
+        assert hs[b].num_levels == len(ref), b
+        for i, R in enumerate(ref):
+            L = hs[b].level(i)
+            np.testing.assert_array_equal(L.xadj.astype(np.int64), R["xadj"])
+            np.testing.assert_array_equal(L.adjncy, R["adjncy"])
+            np.testing.assert_array_equal(L.adjwgt, R["adjwgt"])
+            np.testing.assert_array_equal(L.vwgt, R["vwgt"])
+            if i < len(ref) - 1:
+                np.testing.assert_array_equal(L.match, R["match"])
+                np.testing.assert_array_equal(L.cmap, R["cmap"])
+    for h in hs:
+        h.release()
+
+
+def test_batch512_equals_alone(engine):
+    """The benchmark's batch: 512 graphs of the configured distribution (60%
+    RGG with n log-uniform in [1k, 200k], 40% grids with side U{32..400},
+    adjwgt U[1,16], k U{2..64}) in one batched coarsening; every graph's
+    level shapes equal its coarsening alone, and every level array of every
+    8th graph.  (A batched level reports the rounds of its union step, which
+    runs until the slowest graph of the union has finished.)"""
+    import bench
+    gs, ks = bench.batch_graphs(512)
+    say(f"[batch512 vs alone] sum n={sum(g.n for g in gs)}")
+    seeds = [1000 + b for b in range(len(gs))]
+    hs = engine.coarsen_batch(gs, ks, seed=seeds)
+    for b, g in enumerate(gs):
+        alone = engine.coarsen(g, ks[b], seed=seeds[b], on_device=True)
+        if b % 8 == 0:
+            levels_equal(hs[b], alone)
+        else:
+            assert hs[b].num_levels == alone.num_levels
+            for i in range(alone.num_levels):
+                x, y = hs[b].info(i), alone.info(i)
+                assert (x.n, x.m) == (y.n, y.m), (b, i)
+                assert x.rounds >= y.rounds, (b, i)
+        alone.release()
+    for h in hs:
+        h.release()
+
+
 def test_batch_device_input_and_int64(engine):
     gs = batch_graphs()[:6]
     cg, voff, eoff = concat_graphs(gs)

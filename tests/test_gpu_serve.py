@@ -73,3 +73,23 @@ def test_server_rejects_bad_input(engine):
         t = srv.submit(g, 2)
         part, rep = srv.wait(t)
         assert len(part) == 3
+
+
+def test_server_mix_with_large_job(engine):
+    """The serving configuration's large job, a 160^3 7-point grid (4,096,000
+    vertices, k=128), submitted among small jobs of the batch distribution:
+    every result equals the same job partitioned alone."""
+    big = graphs.grid3d(160, 160, 160)
+    jobs = [(with_weights(graphs.rgg2d(3000, 8.0, wmax=16, seed=21)), 8), (big, 128),
+            (with_weights(graphs.grid2d(90, 70, wmax=16, seed=22)), 16),
+            (with_weights(graphs.rgg2d(12000, 8.0, wmax=16, seed=23)), 32)]
+    say(f"[serve] large job n={big.n} k=128 among {len(jobs) - 1} small ones")
+    alone = [engine.partition(g, k, seed=1) for g, k in jobs]
+    with Server(0, max_slots=16) as srv:
+        tickets = [srv.submit(g, k, seed=1) for g, k in jobs]
+        res = [srv.wait(t) for t in tickets]
+    for (part, rep), (apart, arep) in zip(res, alone):
+        assert np.array_equal(part, apart)
+        assert rep.edgecut == arep.edgecut
+    pw = np.bincount(res[1][0], minlength=128)
+    assert pw.max() * 128 / big.n <= 1.03 + 1e-6
