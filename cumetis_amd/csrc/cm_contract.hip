@@ -25,21 +25,25 @@ constexpr int FIRST_SORTED_BIN = 4;  // segsort bins from REG_SEG + 1 = 17 keys 
 static_assert(REG_SEG == 16, "FIRST_SORTED_BIN must be the first bin above REG_SEG");
 
 // ------------------------------------------------------------- numbering
-// NUM_PER consecutive vertices per thread, tiles of 1024 * NUM_PER vertices
-// (fewer tiles = a shorter look-back chain).  A vertex still unmatched
-// (match = -1, the matching left unfinished) becomes its own coarse vertex
-// and its match entry is completed here; a representative also gives its
-// partner the coarse id.  Four look-back chains: coarse count and segment
-// length (sums, walked by wave 0), largest coarse vertex weight and longest
-// segment (maxima, walked by wave 1); the last tile stores the totals:
-// totals[0] = nc, totals[3] = largest coarse vertex weight, totals[4] =
-// longest segment (a bound on the longest coarse row).
+// Tiles of 1024 * NUM_PER vertices; thread x of a tile takes the vertices
+// tile_base + t * 1024 + x (t < NUM_PER), so that every load and store of a
+// per-vertex array by a wave covers 64 consecutive entries (fewer tiles =
+// a shorter look-back chain).  A vertex still unmatched (match = -1, the
+// matching left unfinished) becomes its own coarse vertex and its match
+// entry is completed here; a representative also gives its partner the
+// coarse id.  Coarse count and segment length are scanned together as one
+// word (length << 16 | count; a tile counts at most 8192 vertices).  Four
+// look-back chains: coarse count and segment length (sums, walked by wave
+// 0), largest coarse vertex weight and longest segment (maxima, walked by
+// wave 1); the last tile stores the totals: totals[0] = nc, totals[3] =
+// largest coarse vertex weight, totals[4] = longest segment (a bound on the
+// longest coarse row).
 constexpr int NUM_PER = 2;
 // Levels of at most tune.number_one_tile_max (<= 1024 * NUM_PER_ONE)
 // vertices are numbered by one tile: no look-back at all (each chain step
 // goes through the memory-side coherence point, a few us on the small
 // levels of a hierarchy).
-constexpr int NUM_PER_ONE = 8;
+constexpr int NUM_PER_ONE = 4;  // 8 would spill: 1024-thread tiles have 128 VGPRs
 
 template <typename OffT, int NUM_PER>
 __global__ __launch_bounds__(1024) void k_number(int64_t n, const OffT *__restrict__ xadj,
@@ -50,61 +54,85 @@ __global__ __launch_bounds__(1024) void k_number(int64_t n, const OffT *__restri
                                                  uint64_t *st_c, uint64_t *st_l, uint64_t *st_m, uint64_t *st_n,
                                                  unsigned *ctr, int64_t *totals, HcLists sel, uint32_t epoch)
 {
-  __shared__ uint64_t sh_c[16], sh_l[16], sh_m[16], sh_n[16];
+  static_assert(1024 * NUM_PER < (1 << 16), "a tile's coarse count fits the low 16 bits of the scan word");
+  constexpr int NW = 1024 / CM_WAVE;  // waves per tile
+  constexpr int NE = NW * NUM_PER;    // (stripe, wave) slots, in vertex order
+  __shared__ uint64_t sh_x[NE], sh_m[NW], sh_n[NW], sh_t[2];
   const int tile = tile_ticket(ctr);
-  const int64_t i0 = ((int64_t)tile * 1024 + threadIdx.x) * NUM_PER;
-  uint64_t fl[NUM_PER], ln[NUM_PER], flag = 0, len = 0, mw = 0, ml = 0;
+  const int64_t i0 = (int64_t)tile * 1024 * NUM_PER + threadIdx.x;
+  const int w = threadIdx.x >> 6, l = lane();
+  uint64_t xin[NUM_PER], mw = 0, ml = 0;
+  uint32_t ln[NUM_PER];
   int32_t uu[NUM_PER], cw[NUM_PER], u0[NUM_PER];
+  bool fl[NUM_PER];
   // two dependency levels (match, then rows and weights of both ends), each
   // loaded unconditionally through indices clamped to n - 1
 #pragma unroll
   for (int t = 0; t < NUM_PER; ++t) {
-    const int64_t ic = i0 + t < n ? i0 + t : n - 1;
-    u0[t] = match_in[ic];
+    const int64_t i = i0 + (int64_t)t * 1024;
+    u0[t] = match_in[i < n ? i : n - 1];
   }
 #pragma unroll
   for (int t = 0; t < NUM_PER; ++t) {
-    const int64_t i = i0 + t;
+    const int64_t i = i0 + (int64_t)t * 1024;
     const int64_t ic = i < n ? i : n - 1;
     const int32_t u = u0[t] < 0 ? (int32_t)ic : u0[t];
     const OffT xa = xadj[ic], xb = xadj[ic + 1], ya = xadj[u], yb = xadj[u + 1];
     const int32_t wa = vwgt[ic], wb = vwgt[u];
     uu[t] = u;
-    fl[t] = (i < n && i <= u) ? 1 : 0;
-    ln[t] = fl[t] ? (uint64_t)(xb - xa) + (u != i ? (uint64_t)(yb - ya) : 0) : 0;
+    fl[t] = i < n && i <= u;
+    ln[t] = fl[t] ? (uint32_t)(xb - xa) + (u != i ? (uint32_t)(yb - ya) : 0u) : 0u;
     cw[t] = fl[t] ? wa + (u != i ? wb : 0) : 0;
     if (i < n && (u0[t] < 0 || match_in != match)) match[i] = u;
     if (fl[t]) {
       mw = (uint64_t)(uint32_t)cw[t] > mw ? (uint64_t)(uint32_t)cw[t] : mw;
       ml = ln[t] > ml ? ln[t] : ml;
     }
-    flag += fl[t];
-    len += ln[t];
   }
-  const int w = threadIdx.x >> 6, l = lane();
-  const uint64_t ic = wave_incl_scan<uint64_t>(flag), il = wave_incl_scan<uint64_t>(len);
+  // per stripe: the wave-inclusive scan of (length << 16 | count)
+#pragma unroll
+  for (int t = 0; t < NUM_PER; ++t) {
+    const uint64_t x = ((uint64_t)ln[t] << 16) | (fl[t] ? 1u : 0u);
+    const uint64_t inc = wave_incl_scan<uint64_t>(x);
+    xin[t] = inc - x;
+    if (l == CM_WAVE - 1) sh_x[t * NW + w] = inc;
+  }
   mw = wave_max<uint64_t>(mw);
   ml = wave_max<uint64_t>(ml);
   if (l == CM_WAVE - 1) {
-    sh_c[w] = ic;
-    sh_l[w] = il;
     sh_m[w] = mw;
     sh_n[w] = ml;
   }
   __syncthreads();
   if (w == 0) {
-    const uint64_t a = l < 16 ? sh_c[l] : 0, b = l < 16 ? sh_l[l] : 0;
-    const uint64_t ia = wave_incl_scan<uint64_t>(a), ib = wave_incl_scan<uint64_t>(b);
-    const uint64_t ta = __shfl(ia, 15, CM_WAVE), tb = __shfl(ib, 15, CM_WAVE);
-    const uint64_t ea = lookback(st_c, tile, ta, epoch);
-    const uint64_t eb = lookback(st_l, tile, tb, epoch);
-    if (l < 16) {
-      sh_c[l] = ea + ia - a;
-      sh_l[l] = eb + ib - b;
+    // exclusive prefix over the NE slots (stripe-major, then wave: the
+    // vertex order), EPL consecutive slots per lane
+    constexpr int EPL = (NE + CM_WAVE - 1) / CM_WAVE;
+    uint64_t s[EPL], sum = 0;
+#pragma unroll
+    for (int e = 0; e < EPL; ++e) {
+      const int j = l * EPL + e;
+      s[e] = j < NE ? sh_x[j] : 0;
+      sum += s[e];
+    }
+    const uint64_t inc = wave_incl_scan<uint64_t>(sum);
+    const uint64_t tot = __shfl(inc, CM_WAVE - 1, CM_WAVE);
+    const uint64_t ec = lookback(st_c, tile, tot & 0xFFFFu, epoch);
+    const uint64_t el = lookback(st_l, tile, tot >> 16, epoch);
+    uint64_t run = inc - sum;
+#pragma unroll
+    for (int e = 0; e < EPL; ++e) {
+      const int j = l * EPL + e;
+      if (j < NE) sh_x[j] = run;
+      run += s[e];
+    }
+    if (l == 0) {
+      sh_t[0] = ec;
+      sh_t[1] = el;
     }
   } else if (w == 1) {
-    const uint64_t bm = wave_reduce<LbMax>(l < 16 ? sh_m[l] : 0);
-    const uint64_t bn = wave_reduce<LbMax>(l < 16 ? sh_n[l] : 0);
+    const uint64_t bm = wave_reduce<LbMax>(l < NW ? sh_m[l] : 0);
+    const uint64_t bn = wave_reduce<LbMax>(l < NW ? sh_n[l] : 0);
     const uint64_t em = lookback<LbMax>(st_m, tile, bm, epoch);
     const uint64_t en = lookback<LbMax>(st_n, tile, bn, epoch);
     if (l == 0 && (int64_t)(tile + 1) * 1024 * NUM_PER >= n) {  // the last tile
@@ -113,12 +141,12 @@ __global__ __launch_bounds__(1024) void k_number(int64_t n, const OffT *__restri
     }
   }
   __syncthreads();
-  uint64_t c = sh_c[w] + ic - flag, off = sh_l[w] + il - len;
-  int32_t cid[NUM_PER];
+  const uint64_t ec = sh_t[0], el = sh_t[1];
 #pragma unroll
   for (int t = 0; t < NUM_PER; ++t) {
-    const int64_t i = i0 + t;
-    cid[t] = (int32_t)c;
+    const int64_t i = i0 + (int64_t)t * 1024;
+    const uint64_t ex = sh_x[t * NW + w] + xin[t];
+    const uint64_t c = ec + (ex & 0xFFFFu), off = el + (ex >> 16);
     if (i < n) {
       if (fl[t]) {
         const int32_t u = uu[t];
@@ -129,12 +157,11 @@ __global__ __launch_bounds__(1024) void k_number(int64_t n, const OffT *__restri
         cvwgt[c] = cw[t];
       }
       if (i == n - 1) {
-        totals[0] = (int64_t)(c + fl[t]);
-        segoff[c + fl[t]] = (OffT)(off + ln[t]);
+        totals[0] = (int64_t)(c + (fl[t] ? 1 : 0));
+        segoff[c + (fl[t] ? 1 : 0)] = (OffT)(off + ln[t]);
       }
     }
-    c += fl[t];
-    off += ln[t];
+    uu[t] = (int32_t)c;  // from here on: the coarse id
   }
   // the hash contraction's select step: segments above REG_SEG into the
   // item lists (all threads take part in the block-wide reservation)
@@ -142,11 +169,11 @@ __global__ __launch_bounds__(1024) void k_number(int64_t n, const OffT *__restri
     unsigned cnt[HC_N_COUNTERS] = {}, at[HC_N_COUNTERS];
 #pragma unroll
     for (int t = 0; t < NUM_PER; ++t)
-      if (i0 + t < n && fl[t]) hc_count((int64_t)ln[t], cnt);
+      if (i0 + (int64_t)t * 1024 < n && fl[t]) hc_count((int64_t)ln[t], cnt);
     hc_reserve<16>(cnt, at, sel.ctr);
 #pragma unroll
     for (int t = 0; t < NUM_PER; ++t)
-      if (i0 + t < n && fl[t]) hc_write(cid[t], (int64_t)ln[t], at, sel);
+      if (i0 + (int64_t)t * 1024 < n && fl[t]) hc_write(uu[t], (int64_t)ln[t], at, sel);
   }
 }
 

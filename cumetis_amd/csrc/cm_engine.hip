@@ -737,8 +737,8 @@ int cm_ctx_set_tuning(cm_ctx *ctx, const cm_tuning *t)
   CM_REQUIRE(t->compact_round >= 2 && t->compact_round <= 32, CM_ERR_INVALID, "compact_round in 2..32");
   CM_REQUIRE(t->compact_min_n >= 1, CM_ERR_INVALID, "compact_min_n >= 1");
   CM_REQUIRE(t->speculate_edges >= 0, CM_ERR_INVALID, "speculate_edges >= 0");
-  CM_REQUIRE(t->number_one_tile_max >= 0 && t->number_one_tile_max <= 8192, CM_ERR_INVALID,
-             "number_one_tile_max in 0..8192");
+  CM_REQUIRE(t->number_one_tile_max >= 0 && t->number_one_tile_max <= 4096, CM_ERR_INVALID,
+             "number_one_tile_max in 0..4096");
   CM_REQUIRE(t->seg_wave_bins >= 1 && t->seg_wave_bins <= 6, CM_ERR_INVALID, "seg_wave_bins in 1..6");
   CM_REQUIRE(t->graph_rounds == 0 || t->graph_rounds == 1, CM_ERR_INVALID, "graph_rounds is 0 or 1");
   cm::sync(ctx);

@@ -12,7 +12,7 @@ constexpr int32_t TUNED_LARGE_LEVEL_BATCH = 13;
 constexpr int32_t TUNED_COMPACT_ROUND = 5;
 constexpr int64_t TUNED_COMPACT_MIN_N = int64_t(1048576);
 constexpr int64_t TUNED_SPECULATE_EDGES = int64_t(16777216);
-constexpr int32_t TUNED_NUMBER_ONE_TILE_MAX = 8192;
+constexpr int32_t TUNED_NUMBER_ONE_TILE_MAX = 4096;
 constexpr int32_t TUNED_SEG_WAVE_BINS = 6;
 constexpr int32_t TUNED_GRAPH_ROUNDS = 0;
 }  // namespace cm

@@ -42,7 +42,7 @@ CANDIDATES = {
     "compact_round": [4, 5, 6],
     "compact_min_n": [1 << 18, 1 << 19, 1 << 20, 1 << 21],
     "speculate_edges": [1 << 22, 1 << 23, 1 << 24, 1 << 25],
-    "number_one_tile_max": [0, 2048, 4096, 8192],
+    "number_one_tile_max": [0, 1024, 2048, 4096],
     "seg_wave_bins": [4, 5, 6],
 }
 # which workloads each parameter can affect

@@ -146,7 +146,7 @@ typedef struct {
   int32_t compact_round;       /* frontier lists after this launch (2..32) */
   int64_t compact_min_n;       /* ... on levels with at least this many vertices */
   int64_t speculate_edges;     /* levels up to this many entries contract before their round check */
-  int32_t number_one_tile_max; /* levels up to this many vertices (0..8192) numbered by one tile */
+  int32_t number_one_tile_max; /* levels up to this many vertices (0..4096) numbered by one tile */
   int32_t seg_wave_bins;       /* segment-length bins 0..this-1 (2..2^this keys) sorted in registers (1..6) */
   int32_t graph_rounds;        /* 1: round batches replay cached hipGraphs */
   int32_t reserved[7];

@@ -42,6 +42,37 @@ def test_number(engine, orc, name):
     np.testing.assert_array_equal(cmap, ecmap)
 
 
+def _random_matching(rng, n, frac):
+    """A symmetric matching of n vertices: random disjoint pairs over a
+    fraction of them, the rest matched to themselves."""
+    match = np.arange(n, dtype=np.int32)
+    perm = rng.permutation(n)
+    k = int(n * frac) // 2
+    a, b = perm[:k], perm[k:2 * k]
+    match[a], match[b] = b, a
+    return match
+
+
+@pytest.mark.parametrize("one_tile", [0, 4096])
+@pytest.mark.parametrize("n", [1, 63, 1023, 1024, 1025, 2047, 2048, 2049, 4095, 4096, 4097, 6143, 8193, 70001])
+def test_number_tile_edges(engine, orc, n, one_tile):
+    """Numbering across the edges of its tiles (2048 vertices, and 4096 for
+    a level numbered by one tile): cmap and nc equal the oracle's."""
+    rng = np.random.default_rng(n)
+    match = _random_matching(rng, n, 0.8)
+    deg = rng.integers(0, 9, size=n)
+    xadj = np.concatenate([[0], np.cumsum(deg)]).astype(np.int32)
+    say(f"[number tiles] n={n} one_tile={one_tile}")
+    old = engine.set_tuning(number_one_tile_max=one_tile)
+    try:
+        cmap, nc = engine.number(xadj, match)
+    finally:
+        engine.set_tuning(**old)
+    ecmap, enc = orc.number(match)
+    assert nc == enc
+    np.testing.assert_array_equal(cmap, ecmap)
+
+
 @pytest.mark.parametrize("order", [CM_ROW_FIRST_SEEN, CM_ROW_ASCENDING])
 @pytest.mark.parametrize("matcher", ["cpu", "handshake"])
 @pytest.mark.parametrize("name", list(CASES))
