@@ -217,7 +217,8 @@ __device__ __forceinline__ void store_host(T *p, T v)
 
 // Called by all lanes of one wave once every value it copies is final (src
 // is the device copy of the slots): lane i stores word i (the slots, then the
-// flags, then the overflow word), so the stores leave together.
+// flags, then the overflow word), so the stores leave together; the tag
+// follows them (a host that sees the tag may read every word).
 __device__ __forceinline__ void readback_write(const Readback &rb, const int64_t *src, int own_slot, int64_t own)
 {
   if (!rb.dst) return;
@@ -233,7 +234,8 @@ __device__ __forceinline__ void readback_write(const Readback &rb, const int64_t
       store_host(&rb.dst[6], (int64_t)*rb.ovf);
     }
   }
-  if (lane() == 0) store_host(&rb.dst[RB_TAG_SLOT], rb.tag);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");  // system scope, the whole wave
+  if (lane() == 0) __hip_atomic_store(&rb.dst[RB_TAG_SLOT], rb.tag, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 // ------------------------------------------------------------- edge order

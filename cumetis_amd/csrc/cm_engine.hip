@@ -3,6 +3,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <vector>
 
@@ -175,6 +176,20 @@ Readback readback_begin(cm_ctx *ctx, int first, int count, const unsigned *ovf)
 
 void readback_end(cm_ctx *ctx, const Readback &rb)
 {
+  if (ctx->rb_early && !ctx->timing) {
+    // The kernel stores the tag after every other word (a system-scope
+    // release), so the words are final once the tag is here: the host goes
+    // on without waiting for the stream's completion signal.  A stream that
+    // completes without sending the tag falls through to the copy below.
+    const volatile int64_t *tag = ctx->h_small + RB_TAG_SLOT;
+    for (unsigned spin = 1;; ++spin) {
+      if (*tag == rb.tag) {
+        std::atomic_thread_fence(std::memory_order_acquire);
+        return;
+      }
+      if ((spin & 255) == 0 && hipStreamQuery(ctx->stream) == hipSuccess) break;
+    }
+  }
   sync(ctx);
   // the stream's completion makes the stores visible; a stale tag would
   // mean they were not, so the words are then copied the ordinary way
@@ -814,6 +829,12 @@ int cm_coarsen(cm_ctx *ctx, const cm_graph_view *g, const cm_coarsen_params *p, 
     const int32_t maxvwgt =
         p->maxvwgt > 0 ? p->maxvwgt : (int32_t)(1.5 * (double)h->levels[0].tvwgt / T);
     int batch = ctx->tune.first_round_batch;
+    // levels go on from their read-back; the sync below ends the call
+    struct Early {
+      cm_ctx *ctx;
+      Early(cm_ctx *c) : ctx(c) { ctx->rb_early = true; }
+      ~Early() { ctx->rb_early = false; }
+    } early(ctx);
     for (;;) {
       cm::Level C = cm::coarsen_level(ctx, h, h->levels.back(), maxvwgt, p->seed, p->max_rounds,
                                       p->row_order, batch);

@@ -130,6 +130,10 @@ struct cm_ctx {
   const unsigned *rb_flags = nullptr;
   int64_t rb_nflags = 0;
   int64_t rb_tag = 0;
+  // rb_early: readback_end returns as soon as the read-back has arrived in
+  // host memory, while the stream may still run (the caller queues only
+  // stream-ordered work after it and synchronises before it returns)
+  bool rb_early = false;
   // unit weights shared by every graph given without weights (never
   // written; earlier, shorter blocks stay valid until the context dies)
   int32_t *ones = nullptr;
@@ -199,7 +203,8 @@ void sync(cm_ctx *ctx);
 // The read-back of a contraction (see Readback): d_small[first, first +
 // count) to h_small, *ovf (if given) to h_small[6], and the registered round
 // flags (ctx->rb_flags, consumed) to h_flags.  begin() describes it for the
-// kernel that sends it; end() synchronises and checks that it arrived.
+// kernel that sends it; end() waits for it (for the stream, unless
+// ctx->rb_early) and checks that it arrived.
 struct Readback;  // cm_common.hpp
 Readback readback_begin(cm_ctx *ctx, int first, int count, const unsigned *ovf = nullptr);
 void readback_end(cm_ctx *ctx, const Readback &rb);
