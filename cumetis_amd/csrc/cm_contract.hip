@@ -33,9 +33,9 @@ static_assert(REG_SEG == 16, "FIRST_SORTED_BIN must be the first bin above REG_S
 // entry is completed here; a representative also gives its partner the
 // coarse id.  Coarse count and segment length are scanned together as one
 // word (length << 16 | count; a tile counts at most 8192 vertices).  Four
-// look-back chains: coarse count and segment length (sums, walked by wave
-// 0), largest coarse vertex weight and longest segment (maxima, walked by
-// wave 1); the last tile stores the totals: totals[0] = nc, totals[3] =
+// look-back chains, walked by waves 0..3: coarse count and segment length
+// (sums), largest coarse vertex weight and longest segment (maxima); the
+// last tile stores the totals: totals[0] = nc, totals[3] =
 // largest coarse vertex weight, totals[4] = longest segment (a bound on the
 // longest coarse row).
 constexpr int NUM_PER = 2;
@@ -57,7 +57,7 @@ __global__ __launch_bounds__(1024) void k_number(int64_t n, const OffT *__restri
   static_assert(1024 * NUM_PER < (1 << 16), "a tile's coarse count fits the low 16 bits of the scan word");
   constexpr int NW = 1024 / CM_WAVE;  // waves per tile
   constexpr int NE = NW * NUM_PER;    // (stripe, wave) slots, in vertex order
-  __shared__ uint64_t sh_x[NE], sh_m[NW], sh_n[NW], sh_t[2];
+  __shared__ uint64_t sh_x[NE], sh_p[NE], sh_m[NW], sh_n[NW], sh_t[2];
   const int tile = tile_ticket(ctr);
   const int64_t i0 = (int64_t)tile * 1024 * NUM_PER + threadIdx.x;
   const int w = threadIdx.x >> 6, l = lane();
@@ -104,10 +104,12 @@ __global__ __launch_bounds__(1024) void k_number(int64_t n, const OffT *__restri
     sh_n[w] = ml;
   }
   __syncthreads();
-  if (w == 0) {
-    // exclusive prefix over the NE slots (stripe-major, then wave: the
-    // vertex order), EPL consecutive slots per lane
-    constexpr int EPL = (NE + CM_WAVE - 1) / CM_WAVE;
+  // waves 0..3 walk one look-back chain each, at the same time (a walk is
+  // a few round trips to the memory-side coherence point); waves 0 and 2
+  // both sum the slots, wave 0 also writes their exclusive prefixes (over
+  // the NE slots, stripe-major, then wave: the vertex order)
+  if (w == 0 || w == 2) {
+    constexpr int EPL = (NE + CM_WAVE - 1) / CM_WAVE;  // consecutive slots per lane
     uint64_t s[EPL], sum = 0;
 #pragma unroll
     for (int e = 0; e < EPL; ++e) {
@@ -117,35 +119,32 @@ __global__ __launch_bounds__(1024) void k_number(int64_t n, const OffT *__restri
     }
     const uint64_t inc = wave_incl_scan<uint64_t>(sum);
     const uint64_t tot = __shfl(inc, CM_WAVE - 1, CM_WAVE);
-    const uint64_t ec = lookback(st_c, tile, tot & 0xFFFFu, epoch);
-    const uint64_t el = lookback(st_l, tile, tot >> 16, epoch);
-    uint64_t run = inc - sum;
+    if (w == 0) {
+      uint64_t run = inc - sum;
 #pragma unroll
-    for (int e = 0; e < EPL; ++e) {
-      const int j = l * EPL + e;
-      if (j < NE) sh_x[j] = run;
-      run += s[e];
+      for (int e = 0; e < EPL; ++e) {
+        const int j = l * EPL + e;
+        if (j < NE) sh_p[j] = run;
+        run += s[e];
+      }
+      const uint64_t ec = lookback(st_c, tile, tot & 0xFFFFu, epoch);
+      if (l == 0) sh_t[0] = ec;
+    } else {
+      const uint64_t el = lookback(st_l, tile, tot >> 16, epoch);
+      if (l == 0) sh_t[1] = el;
     }
-    if (l == 0) {
-      sh_t[0] = ec;
-      sh_t[1] = el;
-    }
-  } else if (w == 1) {
-    const uint64_t bm = wave_reduce<LbMax>(l < NW ? sh_m[l] : 0);
-    const uint64_t bn = wave_reduce<LbMax>(l < NW ? sh_n[l] : 0);
-    const uint64_t em = lookback<LbMax>(st_m, tile, bm, epoch);
-    const uint64_t en = lookback<LbMax>(st_n, tile, bn, epoch);
-    if (l == 0 && (int64_t)(tile + 1) * 1024 * NUM_PER >= n) {  // the last tile
-      totals[3] = (int64_t)(em > bm ? em : bm);
-      totals[4] = (int64_t)(en > bn ? en : bn);
-    }
+  } else if (w == 1 || w == 3) {
+    const uint64_t b = wave_reduce<LbMax>(l < NW ? (w == 1 ? sh_m[l] : sh_n[l]) : 0);
+    const uint64_t e = lookback<LbMax>(w == 1 ? st_m : st_n, tile, b, epoch);
+    if (l == 0 && (int64_t)(tile + 1) * 1024 * NUM_PER >= n)  // the last tile
+      totals[w == 1 ? 3 : 4] = (int64_t)(e > b ? e : b);
   }
   __syncthreads();
   const uint64_t ec = sh_t[0], el = sh_t[1];
 #pragma unroll
   for (int t = 0; t < NUM_PER; ++t) {
     const int64_t i = i0 + (int64_t)t * 1024;
-    const uint64_t ex = sh_x[t * NW + w] + xin[t];
+    const uint64_t ex = sh_p[t * NW + w] + xin[t];
     const uint64_t c = ec + (ex & 0xFFFFu), off = el + (ex >> 16);
     if (i < n) {
       if (fl[t]) {

@@ -593,17 +593,28 @@ __device__ __forceinline__ void small_group(int64_t m0, unsigned ns, const int32
 // One vertex of the thread-per-row path: the accept of round r-1 for every
 // vertex, the proposal of round r for rows of at most short_deg entries.
 // changed / active are or-ed with this vertex's status.
-template <typename OffT>
+// An unmatched vertex, for launch 0 of a level whose match array that
+// launch initialises itself: its scans read this word instead of match[0].
+__device__ const int32_t unmatched_word = -1;
+
+template <typename OffT, bool INIT = false>
 __device__ __forceinline__ void short_row(int64_t v, int64_t n, const OffT *__restrict__ xadj,
                                           const int32_t *__restrict__ adjncy, const int32_t *__restrict__ adjwgt,
                                           const int32_t *__restrict__ vwgt, const MatchOwners &own, int32_t *match,
                                           const int32_t *prev, int32_t *cur, bool have_prev, int propose,
                                           int64_t maxvw, int short_deg, bool &changed, bool &active)
 {
+  // INIT (launch 0 of a level without long rows, which has no k_match_init):
+  // v is unmatched; this thread writes match[v] and the other proposal
+  // half's entry (the prev of launch 0) as k_match_init would
+  if (INIT && v < n) {
+    match[v] = -1;
+    const_cast<int32_t *>(prev)[v] = -1;
+  }
   // Loads in three dependency levels (by v; by p = prev[v]; by prev[p]),
   // each through clamped indices so that a level issues as one batch.
   const int64_t vc = v < n ? v : n - 1;
-  const int32_t mv = match[vc];
+  const int32_t mv = INIT ? -1 : match[vc];
   const int32_t p = have_prev ? prev[vc] : -1;
   if (v < n && mv == -1) {
     const int32_t pc = p >= 0 ? p : (int32_t)v;
@@ -636,8 +647,8 @@ __device__ __forceinline__ void short_row(int64_t v, int64_t n, const OffT *__re
         owner_params(own, v, maxv, seed, base);
         int32_t best = -2;
         uint64_t bk1 = 0, bk2 = 0;
-        scan_row<OffT>(v, b, e, 0, 1, adjncy, adjwgt, vwgt, vw, maxv, seed, base, match, prev, have_prev,
-                       maxvw < 0 || (int64_t)vw + maxvw > maxv, own.wunit, best, bk1, bk2);
+        scan_row<OffT>(v, b, e, 0, 1, adjncy, adjwgt, vwgt, vw, maxv, seed, base, INIT ? &unmatched_word : match,
+                       prev, have_prev, maxvw < 0 || (int64_t)vw + maxvw > maxv, own.wunit, best, bk1, bk2);
         cur[v] = best;
       }
     }
@@ -649,17 +660,23 @@ __device__ __forceinline__ void short_row(int64_t v, int64_t n, const OffT *__re
 // slist[blockIdx.x * VREGION ..] with vcount[blockIdx.x] entries;
 // the remaining blocks take the chunks of hub rows and the row groups of the
 // classes, one wave per item.
-template <typename OffT, bool LIST>
+template <typename OffT, bool LIST, bool INIT = false>
 __device__ __forceinline__ void round_body(int64_t n, const OffT *__restrict__ xadj,
                                            const int32_t *__restrict__ adjncy, const int32_t *__restrict__ adjwgt,
                                            const int32_t *__restrict__ vwgt, const MatchOwners &own, int32_t *match,
                                            const int32_t *prev, int32_t *cur, const LongChunk *chunks,
                                            const RowLists &rl, const unsigned *nchunks, const LongPartial &part,
                                            unsigned *flags, int r, unsigned nshort, int propose, int64_t maxvw,
-                                           int short_deg, const int32_t *slist, const unsigned *vcount)
+                                           int short_deg, const int32_t *slist, const unsigned *vcount,
+                                           int64_t nflags = 0, unsigned *hzero = nullptr)
 {
   if (r >= 2 && !round_live(flags, r - 2)) return;
   const bool have_prev = r >= 1;
+  if (INIT) {  // the rest of k_match_init: the round flags and the hash counters
+    const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gt < 16) hzero[gt] = 0u;
+    for (int64_t i = gt; i < nflags; i += (int64_t)gridDim.x * blockDim.x) flags[i] = 0u;
+  }
   if (blockIdx.x < nshort) {
     __shared__ int s_changed, s_active;
     if (threadIdx.x == 0) {
@@ -675,8 +692,8 @@ __device__ __forceinline__ void round_body(int64_t n, const OffT *__restrict__ x
         short_row<OffT>(list[i], n, xadj, adjncy, adjwgt, vwgt, own, match, prev, cur, have_prev, propose,
                         maxvw, short_deg, changed, active);
     } else {
-      short_row<OffT>((int64_t)blockIdx.x * blockDim.x + threadIdx.x, n, xadj, adjncy, adjwgt, vwgt, own, match,
-                      prev, cur, have_prev, propose, maxvw, short_deg, changed, active);
+      short_row<OffT, INIT>((int64_t)blockIdx.x * blockDim.x + threadIdx.x, n, xadj, adjncy, adjwgt, vwgt, own,
+                            match, prev, cur, have_prev, propose, maxvw, short_deg, changed, active);
     }
     if (have_prev) {
       if (__ballot(changed) && lane() == 0) s_changed = 1;
@@ -695,7 +712,7 @@ __device__ __forceinline__ void round_body(int64_t n, const OffT *__restrict__ x
     }
     return;
   }
-  if (!propose) return;
+  if (INIT || !propose) return;  // INIT: the level has no long rows
   // Long rows.  Work items in order: the hub chunks (one wave each), then
   // the row groups of each class in class order (one wave per group).
   // Whether a row is scanned depends only on state fixed before this launch
@@ -744,6 +761,23 @@ __global__ __launch_bounds__(256, 7) void k_round(int64_t n, const OffT *__restr
                           nshort, propose, maxvw, short_deg, nullptr, nullptr);
 }
 
+// Launch 0 (round 0 proposals only: no accept, no prev), compiled for r = 0.
+// INIT: a level without long rows, whose launch 0 also does the work of
+// k_match_init (the level's first launch then starts the rounds at once).
+template <typename OffT, bool INIT>
+__global__ __launch_bounds__(256, 7) void k_round0(int64_t n, const OffT *__restrict__ xadj,
+                                                const int32_t *__restrict__ adjncy,
+                                                const int32_t *__restrict__ adjwgt,
+                                                const int32_t *__restrict__ vwgt, MatchOwners own, int32_t *match,
+                                                int32_t *prev, int32_t *cur, const LongChunk *chunks, RowLists rl,
+                                                const unsigned *nchunks, LongPartial part, unsigned *flags,
+                                                unsigned nshort, int64_t maxvw, int short_deg, int64_t nflags,
+                                                unsigned *hzero)
+{
+  round_body<OffT, false, INIT>(n, xadj, adjncy, adjwgt, vwgt, own, match, prev, cur, chunks, rl, nchunks, part,
+                                flags, 0, nshort, 1, maxvw, short_deg, nullptr, nullptr, nflags, hzero);
+}
+
 // Launch r over the frontier (vertex regions and compacted lists).
 template <typename OffT>
 __global__ __launch_bounds__(256, 7) void k_round_list(int64_t n, const OffT *__restrict__ xadj,
@@ -787,6 +821,7 @@ struct MatchRun {
   unsigned *flags = nullptr, *nchunks = nullptr;
   unsigned g = 0, glong = 0;
   unsigned ghub = 0;  // blocks of k_hub_reduce (0: the level has no hub row)
+  bool fused_init = false;  // launch 0 initialises match, proposals and flags
   // the frontier lists from launch compact_round + 1 on (large levels only)
   bool compact = false;
   int compact_round = 0;
@@ -851,9 +886,14 @@ static void match_start(cm_ctx *ctx, MatchRun &mr)
   const bool has_hub = L.maxdeg < 0 || L.maxdeg > LONG_CHUNK;
   if (has_hub) fill_u32(ctx, mr.part.scanned, 0u, cap);  // no launch number is 0
   mr.g = nblocks(n, 256);
-  launch(ctx, "match_init", 8.0 * n, k_match_init, dim3(mr.g), dim3(256), 0, n, mr.match, mr.prop, mr.flags,
-         2 * (int64_t)mr.max_rounds + 16, ctx->d_ctr + HASH_CTR_SLOT);
-  ctx->hash_ctr_zero = true;
+  // a level without long rows has launch 0 initialise the state itself (its
+  // thread-per-row blocks cover every vertex); the long-row lists are
+  // counted before launch 0, into words that k_match_init clears
+  mr.fused_init = !has_long;
+  if (!mr.fused_init)
+    launch(ctx, "match_init", 8.0 * n, k_match_init, dim3(mr.g), dim3(256), 0, n, mr.match, mr.prop, mr.flags,
+           2 * (int64_t)mr.max_rounds + 16, ctx->d_ctr + HASH_CTR_SLOT);
+  ctx->hash_ctr_zero = true;  // by k_match_init or by launch 0
   // extra blocks whose waves stride over the long-row chunks (none when the
   // level is known to have no long row)
   if (has_long) {
@@ -887,6 +927,11 @@ static void match_launch_direct(cm_ctx *ctx, MatchRun &mr, int count)
              L.adjncy, L.adjwgt, L.vwgt, mr.own, mr.match, prv, cur, chunks, mr.rl2, counts, mr.part, mr.flags, r,
              gshort, r < mr.max_rounds ? 1 : 0, L.maxvw, mr.short_deg, (const int32_t *)mr.slist,
              (const unsigned *)mr.vcount);
+    else if (r == 0 && r < mr.max_rounds)
+      launch(ctx, "match_round", row_bytes, mr.fused_init ? k_round0<OffT, true> : k_round0<OffT, false>,
+             dim3(gshort + glong), dim3(256), 0, n, (const OffT *)L.xadj, L.adjncy, L.adjwgt, L.vwgt, mr.own,
+             mr.match, const_cast<int32_t *>(prv), cur, chunks, mr.rl, counts, mr.part, mr.flags, gshort, L.maxvw,
+             mr.short_deg, 2 * (int64_t)mr.max_rounds + 16, ctx->d_ctr + HASH_CTR_SLOT);
     else
       launch(ctx, "match_round", r == 0 ? row_bytes : 0.0, k_round<OffT>, dim3(gshort + glong), dim3(256), 0, n,
              (const OffT *)L.xadj, L.adjncy, L.adjwgt, L.vwgt, mr.own, mr.match, prv, cur, chunks, mr.rl, counts,
@@ -928,6 +973,7 @@ static std::string batch_key(const MatchRun &mr, int count)
   put(mr.flags);
   put(mr.nchunks);
   put(mr.g);
+  put(mr.fused_init);
   put(mr.glong);
   put(mr.ghub);
   put(mr.compact);
