@@ -805,13 +805,17 @@ void hash_prepare_impl(cm_ctx *ctx, const Level &L, const int32_t *match, const 
   const unsigned gchunk = (unsigned)std::min<int64_t>(4096, (nchunk + 3) / 4);
   hc.gchunk = gchunk;
   static_assert(2 * HC_WAVE_SPLIT == 1 << 7 && 2 * HC_WAVE_MAX == 1 << 9, "wave table sizes");
+  // the wave passes stride over their device-side item counts: a small
+  // level (whose counts are far below the bounds) dispatches fewer idle
+  // blocks, a large one up to 4096
+  const int64_t gwave = std::min<int64_t>(4096, std::max<int64_t>(256, m / 2048));
   launch(ctx, "hc_dedup_wave", 0.0, k_hc_dedup_wave<OffT, 7>,
-         dim3((unsigned)std::min<int64_t>(4096, (nseg + 3) / 4)), dim3(256), 0, (const int32_t *)wave_items,
+         dim3((unsigned)std::min<int64_t>(gwave, (nseg + 3) / 4)), dim3(256), 0, (const int32_t *)wave_items,
          (const unsigned *)ctr + HC_N_WAVE, segoff, rep, match, xadj, L.adjncy, L.adjwgt, cmap, hc.tc, hc.tw,
          cdeg);
   if (seg_max > HC_WAVE_SPLIT)
     launch(ctx, "hc_dedup_wave_l", 0.0, k_hc_dedup_wave<OffT, 9>,
-           dim3((unsigned)std::min<int64_t>(4096, (nwavel + 3) / 4)), dim3(256), 0, (const int32_t *)wavel_items,
+           dim3((unsigned)std::min<int64_t>(gwave, (nwavel + 3) / 4)), dim3(256), 0, (const int32_t *)wavel_items,
            (const unsigned *)ctr + HC_N_WAVEL, segoff, rep, match, xadj, L.adjncy, L.adjwgt, cmap, hc.tc, hc.tw,
            cdeg);
   static_assert(2 * HC_BLOCK_SPLIT == 1 << 11 && 2 * HC_BLOCK_MAX == 1 << 12, "block table sizes");
