@@ -176,6 +176,11 @@ Readback readback_begin(cm_ctx *ctx, int first, int count, const unsigned *ovf)
 
 void readback_end(cm_ctx *ctx, const Readback &rb)
 {
+  if (ctx->before_wait) {
+    std::function<void()> f;
+    f.swap(ctx->before_wait);
+    f();
+  }
   if (ctx->rb_early && !ctx->timing) {
     // The kernel stores the tag after every other word (a system-scope
     // release), so the words are final once the tag is here: the host goes
@@ -478,9 +483,31 @@ int ref_threshold(int64_t n, int nparts)
 // levels queue at least tune.large_level_batch launches first: a launch
 // after the last live round returns at once (4-7 us on such a level), while
 // a second batch costs a host synchronisation (about 40 us) first.
+//
+// The next level's matching queued ahead: while the host waits for a small
+// level's read-back, the GPU would go idle between the contraction and the
+// next level's first launch.  When the next level surely exists (every
+// coarse vertex has at most two members, so nc >= n / 2 > T) and has no
+// long rows (its rows are unions of two rows of at most maxdeg entries), its
+// first batch of rounds is queued behind the contraction, sized by the fine
+// level's bounds and reading nc from the device.  The batch is dropped when
+// more work is queued behind it (the contraction redone) or when the level's
+// own matching had to be resumed.
+struct Ahead {
+  MatchRun *mr = nullptr;
+  int32_t *mstate = nullptr;  // the rounds' working match of the next level
+  uint64_t seq = 0;           // ctx->launch_seq right after its launches
+};
+
+static void ahead_drop(cm_ctx *ctx, Ahead &a)
+{
+  if (a.mr) match_end(ctx, a.mr);
+  if (a.mstate) ctx->pool.release(a.mstate);
+  a = Ahead{};
+}
 
 static Level coarsen_level(cm_ctx *ctx, cm_hier *h, Level &F, int32_t maxvwgt, uint64_t seed,
-                           int max_rounds, int row_order, int batch)
+                           int max_rounds, int row_order, int batch, Ahead *in, Ahead *out)
 {
   const size_t ob = F.off64 ? 8 : 4;
   auto own = [&](size_t bytes) {
@@ -489,10 +516,12 @@ static Level coarsen_level(cm_ctx *ctx, cm_hier *h, Level &F, int32_t maxvwgt, u
     return p;
   };
   const int64_t n = F.n, m = F.m;
+  const bool adopt = in && in->mr;
   F.match = (int32_t *)own(4 * (n > 0 ? n : 1));
   F.cmap = (int32_t *)own(4 * (n > 0 ? n : 1));
   int32_t *rep = (int32_t *)ctx->pool.alloc(4 * (n > 0 ? n : 1));
-  int32_t *mstate = (int32_t *)ctx->pool.alloc(4 * (n > 0 ? n : 1));  // the rounds' working match
+  // the rounds' working match
+  int32_t *mstate = adopt ? in->mstate : (int32_t *)ctx->pool.alloc(4 * (n > 0 ? n : 1));
   void *segoff = ctx->pool.alloc(ob * (n + 1));
   Level C;
   C.off64 = F.off64;
@@ -503,16 +532,31 @@ static Level coarsen_level(cm_ctx *ctx, cm_hier *h, Level &F, int32_t maxvwgt, u
   MatchOwners mo{};
   mo.maxv0 = maxvwgt;
   mo.seed0 = seed;
-  MatchRun *mr = match_begin(ctx, F, mo, max_rounds, mstate);
+  MatchRun *mr = nullptr;
+  if (adopt) {
+    mr = in->mr;
+    *in = Ahead{};
+    match_adopt(ctx, mr, F);
+  } else {
+    try {
+      mr = match_begin(ctx, F, mo, max_rounds, mstate);
+    } catch (...) {
+      ctx->pool.release(rep);
+      ctx->pool.release(mstate);
+      ctx->pool.release(segoff);
+      throw;
+    }
+  }
   try {
     // Small levels queue one batch of rounds and go straight on to the
     // numbering and contraction of the finished copy of the match: the
     // contraction's synchronisation also brings back the round flags.  In
     // the rare case that the batch did not finish, the rounds continue from
     // their state and the level is redone.  Large levels, where a
-    // synchronisation costs little, check the rounds first.
+    // synchronisation costs little, check the rounds first.  A matching
+    // queued ahead (only on small levels) has its first batch queued.
     const bool speculate = m <= ctx->tune.speculate_edges;
-    if (n > 0) {
+    if (n > 0 && !adopt) {
       // speculative: flags come with the contraction's read-back
       match_queue(ctx, mr, speculate ? batch : std::max(batch, (int)ctx->tune.large_level_batch), !speculate);
       if (!speculate) sync(ctx);
@@ -533,8 +577,27 @@ static Level coarsen_level(cm_ctx *ctx, cm_hier *h, Level &F, int32_t maxvwgt, u
       const bool pre = row_order == CM_ROW_FIRST_SEEN && n > 0 && hash_needed(F);
       if (pre) hash_alloc(ctx, F, n, hc);
       number(ctx, F, F.match, F.cmap, rep, segoff, C.vwgt, false, mstate, pre ? &hc : nullptr);
+      if (out && pass == 0) {
+        ctx->before_wait = [&]() {
+          Level b;  // the next level: its arrays, with this level's sizes as bounds
+          b.n = n;
+          b.m = m;
+          b.off64 = C.off64;
+          b.xadj = C.xadj;
+          b.adjncy = C.adjncy;
+          b.adjwgt = C.adjwgt;
+          b.vwgt = C.vwgt;
+          b.maxdeg = 2 * F.maxdeg;
+          b.maxvw = -1;  // every pair's weight is checked
+          out->mstate = (int32_t *)ctx->pool.alloc(4 * n);
+          out->mr = match_begin_ahead(ctx, b, mo, max_rounds, out->mstate, ctx->d_small, batch);
+          out->seq = ctx->launch_seq;
+        };
+      }
       C.m = contract(ctx, F, F.match, F.cmap, -1, rep, segoff, row_order, C.xadj, C.adjncy, C.adjwgt,
                      &C.maxdeg, &C.n, pre ? &hc : nullptr);
+      ctx->before_wait = nullptr;  // a contraction without read-back did not run it
+      if (out && out->mr && ctx->launch_seq != out->seq) ahead_drop(ctx, *out);  // queued before a redo
       C.maxvw = ctx->h_small[3];
       if (ctx->rb_flags) {  // a contraction that read nothing back left the flags
         CM_HIP(hipMemcpyAsync(ctx->h_flags, ctx->rb_flags, sizeof(unsigned) * ctx->rb_nflags,
@@ -543,10 +606,13 @@ static Level coarsen_level(cm_ctx *ctx, cm_hier *h, Level &F, int32_t maxvwgt, u
         sync(ctx);
       }
       if (match_done(ctx, mr)) break;
+      if (out) ahead_drop(ctx, *out);  // it read the coarse graph of an unfinished matching
     }
     F.rounds = match_finish(ctx, mr);
   } catch (...) {
+    ctx->before_wait = nullptr;
     ctx->rb_flags = nullptr;
+    if (out) ahead_drop(ctx, *out);
     match_end(ctx, mr);
     ctx->pool.release(rep);
     ctx->pool.release(mstate);
@@ -832,12 +898,27 @@ int cm_coarsen(cm_ctx *ctx, const cm_graph_view *g, const cm_coarsen_params *p, 
     // levels go on from their read-back; the sync below ends the call
     struct Early {
       cm_ctx *ctx;
+      cm::Ahead ahead, next;  // the next level's matching, queued ahead
       Early(cm_ctx *c) : ctx(c) { ctx->rb_early = true; }
-      ~Early() { ctx->rb_early = false; }
+      ~Early()
+      {
+        cm::ahead_drop(ctx, ahead);
+        cm::ahead_drop(ctx, next);
+        ctx->rb_early = false;
+      }
     } early(ctx);
     for (;;) {
+      const cm::Level &F = h->levels.back();
+      // the level after the one built now is queued ahead when it surely
+      // exists: nc >= n / 2 > T, within max_levels, rows of at most
+      // 2 * maxdeg <= LONG_ROW entries, a small level (see coarsen_level)
+      const bool ahead_ok = !ctx->timing && !ctx->tune.graph_rounds && F.n / 2 > T &&
+                            F.m <= ctx->tune.speculate_edges && F.n < ctx->tune.compact_min_n &&
+                            F.maxdeg >= 0 && 2 * F.maxdeg <= cm::LONG_ROW &&
+                            (p->max_levels <= 0 || (int)h->levels.size() + 1 <= p->max_levels);
       cm::Level C = cm::coarsen_level(ctx, h, h->levels.back(), maxvwgt, p->seed, p->max_rounds,
-                                      p->row_order, batch);
+                                      p->row_order, batch, &early.ahead, ahead_ok ? &early.next : nullptr);
+      std::swap(early.ahead, early.next);  // early.next is empty: coarsen_level used up early.ahead
       // the next level's first batch: launches for the rounds this level
       // took plus one (launch r accepts round r-1, so r rounds need r + 1
       // launches to show that they finished); a short batch only costs a
@@ -847,7 +928,7 @@ int cm_coarsen(cm_ctx *ctx, const cm_graph_view *g, const cm_coarsen_params *p, 
       const int64_t nf = h->levels.back().n;
       h->levels.push_back(C);
       const bool more = C.n > T && C.n < 0.75 * nf && C.m > C.n / 2;  // cu:1843-1845
-      if (!more) break;
+      if (!more) break;  // (an ahead matching is dropped with `early`)
       if (p->max_levels > 0 && (int)h->levels.size() > p->max_levels) break;
     }
     cm::sync(ctx);

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <functional>
 #include <map>
 #include <memory>
 #include <unordered_map>
@@ -134,6 +135,10 @@ struct cm_ctx {
   // host memory, while the stream may still run (the caller queues only
   // stream-ordered work after it and synchronises before it returns)
   bool rb_early = false;
+  // run once by the next readback_end before it waits (the coarsening
+  // queues the next level's first rounds there, behind the contraction)
+  std::function<void()> before_wait;
+  uint64_t launch_seq = 0;  // kernels launched so far
   // unit weights shared by every graph given without weights (never
   // written; earlier, shorter blocks stay valid until the context dies)
   int32_t *ones = nullptr;
@@ -186,6 +191,7 @@ inline void launch(cm_ctx *ctx, const char *name, double bytes, K kernel, dim3 g
   }
   hipLaunchKernelGGL(kernel, grid, block, shmem, ctx->stream, args...);
   CM_HIP(hipGetLastError());
+  ++ctx->launch_seq;
   if (ctx->timing) {
     b = ctx->ev_get();
     CM_HIP(hipEventRecord(b, ctx->stream));
@@ -241,6 +247,14 @@ struct MatchOwners {
 // themselves; end() releases the state.
 struct MatchRun;
 MatchRun *match_begin(cm_ctx *ctx, const Level &L, const MatchOwners &own, int max_rounds, int32_t *d_match);
+// A matching queued before its level's sizes are on the host: `bound` has
+// the level's arrays and bounds on its sizes (no rows above LONG_ROW); the
+// first `count` launches read the vertex count from d_n.  adopt() gives it
+// the real sizes once they are known (later launches use them) and leaves
+// the flags of the queued launches to the next read-back.
+MatchRun *match_begin_ahead(cm_ctx *ctx, const Level &bound, const MatchOwners &own, int max_rounds,
+                            int32_t *d_match, const int64_t *d_n, int count);
+void match_adopt(cm_ctx *ctx, MatchRun *mr, const Level &L);
 // copy_flags = false leaves the flags to the next read-back (readback_begin).
 void match_queue(cm_ctx *ctx, MatchRun *mr, int count, bool copy_flags = true);
 bool match_done(cm_ctx *ctx, MatchRun *mr);

@@ -755,8 +755,9 @@ __global__ __launch_bounds__(256, 7) void k_round(int64_t n, const OffT *__restr
                                                int32_t *match, const int32_t *prev, int32_t *cur,
                                                const LongChunk *chunks, RowLists rl, const unsigned *nchunks,
                                                LongPartial part, unsigned *flags, int r, unsigned nshort,
-                                               int propose, int64_t maxvw, int short_deg)
+                                               int propose, int64_t maxvw, int short_deg, const int64_t *np)
 {
+  if (np) n = *np;  // queued ahead: the level's count was not yet on the host
   round_body<OffT, false>(n, xadj, adjncy, adjwgt, vwgt, own, match, prev, cur, chunks, rl, nchunks, part, flags, r,
                           nshort, propose, maxvw, short_deg, nullptr, nullptr);
 }
@@ -772,8 +773,9 @@ __global__ __launch_bounds__(256, 7) void k_round0(int64_t n, const OffT *__rest
                                                 int32_t *prev, int32_t *cur, const LongChunk *chunks, RowLists rl,
                                                 const unsigned *nchunks, LongPartial part, unsigned *flags,
                                                 unsigned nshort, int64_t maxvw, int short_deg, int64_t nflags,
-                                                unsigned *hzero)
+                                                unsigned *hzero, const int64_t *np)
 {
+  if (np) n = *np;  // queued ahead: the level's count was not yet on the host
   round_body<OffT, false, INIT>(n, xadj, adjncy, adjwgt, vwgt, own, match, prev, cur, chunks, rl, nchunks, part,
                                 flags, 0, nshort, 1, maxvw, short_deg, nullptr, nullptr, nflags, hzero);
 }
@@ -810,6 +812,9 @@ __global__ __launch_bounds__(256) void k_match_finish(int64_t n, const int32_t *
 // show that the last batch did not finish.
 struct MatchRun {
   const Level *L = nullptr;
+  Level own_level;             // the level of a run queued ahead (L points here)
+  const int64_t *d_n = nullptr;  // queued ahead: launches read the count here
+  int64_t half = 0;            // stride of the two proposal halves (the n it began with)
   MatchOwners own;
   int max_rounds = 0;
   int32_t *match = nullptr;
@@ -847,6 +852,7 @@ static void match_start(cm_ctx *ctx, MatchRun &mr)
     return p;
   };
   mr.prop = (int32_t *)al(sizeof(int32_t) * 2 * n);  // ping-pong
+  mr.half = n;
   // the shared unit array is never written, so a level that points to it has
   // unit edge weights
   mr.own.wunit = L.m > 0 && L.adjwgt == ctx->ones;
@@ -873,7 +879,7 @@ static void match_start(cm_ctx *ctx, MatchRun &mr)
   mr.nchunks2 = mr.nchunks + 8;
   static_assert(NCLASS + 1 <= 8, "list counts");
   mr.compact_round = ctx->tune.compact_round;
-  mr.compact = n >= ctx->tune.compact_min_n && mr.max_rounds > mr.compact_round + 1;
+  mr.compact = !mr.d_n && n >= ctx->tune.compact_min_n && mr.max_rounds > mr.compact_round + 1;
   if (mr.compact) {
     mr.slist = (int32_t *)al(sizeof(int32_t) * (size_t)nblocks(n, VREGION) * VREGION);
     mr.vcount = (unsigned *)al(sizeof(unsigned) * nblocks(n, VREGION));
@@ -916,8 +922,8 @@ static void match_launch_direct(cm_ctx *ctx, MatchRun &mr, int count)
   const double row_bytes = (double)sizeof(OffT) * (n + 1) + 8.0 * L.m + 8.0 * n;
   for (int i = 0; i < count && mr.r <= mr.max_rounds; ++i, ++mr.r) {
     const int r = mr.r;
-    int32_t *cur = mr.prop + (size_t)(r & 1) * n;
-    const int32_t *prv = mr.prop + (size_t)((r + 1) & 1) * n;
+    int32_t *cur = mr.prop + (size_t)(r & 1) * mr.half;
+    const int32_t *prv = mr.prop + (size_t)((r + 1) & 1) * mr.half;
     const bool lst = mr.compact && r > mr.compact_round;
     const LongChunk *chunks = lst ? mr.chunks2 : mr.chunks;
     const unsigned *counts = lst ? mr.nchunks2 : mr.nchunks;
@@ -931,11 +937,11 @@ static void match_launch_direct(cm_ctx *ctx, MatchRun &mr, int count)
       launch(ctx, "match_round", row_bytes, mr.fused_init ? k_round0<OffT, true> : k_round0<OffT, false>,
              dim3(gshort + glong), dim3(256), 0, n, (const OffT *)L.xadj, L.adjncy, L.adjwgt, L.vwgt, mr.own,
              mr.match, const_cast<int32_t *>(prv), cur, chunks, mr.rl, counts, mr.part, mr.flags, gshort, L.maxvw,
-             mr.short_deg, 2 * (int64_t)mr.max_rounds + 16, ctx->d_ctr + HASH_CTR_SLOT);
+             mr.short_deg, 2 * (int64_t)mr.max_rounds + 16, ctx->d_ctr + HASH_CTR_SLOT, mr.d_n);
     else
       launch(ctx, "match_round", r == 0 ? row_bytes : 0.0, k_round<OffT>, dim3(gshort + glong), dim3(256), 0, n,
              (const OffT *)L.xadj, L.adjncy, L.adjwgt, L.vwgt, mr.own, mr.match, prv, cur, chunks, mr.rl, counts,
-             mr.part, mr.flags, r, gshort, r < mr.max_rounds ? 1 : 0, L.maxvw, mr.short_deg);
+             mr.part, mr.flags, r, gshort, r < mr.max_rounds ? 1 : 0, L.maxvw, mr.short_deg, mr.d_n);
     if (mr.ghub && r < mr.max_rounds)
       launch(ctx, "match_hub_reduce", 0.0, k_hub_reduce, dim3(mr.ghub), dim3(256), 0, chunks, counts, mr.part,
              cur, (const unsigned *)mr.flags, r);
@@ -974,6 +980,8 @@ static std::string batch_key(const MatchRun &mr, int count)
   put(mr.nchunks);
   put(mr.g);
   put(mr.fused_init);
+  put(mr.d_n);
+  put(mr.half);
   put(mr.glong);
   put(mr.ghub);
   put(mr.compact);
@@ -1084,6 +1092,55 @@ MatchRun *match_begin(cm_ctx *ctx, const Level &L, const MatchOwners &own, int m
     }
   }
   return mr;
+}
+
+MatchRun *match_begin_ahead(cm_ctx *ctx, const Level &bound, const MatchOwners &own, int max_rounds,
+                            int32_t *d_match, const int64_t *d_n, int count)
+{
+  CM_REQUIRE(bound.n > 0 && bound.maxdeg >= 0 && bound.maxdeg <= LONG_DEG, CM_ERR_INVALID,
+             "a matching queued ahead needs a level without long rows");
+  MatchRun *mr = new MatchRun();
+  mr->own_level = bound;
+  mr->L = &mr->own_level;
+  mr->own = own;
+  mr->max_rounds = max_rounds > 0 ? max_rounds : 4096;
+  mr->match = d_match;
+  mr->d_n = d_n;
+  // the counters are cleared by launch 0, behind the work already queued:
+  // until match_adopt the context keeps its own account of them
+  const bool ctr_zero = ctx->hash_ctr_zero;
+  try {
+    if (bound.off64) {
+      match_start<int64_t>(ctx, *mr);
+      match_launch_direct<int64_t>(ctx, *mr, count);
+    } else {
+      match_start<int32_t>(ctx, *mr);
+      match_launch_direct<int32_t>(ctx, *mr, count);
+    }
+  } catch (...) {
+    ctx->hash_ctr_zero = ctr_zero;
+    match_end(ctx, mr);
+    throw;
+  }
+  ctx->hash_ctr_zero = ctr_zero;
+  return mr;
+}
+
+void match_adopt(cm_ctx *ctx, MatchRun *mr, const Level &L)
+{
+  Level &own = mr->own_level;
+  own.n = L.n;
+  own.m = L.m;
+  own.maxdeg = L.maxdeg;
+  own.maxvw = L.maxvw;
+  mr->d_n = nullptr;
+  mr->g = nblocks(L.n, 256);
+  // the flags of the queued launches come with the next read-back
+  const size_t words = 2 * (size_t)std::max(mr->r - 1, 1);
+  ensure_host_flags(ctx, words);
+  ctx->rb_flags = mr->flags;
+  ctx->rb_nflags = (int64_t)words;
+  ctx->hash_ctr_zero = true;  // launch 0 cleared them after every earlier use
 }
 
 void match_queue(cm_ctx *ctx, MatchRun *mr, int count, bool copy_flags)
