@@ -847,16 +847,34 @@ __global__ __launch_bounds__(SMALL_TILE) void k_contract_small(const int64_t *__
   // come from segments of at most TILE_COPY_SEG entries) are copied in from
   // the staging area; otherwise their slots (written later by the emit
   // pass) keep the sentinel and are skipped.  Windows cover the range up to
-  // the end of the last row written.
+  // the end of the last row written.  A tile whose range is longer than one
+  // window (hashed rows in it, written by the emit pass) moves each window to
+  // the first entry written at or after the end of the previous one, so the
+  // hashed rows' ranges are not walked.
   const bool copied = tc != nullptr && c < nc && !small && len <= TILE_COPY_SEG;
-  const uint64_t my_end = (small || copied) ? ex + (uint64_t)deg : 0;
+  const bool writes = (small || copied) && deg > 0;
+  const uint64_t my_end = writes ? ex + (uint64_t)deg : 0;
   uint64_t span = wave_max<uint64_t>(my_end);
   if (lane() == 0) sh[threadIdx.x >> 6] = span;
   __syncthreads();
   span = 0;
 #pragma unroll
   for (int i = 0; i < SMALL_TILE / CM_WAVE; ++i) span = sh[i] > span ? sh[i] : span;
-  for (uint64_t w0 = 0; w0 < span; w0 += SMALL_STAGE) {
+  // the first entry written at or after `from` (block-wide; all threads call)
+  auto next_written = [&](uint64_t from) -> uint64_t {
+    const uint64_t cand = writes && my_end > from ? (ex > from ? ex : from) : ~0ull;
+    const uint64_t wm = wave_max<uint64_t>(~cand);
+    __syncthreads();  // sh is free again
+    if (lane() == 0) sh[threadIdx.x >> 6] = wm;
+    __syncthreads();
+    uint64_t r = 0;
+#pragma unroll
+    for (int i = 0; i < SMALL_TILE / CM_WAVE; ++i) r = sh[i] > r ? sh[i] : r;
+    return ~r;
+  };
+  const bool skip = span > (uint64_t)SMALL_STAGE;  // block-uniform
+  for (uint64_t w0 = skip ? next_written(0) : 0; w0 < span;
+       w0 = skip ? next_written(w0 + SMALL_STAGE) : w0 + SMALL_STAGE) {
     const int wlen = (int)(span - w0 < (uint64_t)SMALL_STAGE ? span - w0 : (uint64_t)SMALL_STAGE);
     for (int k = threadIdx.x; k < wlen; k += SMALL_TILE) stage_a[k] = -1;
     __syncthreads();
