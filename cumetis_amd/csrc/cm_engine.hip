@@ -222,8 +222,17 @@ uint32_t lookback_begin(cm_ctx *ctx, size_t words)
   return ctx->epoch;
 }
 
+void close_timing_run(cm_ctx *ctx)
+{
+  if (ctx->pending.empty() || ctx->pending.back().b != nullptr) return;
+  hipEvent_t b = ctx->ev_get();
+  CM_HIP(hipEventRecord(b, ctx->stream));
+  ctx->pending.back().b = b;
+}
+
 void sync(cm_ctx *ctx)
 {
+  if (ctx->timing) close_timing_run(ctx);
   CM_HIP(hipStreamSynchronize(ctx->stream));
   if (ctx->timing) ctx->collect_timing();
 }
@@ -675,7 +684,7 @@ void cm_ctx::collect_timing()
     float ms = 0.f;
     CM_HIP(hipEventElapsedTime(&ms, t.a, t.b));
     auto &s = stats[t.name];
-    s.count += 1;
+    s.count += t.count;
     s.total_ms += ms;
     s.algo_bytes += t.bytes;
     ev_free.push_back(t.a);
@@ -743,7 +752,7 @@ int cm_ctx_destroy(cm_ctx *ctx)
   (void)hipStreamSynchronize(ctx->stream);
   for (auto &t : ctx->pending) {
     (void)hipEventDestroy(t.a);
-    (void)hipEventDestroy(t.b);
+    if (t.b) (void)hipEventDestroy(t.b);
   }
   for (auto e : ctx->ev_free) (void)hipEventDestroy(e);
   for (auto &kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second);
@@ -768,8 +777,10 @@ int cm_ctx_set_timing(cm_ctx *ctx, int on)
 {
   if (!ctx) return CM_ERR_INVALID;
   CM_API_BEGIN
+  CM_REQUIRE(on >= 0 && on <= 2, CM_ERR_INVALID, "timing mode is 0, 1 or 2");
   cm::sync(ctx);
   ctx->timing = on != 0;
+  ctx->timing_runs = on == 2;
   CM_API_END(ctx)
 }
 

@@ -4,6 +4,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <string.h>
 
 #include <functional>
 #include <map>
@@ -67,6 +68,8 @@ struct Timed {
   const char *name;
   hipEvent_t a, b;
   double bytes;
+  int64_t count;  // launches between a and b (run timing: consecutive launches of one name)
+  uint64_t seq;   // launch_seq after the last of them
 };
 
 // Rows above LONG_ROW entries are scanned by whole waves in the matching.
@@ -115,7 +118,11 @@ struct cm_ctx {
   cm::Pool pool;
   std::string err;
   bool timing = false;
-  std::vector<cm::Timed> pending;
+  // timing mode 2: consecutive launches of one name share one pair of events
+  // (no event between them), so a run of short launches is timed without
+  // the gap an event adds to every launch
+  bool timing_runs = false;
+  std::vector<cm::Timed> pending;  // run timing: the last one may be open (b == nullptr)
   std::vector<hipEvent_t> ev_free;
   std::map<std::string, cm::KStat> stats;
   // small persistent device buffers
@@ -179,12 +186,38 @@ void ensure_host_flags(cm_ctx *ctx, size_t words);
 // Status words for a look-back launch over `words` tiles; returns the epoch.
 uint32_t lookback_begin(cm_ctx *ctx, size_t words);
 
+// Run timing: records the end event of the open run, if any.
+void close_timing_run(cm_ctx *ctx);
+
 template <typename K, typename... Args>
 inline void launch(cm_ctx *ctx, const char *name, double bytes, K kernel, dim3 grid, dim3 block,
                    size_t shmem, Args... args)
 {
   if (grid.x == 0 || grid.y == 0 || grid.z == 0) return;
   hipEvent_t a = nullptr, b = nullptr;
+  if (ctx->timing && ctx->timing_runs) {
+    // run timing: a launch that follows a launch of the same name joins its
+    // run (no event between them); the run ends at the next launch of
+    // another name or at the next synchronisation (close_timing_run)
+    Timed *open = !ctx->pending.empty() && ctx->pending.back().b == nullptr ? &ctx->pending.back() : nullptr;
+    const bool extend = open && open->seq == ctx->launch_seq && strcmp(open->name, name) == 0;
+    if (!extend) {
+      close_timing_run(ctx);
+      a = ctx->ev_get();
+      CM_HIP(hipEventRecord(a, ctx->stream));
+    }
+    hipLaunchKernelGGL(kernel, grid, block, shmem, ctx->stream, args...);
+    CM_HIP(hipGetLastError());
+    ++ctx->launch_seq;
+    if (extend) {
+      open->bytes += bytes;
+      open->count += 1;
+      open->seq = ctx->launch_seq;
+    } else {
+      ctx->pending.push_back(Timed{name, a, nullptr, bytes, 1, ctx->launch_seq});
+    }
+    return;
+  }
   if (ctx->timing) {
     a = ctx->ev_get();
     CM_HIP(hipEventRecord(a, ctx->stream));
@@ -195,7 +228,7 @@ inline void launch(cm_ctx *ctx, const char *name, double bytes, K kernel, dim3 g
   if (ctx->timing) {
     b = ctx->ev_get();
     CM_HIP(hipEventRecord(b, ctx->stream));
-    ctx->pending.push_back(Timed{name, a, b, bytes});
+    ctx->pending.push_back(Timed{name, a, b, bytes, 1, ctx->launch_seq});
   }
 }
 

@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <chrono>
 #include <mutex>
+#include <thread>
 #include <vector>
 
 #include "cm_internal.hpp"
@@ -74,16 +75,13 @@ int64_t host_cut(const cm_graph_view *g, const int32_t *part)
 
 namespace cm {
 
-void partition_hierarchy(cm_ctx *ctx, const cm_hier *h, int nparts, int32_t *part, double *init_ms,
-                         double *uncoarsen_ms, bool balance)
+cmh::HostGraph coarsest_to_host(cm_ctx *ctx, const cm_hier *h) { return level_to_host(ctx, h->levels.back()); }
+
+void uncoarsen_hierarchy(cm_ctx *ctx, const cm_hier *h, int nparts, const cmh::HostGraph &hc,
+                         const std::vector<int> &where, int32_t *part, bool balance)
 {
-  const double t1 = now_ms();
   const int L = (int)h->levels.size();
   const Level &C = h->levels[L - 1];
-  cmh::HostGraph hc = level_to_host(ctx, C);
-  std::vector<int> where;
-  cmh::initial_partition(hc, nparts, where);
-  const double t2 = now_ms();
   const int64_t tvwgt = h->levels[0].tvwgt;
   std::vector<int64_t> pw(nparts, 0);
   for (int i = 0; i < hc.n; ++i) pw[where[i]] += hc.vwgt[i];
@@ -111,6 +109,17 @@ void partition_hierarchy(cm_ctx *ctx, const cm_hier *h, int nparts, int32_t *par
     throw;
   }
   ctx->pool.release(w0);
+}
+
+void partition_hierarchy(cm_ctx *ctx, const cm_hier *h, int nparts, int32_t *part, double *init_ms,
+                         double *uncoarsen_ms, bool balance)
+{
+  const double t1 = now_ms();
+  cmh::HostGraph hc = coarsest_to_host(ctx, h);
+  std::vector<int> where;
+  cmh::initial_partition(hc, nparts, where);
+  const double t2 = now_ms();
+  uncoarsen_hierarchy(ctx, h, nparts, hc, where, part, balance);
   if (init_ms) *init_ms = t2 - t1;
   if (uncoarsen_ms) *uncoarsen_ms = now_ms() - t2;
 }
@@ -129,7 +138,6 @@ int cm_partition(cm_ctx *ctx, const cm_graph_view *g, const cm_coarsen_params *p
                  cm_part_report *rep)
 {
   if (!ctx || !g || !p || !part) return CM_ERR_INVALID;
-  cm_hier *h = nullptr;
   std::vector<void *> dev;
   auto release_dev = [&]() {
     for (void *q : dev) ctx->pool.release(q);
@@ -164,52 +172,103 @@ int cm_partition(cm_ctx *ctx, const cm_graph_view *g, const cm_coarsen_params *p
     dv.adjncy = (const int32_t *)up(g->adjncy, 4 * g->m);
     dv.adjwgt = (const int32_t *)up(g->adjwgt, 4 * g->m);
     dv.vwgt = (const int32_t *)up(g->vwgt, 4 * g->n);
-    std::vector<int32_t> cur(g->n);
+    // Trials overlap: the host initial partition of trial t runs in a thread
+    // of its own while the GPU coarsens the later trials; the uncoarsenings
+    // then run one after another on the stream.  Each trial's result is the
+    // one it has alone (the initial partition draws from a private stream).
+    struct Trial {
+      cm_hier *h = nullptr;
+      cmh::HostGraph hc;
+      std::vector<int> where;
+      std::thread th;
+      bool failed = false;
+    };
+    struct Trials {
+      cm_ctx *ctx;
+      std::vector<Trial> t;
+      ~Trials()
+      {
+        for (auto &x : t) {
+          if (x.th.joinable()) x.th.join();
+          if (x.h) cm_hier_release(ctx, x.h);
+        }
+      }
+    } tr{ctx, std::vector<Trial>(trials)};
     int64_t best = -1;
     cm_part_report r{};
     r.trials = trials;
+    const int nparts = p->nparts;
+    const double p0 = now_ms();
     for (int t = 0; t < trials; ++t) {
       cm_coarsen_params q = *p;
       q.seed = p->seed + (uint64_t)t;
       if (t > 0) q.flags |= CM_FLAG_TRUSTED_INPUT;  // validated by the first trial
       const double c0 = now_ms();
-      int rc = cm_coarsen(ctx, &dv, &q, &h);
+      int rc = cm_coarsen(ctx, &dv, &q, &tr.t[t].h);
       if (rc != CM_OK) {
         release_dev();
         return rc;
       }
-      const double c1 = now_ms();
-      double init_ms = 0, unc_ms = 0;
-      int32_t *dst = t == 0 ? part : cur.data();
-      cm::partition_hierarchy(ctx, h, p->nparts, dst, &init_ms, &unc_ms, !(p->flags & CM_FLAG_NO_BALANCE));
-      const int64_t cut = host_cut(g, dst);
-      r.coarsen_ms += c1 - c0;
-      r.init_ms += init_ms;
-      r.uncoarsen_ms += unc_ms;
-      if (best < 0 || cut < best) {
-        best = cut;
-        r.best_trial = t;
-        r.levels = (int32_t)h->levels.size();
-        r.coarsest_n = (int32_t)h->levels.back().n;
-        r.edgecut = cut;
-        if (dst != part) std::copy(cur.begin(), cur.end(), part);
-      }
-      cm_hier_release(ctx, h);
-      h = nullptr;
+      r.coarsen_ms += now_ms() - c0;
+      Trial &x = tr.t[t];
+      x.hc = cm::coarsest_to_host(ctx, x.h);
+      x.th = std::thread([&x, nparts]() {
+        try {
+          cmh::initial_partition(x.hc, nparts, x.where);
+        } catch (...) {
+          x.failed = true;
+        }
+      });
     }
+    for (auto &x : tr.t) x.th.join();
+    const double p1 = now_ms();
+    r.init_ms = p1 - p0 - r.coarsen_ms;  // host time the trials' initial partitions add
+    // the uncoarsenings one after another on the stream; each trial's edge
+    // cut is counted on the host (cu:356-366) while the next one runs
+    std::vector<std::vector<int32_t>> parts(trials);
+    std::vector<int64_t> cut(trials, -1);
+    std::vector<int32_t> levels(trials), coarsest(trials);
+    struct Cuts {  // joined before parts and cut go away
+      std::vector<std::thread> th;
+      ~Cuts()
+      {
+        for (auto &x : th)
+          if (x.joinable()) x.join();
+      }
+    } cuts;
+    for (int t = 0; t < trials; ++t) {
+      Trial &x = tr.t[t];
+      CM_REQUIRE(!x.failed, CM_ERR_NOMEM, "initial partition failed (host allocation)");
+      parts[t].resize(g->n);
+      cm::uncoarsen_hierarchy(ctx, x.h, nparts, x.hc, x.where, parts[t].data(), !(p->flags & CM_FLAG_NO_BALANCE));
+      levels[t] = (int32_t)x.h->levels.size();
+      coarsest[t] = (int32_t)x.h->levels.back().n;
+      cm_hier_release(ctx, x.h);
+      x.h = nullptr;
+      cuts.th.emplace_back([&parts, &cut, g, t]() { cut[t] = host_cut(g, parts[t].data()); });
+    }
+    r.uncoarsen_ms = now_ms() - p1;
+    for (auto &x : cuts.th) x.join();
+    for (int t = 0; t < trials; ++t)
+      if (best < 0 || cut[t] < best) {  // the earlier trial on equal cuts
+        best = cut[t];
+        r.best_trial = t;
+        r.levels = levels[t];
+        r.coarsest_n = coarsest[t];
+        r.edgecut = cut[t];
+      }
+    std::copy(parts[r.best_trial].begin(), parts[r.best_trial].end(), part);
     cm::sync(ctx);
     release_dev();
     r.total_ms = now_ms() - t0;
     if (rep) *rep = r;
   } catch (const cm::Error &e) {
     ctx->err = e.what();
-    if (h) cm_hier_release(ctx, h);
     (void)hipStreamSynchronize(ctx->stream);
     release_dev();
     return e.code;
   } catch (const std::exception &e) {
     ctx->err = e.what();
-    if (h) cm_hier_release(ctx, h);
     (void)hipStreamSynchronize(ctx->stream);
     release_dev();
     return CM_ERR_INVALID;

@@ -171,8 +171,10 @@ class Engine:
     def sync(self):
         self._check(self.lib.cm_ctx_sync(self.ctx))
 
-    def set_timing(self, on: bool):
-        self._check(self.lib.cm_ctx_set_timing(self.ctx, 1 if on else 0))
+    def set_timing(self, on):
+        """Kernel timing: False/0 off, True/1 events around every launch, 2
+        events around every run of consecutive launches of one kernel."""
+        self._check(self.lib.cm_ctx_set_timing(self.ctx, int(on)))
 
     def reset_stats(self):
         self._check(self.lib.cm_ctx_reset_stats(self.ctx))

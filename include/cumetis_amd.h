@@ -123,7 +123,10 @@ int cm_ctx_create(int device, void *stream, cm_ctx **out);
 int cm_ctx_destroy(cm_ctx *ctx);
 const char *cm_ctx_last_error(const cm_ctx *ctx);
 int cm_ctx_sync(cm_ctx *ctx);
-/* Per-kernel hipEvent timing on the context stream (off by default). */
+/* Per-kernel hipEvent timing on the context stream (off by default):
+ * on = 1 records events around every launch; on = 2 around every run of
+ * consecutive launches of one kernel (the events then add no gap between
+ * them; a run's time includes the gaps between its launches). */
 int cm_ctx_set_timing(cm_ctx *ctx, int on);
 int cm_ctx_reset_stats(cm_ctx *ctx);
 /* Stats of kernel i (0-based) in name order: returns CM_ERR_INVALID past the
@@ -248,8 +251,12 @@ int cm_project_raw(cm_ctx *ctx, int64_t n, const int32_t *d_cmap, const int32_t 
 int cuMetis_PartGraph(int *nvtxs, int *xadj, int *adjncy, int *vwgt, int *adjwgt, int *nparts,
                       float *tpwgts, float *ubvec, int *objval, int *part);
 
-/* Same pipeline with explicit options and per-phase timings (ms, summed
- * over the trials; levels and coarsest_n are the kept trial's). */
+/* Same pipeline with explicit options and per-phase timings (ms; levels
+ * and coarsest_n are the kept trial's).  The trials' host initial
+ * partitions run in threads of their own while the GPU coarsens the later
+ * trials: coarsen_ms sums the coarsenings, init_ms is the wall time the
+ * initial partitions add beyond them, uncoarsen_ms the wall time of the
+ * trials' uncoarsenings (one after another on the stream). */
 typedef struct {
   double total_ms, coarsen_ms, init_ms, uncoarsen_ms;
   int32_t levels;

@@ -129,3 +129,31 @@ def test_tuning_rejects_out_of_range(engine):
             engine.set_tuning(**bad)
         assert e.value.code == CM_ERR_INVALID
     assert engine.tuning()["first_round_batch"] >= 2
+
+
+@pytest.mark.parametrize("name", ["grid200w100", "rmat14"])
+def test_run_timing_counts_every_launch(engine, name):
+    """Kernel timing by runs (mode 2: one event pair around consecutive
+    launches of one kernel) counts the same launches per kernel as timing
+    every launch (mode 1), and neither changes the hierarchy."""
+    g = with_weights(_graphs()[name])
+    say(f"[run timing] {name}")
+    h = engine.coarsen(g, 16, seed=5)
+    plain = _levels(h)
+    h.release()
+    counts = {}
+    try:
+        for mode in (1, 2):
+            engine.set_timing(mode)
+            engine.reset_stats()
+            h = engine.coarsen(g, 16, seed=5)
+            _same(_levels(h), plain)
+            h.release()
+            counts[mode] = engine.kernel_stats()
+    finally:
+        engine.set_timing(0)
+    assert {k: v["count"] for k, v in counts[1].items()} == {k: v["count"] for k, v in counts[2].items()}
+    assert counts[2]["match_round"]["ms"] > 0 and counts[2]["contract_small"]["ms"] > 0
+    with pytest.raises(EngineError) as e:
+        engine.set_timing(3)
+    assert e.value.code == CM_ERR_INVALID
