@@ -903,13 +903,21 @@ __global__ __launch_bounds__(SMALL_TILE) void k_contract_small(const int64_t *__
     if (copied && rel < wlen && rel + deg > 0) {
       const int r32 = (int)rel;
       const int i0 = r32 < 0 ? -r32 : 0, i1 = deg < wlen - r32 ? deg : wlen - r32;
+      const int32_t *twp = tw ? tw : tc;  // unconditional loads (packed: the weights are in tc)
       for (int i = i0; i < i1; i += COPY_BATCH) {
         int32_t a[COPY_BATCH], w[COPY_BATCH];
 #pragma unroll
         for (int t = 0; t < COPY_BATCH; ++t) {  // clamped loads, issued together
           const int j = i + t < i1 ? i + t : i1 - 1;
           a[t] = tc[s0 + j];
-          w[t] = tw[s0 + j];
+          w[t] = twp[s0 + j];
+        }
+        if (!tw) {  // packed staging: one word per entry
+#pragma unroll
+          for (int t = 0; t < COPY_BATCH; ++t) {
+            w[t] = hc_unpack_w(a[t]);
+            a[t] = hc_unpack_id(a[t]);
+          }
         }
 #pragma unroll
         for (int t = 0; t < COPY_BATCH; ++t)

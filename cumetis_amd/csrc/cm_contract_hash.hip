@@ -245,7 +245,8 @@ __global__ __launch_bounds__(256) void k_hc_dedup_wave(const int32_t *__restrict
                                                        const int32_t *__restrict__ adjncy,
                                                        const int32_t *__restrict__ adjwgt,
                                                        const int32_t *__restrict__ cmap, int32_t *__restrict__ tc,
-                                                       int32_t *__restrict__ tw, int32_t *__restrict__ cdeg)
+                                                       int32_t *__restrict__ tw, int32_t *__restrict__ cdeg,
+                                                       unsigned *ovf)
 {
   constexpr int SLOTS = 1 << LG;
   constexpr int PER = SLOTS / 2 / CM_WAVE;
@@ -305,8 +306,7 @@ __global__ __launch_bounds__(256) void k_hc_dedup_wave(const int32_t *__restrict
       if (hd[t] && cu[t] != c) {
         int r = rk[t];
         if (self_rank >= 0 && r == running - 1) r = self_rank;
-        tc[s0 + r] = cu[t];
-        tw[s0 + r] = ts[sl[t]];
+        hc_stage(tc, tw, s0 + r, cu[t], ts[sl[t]], ovf);
       }
     }
     if (l == 0) cdeg[c] = running - (self_rank >= 0 ? 1 : 0);
@@ -326,7 +326,8 @@ __global__ __launch_bounds__(256) void k_hc_dedup_block(const int32_t *__restric
                                                         const int32_t *__restrict__ adjncy,
                                                         const int32_t *__restrict__ adjwgt,
                                                         const int32_t *__restrict__ cmap, int32_t *__restrict__ tc,
-                                                        int32_t *__restrict__ tw, int32_t *__restrict__ cdeg)
+                                                        int32_t *__restrict__ tw, int32_t *__restrict__ cdeg,
+                                                        unsigned *ovf)
 {
   constexpr int SLOTS = 1 << LG;
   constexpr int PER = SLOTS / 2 / 256;
@@ -391,12 +392,26 @@ __global__ __launch_bounds__(256) void k_hc_dedup_block(const int32_t *__restric
       if (hd[t] && cu[t] != c) {
         int r = rk[t];
         if (self_rank >= 0 && r == running - 1) r = self_rank;
-        tc[s0 + r] = cu[t];
-        tw[s0 + r] = ts[sl[t]];
+        hc_stage(tc, tw, s0 + r, cu[t], ts[sl[t]], ovf);
       }
     }
     if (threadIdx.x == 0) cdeg[c] = running - (self_rank >= 0 ? 1 : 0);
     __syncthreads();
+  }
+}
+
+// Head flag and weight sum of a hub entry.  Unit weights (hw == nullptr):
+// the head byte holds the sum itself (at most 4 without repeated edges; a
+// larger one sets the overflow word and the level is redone).
+__device__ __forceinline__ void hub_mark(uint8_t *head, int32_t *hw, int64_t at, bool h, int32_t sum,
+                                         unsigned *overflow)
+{
+  if (hw) {
+    head[at] = h ? 1 : 0;
+    hw[at] = h ? sum : 0;
+  } else {
+    if (h && sum > 4) *overflow = 1u;
+    head[at] = h ? (uint8_t)(sum > 4 ? 4 : sum) : 0;
   }
 }
 
@@ -531,8 +546,7 @@ __global__ __launch_bounds__(HC_HUB_THREADS) void k_hc_dedup_hub(const HubPart *
         const int32_t i = (int32_t)(e >> lg);
         const int s = (int)(e & (uint32_t)(HC_HUB_SLOTS - 1));
         const bool h = tp[s] == i;
-        head[s0 + i] = h ? 1 : 0;
-        hw[s0 + i] = h ? ts[s] : 0;
+        hub_mark(head, hw, s0 + i, h, ts[s], overflow);
       }
     } else if (bucket) {
       for (int j = wv_id; j < nch; j += NWV) {
@@ -543,8 +557,7 @@ __global__ __launch_bounds__(HC_HUB_THREADS) void k_hc_dedup_hub(const HubPart *
           const int32_t key = bkey[cb + i], pos = bpos[cb + i];
           const int s = hc_find(tk, lg, key);
           const bool h = s >= 0 && tp[s] == pos;
-          head[s0 + pos] = h ? 1 : 0;
-          hw[s0 + pos] = h ? ts[s] : 0;
+          hub_mark(head, hw, s0 + pos, h, h ? ts[s] : 0, overflow);
         }
       }
     } else {
@@ -553,30 +566,29 @@ __global__ __launch_bounds__(HC_HUB_THREADS) void k_hc_dedup_hub(const HubPart *
         if (hc_part(cu, hp.P) != hp.p) continue;
         const int s = hc_find(tk, lg, cu);
         const bool h = s >= 0 && tp[s] == (int32_t)i;
-        head[s0 + i] = h ? 1 : 0;
-        hw[s0 + i] = h ? ts[s] : 0;
+        hub_mark(head, hw, s0 + i, h, h ? ts[s] : 0, overflow);
       }
     }
     __syncthreads();
   }
 }
 
-// Hub chunks: heads per chunk and the rank of the self head inside it.
+// Hub chunks (the dense hub chunk list): heads per chunk and the rank of the
+// self head inside it.
 template <typename OffT>
-__global__ __launch_bounds__(256) void k_hc_hub_count(const SegChunk *__restrict__ chunks, const unsigned *ctr,
+__global__ __launch_bounds__(256) void k_hc_hub_count(const SegChunk *__restrict__ hchunks, const unsigned *ctr,
                                                       const OffT *__restrict__ segoff,
                                                       const uint8_t *__restrict__ head,
                                                       const int32_t *__restrict__ cval, int32_t *__restrict__ ccnt,
                                                       int32_t *__restrict__ cself)
 {
-  const unsigned nq = ctr[HC_N_CHUNK];
+  const unsigned nq = ctr[HC_N_HCHUNK];
   const int l = lane();
   const int64_t nw = (int64_t)gridDim.x * (blockDim.x / CM_WAVE);
   for (int64_t q = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / CM_WAVE; q < nq; q += nw) {
-    const SegChunk ch = chunks[q];
+    const SegChunk ch = hchunks[q];
     const int64_t s0 = (int64_t)segoff[ch.c];
     const int64_t len = (int64_t)segoff[ch.c + 1] - s0;
-    if (len <= HC_BLOCK_MAX) continue;
     const int64_t i0 = (int64_t)ch.k * HC_CHUNK, i1 = len < i0 + HC_CHUNK ? len : i0 + HC_CHUNK;
     int cnt = 0, self = -1;
     for (int64_t ib = i0; ib < i1; ib += EMIT_BATCH * CM_WAVE) {
@@ -604,10 +616,11 @@ __global__ __launch_bounds__(256) void k_hc_hub_count(const SegChunk *__restrict
   }
 }
 
-// One wave per hub: chunk offsets, row length, rank of the self head.
+// One wave per hub: chunk offsets, row length, rank of the self head (all
+// indexed by the dense hub chunk list).
 template <typename OffT>
 __global__ __launch_bounds__(256) void k_hc_hub_scan(const int32_t *__restrict__ hubs, const unsigned *ctr,
-                                                     const SegChunk *__restrict__ chunks,
+                                                     const SegChunk *__restrict__ hchunks,
                                                      const int32_t *__restrict__ ccnt,
                                                      const int32_t *__restrict__ cself, int32_t *__restrict__ coff,
                                                      int32_t *__restrict__ hself, int32_t *__restrict__ htot,
@@ -618,7 +631,7 @@ __global__ __launch_bounds__(256) void k_hc_hub_scan(const int32_t *__restrict__
   const int64_t nw = (int64_t)gridDim.x * (blockDim.x / CM_WAVE);
   for (int64_t h = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / CM_WAVE; h < nh; h += nw) {
     const int32_t first = hubs[h];
-    const SegChunk c0 = chunks[first];
+    const SegChunk c0 = hchunks[first];
     int carry = 0, self = -1;
     for (int k0 = 0; k0 < c0.nch; k0 += CM_WAVE) {
       const int k = k0 + l;
@@ -644,23 +657,22 @@ __global__ __launch_bounds__(256) void k_hc_hub_scan(const int32_t *__restrict__
 // Hub chunks: heads written compacted in position order, the self head
 // dropped and replaced by the last head.
 template <typename OffT>
-__global__ __launch_bounds__(256) void k_hc_hub_write(const SegChunk *__restrict__ chunks, const unsigned *ctr,
+__global__ __launch_bounds__(256) void k_hc_hub_write(const SegChunk *__restrict__ hchunks, const unsigned *ctr,
                                                       const OffT *__restrict__ segoff,
                                                       const uint8_t *__restrict__ head,
                                                       const int32_t *__restrict__ cval, const int32_t *__restrict__ hw,
                                                       const int32_t *__restrict__ coff,
                                                       const int32_t *__restrict__ hself,
                                                       const int32_t *__restrict__ htot, int32_t *__restrict__ tc,
-                                                      int32_t *__restrict__ tw)
+                                                      int32_t *__restrict__ tw, unsigned *ovf)
 {
-  const unsigned nq = ctr[HC_N_CHUNK];
+  const unsigned nq = ctr[HC_N_HCHUNK];
   const int l = lane();
   const int64_t nw = (int64_t)gridDim.x * (blockDim.x / CM_WAVE);
   for (int64_t q = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / CM_WAVE; q < nq; q += nw) {
-    const SegChunk ch = chunks[q];
+    const SegChunk ch = hchunks[q];
     const int64_t s0 = (int64_t)segoff[ch.c];
     const int64_t len = (int64_t)segoff[ch.c + 1] - s0;
-    if (len <= HC_BLOCK_MAX) continue;
     const int self = hself[ch.first], tot = htot[ch.first];
     const int64_t i0 = (int64_t)ch.k * HC_CHUNK, i1 = len < i0 + HC_CHUNK ? len : i0 + HC_CHUNK;
     int r0 = coff[q];
@@ -673,7 +685,7 @@ __global__ __launch_bounds__(256) void k_hc_hub_write(const SegChunk *__restrict
         const uint8_t hb = head[s0 + ii];
         hh[t] = i < i1 && hb;
         cv[t] = cval[s0 + ii];
-        hv[t] = hw[s0 + ii];
+        hv[t] = hw ? hw[s0 + ii] : (int32_t)hb;  // unit weights: the head byte holds the sum
       }
 #pragma unroll
       for (int t = 0; t < EMIT_BATCH; ++t) {
@@ -681,8 +693,7 @@ __global__ __launch_bounds__(256) void k_hc_hub_write(const SegChunk *__restrict
         if (hh[t] && cv[t] != ch.c) {
           int r = r0 + mask_rank(mk);
           if (self >= 0 && r == tot - 1) r = self;
-          tc[s0 + r] = cv[t];
-          tw[s0 + r] = hv[t];
+          hc_stage(tc, tw, s0 + r, cv[t], hv[t], ovf);
         }
         r0 += (int)__popcll(mk);
       }
@@ -691,7 +702,8 @@ __global__ __launch_bounds__(256) void k_hc_hub_write(const SegChunk *__restrict
 }
 
 // ------------------------------------------------------------------ emit
-template <typename OffT>
+// PACKED: the staging holds one word per entry (tw unused).
+template <typename OffT, bool PACKED>
 __global__ __launch_bounds__(256) void k_hc_emit(const SegChunk *__restrict__ chunks, const unsigned *ctr,
                                                  const OffT *__restrict__ segoff, const int32_t *__restrict__ cdeg,
                                                  const OffT *__restrict__ cxadj, const int32_t *__restrict__ tc,
@@ -713,14 +725,19 @@ __global__ __launch_bounds__(256) void k_hc_emit(const SegChunk *__restrict__ ch
       for (int t = 0; t < EMIT_BATCH; ++t) {
         const int64_t i = ib + (int64_t)t * CM_WAVE, ii = i < i1 ? i : i1 - 1;
         a[t] = tc[s0 + ii];
-        w[t] = tw[s0 + ii];
+        if constexpr (!PACKED) w[t] = tw[s0 + ii];
       }
 #pragma unroll
       for (int t = 0; t < EMIT_BATCH; ++t) {
         const int64_t i = ib + (int64_t)t * CM_WAVE;
         if (i < i1) {
-          cadjncy[o0 + i] = a[t];
-          cadjwgt[o0 + i] = w[t];
+          if constexpr (PACKED) {
+            cadjncy[o0 + i] = hc_unpack_id(a[t]);
+            cadjwgt[o0 + i] = hc_unpack_w(a[t]);
+          } else {
+            cadjncy[o0 + i] = a[t];
+            cadjwgt[o0 + i] = w[t];
+          }
         }
       }
     }
@@ -768,7 +785,10 @@ void hash_alloc(cm_ctx *ctx, const Level &L, int64_t nc, HashContract &hc)
   static_assert(HASH_CTR_SLOT + HC_CTR_WORDS <= CTR_SLOTS, "hash counters");
   hc.lists.ctr = ctx->d_ctr + HASH_CTR_SLOT;
   hc.tc = (int32_t *)al(4 * m);
-  hc.tw = (int32_t *)al(4 * m);
+  // unit weights (no weight array) and coarse ids below HC_PACK_MAX_ID:
+  // packed staging, one word per entry
+  const bool unit = !L.adjwgt || L.adjwgt == ctx->ones;  // the shared array of ones is never written
+  hc.tw = !unit || nc >= HC_PACK_MAX_ID ? (int32_t *)al(4 * m) : nullptr;
 }
 
 template <typename OffT>
@@ -798,7 +818,6 @@ void hash_prepare_impl(cm_ctx *ctx, const Level &L, const int32_t *match, const 
     launch(ctx, "hc_select", 0.0, k_hc_select<OffT>, dim3(nblocks(nc, 256 * HC_SELECT)), dim3(256), 0, d_nc,
            segoff, hl);
   }
-  const SegChunk *chunks = hl.chunks;
   const int32_t *wave_items = hl.wave, *wavel_items = hl.wavel, *block_items = hl.block, *blockl_items = hl.blockl,
                 *hubs = hl.hubs;
   const HubPart *parts = hl.parts;
@@ -812,22 +831,22 @@ void hash_prepare_impl(cm_ctx *ctx, const Level &L, const int32_t *match, const 
   launch(ctx, "hc_dedup_wave", 0.0, k_hc_dedup_wave<OffT, 7>,
          dim3((unsigned)std::min<int64_t>(gwave, (nseg + 3) / 4)), dim3(256), 0, (const int32_t *)wave_items,
          (const unsigned *)ctr + HC_N_WAVE, segoff, rep, match, xadj, L.adjncy, L.adjwgt, cmap, hc.tc, hc.tw,
-         cdeg);
+         cdeg, ctr + HC_OVERFLOW);
   if (seg_max > HC_WAVE_SPLIT)
     launch(ctx, "hc_dedup_wave_l", 0.0, k_hc_dedup_wave<OffT, 9>,
            dim3((unsigned)std::min<int64_t>(gwave, (nwavel + 3) / 4)), dim3(256), 0, (const int32_t *)wavel_items,
            (const unsigned *)ctr + HC_N_WAVEL, segoff, rep, match, xadj, L.adjncy, L.adjwgt, cmap, hc.tc, hc.tw,
-           cdeg);
+           cdeg, ctr + HC_OVERFLOW);
   static_assert(2 * HC_BLOCK_SPLIT == 1 << 11 && 2 * HC_BLOCK_MAX == 1 << 12, "block table sizes");
   if (seg_max > HC_WAVE_MAX)
     launch(ctx, "hc_dedup_block", 0.0, k_hc_dedup_block<OffT, 11>, dim3((unsigned)std::min<int64_t>(4096, nblk)),
            dim3(256), 0, (const int32_t *)block_items, (const unsigned *)ctr + HC_N_BLOCK, segoff, rep, match,
-           xadj, L.adjncy, L.adjwgt, cmap, hc.tc, hc.tw, cdeg);
+           xadj, L.adjncy, L.adjwgt, cmap, hc.tc, hc.tw, cdeg, ctr + HC_OVERFLOW);
   if (seg_max > HC_BLOCK_SPLIT)
     launch(ctx, "hc_dedup_block_l", 0.0, k_hc_dedup_block<OffT, 12>,
            dim3((unsigned)std::min<int64_t>(2048, nblkl)), dim3(256), 0, (const int32_t *)blockl_items,
            (const unsigned *)ctr + HC_N_BLOCKL, segoff, rep, match, xadj, L.adjncy, L.adjwgt, cmap, hc.tc, hc.tw,
-           cdeg);
+           cdeg, ctr + HC_OVERFLOW);
   if (m > HC_BLOCK_MAX && seg_max > HC_BLOCK_MAX) {  // a hub needs more than HC_BLOCK_MAX entries
     // unit weights (adjwgt == nullptr): no weight copies; the dedup counts 1 per entry
     int32_t *cval = (int32_t *)al(4 * m), *wv = L.adjwgt ? (int32_t *)al(4 * m) : nullptr;
@@ -838,21 +857,25 @@ void hash_prepare_impl(cm_ctx *ctx, const Level &L, const int32_t *match, const 
            0, (const SegChunk *)hl.hchunks, (const unsigned *)ctr, segoff, rep, match, xadj, L.adjncy, L.adjwgt, cmap,
            cval, wv, bpos, bkey, bw, coff_b);
     uint8_t *head = (uint8_t *)al(m);
-    int32_t *hw = (int32_t *)al(4 * m);
-    int32_t *ccnt = (int32_t *)al(4 * nchunk), *cself = (int32_t *)al(4 * nchunk), *coff = (int32_t *)al(4 * nchunk);
-    int32_t *hself = (int32_t *)al(4 * nchunk), *htot = (int32_t *)al(4 * nchunk);
+    int32_t *hw = L.adjwgt ? (int32_t *)al(4 * m) : nullptr;  // unit weights: sums in the head bytes
+    // per hub chunk (indexed by the dense hub chunk list)
+    const int64_t nhc = b.nhchunk;
+    int32_t *ccnt = (int32_t *)al(4 * nhc), *cself = (int32_t *)al(4 * nhc), *coff = (int32_t *)al(4 * nhc);
+    int32_t *hself = (int32_t *)al(4 * nhc), *htot = (int32_t *)al(4 * nhc);
+    const unsigned ghc = (unsigned)std::min<int64_t>(4096, (nhc + 3) / 4);
     launch(ctx, "hc_dedup_hub", 0.0, k_hc_dedup_hub<OffT>, dim3((unsigned)std::min<int64_t>(1024, npart)),
            dim3(HC_HUB_THREADS), 0, (const HubPart *)parts, (const unsigned *)ctr, segoff, (const int32_t *)cval,
            (const int32_t *)wv, (const int32_t *)bpos, (const int32_t *)bkey, (const int32_t *)bw,
            (const uint16_t *)coff_b, head, hw, ctr + HC_OVERFLOW);
-    launch(ctx, "hc_hub_count", 0.0, k_hc_hub_count<OffT>, dim3(gchunk), dim3(256), 0, chunks,
+    launch(ctx, "hc_hub_count", 0.0, k_hc_hub_count<OffT>, dim3(ghc), dim3(256), 0, (const SegChunk *)hl.hchunks,
            (const unsigned *)ctr, segoff, (const uint8_t *)head, (const int32_t *)cval, ccnt, cself);
     launch(ctx, "hc_hub_scan", 0.0, k_hc_hub_scan<OffT>, dim3((unsigned)std::min<int64_t>(1024, (nhub + 3) / 4)),
-           dim3(256), 0, (const int32_t *)hubs, (const unsigned *)ctr, chunks, (const int32_t *)ccnt,
+           dim3(256), 0, (const int32_t *)hubs, (const unsigned *)ctr, (const SegChunk *)hl.hchunks, (const int32_t *)ccnt,
            (const int32_t *)cself, coff, hself, htot, cdeg);
-    launch(ctx, "hc_hub_write", 0.0, k_hc_hub_write<OffT>, dim3(gchunk), dim3(256), 0, chunks,
+    launch(ctx, "hc_hub_write", 0.0, k_hc_hub_write<OffT>, dim3(ghc), dim3(256), 0, (const SegChunk *)hl.hchunks,
            (const unsigned *)ctr, segoff, (const uint8_t *)head, (const int32_t *)cval, (const int32_t *)hw,
-           (const int32_t *)coff, (const int32_t *)hself, (const int32_t *)htot, hc.tc, hc.tw);
+           (const int32_t *)coff, (const int32_t *)hself, (const int32_t *)htot, hc.tc, hc.tw,
+           ctr + HC_OVERFLOW);
   }
 }
 
@@ -861,14 +884,14 @@ void hash_emit_impl(cm_ctx *ctx, const OffT *segoff, const int32_t *cdeg, const 
                     int32_t *cadjwgt, const HashContract &hc)
 {
   if (hc.seg_max < 0 || hc.seg_max > TILE_COPY_SEG)
-    launch(ctx, "hc_emit", 0.0, k_hc_emit<OffT>, dim3(hc.gchunk), dim3(256), 0, (const SegChunk *)hc.lists.chunks,
-         (const unsigned *)hc.lists.ctr, segoff, cdeg, cxadj, (const int32_t *)hc.tc, (const int32_t *)hc.tw, cadjncy,
-         cadjwgt);
+    launch(ctx, "hc_emit", 0.0, hc.tw ? k_hc_emit<OffT, false> : k_hc_emit<OffT, true>, dim3(hc.gchunk), dim3(256),
+           0, (const SegChunk *)hc.lists.chunks, (const unsigned *)hc.lists.ctr, segoff, cdeg, cxadj,
+           (const int32_t *)hc.tc, (const int32_t *)hc.tw, cadjncy, cadjwgt);
 }
 
 const unsigned *hash_overflow_flag(const HashContract &hc)
 {
-  return hc.seg_max < 0 || hc.seg_max > HC_BLOCK_MAX ? hc.lists.ctr + HC_OVERFLOW : nullptr;
+  return hc.seg_max < 0 || hc.seg_max > HC_BLOCK_MAX || !hc.tw ? hc.lists.ctr + HC_OVERFLOW : nullptr;
 }
 
 template void hash_prepare_impl<int32_t>(cm_ctx *, const Level &, const int32_t *, const int32_t *, int64_t,

@@ -97,8 +97,8 @@ __device__ __forceinline__ void hc_write(int32_t c, int64_t len, unsigned (&at)[
   at[HC_N_BLOCK] += cb ? 1u : 0u;
   at[HC_N_BLOCKL] += cbl ? 1u : 0u;
   if (hub) {
-    hl.hubs[at[HC_N_HUB]] = (int32_t)first;  // a hub is named by its first chunk
     const unsigned hf = at[HC_N_HCHUNK];
+    hl.hubs[at[HC_N_HUB]] = (int32_t)hf;  // a hub is named by its first chunk in hchunks
     for (int k = 0; k < nch; ++k) hl.hchunks[hf + k] = SegChunk{c, (int32_t)hf, nch, k};
     const int P = (int)((len + HC_PART - 1) / HC_PART);
     for (int p = 0; p < P; ++p) hl.parts[at[HC_N_PART] + p] = HubPart{c, p, P, (int32_t)hf};
@@ -139,11 +139,38 @@ __device__ __forceinline__ void hc_reserve(const unsigned (&cnt)[HC_N_COUNTERS],
   }
 }
 
+// Packed staging.  On a level with unit edge weights a coarse neighbour's
+// weight counts the fine edges between the two pairs: at most 4 in a graph
+// without repeated edges.  Such a level stages a hashed row as one word per
+// entry, (weight - 1) in the top two bits above the coarse id (the weight
+// array is then not used); a larger weight (repeated edges in the input)
+// sets the overflow word and the level is redone on the sort-based path.
+constexpr int HC_PACK_SHIFT = 30;
+constexpr int64_t HC_PACK_MAX_ID = (int64_t)1 << HC_PACK_SHIFT;  // coarse ids below this
+__device__ __forceinline__ int32_t hc_pack(int32_t id, int32_t w)
+{
+  return (int32_t)(((uint32_t)(w - 1) << HC_PACK_SHIFT) | (uint32_t)id);
+}
+__device__ __forceinline__ int32_t hc_unpack_id(int32_t x) { return x & (int32_t)(HC_PACK_MAX_ID - 1); }
+__device__ __forceinline__ int32_t hc_unpack_w(int32_t x) { return (int32_t)((uint32_t)x >> HC_PACK_SHIFT) + 1; }
+// One staged entry: both words, or the packed one (tw == nullptr; a weight
+// above 4 sets *ovf).
+__device__ __forceinline__ void hc_stage(int32_t *tc, int32_t *tw, int64_t at, int32_t id, int32_t w, unsigned *ovf)
+{
+  if (tw) {
+    tc[at] = id;
+    tw[at] = w;
+  } else {
+    if (w > 4) *ovf = 1u;
+    tc[at] = hc_pack(id, w);
+  }
+}
+
 // Device state between the two halves: everything the emit step reads.
 struct HashContract {
   std::vector<void *> blocks;  // pool blocks to release
   HcLists lists;
-  int32_t *tc = nullptr, *tw = nullptr;  // compacted rows, at the segment offsets
+  int32_t *tc = nullptr, *tw = nullptr;  // compacted rows, at the segment offsets (tw null: packed)
   unsigned gchunk = 1;
   int64_t seg_max = -1;   // bound on the segment length (-1 unknown)
   int64_t nc_bound = 0;   // the lists hold this many coarse vertices at most

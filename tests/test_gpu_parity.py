@@ -210,6 +210,45 @@ def test_contract_unit_weights(engine, orc, name, order):
     assert cw.sum() == g.m - int((cmap[np.repeat(np.arange(g.n), np.diff(g.xadj))] == cmap[g.adjncy]).sum())
 
 
+def test_contract_unit_weights_repeated_edges(engine, orc):
+    """Unit weights stage hashed rows packed (a coarse edge weight of at most
+    4 beside the id), which holds only without repeated edges.  A graph
+    whose rows repeat some neighbours 5 times gives coarse weights above 4:
+    the packed pass flags it and the level is redone on the sorted path,
+    still equal to the oracle."""
+    from cases import hub_graph
+    g0 = hub_graph(seed=31, side=120, hub_deg=6000, mids=(300, 100))
+    n = g0.n
+    rows = [list(g0.adjncy[g0.xadj[i]:g0.xadj[i + 1]]) for i in range(n)]
+    for a in (n - 1, n - 2):  # the two mid rows (hashed segments)
+        for b in rows[a][:5]:
+            rows[a] += [b] * 4
+            rows[b] += [a] * 4
+    xadj = np.zeros(n + 1, np.int64)
+    np.cumsum([len(r) for r in rows], out=xadj[1:])
+    adj = np.concatenate([np.asarray(r, np.int32) for r in rows])
+    g = Graph(xadj.astype(np.int32), adj, None, None)
+    gw = with_weights(g)
+    say(f"[contract repeated edges] n={g.n} m={g.m}")
+    mv = maxv_for(gw, orc=orc)
+    match, _ = orc.handshake_match(gw.xadj, gw.adjncy, gw.adjwgt, gw.vwgt, mv, seed=5)
+    cmap, nc = orc.number(match)
+    engine.set_timing(True)
+    engine.reset_stats()
+    try:
+        cx, ca, cw, cv = engine.contract(g, match, cmap, nc, CM_ROW_FIRST_SEEN)
+        stats = engine.kernel_stats()
+    finally:
+        engine.set_timing(False)
+    assert "hc_dedup_block" in stats and "contract_gather" in stats  # packed pass, then the sorted path
+    ex, ea, ew, ev = orc.cpu_contract(gw.xadj, gw.adjncy, gw.adjwgt, gw.vwgt, match, cmap, nc)
+    assert ew.max() > 4
+    np.testing.assert_array_equal(cx.astype(np.int64), ex)
+    np.testing.assert_array_equal(cv, ev)
+    np.testing.assert_array_equal(ca, ea)
+    np.testing.assert_array_equal(cw, ew)
+
+
 @pytest.mark.parametrize("name", ["rmat20", "rmat20_off64", "grid1024"])
 def test_match_frontier(engine, orc, name):
     """Levels of 2^20 vertices or more list what is still unmatched after
