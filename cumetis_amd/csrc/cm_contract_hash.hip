@@ -145,7 +145,8 @@ __global__ __launch_bounds__(256) void k_hc_select(const int64_t *__restrict__ n
 // One block per hub chunk (HC_CHUNK entries, 8 per thread).  Every entry's
 // coarse neighbour id goes to cval in segment order.  For hubs of at most
 // HC_PMAX partitions the chunk's entries are also counting-sorted by key
-// partition into bpos/bkey/bw (at the chunk's own segment positions), with
+// partition into bpos/bkey/bw (at the chunk's own segment positions; bpos
+// holds the position inside the chunk, which has HC_CHUNK <= 2^16 entries), with
 // the partition starts in coff[q * (HC_PMAX + 1) + p] (coff[.. + P] = the
 // chunk's length); the order inside a partition's slice is arbitrary (the
 // partition pass takes minima and sums).  Longer hubs keep their weights in
@@ -158,7 +159,7 @@ __global__ __launch_bounds__(256) void k_hc_gather(const SegChunk *__restrict__ 
                                                    const int32_t *__restrict__ adjncy,
                                                    const int32_t *__restrict__ adjwgt,
                                                    const int32_t *__restrict__ cmap, int32_t *__restrict__ cval,
-                                                   int32_t *__restrict__ wv, int32_t *__restrict__ bpos,
+                                                   int32_t *__restrict__ wv, uint16_t *__restrict__ bpos,
                                                    int32_t *__restrict__ bkey, int32_t *__restrict__ bw,
                                                    uint16_t *__restrict__ coff)
 {
@@ -224,7 +225,7 @@ __global__ __launch_bounds__(256) void k_hc_gather(const SegChunk *__restrict__ 
       const int i = t * 256 + tid;
       if (i < cn) {
         const int64_t o = s0 + i0 + (int64_t)(s_cnt[pp[t]] + rk[t]);
-        bpos[o] = (int32_t)(i0 + i);
+        bpos[o] = (uint16_t)i;
         bkey[o] = cv[t];
         if (bw) bw[o] = w[t];  // none for unit weights
       }
@@ -430,7 +431,7 @@ __global__ __launch_bounds__(HC_HUB_THREADS) void k_hc_dedup_hub(const HubPart *
                                                                  const OffT *__restrict__ segoff,
                                                                  const int32_t *__restrict__ cval,
                                                                  const int32_t *__restrict__ wv,
-                                                                 const int32_t *__restrict__ bpos,
+                                                                 const uint16_t *__restrict__ bpos,
                                                                  const int32_t *__restrict__ bkey,
                                                                  const int32_t *__restrict__ bw,
                                                                  const uint16_t *__restrict__ coff,
@@ -520,7 +521,7 @@ __global__ __launch_bounds__(HC_HUB_THREADS) void k_hc_dedup_hub(const HubPart *
           else hi = mid - 1;
         }
         const int64_t o = ok ? s0 + (int64_t)lo * HC_CHUNK + s_sb[lo] + (f - s_so[lo]) : s0;
-        take(ok, bkey[o], bpos[o], bw ? bw[o] : 1);
+        take(ok, bkey[o], lo * HC_CHUNK + (int32_t)bpos[o], bw ? bw[o] : 1);
       }
     } else {
       for (int64_t ib = threadIdx.x; ib < len; ib += SCAN * T) {
@@ -554,7 +555,7 @@ __global__ __launch_bounds__(HC_HUB_THREADS) void k_hc_dedup_hub(const HubPart *
         const int b = co[hp.p], e = co[hp.p + 1];
         const int64_t cb = s0 + (int64_t)j * HC_CHUNK;
         for (int i = b + l; i < e; i += CM_WAVE) {
-          const int32_t key = bkey[cb + i], pos = bpos[cb + i];
+          const int32_t key = bkey[cb + i], pos = j * HC_CHUNK + (int32_t)bpos[cb + i];
           const int s = hc_find(tk, lg, key);
           const bool h = s >= 0 && tp[s] == pos;
           hub_mark(head, hw, s0 + pos, h, h ? ts[s] : 0, overflow);
@@ -823,6 +824,7 @@ void hash_prepare_impl(cm_ctx *ctx, const Level &L, const int32_t *match, const 
   const HubPart *parts = hl.parts;
   const unsigned gchunk = (unsigned)std::min<int64_t>(4096, (nchunk + 3) / 4);
   hc.gchunk = gchunk;
+  static_assert(HC_CHUNK <= 1 << 16, "chunk positions in 16 bits");
   static_assert(2 * HC_WAVE_SPLIT == 1 << 7 && 2 * HC_WAVE_MAX == 1 << 9, "wave table sizes");
   // the wave passes stride over their device-side item counts: a small
   // level (whose counts are far below the bounds) dispatches fewer idle
@@ -850,7 +852,8 @@ void hash_prepare_impl(cm_ctx *ctx, const Level &L, const int32_t *match, const 
   if (m > HC_BLOCK_MAX && seg_max > HC_BLOCK_MAX) {  // a hub needs more than HC_BLOCK_MAX entries
     // unit weights (adjwgt == nullptr): no weight copies; the dedup counts 1 per entry
     int32_t *cval = (int32_t *)al(4 * m), *wv = L.adjwgt ? (int32_t *)al(4 * m) : nullptr;
-    int32_t *bpos = (int32_t *)al(4 * m), *bkey = (int32_t *)al(4 * m);
+    uint16_t *bpos = (uint16_t *)al(2 * m);
+    int32_t *bkey = (int32_t *)al(4 * m);
     int32_t *bw = L.adjwgt ? (int32_t *)al(4 * m) : nullptr;
     uint16_t *coff_b = (uint16_t *)al(sizeof(uint16_t) * (HC_PMAX + 1) * b.nhchunk);
     launch(ctx, "hc_gather", 0.0, k_hc_gather<OffT>, dim3((unsigned)std::min<int64_t>(8192, b.nhchunk)), dim3(256),
@@ -865,7 +868,7 @@ void hash_prepare_impl(cm_ctx *ctx, const Level &L, const int32_t *match, const 
     const unsigned ghc = (unsigned)std::min<int64_t>(4096, (nhc + 3) / 4);
     launch(ctx, "hc_dedup_hub", 0.0, k_hc_dedup_hub<OffT>, dim3((unsigned)std::min<int64_t>(1024, npart)),
            dim3(HC_HUB_THREADS), 0, (const HubPart *)parts, (const unsigned *)ctr, segoff, (const int32_t *)cval,
-           (const int32_t *)wv, (const int32_t *)bpos, (const int32_t *)bkey, (const int32_t *)bw,
+           (const int32_t *)wv, (const uint16_t *)bpos, (const int32_t *)bkey, (const int32_t *)bw,
            (const uint16_t *)coff_b, head, hw, ctr + HC_OVERFLOW);
     launch(ctx, "hc_hub_count", 0.0, k_hc_hub_count<OffT>, dim3(ghc), dim3(256), 0, (const SegChunk *)hl.hchunks,
            (const unsigned *)ctr, segoff, (const uint8_t *)head, (const int32_t *)cval, ccnt, cself);
