@@ -753,7 +753,10 @@ constexpr int SMALL_TILE = 256;    // coarse vertices per tile of k_contract_sma
 constexpr int SMALL_STAGE = 4096;  // output entries staged in LDS per window
 constexpr int COPY_BATCH = 8;      // staged entries per step of a tile-copied hashed row
 
-template <bool FS, typename OffT>
+// COPY: hashed rows are copied from the staging area by the tile (tc/tw
+// given); without it the copy path is compiled out, which frees registers
+// for more waves per SIMD.
+template <bool FS, typename OffT, bool COPY = false>
 __global__ __launch_bounds__(SMALL_TILE) void k_contract_small(const int64_t *__restrict__ ncp,
                                                         const OffT *__restrict__ segoff,
                                                         const int32_t *__restrict__ rep,
@@ -851,7 +854,7 @@ __global__ __launch_bounds__(SMALL_TILE) void k_contract_small(const int64_t *__
   // window (hashed rows in it, written by the emit pass) moves each window to
   // the first entry written at or after the end of the previous one, so the
   // hashed rows' ranges are not walked.
-  const bool copied = tc != nullptr && c < nc && !small && len <= TILE_COPY_SEG;
+  const bool copied = COPY && c < nc && !small && len <= TILE_COPY_SEG;
   const bool writes = (small || copied) && deg > 0;
   const uint64_t my_end = writes ? ex + (uint64_t)deg : 0;
   uint64_t span = wave_max<uint64_t>(my_end);
@@ -900,7 +903,7 @@ __global__ __launch_bounds__(SMALL_TILE) void k_contract_small(const int64_t *__
         }
       }
     }
-    if (copied && rel < wlen && rel + deg > 0) {
+    if (COPY && copied && rel < wlen && rel + deg > 0) {
       const int r32 = (int)rel;
       const int i0 = r32 < 0 ? -r32 : 0, i1 = deg < wlen - r32 ? deg : wlen - r32;
       const int32_t *twp = tw ? tw : tc;  // unconditional loads (packed: the weights are in tc)
@@ -998,7 +1001,9 @@ static int64_t contract_impl(cm_ctx *ctx, const Level &L, const int32_t *match, 
       // the totals come back with this kernel (the emit pass only copies rows)
       ctx->h_small[6] = 0;
       const Readback rb = readback_begin(ctx, 0, 5, hashed ? hash_overflow_flag(hc) : nullptr);
-      launch(ctx, "contract_small", small_bytes, k_contract_small<true, OffT>, dim3((unsigned)tiles), dim3(SMALL_TILE),
+      launch(ctx, "contract_small", small_bytes,
+             tile_copy ? k_contract_small<true, OffT, true> : k_contract_small<true, OffT, false>, dim3((unsigned)tiles),
+             dim3(SMALL_TILE),
              0, d_nc, segoff, rep, match, xadj, L.adjncy, L.adjwgt, cmap, (const int32_t *)cdeg,
              tile_copy ? (const int32_t *)hc.tc : nullptr, tile_copy ? (const int32_t *)hc.tw : nullptr, cxadj,
              cadjncy, cadjwgt, ctx->d_status, ctx->d_ctr, ctx->d_small + 1, ep, rb);
