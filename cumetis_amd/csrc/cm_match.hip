@@ -47,6 +47,9 @@ constexpr int SELECT_PER_THREAD = 16;
 constexpr int MID_GROUP = 16;  // rows per wave of the lane-group classes
 constexpr int WAVE_GROUP = 16;  // rows per wave of the whole-wave (mid) class
 constexpr int ROUND_BATCH = MATCH_BATCH;  // launches queued between host checks
+// Row entries per scan step on levels without long rows (8 was slower on
+// the grid: a short row then loads more clamped duplicates).
+constexpr int MESH_BATCH = 4;
 
 // flags[2r] = 1 if a vertex matched or died in round r, flags[2r+1] = 1 if
 // any vertex is still unmatched after round r.  Round r's accept runs in
@@ -293,7 +296,7 @@ __device__ __forceinline__ bool accepted_prev(const int32_t *prev, int32_t u)
 // Best eligible neighbour of v among vertices unmatched after round r-1, by
 // the strict order (adjwgt, hash(seed,min,max), min, max); -2 if none.
 // Lanes [l0, l0+step) stride the row; the caller reduces across lanes.
-template <typename OffT>
+template <typename OffT, int SCAN_BATCH = 4>
 __device__ __forceinline__ void scan_row(int64_t v, OffT b, OffT e, int l0, int step,
                                          const int32_t *__restrict__ adjncy,
                                          const int32_t *__restrict__ adjwgt,
@@ -307,7 +310,6 @@ __device__ __forceinline__ void scan_row(int64_t v, OffT b, OffT e, int l0, int 
   // Loads go through clamped indices (the last entry of the range, or v
   // itself) and are unconditional: conditional loads compile to a branch and
   // a wait each, which serialises the batch.
-  constexpr int SCAN_BATCH = 4;
   for (OffT j0 = b + l0; j0 < e; j0 += (OffT)SCAN_BATCH * step) {
     int32_t u[SCAN_BATCH], mu[SCAN_BATCH], wu[SCAN_BATCH];
     uint32_t w[SCAN_BATCH];
@@ -597,7 +599,7 @@ __device__ __forceinline__ void small_group(int64_t m0, unsigned ns, const int32
 // launch initialises itself: its scans read this word instead of match[0].
 __device__ const int32_t unmatched_word = -1;
 
-template <typename OffT, bool INIT = false>
+template <typename OffT, bool INIT = false, int SB = 4>
 __device__ __forceinline__ void short_row(int64_t v, int64_t n, const OffT *__restrict__ xadj,
                                           const int32_t *__restrict__ adjncy, const int32_t *__restrict__ adjwgt,
                                           const int32_t *__restrict__ vwgt, const MatchOwners &own, int32_t *match,
@@ -647,7 +649,7 @@ __device__ __forceinline__ void short_row(int64_t v, int64_t n, const OffT *__re
         owner_params(own, v, maxv, seed, base);
         int32_t best = -2;
         uint64_t bk1 = 0, bk2 = 0;
-        scan_row<OffT>(v, b, e, 0, 1, adjncy, adjwgt, vwgt, vw, maxv, seed, base, INIT ? &unmatched_word : match,
+        scan_row<OffT, SB>(v, b, e, 0, 1, adjncy, adjwgt, vwgt, vw, maxv, seed, base, INIT ? &unmatched_word : match,
                        prev, have_prev, maxvw < 0 || (int64_t)vw + maxvw > maxv, own.wunit, best, bk1, bk2);
         cur[v] = best;
       }
@@ -660,7 +662,11 @@ __device__ __forceinline__ void short_row(int64_t v, int64_t n, const OffT *__re
 // slist[blockIdx.x * VREGION ..] with vcount[blockIdx.x] entries;
 // the remaining blocks take the chunks of hub rows and the row groups of the
 // classes, one wave per item.
-template <typename OffT, bool LIST, bool INIT = false>
+// MESH: a level without rows above LONG_DEG entries (every row on the
+// thread-per-row path, no long-row lists): the long-row paths are compiled
+// out, which leaves the kernel without SGPR spills (the full round kernel
+// spills 34 to VGPR lanes); grid -3.6%.
+template <typename OffT, bool LIST, bool INIT = false, bool MESH = false>
 __device__ __forceinline__ void round_body(int64_t n, const OffT *__restrict__ xadj,
                                            const int32_t *__restrict__ adjncy, const int32_t *__restrict__ adjwgt,
                                            const int32_t *__restrict__ vwgt, const MatchOwners &own, int32_t *match,
@@ -692,7 +698,8 @@ __device__ __forceinline__ void round_body(int64_t n, const OffT *__restrict__ x
         short_row<OffT>(list[i], n, xadj, adjncy, adjwgt, vwgt, own, match, prev, cur, have_prev, propose,
                         maxvw, short_deg, changed, active);
     } else {
-      short_row<OffT, INIT>((int64_t)blockIdx.x * blockDim.x + threadIdx.x, n, xadj, adjncy, adjwgt, vwgt, own,
+      short_row<OffT, INIT, MESH ? MESH_BATCH : 4>((int64_t)blockIdx.x * blockDim.x + threadIdx.x, n, xadj, adjncy,
+                                                   adjwgt, vwgt, own,
                             match, prev, cur, have_prev, propose, maxvw, short_deg, changed, active);
     }
     if (have_prev) {
@@ -712,7 +719,8 @@ __device__ __forceinline__ void round_body(int64_t n, const OffT *__restrict__ x
     }
     return;
   }
-  if (INIT || !propose) return;  // INIT: the level has no long rows
+  if constexpr (INIT || MESH) return;  // the level has no long rows
+  if (!propose) return;
   // Long rows.  Work items in order: the hub chunks (one wave each), then
   // the row groups of each class in class order (one wave per group).
   // Whether a row is scanned depends only on state fixed before this launch
@@ -762,6 +770,22 @@ __global__ __launch_bounds__(256, 7) void k_round(int64_t n, const OffT *__restr
                           nshort, propose, maxvw, short_deg, nullptr, nullptr);
 }
 
+// Launch r of a level without long rows (thread-per-row path only).
+template <typename OffT>
+__global__ __launch_bounds__(256, 7) void k_round_mesh(int64_t n, const OffT *__restrict__ xadj,
+                                                    const int32_t *__restrict__ adjncy,
+                                                    const int32_t *__restrict__ adjwgt,
+                                                    const int32_t *__restrict__ vwgt, MatchOwners own,
+                                                    int32_t *match, const int32_t *prev, int32_t *cur,
+                                                    const LongChunk *chunks, RowLists rl, const unsigned *nchunks,
+                                                    LongPartial part, unsigned *flags, int r, unsigned nshort,
+                                                    int propose, int64_t maxvw, int short_deg, const int64_t *np)
+{
+  if (np) n = *np;  // queued ahead: the level's count was not yet on the host
+  round_body<OffT, false, false, true>(n, xadj, adjncy, adjwgt, vwgt, own, match, prev, cur, chunks, rl, nchunks, part,
+                                       flags, r, nshort, propose, maxvw, short_deg, nullptr, nullptr);
+}
+
 // Launch 0 (round 0 proposals only: no accept, no prev), compiled for r = 0.
 // INIT: a level without long rows, whose launch 0 also does the work of
 // k_match_init (the level's first launch then starts the rounds at once).
@@ -776,7 +800,7 @@ __global__ __launch_bounds__(256, 7) void k_round0(int64_t n, const OffT *__rest
                                                 unsigned *hzero, const int64_t *np)
 {
   if (np) n = *np;  // queued ahead: the level's count was not yet on the host
-  round_body<OffT, false, INIT>(n, xadj, adjncy, adjwgt, vwgt, own, match, prev, cur, chunks, rl, nchunks, part,
+  round_body<OffT, false, INIT, INIT>(n, xadj, adjncy, adjwgt, vwgt, own, match, prev, cur, chunks, rl, nchunks, part,
                                 flags, 0, nshort, 1, maxvw, short_deg, nullptr, nullptr, nflags, hzero);
 }
 
@@ -939,7 +963,8 @@ static void match_launch_direct(cm_ctx *ctx, MatchRun &mr, int count)
              mr.match, const_cast<int32_t *>(prv), cur, chunks, mr.rl, counts, mr.part, mr.flags, gshort, L.maxvw,
              mr.short_deg, 2 * (int64_t)mr.max_rounds + 16, ctx->d_ctr + HASH_CTR_SLOT, mr.d_n);
     else
-      launch(ctx, "match_round", r == 0 ? row_bytes : 0.0, k_round<OffT>, dim3(gshort + glong), dim3(256), 0, n,
+      launch(ctx, "match_round", r == 0 ? row_bytes : 0.0, mr.fused_init ? k_round_mesh<OffT> : k_round<OffT>,
+             dim3(gshort + glong), dim3(256), 0, n,
              (const OffT *)L.xadj, L.adjncy, L.adjwgt, L.vwgt, mr.own, mr.match, prv, cur, chunks, mr.rl, counts,
              mr.part, mr.flags, r, gshort, r < mr.max_rounds ? 1 : 0, L.maxvw, mr.short_deg, mr.d_n);
     if (mr.ghub && r < mr.max_rounds)
