@@ -45,9 +45,9 @@ def _digest(src: Path, flags) -> str:
     return h.hexdigest()[:16]
 
 
-def _compile(src: Path, flags) -> Path:
+def _compile(src: Path, flags, force: bool = False) -> Path:
     obj = BUILD / f"{src.stem}-{_digest(src, flags)}.o"
-    if obj.exists():
+    if obj.exists() and not force:
         return obj
     cmd = [HIPCC, *flags, "-c", str(src), "-o", str(obj) + ".tmp"]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -57,14 +57,16 @@ def _compile(src: Path, flags) -> Path:
     return obj
 
 
-def build(verbose: bool = False) -> Path:
+def build(verbose: bool = False, force: bool = False) -> Path:
+    """Compile and link; force recompiles every object (otherwise objects
+    whose source, headers and flags are unchanged are reused)."""
     BUILD.mkdir(exist_ok=True)
     hip, cpp = _sources()
     jobs = [(s, COMMON + DEVICE + ["-x", "hip"]) for s in hip]
     jobs += [(s, COMMON + ["-x", "c++"]) for s in cpp if s.name != "cm_cli.cpp"]
     workers = min(8, max(1, len(jobs)))
     with cf.ThreadPoolExecutor(workers) as ex:
-        objs = list(ex.map(lambda j: _compile(*j), jobs))
+        objs = list(ex.map(lambda j: _compile(*j, force=force), jobs))
     tmp = LIB.with_suffix(".so.tmp")
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -73,7 +75,7 @@ def build(verbose: bool = False) -> Path:
     os.replace(tmp, LIB)
     cli_src = CSRC / "host" / "cm_cli.cpp"
     if cli_src.exists():
-        cli_obj = _compile(cli_src, COMMON + ["-x", "c++"])
+        cli_obj = _compile(cli_src, COMMON + ["-x", "c++"], force=force)
         cmd = [HIPCC, "-o", str(CLI), str(cli_obj), f"-L{PKG}", "-lcumetis_amd",
                f"-Wl,-rpath,$ORIGIN"]
         r = subprocess.run(cmd, capture_output=True, text=True)
@@ -86,7 +88,7 @@ def build(verbose: bool = False) -> Path:
 
 def build_oracle() -> Path:
     """Compile the CPU oracle (test infrastructure) with its Makefile."""
-    r = subprocess.run(["make", "-s", "-C", str(ROOT / "oracle")], capture_output=True, text=True)
+    r = subprocess.run(["make", "-s", "-B", "-C", str(ROOT / "oracle")], capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"oracle build failed\n{r.stdout}\n{r.stderr}")
     return ROOT / "oracle" / "liboracle.so"
