@@ -73,15 +73,44 @@ __global__ __launch_bounds__(256) void k_check_batch(int64_t N, int64_t M, const
   if (f) atomicOr(bad, f);
 }
 
-// level 0: local -> global neighbour ids, and the graph slot of each vertex
+// level 0: local -> global neighbour ids (adj_global may be adj_local).
+// Thread t of a block takes entries base + t + k * 256 (k < GLOB_PER): one
+// slot search for the first, then the slot advances with the entries.
+constexpr int GLOB_PER = 8;
 __global__ __launch_bounds__(256) void k_globalize(int64_t m, const int32_t *adj_local,
                                                    const int64_t *eoff, const int64_t *voff, int nb,
                                                    int32_t *adj_global)
 {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= m) return;
-  const int s = find_slot(eoff, nb, e);
-  adj_global[e] = (int32_t)(adj_local[e] + voff[s]);
+  const int64_t e0 = (int64_t)blockIdx.x * 256 * GLOB_PER + threadIdx.x;
+  if (e0 >= m) return;
+  int s = find_slot(eoff, nb, e0);
+#pragma unroll
+  for (int k = 0; k < GLOB_PER; ++k) {
+    const int64_t e = e0 + (int64_t)k * 256;
+    if (e >= m) break;
+    while (eoff[s + 1] <= e) ++s;
+    adj_global[e] = (int32_t)(adj_local[e] + voff[s]);
+  }
+}
+
+// Total vertex weight of every graph (one block per graph, grids stride).
+__global__ __launch_bounds__(1024) void k_graph_vw_sums(int nb, const int64_t *voff, const int32_t *vw,
+                                                        int64_t *sums)
+{
+  __shared__ int64_t sh[16];
+  for (int b = blockIdx.x; b < nb; b += gridDim.x) {
+    int64_t acc = 0;
+    for (int64_t i = voff[b] + threadIdx.x; i < voff[b + 1]; i += 1024) acc += vw[i];
+    acc = wave_sum<int64_t>(acc);
+    if (lane() == 0) sh[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int64_t t = 0;
+      for (int w = 0; w < 16; ++w) t += sh[w];
+      sums[b] = t;
+    }
+    __syncthreads();
+  }
 }
 
 __global__ __launch_bounds__(256) void k_slots(int64_t n, const int64_t *voff, int nb, int32_t *gid)
@@ -131,12 +160,19 @@ __global__ __launch_bounds__(256) void k_compact_edges(int64_t m_new, const int6
                                                        const int32_t *owgt, int32_t *nadj,
                                                        int32_t *nwgt)
 {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= m_new) return;
-  const int s = find_slot(neb, ns, e);
-  const int64_t src = oeb[s] + (e - neb[s]);
-  nadj[e] = (int32_t)(oadj[src] - ovb[s] + nvb[s]);
-  nwgt[e] = owgt[src];
+  // GLOB_PER entries per thread, the slot advancing with them (as k_globalize)
+  const int64_t e0 = (int64_t)blockIdx.x * 256 * GLOB_PER + threadIdx.x;
+  if (e0 >= m_new) return;
+  int s = find_slot(neb, ns, e0);
+#pragma unroll
+  for (int k = 0; k < GLOB_PER; ++k) {
+    const int64_t e = e0 + (int64_t)k * 256;
+    if (e >= m_new) break;
+    while (neb[s + 1] <= e) ++s;
+    const int64_t src = oeb[s] + (e - neb[s]);
+    nadj[e] = (int32_t)(oadj[src] - ovb[s] + nvb[s]);
+    nwgt[e] = owgt[src];
+  }
 }
 
 struct Slot {
@@ -168,26 +204,30 @@ static void batch_impl(cm_ctx *ctx, int nb, const cm_graph_view *g, const int64_
     return p;
   };
   const int64_t N = voff[nb], M = eoff[nb];
-  // ---- level-0 union on the device
-  OffT *xadj = (OffT *)own(sizeof(OffT) * (N + 1));
-  int32_t *adj = (int32_t *)own(4 * M), *wgt = (int32_t *)own(4 * M), *vw = (int32_t *)own(4 * N);
+  // ---- level-0 union on the device.  A device input is referenced where
+  // it can be (offsets and weights are only read; the caller keeps them alive
+  // with the hierarchies, as for cm_coarsen); neighbour ids are made global
+  // in an array of their own.
+  const bool dev = g->on_device != 0;
+  OffT *xadj = dev ? (OffT *)const_cast<void *>(g->xadj) : (OffT *)own(sizeof(OffT) * (N + 1));
+  int32_t *adj = (int32_t *)own(4 * M);
+  int32_t *wgt = dev && g->adjwgt ? const_cast<int32_t *>(g->adjwgt) : (int32_t *)own(4 * M);
+  int32_t *vw = dev && g->vwgt ? const_cast<int32_t *>(g->vwgt) : (int32_t *)own(4 * N);
   int64_t *d_voff = (int64_t *)tmp(8 * (nb + 1)), *d_eoff = (int64_t *)tmp(8 * (nb + 1));
   CM_HIP(hipMemcpyAsync(d_voff, voff, 8 * (nb + 1), hipMemcpyHostToDevice, ctx->stream));
   CM_HIP(hipMemcpyAsync(d_eoff, eoff, 8 * (nb + 1), hipMemcpyHostToDevice, ctx->stream));
-  const hipMemcpyKind kind = g->on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
-  CM_HIP(hipMemcpyAsync(xadj, g->xadj, sizeof(OffT) * (N + 1), kind, ctx->stream));
-  int32_t *adj_local = (int32_t *)tmp(4 * M);
-  if (M) CM_HIP(hipMemcpyAsync(adj_local, g->adjncy, 4 * M, kind, ctx->stream));
-  if (g->adjwgt) {
-    if (M) CM_HIP(hipMemcpyAsync(wgt, g->adjwgt, 4 * M, kind, ctx->stream));
-  } else {
-    fill_u32(ctx, wgt, 1u, M);
+  if (!dev) CM_HIP(hipMemcpyAsync(xadj, g->xadj, sizeof(OffT) * (N + 1), hipMemcpyHostToDevice, ctx->stream));
+  // local ids: read in place from a device input, copied into the union
+  // array (and made global there) from a host input
+  const int32_t *adj_local = g->adjncy;
+  if (!dev) {
+    if (M) CM_HIP(hipMemcpyAsync(adj, g->adjncy, 4 * M, hipMemcpyHostToDevice, ctx->stream));
+    adj_local = adj;
   }
-  if (g->vwgt) {
-    if (N) CM_HIP(hipMemcpyAsync(vw, g->vwgt, 4 * N, kind, ctx->stream));
-  } else {
-    fill_u32(ctx, vw, 1u, N);
-  }
+  if (!g->adjwgt) fill_u32(ctx, wgt, 1u, M);
+  else if (!dev && M) CM_HIP(hipMemcpyAsync(wgt, g->adjwgt, 4 * M, hipMemcpyHostToDevice, ctx->stream));
+  if (!g->vwgt) fill_u32(ctx, vw, 1u, N);
+  else if (!dev && N) CM_HIP(hipMemcpyAsync(vw, g->vwgt, 4 * N, hipMemcpyHostToDevice, ctx->stream));
   bool trusted = true;
   for (int b = 0; b < nb; ++b) trusted = trusted && (params[b].flags & CM_FLAG_TRUSTED_INPUT);
   if (!trusted) {
@@ -204,12 +244,15 @@ static void batch_impl(cm_ctx *ctx, int nb, const cm_graph_view *g, const int64_
     CM_REQUIRE(!(h & 4), CM_ERR_INPUT, "a graph of the batch has a self loop");
     CM_REQUIRE(!(h & 8), CM_ERR_INPUT, "edge weights must be positive");
   }
-  launch(ctx, "batch_globalize", 0.0, k_globalize, dim3(nblocks(M, 256)), dim3(256), 0, M,
-         (const int32_t *)adj_local, (const int64_t *)d_eoff, (const int64_t *)d_voff, nb, adj);
-  // per-graph totals of vertex weights
-  std::vector<int32_t> hv(N);
+  launch(ctx, "batch_globalize", 0.0, k_globalize, dim3(nblocks(M, 256 * GLOB_PER)), dim3(256), 0, M,
+         adj_local, (const int64_t *)d_eoff, (const int64_t *)d_voff, nb, adj);
+  // per-graph totals of vertex weights, summed on the device
+  std::vector<int64_t> tvw(nb, 0);
   if (g->vwgt && N) {
-    CM_HIP(hipMemcpyAsync(hv.data(), vw, 4 * N, hipMemcpyDeviceToHost, ctx->stream));
+    int64_t *d_sums = (int64_t *)tmp(8 * nb);
+    launch(ctx, "batch_vw_sums", 4.0 * N, k_graph_vw_sums, dim3((unsigned)std::min(nb, 4096)), dim3(1024), 0, nb,
+           (const int64_t *)d_voff, (const int32_t *)vw, d_sums);
+    CM_HIP(hipMemcpyAsync(tvw.data(), d_sums, 8 * nb, hipMemcpyDeviceToHost, ctx->stream));
   }
   queue_max_degree(ctx, N, off64, xadj, vw);
   sync(ctx);
@@ -220,11 +263,7 @@ static void batch_impl(cm_ctx *ctx, int nb, const cm_graph_view *g, const int64_
     out[b] = new cm_hier();
     out[b]->shared = shared;
     const int64_t n = voff[b + 1] - voff[b], m = eoff[b + 1] - eoff[b];
-    int64_t tv = n;
-    if (g->vwgt) {
-      tv = 0;
-      for (int64_t i = voff[b]; i < voff[b + 1]; ++i) tv += hv[i];
-    }
+    const int64_t tv = g->vwgt ? tvw[b] : n;
     const cm_coarsen_params &p = params[b];
     const int T = p.threshold > 0 ? p.threshold : ref_threshold(n, p.nparts);
     const int32_t maxv = p.maxvwgt > 0 ? p.maxvwgt : (int32_t)(1.5 * (double)tv / T);
@@ -407,7 +446,7 @@ static void batch_impl(cm_ctx *ctx, int nb, const cm_graph_view *g, const int64_
     launch(ctx, "batch_compact_v", 0.0, k_compact_vertices<OffT>, dim3(nblocks(n2 + 1, 256)), dim3(256), 0,
            n2, (const int64_t *)d_nvb, nk, (const int64_t *)d_ovb, (const int64_t *)d_oeb,
            (const int64_t *)d_neb, (const OffT *)cxadj, (const int32_t *)cvw, x2, v2, g2);
-    launch(ctx, "batch_compact_e", 0.0, k_compact_edges, dim3(nblocks(m2, 256)), dim3(256), 0, m2,
+    launch(ctx, "batch_compact_e", 0.0, k_compact_edges, dim3(nblocks(m2, 256 * GLOB_PER)), dim3(256), 0, m2,
            (const int64_t *)d_neb, nk, (const int64_t *)d_oeb, (const int64_t *)d_ovb,
            (const int64_t *)d_nvb, (const int32_t *)cadj, (const int32_t *)cwgt, a2, w2);
     // the continuing graphs' current level now lives in the compacted copy
