@@ -58,19 +58,6 @@ cmh::HostGraph level_to_host(cm_ctx *ctx, const cm::Level &L)
   return g;
 }
 
-int64_t host_cut(const cm_graph_view *g, const int32_t *part)
-{
-  int64_t cut = 0;
-  for (int64_t i = 0; i < g->n; ++i) {
-    const int64_t b = g->offset_type == CM_OFF64 ? ((const int64_t *)g->xadj)[i] : ((const int32_t *)g->xadj)[i];
-    const int64_t e = g->offset_type == CM_OFF64 ? ((const int64_t *)g->xadj)[i + 1]
-                                                 : ((const int32_t *)g->xadj)[i + 1];
-    for (int64_t j = b; j < e; ++j)
-      if (part[i] != part[g->adjncy[j]]) cut += g->adjwgt ? g->adjwgt[j] : 1;
-  }
-  return cut / 2;
-}
-
 }  // namespace
 
 namespace cm {
@@ -78,7 +65,7 @@ namespace cm {
 cmh::HostGraph coarsest_to_host(cm_ctx *ctx, const cm_hier *h) { return level_to_host(ctx, h->levels.back()); }
 
 void uncoarsen_hierarchy(cm_ctx *ctx, const cm_hier *h, int nparts, const cmh::HostGraph &hc,
-                         const std::vector<int> &where, int32_t *part, bool balance)
+                         const std::vector<int> &where, int32_t *part, bool balance, int64_t *cut)
 {
   const int L = (int)h->levels.size();
   const Level &C = h->levels[L - 1];
@@ -86,6 +73,7 @@ void uncoarsen_hierarchy(cm_ctx *ctx, const cm_hier *h, int nparts, const cmh::H
   std::vector<int64_t> pw(nparts, 0);
   for (int i = 0; i < hc.n; ++i) pw[where[i]] += hc.vwgt[i];
   int32_t *w0 = (int32_t *)ctx->pool.alloc(4 * (C.n > 0 ? C.n : 1)), *w1 = nullptr;
+  unsigned long long *d_cut = nullptr, h_cut = 0;
   try {
     if (C.n) CM_HIP(hipMemcpyAsync(w0, where.data(), 4 * C.n, hipMemcpyHostToDevice, ctx->stream));
     if (balance) balance_level(ctx, C, nparts, w0, pw, tvwgt);
@@ -101,25 +89,33 @@ void uncoarsen_hierarchy(cm_ctx *ctx, const cm_hier *h, int nparts, const cmh::H
       refine_level(ctx, F, nparts, w0, pw, tvwgt);
     }
     const int64_t n0 = h->levels[0].n;
+    if (cut) {
+      d_cut = (unsigned long long *)ctx->pool.alloc(sizeof(unsigned long long));
+      level_cut(ctx, h->levels[0], w0, d_cut);
+      CM_HIP(hipMemcpyAsync(&h_cut, d_cut, sizeof(h_cut), hipMemcpyDeviceToHost, ctx->stream));
+    }
     if (n0) CM_HIP(hipMemcpyAsync(part, w0, 4 * n0, hipMemcpyDeviceToHost, ctx->stream));
     sync(ctx);
   } catch (...) {
     ctx->pool.release(w0);
     if (w1) ctx->pool.release(w1);
+    if (d_cut) ctx->pool.release(d_cut);
     throw;
   }
   ctx->pool.release(w0);
+  if (d_cut) ctx->pool.release(d_cut);
+  if (cut) *cut = (int64_t)(h_cut / 2);
 }
 
 void partition_hierarchy(cm_ctx *ctx, const cm_hier *h, int nparts, int32_t *part, double *init_ms,
-                         double *uncoarsen_ms, bool balance)
+                         double *uncoarsen_ms, bool balance, int64_t *cut)
 {
   const double t1 = now_ms();
   cmh::HostGraph hc = coarsest_to_host(ctx, h);
   std::vector<int> where;
   cmh::initial_partition(hc, nparts, where);
   const double t2 = now_ms();
-  uncoarsen_hierarchy(ctx, h, nparts, hc, where, part, balance);
+  uncoarsen_hierarchy(ctx, h, nparts, hc, where, part, balance, cut);
   if (init_ms) *init_ms = t2 - t1;
   if (uncoarsen_ms) *uncoarsen_ms = now_ms() - t2;
 }
@@ -223,32 +219,23 @@ int cm_partition(cm_ctx *ctx, const cm_graph_view *g, const cm_coarsen_params *p
     for (auto &x : tr.t) x.th.join();
     const double p1 = now_ms();
     r.init_ms = p1 - p0 - r.coarsen_ms;  // host time the trials' initial partitions add
-    // the uncoarsenings one after another on the stream; each trial's edge
-    // cut is counted on the host (cu:356-366) while the next one runs
+    // the uncoarsenings one after another on the stream, each ending with
+    // its level-0 edge cut counted on the device
     std::vector<std::vector<int32_t>> parts(trials);
     std::vector<int64_t> cut(trials, -1);
     std::vector<int32_t> levels(trials), coarsest(trials);
-    struct Cuts {  // joined before parts and cut go away
-      std::vector<std::thread> th;
-      ~Cuts()
-      {
-        for (auto &x : th)
-          if (x.joinable()) x.join();
-      }
-    } cuts;
     for (int t = 0; t < trials; ++t) {
       Trial &x = tr.t[t];
       CM_REQUIRE(!x.failed, CM_ERR_NOMEM, "initial partition failed (host allocation)");
       parts[t].resize(g->n);
-      cm::uncoarsen_hierarchy(ctx, x.h, nparts, x.hc, x.where, parts[t].data(), !(p->flags & CM_FLAG_NO_BALANCE));
+      cm::uncoarsen_hierarchy(ctx, x.h, nparts, x.hc, x.where, parts[t].data(), !(p->flags & CM_FLAG_NO_BALANCE),
+                              &cut[t]);
       levels[t] = (int32_t)x.h->levels.size();
       coarsest[t] = (int32_t)x.h->levels.back().n;
       cm_hier_release(ctx, x.h);
       x.h = nullptr;
-      cuts.th.emplace_back([&parts, &cut, g, t]() { cut[t] = host_cut(g, parts[t].data()); });
     }
     r.uncoarsen_ms = now_ms() - p1;
-    for (auto &x : cuts.th) x.join();
     for (int t = 0; t < trials; ++t)
       if (best < 0 || cut[t] < best) {  // the earlier trial on equal cuts
         best = cut[t];

@@ -423,4 +423,55 @@ int refine_level(cm_ctx *ctx, const Level &L, int nparts, int32_t *d_where, std:
                  : refine_impl<int32_t>(ctx, L, nparts, d_where, pw, tvwgt);
 }
 
+// A block owns 256 rows and strides over their entries, finding each entry's
+// row by binary search in the block's offsets: a long row is spread over the
+// block instead of one thread.
+template <typename OffT>
+__global__ __launch_bounds__(256) void k_level_cut(int64_t n, const OffT *__restrict__ xadj,
+                                                   const int32_t *__restrict__ adjncy,
+                                                   const int32_t *__restrict__ adjwgt,
+                                                   const int32_t *__restrict__ where,
+                                                   const int32_t *__restrict__ where_nb,
+                                                   unsigned long long *sum)
+{
+  __shared__ int64_t s_off[257];
+  __shared__ int32_t s_part[256];
+  const int64_t v0 = (int64_t)blockIdx.x * 256;
+  const int rows = (int)min<int64_t>(256, n - v0);
+  const int t = threadIdx.x;
+  if (t < rows) {
+    s_off[t] = (int64_t)xadj[v0 + t];
+    s_part[t] = where[v0 + t];
+  }
+  if (t == 0) s_off[rows] = (int64_t)xadj[v0 + rows];
+  __syncthreads();
+  int64_t cut = 0;
+  for (int64_t e = s_off[0] + t; e < s_off[rows]; e += 256) {
+    int lo = 0, hi = rows;  // s_off[lo] <= e < s_off[hi]
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (s_off[mid] <= e) lo = mid;
+      else hi = mid;
+    }
+    if (where_nb[adjncy[e]] != s_part[lo]) cut += adjwgt[e];
+  }
+  cut = wave_sum<int64_t>(cut);
+  if (lane() == 0 && cut) atomicAdd(sum, (unsigned long long)cut);
+}
+
+void level_cut(cm_ctx *ctx, const Level &L, const int32_t *d_where, unsigned long long *d_sum)
+{
+  fill_u64(ctx, d_sum, 0ull, 1);
+  if (L.n == 0) return;
+  const int32_t *adj = L.adjncy - L.ebase, *wgt = L.adjwgt - L.ebase;
+  const int32_t *where_nb = d_where - L.vbase;
+  const double bytes = (double)L.m * 12.0 + (double)L.n * (L.off64 ? 12.0 : 8.0);
+  if (L.off64)
+    launch(ctx, "level_cut", bytes, k_level_cut<int64_t>, dim3(nblocks(L.n, 256)), dim3(256), 0, L.n,
+           (const int64_t *)L.xadj, adj, wgt, d_where, where_nb, d_sum);
+  else
+    launch(ctx, "level_cut", bytes, k_level_cut<int32_t>, dim3(nblocks(L.n, 256)), dim3(256), 0, L.n,
+           (const int32_t *)L.xadj, adj, wgt, d_where, where_nb, d_sum);
+}
+
 }  // namespace cm

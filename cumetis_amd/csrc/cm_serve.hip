@@ -99,10 +99,13 @@ __global__ void k_union_bases(int ns, const int64_t *vbase, const int32_t *cmap,
 
 }  // namespace
 
-// A job's host graph, shared by its trials.
+// A job's host graph, shared by its trials, with its largest row and vertex
+// weight (taken while the submit checks it).
 struct JobGraph {
   std::vector<int64_t> xadj;
   std::vector<int32_t> adjncy, adjwgt, vwgt;
+  int64_t maxdeg = 0;
+  int64_t maxvw = 0;
 };
 
 struct Request;
@@ -138,6 +141,11 @@ struct Request {
   int status = CM_OK;
   bool done = false;
   std::vector<std::unique_ptr<Job>> jobs;
+  // level 0 on the device, uploaded when the first trial is admitted and
+  // released (to the coarsening context's pool) when the last one finishes
+  void *d_xadj = nullptr;
+  int32_t *d_adjncy = nullptr, *d_adjwgt = nullptr, *d_vwgt = nullptr;
+  int dev_left = 0;  // trials that have not finished
 };
 
 }  // namespace cm
@@ -316,32 +324,45 @@ static void loop_main(cm_server *S)
     }
     std::vector<Job *> retire;
     try {
-      for (Job *j : fresh) {  // level 0 on the device
+      for (Job *j : fresh) {  // level 0 on the device, once per request
         cm_ctx *ctx = S->cctx;
+        Request *r = j->req;
+        const JobGraph &G = *j->gr;
+        if (!r->d_xadj) {
+          void *b[4] = {};
+          try {
+            b[0] = ctx->pool.alloc(8 * (j->n + 1));
+            b[1] = ctx->pool.alloc(4 * (j->m > 0 ? j->m : 1));
+            b[2] = ctx->pool.alloc(4 * (j->m > 0 ? j->m : 1));
+            b[3] = ctx->pool.alloc(4 * (j->n > 0 ? j->n : 1));
+          } catch (...) {  // all four or none: later trials test d_xadj alone
+            for (void *q : b)
+              if (q) ctx->pool.release(q);
+            throw;
+          }
+          r->d_xadj = b[0];
+          r->d_adjncy = (int32_t *)b[1];
+          r->d_adjwgt = (int32_t *)b[2];
+          r->d_vwgt = (int32_t *)b[3];
+          CM_HIP(hipMemcpyAsync(r->d_xadj, G.xadj.data(), 8 * (j->n + 1), hipMemcpyHostToDevice, ctx->stream));
+          if (j->m) {
+            CM_HIP(hipMemcpyAsync(r->d_adjncy, G.adjncy.data(), 4 * j->m, hipMemcpyHostToDevice, ctx->stream));
+            CM_HIP(hipMemcpyAsync(r->d_adjwgt, G.adjwgt.data(), 4 * j->m, hipMemcpyHostToDevice, ctx->stream));
+          }
+          if (j->n) CM_HIP(hipMemcpyAsync(r->d_vwgt, G.vwgt.data(), 4 * j->n, hipMemcpyHostToDevice, ctx->stream));
+        }
         j->h = new cm_hier();
         Level L0;
         L0.n = j->n;
         L0.m = j->m;
         L0.off64 = true;
-        L0.xadj = ctx->pool.alloc(8 * (j->n + 1));
-        L0.adjncy = (int32_t *)ctx->pool.alloc(4 * (j->m > 0 ? j->m : 1));
-        L0.adjwgt = (int32_t *)ctx->pool.alloc(4 * (j->m > 0 ? j->m : 1));
-        L0.vwgt = (int32_t *)ctx->pool.alloc(4 * (j->n > 0 ? j->n : 1));
-        j->h->owned = {L0.xadj, L0.adjncy, L0.adjwgt, L0.vwgt};
-        const JobGraph &G = *j->gr;
-        CM_HIP(hipMemcpyAsync(L0.xadj, G.xadj.data(), 8 * (j->n + 1), hipMemcpyHostToDevice, ctx->stream));
-        if (j->m) {
-          CM_HIP(hipMemcpyAsync(L0.adjncy, G.adjncy.data(), 4 * j->m, hipMemcpyHostToDevice, ctx->stream));
-          CM_HIP(hipMemcpyAsync(L0.adjwgt, G.adjwgt.data(), 4 * j->m, hipMemcpyHostToDevice, ctx->stream));
-        }
-        if (j->n) CM_HIP(hipMemcpyAsync(L0.vwgt, G.vwgt.data(), 4 * j->n, hipMemcpyHostToDevice, ctx->stream));
+        L0.xadj = r->d_xadj;
+        L0.adjncy = r->d_adjncy;
+        L0.adjwgt = r->d_adjwgt;
+        L0.vwgt = r->d_vwgt;
         L0.tvwgt = j->tvwgt;
-        L0.maxdeg = 0;
-        L0.maxvw = 0;
-        for (int64_t v = 0; v < j->n; ++v) {
-          L0.maxdeg = std::max(L0.maxdeg, G.xadj[v + 1] - G.xadj[v]);
-          L0.maxvw = std::max<int64_t>(L0.maxvw, G.vwgt[v]);
-        }
+        L0.maxdeg = G.maxdeg;
+        L0.maxvw = G.maxvw;
         j->h->levels.push_back(L0);
         j->cur_n = j->n;
       }
@@ -371,6 +392,14 @@ static void loop_main(cm_server *S)
   }
 }
 
+static void release_level0(cm_server *S, Request *r)
+{
+  for (void *q : {r->d_xadj, (void *)r->d_adjncy, (void *)r->d_adjwgt, (void *)r->d_vwgt})
+    if (q) S->cctx->pool.release(q);
+  r->d_xadj = nullptr;
+  r->d_adjncy = r->d_adjwgt = r->d_vwgt = nullptr;
+}
+
 static void finish_main(cm_server *S, cm_ctx *fctx)
 {
   hipSetDevice(fctx->device);
@@ -388,19 +417,12 @@ static void finish_main(cm_server *S, cm_ctx *fctx)
     int64_t cut = 0;
     if (j->status == CM_OK) {
       try {
-        part.assign(j->n, 0);
-        const double t0 = now_ms();
+        part.assign(j->n, 0);  // one part: a cut of 0
         if (j->p.nparts > 1 && j->n > 0)
           partition_hierarchy(fctx, j->h, j->p.nparts, part.data(), &rep.init_ms, &rep.uncoarsen_ms,
-                              !(j->p.flags & CM_FLAG_NO_BALANCE));
+                              !(j->p.flags & CM_FLAG_NO_BALANCE), &cut);
         rep.levels = (int32_t)j->h->levels.size();
         rep.coarsest_n = (int32_t)j->h->levels.back().n;
-        const JobGraph &G = *j->gr;
-        for (int64_t i = 0; i < j->n; ++i)
-          for (int64_t e = G.xadj[i]; e < G.xadj[i + 1]; ++e)
-            if (part[i] != part[G.adjncy[e]]) cut += G.adjwgt[e];
-        cut /= 2;
-        rep.uncoarsen_ms = now_ms() - t0 - rep.init_ms;
       } catch (const Error &e) {
         j->status = e.code;
       }
@@ -413,6 +435,7 @@ static void finish_main(cm_server *S, cm_ctx *fctx)
     std::lock_guard<std::mutex> lk(S->mu);
     Request *r = j->req;
     S->used -= j->footprint;
+    if (--r->dev_left == 0) release_level0(S, r);
     if (j->status != CM_OK && r->status == CM_OK) r->status = j->status;
     if (j->status == CM_OK) {
       r->rep.init_ms += rep.init_ms;
@@ -490,6 +513,8 @@ int cm_server_submit(cm_server *S, const cm_graph_view *g, const cm_coarsen_para
   gr->vwgt = g->vwgt ? std::vector<int32_t>(g->vwgt, g->vwgt + g->n) : std::vector<int32_t>(g->n, 1);
   for (int64_t i = 0; i < g->n; ++i) {
     if (gr->xadj[i] > gr->xadj[i + 1]) return CM_ERR_INPUT;
+    gr->maxdeg = std::max(gr->maxdeg, gr->xadj[i + 1] - gr->xadj[i]);
+    gr->maxvw = std::max<int64_t>(gr->maxvw, gr->vwgt[i]);
     for (int64_t e = gr->xadj[i]; e < gr->xadj[i + 1]; ++e)
       if (gr->adjncy[e] < 0 || gr->adjncy[e] >= g->n || gr->adjncy[e] == i || gr->adjwgt[e] <= 0)
         return CM_ERR_INPUT;
@@ -500,6 +525,7 @@ int cm_server_submit(cm_server *S, const cm_graph_view *g, const cm_coarsen_para
   const int32_t maxv = p->maxvwgt > 0 ? p->maxvwgt : (int32_t)(1.5 * (double)tvwgt / T);
   auto r = std::make_unique<cm::Request>();
   r->trials = p->nparts > 1 && g->n > 0 ? (p->trials > 0 ? p->trials : CM_DEFAULT_TRIALS) : 1;
+  r->dev_left = r->trials;
   r->t_submit = cm::now_ms();
   for (int t = 0; t < r->trials; ++t) {
     auto j = std::make_unique<cm::Job>();
@@ -572,12 +598,14 @@ int cm_server_destroy(cm_server *S)
   if (S->loop.joinable()) S->loop.join();
   for (auto &t : S->finishers)
     if (t.joinable()) t.join();
-  for (auto &kv : S->jobs)
+  for (auto &kv : S->jobs) {
     for (auto &j : kv.second->jobs)
       if (j->h) {
         for (void *q : j->h->owned) S->cctx->pool.release(q);
         delete j->h;
       }
+    cm::release_level0(S, kv.second.get());
+  }
   S->jobs.clear();
   for (cm_ctx *f : S->fctx) cm_ctx_destroy(f);
   cm_ctx_destroy(S->cctx);

@@ -1,5 +1,7 @@
 """GPU: the continuous-batching server.  Every job's partition must be the
-same alone (cm_partition) and in the mix."""
+same alone (cm_partition) and in the mix, and its reported edge cut the
+oracle's cut of that partition.  A request's trials (4 by default) share one
+device copy of its graph."""
 import threading
 import time
 
@@ -24,7 +26,7 @@ def mix():
     return [(with_weights(g), k) for g, k in gs]
 
 
-def test_server_results_equal_alone(engine):
+def test_server_results_equal_alone(engine, orc):
     jobs = mix()
     say(f"[serve] {len(jobs)} jobs, staggered submits")
     alone = [engine.partition(g, k, seed=1) for g, k in jobs]
@@ -39,6 +41,7 @@ def test_server_results_equal_alone(engine):
     for (part, rep), (apart, arep), (g, k) in zip(res, alone, jobs):
         assert np.array_equal(part, apart)
         assert rep.edgecut == arep.edgecut and rep.levels == arep.levels
+        assert rep.edgecut == orc.edgecut(g.xadj, g.adjncy, g.adjwgt, part)
     assert st["completed"] == len(jobs)
     assert st["max_active"] >= 2, st
 
@@ -75,7 +78,7 @@ def test_server_rejects_bad_input(engine):
         assert len(part) == 3
 
 
-def test_server_mix_with_large_job(engine):
+def test_server_mix_with_large_job(engine, orc):
     """The serving configuration's large job, a 160^3 7-point grid (4,096,000
     vertices, k=128), submitted among small jobs of the batch distribution:
     every result equals the same job partitioned alone."""
@@ -88,8 +91,9 @@ def test_server_mix_with_large_job(engine):
     with Server(0, max_slots=16) as srv:
         tickets = [srv.submit(g, k, seed=1) for g, k in jobs]
         res = [srv.wait(t) for t in tickets]
-    for (part, rep), (apart, arep) in zip(res, alone):
+    for (part, rep), (apart, arep), (g, k) in zip(res, alone, jobs):
         assert np.array_equal(part, apart)
         assert rep.edgecut == arep.edgecut
+        assert rep.edgecut == orc.edgecut(g.xadj, g.adjncy, g.adjwgt, part)
     pw = np.bincount(res[1][0], minlength=128)
     assert pw.max() * 128 / big.n <= 1.03 + 1e-6
