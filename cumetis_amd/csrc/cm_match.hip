@@ -50,6 +50,9 @@ constexpr int ROUND_BATCH = MATCH_BATCH;  // launches queued between host checks
 // Row entries per scan step on levels without long rows (8 was slower on
 // the grid: a short row then loads more clamped duplicates).
 constexpr int MESH_BATCH = 4;
+// Levels with long rows and at least this many vertices keep taken bits
+// (a 4 MB match[] is the L2 of one XCD; its bit array is 128 KB).
+constexpr int64_t BITS_MIN_N = 1 << 20;
 
 // flags[2r] = 1 if a vertex matched or died in round r, flags[2r+1] = 1 if
 // any vertex is still unmatched after round r.  Round r's accept runs in
@@ -82,12 +85,14 @@ __device__ __forceinline__ bool key_better(uint64_t a1, uint64_t a2, uint64_t b1
   return a1 > b1 || (a1 == b1 && a2 > b2);
 }
 
-// Also clears the hash contraction's item counters (hzero, 16 words).
+// Also clears the hash contraction's item counters (hzero, 16 words) and the
+// taken bits (nbw words, at most n).
 __global__ __launch_bounds__(256) void k_match_init(int64_t n, int32_t *match, int32_t *prop, unsigned *flags,
-                                                    int64_t nflags, unsigned *hzero)
+                                                    int64_t nflags, unsigned *hzero, uint64_t *bits, int64_t nbw)
 {
   const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (v < 16) hzero[v] = 0u;
+  if (v < nbw) bits[v] = 0u;
   if (v < n) {
     match[v] = -1;
     prop[v] = -1;
@@ -296,13 +301,16 @@ __device__ __forceinline__ bool accepted_prev(const int32_t *prev, int32_t u)
 // Best eligible neighbour of v among vertices unmatched after round r-1, by
 // the strict order (adjwgt, hash(seed,min,max), min, max); -2 if none.
 // Lanes [l0, l0+step) stride the row; the caller reduces across lanes.
-template <typename OffT, int SCAN_BATCH = 4>
+// BITS: a neighbour's matched state comes from the taken bits (bit u of the
+// bit array: match[u] != -1), which stay in the L2 where match[] (32 times
+// larger) would not.
+template <typename OffT, int SCAN_BATCH = 4, bool BITS = false>
 __device__ __forceinline__ void scan_row(int64_t v, OffT b, OffT e, int l0, int step,
                                          const int32_t *__restrict__ adjncy,
                                          const int32_t *__restrict__ adjwgt,
                                          const int32_t *__restrict__ vwgt, int32_t vw, int32_t maxv,
                                          uint64_t seed, int64_t base, const int32_t *match,
-                                         const int32_t *prev, bool have_prev, bool check_w, bool wunit,
+                                         const uint32_t *bits, const int32_t *prev, bool have_prev, bool check_w, bool wunit,
                                          int32_t &best, uint64_t &bk1, uint64_t &bk2)
 {
   // SCAN_BATCH entries per lane per step: their row loads and the gathers
@@ -331,7 +339,10 @@ __device__ __forceinline__ void scan_row(int64_t v, OffT b, OffT e, int l0, int 
       const int32_t uu = look ? u[t] : (int32_t)v;
       // round 0 runs before any accept, so every neighbour is unmatched: all
       // lanes read one shared word instead of gathering
-      mu[t] = match[have_prev ? uu : 0];
+      if constexpr (BITS)
+        mu[t] = (bits[have_prev ? (uint32_t)uu >> 5 : 0u] >> (uu & 31)) & 1u ? 0 : -1;
+      else
+        mu[t] = match[have_prev ? uu : 0];
       wu[t] = check_w ? vwgt[uu] : 0;  // without check_w no pair can exceed maxv
       if (!look) u[t] = -1;
     }
@@ -382,11 +393,12 @@ struct LongPartial {
 // to part[q] with plain stores; the next kernel on the stream reduces the
 // row's chunks (a hand-off inside the launch would need an L2 writeback and
 // invalidate per chunk for its release/acquire pair).
-template <typename OffT>
+template <typename OffT, bool BITS>
 __device__ __forceinline__ void hub_chunk(int64_t q, const LongChunk *chunks, const OffT *__restrict__ xadj,
                                           const int32_t *__restrict__ adjncy, const int32_t *__restrict__ adjwgt,
                                           const int32_t *__restrict__ vwgt, const MatchOwners &own,
-                                          const int32_t *match, const int32_t *prev, int32_t *cur,
+                                          const int32_t *match, const uint32_t *bits, const int32_t *prev,
+                                          int32_t *cur,
                                           const LongPartial &part, bool have_prev, int64_t maxvw,
                                           unsigned epoch)
 {
@@ -420,8 +432,8 @@ __device__ __forceinline__ void hub_chunk(int64_t q, const LongChunk *chunks, co
   uint64_t bk1 = 0, bk2 = 0;
   const OffT b = xb + (OffT)c.k * LONG_CHUNK, e0 = xe;
   const OffT e = e0 - b > (OffT)LONG_CHUNK ? b + (OffT)LONG_CHUNK : e0;
-  scan_row<OffT>(v, b, e, l, CM_WAVE, adjncy, adjwgt, vwgt, vw, maxv, seed, base, match, prev, have_prev,
-                 maxvw < 0 || (int64_t)vw + maxvw > maxv, own.wunit, best, bk1, bk2);
+  scan_row<OffT, 4, BITS>(v, b, e, l, CM_WAVE, adjncy, adjwgt, vwgt, vw, maxv, seed, base, match, bits, prev,
+                          have_prev, maxvw < 0 || (int64_t)vw + maxvw > maxv, own.wunit, best, bk1, bk2);
   wave_best(best, bk1, bk2);
   if (l == 0) {
     part.best[q] = best;
@@ -464,11 +476,12 @@ __global__ __launch_bounds__(256) void k_hub_reduce(const LongChunk *chunks, con
 // then the wave scans the rows that need it one after another, with each
 // row's parameters read from its lane.  A wave thus waits for the state
 // loads once per group instead of once per row.
-template <typename OffT>
+template <typename OffT, bool BITS>
 __device__ __forceinline__ void mid_group(int64_t m0, unsigned nm, const int32_t *mids,
                                           const OffT *__restrict__ xadj, const int32_t *__restrict__ adjncy,
                                           const int32_t *__restrict__ adjwgt, const int32_t *__restrict__ vwgt,
-                                          const MatchOwners &own, const int32_t *match, const int32_t *prev,
+                                          const MatchOwners &own, const int32_t *match, const uint32_t *bits,
+                                          const int32_t *prev,
                                           int32_t *cur, bool have_prev, int64_t maxvw)
 {
   const int l = lane();
@@ -502,8 +515,9 @@ __device__ __forceinline__ void mid_group(int64_t m0, unsigned nm, const int32_t
     const OffT bi = (OffT)bcast((int64_t)xb, i), ei = (OffT)bcast((int64_t)xe, i);
     int32_t best = -2;
     uint64_t bk1 = 0, bk2 = 0;
-    scan_row<OffT>(vi, bi, ei, l, CM_WAVE, adjncy, adjwgt, vwgt, bcast(vw, i), bcast(maxv, i), bcast(seed, i),
-                   bcast(base, i), match, prev, have_prev, bcast(cw, i) != 0, own.wunit, best, bk1, bk2);
+    scan_row<OffT, 4, BITS>(vi, bi, ei, l, CM_WAVE, adjncy, adjwgt, vwgt, bcast(vw, i), bcast(maxv, i),
+                            bcast(seed, i), bcast(base, i), match, bits, prev, have_prev, bcast(cw, i) != 0,
+                            own.wunit, best, bk1, bk2);
     wave_best(best, bk1, bk2);
     if (l == 0) cur[vi] = best;
   }
@@ -530,11 +544,12 @@ __device__ __forceinline__ void group_best(int32_t &best, uint64_t &bk1, uint64_
 // state as in mid_group (lane i for row i), then the rows to scan go
 // CM_WAVE / SUB at a time, one per group of SUB lanes (such a row is one step
 // of SCAN_BATCH entries per lane).
-template <int SUB, typename OffT>
+template <int SUB, typename OffT, bool BITS>
 __device__ __forceinline__ void small_group(int64_t m0, unsigned ns, const int32_t *smids,
                                             const OffT *__restrict__ xadj, const int32_t *__restrict__ adjncy,
                                             const int32_t *__restrict__ adjwgt, const int32_t *__restrict__ vwgt,
-                                            const MatchOwners &own, const int32_t *match, const int32_t *prev,
+                                            const MatchOwners &own, const int32_t *match, const uint32_t *bits,
+                                            const int32_t *prev,
                                             int32_t *cur, bool have_prev, int64_t maxvw)
 {
   const int l = lane();
@@ -577,8 +592,8 @@ __device__ __forceinline__ void small_group(int64_t m0, unsigned ns, const int32
     int32_t best = -2;
     uint64_t bk1 = 0, bk2 = 0;
     if (i >= 0)
-      scan_row<OffT>(vi, (OffT)bi, (OffT)ei, gl, SUB, adjncy, adjwgt, vwgt, vwi, maxvi, seedi, basei, match, prev,
-                     have_prev, cwi != 0, own.wunit, best, bk1, bk2);
+      scan_row<OffT, 4, BITS>(vi, (OffT)bi, (OffT)ei, gl, SUB, adjncy, adjwgt, vwgt, vwi, maxvi, seedi, basei,
+                              match, bits, prev, have_prev, cwi != 0, own.wunit, best, bk1, bk2);
     group_best<SUB>(best, bk1, bk2);
     if (i >= 0 && gl == 0) cur[vi] = best;
   }
@@ -599,12 +614,14 @@ __device__ __forceinline__ void small_group(int64_t m0, unsigned ns, const int32
 // launch initialises itself: its scans read this word instead of match[0].
 __device__ const int32_t unmatched_word = -1;
 
-template <typename OffT, bool INIT = false, int SB = 4>
+// taken is set when v (< n) is matched after this accept.
+template <typename OffT, bool INIT = false, int SB = 4, bool BITS = false>
 __device__ __forceinline__ void short_row(int64_t v, int64_t n, const OffT *__restrict__ xadj,
                                           const int32_t *__restrict__ adjncy, const int32_t *__restrict__ adjwgt,
                                           const int32_t *__restrict__ vwgt, const MatchOwners &own, int32_t *match,
-                                          const int32_t *prev, int32_t *cur, bool have_prev, int propose,
-                                          int64_t maxvw, int short_deg, bool &changed, bool &active)
+                                          const uint32_t *bits, const int32_t *prev, int32_t *cur, bool have_prev,
+                                          int propose, int64_t maxvw, int short_deg, bool &changed, bool &active,
+                                          bool &taken)
 {
   // INIT (launch 0 of a level without long rows, which has no k_match_init):
   // v is unmatched; this thread writes match[v] and the other proposal
@@ -618,6 +635,7 @@ __device__ __forceinline__ void short_row(int64_t v, int64_t n, const OffT *__re
   const int64_t vc = v < n ? v : n - 1;
   const int32_t mv = INIT ? -1 : match[vc];
   const int32_t p = have_prev ? prev[vc] : -1;
+  taken = v < n && mv != -1;
   if (v < n && mv == -1) {
     const int32_t pc = p >= 0 ? p : (int32_t)v;
     const int32_t pp = have_prev ? prev[pc] : -1;  // prev[p]
@@ -637,6 +655,7 @@ __device__ __forceinline__ void short_row(int64_t v, int64_t n, const OffT *__re
     }
     changed |= done;
     active |= !done;
+    taken = done;
     if (!done && propose && (int64_t)(e - b) <= short_deg) {
       // p accepted in round r-1 (from the proposals alone): dead or mutual
       const bool p_acc = pp == -2 || (pp >= 0 && ppp == p);
@@ -649,8 +668,9 @@ __device__ __forceinline__ void short_row(int64_t v, int64_t n, const OffT *__re
         owner_params(own, v, maxv, seed, base);
         int32_t best = -2;
         uint64_t bk1 = 0, bk2 = 0;
-        scan_row<OffT, SB>(v, b, e, 0, 1, adjncy, adjwgt, vwgt, vw, maxv, seed, base, INIT ? &unmatched_word : match,
-                       prev, have_prev, maxvw < 0 || (int64_t)vw + maxvw > maxv, own.wunit, best, bk1, bk2);
+        scan_row<OffT, SB, BITS>(v, b, e, 0, 1, adjncy, adjwgt, vwgt, vw, maxv, seed, base,
+                                 INIT ? &unmatched_word : match, bits, prev, have_prev,
+                                 maxvw < 0 || (int64_t)vw + maxvw > maxv, own.wunit, best, bk1, bk2);
         cur[v] = best;
       }
     }
@@ -666,11 +686,18 @@ __device__ __forceinline__ void short_row(int64_t v, int64_t n, const OffT *__re
 // thread-per-row path, no long-row lists): the long-row paths are compiled
 // out, which leaves the kernel without SGPR spills (the full round kernel
 // spills 34 to VGPR lanes); grid -3.6%.
-template <typename OffT, bool LIST, bool INIT = false, bool MESH = false>
+// BITS: large levels with long rows keep the taken bits beside match[] (set
+// by the accepts; scans read them).  A wave of the thread-per-row blocks owns
+// the two words of its 64 vertices and stores them whole; a vertex region of
+// the list launches owns its 32 words and or-s its new matches into them.
+// A reader in the same launch may see a word before or after this launch's
+// store: the bits it adds are the vertices accepted in round r-1, which the
+// scan also finds through the proposals, so both give one answer.
+template <typename OffT, bool LIST, bool INIT = false, bool MESH = false, bool BITS = false>
 __device__ __forceinline__ void round_body(int64_t n, const OffT *__restrict__ xadj,
                                            const int32_t *__restrict__ adjncy, const int32_t *__restrict__ adjwgt,
                                            const int32_t *__restrict__ vwgt, const MatchOwners &own, int32_t *match,
-                                           const int32_t *prev, int32_t *cur, const LongChunk *chunks,
+                                           uint32_t *bits, const int32_t *prev, int32_t *cur, const LongChunk *chunks,
                                            const RowLists &rl, const unsigned *nchunks, const LongPartial &part,
                                            unsigned *flags, int r, unsigned nshort, int propose, int64_t maxvw,
                                            int short_deg, const int32_t *slist, const unsigned *vcount,
@@ -685,22 +712,43 @@ __device__ __forceinline__ void round_body(int64_t n, const OffT *__restrict__ x
   }
   if (blockIdx.x < nshort) {
     __shared__ int s_changed, s_active;
+    __shared__ uint32_t s_bits[VREGION / 32];
     if (threadIdx.x == 0) {
       s_changed = 0;
       s_active = 0;
     }
+    if (BITS && LIST && threadIdx.x < VREGION / 32) s_bits[threadIdx.x] = 0u;
     __syncthreads();
     bool changed = false, active = false;
     if constexpr (LIST) {  // the frontier: only vertices still unmatched are listed
       const unsigned cnt = vcount[blockIdx.x];
       const int32_t *list = slist + (int64_t)blockIdx.x * VREGION;
-      for (unsigned i = threadIdx.x; i < cnt; i += blockDim.x)
-        short_row<OffT>(list[i], n, xadj, adjncy, adjwgt, vwgt, own, match, prev, cur, have_prev, propose,
-                        maxvw, short_deg, changed, active);
+      for (unsigned i = threadIdx.x; i < cnt; i += blockDim.x) {
+        const int32_t v = list[i];
+        bool taken = false;
+        short_row<OffT, false, 4, BITS>(v, n, xadj, adjncy, adjwgt, vwgt, own, match, bits, prev, cur, have_prev,
+                                        propose, maxvw, short_deg, changed, active, taken);
+        // region blockIdx.x holds vertices [blockIdx.x * VREGION, + VREGION)
+        if (BITS && taken) atomicOr(&s_bits[(v & (VREGION - 1)) >> 5], 1u << (v & 31));
+      }
+      if constexpr (BITS) {
+        __syncthreads();
+        if (have_prev && threadIdx.x < VREGION / 32 && s_bits[threadIdx.x]) {
+          uint32_t *w = bits + (int64_t)blockIdx.x * (VREGION / 32) + threadIdx.x;
+          *w |= s_bits[threadIdx.x];  // no other block writes this word in this launch
+        }
+      }
     } else {
-      short_row<OffT, INIT, MESH ? MESH_BATCH : 4>((int64_t)blockIdx.x * blockDim.x + threadIdx.x, n, xadj, adjncy,
-                                                   adjwgt, vwgt, own,
-                            match, prev, cur, have_prev, propose, maxvw, short_deg, changed, active);
+      const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+      bool taken = false;
+      short_row<OffT, INIT, MESH ? MESH_BATCH : 4, BITS>(v, n, xadj, adjncy, adjwgt, vwgt, own, match, bits, prev,
+                                                         cur, have_prev, propose, maxvw, short_deg, changed, active,
+                                                         taken);
+      if constexpr (BITS) {
+        const uint64_t mk = __ballot(taken);
+        const int64_t w0 = v - lane();  // the wave's first vertex, a multiple of 64
+        if (have_prev && lane() == 0 && w0 < n) reinterpret_cast<uint64_t *>(bits)[w0 >> 6] = mk;
+      }
     }
     if (have_prev) {
       if (__ballot(changed) && lane() == 0) s_changed = 1;
@@ -737,37 +785,38 @@ __device__ __forceinline__ void round_body(int64_t n, const OffT *__restrict__ x
   const int64_t nw = (int64_t)(gridDim.x - nshort) * (blockDim.x / CM_WAVE);
   for (int64_t it = ((int64_t)(blockIdx.x - nshort) * blockDim.x + threadIdx.x) / CM_WAVE; it < items; it += nw) {
     if (it < nh)
-      hub_chunk<OffT>(it, chunks, xadj, adjncy, adjwgt, vwgt, own, match, prev, cur, part, have_prev, maxvw,
-                      (unsigned)r + 1u);
+      hub_chunk<OffT, BITS>(it, chunks, xadj, adjncy, adjwgt, vwgt, own, match, bits, prev, cur, part, have_prev,
+                            maxvw, (unsigned)r + 1u);
     else if (it < gend[0])
-      small_group<class_lanes(0), OffT>((it - nh) * MID_GROUP, nchunks[1], rl.rows[0], xadj, adjncy, adjwgt, vwgt,
-                                        own, match, prev, cur, have_prev, maxvw);
+      small_group<class_lanes(0), OffT, BITS>((it - nh) * MID_GROUP, nchunks[1], rl.rows[0], xadj, adjncy, adjwgt,
+                                              vwgt, own, match, bits, prev, cur, have_prev, maxvw);
     else if (it < gend[1])
-      small_group<class_lanes(1), OffT>((it - gend[0]) * MID_GROUP, nchunks[2], rl.rows[1], xadj, adjncy, adjwgt,
-                                        vwgt, own, match, prev, cur, have_prev, maxvw);
+      small_group<class_lanes(1), OffT, BITS>((it - gend[0]) * MID_GROUP, nchunks[2], rl.rows[1], xadj, adjncy,
+                                              adjwgt, vwgt, own, match, bits, prev, cur, have_prev, maxvw);
     else if (it < gend[2])
-      small_group<class_lanes(2), OffT>((it - gend[1]) * MID_GROUP, nchunks[3], rl.rows[2], xadj, adjncy, adjwgt,
-                                        vwgt, own, match, prev, cur, have_prev, maxvw);
+      small_group<class_lanes(2), OffT, BITS>((it - gend[1]) * MID_GROUP, nchunks[3], rl.rows[2], xadj, adjncy,
+                                              adjwgt, vwgt, own, match, bits, prev, cur, have_prev, maxvw);
     else
-      mid_group<OffT>((it - gend[2]) * WAVE_GROUP, nchunks[1 + MID_CLASS], rl.rows[MID_CLASS], xadj, adjncy, adjwgt,
-                      vwgt, own, match, prev, cur, have_prev, maxvw);
+      mid_group<OffT, BITS>((it - gend[2]) * WAVE_GROUP, nchunks[1 + MID_CLASS], rl.rows[MID_CLASS], xadj, adjncy,
+                            adjwgt, vwgt, own, match, bits, prev, cur, have_prev, maxvw);
   }
 }
 
 // Launch r of the handshake over all vertices and the level's long-row lists.
-template <typename OffT>
+template <typename OffT, bool BITS>
 __global__ __launch_bounds__(256, 7) void k_round(int64_t n, const OffT *__restrict__ xadj,
                                                const int32_t *__restrict__ adjncy,
                                                const int32_t *__restrict__ adjwgt,
                                                const int32_t *__restrict__ vwgt, MatchOwners own,
-                                               int32_t *match, const int32_t *prev, int32_t *cur,
+                                               int32_t *match, uint32_t *bits, const int32_t *prev, int32_t *cur,
                                                const LongChunk *chunks, RowLists rl, const unsigned *nchunks,
                                                LongPartial part, unsigned *flags, int r, unsigned nshort,
                                                int propose, int64_t maxvw, int short_deg, const int64_t *np)
 {
   if (np) n = *np;  // queued ahead: the level's count was not yet on the host
-  round_body<OffT, false>(n, xadj, adjncy, adjwgt, vwgt, own, match, prev, cur, chunks, rl, nchunks, part, flags, r,
-                          nshort, propose, maxvw, short_deg, nullptr, nullptr);
+  round_body<OffT, false, false, false, BITS>(n, xadj, adjncy, adjwgt, vwgt, own, match, bits, prev, cur, chunks, rl,
+                                              nchunks, part, flags, r, nshort, propose, maxvw, short_deg, nullptr,
+                                              nullptr);
 }
 
 // Launch r of a level without long rows (thread-per-row path only).
@@ -782,8 +831,8 @@ __global__ __launch_bounds__(256, 7) void k_round_mesh(int64_t n, const OffT *__
                                                     int propose, int64_t maxvw, int short_deg, const int64_t *np)
 {
   if (np) n = *np;  // queued ahead: the level's count was not yet on the host
-  round_body<OffT, false, false, true>(n, xadj, adjncy, adjwgt, vwgt, own, match, prev, cur, chunks, rl, nchunks, part,
-                                       flags, r, nshort, propose, maxvw, short_deg, nullptr, nullptr);
+  round_body<OffT, false, false, true>(n, xadj, adjncy, adjwgt, vwgt, own, match, nullptr, prev, cur, chunks, rl,
+                                       nchunks, part, flags, r, nshort, propose, maxvw, short_deg, nullptr, nullptr);
 }
 
 // Launch 0 (round 0 proposals only: no accept, no prev), compiled for r = 0.
@@ -800,24 +849,27 @@ __global__ __launch_bounds__(256, 7) void k_round0(int64_t n, const OffT *__rest
                                                 unsigned *hzero, const int64_t *np)
 {
   if (np) n = *np;  // queued ahead: the level's count was not yet on the host
-  round_body<OffT, false, INIT, INIT>(n, xadj, adjncy, adjwgt, vwgt, own, match, prev, cur, chunks, rl, nchunks, part,
-                                flags, 0, nshort, 1, maxvw, short_deg, nullptr, nullptr, nflags, hzero);
+  round_body<OffT, false, INIT, INIT>(n, xadj, adjncy, adjwgt, vwgt, own, match, nullptr, prev, cur, chunks, rl,
+                                      nchunks, part, flags, 0, nshort, 1, maxvw, short_deg, nullptr, nullptr, nflags,
+                                      hzero);
 }
 
 // Launch r over the frontier (vertex regions and compacted lists).
-template <typename OffT>
+template <typename OffT, bool BITS>
 __global__ __launch_bounds__(256, 7) void k_round_list(int64_t n, const OffT *__restrict__ xadj,
                                                     const int32_t *__restrict__ adjncy,
                                                     const int32_t *__restrict__ adjwgt,
                                                     const int32_t *__restrict__ vwgt, MatchOwners own,
-                                                    int32_t *match, const int32_t *prev, int32_t *cur,
+                                                    int32_t *match, uint32_t *bits, const int32_t *prev,
+                                                    int32_t *cur,
                                                     const LongChunk *chunks, RowLists rl, const unsigned *nchunks,
                                                     LongPartial part, unsigned *flags, int r, unsigned nshort,
                                                     int propose, int64_t maxvw, int short_deg,
                                                     const int32_t *slist, const unsigned *vcount)
 {
-  round_body<OffT, true>(n, xadj, adjncy, adjwgt, vwgt, own, match, prev, cur, chunks, rl, nchunks, part, flags, r,
-                         nshort, propose, maxvw, short_deg, slist, vcount);
+  round_body<OffT, true, false, false, BITS>(n, xadj, adjncy, adjwgt, vwgt, own, match, bits, prev, cur, chunks, rl,
+                                             nchunks, part, flags, r, nshort, propose, maxvw, short_deg, slist,
+                                             vcount);
 }
 
 // out[v] = match[v], with vertices still unmatched (-1) matched to themselves;
@@ -843,6 +895,7 @@ struct MatchRun {
   int max_rounds = 0;
   int32_t *match = nullptr;
   int32_t *prop = nullptr;
+  uint32_t *bits = nullptr;     // taken bits (large levels with long rows)
   LongChunk *chunks = nullptr;  // hub chunks
   RowLists rl{};                // rows by class
   int short_deg = LONG_DEG;     // longest row on the thread-per-row path
@@ -920,9 +973,11 @@ static void match_start(cm_ctx *ctx, MatchRun &mr)
   // thread-per-row blocks cover every vertex); the long-row lists are
   // counted before launch 0, into words that k_match_init clears
   mr.fused_init = !has_long;
+  const int64_t nbw = !mr.fused_init && n >= BITS_MIN_N ? (n + 63) / 64 : 0;
+  if (nbw) mr.bits = (uint32_t *)al(sizeof(uint64_t) * nbw);
   if (!mr.fused_init)
     launch(ctx, "match_init", 8.0 * n, k_match_init, dim3(mr.g), dim3(256), 0, n, mr.match, mr.prop, mr.flags,
-           2 * (int64_t)mr.max_rounds + 16, ctx->d_ctr + HASH_CTR_SLOT);
+           2 * (int64_t)mr.max_rounds + 16, ctx->d_ctr + HASH_CTR_SLOT, (uint64_t *)mr.bits, nbw);
   ctx->hash_ctr_zero = true;  // by k_match_init or by launch 0
   // extra blocks whose waves stride over the long-row chunks (none when the
   // level is known to have no long row)
@@ -953,20 +1008,25 @@ static void match_launch_direct(cm_ctx *ctx, MatchRun &mr, int count)
     const unsigned *counts = lst ? mr.nchunks2 : mr.nchunks;
     const unsigned gshort = lst ? mr.gs : mr.g, glong = mr.glong;
     if (lst)
-      launch(ctx, "match_round", 0.0, k_round_list<OffT>, dim3(gshort + glong), dim3(256), 0, n, (const OffT *)L.xadj,
-             L.adjncy, L.adjwgt, L.vwgt, mr.own, mr.match, prv, cur, chunks, mr.rl2, counts, mr.part, mr.flags, r,
-             gshort, r < mr.max_rounds ? 1 : 0, L.maxvw, mr.short_deg, (const int32_t *)mr.slist,
+      launch(ctx, "match_round", 0.0, mr.bits ? k_round_list<OffT, true> : k_round_list<OffT, false>,
+             dim3(gshort + glong), dim3(256), 0, n, (const OffT *)L.xadj, L.adjncy, L.adjwgt, L.vwgt, mr.own,
+             mr.match, mr.bits, prv, cur, chunks, mr.rl2, counts, mr.part, mr.flags, r, gshort,
+             r < mr.max_rounds ? 1 : 0, L.maxvw, mr.short_deg, (const int32_t *)mr.slist,
              (const unsigned *)mr.vcount);
     else if (r == 0 && r < mr.max_rounds)
       launch(ctx, "match_round", row_bytes, mr.fused_init ? k_round0<OffT, true> : k_round0<OffT, false>,
              dim3(gshort + glong), dim3(256), 0, n, (const OffT *)L.xadj, L.adjncy, L.adjwgt, L.vwgt, mr.own,
              mr.match, const_cast<int32_t *>(prv), cur, chunks, mr.rl, counts, mr.part, mr.flags, gshort, L.maxvw,
              mr.short_deg, 2 * (int64_t)mr.max_rounds + 16, ctx->d_ctr + HASH_CTR_SLOT, mr.d_n);
-    else
-      launch(ctx, "match_round", r == 0 ? row_bytes : 0.0, mr.fused_init ? k_round_mesh<OffT> : k_round<OffT>,
-             dim3(gshort + glong), dim3(256), 0, n,
+    else if (mr.fused_init)
+      launch(ctx, "match_round", r == 0 ? row_bytes : 0.0, k_round_mesh<OffT>, dim3(gshort + glong), dim3(256), 0, n,
              (const OffT *)L.xadj, L.adjncy, L.adjwgt, L.vwgt, mr.own, mr.match, prv, cur, chunks, mr.rl, counts,
              mr.part, mr.flags, r, gshort, r < mr.max_rounds ? 1 : 0, L.maxvw, mr.short_deg, mr.d_n);
+    else
+      launch(ctx, "match_round", r == 0 ? row_bytes : 0.0, mr.bits ? k_round<OffT, true> : k_round<OffT, false>,
+             dim3(gshort + glong), dim3(256), 0, n, (const OffT *)L.xadj, L.adjncy, L.adjwgt, L.vwgt, mr.own,
+             mr.match, mr.bits, prv, cur, chunks, mr.rl, counts, mr.part, mr.flags, r, gshort,
+             r < mr.max_rounds ? 1 : 0, L.maxvw, mr.short_deg, mr.d_n);
     if (mr.ghub && r < mr.max_rounds)
       launch(ctx, "match_hub_reduce", 0.0, k_hub_reduce, dim3(mr.ghub), dim3(256), 0, chunks, counts, mr.part,
              cur, (const unsigned *)mr.flags, r);
@@ -997,6 +1057,7 @@ static std::string batch_key(const MatchRun &mr, int count)
   put(mr.max_rounds);
   put(mr.match);
   put(mr.prop);
+  put(mr.bits);
   put(mr.chunks);
   put(mr.rl);
   put(mr.short_deg);
