@@ -14,6 +14,9 @@
 // vertex ids and the seed makes the result independent of launch shape and
 // of a graph's position inside a batch.  The fine CSR is never reordered
 // (the reference sorts every row in place, cu:1723).
+// Launch r does the accepts of round r-1 and the proposals of round r.  On
+// large levels with long rows those are two kernels (k_accept, then the
+// round kernel reading complete taken bits); elsewhere one fused kernel.
 #include "cm_common.hpp"
 #include "cm_internal.hpp"
 
@@ -50,9 +53,11 @@ constexpr int ROUND_BATCH = MATCH_BATCH;  // launches queued between host checks
 // Row entries per scan step on levels without long rows (8 was slower on
 // the grid: a short row then loads more clamped duplicates).
 constexpr int MESH_BATCH = 4;
-// Levels with long rows and at least this many vertices keep taken bits
-// (a 4 MB match[] is the L2 of one XCD; its bit array is 128 KB).
+// Levels with long rows and at least this many vertices run their accepts in
+// a launch of their own and keep taken bits (a 4 MB match[] is the L2 of one
+// XCD; its bit array is 128 KB).
 constexpr int64_t BITS_MIN_N = 1 << 20;
+constexpr int TK_MATCH = 0, TK_SPLIT = 1;
 
 // flags[2r] = 1 if a vertex matched or died in round r, flags[2r+1] = 1 if
 // any vertex is still unmatched after round r.  Round r's accept runs in
@@ -301,16 +306,19 @@ __device__ __forceinline__ bool accepted_prev(const int32_t *prev, int32_t u)
 // Best eligible neighbour of v among vertices unmatched after round r-1, by
 // the strict order (adjwgt, hash(seed,min,max), min, max); -2 if none.
 // Lanes [l0, l0+step) stride the row; the caller reduces across lanes.
-// BITS: a neighbour's matched state comes from the taken bits (bit u of the
-// bit array: match[u] != -1), which stay in the L2 where match[] (32 times
-// larger) would not.
-template <typename OffT, int SCAN_BATCH = 4, bool BITS = false>
+// TK (how a neighbour's state is read): TK_MATCH, match[u] plus the proposals
+// of round r-1 (accepted_prev: two dependent gathers per candidate that would
+// beat the current best); TK_SPLIT, the taken bits alone (bit u: match[u] !=
+// -1; they stay in the L2 where match[], 32 times larger, would not),
+// complete because k_accept did round r-1's accepts in the launch before.
+template <typename OffT, int SCAN_BATCH = 4, int TK = TK_MATCH>
 __device__ __forceinline__ void scan_row(int64_t v, OffT b, OffT e, int l0, int step,
                                          const int32_t *__restrict__ adjncy,
                                          const int32_t *__restrict__ adjwgt,
                                          const int32_t *__restrict__ vwgt, int32_t vw, int32_t maxv,
                                          uint64_t seed, int64_t base, const int32_t *match,
-                                         const uint32_t *bits, const int32_t *prev, bool have_prev, bool check_w, bool wunit,
+                                         const uint32_t *bits, const int32_t *prev, bool have_prev, bool check_w,
+                                         bool wunit,
                                          int32_t &best, uint64_t &bk1, uint64_t &bk2)
 {
   // SCAN_BATCH entries per lane per step: their row loads and the gathers
@@ -339,7 +347,7 @@ __device__ __forceinline__ void scan_row(int64_t v, OffT b, OffT e, int l0, int 
       const int32_t uu = look ? u[t] : (int32_t)v;
       // round 0 runs before any accept, so every neighbour is unmatched: all
       // lanes read one shared word instead of gathering
-      if constexpr (BITS)
+      if constexpr (TK == TK_SPLIT)
         mu[t] = (bits[have_prev ? (uint32_t)uu >> 5 : 0u] >> (uu & 31)) & 1u ? 0 : -1;
       else
         mu[t] = match[have_prev ? uu : 0];
@@ -355,7 +363,8 @@ __device__ __forceinline__ void scan_row(int64_t v, OffT b, OffT e, int l0, int 
       const uint64_t k2 = ((uint64_t)a << 32) | bb;
       // the two dependent gathers of accepted_prev only for edges that would
       // beat the current best (about ln(degree) of them for random keys)
-      if ((best == -2 || key_better(k1, k2, bk1, bk2)) && !(have_prev && accepted_prev(prev, u[t]))) {
+      if ((best == -2 || key_better(k1, k2, bk1, bk2)) &&
+          !(TK != TK_SPLIT && have_prev && accepted_prev(prev, u[t]))) {
         best = u[t];
         bk1 = k1;
         bk2 = k2;
@@ -393,7 +402,7 @@ struct LongPartial {
 // to part[q] with plain stores; the next kernel on the stream reduces the
 // row's chunks (a hand-off inside the launch would need an L2 writeback and
 // invalidate per chunk for its release/acquire pair).
-template <typename OffT, bool BITS>
+template <typename OffT, int TK>
 __device__ __forceinline__ void hub_chunk(int64_t q, const LongChunk *chunks, const OffT *__restrict__ xadj,
                                           const int32_t *__restrict__ adjncy, const int32_t *__restrict__ adjwgt,
                                           const int32_t *__restrict__ vwgt, const MatchOwners &own,
@@ -408,16 +417,18 @@ __device__ __forceinline__ void hub_chunk(int64_t q, const LongChunk *chunks, co
   // loads by dependency level, as in the short path.  A vertex matched
   // before this launch wrote no proposal in round r-1, so its prev entry
   // is stale and must not be followed: test match first.
+  // TK_SPLIT: match[] is complete, so accepted_prev is never needed
+  constexpr bool ACC = TK != TK_SPLIT;
   const int32_t mv = match[v];
   const int32_t p = have_prev ? prev[v] : -1;
   if (mv != -1) return;
   const int32_t pc = p >= 0 ? p : v;
-  const int32_t pp = have_prev ? prev[pc] : -1;
+  const int32_t pp = ACC && have_prev ? prev[pc] : -1;
   const int32_t mp = match[pc];
   const OffT xb = xadj[v], xe = xadj[v + 1];
   const int32_t vw = vwgt[v];
-  const int32_t ppp = have_prev ? prev[pp >= 0 ? pp : pc] : -1;
-  const bool v_acc = have_prev && (p == -2 || (p >= 0 && pp == v));  // accepted_prev(prev, v)
+  const int32_t ppp = ACC && have_prev ? prev[pp >= 0 ? pp : pc] : -1;
+  const bool v_acc = ACC && have_prev && (p == -2 || (p >= 0 && pp == v));  // accepted_prev(prev, v)
   if (v_acc) return;
   const bool p_acc = pp == -2 || (pp >= 0 && ppp == p);
   if (p >= 0 && mp == -1 && !p_acc) {
@@ -432,7 +443,7 @@ __device__ __forceinline__ void hub_chunk(int64_t q, const LongChunk *chunks, co
   uint64_t bk1 = 0, bk2 = 0;
   const OffT b = xb + (OffT)c.k * LONG_CHUNK, e0 = xe;
   const OffT e = e0 - b > (OffT)LONG_CHUNK ? b + (OffT)LONG_CHUNK : e0;
-  scan_row<OffT, 4, BITS>(v, b, e, l, CM_WAVE, adjncy, adjwgt, vwgt, vw, maxv, seed, base, match, bits, prev,
+  scan_row<OffT, 4, TK>(v, b, e, l, CM_WAVE, adjncy, adjwgt, vwgt, vw, maxv, seed, base, match, bits, prev,
                           have_prev, maxvw < 0 || (int64_t)vw + maxvw > maxv, own.wunit, best, bk1, bk2);
   wave_best(best, bk1, bk2);
   if (l == 0) {
@@ -476,7 +487,7 @@ __global__ __launch_bounds__(256) void k_hub_reduce(const LongChunk *chunks, con
 // then the wave scans the rows that need it one after another, with each
 // row's parameters read from its lane.  A wave thus waits for the state
 // loads once per group instead of once per row.
-template <typename OffT, bool BITS>
+template <typename OffT, int TK>
 __device__ __forceinline__ void mid_group(int64_t m0, unsigned nm, const int32_t *mids,
                                           const OffT *__restrict__ xadj, const int32_t *__restrict__ adjncy,
                                           const int32_t *__restrict__ adjwgt, const int32_t *__restrict__ vwgt,
@@ -498,10 +509,11 @@ __device__ __forceinline__ void mid_group(int64_t m0, unsigned nm, const int32_t
   owner_params(own, v, maxv, seed, base);
   // v's own prev entry is stale once v is matched: it is not followed
   const int32_t pc = mv == -1 && p >= 0 ? p : v;
-  const int32_t pp = have_prev ? prev[pc] : -1;
+  constexpr bool ACC = TK != TK_SPLIT;  // TK_SPLIT: match[] is complete
+  const int32_t pp = ACC && have_prev ? prev[pc] : -1;
   const int32_t mp = match[pc];
-  const int32_t ppp = have_prev ? prev[pp >= 0 ? pp : pc] : -1;
-  const bool v_acc = have_prev && (p == -2 || (p >= 0 && pp == v));  // accepted_prev(prev, v)
+  const int32_t ppp = ACC && have_prev ? prev[pp >= 0 ? pp : pc] : -1;
+  const bool v_acc = ACC && have_prev && (p == -2 || (p >= 0 && pp == v));  // accepted_prev(prev, v)
   const bool p_acc = pp == -2 || (pp >= 0 && ppp == p);
   const bool live = has && mv == -1 && !v_acc;
   const bool reuse = p >= 0 && mp == -1 && !p_acc;
@@ -515,7 +527,7 @@ __device__ __forceinline__ void mid_group(int64_t m0, unsigned nm, const int32_t
     const OffT bi = (OffT)bcast((int64_t)xb, i), ei = (OffT)bcast((int64_t)xe, i);
     int32_t best = -2;
     uint64_t bk1 = 0, bk2 = 0;
-    scan_row<OffT, 4, BITS>(vi, bi, ei, l, CM_WAVE, adjncy, adjwgt, vwgt, bcast(vw, i), bcast(maxv, i),
+    scan_row<OffT, 4, TK>(vi, bi, ei, l, CM_WAVE, adjncy, adjwgt, vwgt, bcast(vw, i), bcast(maxv, i),
                             bcast(seed, i), bcast(base, i), match, bits, prev, have_prev, bcast(cw, i) != 0,
                             own.wunit, best, bk1, bk2);
     wave_best(best, bk1, bk2);
@@ -544,7 +556,7 @@ __device__ __forceinline__ void group_best(int32_t &best, uint64_t &bk1, uint64_
 // state as in mid_group (lane i for row i), then the rows to scan go
 // CM_WAVE / SUB at a time, one per group of SUB lanes (such a row is one step
 // of SCAN_BATCH entries per lane).
-template <int SUB, typename OffT, bool BITS>
+template <int SUB, typename OffT, int TK>
 __device__ __forceinline__ void small_group(int64_t m0, unsigned ns, const int32_t *smids,
                                             const OffT *__restrict__ xadj, const int32_t *__restrict__ adjncy,
                                             const int32_t *__restrict__ adjwgt, const int32_t *__restrict__ vwgt,
@@ -565,10 +577,11 @@ __device__ __forceinline__ void small_group(int64_t m0, unsigned ns, const int32
   int64_t base;
   owner_params(own, v, maxv, seed, base);
   const int32_t pc = mv == -1 && p >= 0 ? p : v;
-  const int32_t pp = have_prev ? prev[pc] : -1;
+  constexpr bool ACC = TK != TK_SPLIT;  // TK_SPLIT: match[] is complete
+  const int32_t pp = ACC && have_prev ? prev[pc] : -1;
   const int32_t mp = match[pc];
-  const int32_t ppp = have_prev ? prev[pp >= 0 ? pp : pc] : -1;
-  const bool v_acc = have_prev && (p == -2 || (p >= 0 && pp == v));  // accepted_prev(prev, v)
+  const int32_t ppp = ACC && have_prev ? prev[pp >= 0 ? pp : pc] : -1;
+  const bool v_acc = ACC && have_prev && (p == -2 || (p >= 0 && pp == v));  // accepted_prev(prev, v)
   const bool p_acc = pp == -2 || (pp >= 0 && ppp == p);
   const bool live = has && mv == -1 && !v_acc;
   const bool reuse = p >= 0 && mp == -1 && !p_acc;
@@ -592,7 +605,7 @@ __device__ __forceinline__ void small_group(int64_t m0, unsigned ns, const int32
     int32_t best = -2;
     uint64_t bk1 = 0, bk2 = 0;
     if (i >= 0)
-      scan_row<OffT, 4, BITS>(vi, (OffT)bi, (OffT)ei, gl, SUB, adjncy, adjwgt, vwgt, vwi, maxvi, seedi, basei,
+      scan_row<OffT, 4, TK>(vi, (OffT)bi, (OffT)ei, gl, SUB, adjncy, adjwgt, vwgt, vwi, maxvi, seedi, basei,
                               match, bits, prev, have_prev, cwi != 0, own.wunit, best, bk1, bk2);
     group_best<SUB>(best, bk1, bk2);
     if (i >= 0 && gl == 0) cur[vi] = best;
@@ -614,14 +627,13 @@ __device__ __forceinline__ void small_group(int64_t m0, unsigned ns, const int32
 // launch initialises itself: its scans read this word instead of match[0].
 __device__ const int32_t unmatched_word = -1;
 
-// taken is set when v (< n) is matched after this accept.
-template <typename OffT, bool INIT = false, int SB = 4, bool BITS = false>
+// TK_SPLIT: no accept here (k_accept did it), and match[] is complete.
+template <typename OffT, bool INIT = false, int SB = 4, int TK = TK_MATCH>
 __device__ __forceinline__ void short_row(int64_t v, int64_t n, const OffT *__restrict__ xadj,
                                           const int32_t *__restrict__ adjncy, const int32_t *__restrict__ adjwgt,
                                           const int32_t *__restrict__ vwgt, const MatchOwners &own, int32_t *match,
                                           const uint32_t *bits, const int32_t *prev, int32_t *cur, bool have_prev,
-                                          int propose, int64_t maxvw, int short_deg, bool &changed, bool &active,
-                                          bool &taken)
+                                          int propose, int64_t maxvw, int short_deg, bool &changed, bool &active)
 {
   // INIT (launch 0 of a level without long rows, which has no k_match_init):
   // v is unmatched; this thread writes match[v] and the other proposal
@@ -635,16 +647,16 @@ __device__ __forceinline__ void short_row(int64_t v, int64_t n, const OffT *__re
   const int64_t vc = v < n ? v : n - 1;
   const int32_t mv = INIT ? -1 : match[vc];
   const int32_t p = have_prev ? prev[vc] : -1;
-  taken = v < n && mv != -1;
   if (v < n && mv == -1) {
+    constexpr bool ACC = TK != TK_SPLIT;
     const int32_t pc = p >= 0 ? p : (int32_t)v;
-    const int32_t pp = have_prev ? prev[pc] : -1;  // prev[p]
+    const int32_t pp = ACC && have_prev ? prev[pc] : -1;  // prev[p]
     const int32_t mp = match[pc];
     const OffT b = xadj[v], e = xadj[v + 1];
     const int32_t vw = vwgt[v];
-    const int32_t ppp = have_prev ? prev[pp >= 0 ? pp : pc] : -1;  // prev[prev[p]]
+    const int32_t ppp = ACC && have_prev ? prev[pp >= 0 ? pp : pc] : -1;  // prev[prev[p]]
     bool done = false;
-    if (have_prev) {
+    if (ACC && have_prev) {
       if (p == -2) {
         match[v] = (int32_t)v;
         done = true;
@@ -655,7 +667,6 @@ __device__ __forceinline__ void short_row(int64_t v, int64_t n, const OffT *__re
     }
     changed |= done;
     active |= !done;
-    taken = done;
     if (!done && propose && (int64_t)(e - b) <= short_deg) {
       // p accepted in round r-1 (from the proposals alone): dead or mutual
       const bool p_acc = pp == -2 || (pp >= 0 && ppp == p);
@@ -668,7 +679,7 @@ __device__ __forceinline__ void short_row(int64_t v, int64_t n, const OffT *__re
         owner_params(own, v, maxv, seed, base);
         int32_t best = -2;
         uint64_t bk1 = 0, bk2 = 0;
-        scan_row<OffT, SB, BITS>(v, b, e, 0, 1, adjncy, adjwgt, vwgt, vw, maxv, seed, base,
+        scan_row<OffT, SB, TK>(v, b, e, 0, 1, adjncy, adjwgt, vwgt, vw, maxv, seed, base,
                                  INIT ? &unmatched_word : match, bits, prev, have_prev,
                                  maxvw < 0 || (int64_t)vw + maxvw > maxv, own.wunit, best, bk1, bk2);
         cur[v] = best;
@@ -686,24 +697,22 @@ __device__ __forceinline__ void short_row(int64_t v, int64_t n, const OffT *__re
 // thread-per-row path, no long-row lists): the long-row paths are compiled
 // out, which leaves the kernel without SGPR spills (the full round kernel
 // spills 34 to VGPR lanes); grid -3.6%.
-// BITS: large levels with long rows keep the taken bits beside match[] (set
-// by the accepts; scans read them).  A wave of the thread-per-row blocks owns
-// the two words of its 64 vertices and stores them whole; a vertex region of
-// the list launches owns its 32 words and or-s its new matches into them.
-// A reader in the same launch may see a word before or after this launch's
-// store: the bits it adds are the vertices accepted in round r-1, which the
-// scan also finds through the proposals, so both give one answer.
-template <typename OffT, bool LIST, bool INIT = false, bool MESH = false, bool BITS = false>
+// TK_SPLIT: the accepts (and taken bits and round flags) are k_accept's,
+// launched just before; this launch only proposes, and not at all when round
+// r-1 left nothing to do.
+template <typename OffT, bool LIST, bool INIT = false, bool MESH = false, int TK = TK_MATCH>
 __device__ __forceinline__ void round_body(int64_t n, const OffT *__restrict__ xadj,
                                            const int32_t *__restrict__ adjncy, const int32_t *__restrict__ adjwgt,
                                            const int32_t *__restrict__ vwgt, const MatchOwners &own, int32_t *match,
-                                           uint32_t *bits, const int32_t *prev, int32_t *cur, const LongChunk *chunks,
+                                           const uint32_t *bits, const int32_t *prev, int32_t *cur,
+                                           const LongChunk *chunks,
                                            const RowLists &rl, const unsigned *nchunks, const LongPartial &part,
                                            unsigned *flags, int r, unsigned nshort, int propose, int64_t maxvw,
                                            int short_deg, const int32_t *slist, const unsigned *vcount,
                                            int64_t nflags = 0, unsigned *hzero = nullptr)
 {
   if (r >= 2 && !round_live(flags, r - 2)) return;
+  if (TK == TK_SPLIT && r >= 1 && !round_live(flags, r - 1)) return;
   const bool have_prev = r >= 1;
   if (INIT) {  // the rest of k_match_init: the round flags and the hash counters
     const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -712,45 +721,24 @@ __device__ __forceinline__ void round_body(int64_t n, const OffT *__restrict__ x
   }
   if (blockIdx.x < nshort) {
     __shared__ int s_changed, s_active;
-    __shared__ uint32_t s_bits[VREGION / 32];
     if (threadIdx.x == 0) {
       s_changed = 0;
       s_active = 0;
     }
-    if (BITS && LIST && threadIdx.x < VREGION / 32) s_bits[threadIdx.x] = 0u;
     __syncthreads();
     bool changed = false, active = false;
     if constexpr (LIST) {  // the frontier: only vertices still unmatched are listed
       const unsigned cnt = vcount[blockIdx.x];
       const int32_t *list = slist + (int64_t)blockIdx.x * VREGION;
-      for (unsigned i = threadIdx.x; i < cnt; i += blockDim.x) {
-        const int32_t v = list[i];
-        bool taken = false;
-        short_row<OffT, false, 4, BITS>(v, n, xadj, adjncy, adjwgt, vwgt, own, match, bits, prev, cur, have_prev,
-                                        propose, maxvw, short_deg, changed, active, taken);
-        // region blockIdx.x holds vertices [blockIdx.x * VREGION, + VREGION)
-        if (BITS && taken) atomicOr(&s_bits[(v & (VREGION - 1)) >> 5], 1u << (v & 31));
-      }
-      if constexpr (BITS) {
-        __syncthreads();
-        if (have_prev && threadIdx.x < VREGION / 32 && s_bits[threadIdx.x]) {
-          uint32_t *w = bits + (int64_t)blockIdx.x * (VREGION / 32) + threadIdx.x;
-          *w |= s_bits[threadIdx.x];  // no other block writes this word in this launch
-        }
-      }
+      for (unsigned i = threadIdx.x; i < cnt; i += blockDim.x)
+        short_row<OffT, false, 4, TK>(list[i], n, xadj, adjncy, adjwgt, vwgt, own, match, bits, prev, cur,
+                                      have_prev, propose, maxvw, short_deg, changed, active);
     } else {
-      const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-      bool taken = false;
-      short_row<OffT, INIT, MESH ? MESH_BATCH : 4, BITS>(v, n, xadj, adjncy, adjwgt, vwgt, own, match, bits, prev,
-                                                         cur, have_prev, propose, maxvw, short_deg, changed, active,
-                                                         taken);
-      if constexpr (BITS) {
-        const uint64_t mk = __ballot(taken);
-        const int64_t w0 = v - lane();  // the wave's first vertex, a multiple of 64
-        if (have_prev && lane() == 0 && w0 < n) reinterpret_cast<uint64_t *>(bits)[w0 >> 6] = mk;
-      }
+      short_row<OffT, INIT, MESH ? MESH_BATCH : 4, TK>((int64_t)blockIdx.x * blockDim.x + threadIdx.x, n, xadj,
+                                                       adjncy, adjwgt, vwgt, own, match, bits, prev, cur, have_prev,
+                                                       propose, maxvw, short_deg, changed, active);
     }
-    if (have_prev) {
+    if (TK != TK_SPLIT && have_prev) {
       if (__ballot(changed) && lane() == 0) s_changed = 1;
       if (__ballot(active) && lane() == 0) s_active = 1;
       __syncthreads();
@@ -785,36 +773,36 @@ __device__ __forceinline__ void round_body(int64_t n, const OffT *__restrict__ x
   const int64_t nw = (int64_t)(gridDim.x - nshort) * (blockDim.x / CM_WAVE);
   for (int64_t it = ((int64_t)(blockIdx.x - nshort) * blockDim.x + threadIdx.x) / CM_WAVE; it < items; it += nw) {
     if (it < nh)
-      hub_chunk<OffT, BITS>(it, chunks, xadj, adjncy, adjwgt, vwgt, own, match, bits, prev, cur, part, have_prev,
+      hub_chunk<OffT, TK>(it, chunks, xadj, adjncy, adjwgt, vwgt, own, match, bits, prev, cur, part, have_prev,
                             maxvw, (unsigned)r + 1u);
     else if (it < gend[0])
-      small_group<class_lanes(0), OffT, BITS>((it - nh) * MID_GROUP, nchunks[1], rl.rows[0], xadj, adjncy, adjwgt,
+      small_group<class_lanes(0), OffT, TK>((it - nh) * MID_GROUP, nchunks[1], rl.rows[0], xadj, adjncy, adjwgt,
                                               vwgt, own, match, bits, prev, cur, have_prev, maxvw);
     else if (it < gend[1])
-      small_group<class_lanes(1), OffT, BITS>((it - gend[0]) * MID_GROUP, nchunks[2], rl.rows[1], xadj, adjncy,
+      small_group<class_lanes(1), OffT, TK>((it - gend[0]) * MID_GROUP, nchunks[2], rl.rows[1], xadj, adjncy,
                                               adjwgt, vwgt, own, match, bits, prev, cur, have_prev, maxvw);
     else if (it < gend[2])
-      small_group<class_lanes(2), OffT, BITS>((it - gend[1]) * MID_GROUP, nchunks[3], rl.rows[2], xadj, adjncy,
+      small_group<class_lanes(2), OffT, TK>((it - gend[1]) * MID_GROUP, nchunks[3], rl.rows[2], xadj, adjncy,
                                               adjwgt, vwgt, own, match, bits, prev, cur, have_prev, maxvw);
     else
-      mid_group<OffT, BITS>((it - gend[2]) * WAVE_GROUP, nchunks[1 + MID_CLASS], rl.rows[MID_CLASS], xadj, adjncy,
+      mid_group<OffT, TK>((it - gend[2]) * WAVE_GROUP, nchunks[1 + MID_CLASS], rl.rows[MID_CLASS], xadj, adjncy,
                             adjwgt, vwgt, own, match, bits, prev, cur, have_prev, maxvw);
   }
 }
 
 // Launch r of the handshake over all vertices and the level's long-row lists.
-template <typename OffT, bool BITS>
+template <typename OffT, int TK>
 __global__ __launch_bounds__(256, 7) void k_round(int64_t n, const OffT *__restrict__ xadj,
                                                const int32_t *__restrict__ adjncy,
                                                const int32_t *__restrict__ adjwgt,
                                                const int32_t *__restrict__ vwgt, MatchOwners own,
-                                               int32_t *match, uint32_t *bits, const int32_t *prev, int32_t *cur,
+                                               int32_t *match, const uint32_t *bits, const int32_t *prev, int32_t *cur,
                                                const LongChunk *chunks, RowLists rl, const unsigned *nchunks,
                                                LongPartial part, unsigned *flags, int r, unsigned nshort,
                                                int propose, int64_t maxvw, int short_deg, const int64_t *np)
 {
   if (np) n = *np;  // queued ahead: the level's count was not yet on the host
-  round_body<OffT, false, false, false, BITS>(n, xadj, adjncy, adjwgt, vwgt, own, match, bits, prev, cur, chunks, rl,
+  round_body<OffT, false, false, false, TK>(n, xadj, adjncy, adjwgt, vwgt, own, match, bits, prev, cur, chunks, rl,
                                               nchunks, part, flags, r, nshort, propose, maxvw, short_deg, nullptr,
                                               nullptr);
 }
@@ -855,21 +843,77 @@ __global__ __launch_bounds__(256, 7) void k_round0(int64_t n, const OffT *__rest
 }
 
 // Launch r over the frontier (vertex regions and compacted lists).
-template <typename OffT, bool BITS>
+template <typename OffT, int TK>
 __global__ __launch_bounds__(256, 7) void k_round_list(int64_t n, const OffT *__restrict__ xadj,
                                                     const int32_t *__restrict__ adjncy,
                                                     const int32_t *__restrict__ adjwgt,
                                                     const int32_t *__restrict__ vwgt, MatchOwners own,
-                                                    int32_t *match, uint32_t *bits, const int32_t *prev,
+                                                    int32_t *match, const uint32_t *bits, const int32_t *prev,
                                                     int32_t *cur,
                                                     const LongChunk *chunks, RowLists rl, const unsigned *nchunks,
                                                     LongPartial part, unsigned *flags, int r, unsigned nshort,
                                                     int propose, int64_t maxvw, int short_deg,
                                                     const int32_t *slist, const unsigned *vcount)
 {
-  round_body<OffT, true, false, false, BITS>(n, xadj, adjncy, adjwgt, vwgt, own, match, bits, prev, cur, chunks, rl,
+  round_body<OffT, true, false, false, TK>(n, xadj, adjncy, adjwgt, vwgt, own, match, bits, prev, cur, chunks, rl,
                                              nchunks, part, flags, r, nshort, propose, maxvw, short_deg, slist,
                                              vcount);
+}
+
+// The accepts of round r-1 on a TK_SPLIT level (launch r's first kernel):
+// every vertex still unmatched (LIST: each listed vertex of region
+// blockIdx.x) takes its mutual partner, or itself if it proposed to no one
+// (-2).  It writes the taken bits, and round r-1's flags as round_body does
+// on the other levels.
+template <bool LIST>
+__global__ __launch_bounds__(256) void k_accept(int64_t n, int32_t *match, uint32_t *bits, const int32_t *prev,
+                                                unsigned *flags, int r, const int32_t *slist, const unsigned *vcount)
+{
+  if (r >= 2 && !round_live(flags, r - 2)) return;
+  __shared__ int s_changed, s_active;
+  __shared__ uint32_t s_bits[VREGION / 32];
+  if (threadIdx.x == 0) {
+    s_changed = 0;
+    s_active = 0;
+  }
+  if (LIST && threadIdx.x < VREGION / 32) s_bits[threadIdx.x] = 0u;
+  __syncthreads();
+  bool changed = false, active = false;
+  // v < n; returns whether v is matched after the accept
+  auto accept = [&](int64_t v) -> bool {
+    const int32_t mv = match[v];
+    const int32_t p = prev[v];
+    if (mv != -1) return true;
+    const int32_t pp = prev[p >= 0 ? (int64_t)p : v];
+    const int32_t m = p == -2 ? (int32_t)v : (p >= 0 && pp == (int32_t)v ? p : -1);
+    if (m != -1) match[v] = m;
+    changed |= m != -1;
+    active |= m == -1;
+    return m != -1;
+  };
+  if constexpr (LIST) {
+    const unsigned cnt = vcount[blockIdx.x];
+    const int32_t *list = slist + (int64_t)blockIdx.x * VREGION;
+    for (unsigned i = threadIdx.x; i < cnt; i += blockDim.x) {
+      const int32_t v = list[i];  // region blockIdx.x: [blockIdx.x * VREGION, + VREGION)
+      if (accept(v)) atomicOr(&s_bits[(v & (VREGION - 1)) >> 5], 1u << (v & 31));
+    }
+  } else {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t mk = __ballot(v < n && accept(v));
+    const int64_t w0 = v - lane();  // the wave's first vertex, a multiple of 64
+    if (lane() == 0 && w0 < n) reinterpret_cast<uint64_t *>(bits)[w0 >> 6] = mk;
+  }
+  if (__ballot(changed) && lane() == 0) s_changed = 1;
+  if (__ballot(active) && lane() == 0) s_active = 1;
+  __syncthreads();
+  if (LIST && threadIdx.x < VREGION / 32 && s_bits[threadIdx.x])
+    bits[(int64_t)blockIdx.x * (VREGION / 32) + threadIdx.x] |= s_bits[threadIdx.x];  // the region's own words
+  if (threadIdx.x == 0) {
+    unsigned *f = flags + 2 * (r - 1);
+    if (s_changed) f[0] = 1u;
+    if (s_active) f[1] = 1u;
+  }
 }
 
 // out[v] = match[v], with vertices still unmatched (-1) matched to themselves;
@@ -1007,8 +1051,15 @@ static void match_launch_direct(cm_ctx *ctx, MatchRun &mr, int count)
     const LongChunk *chunks = lst ? mr.chunks2 : mr.chunks;
     const unsigned *counts = lst ? mr.nchunks2 : mr.nchunks;
     const unsigned gshort = lst ? mr.gs : mr.g, glong = mr.glong;
-    if (lst)
-      launch(ctx, "match_round", 0.0, mr.bits ? k_round_list<OffT, true> : k_round_list<OffT, false>,
+    const int tk = mr.bits ? TK_SPLIT : TK_MATCH;
+    if (tk == TK_SPLIT && r >= 1)
+      launch(ctx, "match_accept", 0.0, lst ? k_accept<true> : k_accept<false>, dim3(gshort), dim3(256), 0, n,
+             mr.match, mr.bits, prv, mr.flags, r, (const int32_t *)mr.slist, (const unsigned *)mr.vcount);
+    if (tk == TK_SPLIT && r >= 1 && r == mr.max_rounds) {
+      // the last launch only accepts: k_accept was all of it
+    } else if (lst)
+      launch(ctx, "match_round", 0.0,
+             tk == TK_SPLIT ? k_round_list<OffT, TK_SPLIT> : k_round_list<OffT, TK_MATCH>,
              dim3(gshort + glong), dim3(256), 0, n, (const OffT *)L.xadj, L.adjncy, L.adjwgt, L.vwgt, mr.own,
              mr.match, mr.bits, prv, cur, chunks, mr.rl2, counts, mr.part, mr.flags, r, gshort,
              r < mr.max_rounds ? 1 : 0, L.maxvw, mr.short_deg, (const int32_t *)mr.slist,
@@ -1023,7 +1074,8 @@ static void match_launch_direct(cm_ctx *ctx, MatchRun &mr, int count)
              (const OffT *)L.xadj, L.adjncy, L.adjwgt, L.vwgt, mr.own, mr.match, prv, cur, chunks, mr.rl, counts,
              mr.part, mr.flags, r, gshort, r < mr.max_rounds ? 1 : 0, L.maxvw, mr.short_deg, mr.d_n);
     else
-      launch(ctx, "match_round", r == 0 ? row_bytes : 0.0, mr.bits ? k_round<OffT, true> : k_round<OffT, false>,
+      launch(ctx, "match_round", r == 0 ? row_bytes : 0.0,
+             tk == TK_SPLIT ? k_round<OffT, TK_SPLIT> : k_round<OffT, TK_MATCH>,
              dim3(gshort + glong), dim3(256), 0, n, (const OffT *)L.xadj, L.adjncy, L.adjwgt, L.vwgt, mr.own,
              mr.match, mr.bits, prv, cur, chunks, mr.rl, counts, mr.part, mr.flags, r, gshort,
              r < mr.max_rounds ? 1 : 0, L.maxvw, mr.short_deg, mr.d_n);
