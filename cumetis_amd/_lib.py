@@ -21,6 +21,7 @@ CM_OFF32, CM_OFF64 = 0, 1
 CM_ROW_FIRST_SEEN, CM_ROW_ASCENDING = 0, 1
 CM_FLAG_TRUSTED_INPUT = 1
 CM_FLAG_NO_BALANCE = 2
+CM_FLAG_EXACT_INITPART = 4
 CM_DEFAULT_TRIALS = 4
 
 
@@ -117,6 +118,8 @@ _PROTOS = {
                                C.POINTER(_P), C.POINTER(_P)]),
     "cm_initial_partition": (C.c_int, [C.c_int64, _P, _P, _P, _P, C.c_int32, _P,
                                        C.POINTER(C.c_int64)]),
+    "cm_initial_partition_par": (C.c_int, [C.c_int64, _P, _P, _P, _P, C.c_int32, _P,
+                                           C.POINTER(C.c_int64)]),
 }
 
 _lib = None

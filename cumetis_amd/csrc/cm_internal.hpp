@@ -328,9 +328,11 @@ void segsort_u64(cm_ctx *ctx, int64_t nseg, const void *d_segoff, bool off64, in
                  uint64_t *d_keys, uint32_t *d_vals);
 // multilevel partition of an existing hierarchy: initial partition of the
 // coarsest level (host), then per level projection + one refinement pass.
-// part receives level-0 assignments (host), *cut (when given) their edge cut.
+// part receives level-0 assignments (host), *cut (when given) their edge cut;
+// exact_init: the initial partition's serial form at every size.
 void partition_hierarchy(cm_ctx *ctx, const cm_hier *h, int nparts, int32_t *part, double *init_ms,
-                         double *uncoarsen_ms, bool balance = true, int64_t *cut = nullptr);
+                         double *uncoarsen_ms, bool balance = true, int64_t *cut = nullptr,
+                         bool exact_init = false);
 // (d) projection
 void project(cm_ctx *ctx, int64_t n, const int32_t *d_cmap, const int32_t *d_where_c,
              int32_t *d_where_f);

@@ -108,12 +108,12 @@ void uncoarsen_hierarchy(cm_ctx *ctx, const cm_hier *h, int nparts, const cmh::H
 }
 
 void partition_hierarchy(cm_ctx *ctx, const cm_hier *h, int nparts, int32_t *part, double *init_ms,
-                         double *uncoarsen_ms, bool balance, int64_t *cut)
+                         double *uncoarsen_ms, bool balance, int64_t *cut, bool exact_init)
 {
   const double t1 = now_ms();
   cmh::HostGraph hc = coarsest_to_host(ctx, h);
   std::vector<int> where;
-  cmh::initial_partition(hc, nparts, where);
+  cmh::initial_partition(hc, nparts, where, exact_init ? cmh::InitForm::EXACT : cmh::InitForm::AUTO);
   const double t2 = now_ms();
   uncoarsen_hierarchy(ctx, h, nparts, hc, where, part, balance, cut);
   if (init_ms) *init_ms = t2 - t1;
@@ -208,9 +208,10 @@ int cm_partition(cm_ctx *ctx, const cm_graph_view *g, const cm_coarsen_params *p
       r.coarsen_ms += now_ms() - c0;
       Trial &x = tr.t[t];
       x.hc = cm::coarsest_to_host(ctx, x.h);
-      x.th = std::thread([&x, nparts]() {
+      const bool exact = (p->flags & CM_FLAG_EXACT_INITPART) != 0;
+      x.th = std::thread([&x, nparts, exact]() {
         try {
-          cmh::initial_partition(x.hc, nparts, x.where);
+          cmh::initial_partition(x.hc, nparts, x.where, exact ? cmh::InitForm::EXACT : cmh::InitForm::AUTO);
         } catch (...) {
           x.failed = true;
         }

@@ -4,18 +4,74 @@
 // refinement (cuMetis.cu cu:1879-3062, cu:3091-3108).  The random choices
 // draw glibc rand() in the reference's order (cu:287-300), so results follow
 // the reference stream exactly.
+//
+// Coarsest graphs of PAR_MIN_N vertices or more (the GPU coarsening stalls
+// early on graphs with many isolated vertices and stars, R-MAT: 3.7 M
+// vertices at scale 22, where the serial recursion takes minutes) take a
+// parallel form of the same algorithm: isolated vertices are set aside (they
+// add nothing to the cut) and placed afterwards on the lightest parts, and
+// every recursion step draws from its own stream, seeded from its place in
+// the recursion, so the two halves of a split run on separate threads and
+// the result does not depend on how many threads there are.
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <memory>
+#include <queue>
+#include <thread>
 #include <vector>
 
 #include "cm_host.hpp"
 
 namespace cmh {
 namespace {
+
+constexpr int PAR_MIN_N = 1 << 18;
+
+// Order-preserving parallel loops for the passes over every entry of a large
+// graph (gains, projection, split, contraction): f(lo, hi) runs on contiguous
+// ranges of [0, n), each writing only its own range's results, so the output
+// is the serial loop's in both forms.  Ranges below LOOP_MIN_WORK entries of
+// work run inline.
+constexpr int64_t LOOP_MIN_WORK = 1 << 20;
+
+int loop_threads()
+{
+  static const int t = (int)std::max(1u, std::min(std::thread::hardware_concurrency(), 16u));
+  return t;
+}
+
+// Ranges: ceil(work / LOOP_MIN_WORK) of them, at most loop_threads(); b[c] is
+// the first index of range c, balanced by the entries of xadj.
+std::vector<int> split_ranges(int n, int64_t work, const int64_t *xadj)
+{
+  int nr = (int)std::min<int64_t>(loop_threads(), std::max<int64_t>(1, work / LOOP_MIN_WORK));
+  if (n < 2 * nr) nr = 1;
+  std::vector<int> b(nr + 1, n);
+  b[0] = 0;
+  for (int c = 1; c < nr; ++c)  // the first vertex whose entries start at or after c/nr of them
+    b[c] = (int)(std::lower_bound(xadj, xadj + n, xadj[n] / nr * c) - xadj);
+  return b;
+}
+
+// f(c, lo, hi) for every range c, range 0 on the calling thread
+template <typename F>
+void run_ranges(const std::vector<int> &b, F &&f)
+{
+  std::vector<std::thread> th;
+  for (int c = 1; c + 1 < (int)b.size(); ++c) th.emplace_back([&, c]() { f(c, b[c], b[c + 1]); });
+  f(0, b[0], b[1]);
+  for (auto &t : th) t.join();
+}
+
+template <typename F>
+void par_for(int n, int64_t work, const int64_t *xadj, F &&f)
+{
+  run_ranges(split_ranges(n, work, xadj), [&](int, int lo, int hi) { f(lo, hi); });
+}
 
 // cu:287-300 draws from glibc rand() with its default seed 1.  rand() is
 // random() over a TYPE_3 (128-byte) state; random_r over a private state
@@ -24,12 +80,12 @@ namespace {
 struct RandStream {
   random_data rd;
   char state[128];
-  RandStream()
+  explicit RandStream(unsigned seed = 1)
   {
     memset(&rd, 0, sizeof(rd));
     memset(state, 0, sizeof(state));
-    initstate_r(1, state, sizeof(state), &rd);
-    srandom_r(1, &rd);
+    initstate_r(seed, state, sizeof(state), &rd);
+    srandom_r(seed, &rd);
   }
   int32_t next()
   {
@@ -350,19 +406,22 @@ void compute_2way(Part2 &g)  // cu:1895-1944
   std::fill(g.bndptr.begin(), g.bndptr.end(), -1);
   g.nbnd = 0;
   for (int i = 0; i < g.n; ++i) g.pwgts[g.where[i]] += g.vwgt[i];
-  int cut = 0;
-  for (int i = 0; i < g.n; ++i) {
-    int in = 0, ex = 0;
-    const int me = g.where[i];
-    for (int64_t j = g.xadj[i]; j < g.xadj[i + 1]; ++j)
-      (g.where[g.adjncy[j]] == me ? in : ex) += g.adjwgt[j];
-    g.id[i] = in;
-    g.ed[i] = ex;
-    if (ex > 0 || g.xadj[i] == g.xadj[i + 1]) {
-      g.bnd_insert(i);
-      cut += ex;
+  par_for(g.n, g.m, g.xadj.data(), [&](int lo, int hi) {
+    for (int i = lo; i < hi; ++i) {
+      int in = 0, ex = 0;
+      const int me = g.where[i];
+      for (int64_t j = g.xadj[i]; j < g.xadj[i + 1]; ++j)
+        (g.where[g.adjncy[j]] == me ? in : ex) += g.adjwgt[j];
+      g.id[i] = in;
+      g.ed[i] = ex;
     }
-  }
+  });
+  int cut = 0;
+  for (int i = 0; i < g.n; ++i)
+    if (g.ed[i] > 0 || g.xadj[i] == g.xadj[i + 1]) {
+      g.bnd_insert(i);
+      cut += g.ed[i];
+    }
   g.mincut = cut / 2;
 }
 
@@ -629,19 +688,22 @@ void project_2way(Part2 &g)  // cu:2693-2764
     g.cmap[i] = c.bndptr[k];
   }
   g.nbnd = 0;
-  for (int i = 0; i < g.n; ++i) {
-    int in = 0, ex = 0;
-    if (g.cmap[i] == -1) {
-      for (int64_t j = g.xadj[i]; j < g.xadj[i + 1]; ++j) in += g.adjwgt[j];
-    } else {
-      const int me = g.where[i];
-      for (int64_t j = g.xadj[i]; j < g.xadj[i + 1]; ++j)
-        (g.where[g.adjncy[j]] == me ? in : ex) += g.adjwgt[j];
+  par_for(g.n, g.m, g.xadj.data(), [&](int lo, int hi) {
+    for (int i = lo; i < hi; ++i) {
+      int in = 0, ex = 0;
+      if (g.cmap[i] == -1) {
+        for (int64_t j = g.xadj[i]; j < g.xadj[i + 1]; ++j) in += g.adjwgt[j];
+      } else {
+        const int me = g.where[i];
+        for (int64_t j = g.xadj[i]; j < g.xadj[i + 1]; ++j)
+          (g.where[g.adjncy[j]] == me ? in : ex) += g.adjwgt[j];
+      }
+      g.id[i] = in;
+      g.ed[i] = ex;
     }
-    g.id[i] = in;
-    g.ed[i] = ex;
-    if (ex > 0 || g.xadj[i] == g.xadj[i + 1]) g.bnd_insert(i);
-  }
+  });
+  for (int i = 0; i < g.n; ++i)
+    if (g.ed[i] > 0 || g.xadj[i] == g.xadj[i + 1]) g.bnd_insert(i);
   g.mincut = c.mincut;
   g.pwgts[0] = c.pwgts[0];
   g.pwgts[1] = c.pwgts[1];
@@ -672,28 +734,56 @@ void split(Part2 &g, Part2 &l, Part2 &r)  // cu:2834-2959
   std::vector<int> rename(g.n);
   int cnt[2] = {0, 0};
   for (int i = 0; i < g.n; ++i) rename[i] = cnt[g.where[i]]++;
+  // entries each vertex keeps: all of an interior vertex's, the same-side
+  // ones of a boundary vertex's
+  std::vector<int64_t> kept(g.n);
+  par_for(g.n, g.m, g.xadj.data(), [&](int lo, int hi) {
+    for (int i = lo; i < hi; ++i) {
+      if (g.bndptr[i] == -1) {
+        kept[i] = g.xadj[i + 1] - g.xadj[i];
+        continue;
+      }
+      const int p = g.where[i];
+      int64_t c = 0;
+      for (int64_t j = g.xadj[i]; j < g.xadj[i + 1]; ++j) c += g.where[g.adjncy[j]] == p;
+      kept[i] = c;
+    }
+  });
+  int64_t off[2] = {0, 0};
   for (int p = 0; p < 2; ++p) {
     side[p]->n = cnt[p];
-    side[p]->xadj.assign(1, 0);
-    side[p]->xadj.reserve(cnt[p] + 1);
-    side[p]->vwgt.reserve(cnt[p]);
-    side[p]->label.reserve(cnt[p]);
+    side[p]->xadj.resize(cnt[p] + 1);
+    side[p]->vwgt.resize(cnt[p]);
+    side[p]->label.resize(cnt[p]);
   }
   for (int i = 0; i < g.n; ++i) {
     const int p = g.where[i];
     Part2 &s = *side[p];
-    const bool interior = g.bndptr[i] == -1;
-    for (int64_t j = g.xadj[i]; j < g.xadj[i + 1]; ++j) {
-      const int k = g.adjncy[j];
-      if (interior || g.where[k] == p) {
-        s.adjncy.push_back(rename[k]);
-        s.adjwgt.push_back(g.adjwgt[j]);
+    s.xadj[rename[i]] = off[p];
+    off[p] += kept[i];
+    s.vwgt[rename[i]] = g.vwgt[i];
+    s.label[rename[i]] = g.label[i];
+  }
+  for (int p = 0; p < 2; ++p) {
+    side[p]->xadj[cnt[p]] = off[p];
+    side[p]->adjncy.resize(off[p]);
+    side[p]->adjwgt.resize(off[p]);
+  }
+  par_for(g.n, g.m, g.xadj.data(), [&](int lo, int hi) {
+    for (int i = lo; i < hi; ++i) {
+      const int p = g.where[i];
+      Part2 &s = *side[p];
+      const bool interior = g.bndptr[i] == -1;
+      int64_t o = s.xadj[rename[i]];
+      for (int64_t j = g.xadj[i]; j < g.xadj[i + 1]; ++j) {
+        const int k = g.adjncy[j];
+        if (interior || g.where[k] == p) {
+          s.adjncy[o] = rename[k];
+          s.adjwgt[o++] = g.adjwgt[j];
+        }
       }
     }
-    s.vwgt.push_back(g.vwgt[i]);
-    s.label.push_back(g.label[i]);
-    s.xadj.push_back((int64_t)s.adjncy.size());
-  }
+  });
   for (int p = 0; p < 2; ++p) {
     side[p]->m = side[p]->xadj.back();
     side[p]->set_tvwgt();
@@ -730,12 +820,134 @@ int recursive_bisect(std::unique_ptr<Part2> g, Settings &s, int nparts, int *par
   return obj;
 }
 
+// The parallel form: the step for parts [fpart, fpart + nparts) draws from a
+// stream seeded by (fpart, nparts); its halves run on a new thread and this
+// one when the half is large and a thread is free, inline otherwise.
+constexpr int PAR_TASK_MIN_N = 4096;
+
+struct ParCtx {
+  std::atomic<int> threads{0};
+  int max_threads = 1;
+};
+
+unsigned step_seed(int fpart, int nparts)
+{
+  uint64_t z = ((uint64_t)(uint32_t)fpart << 32 | (uint32_t)nparts) + 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return (unsigned)((z ^ (z >> 31)) >> 32) | 1u;
+}
+
+int recursive_bisect_par(std::unique_ptr<Part2> g, Settings s, int nparts, int *part, float *tp, int fpart,
+                         ParCtx &pc)
+{
+  if (g->n == 0) return 0;
+  RandStream stream(step_seed(fpart, nparts));
+  RandStream *const outer = g_stream;
+  g_stream = &stream;
+  float sum = 0;
+  for (int i = 0; i < (nparts >> 1); ++i) sum += tp[i];
+  const float tp2[2] = {sum, 1.0f - sum};
+  int obj = multilevel_bisect(*g, s, tp2);
+  for (int i = 0; i < g->n; ++i) part[g->label[i]] = g->where[i] + fpart;
+  std::unique_ptr<Part2> l, r;
+  if (nparts > 2) {
+    l.reset(new Part2());
+    r.reset(new Part2());
+    split(*g, *l, *r);
+  }
+  g.reset();
+  g_stream = outer;
+  const float f0 = (float)(1.0 / sum), f1 = (float)(1.0 / (1.0 - sum));
+  for (int i = 0; i < (nparts >> 1); ++i) tp[i] *= f0;
+  for (int i = nparts >> 1; i < nparts; ++i) tp[i] *= f1;
+  const int nl = nparts >> 1, nr = nparts - nl;
+  if (nparts > 3) {
+    int objl = 0;
+    std::thread th;
+    if (l->n >= PAR_TASK_MIN_N && pc.threads.fetch_add(1) < pc.max_threads) {
+      th = std::thread([&, s]() { objl = recursive_bisect_par(std::move(l), s, nl, part, tp, fpart, pc); });
+    } else {
+      if (l->n >= PAR_TASK_MIN_N) pc.threads.fetch_sub(1);
+      objl = recursive_bisect_par(std::move(l), s, nl, part, tp, fpart, pc);
+    }
+    const int objr = recursive_bisect_par(std::move(r), s, nr, part, tp + nl, fpart + nl, pc);
+    if (th.joinable()) {
+      th.join();
+      pc.threads.fetch_sub(1);
+    }
+    obj += objl + objr;
+  } else if (nparts == 3) {
+    obj += recursive_bisect_par(std::move(r), s, nr, part, tp + nl, fpart + nl, pc);
+  }
+  return obj;
+}
+
+int initial_partition_par(const HostGraph &hg, int nparts, std::vector<int> &where)
+{
+  // the graph without its isolated vertices
+  std::unique_ptr<Part2> g(new Part2());
+  std::vector<int> iso;
+  g->xadj.reserve(hg.n + 1);
+  g->xadj.push_back(0);
+  for (int i = 0; i < hg.n; ++i) {
+    const int64_t b = hg.xadj[i], e = hg.xadj[i + 1];
+    if (b == e) {
+      iso.push_back(i);
+      continue;
+    }
+    g->xadj.push_back(g->xadj.back() + (e - b));
+    g->vwgt.push_back(hg.vwgt[i]);
+    g->label.push_back(i);
+  }
+  g->n = (int)g->label.size();
+  std::vector<int> newid(hg.n, -1);
+  for (int i = 0; i < g->n; ++i) newid[g->label[i]] = i;
+  g->adjncy.resize(hg.m());
+  g->adjwgt.resize(hg.m());
+  for (int i = 0; i < g->n; ++i) {
+    const int64_t b = hg.xadj[g->label[i]], o = g->xadj[i];
+    for (int64_t j = 0; j < g->xadj[i + 1] - o; ++j) {
+      g->adjncy[o + j] = newid[hg.adjncy[b + j]];
+      g->adjwgt[o + j] = hg.adjwgt[b + j];
+    }
+  }
+  g->m = g->xadj.back();
+  g->set_tvwgt();
+  ParCtx pc;
+  const unsigned hw = std::thread::hardware_concurrency();
+  pc.max_threads = (int)std::max(1u, std::min(hw ? hw : 1u, 16u)) - 1;
+  std::vector<float> tp(nparts, (float)(1.0 / nparts));
+  int obj = 0;
+  if (g->n > 0) obj = recursive_bisect_par(std::move(g), Settings(), nparts, where.data(), tp.data(), 0, pc);
+  // isolated vertices, in index order, each to the lightest part (the
+  // lowest part id among equals)
+  std::vector<int64_t> pw(nparts, 0);
+  std::vector<char> isolated(hg.n, 0);
+  for (int v : iso) isolated[v] = 1;
+  for (int i = 0; i < hg.n; ++i)
+    if (!isolated[i]) pw[where[i]] += hg.vwgt[i];
+  using Item = std::pair<int64_t, int>;
+  std::priority_queue<Item, std::vector<Item>, std::greater<Item>> heap;
+  for (int p = 0; p < nparts; ++p) heap.push({pw[p], p});
+  for (int v : iso) {
+    Item t = heap.top();
+    heap.pop();
+    where[v] = t.second;
+    t.first += hg.vwgt[v];
+    heap.push(t);
+  }
+  return obj;
+}
+
 }  // namespace
 
-int initial_partition(const HostGraph &hg, int nparts, std::vector<int> &where)
+int initial_partition(const HostGraph &hg, int nparts, std::vector<int> &where, InitForm form)
 {
   where.assign(hg.n, 0);
   if (nparts <= 1 || hg.n == 0) return 0;
+  if (form == InitForm::PARALLEL || (form == InitForm::AUTO && hg.n >= PAR_MIN_N))
+    return initial_partition_par(hg, nparts, where);
   RandStream stream;  // a fresh stream per call: the reference's first-call sequence
   g_stream = &stream;
   std::unique_ptr<Part2> g(new Part2());

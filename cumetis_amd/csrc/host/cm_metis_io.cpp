@@ -348,10 +348,11 @@ int cm_write_partition(const char *path, const int32_t *part, int64_t n)
   return CM_OK;
 }
 
-int cm_initial_partition(int64_t n, const int64_t *xadj, const int32_t *adjncy, const int32_t *adjwgt,
-                         const int32_t *vwgt, int32_t nparts, int32_t *where, int64_t *objval)
+static int initial_partition_call(int64_t n, const int64_t *xadj, const int32_t *adjncy, const int32_t *adjwgt,
+                                  const int32_t *vwgt, int32_t nparts, int32_t *where, int64_t *objval,
+                                  cmh::InitForm form)
 {
-  if (n < 0 || !xadj || (!where && n) || nparts < 1) return CM_ERR_INVALID;
+  if (n < 0 || n >= INT32_MAX || !xadj || (!where && n) || nparts < 1) return CM_ERR_INVALID;
   cmh::HostGraph g;
   g.n = (int)n;
   g.xadj.assign(xadj, xadj + n + 1);
@@ -362,10 +363,22 @@ int cm_initial_partition(int64_t n, const int64_t *xadj, const int32_t *adjncy, 
   if (vwgt) g.vwgt.assign(vwgt, vwgt + n);
   else g.vwgt.assign(n, 1);
   std::vector<int> w;
-  const int obj = cmh::initial_partition(g, nparts, w);
+  const int obj = cmh::initial_partition(g, nparts, w, form);
   for (int64_t i = 0; i < n; ++i) where[i] = w[i];
   if (objval) *objval = obj;
   return CM_OK;
+}
+
+int cm_initial_partition(int64_t n, const int64_t *xadj, const int32_t *adjncy, const int32_t *adjwgt,
+                         const int32_t *vwgt, int32_t nparts, int32_t *where, int64_t *objval)
+{
+  return initial_partition_call(n, xadj, adjncy, adjwgt, vwgt, nparts, where, objval, cmh::InitForm::EXACT);
+}
+
+int cm_initial_partition_par(int64_t n, const int64_t *xadj, const int32_t *adjncy, const int32_t *adjwgt,
+                             const int32_t *vwgt, int32_t nparts, int32_t *where, int64_t *objval)
+{
+  return initial_partition_call(n, xadj, adjncy, adjwgt, vwgt, nparts, where, objval, cmh::InitForm::PARALLEL);
 }
 
 }  // extern "C"

@@ -316,11 +316,15 @@ class Engine:
         return hier
 
     def partition(self, g: Graph, nparts: int, seed: int = 1,
-                  row_order: int = L.CM_ROW_FIRST_SEEN, balance: bool = True, trials: int = 0):
+                  row_order: int = L.CM_ROW_FIRST_SEEN, balance: bool = True, trials: int = 0,
+                  exact_initpart: bool = False):
         """cm_partition; balance=False leaves out the balancing pass (the
-        reference's pipeline has none); trials 0 = CM_DEFAULT_TRIALS."""
+        reference's pipeline has none); trials 0 = CM_DEFAULT_TRIALS;
+        exact_initpart keeps the serial initial partition on coarsest graphs
+        of 2^18 vertices or more (CM_FLAG_EXACT_INITPART)."""
         v, keep = self.host_view(g)
-        p = L.CoarsenParams(nparts, 0, 0, row_order, 0, 0, seed, 0 if balance else L.CM_FLAG_NO_BALANCE, trials)
+        flags = (0 if balance else L.CM_FLAG_NO_BALANCE) | (L.CM_FLAG_EXACT_INITPART if exact_initpart else 0)
+        p = L.CoarsenParams(nparts, 0, 0, row_order, 0, 0, seed, flags, trials)
         part = np.empty(max(g.n, 1), np.int32)
         rep = L.PartReport()
         self._check(self.lib.cm_partition(self.ctx, C.byref(v), C.byref(p), part.ctypes.data,

@@ -340,3 +340,21 @@ def test_partgraph_concurrent_callers(engine):
     for r in out:
         assert r is not None and r[0] == 1 and r[1] == obj
         np.testing.assert_array_equal(r[2], part)
+
+
+def test_partition_rmat_parallel_initpart(engine, orc):
+    """R-MAT scale 19 stops coarsening near 450 K vertices (isolated vertices
+    and stars never shrink), so cm_partition's initial partition takes its
+    parallel form: the same partition on a second call, every part within
+    int(w/k * 1.03) after the balancing pass, and the reported cut the
+    oracle's cut of the partition."""
+    g = graphs.rmat_fast(19, 16, seed=2, offsets=np.int64)
+    k = 64
+    say(f"[rmat19 partition] n={g.n} m={g.m} k={k}, one trial")
+    p1, r1 = engine.partition(g, k, seed=1, trials=1)
+    assert r1.coarsest_n >= 1 << 18
+    p2, r2 = engine.partition(g, k, seed=1, trials=1)
+    assert np.array_equal(p1, p2) and r1.edgecut == r2.edgecut
+    assert r1.edgecut == orc.edgecut(g.xadj, g.adjncy, None, p1)
+    pw = np.bincount(p1, minlength=k)
+    assert pw.max() <= int(g.n / k * 1.03)

@@ -12,7 +12,7 @@ from cumetis_amd import _lib, graphs
 GOLD = Path(__file__).resolve().parent / "golden"
 
 
-def product_initpart(g, k, seed=1):
+def product_initpart(g, k, seed=1, parallel=False):
     lib = _lib.load()
     import oracle.oracle as O
     O.srand(seed)  # both draw from the same glibc stream
@@ -22,8 +22,8 @@ def product_initpart(g, k, seed=1):
     v = np.ascontiguousarray(g.vwgt, np.int32)
     where = np.zeros(max(g.n, 1), np.int32)
     obj = C.c_int64()
-    rc = lib.cm_initial_partition(g.n, x.ctypes.data, a.ctypes.data, w.ctypes.data, v.ctypes.data,
-                                  k, where.ctypes.data, C.byref(obj))
+    fn = lib.cm_initial_partition_par if parallel else lib.cm_initial_partition
+    rc = fn(g.n, x.ctypes.data, a.ctypes.data, w.ctypes.data, v.ctypes.data, k, where.ctypes.data, C.byref(obj))
     assert rc == 0
     return where[:g.n], obj.value
 
@@ -37,6 +37,7 @@ def test_initial_partition_matches_oracle(orc, name, k):
     wo, oo = orc.initpart(g.xadj, g.adjncy, g.adjwgt, g.vwgt, k)
     assert np.array_equal(wp, wo)
     assert op == oo
+    assert op == orc.edgecut(g.xadj, g.adjncy, g.adjwgt, wp)  # the bisection cuts sum to the k-way cut
 
 
 @pytest.mark.parametrize("k", [2, 8, 64])
@@ -185,3 +186,47 @@ def test_write_partition(tmp_path):
     f = tmp_path / "g.graph.part.4"
     assert lib.cm_write_partition(str(f).encode(), part.ctypes.data, 4) == 0
     assert f.read_text() == "3\n0\n2\n1\n"
+
+
+@pytest.mark.parametrize("k", [2, 8, 64])
+def test_parallel_initial_partition(orc, k):
+    """The parallel form that the pipeline takes on coarsest graphs of 2^18
+    vertices or more, here on an R-MAT graph a fifth of whose vertices are
+    isolated: the same result on every call, its objval the oracle's edge cut
+    of its result, and the isolated vertices, in index order, each on the
+    lightest part (the lowest id among equals) after the rest is split."""
+    g = with_weights(graphs.rmat(13, 8, seed=6, offsets=np.int64))
+    deg = np.diff(g.xadj)
+    iso = np.nonzero(deg == 0)[0]
+    assert len(iso) > g.n // 5
+    w1, o1 = product_initpart(g, k, parallel=True)
+    w2, o2 = product_initpart(g, k, parallel=True)
+    assert np.array_equal(w1, w2) and o1 == o2
+    assert 0 <= w1.min() and w1.max() < k
+    assert o1 == orc.edgecut(g.xadj, g.adjncy, g.adjwgt, w1)
+    vw = np.asarray(g.vwgt, np.int64)
+    pw = np.bincount(w1[deg > 0], weights=vw[deg > 0], minlength=k).astype(np.int64)
+    for v in iso:
+        p = int(np.argmin(pw))
+        assert w1[v] == p, v
+        pw[p] += vw[v]
+    assert pw.max() * k <= 1.03 * vw.sum() + k * vw.max()
+
+
+def test_parallel_initial_partition_components(orc):
+    """Two disjoint cliques of 40 vertices and 30 isolated vertices, k = 2:
+    the parallel form separates the cliques (cut 0) and splits the isolated
+    vertices evenly, while the serial form keeps matching the oracle."""
+    n0 = 40
+    src, dst = np.triu_indices(n0, 1)
+    e = np.concatenate([np.stack([src, dst], 1), np.stack([src, dst], 1) + n0])
+    g = with_weights(graphs.from_edges(2 * n0 + 30, e[:, 0], e[:, 1]))
+    wp, op = product_initpart(g, 2, parallel=True)
+    assert op == 0 and orc.edgecut(g.xadj, g.adjncy, g.adjwgt, wp) == 0
+    assert len(set(wp[:n0])) == 1 and len(set(wp[n0:2 * n0])) == 1 and wp[0] != wp[n0]
+    pw = np.bincount(wp, weights=g.vwgt, minlength=2)
+    assert abs(pw[0] - pw[1]) <= g.vwgt.max()
+    we, oe = product_initpart(g, 2)
+    orc.srand(1)
+    wo, oo = orc.initpart(g.xadj, g.adjncy, g.adjwgt, g.vwgt, 2)
+    assert np.array_equal(we, wo) and oe == oo
