@@ -415,11 +415,15 @@ def e2e_leg(eng, g, k, trials=0, reps=3):
     for _ in range(reps):
         part, rep = eng.partition(g, k, seed=1, trials=trials)
         reps_out.append(rep)
+        progress(f"e2e k={k}: {rep.total_ms:.1f} ms (coarsest {rep.coarsest_n})")
     r = min(reps_out, key=lambda x: x.total_ms)
+    vw = np.ones(g.n, np.int64) if g.vwgt is None else np.asarray(g.vwgt, np.int64)
+    pw = np.bincount(part, weights=vw, minlength=k)
     return {"total_ms": round(r.total_ms, 2), "coarsen_ms": round(r.coarsen_ms, 2),
             "init_ms": round(r.init_ms, 2), "uncoarsen_ms": round(r.uncoarsen_ms, 2),
             "trials": int(r.trials), "edgecut": int(r.edgecut), "levels": int(r.levels),
-            "coarsest_n": int(r.coarsest_n), "runs": reps}
+            "coarsest_n": int(r.coarsest_n), "imbalance": round(float(pw.max() * k / vw.sum()), 4),
+            "runs": reps}
 
 
 def serve_leg(count=120, rate=20.0, seed=7, trials=0):

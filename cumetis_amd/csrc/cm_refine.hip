@@ -129,8 +129,10 @@ __global__ __launch_bounds__(256) void k_refine_dest(const int32_t *list, const 
 // lanes).  Up to the first lane that fails, that assumption holds, so those
 // lanes are accepted and the first failing lane is rejected exactly as in
 // the serial walk; the next chunk starts after it.  A chunk with no
-// rejection advances by 64.  Candidates of the other pass take no part (the
-// two passes, from <= to and from > to, hold disjoint candidates).  Moves
+// rejection advances by 64.  Candidates whose destination is already too
+// heavy for them are rejected without stopping the chunk (see below).
+// Candidates of the other pass take no part (the two passes, from <= to and
+// from > to, hold disjoint candidates).  Moves
 // within one part (no connectivity beats the own part) leave the weights
 // unchanged and are not applied.  pw_io: part weights in and out; out[0]:
 // moves applied.
@@ -166,6 +168,20 @@ __global__ __launch_bounds__(CM_WAVE) void k_accept_moves(const uint64_t *__rest
       }
       s_to[l] = has ? to : -1;
       s_me[l] = has ? me : -1;
+      __syncthreads();
+      // A candidate whose destination is already too heavy for it is rejected
+      // whenever the walk reaches it, unless a candidate before it in this
+      // window moves a vertex out of that part first (the candidates before
+      // the window are in pw already): it then neither applies nor stops the
+      // window.  Without this, parts at their bound (R-MAT, k = 256) made
+      // every rejection a window of its own.
+      bool lighten = false;
+      for (int j = 0; j < l; ++j) lighten |= s_me[j] == to && s_to[j] != to;
+      if (has && !lighten && (long long)pw[to] + w > maxw) {
+        has = false;
+        s_to[l] = -1;
+        s_me[l] = -1;
+      }
       s_w[l] = has ? w : 0;
       __syncthreads();
       long long dto = 0, dme = 0;

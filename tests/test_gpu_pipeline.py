@@ -185,13 +185,16 @@ def test_batch_rejects_bad_input(engine, fault):
 @pytest.mark.parametrize("trials", [1, 4])
 @pytest.mark.parametrize("balance", [True, False])
 @pytest.mark.parametrize("name,k", [("delaunay4720", 8), ("grid100", 16), ("rgg", 4), ("hub", 8),
-                                    ("grid60vw", 48)])
+                                    ("grid60vw", 48), ("rmat12", 128)])
 def test_partition_bit_exact_with_oracle_pipeline(engine, orc, name, k, balance, trials):
     """Every refinement pass accepts its moves on the device (one wave, 64
-    candidates a step, restarting after each rejection); grid60vw, with
-    vertex weights 1..60 and 48 parts, makes the part bounds reject many
-    moves."""
-    if name == "grid100":
+    candidates a step, restarting after each rejection, and passing over
+    candidates whose destination is already too heavy for them); grid60vw,
+    with vertex weights 1..60 and 48 parts, and R-MAT scale 12 in 128 parts
+    make the part bounds reject many moves."""
+    if name == "rmat12":
+        g = graphs.rmat(12, 16, seed=4, offsets=np.int32)
+    elif name == "grid100":
         g = graphs.grid2d(100, 100, wmax=100, seed=2)
     elif name == "grid60vw":
         g0 = graphs.grid2d(60, 60, wmax=30, seed=5)
