@@ -11,7 +11,12 @@ resident in HBM.  Legs:
   batch512  512 graphs (60% 2D RGG with n log-uniform in [1k,200k], mean
             degree 8; 40% grids with side U{32..400}; adjwgt U[1,16];
             k U{2,4,..,64}): one batched coarsening against the same graphs
-            coarsened one by one.
+            coarsened one by one;
+  delaunay4720, serve (a Poisson stream of partition requests through the
+            server), e2e (cm_partition of the grid, k=64) and e2e_rmat22
+            (cm_partition of the R-MAT graph, k=256, one trial: the host
+            initial partition of its 3.7 M-vertex coarsest graph takes most
+            of it).
 Prints ONE JSON line: the main leg's metric, its dominant kernel's roofline
 (algorithmic bytes / hipEvent-timed launch duration on the engine's stream)
 and the reference CPU coarsening path timed on this host; the other legs are
@@ -470,10 +475,10 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--order", default="first_seen", choices=["first_seen", "ascending"])
-    ap.add_argument("--legs", default="grid725,delaunay4720,rmat22,batch512,serve,e2e",
+    ap.add_argument("--legs", default="grid725,delaunay4720,rmat22,e2e_rmat22,batch512,serve,e2e",
                     help="comma list of: grid725 (always run), delaunay4720, rmat22, batch512, serve, "
                          "e2e (grid 725^2 k=64 through cm_partition), e2e_rmat22 (R-MAT s22 k=256, one "
-                         "trial: minutes of host initial partition)")
+                         "trial, about 25 s: the host initial partition of a 3.7 M-vertex coarsest graph)")
     ap.add_argument("--batch-count", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
