@@ -12,6 +12,8 @@ resident in HBM.  Legs:
             degree 8; 40% grids with side U{32..400}; adjwgt U[1,16];
             k U{2,4,..,64}): one batched coarsening against the same graphs
             coarsened one by one;
+  ascending  grid725 and rmat22 with CM_ROW_ASCENDING (the reference GPU
+            contraction's row order: the binned segmented sort);
   delaunay4720, serve (a Poisson stream of partition requests through the
             server), e2e (cm_partition of the grid, k=64) and e2e_rmat22
             (cm_partition of the R-MAT graph, k=256, one trial: the host
@@ -475,8 +477,9 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--order", default="first_seen", choices=["first_seen", "ascending"])
-    ap.add_argument("--legs", default="grid725,delaunay4720,rmat22,e2e_rmat22,batch512,serve,e2e",
-                    help="comma list of: grid725 (always run), delaunay4720, rmat22, batch512, serve, "
+    ap.add_argument("--legs", default="grid725,delaunay4720,rmat22,ascending,e2e_rmat22,batch512,serve,e2e",
+                    help="comma list of: grid725 (always run), delaunay4720, rmat22, ascending (grid725 and "
+                         "rmat22 with ascending rows: the binned segmented sort), batch512, serve, "
                          "e2e (grid 725^2 k=64 through cm_partition), e2e_rmat22 (R-MAT s22 k=256, one "
                          "trial, about 25 s: the host initial partition of a 3.7 M-vertex coarsest graph)")
     ap.add_argument("--batch-count", type=int, default=512)
@@ -527,10 +530,22 @@ def main():
             progress("rmat22: cpu baseline")
             o["cpu_baseline"] = cpu_baseline(gr, 256, budget_s=10.0)
         extra["rmat22"] = o
+        if "ascending" in legs and args.order == "first_seen":
+            progress("rmat22: ascending rows")
+            oa, _, pa = coarsen_leg(eng, gr, 256, L.CM_ROW_ASCENDING, 2, max(5, args.steps // 4), profile=True)
+            extra["rmat22_ascending"] = {"ms": oa["ms"], "medges_per_s": oa["medges_per_s"],
+                                         "kernels_ms_per_step": pa}
         if "e2e_rmat22" in legs:
             progress("rmat22: end to end (one trial)")
             extra["e2e_rmat22"] = e2e_leg(eng, gr, 256, trials=1, reps=1)
         del gr
+    if "ascending" in legs and args.order == "first_seen":
+        # the reference GPU contraction's row order (cu:1332): the binned
+        # segmented sort instead of the first-seen hash passes
+        progress("grid725: ascending rows")
+        oa, _, pa = coarsen_leg(eng, g, 64, L.CM_ROW_ASCENDING, args.warmup, args.steps, profile=True)
+        extra["grid725_ascending"] = {"ms": oa["ms"], "medges_per_s": oa["medges_per_s"],
+                                      "p10_ms": oa["p10_ms"], "p90_ms": oa["p90_ms"], "kernels_ms_per_step": pa}
     if "batch512" in legs:
         extra["batch512"] = batch_leg(eng, args.batch_count, order, 3, cpu=not args.no_cpu)
     if "serve" in legs:

@@ -291,6 +291,7 @@ struct Settings {
   int niter = 10;        // cu:377
   int threshold = 200;   // cu:378
   float balance[2] = {0, 0};
+  unsigned try_seed = 0;  // parallel form: the region-growing tries' streams (0: the step's stream)
 };
 
 // cu:830-903 (HEM with degree-bucketed random visit order) + cu:737-825
@@ -615,59 +616,125 @@ struct Unseen {
   }
 };
 
+// One region-growing try (cu:2563-2650): BFS from a random vertex into part
+// 0, then balancing and FM; leaves the try's partition in g.where, its cut in
+// g.mincut.  queue and seen hold n entries.
+void grow_try(Part2 &g, Settings &s, const float *ntp, std::vector<int> &queue, std::vector<int> &seen,
+              Unseen &unseen)
+{
+  const int n = g.n;
+  const int onemax = (int)(s.ub * g.tvwgt * ntp[1]);
+  const int onemin = (int)((1.0 / s.ub) * g.tvwgt * ntp[1]);
+  std::fill(g.where.begin(), g.where.end(), 1);
+  std::fill(seen.begin(), seen.end(), 0);
+  unseen.reset(n);
+  int pw0 = 0, pw1 = g.tvwgt;
+  queue[0] = rnd(n);
+  seen[queue[0]] = 1;
+  unseen.mark(queue[0]);
+  int first = 0, last = 1, nleft = n - 1;
+  bool stalled = false;
+  for (;;) {
+    if (first == last) {  // component exhausted: restart from a random unseen vertex
+      if (nleft == 0 || stalled) break;
+      const int i = unseen.kth(rnd(nleft));
+      queue[0] = i;
+      seen[i] = 1;
+      unseen.mark(i);
+      first = 0;
+      last = 1;
+      --nleft;
+    }
+    const int i = queue[first++];
+    if (pw0 > 0 && pw1 - g.vwgt[i] < onemin) {
+      stalled = true;
+      continue;
+    }
+    g.where[i] = 0;
+    pw0 += g.vwgt[i];
+    pw1 -= g.vwgt[i];
+    if (pw1 <= onemax) break;
+    stalled = false;
+    for (int64_t j = g.xadj[i]; j < g.xadj[i + 1]; ++j) {
+      const int k = g.adjncy[j];
+      if (!seen[k]) {
+        queue[last++] = k;
+        seen[k] = 1;
+        unseen.mark(k);
+        --nleft;
+      }
+    }
+  }
+  compute_2way(g);
+  balance_2way(g, s, ntp);
+  fm_refine(g, ntp, s.niter);
+}
+
+// Tries run concurrently (parallel form) on graphs of this many entries.
+constexpr int64_t GROW_PAR_MIN_M = 1 << 20;
+
 void grow_bisection(Part2 &g, Settings &s, const float *ntp, int niparts)  // cu:2536-2661
 {
   const int n = g.n;
   g.alloc2way();
+  if (s.try_seed != 0) {
+    // parallel form: try t draws from its own stream, on its own copy of the
+    // graph's 2-way state; the first try with the lowest cut wins, as in the
+    // serial loop
+    std::vector<std::unique_ptr<Part2>> tg(niparts);
+    auto one = [&](int t) {
+      Part2 &x = *tg[t];
+      RandStream stream(s.try_seed + 0x9E3779B9u * (unsigned)(t + 1));
+      RandStream *const outer = g_stream;
+      g_stream = &stream;
+      std::vector<int> queue(std::max(n, 1)), seen(n);
+      Unseen unseen;
+      Settings st = s;
+      grow_try(x, st, ntp, queue, seen, unseen);
+      g_stream = outer;
+    };
+    for (int t = 0; t < niparts; ++t) {
+      tg[t].reset(new Part2());
+      Part2 &x = *tg[t];
+      x.n = g.n;
+      x.m = g.m;
+      x.tvwgt = g.tvwgt;
+      if (t == 0 || g.m < GROW_PAR_MIN_M) {  // the inline tries read g's arrays through copies too
+        x.xadj = g.xadj;
+        x.adjncy = g.adjncy;
+        x.adjwgt = g.adjwgt;
+        x.vwgt = g.vwgt;
+      }
+      x.alloc2way();
+    }
+    if (g.m >= GROW_PAR_MIN_M) {
+      std::vector<std::thread> th;
+      for (int t = 1; t < niparts; ++t)
+        th.emplace_back([&, t]() {
+          Part2 &x = *tg[t];
+          x.xadj = g.xadj;
+          x.adjncy = g.adjncy;
+          x.adjwgt = g.adjwgt;
+          x.vwgt = g.vwgt;
+          one(t);
+        });
+      one(0);
+      for (auto &t : th) t.join();
+    } else {
+      for (int t = 0; t < niparts; ++t) one(t);
+    }
+    int bt = 0;
+    for (int t = 1; t < niparts; ++t)
+      if (tg[t]->mincut < tg[bt]->mincut) bt = t;
+    g.mincut = tg[bt]->mincut;
+    g.where = tg[bt]->where;
+    return;
+  }
   std::vector<int> best(n), queue(std::max(n, 1)), seen(n);
   Unseen unseen;
-  const int onemax = (int)(s.ub * g.tvwgt * ntp[1]);
-  const int onemin = (int)((1.0 / s.ub) * g.tvwgt * ntp[1]);
   int bestcut = 0;
   for (int it = 0; it < niparts; ++it) {
-    std::fill(g.where.begin(), g.where.end(), 1);
-    std::fill(seen.begin(), seen.end(), 0);
-    unseen.reset(n);
-    int pw0 = 0, pw1 = g.tvwgt;
-    queue[0] = rnd(n);
-    seen[queue[0]] = 1;
-    unseen.mark(queue[0]);
-    int first = 0, last = 1, nleft = n - 1;
-    bool stalled = false;
-    for (;;) {
-      if (first == last) {  // component exhausted: restart from a random unseen vertex
-        if (nleft == 0 || stalled) break;
-        const int i = unseen.kth(rnd(nleft));
-        queue[0] = i;
-        seen[i] = 1;
-        unseen.mark(i);
-        first = 0;
-        last = 1;
-        --nleft;
-      }
-      const int i = queue[first++];
-      if (pw0 > 0 && pw1 - g.vwgt[i] < onemin) {
-        stalled = true;
-        continue;
-      }
-      g.where[i] = 0;
-      pw0 += g.vwgt[i];
-      pw1 -= g.vwgt[i];
-      if (pw1 <= onemax) break;
-      stalled = false;
-      for (int64_t j = g.xadj[i]; j < g.xadj[i + 1]; ++j) {
-        const int k = g.adjncy[j];
-        if (!seen[k]) {
-          queue[last++] = k;
-          seen[k] = 1;
-          unseen.mark(k);
-          --nleft;
-        }
-      }
-    }
-    compute_2way(g);
-    balance_2way(g, s, ntp);
-    fm_refine(g, ntp, s.niter);
+    grow_try(g, s, ntp, queue, seen, unseen);
     if (it == 0 || bestcut > g.mincut) {
       bestcut = g.mincut;
       best = g.where;
@@ -845,6 +912,7 @@ int recursive_bisect_par(std::unique_ptr<Part2> g, Settings s, int nparts, int *
   RandStream stream(step_seed(fpart, nparts));
   RandStream *const outer = g_stream;
   g_stream = &stream;
+  s.try_seed = step_seed(fpart + nparts, ~(unsigned)nparts);  // the region-growing tries' streams
   float sum = 0;
   for (int i = 0; i < (nparts >> 1); ++i) sum += tp[i];
   const float tp2[2] = {sum, 1.0f - sum};
