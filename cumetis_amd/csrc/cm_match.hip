@@ -916,6 +916,138 @@ __global__ __launch_bounds__(256) void k_accept(int64_t n, int32_t *match, uint3
   }
 }
 
+// Small levels without long rows: every round in one workgroup, the state
+// in LDS (16-bit ids: -1 unmatched / no proposal, -2 dead).  It runs what
+// launches 0 .. R of the kernels above would (R = the round budget): launch
+// r's accepts of round r-1 and proposals of round r, the same round flags,
+// the same stop (a round that matched nothing or left nothing unmatched).
+// A level still live after R rounds leaves the state of launch R (match[]
+// after round R-1's accepts, round R's proposals in prop half R & 1) for
+// the ordinary launches to continue from launch R + 1.  Also does launch 0's
+// initialisation of a level without long rows (flags, hash counters).
+constexpr int LDS_MATCH_THREADS = 1024;
+// Up to 4096 vertices (Delaunay 4720's levels after the first: 0.457 ->
+// 0.413 ms per coarsening).  One workgroup scans its vertices' rows one after
+// another, so larger levels are faster as launches of the round kernel (grid
+// 725^2 with this form up to 8192 or 16384 vertices: 0.94 -> 0.97 or 1.03 ms).
+constexpr int64_t LDS_MATCH_MAX_N = 4096;
+constexpr int LDS_ROUNDS = 32;  // the round budget of one k_match_lds
+
+template <typename OffT>
+__device__ __forceinline__ int32_t lds_scan(int32_t v, const OffT *__restrict__ xadj,
+                                            const int32_t *__restrict__ adjncy,
+                                            const int32_t *__restrict__ adjwgt,
+                                            const int32_t *__restrict__ vwgt, const MatchOwners &own,
+                                            const int16_t *s_m)
+{
+  int32_t maxv;
+  uint64_t seed;
+  int64_t base;
+  owner_params(own, v, maxv, seed, base);
+  const int32_t vw = vwgt[v];
+  // 32-bit entry indices: a level of at most LDS_MATCH_MAX_N vertices holds
+  // fewer than 2^31 entries.  (With int64 offsets a 64-bit index here gave
+  // wrong proposals in the rescans of rounds >= 1, and only there; the 32-bit
+  // form matches the oracle and the int32 build.)
+  const int b = (int)xadj[v], e = (int)xadj[v + 1];
+  int32_t best = -2;
+  uint64_t bk1 = 0, bk2 = 0;
+  for (int j0 = b; j0 < e; j0 += 4) {
+    int32_t u[4], wu[4];
+    uint32_t w[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {  // clamped, unconditional loads: one wait per batch
+      const int jj = j0 + t < e ? j0 + t : e - 1;
+      u[t] = adjncy[jj];
+      w[t] = (uint32_t)adjwgt[jj];
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const bool in = j0 + t < e && u[t] != v && s_m[u[t]] == -1;
+      wu[t] = vwgt[in ? u[t] : v];
+      if (!in) u[t] = -1;
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      if (u[t] < 0 || vw + wu[t] > maxv) continue;
+      const uint32_t a = (uint32_t)(min(u[t], v) - base), bb = (uint32_t)(max(u[t], v) - base);
+      const uint64_t k1 = ((uint64_t)w[t] << 32) | edge_hash(seed, a, bb);
+      const uint64_t k2 = ((uint64_t)a << 32) | bb;
+      if (best == -2 || key_better(k1, k2, bk1, bk2)) {
+        best = u[t];
+        bk1 = k1;
+        bk2 = k2;
+      }
+    }
+  }
+  return best;
+}
+
+template <typename OffT>
+__global__ __launch_bounds__(LDS_MATCH_THREADS) void k_match_lds(int64_t n, const OffT *__restrict__ xadj,
+                                                                 const int32_t *__restrict__ adjncy,
+                                                                 const int32_t *__restrict__ adjwgt,
+                                                                 const int32_t *__restrict__ vwgt, MatchOwners own,
+                                                                 int32_t *match, int32_t *prop, int64_t half,
+                                                                 unsigned *flags, int64_t nflags, unsigned *hzero,
+                                                                 int rounds, int max_rounds, const int64_t *np)
+{
+  __shared__ int s_changed, s_active;
+  __shared__ int16_t s_state[2 * LDS_MATCH_MAX_N];  // match, then proposals
+  int16_t *const s_m = s_state, *const s_p = s_state + LDS_MATCH_MAX_N;
+  if (np) n = *np;  // queued ahead: the level's count was not yet on the host
+  const int nv = (int)n;
+  const int t = threadIdx.x;
+  if (t < 16) hzero[t] = 0u;
+  for (int64_t i = t; i < nflags; i += LDS_MATCH_THREADS) flags[i] = 0u;
+  for (int v = t; v < nv; v += LDS_MATCH_THREADS) s_m[v] = -1;
+  __syncthreads();
+  // launch 0: round 0's proposals
+  for (int v = t; v < nv; v += LDS_MATCH_THREADS) s_p[v] = (int16_t)lds_scan<OffT>(v, xadj, adjncy, adjwgt, vwgt, own, s_m);
+  __syncthreads();
+  bool live = true;
+  int r = 1;
+  for (; r <= rounds; ++r) {
+    // launch r: round r-1's accepts (reading proposals only)
+    if (t == 0) {
+      s_changed = 0;
+      s_active = 0;
+    }
+    __syncthreads();
+    bool changed = false, active = false;
+    for (int v = t; v < nv; v += LDS_MATCH_THREADS) {
+      if (s_m[v] != -1) continue;
+      const int p = s_p[v];
+      const int q = p == -2 ? v : (p >= 0 && s_p[p] == v ? p : -1);
+      if (q != -1) s_m[v] = (int16_t)q;
+      changed |= q != -1;
+      active |= q == -1;
+    }
+    if (__ballot(changed) && lane() == 0) s_changed = 1;
+    if (__ballot(active) && lane() == 0) s_active = 1;
+    __syncthreads();
+    live = s_changed && s_active;
+    if (t == 0) {
+      flags[2 * (r - 1)] = s_changed ? 1u : 0u;
+      flags[2 * (r - 1) + 1] = s_active ? 1u : 0u;
+    }
+    if (!live || r == max_rounds) break;  // launch max_rounds only accepts
+    // round r's proposals over the state after round r-1: a proposal is kept
+    // while its target is unmatched (it is then still the best)
+    for (int v = t; v < nv; v += LDS_MATCH_THREADS) {
+      if (s_m[v] != -1) continue;
+      const int p = s_p[v];
+      if (!(p >= 0 && s_m[p] == -1)) s_p[v] = (int16_t)lds_scan<OffT>(v, xadj, adjncy, adjwgt, vwgt, own, s_m);
+    }
+    __syncthreads();
+  }
+  // the working match; a level still live continues from launch rounds + 1
+  for (int v = t; v < nv; v += LDS_MATCH_THREADS) {
+    match[v] = s_m[v];
+    if (live && r > rounds) prop[(int64_t)(rounds & 1) * half + v] = s_p[v];
+  }
+}
+
 // out[v] = match[v], with vertices still unmatched (-1) matched to themselves;
 // out may be match itself.
 __global__ __launch_bounds__(256) void k_match_finish(int64_t n, const int32_t *match, int32_t *out)
@@ -948,6 +1080,7 @@ struct MatchRun {
   unsigned g = 0, glong = 0;
   unsigned ghub = 0;  // blocks of k_hub_reduce (0: the level has no hub row)
   bool fused_init = false;  // launch 0 initialises match, proposals and flags
+  bool lds = false;         // launches 0 .. LDS_ROUNDS are one k_match_lds
   // the frontier lists from launch compact_round + 1 on (large levels only)
   bool compact = false;
   int compact_round = 0;
@@ -1017,6 +1150,7 @@ static void match_start(cm_ctx *ctx, MatchRun &mr)
   // thread-per-row blocks cover every vertex); the long-row lists are
   // counted before launch 0, into words that k_match_init clears
   mr.fused_init = !has_long;
+  mr.lds = mr.fused_init && n <= LDS_MATCH_MAX_N && !mr.compact;
   const int64_t nbw = !mr.fused_init && n >= BITS_MIN_N ? (n + 63) / 64 : 0;
   if (nbw) mr.bits = (uint32_t *)al(sizeof(uint64_t) * nbw);
   if (!mr.fused_init)
@@ -1043,6 +1177,14 @@ static void match_launch_direct(cm_ctx *ctx, MatchRun &mr, int count)
   const Level &L = *mr.L;
   const int64_t n = L.n;
   const double row_bytes = (double)sizeof(OffT) * (n + 1) + 8.0 * L.m + 8.0 * n;
+  if (mr.lds && mr.r == 0 && count > 0) {  // launches 0 .. LDS_ROUNDS in one
+    const int rounds = std::min(LDS_ROUNDS, mr.max_rounds);
+    launch(ctx, "match_lds", row_bytes, k_match_lds<OffT>, dim3(1), dim3(LDS_MATCH_THREADS), 0, n,
+           (const OffT *)L.xadj, L.adjncy, L.adjwgt, L.vwgt, mr.own, mr.match, mr.prop, mr.half, mr.flags,
+           2 * (int64_t)mr.max_rounds + 16, ctx->d_ctr + HASH_CTR_SLOT, rounds, mr.max_rounds, mr.d_n);
+    mr.r = rounds + 1;
+    return;
+  }
   for (int i = 0; i < count && mr.r <= mr.max_rounds; ++i, ++mr.r) {
     const int r = mr.r;
     int32_t *cur = mr.prop + (size_t)(r & 1) * mr.half;

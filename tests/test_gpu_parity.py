@@ -390,7 +390,8 @@ def test_project(engine, orc):
 
 @pytest.mark.parametrize("order", [CM_ROW_FIRST_SEEN, CM_ROW_ASCENDING])
 @pytest.mark.parametrize("name,k", [("grid100w100", 8), ("delaunay4720", 8), ("rmat12", 16), ("hub", 4),
-                                    ("path_increasing", 2), ("rmat12_off64", 16), ("hub_off64", 4)])
+                                    ("path_increasing", 2), ("path_increasing_off64", 2), ("rmat12_off64", 16),
+                                    ("hub_off64", 4)])
 def test_hierarchy(engine, orc, name, k, order):
     if name == "grid100w100":
         g = graphs.grid2d(100, 100, wmax=100, seed=9)
@@ -399,12 +400,16 @@ def test_hierarchy(engine, orc, name, k, order):
     elif name == "hub_off64":
         from cases import hub_graph
         g = hub_graph(offsets=np.int64)
-    elif name == "path_increasing":
+    elif name.startswith("path_increasing"):
         # weights rise along the path: each round matches only the heaviest
         # remaining edge, so level 0 needs 200 rounds, far more than one batch
-        # of launches (the contraction is redone after the rounds resume)
+        # of launches or the 32 rounds of the single-workgroup matching of
+        # small levels (the ordinary launches continue from its state, and
+        # the contraction is redone after the rounds resume)
         n = 400
         g = graphs.from_edges(n, np.arange(n - 1), np.arange(1, n), np.arange(1, n))
+        if name.endswith("off64"):
+            g = Graph(g.xadj.astype(np.int64), g.adjncy, g.adjwgt, g.vwgt)
     else:
         g = CASES[name]
     g = with_weights(g)
