@@ -186,42 +186,64 @@ __global__ __launch_bounds__(256) void k_sort_tiles(const MergeDesc *tiles, int6
 }
 
 // ------------------------------------------------ bin 12: merge-path passes
-// One workgroup writes HUB_TILE outputs of the merge of runs A=[a, a+alen)
-// and B=[a+alen, a+alen+blen); each thread locates its first output with a
-// co-rank binary search and merges MERGE_ITEMS outputs sequentially.
+// One workgroup writes outputs [d0, d1) (at most HUB_TILE) of the merge of
+// runs A=[a, a+alen) and B=[a+alen, a+alen+blen), A first on equal keys.
+// Two threads find the block's co-ranks in global memory; the block then
+// stages the A and B keys those outputs come from (at most HUB_TILE of them
+// together) in LDS, and each thread finds its own co-rank and merges
+// MERGE_ITEMS outputs there.  (Searching and merging straight from global
+// memory made every step a dependent load.)
 constexpr int MERGE_ITEMS = HUB_TILE / 256;
+
+__device__ __forceinline__ int merge_corank(const uint64_t *A, int alen, const uint64_t *B, int blen, int d)
+{
+  int lo = max(0, d - blen), hi = min(d, alen);
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (A[mid] <= B[d - mid - 1]) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
 
 __global__ __launch_bounds__(256) void k_merge(const MergeDesc *descs, int64_t count,
                                                const uint64_t *sk, const uint32_t *sv,
                                                uint64_t *dk, uint32_t *dv)
 {
+  __shared__ uint64_t s_k[HUB_TILE];  // the block's A keys, then its B keys
+  __shared__ int s_c[2];
   for (int64_t g = blockIdx.x; g < count; g += gridDim.x) {
     const MergeDesc q = descs[g];
     const int alen = q.alen, blen = q.blen, total = alen + blen;
     const uint64_t *A = sk + q.a;
     const uint64_t *B = A + alen;
-    const int d = q.d0 + threadIdx.x * MERGE_ITEMS;
-    if (d >= total) continue;
-    const int dend = min(d + MERGE_ITEMS, total);
-    int lo = max(0, d - blen), hi = min(d, alen);
-    while (lo < hi) {
-      const int mid = (lo + hi) >> 1;
-      if (A[mid] <= B[d - mid - 1]) lo = mid + 1;
-      else hi = mid;
-    }
-    int i = lo, j = d - lo;
-    for (int o = d; o < dend; ++o) {
-      const bool ta = j >= blen || (i < alen && A[i] <= B[j]);
-      if (ta) {
-        dk[q.a + o] = A[i];
-        dv[q.a + o] = sv[q.a + i];
-        ++i;
-      } else {
-        dk[q.a + o] = B[j];
-        dv[q.a + o] = sv[q.a + alen + j];
-        ++j;
+    const int d0 = q.d0, d1 = min(d0 + HUB_TILE, total);
+    if (threadIdx.x < 2) s_c[threadIdx.x] = merge_corank(A, alen, B, blen, threadIdx.x ? d1 : d0);
+    __syncthreads();
+    const int i0 = s_c[0], j0 = d0 - i0;
+    const int na = s_c[1] - i0, nb = (d1 - s_c[1]) - j0;
+    for (int x = threadIdx.x; x < na + nb; x += blockDim.x) s_k[x] = x < na ? A[i0 + x] : B[j0 + x - na];
+    __syncthreads();
+    const uint64_t *LA = s_k, *LB = s_k + na;
+    const int d = threadIdx.x * MERGE_ITEMS;
+    if (d < na + nb) {
+      const int dend = min(d + MERGE_ITEMS, na + nb);
+      int i = merge_corank(LA, na, LB, nb, d), j = d - i;
+      for (int o = d; o < dend; ++o) {
+        const bool ta = j >= nb || (i < na && LA[i] <= LB[j]);
+        const int64_t out = q.a + d0 + o;
+        if (ta) {
+          dk[out] = LA[i];
+          dv[out] = sv[q.a + i0 + i];
+          ++i;
+        } else {
+          dk[out] = LB[j];
+          dv[out] = sv[q.a + alen + j0 + j];
+          ++j;
+        }
       }
     }
+    __syncthreads();  // s_k and s_c are reused
   }
 }
 
