@@ -136,7 +136,7 @@ __device__ __forceinline__ void lds_sort(uint64_t *keys, uint32_t *vals, int64_t
   for (int k = 2; k <= S; k <<= 1) {
     for (int j = k >> 1; j > 0; j >>= 1) {
       for (int t = threadIdx.x; t < S / 2; t += blockDim.x) {
-        const int i = 2 * j * (t / j) + (t % j);
+        const int i = ((t & ~(j - 1)) << 1) | (t & (j - 1));  // 2j * (t / j) + t % j, j a power of two
         const int l = i + j;
         const bool up = (i & k) == 0;
         const uint64_t a = sk[i], b = sk[l];
