@@ -175,9 +175,15 @@ __global__ __launch_bounds__(CM_WAVE) void k_accept_moves(const uint64_t *__rest
       // the window are in pw already): it then neither applies nor stops the
       // window.  Without this, parts at their bound (R-MAT, k = 256) made
       // every rejection a window of its own.
-      bool lighten = false;
-      for (int j = 0; j < l; ++j) lighten |= s_me[j] == to && s_to[j] != to;
-      if (has && !lighten && (long long)pw[to] + w > maxw) {
+      // The same holds for a candidate whose source part is already too
+      // light to give the vertex up, unless a candidate before it in this
+      // window moves a vertex into that part first.
+      bool lighten = false, heavier = false;
+      for (int j = 0; j < l; ++j) {
+        lighten |= s_me[j] == to && s_to[j] != to;
+        heavier |= s_to[j] == me && s_me[j] != me;
+      }
+      if (has && ((!lighten && (long long)pw[to] + w > maxw) || (!heavier && (long long)pw[me] - w < minw))) {
         has = false;
         s_to[l] = -1;
         s_me[l] = -1;
