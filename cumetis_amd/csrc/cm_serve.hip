@@ -382,6 +382,9 @@ static void loop_main(cm_server *S)
     } catch (const Error &e) {
       for (Job *j : act) j->status = e.code;
       retire = act;
+    } catch (const std::exception &) {  // host allocation: the thread must survive
+      for (Job *j : act) j->status = CM_ERR_NOMEM;
+      retire = act;
     }
     std::lock_guard<std::mutex> lk(S->mu);
     for (Job *j : retire) {
@@ -425,6 +428,8 @@ static void finish_main(cm_server *S, cm_ctx *fctx)
         rep.coarsest_n = (int32_t)j->h->levels.back().n;
       } catch (const Error &e) {
         j->status = e.code;
+      } catch (const std::exception &) {  // host allocation in the initial partition
+        j->status = CM_ERR_NOMEM;
       }
     }
     if (j->h) {

@@ -19,6 +19,8 @@
 
 #include <algorithm>
 #include <atomic>
+#include <deque>
+#include <exception>
 #include <memory>
 #include <queue>
 #include <thread>
@@ -57,14 +59,50 @@ std::vector<int> split_ranges(int n, int64_t work, const int64_t *xadj)
   return b;
 }
 
+// Threads whose exceptions reach the thread that started them: join()
+// waits for all and rethrows the first; the destructor (unwinding from an
+// exception of the starting thread) joins without rethrowing.
+class Threads {
+ public:
+  template <typename F>
+  void run(F f)
+  {
+    errs_.emplace_back();
+    std::exception_ptr *e = &errs_.back();  // deque: stays valid as it grows
+    th_.emplace_back([f, e]() mutable {
+      try {
+        f();
+      } catch (...) {
+        *e = std::current_exception();
+      }
+    });
+  }
+  void join()
+  {
+    for (auto &t : th_)
+      if (t.joinable()) t.join();
+    for (auto &e : errs_)
+      if (e) std::rethrow_exception(e);
+  }
+  ~Threads()
+  {
+    for (auto &t : th_)
+      if (t.joinable()) t.join();
+  }
+
+ private:
+  std::vector<std::thread> th_;
+  std::deque<std::exception_ptr> errs_;
+};
+
 // f(c, lo, hi) for every range c, range 0 on the calling thread
 template <typename F>
 void run_ranges(const std::vector<int> &b, F &&f)
 {
-  std::vector<std::thread> th;
-  for (int c = 1; c + 1 < (int)b.size(); ++c) th.emplace_back([&, c]() { f(c, b[c], b[c + 1]); });
+  Threads th;
+  for (int c = 1; c + 1 < (int)b.size(); ++c) th.run([&f, &b, c]() { f(c, b[c], b[c + 1]); });
   f(0, b[0], b[1]);
-  for (auto &t : th) t.join();
+  th.join();
 }
 
 template <typename F>
@@ -708,9 +746,9 @@ void grow_bisection(Part2 &g, Settings &s, const float *ntp, int niparts)  // cu
       x.alloc2way();
     }
     if (g.m >= GROW_PAR_MIN_M) {
-      std::vector<std::thread> th;
+      Threads th;
       for (int t = 1; t < niparts; ++t)
-        th.emplace_back([&, t]() {
+        th.run([&, t]() {
           Part2 &x = *tg[t];
           x.xadj = g.xadj;
           x.adjncy = g.adjncy;
@@ -719,7 +757,7 @@ void grow_bisection(Part2 &g, Settings &s, const float *ntp, int niparts)  // cu
           one(t);
         });
       one(0);
-      for (auto &t : th) t.join();
+      th.join();
     } else {
       for (int t = 0; t < niparts; ++t) one(t);
     }
@@ -932,18 +970,18 @@ int recursive_bisect_par(std::unique_ptr<Part2> g, Settings s, int nparts, int *
   const int nl = nparts >> 1, nr = nparts - nl;
   if (nparts > 3) {
     int objl = 0;
-    std::thread th;
+    Threads th;
+    bool spawned = false;
     if (l->n >= PAR_TASK_MIN_N && pc.threads.fetch_add(1) < pc.max_threads) {
-      th = std::thread([&, s]() { objl = recursive_bisect_par(std::move(l), s, nl, part, tp, fpart, pc); });
+      spawned = true;
+      th.run([&, s]() { objl = recursive_bisect_par(std::move(l), s, nl, part, tp, fpart, pc); });
     } else {
       if (l->n >= PAR_TASK_MIN_N) pc.threads.fetch_sub(1);
       objl = recursive_bisect_par(std::move(l), s, nl, part, tp, fpart, pc);
     }
     const int objr = recursive_bisect_par(std::move(r), s, nr, part, tp + nl, fpart + nl, pc);
-    if (th.joinable()) {
-      th.join();
-      pc.threads.fetch_sub(1);
-    }
+    th.join();
+    if (spawned) pc.threads.fetch_sub(1);
     obj += objl + objr;
   } else if (nparts == 3) {
     obj += recursive_bisect_par(std::move(r), s, nr, part, tp + nl, fpart + nl, pc);
