@@ -115,6 +115,7 @@ struct cm_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   bool own_stream = false;
+  std::vector<cm_ctx *> helpers;  // cm_partition's concurrent trials (created on first use)
   cm::Pool pool;
   std::string err;
   bool timing = false;

@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <exception>
 #include <mutex>
 #include <thread>
 #include <vector>
@@ -220,19 +221,46 @@ int cm_partition(cm_ctx *ctx, const cm_graph_view *g, const cm_coarsen_params *p
     for (auto &x : tr.t) x.th.join();
     const double p1 = now_ms();
     r.init_ms = p1 - p0 - r.coarsen_ms;  // host time the trials' initial partitions add
-    // the uncoarsenings one after another on the stream, each ending with
-    // its level-0 edge cut counted on the device
+    // the uncoarsenings at once: trial t > 0 on helper context t - 1 (a
+    // stream, pool and counters of its own, driven by a host thread of its
+    // own), each ending with its level-0 edge cut counted on the device.  The
+    // levels they read were complete before the initial partitions' read-backs.
+    for (auto &x : tr.t) CM_REQUIRE(!x.failed, CM_ERR_NOMEM, "initial partition failed (host allocation)");
+    while ((int)ctx->helpers.size() < trials - 1) {
+      cm_ctx *hc = nullptr;
+      const int rc = cm_ctx_create(ctx->device, nullptr, &hc);
+      CM_REQUIRE(rc == CM_OK, rc, "helper context for the trials");
+      ctx->helpers.push_back(hc);
+    }
+    for (cm_ctx *hc : ctx->helpers) hc->tune = ctx->tune;
     std::vector<std::vector<int32_t>> parts(trials);
     std::vector<int64_t> cut(trials, -1);
     std::vector<int32_t> levels(trials), coarsest(trials);
-    for (int t = 0; t < trials; ++t) {
-      Trial &x = tr.t[t];
-      CM_REQUIRE(!x.failed, CM_ERR_NOMEM, "initial partition failed (host allocation)");
-      parts[t].resize(g->n);
-      cm::uncoarsen_hierarchy(ctx, x.h, nparts, x.hc, x.where, parts[t].data(), !(p->flags & CM_FLAG_NO_BALANCE),
-                              &cut[t]);
-      levels[t] = (int32_t)x.h->levels.size();
-      coarsest[t] = (int32_t)x.h->levels.back().n;
+    std::vector<std::exception_ptr> errs(trials);
+    auto uncoarsen = [&](int t) {
+      cm_ctx *c = t == 0 ? ctx : ctx->helpers[t - 1];
+      try {
+        if (t > 0) CM_HIP(hipSetDevice(c->device));
+        Trial &x = tr.t[t];
+        parts[t].resize(g->n);
+        cm::uncoarsen_hierarchy(c, x.h, nparts, x.hc, x.where, parts[t].data(), !(p->flags & CM_FLAG_NO_BALANCE),
+                                &cut[t]);
+        levels[t] = (int32_t)x.h->levels.size();
+        coarsest[t] = (int32_t)x.h->levels.back().n;
+      } catch (...) {
+        errs[t] = std::current_exception();
+        (void)hipStreamSynchronize(c->stream);  // nothing of it still reads the graph
+      }
+    };
+    {
+      std::vector<std::thread> th;
+      for (int t = 1; t < trials; ++t) th.emplace_back(uncoarsen, t);
+      uncoarsen(0);
+      for (auto &x : th) x.join();
+    }
+    for (auto &e : errs)
+      if (e) std::rethrow_exception(e);
+    for (auto &x : tr.t) {
       cm_hier_release(ctx, x.h);
       x.h = nullptr;
     }

@@ -25,7 +25,7 @@ void write_partition(const std::string &path, const int *part, int n);
 // Recursive multilevel bisection initial partition (cu:3091-3108 ->
 // cu:3046-3062 -> cu:2964-3025).  EXACT: glibc rand() in the reference's
 // order.  PARALLEL: isolated vertices set aside and a stream per recursion
-// step (cm_initpart.cpp).  AUTO: PARALLEL on graphs of 2^18 vertices or
+// step (cm_initpart.cpp).  AUTO: PARALLEL on graphs of 2^14 vertices or
 // more, EXACT below.  Returns the summed bisection cuts, which is the edge
 // cut of the result.
 enum class InitForm { AUTO, EXACT, PARALLEL };

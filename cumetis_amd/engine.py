@@ -321,7 +321,7 @@ class Engine:
         """cm_partition; balance=False leaves out the balancing pass (the
         reference's pipeline has none); trials 0 = CM_DEFAULT_TRIALS;
         exact_initpart keeps the serial initial partition on coarsest graphs
-        of 2^18 vertices or more (CM_FLAG_EXACT_INITPART)."""
+        of 2^14 vertices or more (CM_FLAG_EXACT_INITPART)."""
         v, keep = self.host_view(g)
         flags = (0 if balance else L.CM_FLAG_NO_BALANCE) | (L.CM_FLAG_EXACT_INITPART if exact_initpart else 0)
         p = L.CoarsenParams(nparts, 0, 0, row_order, 0, 0, seed, flags, trials)

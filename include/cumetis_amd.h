@@ -91,8 +91,9 @@ typedef struct {
 #define CM_FLAG_NO_BALANCE 2
 /* cm_partition / cm_server: run the host initial partition in the
  * reference's serial form (one glibc rand() stream, cu:287-300) also when the
- * coarsest graph has 2^18 vertices or more.  Without it such graphs (coarsening
- * stalls on R-MAT's isolated vertices and stars) take a parallel form of the
+ * coarsest graph has 2^14 vertices or more.  Without it such graphs (R-MAT,
+ * whose coarsening stalls on isolated vertices and stars; large meshes in many
+ * parts) take a parallel form of the
  * same recursive bisection: isolated vertices placed last on the lightest
  * parts, one stream per recursion step seeded from its place, the two halves
  * of each split on separate threads; deterministic, but not the reference's
@@ -313,7 +314,7 @@ int cm_write_partition(const char *path, const int32_t *part, int64_t n);
 int cm_initial_partition(int64_t n, const int64_t *xadj, const int32_t *adjncy, const int32_t *adjwgt,
                          const int32_t *vwgt, int32_t nparts, int32_t *where, int64_t *objval);
 /* The parallel form that cm_partition and cm_server use for coarsest graphs
- * of 2^18 vertices or more (see CM_FLAG_EXACT_INITPART), at any size; no
+ * of 2^14 vertices or more (see CM_FLAG_EXACT_INITPART), at any size; no
  * counterpart in the reference, whose serial form takes minutes on such
  * graphs.  Deterministic for any thread count; *objval = the edge cut. */
 int cm_initial_partition_par(int64_t n, const int64_t *xadj, const int32_t *adjncy, const int32_t *adjwgt,

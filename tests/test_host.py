@@ -190,7 +190,7 @@ def test_write_partition(tmp_path):
 
 @pytest.mark.parametrize("k", [2, 8, 64])
 def test_parallel_initial_partition(orc, k):
-    """The parallel form that the pipeline takes on coarsest graphs of 2^18
+    """The parallel form that the pipeline takes on coarsest graphs of 2^14
     vertices or more, here on an R-MAT graph a fifth of whose vertices are
     isolated: the same result on every call, its objval the oracle's edge cut
     of its result, and the isolated vertices, in index order, each on the
