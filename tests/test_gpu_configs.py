@@ -9,6 +9,10 @@
   (xadj, adjncy, adjwgt, vwgt).  The hierarchy stops after this level
   (nc/n = 0.88 >= 0.75, cu:1843-1845), so this is the whole coarsening.
 
+- the serving mix's large job, a 160^3 7-point grid (4,096,000 vertices,
+  24,422,400 entries) given WITHOUT weights, k=128: every level of the
+  hierarchy (11 levels down to 26,855 vertices), level 0's rounds included.
+
 The inputs are the benchmark's own (same generators, seeds and parameters),
 given through host views, so the engine validates them as well.
 """
@@ -106,4 +110,29 @@ def test_rmat20_int64_hierarchy(engine, orc, unit):
             np.testing.assert_array_equal(L.cmap, R["cmap"])
             _, rounds = orc.handshake_match(R["xadj"], R["adjncy"], R["adjwgt"], R["vwgt"], mv, seed=4)
             assert L.rounds == rounds, i
+    h.release()
+
+
+def test_hierarchy_grid160cubed_k128(engine, orc):
+    g = graphs.grid3d(160, 160, 160)
+    assert g.adjwgt is None and g.vwgt is None
+    say(f"[grid160^3 hierarchy] n={g.n} m={g.m} k=128, no weights")
+    h = engine.coarsen(g, 128, seed=1)
+    ref = orc.coarsen(g.xadj, g.adjncy, None, None, 128, matcher="handshake", seed=1)
+    T = orc.threshold(g.n, 128)
+    mv = orc.maxvwgt(g.n, T)
+    assert (T, mv) == (29257, 210)
+    assert h.num_levels == len(ref) == 11
+    for i, R in enumerate(ref):
+        L = h.level(i)
+        assert (L.n, L.m) == (R["n"], len(R["adjncy"])), i
+        np.testing.assert_array_equal(L.xadj.astype(np.int64), R["xadj"])
+        np.testing.assert_array_equal(L.adjncy, R["adjncy"])
+        np.testing.assert_array_equal(L.adjwgt, R["adjwgt"])
+        np.testing.assert_array_equal(L.vwgt, R["vwgt"])
+        if i < len(ref) - 1:
+            np.testing.assert_array_equal(L.match, R["match"])
+            np.testing.assert_array_equal(L.cmap, R["cmap"])
+    _, rounds = orc.handshake_match(ref[0]["xadj"], ref[0]["adjncy"], ref[0]["adjwgt"], ref[0]["vwgt"], mv, seed=1)
+    assert h.level(0).rounds == rounds
     h.release()
