@@ -22,6 +22,7 @@ CM_ROW_FIRST_SEEN, CM_ROW_ASCENDING = 0, 1
 CM_FLAG_TRUSTED_INPUT = 1
 CM_FLAG_NO_BALANCE = 2
 CM_FLAG_EXACT_INITPART = 4
+CM_FLAG_HOST_INIT_COARSEN = 8
 CM_DEFAULT_TRIALS = 4
 
 
@@ -56,7 +57,8 @@ class Tuning(C.Structure):
 class PartReport(C.Structure):
     _fields_ = [("total_ms", C.c_double), ("coarsen_ms", C.c_double), ("init_ms", C.c_double),
                 ("uncoarsen_ms", C.c_double), ("levels", C.c_int32), ("coarsest_n", C.c_int32),
-                ("edgecut", C.c_int64), ("trials", C.c_int32), ("best_trial", C.c_int32)]
+                ("edgecut", C.c_int64), ("trials", C.c_int32), ("best_trial", C.c_int32),
+                ("init_device_steps", C.c_int32), ("reserved", C.c_int32)]
 
 
 # name -> (restype, argtypes); every symbol declared in include/cumetis_amd.h

@@ -749,6 +749,7 @@ int cm_ctx_destroy(cm_ctx *ctx)
 {
   if (!ctx) return CM_ERR_INVALID;
   for (cm_ctx *h : ctx->helpers) cm_ctx_destroy(h);
+  for (cm_ctx *h : ctx->steps.all) cm_ctx_destroy(h);
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   for (auto &t : ctx->pending) {

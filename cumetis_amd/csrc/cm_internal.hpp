@@ -6,6 +6,7 @@
 #include <stdint.h>
 #include <string.h>
 
+#include <condition_variable>
 #include <functional>
 #include <map>
 #include <memory>
@@ -74,6 +75,7 @@ struct Timed {
 
 // Rows above LONG_ROW entries are scanned by whole waves in the matching.
 constexpr int LONG_ROW = 64;
+constexpr int STEP_CTX_MAX = 4;  // contexts lent to device-coarsened initial-partition steps
 
 struct Level {
   int64_t n = 0, m = 0;
@@ -116,6 +118,14 @@ struct cm_ctx {
   hipStream_t stream = nullptr;
   bool own_stream = false;
   std::vector<cm_ctx *> helpers;  // cm_partition's concurrent trials (created on first use)
+  // the parallel initial partition's device-coarsened recursion steps: at
+  // most cm::STEP_CTX_MAX contexts of their own, created on first use, each lent
+  // to one step at a time
+  struct {
+    std::mutex mu;
+    std::condition_variable cv;
+    std::vector<cm_ctx *> all, idle;
+  } steps;
   cm::Pool pool;
   std::string err;
   bool timing = false;
@@ -330,10 +340,10 @@ void segsort_u64(cm_ctx *ctx, int64_t nseg, const void *d_segoff, bool off64, in
 // multilevel partition of an existing hierarchy: initial partition of the
 // coarsest level (host), then per level projection + one refinement pass.
 // part receives level-0 assignments (host), *cut (when given) their edge cut;
-// exact_init: the initial partition's serial form at every size.
-void partition_hierarchy(cm_ctx *ctx, const cm_hier *h, int nparts, int32_t *part, double *init_ms,
-                         double *uncoarsen_ms, bool balance = true, int64_t *cut = nullptr,
-                         bool exact_init = false);
+// flags: CM_FLAG_NO_BALANCE, CM_FLAG_EXACT_INITPART, CM_FLAG_HOST_INIT_COARSEN;
+// rep receives init_ms, uncoarsen_ms and init_device_steps.
+void partition_hierarchy(cm_ctx *ctx, const cm_hier *h, int nparts, int32_t flags, int32_t *part,
+                         cm_part_report *rep, int64_t *cut = nullptr);
 // (d) projection
 void project(cm_ctx *ctx, int64_t n, const int32_t *d_cmap, const int32_t *d_where_c,
              int32_t *d_where_f);

@@ -12,9 +12,11 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <exception>
 #include <mutex>
+#include <new>
 #include <thread>
 #include <vector>
 
@@ -57,6 +59,114 @@ cmh::HostGraph level_to_host(cm_ctx *ctx, const cm::Level &L)
   if (L.vbase)
     for (auto &a : g.adjncy) a -= (int)L.vbase;
   return g;
+}
+
+// The parallel initial partition's large recursion steps coarsen here, each
+// on a context lent from the owner's step contexts (at most STEP_CTX_MAX,
+// created on first use): the step's host graph is uploaded, coarsened by
+// cm_coarsen with the step's threshold, maxvwgt and seed, and its coarser
+// levels and maps read back.  The contexts' cached device memory is freed
+// when the initial partition ends.
+class DeviceSteps final : public cmh::StepCoarsener {
+ public:
+  explicit DeviceSteps(cm_ctx *owner) : owner_(owner) {}
+  ~DeviceSteps() override
+  {
+    if (!used_.load()) return;
+    std::lock_guard<std::mutex> lk(owner_->steps.mu);
+    for (cm_ctx *c : owner_->steps.idle) c->pool.trim();
+  }
+  int steps() const { return steps_.load(); }
+
+  void coarsen(int n, const int64_t *xadj, const int *adjncy, const int *adjwgt, const int *vwgt, int threshold,
+               int maxvwgt, uint64_t seed, std::vector<cmh::StepLevel> &levels) override
+  {
+    used_.store(true);
+    Lease lease(owner_);
+    cm_ctx *c = lease.c;
+    cm_graph_view v{};
+    v.n = n;
+    v.m = xadj[n];
+    v.offset_type = CM_OFF64;
+    v.on_device = 0;
+    v.xadj = xadj;
+    v.adjncy = adjncy;
+    v.adjwgt = adjwgt;
+    v.vwgt = vwgt;
+    cm_coarsen_params p{};
+    p.nparts = 2;
+    p.threshold = threshold;
+    p.maxvwgt = maxvwgt;
+    p.row_order = CM_ROW_FIRST_SEEN;
+    p.seed = seed;
+    p.flags = CM_FLAG_TRUSTED_INPUT;  // a subgraph the pipeline built from a validated graph
+    cm_hier *h = nullptr;
+    const int rc = cm_coarsen(c, &v, &p, &h);
+    if (rc != CM_OK) throw cm::Error(rc, "initial-partition step coarsening: " + c->err);
+    struct Release {
+      cm_ctx *c;
+      cm_hier *h;
+      ~Release() { cm_hier_release(c, h); }
+    } rel{c, h};
+    const int nl = (int)h->levels.size();
+    levels.clear();
+    levels.resize(nl - 1);
+    for (int l = 0; l + 1 < nl; ++l) {
+      const cm::Level &F = h->levels[l];
+      cmh::StepLevel &S = levels[l];
+      S.cmap.resize(F.n);
+      CM_HIP(hipMemcpyAsync(S.cmap.data(), F.cmap, 4 * F.n, hipMemcpyDeviceToHost, c->stream));
+      cmh::HostGraph g = level_to_host(c, h->levels[l + 1]);  // (synchronizes the stream)
+      S.n = g.n;
+      S.xadj = std::move(g.xadj);
+      S.adjncy = std::move(g.adjncy);
+      S.adjwgt = std::move(g.adjwgt);
+      S.vwgt = std::move(g.vwgt);
+    }
+    steps_.fetch_add(1);
+  }
+
+ private:
+  struct Lease {
+    cm_ctx *owner, *c = nullptr;
+    explicit Lease(cm_ctx *o) : owner(o)
+    {
+      auto &st = owner->steps;
+      std::unique_lock<std::mutex> lk(st.mu);
+      st.cv.wait(lk, [&] { return !st.idle.empty() || (int)st.all.size() < cm::STEP_CTX_MAX; });
+      if (!st.idle.empty()) {
+        c = st.idle.back();
+        st.idle.pop_back();
+      } else {
+        const int rc = cm_ctx_create(owner->device, nullptr, &c);
+        CM_REQUIRE(rc == CM_OK, rc, "context for an initial-partition step");
+        st.all.push_back(c);
+      }
+      c->tune = owner->tune;
+    }
+    ~Lease()
+    {
+      {
+        std::lock_guard<std::mutex> lk(owner->steps.mu);
+        owner->steps.idle.push_back(c);
+      }
+      owner->steps.cv.notify_one();
+    }
+  };
+  cm_ctx *owner_;
+  std::atomic<int> steps_{0};
+  std::atomic<bool> used_{false};
+};
+
+// A trial's initial-partition error, with host allocation failures as
+// CM_ERR_NOMEM.
+[[noreturn]] void rethrow_init(std::exception_ptr e)
+{
+  try {
+    std::rethrow_exception(e);
+  } catch (const std::bad_alloc &) {
+    throw cm::Error(CM_ERR_NOMEM, "initial partition failed (host allocation)");
+  }
 }
 
 }  // namespace
@@ -108,17 +218,23 @@ void uncoarsen_hierarchy(cm_ctx *ctx, const cm_hier *h, int nparts, const cmh::H
   if (cut) *cut = (int64_t)(h_cut / 2);
 }
 
-void partition_hierarchy(cm_ctx *ctx, const cm_hier *h, int nparts, int32_t *part, double *init_ms,
-                         double *uncoarsen_ms, bool balance, int64_t *cut, bool exact_init)
+void partition_hierarchy(cm_ctx *ctx, const cm_hier *h, int nparts, int32_t flags, int32_t *part,
+                         cm_part_report *rep, int64_t *cut)
 {
   const double t1 = now_ms();
   cmh::HostGraph hc = coarsest_to_host(ctx, h);
   std::vector<int> where;
-  cmh::initial_partition(hc, nparts, where, exact_init ? cmh::InitForm::EXACT : cmh::InitForm::AUTO);
+  {
+    DeviceSteps dev(ctx);
+    cmh::initial_partition(hc, nparts, where,
+                           (flags & CM_FLAG_EXACT_INITPART) ? cmh::InitForm::EXACT : cmh::InitForm::AUTO,
+                           (flags & CM_FLAG_HOST_INIT_COARSEN) ? nullptr : &dev);
+    rep->init_device_steps = dev.steps();
+  }
   const double t2 = now_ms();
-  uncoarsen_hierarchy(ctx, h, nparts, hc, where, part, balance, cut);
-  if (init_ms) *init_ms = t2 - t1;
-  if (uncoarsen_ms) *uncoarsen_ms = now_ms() - t2;
+  uncoarsen_hierarchy(ctx, h, nparts, hc, where, part, !(flags & CM_FLAG_NO_BALANCE), cut);
+  rep->init_ms = t2 - t1;
+  rep->uncoarsen_ms = now_ms() - t2;
 }
 
 }  // namespace cm
@@ -173,12 +289,15 @@ int cm_partition(cm_ctx *ctx, const cm_graph_view *g, const cm_coarsen_params *p
     // of its own while the GPU coarsens the later trials; the uncoarsenings
     // then run one after another on the stream.  Each trial's result is the
     // one it has alone (the initial partition draws from a private stream).
+    // the trials' large initial-partition steps coarsen on the device
+    // (declared before the trials, so that it ends after their threads)
+    DeviceSteps steps(ctx);
     struct Trial {
       cm_hier *h = nullptr;
       cmh::HostGraph hc;
       std::vector<int> where;
       std::thread th;
-      bool failed = false;
+      std::exception_ptr err;
     };
     struct Trials {
       cm_ctx *ctx;
@@ -196,6 +315,7 @@ int cm_partition(cm_ctx *ctx, const cm_graph_view *g, const cm_coarsen_params *p
     r.trials = trials;
     const int nparts = p->nparts;
     const double p0 = now_ms();
+    cmh::StepCoarsener *const sdev = (p->flags & CM_FLAG_HOST_INIT_COARSEN) ? nullptr : &steps;
     for (int t = 0; t < trials; ++t) {
       cm_coarsen_params q = *p;
       q.seed = p->seed + (uint64_t)t;
@@ -210,11 +330,11 @@ int cm_partition(cm_ctx *ctx, const cm_graph_view *g, const cm_coarsen_params *p
       Trial &x = tr.t[t];
       x.hc = cm::coarsest_to_host(ctx, x.h);
       const bool exact = (p->flags & CM_FLAG_EXACT_INITPART) != 0;
-      x.th = std::thread([&x, nparts, exact]() {
+      x.th = std::thread([&x, nparts, exact, sdev]() {
         try {
-          cmh::initial_partition(x.hc, nparts, x.where, exact ? cmh::InitForm::EXACT : cmh::InitForm::AUTO);
+          cmh::initial_partition(x.hc, nparts, x.where, exact ? cmh::InitForm::EXACT : cmh::InitForm::AUTO, sdev);
         } catch (...) {
-          x.failed = true;
+          x.err = std::current_exception();
         }
       });
     }
@@ -225,7 +345,9 @@ int cm_partition(cm_ctx *ctx, const cm_graph_view *g, const cm_coarsen_params *p
     // stream, pool and counters of its own, driven by a host thread of its
     // own), each ending with its level-0 edge cut counted on the device.  The
     // levels they read were complete before the initial partitions' read-backs.
-    for (auto &x : tr.t) CM_REQUIRE(!x.failed, CM_ERR_NOMEM, "initial partition failed (host allocation)");
+    r.init_device_steps = steps.steps();
+    for (auto &x : tr.t)
+      if (x.err) rethrow_init(x.err);
     while ((int)ctx->helpers.size() < trials - 1) {
       cm_ctx *hc = nullptr;
       const int rc = cm_ctx_create(ctx->device, nullptr, &hc);

@@ -422,8 +422,7 @@ static void finish_main(cm_server *S, cm_ctx *fctx)
       try {
         part.assign(j->n, 0);  // one part: a cut of 0
         if (j->p.nparts > 1 && j->n > 0)
-          partition_hierarchy(fctx, j->h, j->p.nparts, part.data(), &rep.init_ms, &rep.uncoarsen_ms,
-                              !(j->p.flags & CM_FLAG_NO_BALANCE), &cut, (j->p.flags & CM_FLAG_EXACT_INITPART) != 0);
+          partition_hierarchy(fctx, j->h, j->p.nparts, j->p.flags, part.data(), &rep, &cut);
         rep.levels = (int32_t)j->h->levels.size();
         rep.coarsest_n = (int32_t)j->h->levels.back().n;
       } catch (const Error &e) {

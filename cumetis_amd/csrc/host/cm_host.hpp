@@ -29,6 +29,28 @@ void write_partition(const std::string &path, const int *part, int n);
 // more, EXACT below.  Returns the summed bisection cuts, which is the edge
 // cut of the result.
 enum class InitForm { AUTO, EXACT, PARALLEL };
-int initial_partition(const HostGraph &g, int nparts, std::vector<int> &where, InitForm form = InitForm::AUTO);
+
+// One coarser level of a recursion step's chain: its graph, and cmap, the
+// map from the next finer level's vertices to this level's.
+struct StepLevel {
+  int n = 0;
+  std::vector<int64_t> xadj;
+  std::vector<int> adjncy, adjwgt, vwgt, cmap;
+};
+// A coarsener the GPU pipeline lends the PARALLEL form for its large
+// recursion steps (graphs of STEP_DEVICE_MIN_M entries or more): it builds
+// the step's chain with the same threshold, maxvwgt and stop rule as the host
+// chain (cu:1854-1875) and returns the coarser levels, finest first.  It
+// throws on a device error.
+struct StepCoarsener {
+  virtual ~StepCoarsener() = default;
+  virtual void coarsen(int n, const int64_t *xadj, const int *adjncy, const int *adjwgt, const int *vwgt,
+                       int threshold, int maxvwgt, uint64_t seed, std::vector<StepLevel> &levels) = 0;
+};
+constexpr int64_t STEP_DEVICE_MIN_M = 1 << 20;
+
+// dev (PARALLEL form only; may be null): the coarsener of the large steps.
+int initial_partition(const HostGraph &g, int nparts, std::vector<int> &where, InitForm form = InitForm::AUTO,
+                      StepCoarsener *dev = nullptr);
 
 }  // namespace cmh

@@ -24,6 +24,7 @@
 #include <exception>
 #include <memory>
 #include <queue>
+#include <stdexcept>
 #include <thread>
 #include <vector>
 
@@ -331,6 +332,7 @@ struct Settings {
   int threshold = 200;   // cu:378
   float balance[2] = {0, 0};
   unsigned try_seed = 0;  // parallel form: the region-growing tries' streams (0: the step's stream)
+  StepCoarsener *dev = nullptr;  // parallel form: the device coarsener of the large steps
 };
 
 // cu:830-903 (HEM with degree-bucketed random visit order) + cu:737-825
@@ -437,6 +439,33 @@ Part2 *cpu_coarsen(Part2 &g, Settings &s)
     c->finer = cur;
     cur = c;
   } while (cur->n > s.threshold && cur->n < 0.75 * cur->finer->n && cur->m > cur->n / 2);
+  return cur;
+}
+
+// The same chain built by the device coarsener (the parallel form's large
+// steps; seeded by the step's try stream).
+Part2 *dev_coarsen(Part2 &g, Settings &s)
+{
+  const int maxvwgt = (int)(1.5 * g.tvwgt / s.threshold);
+  std::vector<StepLevel> lv;
+  s.dev->coarsen(g.n, g.xadj.data(), g.adjncy.data(), g.adjwgt.data(), g.vwgt.data(), s.threshold, maxvwgt,
+                 s.try_seed, lv);
+  if (lv.empty()) throw std::runtime_error("device coarsening returned no level");
+  Part2 *cur = &g;
+  for (StepLevel &L : lv) {
+    cur->cmap = std::move(L.cmap);
+    cur->coarser.reset(new Part2());
+    Part2 *c = cur->coarser.get();
+    c->n = L.n;
+    c->xadj = std::move(L.xadj);
+    c->adjncy = std::move(L.adjncy);
+    c->adjwgt = std::move(L.adjwgt);
+    c->vwgt = std::move(L.vwgt);
+    c->m = c->xadj.back();
+    c->set_tvwgt();
+    c->finer = cur;
+    cur = c;
+  }
   return cur;
 }
 
@@ -820,7 +849,7 @@ int multilevel_bisect(Part2 &g, Settings &s, const float *tp)  // cu:2806-2829
 {
   const float rev = (float)(1.0 / (g.tvwgt > 0 ? g.tvwgt : 1));
   for (int i = 0; i < 2; ++i) s.balance[i] = rev / tp[i];
-  Part2 *c = cpu_coarsen(g, s);
+  Part2 *c = s.dev && g.m >= STEP_DEVICE_MIN_M ? dev_coarsen(g, s) : cpu_coarsen(g, s);
   grow_bisection(*c, s, tp, 5);
   Part2 *x = c;  // cu:2769-2802
   compute_2way(*x);
@@ -990,7 +1019,7 @@ int recursive_bisect_par(std::unique_ptr<Part2> g, Settings s, int nparts, int *
   return obj;
 }
 
-int initial_partition_par(const HostGraph &hg, int nparts, std::vector<int> &where)
+int initial_partition_par(const HostGraph &hg, int nparts, std::vector<int> &where, StepCoarsener *dev)
 {
   // the graph without its isolated vertices
   std::unique_ptr<Part2> g(new Part2());
@@ -1026,7 +1055,9 @@ int initial_partition_par(const HostGraph &hg, int nparts, std::vector<int> &whe
   pc.max_threads = (int)std::max(1u, std::min(hw ? hw : 1u, 16u)) - 1;
   std::vector<float> tp(nparts, (float)(1.0 / nparts));
   int obj = 0;
-  if (g->n > 0) obj = recursive_bisect_par(std::move(g), Settings(), nparts, where.data(), tp.data(), 0, pc);
+  Settings s;
+  s.dev = dev;
+  if (g->n > 0) obj = recursive_bisect_par(std::move(g), s, nparts, where.data(), tp.data(), 0, pc);
   // isolated vertices, in index order, each to the lightest part (the
   // lowest part id among equals)
   std::vector<int64_t> pw(nparts, 0);
@@ -1049,12 +1080,13 @@ int initial_partition_par(const HostGraph &hg, int nparts, std::vector<int> &whe
 
 }  // namespace
 
-int initial_partition(const HostGraph &hg, int nparts, std::vector<int> &where, InitForm form)
+int initial_partition(const HostGraph &hg, int nparts, std::vector<int> &where, InitForm form,
+                      StepCoarsener *dev)
 {
   where.assign(hg.n, 0);
   if (nparts <= 1 || hg.n == 0) return 0;
   if (form == InitForm::PARALLEL || (form == InitForm::AUTO && hg.n >= PAR_MIN_N))
-    return initial_partition_par(hg, nparts, where);
+    return initial_partition_par(hg, nparts, where, dev);
   RandStream stream;  // a fresh stream per call: the reference's first-call sequence
   g_stream = &stream;
   std::unique_ptr<Part2> g(new Part2());

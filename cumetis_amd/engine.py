@@ -317,13 +317,16 @@ class Engine:
 
     def partition(self, g: Graph, nparts: int, seed: int = 1,
                   row_order: int = L.CM_ROW_FIRST_SEEN, balance: bool = True, trials: int = 0,
-                  exact_initpart: bool = False):
+                  exact_initpart: bool = False, host_init_coarsen: bool = False):
         """cm_partition; balance=False leaves out the balancing pass (the
         reference's pipeline has none); trials 0 = CM_DEFAULT_TRIALS;
         exact_initpart keeps the serial initial partition on coarsest graphs
-        of 2^14 vertices or more (CM_FLAG_EXACT_INITPART)."""
+        of 2^14 vertices or more (CM_FLAG_EXACT_INITPART); host_init_coarsen
+        keeps the parallel form's large recursion steps coarsening on the host
+        (CM_FLAG_HOST_INIT_COARSEN)."""
         v, keep = self.host_view(g)
-        flags = (0 if balance else L.CM_FLAG_NO_BALANCE) | (L.CM_FLAG_EXACT_INITPART if exact_initpart else 0)
+        flags = ((0 if balance else L.CM_FLAG_NO_BALANCE) | (L.CM_FLAG_EXACT_INITPART if exact_initpart else 0)
+                 | (L.CM_FLAG_HOST_INIT_COARSEN if host_init_coarsen else 0))
         p = L.CoarsenParams(nparts, 0, 0, row_order, 0, 0, seed, flags, trials)
         part = np.empty(max(g.n, 1), np.int32)
         rep = L.PartReport()

@@ -99,6 +99,14 @@ typedef struct {
  * of each split on separate threads; deterministic, but not the reference's
  * rand order. */
 #define CM_FLAG_EXACT_INITPART 4
+/* cm_partition / cm_server: the parallel form coarsens every recursion step
+ * on the host.  Without it, steps whose graph has 2^20 entries or more
+ * coarsen on the device (cm_coarsen with the step's threshold 200, maxvwgt
+ * int(1.5*w/200) and stop rule, seeded from the step), the rest of the step
+ * (region growing, 2-way FM) staying on the host: a different but equally
+ * deterministic chain, and seconds less on graphs whose coarsest level keeps
+ * a large dense core (R-MAT). */
+#define CM_FLAG_HOST_INIT_COARSEN 8
 
 /* One level of a hierarchy (device pointers owned by the hierarchy; they are
  * read-only for the caller: level 0 of a graph given without weights points
@@ -274,6 +282,8 @@ typedef struct {
   int64_t edgecut;
   int32_t trials;     /* trials run */
   int32_t best_trial; /* the kept trial (its seed is params.seed + best_trial) */
+  int32_t init_device_steps; /* recursion steps of the initial partitions coarsened on the device */
+  int32_t reserved;
 } cm_part_report;
 int cm_partition(cm_ctx *ctx, const cm_graph_view *g, const cm_coarsen_params *p, int32_t *part,
                  cm_part_report *report);

@@ -350,14 +350,23 @@ def test_partition_rmat_parallel_initpart(engine, orc):
     and stars never shrink), so cm_partition's initial partition takes its
     parallel form: the same partition on a second call, every part within
     int(w/k * 1.03) after the balancing pass, and the reported cut the
-    oracle's cut of the partition."""
+    oracle's cut of the partition.  Its recursion steps of 2^20 entries or
+    more coarsen on the device; with CM_FLAG_HOST_INIT_COARSEN none does, and
+    that partition meets the same checks."""
     g = graphs.rmat_fast(19, 16, seed=2, offsets=np.int64)
     k = 64
     say(f"[rmat19 partition] n={g.n} m={g.m} k={k}, one trial")
     p1, r1 = engine.partition(g, k, seed=1, trials=1)
     assert r1.coarsest_n >= 1 << 14
+    assert r1.init_device_steps > 0
     p2, r2 = engine.partition(g, k, seed=1, trials=1)
     assert np.array_equal(p1, p2) and r1.edgecut == r2.edgecut
     assert r1.edgecut == orc.edgecut(g.xadj, g.adjncy, None, p1)
     pw = np.bincount(p1, minlength=k)
     assert pw.max() <= int(g.n / k * 1.03)
+    p3, r3 = engine.partition(g, k, seed=1, trials=1, host_init_coarsen=True)
+    say(f"  device steps {r1.init_device_steps}: init {r1.init_ms:.0f} ms cut {r1.edgecut}; "
+        f"host steps: init {r3.init_ms:.0f} ms cut {r3.edgecut}")
+    assert r3.init_device_steps == 0
+    assert r3.edgecut == orc.edgecut(g.xadj, g.adjncy, None, p3)
+    assert np.bincount(p3, minlength=k).max() <= int(g.n / k * 1.03)
