@@ -760,6 +760,9 @@ int cm_ctx_destroy(cm_ctx *ctx)
   for (auto &kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second);
   if (ctx->h_small) (void)hipHostFree(ctx->h_small);
   if (ctx->h_flags) (void)hipHostFree(ctx->h_flags);
+  if (ctx->h_stage) (void)hipHostFree(ctx->h_stage);
+  for (hipEvent_t e : ctx->stage_ev)
+    if (e) (void)hipEventDestroy(e);
   if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return CM_OK;

@@ -142,6 +142,8 @@ struct cm_ctx {
   int64_t *h_small = nullptr;     // pinned mirror of d_small
   unsigned *h_flags = nullptr;    // pinned copy of matching round flags
   size_t h_flags_cap = 0;         // in words
+  char *h_stage = nullptr;        // two pinned chunks for large staged copies (created on first use)
+  hipEvent_t stage_ev[2] = {nullptr, nullptr};
   int64_t *h_small_dev = nullptr;   // device addresses of h_small and h_flags
   unsigned *h_flags_dev = nullptr;  // (written by a level's last kernel)
   // round flags for the next read-back (readback_begin): the matching's

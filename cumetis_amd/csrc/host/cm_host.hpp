@@ -4,15 +4,46 @@
 #pragma once
 #include <stdint.h>
 
+#include <memory>
 #include <string>
+#include <utility>
 #include <vector>
 
 namespace cmh {
 
+// A vector whose resize leaves new elements uninitialised: the large graph
+// arrays of the initial partition are written whole after they are sized,
+// and a zero fill would touch every fresh page once more, on one thread.
+template <class T>
+struct DefaultInit : std::allocator<T> {
+  using value_type = T;
+  template <class U>
+  struct rebind {
+    using other = DefaultInit<U>;
+  };
+  DefaultInit() noexcept = default;
+  template <class U>
+  DefaultInit(const DefaultInit<U> &) noexcept
+  {
+  }
+  template <class U>
+  void construct(U *p) noexcept(noexcept(::new((void *)p) U))
+  {
+    ::new ((void *)p) U;
+  }
+  template <class U, class... A>
+  void construct(U *p, A &&...a)
+  {
+    ::new ((void *)p) U(std::forward<A>(a)...);
+  }
+};
+template <class T>
+using hvec = std::vector<T, DefaultInit<T>>;
+
 struct HostGraph {
   int n = 0;
-  std::vector<int64_t> xadj;
-  std::vector<int> adjncy, adjwgt, vwgt;
+  hvec<int64_t> xadj;
+  hvec<int> adjncy, adjwgt, vwgt;
   int64_t m() const { return xadj.empty() ? 0 : xadj.back(); }
 };
 
@@ -34,8 +65,8 @@ enum class InitForm { AUTO, EXACT, PARALLEL };
 // map from the next finer level's vertices to this level's.
 struct StepLevel {
   int n = 0;
-  std::vector<int64_t> xadj;
-  std::vector<int> adjncy, adjwgt, vwgt, cmap;
+  hvec<int64_t> xadj;
+  hvec<int> adjncy, adjwgt, vwgt, cmap;
 };
 // A coarsener the GPU pipeline lends the PARALLEL form for its large
 // recursion steps (graphs of STEP_DEVICE_MIN_M entries or more): it builds

@@ -291,8 +291,9 @@ thread_local FmWork tl_fm;
 struct Part2 {
   int n = 0;
   int64_t m = 0;
-  std::vector<int64_t> xadj;
-  std::vector<int> adjncy, adjwgt, vwgt, label, cmap;
+  hvec<int64_t> xadj;  // graph arrays: sized, then written whole
+  hvec<int> adjncy, adjwgt, vwgt, cmap;
+  std::vector<int> label;
   std::vector<int> where, bndptr, bndlist, id, ed;
   int pwgts[2] = {0, 0};
   int tvwgt = 0, mincut = 0, nbnd = 0;
@@ -423,6 +424,8 @@ int coarsen_once(Part2 &g, int maxvwgt, Part2 &c)
     c.xadj[++cv] = out;
   }
   c.m = out;
+  c.adjncy.resize(out);  // (sized for g.m entries above)
+  c.adjwgt.resize(out);
   c.set_tvwgt();
   return nc;
 }
