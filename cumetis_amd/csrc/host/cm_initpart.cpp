@@ -770,7 +770,7 @@ void grow_bisection(Part2 &g, Settings &s, const float *ntp, int niparts)  // cu
       x.n = g.n;
       x.m = g.m;
       x.tvwgt = g.tvwgt;
-      if (t == 0 || g.m < GROW_PAR_MIN_M) {  // the inline tries read g's arrays through copies too
+      if (g.m < GROW_PAR_MIN_M) {  // the inline tries read g's arrays through copies too
         x.xadj = g.xadj;
         x.adjncy = g.adjncy;
         x.adjwgt = g.adjwgt;
@@ -779,14 +779,29 @@ void grow_bisection(Part2 &g, Settings &s, const float *ntp, int niparts)  // cu
       x.alloc2way();
     }
     if (g.m >= GROW_PAR_MIN_M) {
-      Threads th;
+      // try 0 reads g's graph arrays in place (lent to it until the tries
+      // end; the tries only read them); the others copy them on their threads
+      Part2 &x0 = *tg[0];
+      struct Lend {
+        Part2 &a, &b;
+        void swap()
+        {
+          a.xadj.swap(b.xadj);
+          a.adjncy.swap(b.adjncy);
+          a.adjwgt.swap(b.adjwgt);
+          a.vwgt.swap(b.vwgt);
+        }
+        Lend(Part2 &p, Part2 &q) : a(p), b(q) { swap(); }
+        ~Lend() { swap(); }
+      } lend(g, x0);
+      Threads th;  // (joined before the arrays go back)
       for (int t = 1; t < niparts; ++t)
         th.run([&, t]() {
           Part2 &x = *tg[t];
-          x.xadj = g.xadj;
-          x.adjncy = g.adjncy;
-          x.adjwgt = g.adjwgt;
-          x.vwgt = g.vwgt;
+          x.xadj = x0.xadj;
+          x.adjncy = x0.adjncy;
+          x.adjwgt = x0.adjwgt;
+          x.vwgt = x0.vwgt;
           one(t);
         });
       one(0);
@@ -1044,13 +1059,15 @@ int initial_partition_par(const HostGraph &hg, int nparts, std::vector<int> &whe
   for (int i = 0; i < g->n; ++i) newid[g->label[i]] = i;
   g->adjncy.resize(hg.m());
   g->adjwgt.resize(hg.m());
-  for (int i = 0; i < g->n; ++i) {
-    const int64_t b = hg.xadj[g->label[i]], o = g->xadj[i];
-    for (int64_t j = 0; j < g->xadj[i + 1] - o; ++j) {
-      g->adjncy[o + j] = newid[hg.adjncy[b + j]];
-      g->adjwgt[o + j] = hg.adjwgt[b + j];
+  par_for(g->n, hg.m(), g->xadj.data(), [&](int lo, int hi) {
+    for (int i = lo; i < hi; ++i) {
+      const int64_t b = hg.xadj[g->label[i]], o = g->xadj[i];
+      for (int64_t j = 0; j < g->xadj[i + 1] - o; ++j) {
+        g->adjncy[o + j] = newid[hg.adjncy[b + j]];
+        g->adjwgt[o + j] = hg.adjwgt[b + j];
+      }
     }
-  }
+  });
   g->m = g->xadj.back();
   g->set_tvwgt();
   ParCtx pc;

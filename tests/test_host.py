@@ -230,3 +230,24 @@ def test_parallel_initial_partition_components(orc):
     orc.srand(1)
     wo, oo = orc.initpart(g.xadj, g.adjncy, g.adjwgt, g.vwgt, 2)
     assert np.array_equal(we, wo) and oe == oo
+
+
+def test_parallel_initial_partition_stalled_coarsening(orc):
+    """Four stars of 150 K leaves whose centres form a ring: a leaf can only
+    match its centre, so coarsening stops after one level on a graph of more
+    than 2^20 entries and the five region-growing tries run on threads (the
+    first on the step's own arrays, the others on copies).  Same checks:
+    one result on every call, objval the oracle's cut of it."""
+    leaves, stars = 150_000, 4
+    n = stars * (leaves + 1)
+    centre = np.arange(stars) * (leaves + 1)
+    src = np.repeat(centre, leaves)
+    dst = (centre[:, None] + 1 + np.arange(leaves)[None, :]).ravel()
+    ring = np.stack([centre, np.roll(centre, -1)], 1)
+    g = with_weights(graphs.from_edges(n, np.concatenate([src, ring[:, 0]]), np.concatenate([dst, ring[:, 1]])))
+    assert g.m >= 1 << 20
+    w1, o1 = product_initpart(g, 4, parallel=True)
+    w2, o2 = product_initpart(g, 4, parallel=True)
+    assert np.array_equal(w1, w2) and o1 == o2
+    assert 0 <= w1.min() and w1.max() < 4
+    assert o1 == orc.edgecut(g.xadj, g.adjncy, g.adjwgt, w1)
