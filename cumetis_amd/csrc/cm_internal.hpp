@@ -252,6 +252,12 @@ inline unsigned nblocks(int64_t work, int per_block)
 }
 
 void sync(cm_ctx *ctx);
+// Large copies through the context's two pinned chunks (host threads copy
+// one chunk in or out while the other's DMA runs).  download returns when
+// the data is on the host; upload returns with the copy queued on the stream
+// and the host source no longer needed.
+void download(cm_ctx *ctx, void *dst, const void *src, size_t bytes);
+void upload(cm_ctx *ctx, void *dst, const void *src, size_t bytes);
 // The read-back of a contraction (see Readback): d_small[first, first +
 // count) to h_small, *ovf (if given) to h_small[6], and the registered round
 // flags (ctx->rb_flags, consumed) to h_flags.  begin() describes it for the
