@@ -114,7 +114,9 @@ __global__ __launch_bounds__(256) void k_refine_dest(const int32_t *list, const 
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     if (l == 0) {
-      key[q] = ((uint64_t)(uint32_t)to << 32) | (uint32_t)v;
+      // bit 63: the second pass of the walk (from > to), whose candidates
+      // then sort after all of the first pass's
+      key[q] = ((uint64_t)(me > to) << 63) | ((uint64_t)(uint32_t)to << 32) | (uint32_t)v;
       idx[q] = (uint32_t)q;
       from[q] = me;
       vw[q] = vwgt[v];
@@ -131,15 +133,18 @@ __global__ __launch_bounds__(256) void k_refine_dest(const int32_t *list, const 
 // the serial walk; the next chunk starts after it.  A chunk with no
 // rejection advances by 64.  Candidates whose destination is already too
 // heavy for them are rejected without stopping the chunk (see below).
-// Candidates of the other pass take no part (the two passes, from <= to and
-// from > to, hold disjoint candidates).  Moves
+// The serial walk's two passes (from <= to, then from > to; disjoint
+// candidates, each in (destination, vertex) order) are one sweep: the sort
+// key's top bit puts the second pass's candidates after the first's.  Moves
 // within one part (no connectivity beats the own part) leave the weights
 // unchanged and are not applied.  pw_io: part weights in and out; out[0]:
 // moves applied.
+// from_s, vw_s: source part and vertex weight of candidate q in sorted order
+// (k_sorted_gather).  The next window's loads are issued before the current
+// window's checks, for the full advance that most windows make.
 __global__ __launch_bounds__(CM_WAVE) void k_accept_moves(const uint64_t *__restrict__ key,
-                                                          const uint32_t *__restrict__ idx,
-                                                          const int32_t *__restrict__ from,
-                                                          const int32_t *__restrict__ vw, unsigned nc, int nparts,
+                                                          const int32_t *__restrict__ from_s,
+                                                          const int32_t *__restrict__ vw_s, unsigned nc, int nparts,
                                                           int64_t maxw, int64_t minw, int64_t *pw_io,
                                                           int32_t *where, unsigned *out)
 {
@@ -150,22 +155,28 @@ __global__ __launch_bounds__(CM_WAVE) void k_accept_moves(const uint64_t *__rest
   for (int p = l; p < nparts; p += CM_WAVE) pw[p] = (unsigned long long)pw_io[p];
   __syncthreads();
   unsigned applied = 0;
-  for (int pass = 0; pass < 2; ++pass) {
+  struct Cand {
+    uint64_t k;
+    int me, w;
+  };
+  auto fetch = [&](unsigned q) {
+    Cand c{0, 0, 0};
+    if (q < nc) {
+      c.k = key[q];
+      c.me = from_s[q];
+      c.w = vw_s[q];
+    }
+    return c;
+  };
+  {
     unsigned q0 = 0;
+    Cand cur = fetch((unsigned)l);
     while (q0 < nc) {
       const unsigned q = q0 + (unsigned)l;
+      const Cand nxt = fetch(q0 + CM_WAVE + (unsigned)l);  // (used after a full advance)
       bool has = q < nc;
-      int to = 0, me = 0, v = 0;
-      long long w = 0;
-      if (has) {
-        const uint64_t k = key[q];
-        to = (int)(k >> 32);
-        v = (int)(uint32_t)k;
-        const uint32_t i = idx[q];
-        me = from[i];
-        w = vw[i];
-        has = pass == 0 ? me <= to : me > to;
-      }
+      const int to = (int)((cur.k >> 32) & 0x7FFFFFFFu), v = (int)(uint32_t)cur.k, me = cur.me;
+      const long long w = cur.w;
       s_to[l] = has ? to : -1;
       s_me[l] = has ? me : -1;
       __syncthreads();
@@ -210,11 +221,31 @@ __global__ __launch_bounds__(CM_WAVE) void k_accept_moves(const uint64_t *__rest
       }
       applied += (unsigned)__popcll(__ballot(apply));
       __syncthreads();
-      q0 += f < CM_WAVE ? (unsigned)f + 1u : (unsigned)CM_WAVE;
+      if (f < CM_WAVE) {
+        q0 += (unsigned)f + 1u;
+        cur = fetch(q0 + (unsigned)l);
+      } else {
+        q0 += (unsigned)CM_WAVE;
+        cur = nxt;
+      }
     }
   }
   for (int p = l; p < nparts; p += CM_WAVE) pw_io[p] = (int64_t)pw[p];
   if (l == 0) out[0] = applied;
+}
+
+// Source part and vertex weight of the sorted candidates, in sorted order.
+__global__ __launch_bounds__(256) void k_sorted_gather(unsigned nc, const uint32_t *__restrict__ idx,
+                                                       const int32_t *__restrict__ from,
+                                                       const int32_t *__restrict__ vw, int32_t *__restrict__ from_s,
+                                                       int32_t *__restrict__ vw_s)
+{
+  const unsigned q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < nc) {
+    const uint32_t i = idx[q];
+    from_s[q] = from[i];
+    vw_s[q] = vw[i];
+  }
 }
 
 __global__ __launch_bounds__(256) void k_apply_moves(int64_t nm, const int32_t *moves, int32_t *where)
@@ -419,9 +450,13 @@ static int refine_impl(cm_ctx *ctx, const Level &L, int nparts, int32_t *where, 
     int64_t *d_pw = (int64_t *)ctx->pool.alloc(sizeof(int64_t) * nparts);
     CM_HIP(hipMemcpyAsync(d_pw, pw.data(), sizeof(int64_t) * nparts, hipMemcpyHostToDevice, ctx->stream));
     unsigned *d_out = ctx->d_ctr + 6;
+    int32_t *from_s = (int32_t *)ctx->pool.alloc(sizeof(int32_t) * hc);
+    int32_t *vw_s = (int32_t *)ctx->pool.alloc(sizeof(int32_t) * hc);
+    launch(ctx, "refine_gather", 0.0, k_sorted_gather, dim3(nblocks(hc, 256)), dim3(256), 0, hc,
+           (const uint32_t *)idx, (const int32_t *)from, (const int32_t *)vw, from_s, vw_s);
     launch(ctx, "refine_accept", 0.0, k_accept_moves, dim3(1), dim3(CM_WAVE), sizeof(int64_t) * nparts,
-           (const uint64_t *)key, (const uint32_t *)idx, (const int32_t *)from, (const int32_t *)vw, hc, nparts,
-           maxw, minw, d_pw, where, d_out);
+           (const uint64_t *)key, (const int32_t *)from_s, (const int32_t *)vw_s, hc, nparts, maxw, minw, d_pw,
+           where, d_out);
     unsigned hacc = 0;
     CM_HIP(hipMemcpyAsync(pw.data(), d_pw, sizeof(int64_t) * nparts, hipMemcpyDeviceToHost, ctx->stream));
     CM_HIP(hipMemcpyAsync(&hacc, d_out, sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
@@ -432,6 +467,8 @@ static int refine_impl(cm_ctx *ctx, const Level &L, int nparts, int32_t *where, 
     ctx->pool.release(idx);
     ctx->pool.release(from);
     ctx->pool.release(vw);
+    ctx->pool.release(from_s);
+    ctx->pool.release(vw_s);
     ctx->pool.release(segoff);
   }
   ctx->pool.release(list);
