@@ -24,6 +24,7 @@
 namespace cm {
 
 constexpr int REFINE_MAX_PARTS = 2048;
+constexpr int CAND_LONG = 256;  // k_refine_candidates: longer rows are summed by a wave
 
 // xadj values index adjncy/adjwgt directly; neighbour ids index where_nb
 // (for a batch slice these are the slice pointers shifted back by its bases)
@@ -39,14 +40,36 @@ __global__ __launch_bounds__(256) void k_refine_candidates(int64_t n, const OffT
   __shared__ unsigned s_w[4];
   const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   bool cand = false;
+  int me = 0;
+  OffT b = 0, e = 0;
   if (v < n) {
-    const int me = where[v];
+    me = where[v];
+    b = xadj[v];
+    e = xadj[v + 1];
+  }
+  // rows longer than CAND_LONG (R-MAT's hubs) are summed by the whole wave,
+  // one after another; the others by their own lane
+  const bool lng = v < n && e - b > (OffT)CAND_LONG;
+  if (v < n && !lng) {
     int64_t in = 0, ex = 0;
-    for (OffT j = xadj[v]; j < xadj[v + 1]; ++j) {
+    for (OffT j = b; j < e; ++j) {
       if (where_nb[adjncy[j]] == me) in += adjwgt[j];
       else ex += adjwgt[j];
     }
     cand = ex > in;
+  }
+  for (uint64_t lm = __ballot(lng); lm; lm &= lm - 1) {
+    const int src = __ffsll((unsigned long long)lm) - 1;
+    const OffT rb = __shfl(b, src, CM_WAVE), re = __shfl(e, src, CM_WAVE);
+    const int rme = __shfl(me, src, CM_WAVE);
+    int64_t in = 0, ex = 0;
+    for (OffT j = rb + (OffT)lane(); j < re; j += CM_WAVE) {
+      if (where_nb[adjncy[j]] == rme) in += adjwgt[j];
+      else ex += adjwgt[j];
+    }
+    in = wave_sum<int64_t>(in);
+    ex = wave_sum<int64_t>(ex);
+    if (lane() == src) cand = ex > in;
   }
   const uint64_t mk = __ballot(cand);
   const int w = threadIdx.x >> 6;
