@@ -331,29 +331,50 @@ static int64_t device_sum(cm_ctx *ctx, int64_t n, const int32_t *a)
 
 // ------------------------------------------------------ input validation
 // bad[0] |= 1 offsets, 2 neighbour id out of range, 4 self loop, 8 weight <= 0
+constexpr int64_t CHECK_LONG = 256;
 template <typename OffT>
 __global__ __launch_bounds__(256) void k_check_graph(int64_t n, int64_t m, const OffT *xadj,
                                                      const int32_t *adjncy, const int32_t *adjwgt,
                                                      unsigned *bad)
 {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i > n) return;
   unsigned f = 0;
-  if (i == 0 && xadj[0] != 0) f |= 1;
-  if (i == n) {
-    if ((int64_t)xadj[n] != m) f |= 1;
-  } else {
-    const int64_t b = (int64_t)xadj[i], e = (int64_t)xadj[i + 1];
-    if (b > e || b < 0 || e > m) {
-      f |= 1;
+  int64_t b = 0, e = 0;
+  bool lng = false;  // rows above CHECK_LONG entries (hubs) are checked by the whole wave
+  if (i <= n) {
+    if (i == 0 && xadj[0] != 0) f |= 1;
+    if (i == n) {
+      if ((int64_t)xadj[n] != m) f |= 1;
     } else {
-      for (int64_t j = b; j < e; ++j) {
-        const int32_t u = adjncy[j];
-        if (u < 0 || u >= n) f |= 2;
-        else if (u == i) f |= 4;
-        if (adjwgt && adjwgt[j] <= 0) f |= 8;
+      b = (int64_t)xadj[i];
+      e = (int64_t)xadj[i + 1];
+      if (b > e || b < 0 || e > m) {
+        f |= 1;
+      } else if (e - b > CHECK_LONG) {
+        lng = true;
+      } else {
+        for (int64_t j = b; j < e; ++j) {
+          const int32_t u = adjncy[j];
+          if (u < 0 || u >= n) f |= 2;
+          else if (u == i) f |= 4;
+          if (adjwgt && adjwgt[j] <= 0) f |= 8;
+        }
       }
     }
+  }
+  for (uint64_t lm = __ballot(lng); lm; lm &= lm - 1) {
+    const int src = __ffsll((unsigned long long)lm) - 1;
+    const int64_t rb = __shfl(b, src, CM_WAVE), re = __shfl(e, src, CM_WAVE), ri = __shfl(i, src, CM_WAVE);
+    unsigned g = 0;
+    for (int64_t j = rb + lane(); j < re; j += CM_WAVE) {
+      const int32_t u = adjncy[j];
+      if (u < 0 || u >= n) g |= 2;
+      else if (u == ri) g |= 4;
+      if (adjwgt && adjwgt[j] <= 0) g |= 8;
+    }
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) g |= __shfl_xor(g, d, CM_WAVE);
+    if (lane() == src) f |= g;
   }
   if (f) atomicOr(bad, f);
 }

@@ -9,7 +9,7 @@ import pytest
 from cases import hub_graph, named_cases, with_weights
 from conftest import say
 from cumetis_amd import graphs
-from cumetis_amd._lib import CM_ROW_ASCENDING, CM_ROW_FIRST_SEEN
+from cumetis_amd._lib import CM_ERR_INPUT, CM_ROW_ASCENDING, CM_ROW_FIRST_SEEN, EngineError
 from cumetis_amd.engine import Graph, concat_graphs
 
 pytestmark = pytest.mark.gpu
@@ -370,3 +370,27 @@ def test_partition_rmat_parallel_initpart(engine, orc):
     assert r3.init_device_steps == 0
     assert r3.edgecut == orc.edgecut(g.xadj, g.adjncy, None, p3)
     assert np.bincount(p3, minlength=k).max() <= int(g.n / k * 1.03)
+
+
+@pytest.mark.parametrize("fault", ["range", "loop", "weight"])
+def test_coarsen_rejects_bad_entry_in_long_row(engine, fault):
+    """Input validation of a row checked by the whole wave (a star of 5000
+    leaves, its centre's row past the 256-entry bound): an out-of-range id, a
+    self loop or a zero weight deep in that row gives CM_ERR_INPUT, and the
+    same graph without the fault coarsens."""
+    leaves = 5000
+    g = with_weights(graphs.from_edges(leaves + 1, np.zeros(leaves, np.int64), np.arange(1, leaves + 1)))
+    engine.coarsen(g, 2).release()
+    j = int(g.xadj[0]) + 3001  # inside vertex 0's row
+    adj, wgt = g.adjncy.copy(), g.adjwgt.copy()
+    if fault == "range":
+        adj[j] = g.n + 7
+    elif fault == "loop":
+        adj[j] = 0
+    else:
+        wgt[j] = 0
+    bad = Graph(g.xadj, adj, wgt, g.vwgt)
+    say(f"[validate] star {leaves} fault={fault}")
+    with pytest.raises(EngineError) as ei:
+        engine.coarsen(bad, 2)
+    assert ei.value.code == CM_ERR_INPUT
